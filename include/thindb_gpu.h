@@ -1,0 +1,256 @@
+/*
+ * thindb_gpu.h — C-ABI of the MI355X GROUP BY engine (libthindb_gpu.so).
+ *
+ * The library replaces thinDB's CPU grouped operators for the shape
+ *   scan -> filter -> GROUP BY (COUNT/SUM/AVG/MIN/MAX) -> ORDER BY ... LIMIT
+ * behind the engine's `exec.Query` operator contract.  The caller (thinDB's
+ * planner, or any FFI host) keeps decoded column blocks resident in HBM with
+ * `tdbg_column_stage`, describes one grouped query with `tdbg_plan_desc`
+ * and runs it with `tdbg_execute`, which returns ONE result batch: group
+ * keys first, then aggregates, ordered and sliced by ORDER BY / OFFSET /
+ * LIMIT exactly as the reference operators emit it.
+ *
+ * Reference interfaces each entry point stands in for (paths relative to
+ * the thinDB source tree):
+ *   - handshake: `abi_version()` + i32 status, as the dlopen handshake in
+ *     src/api/zigfn.zig:1-26, :165-183, :598-603;
+ *   - column blocks: the decoded-block cache keyed by
+ *     (table_uid, segment_id, row_group_idx, column_idx),
+ *     src/storage/cache.zig:59-70, filled by
+ *     src/storage/segment_reader.zig:115 (decodeColumnMaybeCached);
+ *   - plan: the grouped request src/exec/v2_shape_group_topn.zig:63-101
+ *     (`Request`) with src/exec/aggregate.zig:237-268 (`AggSpec`),
+ *     src/exec/predicate.zig:33-51 (`Predicate`, `PredicateOp`) and
+ *     src/exec/sort.zig:28-31 (`SortSpec`);
+ *   - execute: the first `next()` of src/exec/v2_lowcard_group.zig:1775-1784
+ *     and src/exec/v2_shape_group_topn.zig:620-646, which run the whole
+ *     query and return one `Batch` (src/exec/exec.zig:175-247);
+ *   - result columns: `ColumnView` (src/storage/column.zig:27-60), validity
+ *     bit 1 = valid, bit r at byte r>>3, bit r&7.
+ *
+ * Status codes (all entry points returning int32_t): 0 = ok, negative =
+ * error; `tdbg_last_error()` gives the calling thread's last message.
+ *   TDBG_ERR_UNSUPPORTED  -> the caller declines (returns null from its
+ *                            tryBuild) and runs the CPU operator instead;
+ *   TDBG_ERR_OOM          -> error.MemoryBudgetExceeded (src/memory.zig:31-35);
+ *   TDBG_ERR_CANCELLED    -> error.QueryCancelled (src/exec/exec.zig:913-921);
+ *   anything else         -> error.Unexpected with tdbg_last_error().
+ * No entry point ever returns a partial result.
+ *
+ * Thread safety: every entry point may be called from many threads at once
+ * (thinDB runs one connection thread per query).  One context owns one GPU.
+ */
+#ifndef THINDB_GPU_H
+#define THINDB_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDBG_ABI_VERSION 1u
+
+enum tdbg_status {
+  TDBG_OK = 0,
+  TDBG_PENDING = 1,            /* tdbg_poll: not finished yet */
+  TDBG_ERR_INVALID = -1,       /* malformed argument or plan */
+  TDBG_ERR_UNSUPPORTED = -2,   /* shape not covered: decline to the CPU path */
+  TDBG_ERR_OOM = -3,           /* device memory / admission budget exhausted */
+  TDBG_ERR_CANCELLED = -4,     /* tdbg_cancel() observed */
+  TDBG_ERR_DEVICE = -5,        /* HIP runtime or device failure */
+  TDBG_ERR_NOT_STAGED = -6     /* a referenced column block is not resident */
+};
+
+/* Physical column types; mirrors the integer/float subset of
+ * src/types.zig (DECIMAL(p<=18,s) is an i64 mantissa, types.zig:377-383;
+ * DATETIME is i64 microseconds and DATE i32 days, DESIGN.md:58-75). */
+enum tdbg_type {
+  TDBG_BOOLEAN = 1,    /* u8 0/1 */
+  TDBG_TINYINT = 2,    /* i8 */
+  TDBG_SMALLINT = 3,   /* i16 */
+  TDBG_INT = 4,        /* i32 */
+  TDBG_BIGINT = 5,     /* i64 */
+  TDBG_DATE = 6,       /* i32 days */
+  TDBG_DATETIME = 7,   /* i64 microseconds */
+  TDBG_DECIMAL64 = 8,  /* i64 mantissa, scale in the spec */
+  TDBG_FLOAT = 9,      /* f32 */
+  TDBG_DOUBLE = 10,    /* f64 */
+  TDBG_DECIMAL128 = 11 /* i128 mantissa; result columns only (SUM of DECIMAL) */
+};
+
+/* src/exec/aggregate.zig:88-92 (count, sum, min, max, avg); COUNT(*) is
+ * count with no column (aggregate.zig:244-246). */
+enum tdbg_agg_op {
+  TDBG_AGG_COUNT_STAR = 0,
+  TDBG_AGG_COUNT = 1,
+  TDBG_AGG_SUM = 2,
+  TDBG_AGG_AVG = 3,
+  TDBG_AGG_MIN = 4,
+  TDBG_AGG_MAX = 5
+};
+
+/* src/exec/predicate.zig:33 `PredicateOp = enum { eq, neq, lt, lte, gt, gte }`. */
+enum tdbg_pred_op {
+  TDBG_EQ = 0,
+  TDBG_NEQ = 1,
+  TDBG_LT = 2,
+  TDBG_LTE = 3,
+  TDBG_GT = 4,
+  TDBG_GTE = 5
+};
+
+typedef struct tdbg_context tdbg_context;
+typedef struct tdbg_plan tdbg_plan;
+typedef struct tdbg_result tdbg_result;
+typedef struct tdbg_ticket tdbg_ticket;
+
+/* Identity of one decoded column block: src/storage/cache.zig:59-70 (`Key`). */
+typedef struct tdbg_column_key {
+  uint64_t table_uid;
+  uint64_t segment_id;
+  uint32_t row_group_idx;
+  uint32_t column_idx;
+} tdbg_column_key;
+
+/* Zone-map statistics of one block (src/storage/format.zig:265-270).  When
+ * `has_minmax` is 0 the library computes them from the host buffer. */
+typedef struct tdbg_block_stats {
+  int64_t min_i, max_i;  /* integer-family columns */
+  double min_f, max_f;   /* FLOAT / DOUBLE columns */
+  uint32_t has_minmax;
+  uint32_t pad;
+} tdbg_block_stats;
+
+/* One surviving row group of the scan (src/exec/rg_tiles.zig:1-30). */
+typedef struct tdbg_rg_ref {
+  uint64_t segment_id;
+  uint32_t row_group_idx;
+  uint32_t pad;
+} tdbg_rg_ref;
+
+/* One GROUP BY key part.  Nullable key columns are not covered (the caller
+ * declines), as in src/exec/v2_shape_group_topn.zig:2051-2094's packed path
+ * restricted to non-null parts. */
+typedef struct tdbg_key_part {
+  uint32_t column_idx;
+  uint32_t type; /* enum tdbg_type, integer family only */
+} tdbg_key_part;
+
+/* One aggregate: src/exec/aggregate.zig:237-268 (`AggSpec`), resolved to
+ * the input column's physical type. */
+typedef struct tdbg_agg_spec {
+  uint32_t op;          /* enum tdbg_agg_op */
+  int32_t column_idx;   /* -1 for COUNT(*) */
+  uint32_t input_type;  /* enum tdbg_type of the input column */
+  uint8_t input_scale;  /* DECIMAL scale, else 0 */
+  uint8_t nullable;     /* input column carries a validity bitmap */
+  uint8_t pad[2];
+} tdbg_agg_spec;
+
+/* One `col op literal` conjunct of the WHERE (predicate.zig:35-51).  The
+ * literal is already coerced to the column's type: `i64` for the integer
+ * family, `f64` for FLOAT/DOUBLE.  A NULL value fails every comparison. */
+typedef struct tdbg_predicate {
+  uint32_t column_idx;
+  uint32_t type;     /* enum tdbg_type of the column */
+  uint32_t op;       /* enum tdbg_pred_op */
+  uint32_t nullable; /* column carries a validity bitmap */
+  int64_t i64;
+  double f64;
+} tdbg_predicate;
+
+/* ORDER BY item (src/exec/sort.zig:28-31).  `target` indexes the output
+ * columns: [0, n_keys) are the keys, [n_keys, n_keys + n_aggs) the aggs. */
+typedef struct tdbg_order_spec {
+  uint32_t target;
+  uint32_t desc;
+} tdbg_order_spec;
+
+/* A grouped query: the part of v2_shape_group_topn.zig:63-101 (`Request`)
+ * the GPU operator consumes.  `limit == 0` means no limit, as in
+ * src/exec/v2_lowcard_group.zig:2072-2081.  The WHERE is the AND of `preds`. */
+typedef struct tdbg_plan_desc {
+  uint64_t table_uid;
+  uint32_t n_keys;
+  uint32_t n_aggs;
+  const tdbg_key_part* keys;
+  const tdbg_agg_spec* aggs;
+  uint32_t n_preds;
+  uint32_t n_order;
+  const tdbg_predicate* preds;
+  const tdbg_order_spec* order;
+  uint64_t limit;
+  uint64_t offset;
+  uint64_t est_groups; /* planner's NDV-product bound; 0 = unknown */
+} tdbg_plan_desc;
+
+/* One result column, library-owned host memory valid until
+ * tdbg_result_free().  DECIMAL128 values are 16-byte little-endian two's
+ * complement.  `validity` is NULL when no row is NULL. */
+typedef struct tdbg_column_view {
+  uint32_t type;  /* enum tdbg_type */
+  uint32_t scale; /* DECIMAL scale, else 0 */
+  uint32_t nullable;
+  uint32_t pad;
+  const void* data;
+  const uint8_t* validity;
+  uint64_t n_rows;
+} tdbg_column_view;
+
+/* Device-side timing of the last execution of a plan (HIP events on the
+ * plan's stream), for measurement. */
+typedef struct tdbg_exec_stats {
+  double total_ms;      /* whole execution on the device */
+  double scan_ms;       /* fused scan/filter/aggregate kernel(s) */
+  uint64_t rows_scanned;
+  uint64_t bytes_scanned; /* native-width bytes of every referenced column */
+  uint64_t n_groups;
+  uint32_t lds_mode;    /* 0 none, 1 direct, 2 hash */
+  uint32_t global_mode; /* 1 dense, 2 hash */
+} tdbg_exec_stats;
+
+/* ---- handshake / context --------------------------------------------- */
+uint32_t tdbg_abi_version(void);
+const char* tdbg_last_error(void);
+/* `hbm_budget_bytes` caps what the context allocates (0 = 90% of the
+ * device); beyond it calls fail with TDBG_ERR_OOM. */
+int32_t tdbg_context_create(int32_t device, uint64_t hbm_budget_bytes, tdbg_context** out);
+void tdbg_context_destroy(tdbg_context* ctx);
+uint64_t tdbg_context_bytes_in_use(const tdbg_context* ctx);
+
+/* ---- HBM column cache (src/storage/cache.zig:1-8, :59-70) -------------- */
+/* Copies one decoded block (`n_rows` values of `type`, plus an optional
+ * validity bitmap) into HBM.  Blocks of one row group must agree on n_rows.
+ * Segments are immutable, so a staged block never needs invalidation. */
+int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type,
+                          const void* host_values, const uint8_t* host_validity,
+                          uint64_t n_rows, const tdbg_block_stats* stats);
+int32_t tdbg_column_evict(tdbg_context* ctx, const tdbg_column_key* key);
+/* Pre-sizes a table's HBM arenas for `rows` rows (a hint; arenas grow). */
+int32_t tdbg_table_reserve(tdbg_context* ctx, uint64_t table_uid, uint64_t rows);
+int32_t tdbg_table_drop(tdbg_context* ctx, uint64_t table_uid);
+
+/* ---- plans and execution ----------------------------------------------- */
+/* TDBG_ERR_UNSUPPORTED means "decline": the caller runs its CPU operator. */
+int32_t tdbg_plan_create(tdbg_context* ctx, const tdbg_plan_desc* desc, tdbg_plan** out);
+void tdbg_plan_destroy(tdbg_plan* plan);
+/* Runs the query over the listed row groups (NULL = every staged row group
+ * of the table, in staging order) and returns one result batch. */
+int32_t tdbg_execute(tdbg_plan* plan, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out);
+int32_t tdbg_plan_last_stats(const tdbg_plan* plan, tdbg_exec_stats* out);
+/* Asks a running or queued execution of `plan` to stop; it then returns
+ * TDBG_ERR_CANCELLED.  Cleared when the next execution starts. */
+int32_t tdbg_cancel(tdbg_plan* plan);
+
+uint64_t tdbg_result_num_rows(const tdbg_result* r);
+uint32_t tdbg_result_num_columns(const tdbg_result* r);
+int32_t tdbg_result_column(const tdbg_result* r, uint32_t i, tdbg_column_view* out);
+void tdbg_result_free(tdbg_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* THINDB_GPU_H */

@@ -1,0 +1,815 @@
+/*
+ * oracle.c — CPU restatement of thinDB's grouped aggregate (scan -> filter
+ * -> GROUP BY -> ORDER BY/LIMIT), the checker for the GPU engine.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg; never by the product.
+ *
+ * What it restates (thinDB source tree, read as text):
+ *   - the request gate and agg planning: src/exec/v2_lowcard_group.zig:234-373
+ *     (declines: >8 keys, >16 aggs, non-integer keys, temporal SUM/AVG,
+ *     decimal128 inputs) and the silo's <=128-bit key packing,
+ *     src/exec/v2_shape_group_topn.zig:2051-2094;
+ *   - key packing: packIntPart, v2_lowcard_group.zig:886-948
+ *     (`(bits(v) & mask(width)) << offset`, widths from keyTypeBits :184-192);
+ *   - the worker model: tiles of TILE_RGS=16 row groups claimed from a shared
+ *     cursor (rg_tiles.zig:1-30; v2_lowcard_group.zig:839-854), scanned in
+ *     sub-batches of autoScanBatch rows (exec.zig:77-85);
+ *   - runs of equal adjacent keys and one gid per run: groupRows/runEnds,
+ *     v2_lowcard_group.zig:1054-1175, with the direct gid array of
+ *     resolveRunsDirect :1095-1118 / directRangeOf :509-530;
+ *   - the fold kernels: foldCountCol/foldSumInt/foldSumIntDense/
+ *     foldSumFloat/foldExtremeInt/foldExtremeFloat, :1206-1347 (run
+ *     accumulators, i128 integer slots, f64 per-run sums, first value seeds
+ *     MIN/MAX);
+ *   - the worker merge: mergeScalars, :1938-1999 (i128 add, f64 add, min/max
+ *     with the `had` rule);
+ *   - ORDER BY: SortCtx.lessThan/compareBySpec, :2198-2271 (ties by gid,
+ *     BIGINT SUM by its wrapped value, AVG by the unscaled mean, floats by
+ *     types.floatOrder, src/types.zig:579-587);
+ *   - OFFSET/LIMIT: emitSorted :2069-2087 (limit 0 = all);
+ *   - emit: emitGroup/appendInt/appendFloat :2089-2311 and the output types
+ *     of src/exec/aggregate.zig:3700-3735 (COUNT -> BIGINT, SUM int -> BIGINT
+ *     wrapped by @truncate, SUM decimal -> DECIMAL128, SUM float -> DOUBLE,
+ *     AVG -> DOUBLE = f64(sum)/f64(n)/10^scale, MIN/MAX -> input type,
+ *     NULL when the group has no non-null input).
+ *   - hashing: mix64/hashU128 and the tiered key hash of IntKeyMemsetTable,
+ *     src/exec/group_table.zig:45-56, :737-790; capacityFor :1711-1717.
+ */
+#define _GNU_SOURCE
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+#define MAX_KEYS 8
+#define MAX_AGGS 16
+#define TILE_RGS 16
+#define EMPTY_GID 0xffffffffu
+#define DIRECT_MAX_SPAN (1u << 18)
+#define DIRECT_SPAN_PER_GROUP 64u
+
+static __thread char g_err[512];
+static void set_err(const char* msg) { snprintf(g_err, sizeof g_err, "%s", msg); }
+const char* oracle_last_error(void) { return g_err; }
+
+/* ---- types ------------------------------------------------------------ */
+
+static int type_bytes(uint32_t t) {
+  switch (t) {
+    case TDBG_BOOLEAN: case TDBG_TINYINT: return 1;
+    case TDBG_SMALLINT: return 2;
+    case TDBG_INT: case TDBG_DATE: case TDBG_FLOAT: return 4;
+    case TDBG_BIGINT: case TDBG_DATETIME: case TDBG_DECIMAL64: case TDBG_DOUBLE: return 8;
+    case TDBG_DECIMAL128: return 16;
+    default: return 0;
+  }
+}
+/* keyTypeBits, v2_lowcard_group.zig:184-192 */
+static int key_type_bits(uint32_t t) {
+  switch (t) {
+    case TDBG_BOOLEAN: case TDBG_TINYINT: return 8;
+    case TDBG_SMALLINT: return 16;
+    case TDBG_INT: case TDBG_DATE: return 32;
+    case TDBG_BIGINT: case TDBG_DATETIME: case TDBG_DECIMAL64: return 64;
+    default: return 0;
+  }
+}
+static int is_float_type(uint32_t t) { return t == TDBG_FLOAT || t == TDBG_DOUBLE; }
+
+static inline int64_t load_i(const or_column* c, uint64_t r) {
+  switch (c->type) {
+    case TDBG_BOOLEAN: return ((const uint8_t*)c->data)[r];
+    case TDBG_TINYINT: return ((const int8_t*)c->data)[r];
+    case TDBG_SMALLINT: return ((const int16_t*)c->data)[r];
+    case TDBG_INT: case TDBG_DATE: return ((const int32_t*)c->data)[r];
+    default: return ((const int64_t*)c->data)[r];
+  }
+}
+static inline double load_f(const or_column* c, uint64_t r) {
+  if (c->type == TDBG_FLOAT) return (double)((const float*)c->data)[r];
+  return ((const double*)c->data)[r];
+}
+static inline int is_valid(const or_column* c, uint64_t r) {
+  if (!c->validity) return 1;
+  return (c->validity[r >> 3] >> (r & 7)) & 1;
+}
+
+/* ---- hashing (group_table.zig:45-56) ------------------------------------ */
+static inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+/* ---- compiled plan -------------------------------------------------------- */
+
+typedef struct {
+  uint32_t col;
+  uint32_t type;
+  uint32_t offset, width;
+} key_part;
+
+enum { OP_COUNT_STAR, OP_COUNT_COL, OP_SUM, OP_AVG, OP_MIN, OP_MAX };
+
+typedef struct {
+  int op;
+  int col; /* -1 for COUNT(*) */
+  int is_float;
+  uint32_t in_type;
+  uint32_t out_type;
+  uint32_t scale;
+  int nullable;
+} agg_plan;
+
+typedef struct {
+  const or_table* t;
+  const tdbg_plan_desc* d;
+  key_part parts[MAX_KEYS];
+  uint32_t n_parts;
+  uint32_t key_bits;
+  agg_plan aggs[MAX_AGGS];
+  uint32_t n_aggs;
+  uint32_t batch_rows;
+  /* direct gid array for a lone key part (directRangeOf) */
+  int direct;
+  uint64_t direct_base, direct_mask;
+  uint32_t direct_span;
+} cplan;
+
+/* ---- per-worker state ----------------------------------------------------- */
+
+typedef struct {
+  /* open-addressing table, capacityFor/0.75 load (group_table.zig) */
+  u128* tkeys;
+  uint32_t* tgids;
+  uint64_t cap, len;
+  /* per-gid state, stride n_aggs (v2_lowcard_group.zig WState) */
+  u128* keys;
+  uint64_t* counts;
+  i128* slots; /* integer states; floats live in fslots */
+  double* fslots;
+  double* csum; /* Neumaier: running sum + compensation per float agg */
+  double* ccomp;
+  uint64_t* ns;
+  uint64_t n_groups, gcap;
+  uint32_t* direct_gids;
+  /* scratch */
+  uint64_t* sel;
+  u128* bkeys;
+  uint32_t* ends;
+  uint32_t* run_gids;
+  int err;
+} wstate;
+
+static int ws_init(wstate* w, const cplan* p, uint64_t est) {
+  memset(w, 0, sizeof *w);
+  uint64_t need = est * 4 / 3 + 1, cap = 16;
+  while (cap < need) cap *= 2;
+  w->cap = cap;
+  w->tkeys = malloc(cap * sizeof(u128));
+  w->tgids = malloc(cap * sizeof(uint32_t));
+  if (!w->tkeys || !w->tgids) return -1;
+  for (uint64_t i = 0; i < cap; i++) w->tgids[i] = EMPTY_GID;
+  if (p->direct) {
+    w->direct_gids = malloc((size_t)p->direct_span * sizeof(uint32_t));
+    if (!w->direct_gids) return -1;
+    for (uint32_t i = 0; i < p->direct_span; i++) w->direct_gids[i] = EMPTY_GID;
+  }
+  size_t b = p->batch_rows + 8;
+  w->sel = malloc(b * sizeof(uint64_t));
+  w->bkeys = malloc(b * sizeof(u128));
+  w->ends = malloc(b * sizeof(uint32_t));
+  w->run_gids = malloc(b * sizeof(uint32_t));
+  if (!w->sel || !w->bkeys || !w->ends || !w->run_gids) return -1;
+  return 0;
+}
+
+static void ws_free(wstate* w) {
+  free(w->tkeys); free(w->tgids); free(w->keys); free(w->counts); free(w->slots);
+  free(w->fslots); free(w->csum); free(w->ccomp); free(w->ns); free(w->direct_gids);
+  free(w->sel); free(w->bkeys); free(w->ends); free(w->run_gids);
+}
+
+static uint64_t hash_key(const cplan* p, u128 key) {
+  uint64_t lo = (uint64_t)key, hi = (uint64_t)(key >> 64);
+  if (p->key_bits <= 32) return mix64((uint64_t)(uint32_t)lo);
+  if (p->key_bits <= 96) return mix64(lo ^ mix64((uint64_t)(uint32_t)hi));
+  return mix64(lo ^ mix64(hi));
+}
+
+static int ws_grow_groups(wstate* w, uint32_t n_aggs) {
+  uint64_t ng = w->gcap ? w->gcap * 2 : 64;
+  w->keys = realloc(w->keys, ng * sizeof(u128));
+  w->counts = realloc(w->counts, ng * sizeof(uint64_t));
+  w->slots = realloc(w->slots, ng * n_aggs * sizeof(i128) + 16);
+  w->fslots = realloc(w->fslots, ng * n_aggs * sizeof(double) + 8);
+  w->csum = realloc(w->csum, ng * n_aggs * sizeof(double) + 8);
+  w->ccomp = realloc(w->ccomp, ng * n_aggs * sizeof(double) + 8);
+  w->ns = realloc(w->ns, ng * n_aggs * sizeof(uint64_t) + 8);
+  if (!w->keys || !w->counts || !w->slots || !w->fslots || !w->csum || !w->ccomp || !w->ns) return -1;
+  w->gcap = ng;
+  return 0;
+}
+
+static int ws_table_grow(wstate* w, const cplan* p) {
+  uint64_t nc = w->cap * 2;
+  u128* nk = malloc(nc * sizeof(u128));
+  uint32_t* ng = malloc(nc * sizeof(uint32_t));
+  if (!nk || !ng) { free(nk); free(ng); return -1; }
+  for (uint64_t i = 0; i < nc; i++) ng[i] = EMPTY_GID;
+  for (uint64_t i = 0; i < w->cap; i++) {
+    if (w->tgids[i] == EMPTY_GID) continue;
+    uint64_t h = hash_key(p, w->tkeys[i]) & (nc - 1);
+    while (ng[h] != EMPTY_GID) h = (h + 1) & (nc - 1);
+    nk[h] = w->tkeys[i];
+    ng[h] = w->tgids[i];
+  }
+  free(w->tkeys); free(w->tgids);
+  w->tkeys = nk; w->tgids = ng; w->cap = nc;
+  return 0;
+}
+
+/* getOrPut + addGroup (v2_lowcard_group.zig:1182-1198): a new gid starts
+ * with zeroed accumulators. */
+static uint32_t ws_get_or_add(wstate* w, const cplan* p, u128 key) {
+  uint64_t mask = w->cap - 1;
+  uint64_t h = hash_key(p, key) & mask;
+  while (w->tgids[h] != EMPTY_GID) {
+    if (w->tkeys[h] == key) return w->tgids[h];
+    h = (h + 1) & mask;
+  }
+  if (w->n_groups == w->gcap && ws_grow_groups(w, p->n_aggs)) { w->err = 1; return 0; }
+  uint32_t gid = (uint32_t)w->n_groups++;
+  w->keys[gid] = key;
+  w->counts[gid] = 0;
+  for (uint32_t i = 0; i < p->n_aggs; i++) {
+    size_t o = (size_t)gid * p->n_aggs + i;
+    w->slots[o] = 0; w->fslots[o] = 0; w->csum[o] = 0; w->ccomp[o] = 0; w->ns[o] = 0;
+  }
+  if ((w->len + 1) * 4 > w->cap * 3) {
+    if (ws_table_grow(w, p)) { w->err = 1; return 0; }
+    mask = w->cap - 1;
+    h = hash_key(p, key) & mask;
+    while (w->tgids[h] != EMPTY_GID) h = (h + 1) & mask;
+  }
+  w->tkeys[h] = key;
+  w->tgids[h] = gid;
+  w->len++;
+  return gid;
+}
+
+static inline void neumaier_add(double* s, double* c, double x) {
+  double t = *s + x;
+  if (fabs(*s) >= fabs(x)) *c += (*s - t) + x; else *c += (x - t) + *s;
+  *s = t;
+}
+
+/* ---- filter (predicate.zig leaves, AND-ed) ---------------------------------- */
+static inline int pred_pass(const tdbg_predicate* pr, const or_column* c, uint64_t r) {
+  if (!is_valid(c, r)) return 0; /* NULL fails every comparison */
+  if (is_float_type(pr->type)) {
+    double v = load_f(c, r), x = pr->f64;
+    switch (pr->op) {
+      case TDBG_EQ: return v == x;
+      case TDBG_NEQ: return v != x;
+      case TDBG_LT: return v < x;
+      case TDBG_LTE: return v <= x;
+      case TDBG_GT: return v > x;
+      default: return v >= x;
+    }
+  }
+  int64_t v = load_i(c, r), x = pr->i64;
+  switch (pr->op) {
+    case TDBG_EQ: return v == x;
+    case TDBG_NEQ: return v != x;
+    case TDBG_LT: return v < x;
+    case TDBG_LTE: return v <= x;
+    case TDBG_GT: return v > x;
+    default: return v >= x;
+  }
+}
+
+/* ---- one sub-batch: foldBatch (v2_lowcard_group.zig:977-1050) --------------- */
+static void fold_batch(wstate* w, const cplan* p, uint64_t lo, uint64_t hi) {
+  const or_table* t = p->t;
+  const tdbg_plan_desc* d = p->d;
+  uint64_t n = 0;
+  for (uint64_t r = lo; r < hi; r++) {
+    int ok = 1;
+    for (uint32_t k = 0; k < d->n_preds && ok; k++) ok = pred_pass(&d->preds[k], &t->cols[d->preds[k].column_idx], r);
+    if (ok) w->sel[n++] = r;
+  }
+  if (n == 0) return;
+  /* packKeysForPart / packIntPart */
+  for (uint64_t i = 0; i < n; i++) {
+    u128 key = 0;
+    for (uint32_t k = 0; k < p->n_parts; k++) {
+      const key_part* kp = &p->parts[k];
+      uint64_t raw = (uint64_t)load_i(&t->cols[kp->col], w->sel[i]);
+      uint64_t m = kp->width >= 64 ? ~0ull : ((1ull << kp->width) - 1);
+      key |= (u128)(raw & m) << kp->offset;
+    }
+    w->bkeys[i] = key;
+  }
+  /* runEnds */
+  uint32_t runs = 0;
+  for (uint64_t i = 1; i < n; i++)
+    if (w->bkeys[i] != w->bkeys[i - 1]) w->ends[runs++] = (uint32_t)i;
+  w->ends[runs++] = (uint32_t)n;
+  /* gid per run: direct array or table (resolveRunsDirect / groupRows) */
+  uint32_t s = 0;
+  for (uint32_t ri = 0; ri < runs; ri++) {
+    u128 key = w->bkeys[s];
+    uint32_t gid;
+    if (p->direct) {
+      uint64_t at = ((uint64_t)key - p->direct_base) & p->direct_mask;
+      int in_range = at < p->direct_span;
+      if (in_range && w->direct_gids[at] != EMPTY_GID) {
+        gid = w->direct_gids[at];
+      } else {
+        gid = ws_get_or_add(w, p, key);
+        if (in_range) w->direct_gids[at] = gid;
+      }
+    } else {
+      gid = ws_get_or_add(w, p, key);
+    }
+    if (w->err) return;
+    w->run_gids[ri] = gid;
+    w->counts[gid] += w->ends[ri] - s;
+    s = w->ends[ri];
+  }
+  /* fold kernels, one aggregate at a time over the runs */
+  for (uint32_t a = 0; a < p->n_aggs; a++) {
+    const agg_plan* ag = &p->aggs[a];
+    if (ag->op == OP_COUNT_STAR) continue;
+    const or_column* c = &t->cols[ag->col];
+    uint32_t lo_i = 0;
+    for (uint32_t ri = 0; ri < runs; ri++) {
+      uint32_t hi_i = w->ends[ri];
+      size_t o = (size_t)w->run_gids[ri] * p->n_aggs + a;
+      if (ag->op == OP_COUNT_COL) {
+        uint64_t cnt = 0;
+        for (uint32_t i = lo_i; i < hi_i; i++) cnt += is_valid(c, w->sel[i]);
+        w->ns[o] += cnt;
+      } else if (ag->op == OP_SUM || ag->op == OP_AVG) {
+        uint64_t cnt = 0;
+        if (ag->is_float) {
+          double acc = 0;
+          for (uint32_t i = lo_i; i < hi_i; i++) {
+            uint64_t r = w->sel[i];
+            if (!is_valid(c, r)) continue;
+            double v = load_f(c, r);
+            acc += v;
+            neumaier_add(&w->csum[o], &w->ccomp[o], v);
+            cnt++;
+          }
+          if (cnt) { w->ns[o] += cnt; w->fslots[o] = w->fslots[o] + acc; }
+        } else {
+          i128 acc = 0;
+          for (uint32_t i = lo_i; i < hi_i; i++) {
+            uint64_t r = w->sel[i];
+            if (!is_valid(c, r)) continue;
+            acc += load_i(c, r);
+            cnt++;
+          }
+          if (cnt) { w->ns[o] += cnt; w->slots[o] += acc; }
+        }
+      } else { /* MIN / MAX: first value seeds (was_empty) */
+        int is_min = ag->op == OP_MIN;
+        uint64_t cnt = 0;
+        if (ag->is_float) {
+          double best = 0;
+          for (uint32_t i = lo_i; i < hi_i; i++) {
+            uint64_t r = w->sel[i];
+            if (!is_valid(c, r)) continue;
+            double v = load_f(c, r);
+            if (cnt == 0 || (is_min ? v < best : v > best)) best = v;
+            cnt++;
+          }
+          if (cnt) {
+            int was_empty = w->ns[o] == 0;
+            w->ns[o] += cnt;
+            int better = is_min ? best < w->fslots[o] : best > w->fslots[o];
+            if (was_empty || better) w->fslots[o] = best;
+          }
+        } else {
+          i128 best = 0;
+          for (uint32_t i = lo_i; i < hi_i; i++) {
+            uint64_t r = w->sel[i];
+            if (!is_valid(c, r)) continue;
+            i128 v = load_i(c, r);
+            if (cnt == 0 || (is_min ? v < best : v > best)) best = v;
+            cnt++;
+          }
+          if (cnt) {
+            int was_empty = w->ns[o] == 0;
+            w->ns[o] += cnt;
+            int better = is_min ? best < w->slots[o] : best > w->slots[o];
+            if (was_empty || better) w->slots[o] = best;
+          }
+        }
+      }
+      lo_i = hi_i;
+    }
+  }
+}
+
+/* ---- workers: tiles of TILE_RGS row groups from a shared cursor ------------- */
+typedef struct {
+  const cplan* p;
+  wstate* w;
+  _Atomic uint64_t* cursor;
+  uint64_t n_tiles;
+} worker_arg;
+
+static void* worker_main(void* arg) {
+  worker_arg* a = arg;
+  const cplan* p = a->p;
+  const or_table* t = p->t;
+  uint64_t tile_rows = (uint64_t)t->rg_rows * TILE_RGS;
+  for (;;) {
+    uint64_t tile = atomic_fetch_add(a->cursor, 1);
+    if (tile >= a->n_tiles) break;
+    uint64_t lo = tile * tile_rows, hi = lo + tile_rows;
+    if (hi > t->n_rows) hi = t->n_rows;
+    /* each row group is scanned in autoScanBatch sub-batches */
+    for (uint64_t rg = lo; rg < hi; rg += t->rg_rows) {
+      uint64_t rg_end = rg + t->rg_rows < hi ? rg + t->rg_rows : hi;
+      for (uint64_t b = rg; b < rg_end; b += p->batch_rows) {
+        uint64_t be = b + p->batch_rows < rg_end ? b + p->batch_rows : rg_end;
+        fold_batch(a->w, p, b, be);
+        if (a->w->err) return NULL;
+      }
+    }
+  }
+  return NULL;
+}
+
+/* ---- result ---------------------------------------------------------------- */
+struct or_result {
+  uint64_t n_rows, n_groups;
+  uint32_t n_cols;
+  tdbg_column_view cols[MAX_KEYS + MAX_AGGS];
+  double* comp[MAX_KEYS + MAX_AGGS];
+};
+
+uint64_t oracle_result_num_rows(const or_result* r) { return r->n_rows; }
+uint64_t oracle_result_num_groups(const or_result* r) { return r->n_groups; }
+uint32_t oracle_result_num_columns(const or_result* r) { return r->n_cols; }
+int32_t oracle_result_column(const or_result* r, uint32_t i, tdbg_column_view* out) {
+  if (i >= r->n_cols) return TDBG_ERR_INVALID;
+  *out = r->cols[i];
+  return 0;
+}
+int32_t oracle_result_compensated(const or_result* r, uint32_t i, double* out) {
+  if (i >= r->n_cols || !r->comp[i]) return TDBG_ERR_INVALID;
+  memcpy(out, r->comp[i], r->n_rows * sizeof(double));
+  return 0;
+}
+void oracle_result_free(or_result* r) {
+  if (!r) return;
+  for (uint32_t i = 0; i < r->n_cols; i++) {
+    free((void*)r->cols[i].data);
+    free((void*)r->cols[i].validity);
+    free(r->comp[i]);
+  }
+  free(r);
+}
+
+/* ---- sort (SortCtx.lessThan / compareBySpec) --------------------------------- */
+typedef struct {
+  const cplan* p;
+  const wstate* m;
+} sort_ctx;
+static __thread const sort_ctx* g_sort;
+
+static int float_order(double a, double b) { /* types.floatOrder: NaN largest */
+  int an = isnan(a), bn = isnan(b);
+  if (an || bn) return an && bn ? 0 : (an ? 1 : -1);
+  return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+static int64_t unpack_part(u128 key, const key_part* kp) {
+  uint64_t raw = (uint64_t)(key >> kp->offset);
+  if (kp->width >= 64) return (int64_t)raw;
+  uint32_t sh = 64 - kp->width;
+  return ((int64_t)(raw << sh)) >> sh;
+}
+
+static int compare_spec(const tdbg_order_spec* sp, uint32_t a, uint32_t b) {
+  const cplan* p = g_sort->p;
+  const wstate* m = g_sort->m;
+  if (sp->target < p->n_parts) {
+    int64_t x = unpack_part(m->keys[a], &p->parts[sp->target]);
+    int64_t y = unpack_part(m->keys[b], &p->parts[sp->target]);
+    return x < y ? -1 : (x > y ? 1 : 0);
+  }
+  uint32_t i = sp->target - p->n_parts;
+  const agg_plan* ag = &p->aggs[i];
+  size_t oa = (size_t)a * p->n_aggs + i, ob = (size_t)b * p->n_aggs + i;
+  switch (ag->op) {
+    case OP_COUNT_STAR: return m->counts[a] < m->counts[b] ? -1 : (m->counts[a] > m->counts[b]);
+    case OP_COUNT_COL: return m->ns[oa] < m->ns[ob] ? -1 : (m->ns[oa] > m->ns[ob]);
+    case OP_SUM: case OP_MIN: case OP_MAX:
+      if (ag->is_float) return float_order(m->fslots[oa], m->fslots[ob]);
+      if (ag->out_type == TDBG_BIGINT) {
+        int64_t x = (int64_t)(uint64_t)m->slots[oa], y = (int64_t)(uint64_t)m->slots[ob];
+        return x < y ? -1 : (x > y);
+      }
+      return m->slots[oa] < m->slots[ob] ? -1 : (m->slots[oa] > m->slots[ob]);
+    default: { /* AVG: unscaled mean, 0 for an empty group */
+      uint64_t an = m->ns[oa], bn = m->ns[ob];
+      double af = an == 0 ? 0 : (ag->is_float ? m->fslots[oa] / (double)an : (double)m->slots[oa] / (double)an);
+      double bf = bn == 0 ? 0 : (ag->is_float ? m->fslots[ob] / (double)bn : (double)m->slots[ob] / (double)bn);
+      return float_order(af, bf);
+    }
+  }
+}
+
+static int gid_cmp(const void* pa, const void* pb) {
+  uint32_t a = *(const uint32_t*)pa, b = *(const uint32_t*)pb;
+  const tdbg_plan_desc* d = g_sort->p->d;
+  for (uint32_t s = 0; s < d->n_order; s++) {
+    int o = compare_spec(&d->order[s], a, b);
+    if (o < 0) return d->order[s].desc ? 1 : -1;
+    if (o > 0) return d->order[s].desc ? -1 : 1;
+  }
+  return a < b ? -1 : (a > b);
+}
+
+/* ---- emit (emitGroup / appendInt / appendFloat) ------------------------------ */
+static void put_int(void* data, uint32_t type, uint64_t row, i128 v) {
+  switch (type) {
+    case TDBG_BOOLEAN: ((uint8_t*)data)[row] = (uint8_t)v; break;
+    case TDBG_TINYINT: ((int8_t*)data)[row] = (int8_t)v; break;
+    case TDBG_SMALLINT: ((int16_t*)data)[row] = (int16_t)v; break;
+    case TDBG_INT: case TDBG_DATE: ((int32_t*)data)[row] = (int32_t)v; break;
+    case TDBG_DECIMAL128: ((i128*)data)[row] = v; break;
+    case TDBG_DOUBLE: ((double*)data)[row] = (double)v; break;
+    default: ((int64_t*)data)[row] = (int64_t)(uint64_t)v; break; /* @truncate */
+  }
+}
+
+static int emit(const cplan* p, const wstate* m, or_result** out) {
+  const tdbg_plan_desc* d = p->d;
+  uint64_t ng = m->n_groups;
+  uint32_t* order = malloc((ng + 1) * sizeof(uint32_t));
+  if (!order) return -1;
+  for (uint64_t i = 0; i < ng; i++) order[i] = (uint32_t)i;
+  if (d->n_order > 0) {
+    sort_ctx ctx = {p, m};
+    g_sort = &ctx;
+    qsort(order, ng, sizeof(uint32_t), gid_cmp);
+    g_sort = NULL;
+  }
+  uint64_t start = d->offset < ng ? d->offset : ng;
+  uint64_t end = d->limit == 0 ? ng : (start + d->limit < ng ? start + d->limit : ng);
+  uint64_t nr = end - start;
+  or_result* r = calloc(1, sizeof *r);
+  if (!r) { free(order); return -1; }
+  r->n_rows = nr;
+  r->n_groups = ng;
+  r->n_cols = p->n_parts + p->n_aggs;
+  for (uint32_t k = 0; k < p->n_parts; k++) {
+    tdbg_column_view* c = &r->cols[k];
+    c->type = p->parts[k].type;
+    c->n_rows = nr;
+    void* data = calloc(nr + 1, type_bytes(c->type));
+    for (uint64_t i = 0; i < nr; i++) put_int(data, c->type, i, unpack_part(m->keys[order[start + i]], &p->parts[k]));
+    c->data = data;
+  }
+  for (uint32_t a = 0; a < p->n_aggs; a++) {
+    const agg_plan* ag = &p->aggs[a];
+    tdbg_column_view* c = &r->cols[p->n_parts + a];
+    c->type = ag->out_type;
+    c->scale = (ag->out_type == TDBG_DECIMAL128 || ag->out_type == TDBG_DECIMAL64) ? ag->scale : 0;
+    c->nullable = !(ag->op == OP_COUNT_STAR || ag->op == OP_COUNT_COL);
+    c->n_rows = nr;
+    void* data = calloc(nr + 1, type_bytes(c->type));
+    uint8_t* valid = calloc(nr / 8 + 1, 1);
+    double* comp = NULL;
+    int want_comp = ag->is_float && (ag->op == OP_SUM || ag->op == OP_AVG);
+    if (want_comp) comp = calloc(nr + 1, sizeof(double));
+    int any_null = 0;
+    for (uint64_t i = 0; i < nr; i++) {
+      uint32_t gid = order[start + i];
+      size_t o = (size_t)gid * p->n_aggs + a;
+      uint64_t ns = m->ns[o];
+      int is_null = 0;
+      switch (ag->op) {
+        case OP_COUNT_STAR: ((int64_t*)data)[i] = (int64_t)m->counts[gid]; break;
+        case OP_COUNT_COL: ((int64_t*)data)[i] = (int64_t)ns; break;
+        case OP_SUM: case OP_MIN: case OP_MAX:
+          if (ns == 0) { is_null = 1; break; }
+          if (ag->is_float) {
+            if (ag->out_type == TDBG_FLOAT) ((float*)data)[i] = (float)m->fslots[o];
+            else ((double*)data)[i] = m->fslots[o];
+            if (comp) comp[i] = m->csum[o] + m->ccomp[o];
+          } else {
+            put_int(data, ag->out_type, i, m->slots[o]);
+          }
+          break;
+        default: /* AVG */
+          if (ns == 0) { is_null = 1; break; }
+          if (ag->is_float) {
+            ((double*)data)[i] = m->fslots[o] / (double)ns;
+            if (comp) comp[i] = (m->csum[o] + m->ccomp[o]) / (double)ns;
+          } else {
+            ((double*)data)[i] = (double)m->slots[o] / (double)ns / pow(10.0, (double)ag->scale);
+          }
+          break;
+      }
+      if (is_null) any_null = 1; else valid[i >> 3] |= (uint8_t)(1u << (i & 7));
+    }
+    c->data = data;
+    if (any_null) c->validity = valid; else free(valid);
+    r->comp[p->n_parts + a] = comp;
+  }
+  free(order);
+  *out = r;
+  return 0;
+}
+
+/* ---- merge (mergeScalars) ---------------------------------------------------- */
+static int merge_into(wstate* m, const cplan* p, const wstate* w) {
+  for (uint64_t g = 0; g < w->n_groups; g++) {
+    uint32_t mg = ws_get_or_add(m, p, w->keys[g]);
+    if (m->err) return -1;
+    m->counts[mg] += w->counts[g];
+    for (uint32_t i = 0; i < p->n_aggs; i++) {
+      const agg_plan* ag = &p->aggs[i];
+      size_t wo = (size_t)g * p->n_aggs + i, mo = (size_t)mg * p->n_aggs + i;
+      uint64_t w_ns = w->ns[wo];
+      if (ag->op == OP_COUNT_STAR) w_ns = w->counts[g];
+      if (w_ns == 0) continue;
+      uint64_t had = m->ns[mo];
+      m->ns[mo] += w_ns;
+      switch (ag->op) {
+        case OP_SUM: case OP_AVG:
+          if (ag->is_float) {
+            m->fslots[mo] = m->fslots[mo] + w->fslots[wo];
+            neumaier_add(&m->csum[mo], &m->ccomp[mo], w->csum[wo]);
+            m->ccomp[mo] += w->ccomp[wo];
+          } else {
+            m->slots[mo] += w->slots[wo];
+          }
+          break;
+        case OP_MIN:
+          if (ag->is_float) { if (had == 0 || w->fslots[wo] < m->fslots[mo]) m->fslots[mo] = w->fslots[wo]; }
+          else if (had == 0 || w->slots[wo] < m->slots[mo]) m->slots[mo] = w->slots[wo];
+          break;
+        case OP_MAX:
+          if (ag->is_float) { if (had == 0 || w->fslots[wo] > m->fslots[mo]) m->fslots[mo] = w->fslots[wo]; }
+          else if (had == 0 || w->slots[wo] > m->slots[mo]) m->slots[mo] = w->slots[wo];
+          break;
+        default: break;
+      }
+    }
+  }
+  return 0;
+}
+
+/* ---- plan compile ------------------------------------------------------------ */
+static int compile(cplan* p, const tdbg_plan_desc* d, const or_table* t) {
+  memset(p, 0, sizeof *p);
+  p->t = t;
+  p->d = d;
+  if (d->n_keys == 0 || d->n_keys > MAX_KEYS) { set_err("key count"); return TDBG_ERR_UNSUPPORTED; }
+  if (d->n_aggs == 0 || d->n_aggs > MAX_AGGS) { set_err("agg count"); return TDBG_ERR_UNSUPPORTED; }
+  uint32_t bits = 0;
+  for (uint32_t k = 0; k < d->n_keys; k++) {
+    const tdbg_key_part* kp = &d->keys[k];
+    if (kp->column_idx >= t->n_cols) { set_err("key column"); return TDBG_ERR_INVALID; }
+    if (t->cols[kp->column_idx].type != kp->type) { set_err("key type"); return TDBG_ERR_INVALID; }
+    int w = key_type_bits(kp->type);
+    if (!w) { set_err("key type not integer"); return TDBG_ERR_UNSUPPORTED; }
+    if (t->cols[kp->column_idx].validity) { set_err("nullable key"); return TDBG_ERR_UNSUPPORTED; }
+    p->parts[k] = (key_part){kp->column_idx, kp->type, bits, (uint32_t)w};
+    bits += (uint32_t)w;
+  }
+  if (bits > 128) { set_err("key wider than 128 bits"); return TDBG_ERR_UNSUPPORTED; }
+  p->n_parts = d->n_keys;
+  p->key_bits = bits;
+  uint64_t row_bytes = 0;
+  for (uint32_t k = 0; k < d->n_keys; k++) row_bytes += type_bytes(d->keys[k].type);
+  for (uint32_t a = 0; a < d->n_aggs; a++) {
+    const tdbg_agg_spec* s = &d->aggs[a];
+    agg_plan* ag = &p->aggs[a];
+    ag->col = s->column_idx;
+    ag->in_type = s->input_type;
+    ag->scale = s->input_scale;
+    if (s->op == TDBG_AGG_COUNT_STAR) { ag->op = OP_COUNT_STAR; ag->out_type = TDBG_BIGINT; ag->col = -1; continue; }
+    if (s->column_idx < 0 || (uint32_t)s->column_idx >= t->n_cols) { set_err("agg column"); return TDBG_ERR_INVALID; }
+    if (t->cols[s->column_idx].type != s->input_type) { set_err("agg type"); return TDBG_ERR_INVALID; }
+    ag->nullable = t->cols[s->column_idx].validity != NULL;
+    row_bytes += type_bytes(s->input_type);
+    if (s->op == TDBG_AGG_COUNT) { ag->op = OP_COUNT_COL; ag->out_type = TDBG_BIGINT; continue; }
+    int fl = is_float_type(s->input_type);
+    if (!fl && !key_type_bits(s->input_type)) { set_err("agg input type"); return TDBG_ERR_UNSUPPORTED; }
+    if ((s->input_type == TDBG_DATE || s->input_type == TDBG_DATETIME) &&
+        (s->op == TDBG_AGG_SUM || s->op == TDBG_AGG_AVG)) { set_err("temporal SUM/AVG"); return TDBG_ERR_UNSUPPORTED; }
+    ag->is_float = fl;
+    switch (s->op) {
+      case TDBG_AGG_SUM:
+        ag->op = OP_SUM;
+        ag->out_type = fl ? TDBG_DOUBLE : (s->input_type == TDBG_DECIMAL64 ? TDBG_DECIMAL128 : TDBG_BIGINT);
+        break;
+      case TDBG_AGG_AVG: ag->op = OP_AVG; ag->out_type = TDBG_DOUBLE; break;
+      case TDBG_AGG_MIN: ag->op = OP_MIN; ag->out_type = s->input_type; break;
+      case TDBG_AGG_MAX: ag->op = OP_MAX; ag->out_type = s->input_type; break;
+      default: set_err("agg op"); return TDBG_ERR_UNSUPPORTED;
+    }
+  }
+  p->n_aggs = d->n_aggs;
+  for (uint32_t i = 0; i < d->n_preds; i++) {
+    if (d->preds[i].column_idx >= t->n_cols) { set_err("predicate column"); return TDBG_ERR_INVALID; }
+    if (d->preds[i].op > TDBG_GTE) { set_err("predicate op"); return TDBG_ERR_INVALID; }
+  }
+  for (uint32_t i = 0; i < d->n_order; i++)
+    if (d->order[i].target >= d->n_keys + d->n_aggs) { set_err("order target"); return TDBG_ERR_INVALID; }
+  /* autoScanBatch (exec.zig:77-85) */
+  uint64_t raw = row_bytes ? ((1u << 20) / 8) / row_bytes : 32768;
+  if (raw < 2048) raw = 2048;
+  if (raw > 32768) raw = 32768;
+  p->batch_rows = (uint32_t)(raw & ~63ull);
+  /* directRangeOf (:509-530) over the key column's min/max */
+  if (d->n_keys == 1 && t->n_rows > 0) {
+    const or_column* c = &t->cols[d->keys[0].column_idx];
+    int64_t mn = load_i(c, 0), mx = mn;
+    for (uint64_t r = 1; r < t->n_rows; r++) {
+      int64_t v = load_i(c, r);
+      if (v < mn) mn = v;
+      if (v > mx) mx = v;
+    }
+    uint64_t mask = p->parts[0].width >= 64 ? ~0ull : ((1ull << p->parts[0].width) - 1);
+    uint64_t base = (uint64_t)mn & mask;
+    uint64_t span = (((uint64_t)mx - (uint64_t)mn) & mask) + 1;
+    uint64_t est = d->est_groups ? d->est_groups : span;
+    if (span != 0 && span <= DIRECT_MAX_SPAN && span <= DIRECT_SPAN_PER_GROUP * (est ? est : 1)) {
+      p->direct = 1;
+      p->direct_base = base;
+      p->direct_mask = mask;
+      p->direct_span = (uint32_t)span;
+    }
+  }
+  return 0;
+}
+
+int32_t oracle_group_by(const tdbg_plan_desc* d, const or_table* t, uint32_t dop, or_result** out) {
+  cplan p;
+  int rc = compile(&p, d, t);
+  if (rc) return rc;
+  if (dop == 0) dop = 1;
+  if (t->rg_rows == 0) { set_err("rg_rows"); return TDBG_ERR_INVALID; }
+  uint64_t n_rgs = (t->n_rows + t->rg_rows - 1) / t->rg_rows;
+  uint64_t n_tiles = (n_rgs + TILE_RGS - 1) / TILE_RGS;
+  if (dop > n_tiles && n_tiles > 0) dop = (uint32_t)n_tiles;
+  uint64_t est = d->est_groups ? d->est_groups : 1024;
+  wstate* ws = calloc(dop, sizeof(wstate));
+  worker_arg* args = calloc(dop, sizeof(worker_arg));
+  pthread_t* th = calloc(dop, sizeof(pthread_t));
+  _Atomic uint64_t cursor = 0;
+  int fail = !ws || !args || !th;
+  for (uint32_t i = 0; i < dop && !fail; i++) {
+    if (ws_init(&ws[i], &p, est)) fail = 1;
+    args[i] = (worker_arg){&p, &ws[i], &cursor, n_tiles};
+  }
+  if (!fail) {
+    if (dop == 1) {
+      worker_main(&args[0]);
+    } else {
+      for (uint32_t i = 0; i < dop; i++) pthread_create(&th[i], NULL, worker_main, &args[i]);
+      for (uint32_t i = 0; i < dop; i++) pthread_join(th[i], NULL);
+    }
+    for (uint32_t i = 0; i < dop; i++) if (ws[i].err) fail = 1;
+  }
+  wstate merged;
+  int have_merged = 0;
+  if (!fail) {
+    if (dop == 1) {
+      merged = ws[0];
+      memset(&ws[0], 0, sizeof ws[0]);
+    } else {
+      if (ws_init(&merged, &p, est)) fail = 1;
+      for (uint32_t i = 0; i < dop && !fail; i++) if (merge_into(&merged, &p, &ws[i])) fail = 1;
+    }
+    have_merged = 1;
+  }
+  if (!fail) {
+    /* COUNT(*) reads `counts`; the worker slots only track ns for inputs */
+    if (emit(&p, &merged, out)) fail = 1;
+  }
+  if (have_merged) ws_free(&merged);
+  for (uint32_t i = 0; ws && i < dop; i++) ws_free(&ws[i]);
+  free(ws); free(args); free(th);
+  if (fail) { set_err("out of memory"); return TDBG_ERR_OOM; }
+  return 0;
+}

@@ -1,0 +1,297 @@
+"""Shared test helpers: golden tables, a pure-Python brute force, and result
+comparison under the reference's tie rule."""
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from thindb_amd.exec import And, AggSpec, Column, Predicate, Request, SortSpec, Table
+from thindb_amd.gpu import Type
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+TYPE_NAMES = {t.name: t for t in Type}
+INT_RANGE = {Type.BOOLEAN: (0, 1), Type.TINYINT: (-128, 127), Type.SMALLINT: (-32768, 32767),
+             Type.INT: (-2 ** 31, 2 ** 31 - 1), Type.DATE: (-2 ** 31, 2 ** 31 - 1),
+             Type.BIGINT: (-2 ** 63, 2 ** 63 - 1), Type.DATETIME: (-2 ** 63, 2 ** 63 - 1),
+             Type.DECIMAL64: (-2 ** 63, 2 ** 63 - 1)}
+
+
+def golden(name: str) -> dict:
+    return json.loads((GOLDEN / f"{name}.json").read_text())
+
+
+def _schema(spec) -> List[Column]:
+    out = []
+    for s in spec:
+        out.append(Column(s[0], TYPE_NAMES[s[1]], bool(s[2]), s[3] if len(s) > 3 else 0))
+    return out
+
+
+def wrap_table() -> Table:
+    g = golden("wrap_sum")
+    schema = _schema(g["schema"])
+    f = g["filler"]
+    n = len(g["head"]) + f["rows"]
+    ids = np.empty(n, np.int64)
+    gs = np.empty(n, np.int32)
+    hk = np.empty(n, np.int32)
+    v = np.zeros(n, np.int64)
+    valid = np.ones(n, bool)
+    for i, (a, b, c, d) in enumerate(g["head"]):
+        ids[i], gs[i], hk[i] = a, b, c
+        if d is None:
+            valid[i] = False
+        else:
+            v[i] = d
+    h = len(g["head"])
+    ids[h:] = f["id0"] + np.arange(f["rows"])
+    gs[h:] = f["g"]
+    hk[h:] = f["hk0"] + np.arange(f["rows"])
+    v[h:] = f["v"]
+    return Table(schema, {"id": ids, "g": gs, "hk": hk, "v": v}, {"v": valid}, rg_rows=g["row_group_size"])
+
+
+def decimal_table() -> Table:
+    g = golden("decimal_group")
+    rows = g["rows"]
+    return Table(_schema(g["schema"]), {"id": np.array([r[0] for r in rows], np.int32),
+                                        "grp": np.array([r[1] for r in rows], np.int32),
+                                        "v": np.array([r[2] for r in rows], np.int64)},
+                 {"v": np.ones(len(rows), bool)})
+
+
+def float_table() -> Table:
+    g = golden("float_group")
+    rows = g["rows"]
+    return Table(_schema(g["schema"]), {"id": np.array([r[0] for r in rows], np.int64),
+                                        "k": np.array([r[1] for r in rows], np.int16),
+                                        "sd": np.array([r[2] for r in rows], np.float64),
+                                        "sf": np.array([r[3] for r in rows], np.float32)})
+
+
+def heavy_table() -> Table:
+    g = golden("heavy_groups")
+    k = []
+    for grp in range(g["heavy_groups"]):
+        k += [grp] * (6000 - 250 * grp)
+    k += [1000 + j for j in range(g["unique_rows"])]
+    k = np.array(k, np.int64)
+    total = len(k)
+    assert total == g["total"]
+    i = np.arange(total)
+    v = (i % 7).astype(np.int32)
+    ids = (i * 7919) % total
+    order = np.argsort(ids, kind="stable")  # physical order is the order key
+    schema = [Column("id", Type.BIGINT), Column("k", Type.BIGINT), Column("v", Type.INT)]
+    return Table(schema, {"id": ids[order].astype(np.int64), "k": k[order], "v": v[order]},
+                 rg_rows=g["row_group_size"])
+
+
+# ---- brute force (pure Python, exact) -------------------------------------------------
+
+def _row_passes(table: Table, leaves: Sequence[Predicate], r: int) -> bool:
+    for p in leaves:
+        c = table.column(p.col)
+        if c.name in table.validity and not table.validity[c.name][r]:
+            return False
+        x = table.columns[c.name][r]
+        lit = p.val
+        if c.type == Type.DECIMAL64:
+            from decimal import Decimal
+            lit = Decimal(repr(lit) if isinstance(lit, float) else lit).scaleb(c.scale)
+            x = int(x)
+        elif c.type in (Type.FLOAT, Type.DOUBLE):
+            x = float(x)
+        else:
+            x = int(x)
+        ok = {"eq": x == lit, "neq": x != lit, "lt": x < lit, "lte": x <= lit, "gt": x > lit, "gte": x >= lit}[p.op]
+        if not ok:
+            return False
+    return True
+
+
+def brute_force(table: Table, request: Request) -> Dict[tuple, list]:
+    """Every group's aggregates, straight from the definitions: exact
+    integers, math.fsum for floats, NULL (None) for no non-null input."""
+    leaves = []
+    if request.where_filter is not None:
+        stack = [request.where_filter]
+        while stack:
+            e = stack.pop()
+            if isinstance(e, And):
+                stack.extend(e.children)
+            else:
+                leaves.append(e)
+    groups: Dict[tuple, list] = {}
+    for r in range(table.n_rows):
+        if not _row_passes(table, leaves, r):
+            continue
+        key = tuple(int(table.columns[table.column(k).name][r]) for k in request.group_cols)
+        st = groups.setdefault(key, [[0, []] for _ in request.aggs])
+        for a, s in zip(request.aggs, st):
+            s[0] += 1  # rows
+            if a.col is None:
+                continue
+            c = table.column(a.col)
+            if c.name in table.validity and not table.validity[c.name][r]:
+                continue
+            x = table.columns[c.name][r]
+            s[1].append(float(x) if c.type in (Type.FLOAT, Type.DOUBLE) else int(x))
+    out = {}
+    for key, st in groups.items():
+        vals = []
+        for a, (rows, xs) in zip(request.aggs, st):
+            f = a.func.lower()
+            if f == "count":
+                vals.append(rows if a.col is None else len(xs))
+                continue
+            if not xs:
+                vals.append(None)
+                continue
+            c = table.column(a.col)
+            fl = c.type in (Type.FLOAT, Type.DOUBLE)
+            if f == "sum":
+                if fl:
+                    vals.append(math.fsum(xs))
+                elif c.type == Type.DECIMAL64:
+                    vals.append(sum(xs))
+                else:
+                    s = sum(xs) & ((1 << 64) - 1)
+                    vals.append(s - (1 << 64) if s >> 63 else s)
+            elif f == "avg":
+                if fl:
+                    vals.append(math.fsum(xs) / len(xs))
+                else:
+                    vals.append(float(sum(xs)) / float(len(xs)) / (10.0 ** c.scale))
+            elif f == "min":
+                vals.append(min(xs))
+            else:
+                vals.append(max(xs))
+        out[key] = vals
+    return out
+
+
+# ---- result comparison --------------------------------------------------------------------
+
+def columns_of(result) -> List[tuple]:
+    """(values, valid-bool) per column, from a GPU Batch or an OracleResult."""
+    out = []
+    if hasattr(result, "values") and hasattr(result, "row_count"):  # Batch
+        for v in result.values:
+            n = len(v.values)
+            valid = np.ones(n, bool) if v.validity is None else \
+                np.unpackbits(v.validity, bitorder="little")[:n].astype(bool)
+            out.append((v.values, valid))
+    else:
+        for c in result.columns:
+            out.append((c.values, c.valid))
+    return out
+
+
+def rows_of(result) -> List[tuple]:
+    cols = columns_of(result)
+    n = len(cols[0][0]) if cols else 0
+    rows = []
+    for r in range(n):
+        row = []
+        for vals, valid in cols:
+            if not valid[r]:
+                row.append(None)
+            else:
+                x = vals[r]
+                row.append(x.item() if hasattr(x, "item") else x)
+        rows.append(tuple(row))
+    return rows
+
+
+def _close(a, b, rel: float) -> bool:
+    if a is None or b is None:
+        return a is b
+    if isinstance(a, float) or isinstance(b, float):
+        a, b = float(a), float(b)
+        if a == b:
+            return True
+        return abs(a - b) <= rel * max(abs(a), abs(b))
+    return a == b
+
+
+def assert_same_rows(got: List[tuple], want: List[tuple], float_cols: Sequence[int] = (), rel: float = 0.0,
+                     n_keys: int = 1, order_cols: Optional[Sequence[int]] = None) -> None:
+    """Compares two results under the reference's tie rule: with ORDER BY
+    (`order_cols` = the output columns ranked on), the ranked values agree
+    position by position, every returned row is a true group row, and the
+    rows strictly above the last ranked value agree as a set.  Without
+    ORDER BY the rows agree as a set.  Columns in `float_cols` compare
+    within relative tolerance `rel`; everything else bit-exactly."""
+    assert len(got) == len(want), f"{len(got)} rows vs {len(want)}"
+
+    def eq_row(a, b):
+        return all(_close(x, y, rel) if i in float_cols else x == y for i, (x, y) in enumerate(zip(a, b)))
+
+    if order_cols is None:
+        ga = sorted(got, key=lambda r: r[:n_keys])
+        wa = sorted(want, key=lambda r: r[:n_keys])
+        for a, b in zip(ga, wa):
+            assert eq_row(a, b), f"{a} != {b}"
+        return
+    for a, b in zip(got, want):
+        for i in order_cols:
+            assert _close(a[i], b[i], rel) if i in float_cols else a[i] == b[i], f"ranked value {a} vs {b}"
+    if not got:
+        return
+    last = tuple(want[-1][i] for i in order_cols)
+    above_g = [r for r in got if tuple(r[i] for i in order_cols) != last]
+    above_w = [r for r in want if tuple(r[i] for i in order_cols) != last]
+    assert len(above_g) == len(above_w)
+    by_key = {r[:n_keys]: r for r in above_w}
+    for r in above_g:
+        assert r[:n_keys] in by_key and eq_row(r, by_key[r[:n_keys]]), f"row {r} not in expected"
+
+
+def assert_rows_in_groups(got: List[tuple], groups: Dict[tuple, list], n_keys: int, float_cols: Sequence[int] = (),
+                          rel: float = 0.0) -> None:
+    """Every returned row equals its group's brute-force aggregates."""
+    for r in got:
+        key = tuple(r[:n_keys])
+        assert key in groups, f"group {key} does not exist"
+        want = groups[key]
+        for j, (x, y) in enumerate(zip(r[n_keys:], want)):
+            i = n_keys + j
+            if i in float_cols:
+                assert _close(x, y, rel), f"group {key} col {i}: {x} vs {y}"
+            else:
+                assert x == y, f"group {key} col {i}: {x} vs {y}"
+
+
+def random_table(rng: np.random.Generator, n: int, rg_rows: int = 1000, nulls: bool = True) -> Table:
+    """Mixed-type table for the brute-force and parity tests."""
+    schema = [Column("k1", Type.INT), Column("k2", Type.TINYINT), Column("b", Type.BIGINT, nulls),
+              Column("i", Type.INT), Column("s", Type.SMALLINT, nulls), Column("d", Type.DOUBLE, nulls),
+              Column("f", Type.FLOAT), Column("m", Type.DECIMAL64, False, 2), Column("t", Type.DATETIME)]
+    cols = {
+        "k1": rng.integers(-50, 50, n).astype(np.int32),
+        "k2": rng.integers(-3, 4, n).astype(np.int8),
+        "b": rng.integers(-2 ** 62, 2 ** 62, n, dtype=np.int64),
+        "i": rng.integers(-2 ** 31, 2 ** 31 - 1, n).astype(np.int32),
+        "s": rng.integers(-30000, 30000, n).astype(np.int16),
+        "d": rng.normal(0, 1e3, n),
+        "f": rng.normal(0, 10, n).astype(np.float32),
+        "m": rng.integers(-10 ** 12, 10 ** 12, n, dtype=np.int64),
+        "t": rng.integers(0, 10 ** 12, n, dtype=np.int64),
+    }
+    validity = {}
+    if nulls:
+        validity = {"b": rng.random(n) > 0.1, "s": rng.random(n) > 0.5, "d": rng.random(n) > 0.2}
+    return Table(schema, cols, validity, rg_rows=rg_rows)
+
+
+ALL_AGGS = [AggSpec("count", None, "c"), AggSpec("count", "s", "cs"), AggSpec("sum", "b", "sb"),
+            AggSpec("sum", "i", "si"), AggSpec("avg", "i", "ai"), AggSpec("min", "s", "mins"),
+            AggSpec("max", "b", "maxb"), AggSpec("sum", "d", "sd"), AggSpec("avg", "d", "ad"),
+            AggSpec("min", "f", "minf"), AggSpec("max", "d", "maxd"), AggSpec("sum", "m", "sm"),
+            AggSpec("avg", "m", "am"), AggSpec("min", "t", "mint")]

@@ -1,0 +1,267 @@
+"""GPU parity: the HIP engine, called through the C-ABI, against the oracle.
+
+Integer, decimal, COUNT, MIN and MAX results are compared bit-exactly; DOUBLE
+SUM/AVG within 1e-12 relative error (the worst error is printed).  ORDER BY
+results follow the reference's tie rule (helpers.assert_same_rows).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import (ALL_AGGS, assert_rows_in_groups, assert_same_rows, brute_force, decimal_table, float_table,
+                     golden, heavy_table, random_table, rows_of, wrap_table)
+from thindb_amd import gen
+from thindb_amd.exec import (And, AggSpec, Batch, Column, GpuSession, Predicate, Request, SortSpec, Table,
+                             try_build_gpu_group)
+from thindb_amd.gpu import Type
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+def run_gpu(session, table, req):
+    print(f"[gpu] {table.n_rows} rows keys={list(req.group_cols)} aggs={len(req.aggs)} "
+          f"order={[(s.col, s.desc) for s in req.order_specs]} limit={req.limit}", flush=True)
+    q = try_build_gpu_group(session, table, req)
+    assert q is not None, "declined"
+    b = q.next()
+    assert q.next() is None
+    stats = q.plan.stats()
+    q.deinit()
+    return b, stats
+
+
+def float_cols_of(table, req):
+    nk = len(req.group_cols)
+    out = []
+    for j, a in enumerate(req.aggs):
+        if a.col is None:
+            continue
+        c = table.column(a.col)
+        if a.func.lower() == "avg" or (a.func.lower() == "sum" and c.type in (Type.FLOAT, Type.DOUBLE)):
+            out.append(nk + j)
+    return out
+
+
+def worst_rel(got, want, cols):
+    w = 0.0
+    by = {r[:1]: r for r in want}
+    for r in got:
+        o = by.get(r[:1])
+        if o is None:
+            continue
+        for i in cols:
+            if r[i] is None or o[i] is None or o[i] == 0:
+                continue
+            w = max(w, abs(r[i] - o[i]) / abs(o[i]))
+    return w
+
+
+def check_vs_oracle(session, oracle_lib, table, req, dop=4):
+    b, st = run_gpu(session, table, req)
+    got = rows_of(b)
+    want_res = oracle_lib.run(table, req, dop)
+    want = rows_of(want_res)
+    fc = float_cols_of(table, req)
+    nk = len(req.group_cols)
+    order_cols = None
+    if req.order_specs:
+        names = [c.name.lower() for c in b.schema]
+        order_cols = [names.index(s.col.lower()) for s in req.order_specs]
+    assert_same_rows(got, want, fc, REL, nk, order_cols)
+    if fc and got:
+        print(f"  worst DOUBLE rel error vs oracle: {worst_rel(got, want, fc):.3e}", flush=True)
+    return got, want, st
+
+
+# ---- the reference's known-answer tests ---------------------------------------------------
+
+def test_wrap_sum_known_answers(session):
+    g = golden("wrap_sum")
+    t = wrap_table()
+    for q in g["queries"]:
+        req = Request(q["keys"], [AggSpec("sum", "v", "s")], [SortSpec(c, d) for c, d in q["order"]], q["limit"])
+        b, _ = run_gpu(session, t, req)
+        assert [list(r) for r in rows_of(b)] == q["expect"], q["sql"]
+    b, _ = run_gpu(session, t, Request(["hk"], [AggSpec("sum", "v", "s")]))
+    rows = rows_of(b)
+    assert len(rows) == g["all_hk_groups"]
+    want = {1: g["WRAP_G1"], 2: g["WRAP_G2"], 3: g["WRAP_G3"]}
+    assert all(s == want.get(k, 0) for k, s in rows)
+
+
+def test_decimal_known_answers(session):
+    g = golden("decimal_group")
+    t = decimal_table()
+    b, _ = run_gpu(session, t, Request(["grp"], [AggSpec("sum", "v", "sv"), AggSpec("avg", "v", "av")],
+                                       [SortSpec("grp")]))
+    assert b.values[1].type == Type.DECIMAL128 and b.values[1].scale == 4
+    rows = rows_of(b)
+    assert [[r[0], r[1]] for r in rows] == g["sum"]
+    assert [r[2] for r in rows] == [e for _, e in g["avg_exact"]]
+
+
+def test_float_known_answers(session):
+    g = golden("float_group")
+    t = float_table()
+    aggs = [AggSpec("sum", "sd", "s"), AggSpec("min", "sd", "mn"), AggSpec("max", "sd", "mx"),
+            AggSpec("count", None, "c"), AggSpec("avg", "sd", "a"), AggSpec("sum", "sf", "sf"),
+            AggSpec("min", "sf", "mnf"), AggSpec("max", "sf", "mxf")]
+    b, _ = run_gpu(session, t, Request(["k"], aggs, [SortSpec("k")]))
+    for r in rows_of(b):
+        e = g["expect"][str(r[0])]
+        assert (r[1], r[2], r[3], r[4], r[5]) == (e["sum"], e["min"], e["max"], e["count"], e["avg"])
+        assert (r[6], r[7], r[8]) == (e["sum"], e["min"], e["max"])
+
+
+def test_heavy_groups_known_answers(session):
+    g = golden("heavy_groups")
+    t = heavy_table()
+    req = Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s")], [SortSpec("c", True)], 10)
+    b, _ = run_gpu(session, t, req)
+    assert [list(r) for r in rows_of(b)] == g["top10"]
+
+
+# ---- random typed tables: every aggregate kind, NULLs, filters, orders --------------------
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_random_tables_vs_oracle(session, oracle_lib, seed):
+    rng = np.random.default_rng(seed)
+    t = random_table(rng, 50_000, rg_rows=1000 + seed)
+    where = And((Predicate("i", "gt", -2 ** 30), Predicate("t", "lt", 7 * 10 ** 11), Predicate("s", "neq", 0),
+                 Predicate("d", "gte", -1500.5)))
+    cases = [
+        Request(["k1"], ALL_AGGS),
+        Request(["k1", "k2"], ALL_AGGS, where_filter=where),
+        Request(["k2"], ALL_AGGS, [SortSpec("sb", True)], where_filter=where),
+        Request(["k1"], ALL_AGGS, [SortSpec("ad"), SortSpec("k1", True)], limit=13, offset=4),
+        Request(["k1", "k2"], ALL_AGGS, [SortSpec("c", True), SortSpec("sm", True)], limit=50),
+        Request(["k1"], ALL_AGGS, [SortSpec("mins"), SortSpec("maxd", True)], limit=20),
+        Request(["k1"], ALL_AGGS, [SortSpec("minf"), SortSpec("am")], limit=0, offset=7),
+        Request(["k2", "k1"], ALL_AGGS, [SortSpec("mint", True), SortSpec("maxb")], limit=5),
+    ]
+    for req in cases:
+        got, want, _ = check_vs_oracle(session, oracle_lib, t, req)
+        groups = brute_force(t, req) if t.n_rows <= 50_000 and len(req.group_cols) == 1 and not req.where_filter \
+            else None
+        if groups is not None:
+            assert_rows_in_groups(got, groups, len(req.group_cols), float_cols_of(t, req), REL)
+
+
+def test_plain_config_bit_exact(session, oracle_lib):
+    """The CPU-runnable plain workload: 4M rows, GROUP BY k (1024 groups)."""
+    t = gen.table("plain", 4_000_000, seed=1)
+    req = Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"), AggSpec("min", "v", "mn"),
+                          AggSpec("max", "v", "mx")])
+    got, want, st = check_vs_oracle(session, oracle_lib, t, req, dop=8)
+    assert len(got) == 1024
+    assert sum(r[1] for r in got) == 4_000_000
+    print(f"  plain: lds_mode={st.lds_mode} global_mode={st.global_mode} scan {st.scan_ms:.3f} ms")
+
+
+def test_filtered_config_small(session, oracle_lib):
+    t = gen.table("filtered", 3_000_000, seed=2)
+    where = And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI)))
+    req = Request(["region"], [AggSpec("count", None, "c"), AggSpec("sum", "amount", "s"),
+                               AggSpec("avg", "price", "a")], [SortSpec("s", True)], 100, where_filter=where)
+    got, want, st = check_vs_oracle(session, oracle_lib, t, req, dop=8)
+    assert len(got) == 100
+    # all groups, no ORDER BY: same group set and aggregates
+    req_all = Request(["region"], req.aggs, where_filter=where)
+    got_all, want_all, st = check_vs_oracle(session, oracle_lib, t, req_all, dop=8)
+    mask = (t.columns["ts"] >= gen.FILTER_LO) & (t.columns["ts"] < gen.FILTER_HI)
+    assert sum(r[1] for r in got_all) == int(mask.sum())
+    print(f"  filtered: lds_mode={st.lds_mode} global_mode={st.global_mode}")
+
+
+def test_highcard_config_small(session, oracle_lib):
+    t = gen.table("highcard", 2_000_000, seed=3)
+    req = Request(["user_id", "bucket"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"),
+                                          AggSpec("max", "w", "m")], [SortSpec("c", True)], 10)
+    got, want, st = check_vs_oracle(session, oracle_lib, t, req, dop=8)
+    assert len(got) == 10
+    req_all = Request(["user_id", "bucket"], req.aggs)
+    got_all, want_all, st = check_vs_oracle(session, oracle_lib, t, req_all, dop=8)
+    print(f"  highcard: {len(got_all)} groups lds_mode={st.lds_mode} global_mode={st.global_mode}")
+
+
+def test_hash_table_mode(session, oracle_lib):
+    """Keys spread over a wide domain take the HBM hash table."""
+    rng = np.random.default_rng(5)
+    n = 400_000
+    keys = rng.integers(-2 ** 40, 2 ** 40, 5000)
+    t = Table([Column("k", Type.BIGINT), Column("v", Type.INT), Column("d", Type.DOUBLE)],
+              {"k": keys[rng.integers(0, 5000, n)], "v": rng.integers(-100, 100, n).astype(np.int32),
+               "d": rng.random(n)})
+    req = Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"), AggSpec("avg", "d", "a"),
+                          AggSpec("min", "v", "m")], [SortSpec("s", True), SortSpec("k")], 100)
+    got, want, st = check_vs_oracle(session, oracle_lib, t, req)
+    assert st.global_mode == 2
+    got, want, st = check_vs_oracle(session, oracle_lib, t, Request(["k"], req.aggs))
+    assert len(got) == 5000
+
+
+# ---- edge cases ----------------------------------------------------------------------------
+
+def test_empty_and_ragged(session, oracle_lib):
+    rng = np.random.default_rng(9)
+    t = random_table(rng, 10_001, rg_rows=333, nulls=True)
+    # a predicate no row satisfies, an empty range, a limit past the end, an offset past the end
+    for req in [Request(["k1"], ALL_AGGS, where_filter=Predicate("i", "gt", 2 ** 31)),
+                Request(["k1"], ALL_AGGS, where_filter=And((Predicate("i", "gt", 5), Predicate("i", "lt", 3)))),
+                Request(["k1"], ALL_AGGS, [SortSpec("c")], limit=10_000),
+                Request(["k1"], ALL_AGGS, [SortSpec("c")], limit=5, offset=10_000),
+                Request(["k2"], ALL_AGGS, where_filter=Predicate("d", "eq", 0.25))]:
+        check_vs_oracle(session, oracle_lib, t, req)
+    # one-row table
+    one = Table([Column("k", Type.SMALLINT), Column("v", Type.BIGINT, True)], {"k": np.array([7], np.int16),
+                                                                                 "v": np.array([5], np.int64)},
+                {"v": np.array([False])})
+    b, _ = run_gpu(session, one, Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"),
+                                                 AggSpec("count", "v", "cv"), AggSpec("min", "v", "m")]))
+    assert rows_of(b) == [(7, 1, None, 0, None)]
+
+
+def test_declines(session):
+    rng = np.random.default_rng(0)
+    t = random_table(rng, 1000)
+    assert try_build_gpu_group(session, t, Request(["b"], [AggSpec("count", None, "c")])) is None
+    assert try_build_gpu_group(session, t, Request(["d"], [AggSpec("count", None, "c")])) is None
+    assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("sum", "t", "x")])) is None
+    assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("count", None, "c")],
+                                                   having_filter=Predicate("c", "gt", 1))) is None
+
+
+def test_plan_reuse_same_result(session):
+    t = gen.table("plain", 500_000, seed=4)
+    req = Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s")], [SortSpec("s", True)], 50)
+    q = try_build_gpu_group(session, t, req)
+    a = rows_of(q.next())
+    for rgs in (None, [(rg.segment_id, rg.idx) for rg in t.row_groups]):
+        cols = q.plan.execute(rgs)
+        assert rows_of(Batch(q.output_schema(), cols, len(cols[0].values))) == a
+    q.deinit()
+
+
+# ---- full-size properties (config sizes; no oracle run) ------------------------------------
+
+@pytest.mark.slow
+def test_filtered_full_size_checksums(session):
+    """200M rows: sum of COUNT(*) = filtered rows; sum of SUM(amount) =
+    the filtered amount total (exact, split into 32-bit halves)."""
+    n = 200_000_000
+    t = gen.table("filtered", n, seed=2)
+    where = And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI)))
+    req = Request(["region"], [AggSpec("count", None, "c"), AggSpec("sum", "amount", "s")], where_filter=where)
+    b, st = run_gpu(session, t, req)
+    rows = rows_of(b)
+    mask = (t.columns["ts"] >= gen.FILTER_LO) & (t.columns["ts"] < gen.FILTER_HI)
+    amt = t.columns["amount"][mask]
+    want = int(np.sum(amt >> 32)) * (1 << 32) + int(np.sum(amt & 0xffffffff))
+    assert sum(r[1] for r in rows) == int(mask.sum())
+    assert sum(r[2] for r in rows) == want
+    print(f"  200M filtered: scan {st.scan_ms:.3f} ms, {st.bytes_scanned / st.scan_ms / 1e6:.1f} GB/s", flush=True)
+    session.evict_table(t)

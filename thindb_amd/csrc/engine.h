@@ -1,0 +1,194 @@
+// engine.h — structures shared by the host runtime and the gfx950 kernels of
+// libthindb_gpu.so.  Everything here is plain data passed by value as kernel
+// arguments (uniform across a launch, read through scalar loads).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tdbg {
+
+constexpr int kMaxCols = 8;     // distinct columns one plan reads
+constexpr int kMaxPreds = 8;    // merged predicate terms
+constexpr int kMaxParts = 8;    // GROUP BY key parts (src/exec/v2_lowcard_group.zig:114)
+constexpr int kMaxAggs = 16;    // aggregates (src/exec/v2_lowcard_group.zig:115)
+constexpr int kVec = 4;         // consecutive rows per thread per strip
+constexpr uint32_t kRowAlign = 256;   // row-group starts in an arena are multiples of this
+constexpr uint64_t kEmptyKey = ~0ull; // empty hash slot (relative keys stay below 2^63)
+constexpr int kLdsProbe = 32;   // LDS probe limit before a row spills to HBM
+constexpr uint32_t kLdsBudget = 160u * 1024u - 1024u;
+
+// Physical element kinds of a staged column.
+enum ColKind : uint32_t { K_U8 = 0, K_I8, K_I16, K_I32, K_I64, K_F32, K_F64 };
+
+struct ColDesc {
+  const void* data;        // arena base; element r is physical row r
+  const uint32_t* valid;   // validity bitmap words (bit 1 = valid) or null
+  uint32_t kind;
+  uint32_t pad;
+};
+
+// A predicate term after merging the WHERE's leaves per column:
+// integer ranges are inclusive [lo_i, hi_i]; float ranges carry flags.
+enum PredKind : uint32_t { P_RANGE_I = 0, P_RANGE_F, P_NEQ_I, P_NEQ_F };
+struct PredDesc {
+  uint32_t kind;
+  uint32_t col;
+  uint32_t lo_incl, hi_incl;  // float ranges
+  int64_t lo_i, hi_i;
+  double lo_f, hi_f;
+};
+
+// Key part: relative value rel = v - base, 0 <= rel < span, folded into a
+// mixed-radix group index key = sum(rel_i * stride_i).
+struct PartDesc {
+  uint32_t col;
+  uint32_t pad;
+  int64_t base;
+  uint64_t span;
+  uint64_t stride;
+};
+
+// Aggregate kernels.  Integer SUM/AVG accumulate exactly: in LDS either one
+// i64 (when rows-per-workgroup * max|v| < 2^63) or split into a u64 sum of
+// the low 32 bits and an i64 sum of the high 32 bits; in HBM as i128.
+enum AggKind : uint32_t {
+  A_COUNT_COL = 0,  // COUNT(col): non-null count
+  A_SUM_I,          // SUM / AVG of integers
+  A_SUM_F,          // SUM / AVG of floats (f64)
+  A_MIN_I, A_MAX_I, // integer MIN/MAX (i64)
+  A_MIN_F, A_MAX_F  // float MIN/MAX through the order-preserving i64 map
+};
+struct AggDesc {
+  uint32_t kind;
+  uint32_t col;       // index into cols[]
+  uint32_t nullable;  // keeps its own non-null count
+  uint32_t split;     // A_SUM_I: LDS keeps split 32-bit halves
+  uint32_t lds_off;   // byte offset of this agg's LDS array(s)
+  uint32_t lds_ns_off;// byte offset of its LDS non-null counter array
+  uint32_t g_off;     // byte offset of its state in the HBM record
+  uint32_t g_ns_off;  // byte offset of its non-null count in the record
+};
+
+enum LdsMode : uint32_t { LDS_NONE = 0, LDS_DIRECT = 1, LDS_HASH = 2 };
+enum GlobalMode : uint32_t { G_DENSE = 1, G_HASH = 2 };
+
+struct Tile {
+  uint64_t row0;
+  uint32_t n;
+  uint32_t pad;
+};
+
+// Error bits raised by kernels (host maps them to status codes).
+enum ErrBit : uint32_t { ERR_KEY_RANGE = 1u, ERR_HASH_FULL = 2u };
+
+struct ScanParams {
+  ColDesc cols[kMaxCols];
+  PredDesc preds[kMaxPreds];
+  PartDesc parts[kMaxParts];
+  AggDesc aggs[kMaxAggs];
+  uint32_t n_cols, n_preds, n_parts, n_aggs;
+  // LDS table
+  uint32_t lds_mode;
+  uint32_t lds_cap;        // slots
+  uint32_t lds_key64;      // hash keys are u64 (else u32)
+  uint32_t lds_keys_off;   // byte offsets inside dynamic LDS
+  uint32_t lds_count_off;
+  uint32_t lds_bytes;
+  // HBM table of records
+  uint32_t g_mode;
+  uint32_t g_rec;          // record stride in bytes (multiple of 8)
+  uint32_t g_count_off;    // byte offset of the u64 row count in a record
+  uint32_t g_key_off;      // byte offset of the u64 key (hash mode)
+  uint64_t g_cap;          // records (dense: key span; hash: power of two)
+  uint8_t* g_state;
+  // work
+  const Tile* tiles;
+  uint32_t n_tiles;
+  uint32_t pad0;
+  uint32_t* err;
+  uint64_t* g_inserted;    // hash mode: number of claimed records
+};
+
+// ---- ordering / emission -------------------------------------------------
+
+// One ORDER BY component value source (64-bit order-preserving words).
+enum CompSrc : uint32_t {
+  C_KEY = 0,      // key part `idx`: i64 value
+  C_COUNT,        // row count
+  C_U64_AT,       // u64 at record offset (COUNT(col) non-null count)
+  C_I64_AT,       // i64 at record offset (MIN/MAX int, wrapped BIGINT SUM low word)
+  C_I128_HI,      // high word of an i128 at record offset (DECIMAL SUM)
+  C_I128_LO,      // low word (unsigned)
+  C_F64_AT,       // f64 at record offset
+  C_FORD_AT,      // float MIN/MAX stored through the order map
+  C_AVG_I,        // i128 sum / n  (unscaled mean)
+  C_AVG_F         // f64 sum / n
+};
+struct CompDesc {
+  uint32_t src;
+  uint32_t idx;      // key part index
+  uint32_t off;      // record offset of the value
+  uint32_t ns_off;   // record offset of the non-null count, or ~0 (use count)
+  uint32_t desc;
+  uint32_t pad;
+};
+
+constexpr int kMaxComps = 12;
+struct OrderParams {
+  CompDesc comps[kMaxComps];
+  uint32_t n_comps;
+  uint32_t n_parts;
+  PartDesc parts[kMaxParts];
+  const uint8_t* g_state;
+  uint32_t g_rec, g_count_off, g_key_off, g_mode;
+};
+
+// Output column writer for the finalize kernel.
+enum OutKind : uint32_t {
+  O_KEY = 0,     // key part value, narrowed to the key type
+  O_COUNT,       // COUNT(*)
+  O_NS,          // COUNT(col)
+  O_SUM_I64,     // integer SUM -> BIGINT (wrap)
+  O_SUM_I128,    // DECIMAL SUM -> DECIMAL128
+  O_SUM_F64,     // float SUM -> DOUBLE
+  O_AVG_I,       // integer/decimal AVG -> DOUBLE
+  O_AVG_F,       // float AVG -> DOUBLE
+  O_MINMAX_I,    // integer MIN/MAX -> input type
+  O_MINMAX_F     // float MIN/MAX -> input type
+};
+struct OutDesc {
+  uint32_t kind;
+  uint32_t idx;      // key part
+  uint32_t bytes;    // element width of the output type
+  uint32_t is_f32;   // float output narrowed to f32
+  uint32_t off;      // record offset of the state
+  uint32_t ns_off;   // record offset of the non-null count or ~0
+  uint64_t data_off; // byte offset of the column in the output buffer
+  uint64_t valid_off;// byte offset of its validity bytes, or ~0
+  double scale_div;  // 10^scale for AVG of DECIMAL
+};
+constexpr int kMaxOut = kMaxParts + kMaxAggs;
+struct FinalParams {
+  OutDesc outs[kMaxOut];
+  uint32_t n_outs;
+  uint32_t n_parts;
+  PartDesc parts[kMaxParts];
+  const uint8_t* g_state;
+  uint32_t g_rec, g_count_off, g_key_off, g_mode;
+};
+
+// ---- kernel launchers (defined in the .hip files) -------------------------
+hipError_t launch_fill_records(uint8_t* state, uint64_t n_rec, uint32_t rec_bytes, const uint64_t* tmpl,
+                               hipStream_t s);
+hipError_t launch_scan_agg(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s);
+hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_counts, uint32_t* gids,
+                          uint64_t* n_out, hipStream_t s);
+hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_t need, void* scratch,
+                       size_t scratch_bytes, uint32_t* out_sel, hipStream_t s);
+size_t sort_scratch_bytes(uint64_t n, uint32_t n_comps);
+hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
+                           hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+}  // namespace tdbg

@@ -1,0 +1,672 @@
+// order.hip — group compaction, ORDER BY / LIMIT selection and result
+// emission on gfx950.
+//
+// Replaces the reference's emit half: emitSorted/SortCtx.lessThan/
+// compareBySpec and emitGroup (src/exec/v2_lowcard_group.zig:2069-2311), the
+// silo's top-N (src/exec/silo_group_core.zig:4856-5210, adapter
+// src/exec/v2_shape_group_topn.zig:1095-1180, :1399-1470) and the bounded
+// exec.TopN (src/exec/topn.zig:1-60).
+//
+// Order semantics: every ORDER BY item becomes one or two 64-bit words whose
+// unsigned order is the reference's order for that item (desc = bitwise
+// NOT); the final tie-break is the group's position in slot order.  The
+// reference breaks ties by gid (first-seen order), which its parallel scan
+// leaves unspecified; slot order is one admissible choice and is
+// deterministic here.
+#include "engine.h"
+
+namespace tdbg {
+
+// ---- shared helpers ----------------------------------------------------------
+
+__device__ __forceinline__ uint64_t ord_i64(int64_t v) { return (uint64_t)v ^ 0x8000000000000000ull; }
+
+// types.floatOrder (src/types.zig:579-587): -0 == +0, NaN largest.
+__device__ __forceinline__ uint64_t ord_f64(double x) {
+  if (x != x) return ~0ull;
+  if (x == 0.0) x = 0.0;
+  uint64_t b = (uint64_t)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Inverse of the order-preserving i64 map used for float MIN/MAX states.
+__device__ __forceinline__ double unord_f64(int64_t k) {
+  int64_t s = k >= 0 ? k : (k ^ 0x7fffffffffffffffll);
+  return __longlong_as_double(s);
+}
+
+// i128 -> f64 with round-to-nearest-even (Zig's @floatFromInt).
+__device__ __forceinline__ double i128_to_f64(__int128 v) {
+  const bool neg = v < 0;
+  unsigned __int128 a = neg ? (unsigned __int128)(-(v + 1)) + 1u : (unsigned __int128)v;
+  const uint64_t hi = (uint64_t)(a >> 64);
+  double r;
+  if (hi == 0) {
+    r = (double)(uint64_t)a;  // u64 -> f64 is correctly rounded
+  } else {
+    const int lz = __clzll(hi);
+    const int drop = 64 - lz;  // bits beyond a 64-bit window
+    uint64_t m = (uint64_t)(a >> drop);
+    const unsigned __int128 rest = a & ((((unsigned __int128)1) << drop) - 1u);
+    if (rest != 0) m |= 1u;  // sticky bit below the rounding position
+    r = ldexp((double)m, drop);
+  }
+  return neg ? -r : r;
+}
+
+__device__ __forceinline__ __int128 load_i128(const uint8_t* p) {
+  const uint64_t lo = *reinterpret_cast<const uint64_t*>(p);
+  const int64_t hi = *reinterpret_cast<const int64_t*>(p + 8);
+  return (__int128)(((unsigned __int128)(uint64_t)hi << 64) | lo);
+}
+
+__device__ __forceinline__ uint64_t group_key(const uint8_t* rec, uint32_t gid, uint32_t g_mode, uint32_t key_off) {
+  return g_mode == G_DENSE ? (uint64_t)gid : *reinterpret_cast<const uint64_t*>(rec + key_off);
+}
+
+__device__ __forceinline__ int64_t part_value(const PartDesc& pt, uint64_t key) {
+  const uint64_t rel = (key / pt.stride) % pt.span;
+  return (int64_t)((uint64_t)pt.base + rel);
+}
+
+__device__ __forceinline__ uint64_t rec_ns(const uint8_t* rec, uint32_t ns_off, uint32_t count_off) {
+  return *reinterpret_cast<const uint64_t*>(rec + (ns_off != ~0u ? ns_off : count_off));
+}
+
+// One ORDER BY word of one group.
+__device__ uint64_t comp_word(const OrderParams& P, const CompDesc& c, uint32_t gid) {
+  const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
+  uint64_t w = 0;
+  switch (c.src) {
+    case C_KEY: w = ord_i64(part_value(P.parts[c.idx], group_key(rec, gid, P.g_mode, P.g_key_off))); break;
+    case C_COUNT: w = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off); break;
+    case C_U64_AT: w = *reinterpret_cast<const uint64_t*>(rec + c.off); break;
+    case C_I64_AT:
+    case C_I128_HI:
+    case C_I128_LO:
+    case C_F64_AT:
+    case C_FORD_AT: {
+      // A group with no non-null input compares as a zero slot
+      // (compareBySpec reads the untouched slot).
+      const bool empty = rec_ns(rec, c.ns_off, P.g_count_off) == 0;
+      if (c.src == C_I64_AT) w = ord_i64(empty ? 0 : *reinterpret_cast<const int64_t*>(rec + c.off));
+      else if (c.src == C_I128_HI) w = ord_i64(empty ? 0 : *reinterpret_cast<const int64_t*>(rec + c.off + 8));
+      else if (c.src == C_I128_LO) w = empty ? 0 : *reinterpret_cast<const uint64_t*>(rec + c.off);
+      else if (c.src == C_F64_AT) w = ord_f64(empty ? 0.0 : *reinterpret_cast<const double*>(rec + c.off));
+      else w = ord_f64(empty ? 0.0 : unord_f64(*reinterpret_cast<const int64_t*>(rec + c.off)));
+      break;
+    }
+    case C_AVG_I: {
+      const uint64_t n = rec_ns(rec, c.ns_off, P.g_count_off);
+      w = ord_f64(n == 0 ? 0.0 : i128_to_f64(load_i128(rec + c.off)) / (double)n);
+      break;
+    }
+    default: {  // C_AVG_F
+      const uint64_t n = rec_ns(rec, c.ns_off, P.g_count_off);
+      w = ord_f64(n == 0 ? 0.0 : *reinterpret_cast<const double*>(rec + c.off) / (double)n);
+      break;
+    }
+  }
+  return c.desc ? ~w : w;
+}
+
+// ---- compaction: occupied records in slot order --------------------------------
+
+constexpr uint32_t kCompactThreads = 256;
+constexpr uint32_t kCompactPer = 16;
+constexpr uint32_t kCompactChunk = kCompactThreads * kCompactPer;
+
+__device__ __forceinline__ bool occupied(const OrderParams& P, uint64_t i) {
+  return *reinterpret_cast<const uint64_t*>(P.g_state + i * P.g_rec + P.g_count_off) != 0;
+}
+
+__global__ void __launch_bounds__(kCompactThreads) compact_count_kernel(const OrderParams P, uint64_t n_rec,
+                                                                        uint32_t* block_counts) {
+  __shared__ uint32_t red[kCompactThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + (uint64_t)threadIdx.x * kCompactPer;
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if (base + j < n_rec && occupied(P, base + j)) c++;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kCompactThreads / 64; w++) t += red[w];
+    block_counts[blockIdx.x] = t;
+  }
+}
+
+// Exclusive scan of `n` u32 counts by one workgroup; total -> *total.
+__global__ void __launch_bounds__(1024) scan_counts_kernel(uint32_t* counts, uint32_t n, uint64_t* total) {
+  __shared__ uint64_t part[1024];
+  const uint32_t per = (n + 1023) / 1024;
+  const uint32_t lo = threadIdx.x * per, hi = lo + per < n ? lo + per : n;
+  uint64_t s = 0;
+  for (uint32_t i = lo; i < hi; i++) s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t o = 1; o < 1024; o <<= 1) {
+    uint64_t v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint64_t run = part[threadIdx.x] - s;
+  for (uint32_t i = lo; i < hi; i++) {
+    uint32_t c = counts[i];
+    counts[i] = (uint32_t)run;
+    run += c;
+  }
+  if (threadIdx.x == 1023) *total = part[1023];
+}
+
+__global__ void __launch_bounds__(kCompactThreads) compact_write_kernel(const OrderParams P, uint64_t n_rec,
+                                                                        const uint32_t* block_offs,
+                                                                        uint32_t* gids) {
+  __shared__ uint32_t pre[kCompactThreads];
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + (uint64_t)threadIdx.x * kCompactPer;
+  uint32_t bits = 0, c = 0;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if (base + j < n_rec && occupied(P, base + j)) { bits |= 1u << j; c++; }
+  pre[threadIdx.x] = c;
+  __syncthreads();
+  for (uint32_t o = 1; o < kCompactThreads; o <<= 1) {
+    uint32_t v = threadIdx.x >= o ? pre[threadIdx.x - o] : 0;
+    __syncthreads();
+    pre[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t out = block_offs[blockIdx.x] + pre[threadIdx.x] - c;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if ((bits >> j) & 1u) gids[out++] = (uint32_t)(base + j);
+}
+
+hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_counts, uint32_t* gids,
+                          uint64_t* n_out, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)((n_rec + kCompactChunk - 1) / kCompactChunk);
+  if (blocks == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(compact_count_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, p, n_rec, block_counts);
+  hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, block_counts, blocks, n_out);
+  hipLaunchKernelGGL(compact_write_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, p, n_rec,
+                     (const uint32_t*)block_counts, gids);
+  return hipGetLastError();
+}
+
+// ---- order words ---------------------------------------------------------------
+
+__global__ void make_comps_kernel(const OrderParams P, const uint32_t* __restrict__ gids, uint64_t n,
+                                  uint64_t* comps) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t gid = gids[i];
+    for (uint32_t c = 0; c < P.n_comps; c++) comps[c * n + i] = comp_word(P, P.comps[c], gid);
+  }
+}
+
+// Composite order of positions a, b: words 0..n_comps-1, then position.
+__device__ __forceinline__ bool pos_less(const uint64_t* comps, uint32_t n_comps, uint64_t n, uint32_t a,
+                                         uint64_t wa, uint32_t b, uint64_t wb) {
+  if (wa != wb) return wa < wb;
+  for (uint32_t c = 1; c < n_comps; c++) {
+    const uint64_t xa = comps[c * n + a], xb = comps[c * n + b];
+    if (xa != xb) return xa < xb;
+  }
+  return a < b;
+}
+
+// ---- small sort: one workgroup, bitonic over (word0, position) in LDS -----------
+
+constexpr uint32_t kSmallSort = 4096;
+
+__global__ void __launch_bounds__(1024) small_sort_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n_all,
+                                                          const uint32_t* pos_in, uint32_t n, uint32_t* pos_out) {
+  __shared__ uint32_t sp[kSmallSort];
+  __shared__ uint64_t sw[kSmallSort];
+  uint32_t m = 1;
+  while (m < n) m <<= 1;
+  for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+    if (i < n) {
+      const uint32_t p = pos_in ? pos_in[i] : i;
+      sp[i] = p;
+      sw[i] = comps[p];
+    } else {
+      sp[i] = 0xffffffffu;
+      sw[i] = ~0ull;
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const uint32_t l = i ^ j;
+        if (l > i) {
+          const bool up = (i & k) == 0;
+          const uint32_t pa = sp[i], pb = sp[l];
+          bool less;  // element l < element i
+          if (pb == 0xffffffffu) less = false;
+          else if (pa == 0xffffffffu) less = true;
+          else less = pos_less(comps, n_comps, n_all, pb, sw[l], pa, sw[i]);
+          if (less == up) {
+            sp[i] = pb; sp[l] = pa;
+            const uint64_t t = sw[i]; sw[i] = sw[l]; sw[l] = t;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) pos_out[i] = sp[i];
+}
+
+// ---- per-workgroup top-k (k <= 64) ----------------------------------------------
+
+constexpr uint32_t kTopThreads = 256;
+constexpr uint32_t kTopPer = 16;
+constexpr uint32_t kTopChunk = kTopThreads * kTopPer;
+
+__global__ void __launch_bounds__(kTopThreads) block_topk_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n,
+                                                                 uint32_t k, uint32_t* cand) {
+  __shared__ uint64_t rw[kTopThreads / 64];
+  __shared__ uint32_t rp[kTopThreads / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * kTopChunk + threadIdx.x;
+  uint32_t taken = 0;  // bit j: item j already emitted
+  for (uint32_t r = 0; r < k; r++) {
+    // this thread's best remaining item
+    uint64_t bw = ~0ull;
+    uint32_t bp = 0xffffffffu;
+    for (uint32_t j = 0; j < kTopPer; j++) {
+      const uint64_t p = base + (uint64_t)j * kTopThreads;
+      if (p >= n || ((taken >> j) & 1u)) continue;
+      const uint64_t w = comps[p];
+      if (bp == 0xffffffffu || pos_less(comps, n_comps, n, (uint32_t)p, w, bp, bw)) { bw = w; bp = (uint32_t)p; }
+    }
+    // wave reduction
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t ow = __shfl_xor(bw, o);
+      const uint32_t op = __shfl_xor(bp, o);
+      if (op != 0xffffffffu && (bp == 0xffffffffu || pos_less(comps, n_comps, n, op, ow, bp, bw))) { bw = ow; bp = op; }
+    }
+    if ((threadIdx.x & 63) == 0) { rw[threadIdx.x >> 6] = bw; rp[threadIdx.x >> 6] = bp; }
+    __syncthreads();
+    uint64_t fw = rw[0];
+    uint32_t fp = rp[0];
+    for (uint32_t w = 1; w < kTopThreads / 64; w++) {
+      if (rp[w] != 0xffffffffu && (fp == 0xffffffffu || pos_less(comps, n_comps, n, rp[w], rw[w], fp, fw))) {
+        fw = rw[w]; fp = rp[w];
+      }
+    }
+    __syncthreads();
+    // Every workgroup but the last holds at least k items, so the
+    // candidates land contiguously: workgroup b writes [b*k, b*k + its count).
+    if (threadIdx.x == 0 && fp != 0xffffffffu) cand[(uint64_t)blockIdx.x * k + r] = fp;
+    if (fp != 0xffffffffu && (fp % kTopThreads) == threadIdx.x && fp / kTopChunk == blockIdx.x) {
+      taken |= 1u << (uint32_t)((fp - blockIdx.x * (uint64_t)kTopChunk) / kTopThreads);
+    }
+  }
+}
+
+// ---- stable LSD radix sort of positions by the composite order --------------------
+
+constexpr uint32_t kRsThreads = 256;
+constexpr uint32_t kRsPer = 8;
+constexpr uint32_t kRsTile = kRsThreads * kRsPer;
+
+__global__ void gather_keys_kernel(const uint64_t* comp, const uint32_t* pos, uint64_t n, uint64_t* keys) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    keys[i] = comp[pos[i]];
+}
+
+__global__ void minmax_kernel(const uint64_t* keys, uint64_t n, uint64_t* mm) {
+  uint64_t lo = ~0ull, hi = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = keys[i];
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin((unsigned long long*)&mm[0], (unsigned long long)lo);
+    atomicMax((unsigned long long*)&mm[1], (unsigned long long)hi);
+  }
+}
+
+__global__ void init_mm_kernel(uint64_t* mm) {
+  mm[0] = ~0ull;
+  mm[1] = 0;
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t k, const uint64_t* mm, uint32_t shift) {
+  return (uint32_t)(((k - mm[0]) >> shift) & 0xffu);
+}
+
+// hist[d * n_blocks + b] = number of digit d in tile b.  Skipped (all zero)
+// when the key range needs no bits at this shift.
+__global__ void __launch_bounds__(kRsThreads) rs_hist_kernel(const uint64_t* keys, uint64_t n, const uint64_t* mm,
+                                                             uint32_t shift, uint32_t* hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  for (uint32_t j = 0; j < kRsPer; j++) {
+    const uint64_t i = base + (uint64_t)j * kRsThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[digit_of(keys[i], mm, shift)], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Generic exclusive scan of u32 in three phases (chunk sums, scan, apply).
+constexpr uint32_t kScanChunk = 2048;
+__global__ void scan_chunk_sums_kernel(const uint32_t* a, uint64_t n, uint32_t* sums) {
+  __shared__ uint32_t red[4];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  uint32_t s = 0;
+  for (uint32_t j = threadIdx.x; j < kScanChunk; j += 256)
+    if (base + j < n) s += a[base + j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+__global__ void scan_apply_kernel(uint32_t* a, uint64_t n, const uint32_t* offs) {
+  __shared__ uint32_t v[kScanChunk];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+  for (uint32_t j = threadIdx.x; j < kScanChunk; j += 256) v[j] = base + j < n ? a[base + j] : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = offs[blockIdx.x];
+    for (uint32_t j = 0; j < kScanChunk; j++) {
+      const uint32_t c = v[j];
+      v[j] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (uint32_t j = threadIdx.x; j < kScanChunk; j += 256)
+    if (base + j < n) a[base + j] = v[j];
+}
+
+// Stable scatter: round j handles items j*256 + t, so (round, wave, lane)
+// order is item order; equal digits keep their relative order.
+__global__ void __launch_bounds__(kRsThreads) rs_scatter_kernel(const uint64_t* keys_in, const uint32_t* pos_in,
+                                                                uint64_t n, const uint64_t* mm, uint32_t shift,
+                                                                const uint32_t* offs, uint64_t* keys_out,
+                                                                uint32_t* pos_out) {
+  __shared__ uint32_t wave_cnt[kRsThreads / 64][256];
+  __shared__ uint32_t wave_off[kRsThreads / 64][256];
+  __shared__ uint32_t run[256];
+  const uint32_t t = threadIdx.x, w = t >> 6, lane = t & 63;
+  for (uint32_t x = 0; x < kRsThreads / 64; x++) wave_cnt[x][t] = 0;
+  run[t] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  for (uint32_t j = 0; j < kRsPer; j++) {
+    const uint64_t i = base + (uint64_t)j * kRsThreads + t;
+    const bool live = i < n;
+    uint64_t k = 0;
+    uint32_t p = 0, d = 0;
+    if (live) {
+      k = keys_in[i];
+      p = pos_in[i];
+      d = digit_of(k, mm, shift);
+    }
+    uint64_t m = __ballot(live);
+    for (uint32_t b = 0; b < 8; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = __popcll(m & lt);
+    if (live && rank == 0) wave_cnt[w][d] = __popcll(m);
+    __syncthreads();
+    {
+      uint32_t o = run[t];
+      for (uint32_t x = 0; x < kRsThreads / 64; x++) {
+        const uint32_t c = wave_cnt[x][t];
+        wave_off[x][t] = o;
+        o += c;
+        wave_cnt[x][t] = 0;
+      }
+      run[t] = o;
+    }
+    __syncthreads();
+    if (live) {
+      const uint64_t dst = (uint64_t)offs[(uint64_t)d * gridDim.x + blockIdx.x] + wave_off[w][d] + rank;
+      keys_out[dst] = k;
+      pos_out[dst] = p;
+    }
+    __syncthreads();
+  }
+}
+
+static inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap) {
+  uint64_t g = (n + block - 1) / block;
+  if (g > cap) g = cap;
+  return g == 0 ? 1 : (uint32_t)g;
+}
+
+// Scratch layout for the sorter.
+struct SortScratch {
+  uint64_t* comps;     // n_comps * n
+  uint64_t* keys_a;    // n
+  uint64_t* keys_b;    // n
+  uint32_t* pos_a;     // n
+  uint32_t* pos_b;     // n
+  uint32_t* hist;      // 256 * tiles
+  uint32_t* sums;      // hist chunks
+  uint32_t* sums2;     // chunk sums of sums (one scan level more)
+  uint64_t* mm;        // 2 words
+  uint64_t* total;     // 1 word
+};
+
+static inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t sort_scratch_bytes(uint64_t n, uint32_t n_comps) {
+  const uint64_t tiles = (n + kRsTile - 1) / kRsTile + 1;
+  const uint64_t hist = 256 * tiles;
+  const uint64_t chunks = (hist + kScanChunk - 1) / kScanChunk + 1;
+  return al256(n_comps * n * 8) + 2 * al256(n * 8) + 2 * al256(n * 4) + al256(hist * 4) + al256(chunks * 4) +
+         al256(((chunks + kScanChunk - 1) / kScanChunk + 1) * 4) + 4 * 256;
+}
+
+static SortScratch carve(void* base, uint64_t n, uint32_t n_comps) {
+  const uint64_t tiles = (n + kRsTile - 1) / kRsTile + 1;
+  const uint64_t hist = 256 * tiles;
+  const uint64_t chunks = (hist + kScanChunk - 1) / kScanChunk + 1;
+  uint8_t* p = static_cast<uint8_t*>(base);
+  SortScratch s;
+  s.comps = reinterpret_cast<uint64_t*>(p); p += al256(n_comps * n * 8);
+  s.keys_a = reinterpret_cast<uint64_t*>(p); p += al256(n * 8);
+  s.keys_b = reinterpret_cast<uint64_t*>(p); p += al256(n * 8);
+  s.pos_a = reinterpret_cast<uint32_t*>(p); p += al256(n * 4);
+  s.pos_b = reinterpret_cast<uint32_t*>(p); p += al256(n * 4);
+  s.hist = reinterpret_cast<uint32_t*>(p); p += al256(hist * 4);
+  s.sums = reinterpret_cast<uint32_t*>(p); p += al256(chunks * 4);
+  s.sums2 = reinterpret_cast<uint32_t*>(p); p += al256(((chunks + kScanChunk - 1) / kScanChunk + 1) * 4);
+  s.mm = reinterpret_cast<uint64_t*>(p); p += 256;
+  s.total = reinterpret_cast<uint64_t*>(p);
+  return s;
+}
+
+// Exclusive scan of a u32 array of length m (m <= 2048^3).
+static void scan_u32(uint32_t* a, uint64_t m, uint32_t* sums, uint32_t* sums2, uint64_t* total, hipStream_t s) {
+  const uint32_t chunks = (uint32_t)((m + kScanChunk - 1) / kScanChunk);
+  hipLaunchKernelGGL(scan_chunk_sums_kernel, dim3(chunks), dim3(256), 0, s, a, m, sums);
+  if (chunks <= 1024 * 64) {
+    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, sums, chunks, total);
+  } else {
+    const uint32_t c2 = (chunks + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(scan_chunk_sums_kernel, dim3(c2), dim3(256), 0, s, sums, (uint64_t)chunks, sums2);
+    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, sums2, c2, total);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(c2), dim3(256), 0, s, sums, (uint64_t)chunks, (const uint32_t*)sums2);
+  }
+  hipLaunchKernelGGL(scan_apply_kernel, dim3(chunks), dim3(256), 0, s, a, m, (const uint32_t*)sums);
+}
+
+// Sorts m positions (pos_a) by the composite order; result in pos_a.
+static hipError_t radix_sort_positions(SortScratch& S, uint64_t n_all, uint32_t n_comps, uint64_t m,
+                                       hipStream_t s) {
+  const uint32_t tiles = (uint32_t)((m + kRsTile - 1) / kRsTile);
+  for (int c = (int)n_comps - 1; c >= 0; c--) {
+    const uint64_t* comp = S.comps + (uint64_t)c * n_all;
+    hipLaunchKernelGGL(gather_keys_kernel, dim3(grid_for(m, 256, 8192)), dim3(256), 0, s, comp,
+                       (const uint32_t*)S.pos_a, m, S.keys_a);
+    hipLaunchKernelGGL(init_mm_kernel, dim3(1), dim3(1), 0, s, S.mm);
+    hipLaunchKernelGGL(minmax_kernel, dim3(grid_for(m, 256, 1024)), dim3(256), 0, s, (const uint64_t*)S.keys_a, m,
+                       S.mm);
+    uint64_t mm[2];
+    hipError_t e = hipMemcpyAsync(mm, S.mm, sizeof mm, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const uint64_t range = mm[1] - mm[0];
+    int bits = range == 0 ? 0 : 64 - __builtin_clzll(range);
+    for (int shift = 0; shift < bits; shift += 8) {
+      hipLaunchKernelGGL(rs_hist_kernel, dim3(tiles), dim3(kRsThreads), 0, s, (const uint64_t*)S.keys_a, m,
+                         (const uint64_t*)S.mm, (uint32_t)shift, S.hist);
+      scan_u32(S.hist, (uint64_t)tiles * 256, S.sums, S.sums2, S.total, s);
+      hipLaunchKernelGGL(rs_scatter_kernel, dim3(tiles), dim3(kRsThreads), 0, s, (const uint64_t*)S.keys_a,
+                         (const uint32_t*)S.pos_a, m, (const uint64_t*)S.mm, (uint32_t)shift,
+                         (const uint32_t*)S.hist, S.keys_b, S.pos_b);
+      uint64_t* tk = S.keys_a; S.keys_a = S.keys_b; S.keys_b = tk;
+      uint32_t* tp = S.pos_a; S.pos_a = S.pos_b; S.pos_b = tp;
+    }
+  }
+  return hipGetLastError();
+}
+
+__global__ void iota_kernel(uint32_t* p, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = (uint32_t)i;
+}
+
+__global__ void select_gids_kernel(const uint32_t* gids, const uint32_t* pos, uint64_t k, uint32_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = gids[pos[i]];
+}
+
+// Writes the first `need` gids in ORDER BY order into out_sel.
+hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_t need, void* scratch,
+                       size_t scratch_bytes, uint32_t* out_sel, hipStream_t s) {
+  if (n == 0 || need == 0) return hipSuccess;
+  if (scratch_bytes < sort_scratch_bytes(n, p.n_comps)) return hipErrorInvalidValue;
+  SortScratch S = carve(scratch, n, p.n_comps);
+  hipLaunchKernelGGL(make_comps_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, p, (const uint32_t*)gids,
+                     n, S.comps);
+  if (n <= kSmallSort) {
+    hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps, n,
+                       (const uint32_t*)nullptr, (uint32_t)n, S.pos_a);
+  } else if (need <= 64) {
+    const uint32_t blocks = (uint32_t)((n + kTopChunk - 1) / kTopChunk);
+    hipLaunchKernelGGL(block_topk_kernel, dim3(blocks), dim3(kTopThreads), 0, s, (const uint64_t*)S.comps,
+                       p.n_comps, n, (uint32_t)need, S.pos_a);
+    const uint64_t last = n - (uint64_t)(blocks - 1) * kTopChunk;
+    const uint64_t m = (uint64_t)(blocks - 1) * need + (last < need ? last : need);
+    if (m <= kSmallSort) {
+      hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps, n,
+                         (const uint32_t*)S.pos_a, (uint32_t)m, S.pos_b);
+      uint32_t* t = S.pos_a; S.pos_a = S.pos_b; S.pos_b = t;
+    } else {
+      hipError_t e2 = radix_sort_positions(S, n, p.n_comps, m, s);
+      if (e2 != hipSuccess) return e2;
+    }
+  } else {
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, S.pos_a, n);
+    hipError_t e = radix_sort_positions(S, n, p.n_comps, n, s);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(select_gids_kernel, dim3(grid_for(need, 256, 4096)), dim3(256), 0, s, (const uint32_t*)gids,
+                     (const uint32_t*)S.pos_a, need, out_sel);
+  return hipGetLastError();
+}
+
+// ---- finalize: typed result columns ------------------------------------------------
+
+__device__ __forceinline__ void put(uint8_t* out, uint64_t off, uint32_t bytes, uint64_t row, uint64_t v) {
+  uint8_t* p = out + off + row * bytes;
+  switch (bytes) {
+    case 1: *p = (uint8_t)v; break;
+    case 2: *reinterpret_cast<uint16_t*>(p) = (uint16_t)v; break;
+    case 4: *reinterpret_cast<uint32_t*>(p) = (uint32_t)v; break;
+    default: *reinterpret_cast<uint64_t*>(p) = v; break;
+  }
+}
+
+__global__ void finalize_kernel(const FinalParams P, const uint32_t* __restrict__ sel, uint64_t n_out,
+                                uint8_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_out; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t gid = sel[i];
+    const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
+    const uint64_t count = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off);
+    for (uint32_t c = 0; c < P.n_outs; c++) {
+      const OutDesc& o = P.outs[c];
+      uint8_t valid = 1;
+      switch (o.kind) {
+        case O_KEY:
+          put(out, o.data_off, o.bytes, i, (uint64_t)part_value(P.parts[o.idx], group_key(rec, gid, P.g_mode, P.g_key_off)));
+          break;
+        case O_COUNT: put(out, o.data_off, 8, i, count); break;
+        case O_NS: put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
+        default: {
+          const uint64_t ns = o.ns_off != ~0u ? *reinterpret_cast<const uint64_t*>(rec + o.ns_off) : count;
+          if (ns == 0) {
+            valid = 0;
+            if (o.bytes == 16) {
+              put(out, o.data_off, 8, 2 * i, 0);
+              put(out, o.data_off, 8, 2 * i + 1, 0);
+            } else {
+              put(out, o.data_off, o.bytes, i, 0);
+            }
+            break;
+          }
+          switch (o.kind) {
+            case O_SUM_I64: put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
+            case O_SUM_I128:
+              put(out, o.data_off, 8, 2 * i, *reinterpret_cast<const uint64_t*>(rec + o.off));
+              put(out, o.data_off, 8, 2 * i + 1, *reinterpret_cast<const uint64_t*>(rec + o.off + 8));
+              break;
+            case O_SUM_F64:
+              put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off));
+              break;
+            case O_AVG_I: {
+              // AVG = f64(sum) / f64(n) / 10^scale (v2_lowcard_group.zig:2128-2139)
+              double a = i128_to_f64(load_i128(rec + o.off)) / (double)ns;
+              if (o.scale_div != 1.0) a = a / o.scale_div;
+              put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(a));
+              break;
+            }
+            case O_AVG_F: {
+              const double a = *reinterpret_cast<const double*>(rec + o.off) / (double)ns;
+              put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(a));
+              break;
+            }
+            case O_MINMAX_I: put(out, o.data_off, o.bytes, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
+            default: {  // O_MINMAX_F
+              const double v = unord_f64(*reinterpret_cast<const int64_t*>(rec + o.off));
+              if (o.is_f32) {
+                const float f = (float)v;
+                put(out, o.data_off, 4, i, (uint64_t)(uint32_t)__float_as_uint(f));
+              } else {
+                put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(v));
+              }
+              break;
+            }
+          }
+        }
+      }
+      if (o.valid_off != ~0ull) out[o.valid_off + i] = valid;
+    }
+  }
+}
+
+hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out, hipStream_t s) {
+  if (n_out == 0) return hipSuccess;
+  hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(n_out, 256, 8192)), dim3(256), 0, s, p, sel, n_out, out);
+  return hipGetLastError();
+}
+
+}  // namespace tdbg

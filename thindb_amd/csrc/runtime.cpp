@@ -1,0 +1,1151 @@
+// runtime.cpp — host side of libthindb_gpu.so: the C-ABI of
+// include/thindb_gpu.h over HIP.
+//
+//  * Context: one GPU, an HBM byte budget (admission: allocations beyond it
+//    fail with TDBG_ERR_OOM, the analog of src/memory.zig:31-35), and the
+//    resident tables.
+//  * HBM column store: per (table_uid, column_idx) one contiguous arena;
+//    every staged row group gets one row range [row_off, row_off + n) shared
+//    by all its columns, row_off a multiple of kRowAlign, so a scan is a
+//    list of row ranges over flat arrays (the decoded-block cache of
+//    src/storage/cache.zig:1-8 keyed like cache.Key, :59-70; segments are
+//    immutable, so nothing is ever invalidated).
+//  * Plan: the validated request; at execution it is specialised to the
+//    table's current statistics (key spans -> dense or hashed HBM table,
+//    LDS table shape, exact-sum mode) and run as one stream of kernels:
+//    record init -> fused scan/filter/aggregate -> compaction -> ORDER BY
+//    selection -> finalize -> one device-to-host copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/thindb_gpu.h"
+#include "engine.h"
+
+using namespace tdbg;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int32_t hip_fail(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory) return fail(TDBG_ERR_OOM, "%s: %s", what, hipGetErrorString(e));
+  return fail(TDBG_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                   \
+  do {                                        \
+    hipError_t e_ = (expr);                   \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int type_bytes(uint32_t t) {
+  switch (t) {
+    case TDBG_BOOLEAN: case TDBG_TINYINT: return 1;
+    case TDBG_SMALLINT: return 2;
+    case TDBG_INT: case TDBG_DATE: case TDBG_FLOAT: return 4;
+    case TDBG_BIGINT: case TDBG_DATETIME: case TDBG_DECIMAL64: case TDBG_DOUBLE: return 8;
+    case TDBG_DECIMAL128: return 16;
+    default: return 0;
+  }
+}
+bool is_int_type(uint32_t t) { return t >= TDBG_BOOLEAN && t <= TDBG_DECIMAL64; }
+bool is_float_type(uint32_t t) { return t == TDBG_FLOAT || t == TDBG_DOUBLE; }
+uint32_t kind_of(uint32_t t) {
+  switch (t) {
+    case TDBG_BOOLEAN: return K_U8;
+    case TDBG_TINYINT: return K_I8;
+    case TDBG_SMALLINT: return K_I16;
+    case TDBG_INT: case TDBG_DATE: return K_I32;
+    case TDBG_FLOAT: return K_F32;
+    case TDBG_DOUBLE: return K_F64;
+    default: return K_I64;
+  }
+}
+
+uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+struct ColumnArena {
+  uint32_t type = 0;
+  uint32_t elem = 0;
+  uint8_t* d_data = nullptr;
+  uint32_t* d_valid = nullptr;  // allocated once any block is nullable
+  bool has_stats = false;
+  int64_t min_i = 0, max_i = 0;
+  double min_f = 0, max_f = 0;
+};
+
+struct BlockInfo {
+  bool staged = false;
+  bool has_stats = false;
+  tdbg_block_stats stats{};
+};
+
+struct RowGroupEntry {
+  uint64_t segment_id = 0;
+  uint32_t rg_idx = 0;
+  uint32_t n_rows = 0;
+  uint64_t row_off = 0;
+  std::map<uint32_t, BlockInfo> blocks;
+};
+
+struct Table {
+  uint64_t uid = 0;
+  uint64_t cap_rows = 0;
+  uint64_t next_row = 0;
+  uint64_t reserve = 0;
+  uint64_t version = 0;
+  std::map<uint32_t, ColumnArena> cols;
+  std::vector<RowGroupEntry> rgs;
+  std::unordered_map<uint64_t, std::vector<size_t>> rg_index;  // hash(seg, rg) -> entries
+};
+
+uint64_t rg_hash(uint64_t seg, uint32_t rg) { return seg * 0x9e3779b97f4a7c15ull ^ rg; }
+
+}  // namespace
+
+struct tdbg_context {
+  int device = 0;
+  uint64_t budget = 0;
+  std::atomic<uint64_t> in_use{0};
+  std::shared_mutex mu;
+  std::unordered_map<uint64_t, std::unique_ptr<Table>> tables;
+  int n_cu = 256;
+  int max_lds = 160 * 1024;
+};
+
+namespace {
+
+int32_t dev_alloc(tdbg_context* ctx, void** p, uint64_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  uint64_t now = ctx->in_use.fetch_add(bytes) + bytes;
+  if (now > ctx->budget) {
+    ctx->in_use.fetch_sub(bytes);
+    return fail(TDBG_ERR_OOM, "HBM budget exceeded: %llu of %llu bytes", (unsigned long long)now,
+                (unsigned long long)ctx->budget);
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    ctx->in_use.fetch_sub(bytes);
+    *p = nullptr;
+    (void)hipGetLastError();
+    return hip_fail(e, "hipMalloc");
+  }
+  return TDBG_OK;
+}
+
+void dev_free(tdbg_context* ctx, void* p, uint64_t bytes) {
+  if (!p) return;
+  if (bytes == 0) bytes = 256;
+  (void)hipFree(p);
+  ctx->in_use.fetch_sub(bytes);
+}
+
+uint64_t arena_bytes(uint64_t cap_rows, uint32_t elem) { return cap_rows * elem + 64; }
+uint64_t valid_bytes(uint64_t cap_rows) { return cap_rows / 8 + 64; }
+
+void free_table(tdbg_context* ctx, Table* t) {
+  for (auto& kv : t->cols) {
+    dev_free(ctx, kv.second.d_data, arena_bytes(t->cap_rows, kv.second.elem));
+    if (kv.second.d_valid) dev_free(ctx, kv.second.d_valid, valid_bytes(t->cap_rows));
+  }
+  t->cols.clear();
+}
+
+// Grows every arena of `t` to hold `rows` rows (rounded to kRowAlign).
+int32_t table_grow(tdbg_context* ctx, Table* t, uint64_t rows) {
+  uint64_t cap = std::max<uint64_t>(rows, t->reserve);
+  cap = std::max<uint64_t>(cap, t->cap_rows * 2);
+  cap = align_up(std::max<uint64_t>(cap, 1), kRowAlign);
+  for (auto& kv : t->cols) {
+    ColumnArena& a = kv.second;
+    void* nd = nullptr;
+    int32_t rc = dev_alloc(ctx, &nd, arena_bytes(cap, a.elem));
+    if (rc) return rc;
+    if (a.d_data && t->next_row) HIP_TRY(hipMemcpy(nd, a.d_data, t->next_row * a.elem, hipMemcpyDeviceToDevice), "grow copy");
+    dev_free(ctx, a.d_data, arena_bytes(t->cap_rows, a.elem));
+    a.d_data = static_cast<uint8_t*>(nd);
+    if (a.d_valid) {
+      void* nv = nullptr;
+      rc = dev_alloc(ctx, &nv, valid_bytes(cap));
+      if (rc) return rc;
+      HIP_TRY(hipMemset(nv, 0xff, valid_bytes(cap)), "grow validity");
+      if (t->next_row) HIP_TRY(hipMemcpy(nv, a.d_valid, t->next_row / 8 + 1, hipMemcpyDeviceToDevice), "grow validity copy");
+      dev_free(ctx, a.d_valid, valid_bytes(t->cap_rows));
+      a.d_valid = static_cast<uint32_t*>(nv);
+    }
+  }
+  t->cap_rows = cap;
+  return TDBG_OK;
+}
+
+void host_stats(uint32_t type, const void* v, const uint8_t* valid, uint64_t n, tdbg_block_stats* s) {
+  s->has_minmax = 0;
+  bool first = true;
+  for (uint64_t r = 0; r < n; r++) {
+    if (valid && !((valid[r >> 3] >> (r & 7)) & 1)) continue;
+    if (is_float_type(type)) {
+      double x = type == TDBG_FLOAT ? (double)static_cast<const float*>(v)[r] : static_cast<const double*>(v)[r];
+      if (x != x) continue;
+      if (first) { s->min_f = s->max_f = x; first = false; }
+      s->min_f = std::min(s->min_f, x);
+      s->max_f = std::max(s->max_f, x);
+    } else {
+      int64_t x;
+      switch (type) {
+        case TDBG_BOOLEAN: x = static_cast<const uint8_t*>(v)[r]; break;
+        case TDBG_TINYINT: x = static_cast<const int8_t*>(v)[r]; break;
+        case TDBG_SMALLINT: x = static_cast<const int16_t*>(v)[r]; break;
+        case TDBG_INT: case TDBG_DATE: x = static_cast<const int32_t*>(v)[r]; break;
+        default: x = static_cast<const int64_t*>(v)[r]; break;
+      }
+      if (first) { s->min_i = s->max_i = x; first = false; }
+      s->min_i = std::min(s->min_i, x);
+      s->max_i = std::max(s->max_i, x);
+    }
+  }
+  s->has_minmax = first ? 0 : 1;
+}
+
+}  // namespace
+
+// ---- result ----------------------------------------------------------------
+
+struct tdbg_result {
+  uint64_t n_rows = 0;
+  std::vector<tdbg_column_view> cols;
+  std::vector<std::vector<uint8_t>> data;
+  std::vector<std::vector<uint8_t>> valid;
+};
+
+// ---- plan --------------------------------------------------------------------
+
+namespace {
+
+struct AggOut {
+  uint32_t op;          // tdbg_agg_op
+  int kagg;             // index into kernel aggs, -1 when served by the row count
+  uint32_t in_type, out_type, scale;
+  bool is_float;
+  bool nullable;
+};
+
+struct Prepared {
+  uint64_t version = ~0ull;
+  uint64_t rows_hint = 0;
+  ScanParams sp{};
+  OrderParams op{};
+  FinalParams fp{};
+  uint32_t grid = 0, block = 0;
+  std::vector<uint64_t> tmpl;  // one initialised record
+  bool empty_domain = false;   // predicates admit no row
+  std::vector<uint32_t> used_cols;  // table column per ScanParams col
+};
+
+}  // namespace
+
+struct tdbg_plan {
+  tdbg_context* ctx = nullptr;
+  uint64_t table_uid = 0;
+  std::vector<tdbg_key_part> keys;
+  std::vector<tdbg_agg_spec> aggs;
+  std::vector<tdbg_predicate> preds;
+  std::vector<tdbg_order_spec> order;
+  uint64_t limit = 0, offset = 0, est_groups = 0;
+  std::vector<AggOut> outs;
+  std::mutex mu;
+  std::atomic<bool> cancel{false};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  Prepared prep;
+  // device buffers, grown on demand
+  struct Buf {
+    void* p = nullptr;
+    uint64_t bytes = 0;
+  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout;
+  tdbg_exec_stats last{};
+};
+
+namespace {
+
+int32_t ensure(tdbg_plan* pl, tdbg_plan::Buf& b, uint64_t bytes) {
+  if (b.bytes >= bytes && b.p) return TDBG_OK;
+  dev_free(pl->ctx, b.p, b.bytes);
+  b.p = nullptr;
+  b.bytes = 0;
+  uint64_t want = std::max<uint64_t>(bytes, 256);
+  int32_t rc = dev_alloc(pl->ctx, &b.p, want);
+  if (rc) return rc;
+  b.bytes = want;
+  return TDBG_OK;
+}
+
+void release(tdbg_plan* pl, tdbg_plan::Buf& b) {
+  dev_free(pl->ctx, b.p, b.bytes);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+// Validates the request; TDBG_ERR_UNSUPPORTED = decline to the CPU path
+// (the gates of src/exec/v2_lowcard_group.zig:234-373 and
+// src/exec/v2_shape_group_topn.zig:2051-2250 that apply here).
+int32_t validate(tdbg_plan* pl) {
+  if (pl->keys.empty() || pl->keys.size() > (size_t)kMaxParts) return fail(TDBG_ERR_UNSUPPORTED, "1..8 group keys");
+  if (pl->aggs.empty() || pl->aggs.size() > (size_t)kMaxAggs) return fail(TDBG_ERR_UNSUPPORTED, "1..16 aggregates");
+  for (auto& k : pl->keys)
+    if (!is_int_type(k.type)) return fail(TDBG_ERR_UNSUPPORTED, "non-integer group key");
+  std::vector<uint32_t> cols;
+  auto use = [&](uint32_t c) {
+    if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
+  };
+  for (auto& k : pl->keys) use(k.column_idx);
+  for (auto& a : pl->aggs) {
+    if (a.op > TDBG_AGG_MAX) return fail(TDBG_ERR_INVALID, "aggregate op %u", a.op);
+    if (a.op == TDBG_AGG_COUNT_STAR) continue;
+    if (a.column_idx < 0) return fail(TDBG_ERR_INVALID, "aggregate without a column");
+    if (!is_int_type(a.input_type) && !is_float_type(a.input_type))
+      return fail(TDBG_ERR_UNSUPPORTED, "aggregate input type %u", a.input_type);
+    if ((a.input_type == TDBG_DATE || a.input_type == TDBG_DATETIME) && (a.op == TDBG_AGG_SUM || a.op == TDBG_AGG_AVG))
+      return fail(TDBG_ERR_UNSUPPORTED, "temporal SUM/AVG");
+    use((uint32_t)a.column_idx);
+  }
+  for (auto& p : pl->preds) {
+    if (p.op > TDBG_GTE) return fail(TDBG_ERR_INVALID, "predicate op %u", p.op);
+    if (!is_int_type(p.type) && !is_float_type(p.type)) return fail(TDBG_ERR_UNSUPPORTED, "predicate type");
+    use(p.column_idx);
+  }
+  if (cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than 8 distinct columns");
+  if (pl->order.size() > 6) return fail(TDBG_ERR_UNSUPPORTED, "more than 6 ORDER BY items");
+  for (auto& o : pl->order)
+    if (o.target >= pl->keys.size() + pl->aggs.size()) return fail(TDBG_ERR_INVALID, "ORDER BY target");
+  // output types: src/exec/aggregate.zig:3700-3735
+  for (auto& a : pl->aggs) {
+    AggOut o{};
+    o.op = a.op;
+    o.kagg = -1;
+    o.in_type = a.input_type;
+    o.scale = a.input_scale;
+    o.is_float = is_float_type(a.input_type);
+    o.nullable = a.nullable != 0;
+    switch (a.op) {
+      case TDBG_AGG_COUNT_STAR: case TDBG_AGG_COUNT: o.out_type = TDBG_BIGINT; break;
+      case TDBG_AGG_SUM:
+        o.out_type = o.is_float ? TDBG_DOUBLE : (a.input_type == TDBG_DECIMAL64 ? TDBG_DECIMAL128 : TDBG_BIGINT);
+        break;
+      case TDBG_AGG_AVG: o.out_type = TDBG_DOUBLE; break;
+      default: o.out_type = a.input_type; break;
+    }
+    pl->outs.push_back(o);
+  }
+  return TDBG_OK;
+}
+
+struct MergedPred {
+  uint32_t col;
+  bool is_float;
+  bool empty = false;
+  int64_t lo_i = INT64_MIN, hi_i = INT64_MAX;
+  double lo_f = -HUGE_VAL, hi_f = HUGE_VAL;
+  bool lo_incl = true, hi_incl = true;
+};
+
+// The WHERE's `col op literal` leaves merged per column into one range term
+// (plus NEQ terms).  Returns false when some column's range is empty.
+bool merge_preds(const std::vector<tdbg_predicate>& preds, std::vector<MergedPred>& ranges,
+                 std::vector<tdbg_predicate>& neqs) {
+  for (auto& p : preds) {
+    if (p.op == TDBG_NEQ) {
+      neqs.push_back(p);
+      continue;
+    }
+    MergedPred* m = nullptr;
+    for (auto& r : ranges)
+      if (r.col == p.column_idx) m = &r;
+    if (!m) {
+      ranges.push_back(MergedPred{p.column_idx, is_float_type(p.type)});
+      m = &ranges.back();
+    }
+    if (!m->is_float) {
+      const int64_t x = p.i64;
+      switch (p.op) {
+        case TDBG_EQ: m->lo_i = std::max(m->lo_i, x); m->hi_i = std::min(m->hi_i, x); break;
+        case TDBG_LT: if (x == INT64_MIN) m->empty = true; else m->hi_i = std::min(m->hi_i, x - 1); break;
+        case TDBG_LTE: m->hi_i = std::min(m->hi_i, x); break;
+        case TDBG_GT: if (x == INT64_MAX) m->empty = true; else m->lo_i = std::max(m->lo_i, x + 1); break;
+        default: m->lo_i = std::max(m->lo_i, x); break;
+      }
+      if (m->lo_i > m->hi_i) m->empty = true;
+    } else {
+      const double x = p.f64;
+      if (x != x) { m->empty = true; continue; }
+      auto raise_lo = [&](double v, bool incl) {
+        if (v > m->lo_f || (v == m->lo_f && m->lo_incl && !incl)) { m->lo_f = v; m->lo_incl = incl; }
+      };
+      auto lower_hi = [&](double v, bool incl) {
+        if (v < m->hi_f || (v == m->hi_f && m->hi_incl && !incl)) { m->hi_f = v; m->hi_incl = incl; }
+      };
+      switch (p.op) {
+        case TDBG_EQ: raise_lo(x, true); lower_hi(x, true); break;
+        case TDBG_LT: lower_hi(x, false); break;
+        case TDBG_LTE: lower_hi(x, true); break;
+        case TDBG_GT: raise_lo(x, false); break;
+        default: raise_lo(x, true); break;
+      }
+      if (m->lo_f > m->hi_f || (m->lo_f == m->hi_f && !(m->lo_incl && m->hi_incl))) m->empty = true;
+    }
+  }
+  for (auto& r : ranges)
+    if (r.empty) return false;
+  return true;
+}
+
+// rowGroupCanMatch (src/exec/scan.zig:3539-3552) over one block's zone map.
+bool block_can_match(const MergedPred& m, const BlockInfo& b) {
+  if (!b.has_stats || !b.stats.has_minmax) return true;
+  if (!m.is_float) return !(b.stats.max_i < m.lo_i || b.stats.min_i > m.hi_i);
+  const double lo = b.stats.min_f, hi = b.stats.max_f;
+  if (hi < m.lo_f || (hi == m.lo_f && !m.lo_incl)) return false;
+  if (lo > m.hi_f || (lo == m.hi_f && !m.hi_incl)) return false;
+  return true;
+}
+
+constexpr uint32_t kTileRows = 16384;
+constexpr uint64_t kDenseMax = 1ull << 24;
+
+// Specialises the plan to the table's statistics and the scanned row count.
+int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles, const std::vector<MergedPred>& ranges,
+                const std::vector<tdbg_predicate>& neqs) {
+  Prepared& P = pl->prep;
+  P = Prepared{};
+  ScanParams& sp = P.sp;
+  auto col_slot = [&](uint32_t table_col) -> uint32_t {
+    for (uint32_t i = 0; i < P.used_cols.size(); i++)
+      if (P.used_cols[i] == table_col) return i;
+    P.used_cols.push_back(table_col);
+    return (uint32_t)P.used_cols.size() - 1;
+  };
+  // predicates
+  for (auto& r : ranges) {
+    PredDesc d{};
+    d.kind = r.is_float ? P_RANGE_F : P_RANGE_I;
+    d.col = col_slot(r.col);
+    d.lo_i = r.lo_i; d.hi_i = r.hi_i;
+    d.lo_f = r.lo_f; d.hi_f = r.hi_f;
+    d.lo_incl = r.lo_incl; d.hi_incl = r.hi_incl;
+    sp.preds[sp.n_preds++] = d;
+  }
+  for (auto& q : neqs) {
+    PredDesc d{};
+    d.kind = is_float_type(q.type) ? P_NEQ_F : P_NEQ_I;
+    d.col = col_slot(q.column_idx);
+    d.lo_i = q.i64;
+    d.lo_f = q.f64;
+    sp.preds[sp.n_preds++] = d;
+  }
+  if (sp.n_preds > (uint32_t)kMaxPreds) return fail(TDBG_ERR_UNSUPPORTED, "too many predicate terms");
+  // key parts: relative values in a mixed-radix index
+  unsigned __int128 domain = 1;
+  for (size_t i = 0; i < pl->keys.size(); i++) {
+    auto it = t->cols.find(pl->keys[i].column_idx);
+    if (it == t->cols.end()) return fail(TDBG_ERR_NOT_STAGED, "key column %u not staged", pl->keys[i].column_idx);
+    const ColumnArena& a = it->second;
+    if (a.type != pl->keys[i].type) return fail(TDBG_ERR_INVALID, "key column %u type mismatch", pl->keys[i].column_idx);
+    if (a.d_valid) return fail(TDBG_ERR_UNSUPPORTED, "nullable group key");
+    PartDesc d{};
+    d.col = col_slot(pl->keys[i].column_idx);
+    d.base = a.has_stats ? a.min_i : 0;
+    d.span = a.has_stats ? (uint64_t)a.max_i - (uint64_t)a.min_i + 1 : 1;
+    if (d.span == 0) return fail(TDBG_ERR_UNSUPPORTED, "key domain of 2^64 values");
+    d.stride = (uint64_t)domain;
+    domain *= d.span;
+    if (domain >= ((unsigned __int128)1 << 62)) return fail(TDBG_ERR_UNSUPPORTED, "key domain wider than 62 bits");
+    sp.parts[sp.n_parts++] = d;
+  }
+  const uint64_t dom = (uint64_t)domain;
+  // aggregates (kernel states shared by aggs of the same class and column)
+  struct KAgg { uint32_t kind; uint32_t col; bool nullable; };
+  std::vector<KAgg> kaggs;
+  for (size_t i = 0; i < pl->aggs.size(); i++) {
+    const tdbg_agg_spec& a = pl->aggs[i];
+    AggOut& o = pl->outs[i];
+    o.kagg = -1;
+    if (a.op == TDBG_AGG_COUNT_STAR) continue;
+    auto it = t->cols.find((uint32_t)a.column_idx);
+    if (it == t->cols.end()) return fail(TDBG_ERR_NOT_STAGED, "column %d not staged", a.column_idx);
+    if (it->second.type != a.input_type) return fail(TDBG_ERR_INVALID, "column %d type mismatch", a.column_idx);
+    const bool nullable = it->second.d_valid != nullptr;
+    o.nullable = nullable;
+    if (a.op == TDBG_AGG_COUNT && !nullable) continue;  // = COUNT(*)
+    uint32_t kind;
+    switch (a.op) {
+      case TDBG_AGG_COUNT: kind = A_COUNT_COL; break;
+      case TDBG_AGG_SUM: case TDBG_AGG_AVG: kind = o.is_float ? A_SUM_F : A_SUM_I; break;
+      case TDBG_AGG_MIN: kind = o.is_float ? A_MIN_F : A_MIN_I; break;
+      default: kind = o.is_float ? A_MAX_F : A_MAX_I; break;
+    }
+    int found = -1;
+    for (size_t k = 0; k < kaggs.size(); k++)
+      if (kaggs[k].kind == kind && kaggs[k].col == (uint32_t)a.column_idx) found = (int)k;
+    if (found < 0) {
+      kaggs.push_back({kind, (uint32_t)a.column_idx, nullable});
+      found = (int)kaggs.size() - 1;
+    }
+    o.kagg = found;
+  }
+  // HBM record layout
+  const bool dense = dom <= kDenseMax || (pl->est_groups && dom <= 4 * pl->est_groups);
+  sp.g_mode = dense ? G_DENSE : G_HASH;
+  uint32_t off = 0;
+  if (!dense) { sp.g_key_off = off; off += 8; }
+  sp.g_count_off = off; off += 8;
+  std::vector<uint64_t> tmpl;
+  auto tmpl_put = [&](uint32_t at, uint64_t v) {
+    if (tmpl.size() < at / 8 + 1) tmpl.resize(at / 8 + 1, 0);
+    tmpl[at / 8] = v;
+  };
+  if (!dense) tmpl_put(sp.g_key_off, kEmptyKey);
+  tmpl_put(sp.g_count_off, 0);
+  sp.n_aggs = (uint32_t)kaggs.size();
+  for (size_t k = 0; k < kaggs.size(); k++) {
+    AggDesc& d = sp.aggs[k];
+    d = AggDesc{};
+    d.kind = kaggs[k].kind;
+    d.col = col_slot(kaggs[k].col);
+    d.nullable = kaggs[k].nullable && d.kind != A_COUNT_COL;
+    d.g_off = off;
+    uint64_t init = 0;
+    if (d.kind == A_MIN_I || d.kind == A_MIN_F) init = 0x7fffffffffffffffull;
+    if (d.kind == A_MAX_I || d.kind == A_MAX_F) init = 0x8000000000000000ull;
+    tmpl_put(off, init);
+    off += 8;
+    if (d.kind == A_SUM_I) { tmpl_put(off, 0); off += 8; }
+    d.g_ns_off = ~0u;
+    if (d.nullable) { d.g_ns_off = off; tmpl_put(off, 0); off += 8; }
+  }
+  sp.g_rec = std::max<uint32_t>(8, off);
+  tmpl.resize(sp.g_rec / 8, 0);
+  P.tmpl = tmpl;
+  if (P.used_cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than 8 distinct columns");
+  // columns
+  sp.n_cols = (uint32_t)P.used_cols.size();
+  for (uint32_t i = 0; i < sp.n_cols; i++) {
+    auto it = t->cols.find(P.used_cols[i]);
+    if (it == t->cols.end()) return fail(TDBG_ERR_NOT_STAGED, "column %u not staged", P.used_cols[i]);
+    sp.cols[i].data = it->second.d_data;
+    sp.cols[i].valid = it->second.d_valid;
+    sp.cols[i].kind = kind_of(it->second.type);
+  }
+  // estimated groups
+  uint64_t est = pl->est_groups ? std::min<uint64_t>(pl->est_groups, dom) : std::min<uint64_t>(dom, std::max<uint64_t>(n_rows_scan, 1));
+  if (dense) {
+    sp.g_cap = dom;
+  } else {
+    sp.g_cap = next_pow2(std::max<uint64_t>(1024, est + est / 2 + 1));
+  }
+  // launch shape and LDS table
+  const int n_cu = pl->ctx->n_cu;
+  uint32_t slot_bytes = 4;  // u32 row count
+  for (uint32_t k = 0; k < sp.n_aggs; k++) {
+    const AggDesc& d = sp.aggs[k];
+    slot_bytes += d.kind == A_COUNT_COL ? 4 : 8;
+    if (d.nullable) slot_bytes += 4;
+  }
+  // exact-sum mode needs rows per workgroup; assume one workgroup per CU
+  const uint64_t rows_per_wg = (uint64_t)((n_tiles + n_cu - 1) / std::max(n_cu, 1) + 1) * kTileRows;
+  for (uint32_t k = 0; k < sp.n_aggs; k++) {
+    AggDesc& d = sp.aggs[k];
+    if (d.kind != A_SUM_I) continue;
+    const ColumnArena& a = t->cols[kaggs[k].col];
+    const double m = a.has_stats ? std::max(std::fabs((double)a.min_i), std::fabs((double)a.max_i)) : 9.3e18;
+    d.split = ((double)rows_per_wg * m >= 4.0e18) ? 1u : 0u;
+    if (d.split) slot_bytes += 8;
+  }
+  uint32_t lds_cap = 0;
+  uint32_t key_bytes = dom < 0xffffffffull ? 4 : 8;
+  const uint32_t budget = std::min<uint32_t>(kLdsBudget, (uint32_t)pl->ctx->max_lds - 1024);
+  if (dense && dom * slot_bytes <= budget) {
+    sp.lds_mode = LDS_DIRECT;
+    lds_cap = (uint32_t)dom;
+  } else {
+    const uint32_t cap = budget / (slot_bytes + key_bytes);
+    if (est <= 8ull * cap && cap >= 256) {
+      sp.lds_mode = LDS_HASH;
+      lds_cap = cap;
+    } else {
+      sp.lds_mode = LDS_NONE;
+    }
+  }
+  sp.lds_cap = lds_cap;
+  sp.lds_key64 = key_bytes == 8;
+  uint32_t lo = 0;
+  auto lds_take = [&](uint32_t bytes) {
+    lo = (uint32_t)align_up(lo, 8);
+    uint32_t at = lo;
+    lo += bytes;
+    return at;
+  };
+  if (sp.lds_mode != LDS_NONE) {
+    if (sp.lds_mode == LDS_HASH) sp.lds_keys_off = lds_take(lds_cap * key_bytes);
+    for (uint32_t k = 0; k < sp.n_aggs; k++) {
+      AggDesc& d = sp.aggs[k];
+      if (d.kind == A_COUNT_COL) continue;
+      d.lds_off = lds_take(lds_cap * 8 * ((d.kind == A_SUM_I && d.split) ? 2 : 1));
+    }
+    sp.lds_count_off = lds_take(lds_cap * 4);
+    for (uint32_t k = 0; k < sp.n_aggs; k++) {
+      AggDesc& d = sp.aggs[k];
+      if (d.kind == A_COUNT_COL) d.lds_off = lds_take(lds_cap * 4);
+      if (d.nullable) d.lds_ns_off = lds_take(lds_cap * 4);
+    }
+    sp.lds_bytes = (uint32_t)align_up(lo, 16);
+    if (sp.lds_bytes > budget) return fail(TDBG_ERR_DEVICE, "LDS layout %u bytes over budget", sp.lds_bytes);
+    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(2, (uint32_t)pl->ctx->max_lds / std::max<uint32_t>(sp.lds_bytes, 1)));
+    P.block = per_cu >= 2 ? 512 : 1024;
+    P.grid = (uint32_t)std::min<uint64_t>((uint64_t)n_cu * per_cu, std::max<uint32_t>(n_tiles, 1));
+  } else {
+    P.block = 256;
+    P.grid = (uint32_t)std::min<uint64_t>((uint64_t)n_cu * 8, std::max<uint32_t>(n_tiles, 1) * 4ull);
+  }
+  // order words and output writers
+  OrderParams& op = P.op;
+  FinalParams& fp = P.fp;
+  op.n_parts = fp.n_parts = sp.n_parts;
+  for (uint32_t i = 0; i < sp.n_parts; i++) op.parts[i] = fp.parts[i] = sp.parts[i];
+  op.g_rec = fp.g_rec = sp.g_rec;
+  op.g_count_off = fp.g_count_off = sp.g_count_off;
+  op.g_key_off = fp.g_key_off = sp.g_key_off;
+  op.g_mode = fp.g_mode = sp.g_mode;
+  for (auto& o : pl->order) {
+    const uint32_t nk = (uint32_t)pl->keys.size();
+    auto add = [&](uint32_t src, uint32_t idx, uint32_t at, uint32_t ns_off) {
+      CompDesc c{};
+      c.src = src; c.idx = idx; c.off = at; c.ns_off = ns_off; c.desc = o.desc ? 1 : 0;
+      op.comps[op.n_comps++] = c;
+    };
+    if (o.target < nk) { add(C_KEY, o.target, 0, ~0u); continue; }
+    const AggOut& ao = pl->outs[o.target - nk];
+    if (ao.kagg < 0) { add(C_COUNT, 0, 0, ~0u); continue; }
+    const AggDesc& d = sp.aggs[ao.kagg];
+    switch (ao.op) {
+      case TDBG_AGG_COUNT: add(C_U64_AT, 0, d.g_off, ~0u); break;
+      case TDBG_AGG_SUM:
+        if (ao.is_float) add(C_F64_AT, 0, d.g_off, d.g_ns_off);
+        else if (ao.out_type == TDBG_DECIMAL128) { add(C_I128_HI, 0, d.g_off, d.g_ns_off); add(C_I128_LO, 0, d.g_off, d.g_ns_off); }
+        else add(C_I64_AT, 0, d.g_off, d.g_ns_off);  // BIGINT SUM orders by its wrapped value
+        break;
+      case TDBG_AGG_AVG: add(ao.is_float ? C_AVG_F : C_AVG_I, 0, d.g_off, d.g_ns_off); break;
+      default: add(ao.is_float ? C_FORD_AT : C_I64_AT, 0, d.g_off, d.g_ns_off); break;
+    }
+  }
+  // finalize writers; output buffer offsets are assigned per execution
+  fp.n_outs = 0;
+  for (uint32_t i = 0; i < sp.n_parts; i++) {
+    OutDesc o{};
+    o.kind = O_KEY; o.idx = i; o.bytes = (uint32_t)type_bytes(pl->keys[i].type);
+    o.valid_off = ~0ull; o.ns_off = ~0u; o.scale_div = 1.0;
+    fp.outs[fp.n_outs++] = o;
+  }
+  for (auto& ao : pl->outs) {
+    OutDesc o{};
+    o.ns_off = ~0u; o.valid_off = 0; o.scale_div = 1.0;
+    o.bytes = (uint32_t)type_bytes(ao.out_type);
+    const AggDesc* d = ao.kagg >= 0 ? &sp.aggs[ao.kagg] : nullptr;
+    if (d) { o.off = d->g_off; o.ns_off = d->g_ns_off; }
+    switch (ao.op) {
+      case TDBG_AGG_COUNT_STAR: o.kind = O_COUNT; o.valid_off = ~0ull; break;
+      case TDBG_AGG_COUNT: o.kind = d ? O_NS : O_COUNT; o.valid_off = ~0ull; break;
+      case TDBG_AGG_SUM:
+        o.kind = ao.is_float ? O_SUM_F64 : (ao.out_type == TDBG_DECIMAL128 ? O_SUM_I128 : O_SUM_I64);
+        break;
+      case TDBG_AGG_AVG: {
+        o.kind = ao.is_float ? O_AVG_F : O_AVG_I;
+        double p10 = 1.0;
+        for (uint32_t s = 0; s < ao.scale; s++) p10 *= 10.0;
+        o.scale_div = p10;
+        break;
+      }
+      default:
+        o.kind = ao.is_float ? O_MINMAX_F : O_MINMAX_I;
+        o.is_f32 = ao.in_type == TDBG_FLOAT;
+        break;
+    }
+    fp.outs[fp.n_outs++] = o;
+  }
+  P.version = t->version;
+  P.rows_hint = n_rows_scan;
+  return TDBG_OK;
+}
+
+int32_t sync_check(tdbg_plan* pl, const char* what) {
+  hipError_t e = hipStreamSynchronize(pl->stream);
+  if (e != hipSuccess) return hip_fail(e, what);
+  return TDBG_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+// C-ABI
+// ============================================================================
+
+extern "C" {
+
+uint32_t tdbg_abi_version(void) { return TDBG_ABI_VERSION; }
+const char* tdbg_last_error(void) { return g_last_error.c_str(); }
+
+int32_t tdbg_context_create(int32_t device, uint64_t hbm_budget_bytes, tdbg_context** out) {
+  if (!out) return fail(TDBG_ERR_INVALID, "null out");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(TDBG_ERR_DEVICE, "no HIP device: %s", hipGetErrorString(e));
+  if (device < 0 || device >= n) return fail(TDBG_ERR_INVALID, "device %d of %d", device, n);
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  auto* ctx = new tdbg_context();
+  ctx->device = device;
+  hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+  hipDeviceGetAttribute(&ctx->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  size_t free_b = 0, total_b = 0;
+  hipMemGetInfo(&free_b, &total_b);
+  ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(total_b * 0.9);
+  *out = ctx;
+  return TDBG_OK;
+}
+
+void tdbg_context_destroy(tdbg_context* ctx) {
+  if (!ctx) return;
+  {
+    std::unique_lock<std::shared_mutex> lk(ctx->mu);
+    for (auto& kv : ctx->tables) free_table(ctx, kv.second.get());
+    ctx->tables.clear();
+  }
+  delete ctx;
+}
+
+uint64_t tdbg_context_bytes_in_use(const tdbg_context* ctx) { return ctx ? ctx->in_use.load() : 0; }
+
+int32_t tdbg_table_reserve(tdbg_context* ctx, uint64_t table_uid, uint64_t rows) {
+  if (!ctx) return fail(TDBG_ERR_INVALID, "null context");
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  std::unique_lock<std::shared_mutex> lk(ctx->mu);
+  auto& tp = ctx->tables[table_uid];
+  if (!tp) { tp.reset(new Table()); tp->uid = table_uid; }
+  tp->reserve = std::max(tp->reserve, align_up(rows + kRowAlign * 64, kRowAlign));
+  if (tp->cap_rows < tp->reserve && !tp->cols.empty()) return table_grow(ctx, tp.get(), tp->reserve);
+  return TDBG_OK;
+}
+
+int32_t tdbg_table_drop(tdbg_context* ctx, uint64_t table_uid) {
+  if (!ctx) return fail(TDBG_ERR_INVALID, "null context");
+  std::unique_lock<std::shared_mutex> lk(ctx->mu);
+  auto it = ctx->tables.find(table_uid);
+  if (it == ctx->tables.end()) return TDBG_OK;
+  free_table(ctx, it->second.get());
+  ctx->tables.erase(it);
+  return TDBG_OK;
+}
+
+int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type, const void* host_values,
+                          const uint8_t* host_validity, uint64_t n_rows, const tdbg_block_stats* stats) {
+  if (!ctx || !key) return fail(TDBG_ERR_INVALID, "null argument");
+  if (!is_int_type(type) && !is_float_type(type)) return fail(TDBG_ERR_UNSUPPORTED, "column type %u", type);
+  if (n_rows == 0 || n_rows > 0xffffffffull) return fail(TDBG_ERR_INVALID, "row count %llu", (unsigned long long)n_rows);
+  if (!host_values) return fail(TDBG_ERR_INVALID, "null values");
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  std::unique_lock<std::shared_mutex> lk(ctx->mu);
+  auto& tp = ctx->tables[key->table_uid];
+  if (!tp) { tp.reset(new Table()); tp->uid = key->table_uid; }
+  Table* t = tp.get();
+  // row group entry
+  RowGroupEntry* rg = nullptr;
+  auto& bucket = t->rg_index[rg_hash(key->segment_id, key->row_group_idx)];
+  for (size_t i : bucket)
+    if (t->rgs[i].segment_id == key->segment_id && t->rgs[i].rg_idx == key->row_group_idx) rg = &t->rgs[i];
+  if (rg && rg->n_rows != n_rows) return fail(TDBG_ERR_INVALID, "row group holds %u rows, block has %llu", rg->n_rows, (unsigned long long)n_rows);
+  if (!rg) {
+    const uint64_t off = align_up(t->next_row, kRowAlign);
+    const uint64_t end = align_up(off + n_rows, kRowAlign);
+    if (end > t->cap_rows) {
+      int32_t rc = table_grow(ctx, t, end);
+      if (rc) return rc;
+    }
+    RowGroupEntry e;
+    e.segment_id = key->segment_id;
+    e.rg_idx = key->row_group_idx;
+    e.n_rows = (uint32_t)n_rows;
+    e.row_off = off;
+    t->rgs.push_back(e);
+    bucket.push_back(t->rgs.size() - 1);
+    rg = &t->rgs.back();
+    t->next_row = off + n_rows;
+  }
+  // column arena
+  auto ait = t->cols.find(key->column_idx);
+  if (ait == t->cols.end()) {
+    if (t->cap_rows == 0) {
+      int32_t rc = table_grow(ctx, t, align_up(t->next_row, kRowAlign));
+      if (rc) return rc;
+    }
+    ColumnArena a;
+    a.type = type;
+    a.elem = (uint32_t)type_bytes(type);
+    void* p = nullptr;
+    int32_t rc = dev_alloc(ctx, &p, arena_bytes(t->cap_rows, a.elem));
+    if (rc) return rc;
+    HIP_TRY(hipMemset(p, 0, arena_bytes(t->cap_rows, a.elem)), "arena clear");
+    a.d_data = static_cast<uint8_t*>(p);
+    ait = t->cols.emplace(key->column_idx, a).first;
+  }
+  ColumnArena& a = ait->second;
+  if (a.type != type) return fail(TDBG_ERR_INVALID, "column %u staged as type %u, now %u", key->column_idx, a.type, type);
+  if (host_validity && !a.d_valid) {
+    void* p = nullptr;
+    int32_t rc = dev_alloc(ctx, &p, valid_bytes(t->cap_rows));
+    if (rc) return rc;
+    HIP_TRY(hipMemset(p, 0xff, valid_bytes(t->cap_rows)), "validity init");
+    a.d_valid = static_cast<uint32_t*>(p);
+  }
+  HIP_TRY(hipMemcpy(a.d_data + rg->row_off * a.elem, host_values, n_rows * a.elem, hipMemcpyHostToDevice), "stage values");
+  if (a.d_valid) {
+    uint8_t* vb = reinterpret_cast<uint8_t*>(a.d_valid) + rg->row_off / 8;
+    if (host_validity) HIP_TRY(hipMemcpy(vb, host_validity, (n_rows + 7) / 8, hipMemcpyHostToDevice), "stage validity");
+    else HIP_TRY(hipMemset(vb, 0xff, (n_rows + 7) / 8), "stage validity");
+  }
+  BlockInfo& bi = rg->blocks[key->column_idx];
+  bi.staged = true;
+  if (stats && stats->has_minmax) bi.stats = *stats;
+  else host_stats(type, host_values, host_validity, n_rows, &bi.stats);
+  bi.has_stats = bi.stats.has_minmax != 0;
+  if (bi.has_stats) {
+    if (!a.has_stats) {
+      a.min_i = bi.stats.min_i; a.max_i = bi.stats.max_i;
+      a.min_f = bi.stats.min_f; a.max_f = bi.stats.max_f;
+      a.has_stats = true;
+    } else {
+      a.min_i = std::min(a.min_i, bi.stats.min_i); a.max_i = std::max(a.max_i, bi.stats.max_i);
+      a.min_f = std::min(a.min_f, bi.stats.min_f); a.max_f = std::max(a.max_f, bi.stats.max_f);
+    }
+  }
+  t->version++;
+  return TDBG_OK;
+}
+
+int32_t tdbg_column_evict(tdbg_context* ctx, const tdbg_column_key* key) {
+  if (!ctx || !key) return fail(TDBG_ERR_INVALID, "null argument");
+  std::unique_lock<std::shared_mutex> lk(ctx->mu);
+  auto it = ctx->tables.find(key->table_uid);
+  if (it == ctx->tables.end()) return TDBG_OK;
+  Table* t = it->second.get();
+  auto b = t->rg_index.find(rg_hash(key->segment_id, key->row_group_idx));
+  if (b == t->rg_index.end()) return TDBG_OK;
+  for (size_t i : b->second) {
+    RowGroupEntry& rg = t->rgs[i];
+    if (rg.segment_id != key->segment_id || rg.rg_idx != key->row_group_idx) continue;
+    auto bi = rg.blocks.find(key->column_idx);
+    if (bi != rg.blocks.end()) bi->second.staged = false;
+  }
+  t->version++;
+  return TDBG_OK;
+}
+
+int32_t tdbg_plan_create(tdbg_context* ctx, const tdbg_plan_desc* d, tdbg_plan** out) {
+  if (!ctx || !d || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if ((d->n_keys && !d->keys) || (d->n_aggs && !d->aggs) || (d->n_preds && !d->preds) || (d->n_order && !d->order))
+    return fail(TDBG_ERR_INVALID, "null array in plan");
+  std::unique_ptr<tdbg_plan> pl(new tdbg_plan());
+  pl->ctx = ctx;
+  pl->table_uid = d->table_uid;
+  pl->keys.assign(d->keys, d->keys + d->n_keys);
+  pl->aggs.assign(d->aggs, d->aggs + d->n_aggs);
+  pl->preds.assign(d->preds, d->preds + d->n_preds);
+  pl->order.assign(d->order, d->order + d->n_order);
+  pl->limit = d->limit;
+  pl->offset = d->offset;
+  pl->est_groups = d->est_groups;
+  int32_t rc = validate(pl.get());
+  if (rc) return rc;
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
+  for (auto& e : pl->ev) HIP_TRY(hipEventCreate(&e), "event");
+  *out = pl.release();
+  return TDBG_OK;
+}
+
+void tdbg_plan_destroy(tdbg_plan* pl) {
+  if (!pl) return;
+  (void)hipSetDevice(pl->ctx->device);
+  if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+  for (auto* b : {&pl->state, &pl->tiles, &pl->err, &pl->gids, &pl->counts, &pl->sel, &pl->scratch, &pl->out, &pl->tmpl, &pl->nout})
+    release(pl, *b);
+  for (auto& e : pl->ev) if (e) (void)hipEventDestroy(e);
+  if (pl->stream) (void)hipStreamDestroy(pl->stream);
+  delete pl;
+}
+
+int32_t tdbg_cancel(tdbg_plan* pl) {
+  if (!pl) return fail(TDBG_ERR_INVALID, "null plan");
+  pl->cancel.store(true);
+  return TDBG_OK;
+}
+
+int32_t tdbg_plan_last_stats(const tdbg_plan* pl, tdbg_exec_stats* out) {
+  if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = pl->last;
+  return TDBG_OK;
+}
+
+static int32_t make_result(tdbg_plan* pl, uint64_t n_out, const std::vector<uint8_t>& host,
+                           const std::vector<uint64_t>& data_off, const std::vector<uint64_t>& valid_off,
+                           tdbg_result** out) {
+  auto* r = new tdbg_result();
+  r->n_rows = n_out;
+  const size_t nk = pl->keys.size();
+  const size_t nc = nk + pl->outs.size();
+  r->cols.resize(nc);
+  r->data.resize(nc);
+  r->valid.resize(nc);
+  for (size_t c = 0; c < nc; c++) {
+    tdbg_column_view& rc = r->cols[c];
+    uint32_t type, scale = 0, nullable = 0;
+    if (c < nk) {
+      type = pl->keys[c].type;
+    } else {
+      const AggOut& o = pl->outs[c - nk];
+      type = o.out_type;
+      if (o.out_type == TDBG_DECIMAL128 || o.out_type == TDBG_DECIMAL64) scale = o.scale;
+      nullable = o.op != TDBG_AGG_COUNT_STAR && o.op != TDBG_AGG_COUNT;
+    }
+    const uint64_t bytes = n_out * (uint64_t)type_bytes(type);
+    r->data[c].resize(bytes + 16);
+    if (n_out) memcpy(r->data[c].data(), host.data() + data_off[c], bytes);
+    bool any_null = false;
+    if (valid_off[c] != ~0ull && n_out) {
+      std::vector<uint8_t>& bm = r->valid[c];
+      bm.assign((n_out + 7) / 8, 0);
+      const uint8_t* vb = host.data() + valid_off[c];
+      for (uint64_t i = 0; i < n_out; i++) {
+        if (vb[i]) bm[i >> 3] |= (uint8_t)(1u << (i & 7));
+        else any_null = true;
+      }
+      if (!any_null) bm.clear();
+    }
+    rc.type = type;
+    rc.scale = scale;
+    rc.nullable = nullable;
+    rc.data = r->data[c].data();
+    rc.validity = any_null ? r->valid[c].data() : nullptr;
+    rc.n_rows = n_out;
+  }
+  *out = r;
+  return TDBG_OK;
+}
+
+int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out) {
+  if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> plk(pl->mu);
+  pl->cancel.store(false);
+  tdbg_context* ctx = pl->ctx;
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  std::shared_lock<std::shared_mutex> lk(ctx->mu);
+  pl->last = tdbg_exec_stats{};
+  auto tit = ctx->tables.find(pl->table_uid);
+  if (tit == ctx->tables.end()) return fail(TDBG_ERR_NOT_STAGED, "table %llu has no staged columns", (unsigned long long)pl->table_uid);
+  Table* t = tit->second.get();
+  // the predicate terms and the columns every row group must have staged
+  std::vector<MergedPred> ranges;
+  std::vector<tdbg_predicate> neqs;
+  const bool satisfiable = merge_preds(pl->preds, ranges, neqs);
+  std::vector<uint32_t> need_cols;
+  for (auto& k : pl->keys) need_cols.push_back(k.column_idx);
+  for (auto& a : pl->aggs) if (a.op != TDBG_AGG_COUNT_STAR) need_cols.push_back((uint32_t)a.column_idx);
+  for (auto& p : pl->preds) need_cols.push_back(p.column_idx);
+  std::vector<const RowGroupEntry*> scan;
+  auto consider = [&](const RowGroupEntry& rg) -> int32_t {
+    for (uint32_t c : need_cols) {
+      auto bi = rg.blocks.find(c);
+      if (bi == rg.blocks.end() || !bi->second.staged)
+        return fail(TDBG_ERR_NOT_STAGED, "segment %llu row group %u column %u not staged",
+                    (unsigned long long)rg.segment_id, rg.rg_idx, c);
+    }
+    for (auto& m : ranges) {
+      auto bi = rg.blocks.find(m.col);
+      if (!block_can_match(m, bi->second)) return TDBG_OK;  // zone map prunes it
+    }
+    scan.push_back(&rg);
+    return TDBG_OK;
+  };
+  if (rgs) {
+    for (uint32_t i = 0; i < n_rgs; i++) {
+      auto b = t->rg_index.find(rg_hash(rgs[i].segment_id, rgs[i].row_group_idx));
+      const RowGroupEntry* found = nullptr;
+      if (b != t->rg_index.end())
+        for (size_t j : b->second)
+          if (t->rgs[j].segment_id == rgs[i].segment_id && t->rgs[j].rg_idx == rgs[i].row_group_idx) found = &t->rgs[j];
+      if (!found) return fail(TDBG_ERR_NOT_STAGED, "segment %llu row group %u not staged",
+                              (unsigned long long)rgs[i].segment_id, rgs[i].row_group_idx);
+      int32_t rc = consider(*found);
+      if (rc) return rc;
+    }
+  } else {
+    for (auto& rg : t->rgs) {
+      int32_t rc = consider(rg);
+      if (rc) return rc;
+    }
+  }
+  // tiles
+  std::vector<Tile> tiles;
+  uint64_t rows = 0;
+  if (satisfiable) {
+    for (const RowGroupEntry* rg : scan) {
+      for (uint32_t o = 0; o < rg->n_rows; o += kTileRows) {
+        Tile tl{};
+        tl.row0 = rg->row_off + o;
+        tl.n = std::min<uint32_t>(kTileRows, rg->n_rows - o);
+        tiles.push_back(tl);
+      }
+      rows += rg->n_rows;
+    }
+  }
+  int32_t rc = prepare(pl, t, rows, (uint32_t)tiles.size(), ranges, neqs);
+  if (rc) return rc;
+  Prepared& P = pl->prep;
+  ScanParams& sp = P.sp;
+  // offsets of the output buffer
+  const size_t nk = pl->keys.size();
+  const size_t nc = nk + pl->outs.size();
+  std::vector<uint64_t> data_off(nc, 0), valid_off(nc, ~0ull);
+  const uint64_t need_lim = pl->limit == 0 ? ~0ull : pl->offset + pl->limit;
+  if (tiles.empty()) {
+    std::vector<uint8_t> none;
+    pl->last.rows_scanned = 0;
+    return make_result(pl, 0, none, data_off, valid_off, out);
+  }
+  hipStream_t s = pl->stream;
+  // buffers
+  const uint64_t state_bytes = sp.g_cap * sp.g_rec;
+  if ((rc = ensure(pl, pl->state, state_bytes))) return rc;
+  if ((rc = ensure(pl, pl->tiles, tiles.size() * sizeof(Tile)))) return rc;
+  if ((rc = ensure(pl, pl->err, 64))) return rc;
+  if ((rc = ensure(pl, pl->tmpl, P.tmpl.size() * 8))) return rc;
+  if ((rc = ensure(pl, pl->nout, 64))) return rc;
+  sp.g_state = static_cast<uint8_t*>(pl->state.p);
+  sp.tiles = static_cast<const Tile*>(pl->tiles.p);
+  sp.n_tiles = (uint32_t)tiles.size();
+  sp.err = static_cast<uint32_t*>(pl->err.p);
+  P.op.g_state = sp.g_state;
+  P.fp.g_state = sp.g_state;
+  uint64_t scanned_bytes = 0;
+  for (uint32_t i = 0; i < sp.n_cols; i++) scanned_bytes += rows * (uint64_t)t->cols[P.used_cols[i]].elem;
+
+  HIP_TRY(hipMemcpyAsync(pl->tiles.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, s), "tiles");
+  HIP_TRY(hipMemcpyAsync(pl->tmpl.p, P.tmpl.data(), P.tmpl.size() * 8, hipMemcpyHostToDevice, s), "template");
+  HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
+  HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
+  HIP_TRY(launch_fill_u32(sp.err, 16, 0, s), "init flags");
+  HIP_TRY(hipEventRecord(pl->ev[1], s), "event");
+  HIP_TRY(launch_scan_agg(sp, P.grid, P.block, s), "scan_agg launch");
+  HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
+  if (pl->cancel.load()) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
+  // compaction
+  const uint64_t n_rec = sp.g_cap;
+  if ((rc = ensure(pl, pl->gids, n_rec * 4 + 256))) return rc;
+  if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
+  HIP_TRY(launch_compact(P.op, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->gids.p),
+                         static_cast<uint64_t*>(pl->nout.p), s), "compact");
+  uint64_t n_groups = 0;
+  uint32_t err_bits = 0;
+  HIP_TRY(hipMemcpyAsync(&n_groups, pl->nout.p, 8, hipMemcpyDeviceToHost, s), "group count");
+  HIP_TRY(hipMemcpyAsync(&err_bits, sp.err, 4, hipMemcpyDeviceToHost, s), "error flags");
+  if ((rc = sync_check(pl, "scan"))) return rc;
+  if (err_bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
+  if (err_bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
+  if (pl->cancel.load()) return fail(TDBG_ERR_CANCELLED, "cancelled");
+  // ORDER BY / OFFSET / LIMIT
+  const uint64_t start = std::min<uint64_t>(pl->offset, n_groups);
+  const uint64_t end = std::min<uint64_t>(need_lim, n_groups);
+  const uint64_t n_out = end - start;
+  const uint32_t* sel = static_cast<const uint32_t*>(pl->gids.p) + start;
+  if (!pl->order.empty() && n_groups > 0 && end > 0) {
+    const size_t sb = sort_scratch_bytes(n_groups, P.op.n_comps);
+    if ((rc = ensure(pl, pl->scratch, sb))) return rc;
+    if ((rc = ensure(pl, pl->sel, end * 4 + 256))) return rc;
+    HIP_TRY(launch_sort(P.op, static_cast<uint32_t*>(pl->gids.p), n_groups, end, pl->scratch.p, pl->scratch.bytes,
+                        static_cast<uint32_t*>(pl->sel.p), s), "order");
+    sel = static_cast<const uint32_t*>(pl->sel.p) + start;
+  }
+  // finalize into one output buffer
+  uint64_t ob = 0;
+  for (size_t c = 0; c < nc; c++) {
+    FinalParams& fp = P.fp;
+    OutDesc& o = fp.outs[c];
+    ob = align_up(ob, 16);
+    data_off[c] = ob;
+    o.data_off = ob;
+    ob += n_out * o.bytes;
+    if (o.valid_off != ~0ull) {
+      ob = align_up(ob, 16);
+      valid_off[c] = ob;
+      o.valid_off = ob;
+      ob += n_out;
+    }
+  }
+  if ((rc = ensure(pl, pl->out, ob + 256))) return rc;
+  HIP_TRY(launch_finalize(P.fp, sel, n_out, static_cast<uint8_t*>(pl->out.p), s), "finalize");
+  HIP_TRY(hipEventRecord(pl->ev[3], s), "event");
+  std::vector<uint8_t> host(ob + 16);
+  if (n_out) HIP_TRY(hipMemcpyAsync(host.data(), pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
+  if ((rc = sync_check(pl, "finalize"))) return rc;
+  float ms_total = 0, ms_scan = 0;
+  hipEventElapsedTime(&ms_total, pl->ev[0], pl->ev[3]);
+  hipEventElapsedTime(&ms_scan, pl->ev[1], pl->ev[2]);
+  pl->last.total_ms = ms_total;
+  pl->last.scan_ms = ms_scan;
+  pl->last.rows_scanned = rows;
+  pl->last.bytes_scanned = scanned_bytes;
+  pl->last.n_groups = n_groups;
+  pl->last.lds_mode = sp.lds_mode;
+  pl->last.global_mode = sp.g_mode;
+  return make_result(pl, n_out, host, data_off, valid_off, out);
+}
+
+uint64_t tdbg_result_num_rows(const tdbg_result* r) { return r ? r->n_rows : 0; }
+uint32_t tdbg_result_num_columns(const tdbg_result* r) { return r ? (uint32_t)r->cols.size() : 0; }
+int32_t tdbg_result_column(const tdbg_result* r, uint32_t i, tdbg_column_view* out) {
+  if (!r || !out || i >= r->cols.size()) return fail(TDBG_ERR_INVALID, "column %u", i);
+  *out = r->cols[i];
+  return TDBG_OK;
+}
+void tdbg_result_free(tdbg_result* r) { delete r; }
+
+}  // extern "C"

@@ -1,0 +1,387 @@
+"""ctypes binding of ``libthindb_gpu.so`` (the C-ABI in ``include/thindb_gpu.h``).
+
+This is the FFI layer a host binds; :mod:`thindb_amd.exec` builds thinDB's
+operator interface on top of it.  Loading fails loudly when the library is
+missing: there is no CPU fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libthindb_gpu.so"
+
+ABI_VERSION = 1
+
+
+class Status(enum.IntEnum):
+    OK = 0
+    PENDING = 1
+    INVALID = -1
+    UNSUPPORTED = -2
+    OOM = -3
+    CANCELLED = -4
+    DEVICE = -5
+    NOT_STAGED = -6
+
+
+class Type(enum.IntEnum):
+    """Physical column types (src/types.zig subset)."""
+    BOOLEAN = 1
+    TINYINT = 2
+    SMALLINT = 3
+    INT = 4
+    BIGINT = 5
+    DATE = 6
+    DATETIME = 7
+    DECIMAL64 = 8
+    FLOAT = 9
+    DOUBLE = 10
+    DECIMAL128 = 11
+
+
+NUMPY_DTYPE = {
+    Type.BOOLEAN: np.uint8,
+    Type.TINYINT: np.int8,
+    Type.SMALLINT: np.int16,
+    Type.INT: np.int32,
+    Type.BIGINT: np.int64,
+    Type.DATE: np.int32,
+    Type.DATETIME: np.int64,
+    Type.DECIMAL64: np.int64,
+    Type.FLOAT: np.float32,
+    Type.DOUBLE: np.float64,
+}
+
+
+class AggOp(enum.IntEnum):
+    COUNT_STAR = 0
+    COUNT = 1
+    SUM = 2
+    AVG = 3
+    MIN = 4
+    MAX = 5
+
+
+class PredOp(enum.IntEnum):
+    """src/exec/predicate.zig:33 PredicateOp."""
+    EQ = 0
+    NEQ = 1
+    LT = 2
+    LTE = 3
+    GT = 4
+    GTE = 5
+
+
+# ---- errors (status -> the reference's error set) ---------------------------
+
+class GpuError(RuntimeError):
+    status = Status.DEVICE
+
+
+class Unsupported(GpuError):
+    """The shape is not covered: the caller declines to its CPU operator."""
+    status = Status.UNSUPPORTED
+
+
+class MemoryBudgetExceeded(GpuError):
+    """src/memory.zig:31-35 — device memory or admission budget exhausted."""
+    status = Status.OOM
+
+
+class QueryCancelled(GpuError):
+    status = Status.CANCELLED
+
+
+class NotStaged(GpuError):
+    status = Status.NOT_STAGED
+
+
+class InvalidArgument(GpuError):
+    status = Status.INVALID
+
+
+_ERRORS = {
+    Status.UNSUPPORTED: Unsupported,
+    Status.OOM: MemoryBudgetExceeded,
+    Status.CANCELLED: QueryCancelled,
+    Status.NOT_STAGED: NotStaged,
+    Status.INVALID: InvalidArgument,
+}
+
+
+# ---- C structs ------------------------------------------------------------------
+
+class ColumnKey(C.Structure):
+    _fields_ = [("table_uid", C.c_uint64), ("segment_id", C.c_uint64),
+                ("row_group_idx", C.c_uint32), ("column_idx", C.c_uint32)]
+
+
+class BlockStats(C.Structure):
+    _fields_ = [("min_i", C.c_int64), ("max_i", C.c_int64), ("min_f", C.c_double), ("max_f", C.c_double),
+                ("has_minmax", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class RgRef(C.Structure):
+    _fields_ = [("segment_id", C.c_uint64), ("row_group_idx", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class KeyPart(C.Structure):
+    _fields_ = [("column_idx", C.c_uint32), ("type", C.c_uint32)]
+
+
+class AggSpecC(C.Structure):
+    _fields_ = [("op", C.c_uint32), ("column_idx", C.c_int32), ("input_type", C.c_uint32),
+                ("input_scale", C.c_uint8), ("nullable", C.c_uint8), ("pad", C.c_uint8 * 2)]
+
+
+class PredicateC(C.Structure):
+    _fields_ = [("column_idx", C.c_uint32), ("type", C.c_uint32), ("op", C.c_uint32), ("nullable", C.c_uint32),
+                ("i64", C.c_int64), ("f64", C.c_double)]
+
+
+class OrderSpecC(C.Structure):
+    _fields_ = [("target", C.c_uint32), ("desc", C.c_uint32)]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("table_uid", C.c_uint64), ("n_keys", C.c_uint32), ("n_aggs", C.c_uint32),
+                ("keys", C.POINTER(KeyPart)), ("aggs", C.POINTER(AggSpecC)),
+                ("n_preds", C.c_uint32), ("n_order", C.c_uint32),
+                ("preds", C.POINTER(PredicateC)), ("order", C.POINTER(OrderSpecC)),
+                ("limit", C.c_uint64), ("offset", C.c_uint64), ("est_groups", C.c_uint64)]
+
+
+class ColumnView(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("scale", C.c_uint32), ("nullable", C.c_uint32), ("pad", C.c_uint32),
+                ("data", C.c_void_p), ("validity", C.c_void_p), ("n_rows", C.c_uint64)]
+
+
+class ExecStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("scan_ms", C.c_double), ("rows_scanned", C.c_uint64),
+                ("bytes_scanned", C.c_uint64), ("n_groups", C.c_uint64), ("lds_mode", C.c_uint32),
+                ("global_mode", C.c_uint32)]
+
+
+# Every entry point include/thindb_gpu.h declares, with its ctypes signature.
+EXPORTS = {
+    "tdbg_abi_version": (C.c_uint32, []),
+    "tdbg_last_error": (C.c_char_p, []),
+    "tdbg_context_create": (C.c_int32, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "tdbg_context_destroy": (None, [C.c_void_p]),
+    "tdbg_context_bytes_in_use": (C.c_uint64, [C.c_void_p]),
+    "tdbg_column_stage": (C.c_int32, [C.c_void_p, C.POINTER(ColumnKey), C.c_uint32, C.c_void_p, C.c_void_p,
+                                      C.c_uint64, C.POINTER(BlockStats)]),
+    "tdbg_column_evict": (C.c_int32, [C.c_void_p, C.POINTER(ColumnKey)]),
+    "tdbg_table_reserve": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    "tdbg_table_drop": (C.c_int32, [C.c_void_p, C.c_uint64]),
+    "tdbg_plan_create": (C.c_int32, [C.c_void_p, C.POINTER(PlanDesc), C.POINTER(C.c_void_p)]),
+    "tdbg_plan_destroy": (None, [C.c_void_p]),
+    "tdbg_execute": (C.c_int32, [C.c_void_p, C.POINTER(RgRef), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tdbg_plan_last_stats": (C.c_int32, [C.c_void_p, C.POINTER(ExecStats)]),
+    "tdbg_cancel": (C.c_int32, [C.c_void_p]),
+    "tdbg_result_num_rows": (C.c_uint64, [C.c_void_p]),
+    "tdbg_result_num_columns": (C.c_uint32, [C.c_void_p]),
+    "tdbg_result_column": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(ColumnView)]),
+    "tdbg_result_free": (None, [C.c_void_p]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def load(path: Optional[Path] = None) -> C.CDLL:
+    """Loads the engine library and binds every export; raises if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else _LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built: run __graft_entry__.build() (the GPU engine has no fallback)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.tdbg_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"ABI version {lib.tdbg_abi_version()} != {ABI_VERSION}")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int) -> int:
+    if rc >= 0:
+        return rc
+    msg = load().tdbg_last_error().decode(errors="replace")
+    st = Status(rc) if rc in Status._value2member_map_ else Status.DEVICE
+    raise _ERRORS.get(st, GpuError)(f"{st.name}: {msg}")
+
+
+# ---- objects -------------------------------------------------------------------
+
+class Context:
+    """One GPU's engine context (streams, HBM column store, admission budget)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        lib = load()
+        h = C.c_void_p()
+        check(lib.tdbg_context_create(device, hbm_budget_bytes, C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def bytes_in_use(self) -> int:
+        return int(self._lib.tdbg_context_bytes_in_use(self._h))
+
+    def stage(self, table_uid: int, segment_id: int, rg_idx: int, column_idx: int, type_: Type,
+              values: np.ndarray, validity: Optional[np.ndarray] = None, stats=None) -> None:
+        """Copies one decoded column block into HBM (cache.Key identity)."""
+        values = np.ascontiguousarray(values, dtype=NUMPY_DTYPE[Type(type_)])
+        key = ColumnKey(table_uid, segment_id, rg_idx, column_idx)
+        vptr = None
+        if validity is not None:
+            validity = np.ascontiguousarray(validity, dtype=np.uint8)
+            vptr = validity.ctypes.data_as(C.c_void_p)
+        st = None
+        if stats is not None:
+            st = BlockStats()
+            st.has_minmax = 1
+            if Type(type_) in (Type.FLOAT, Type.DOUBLE):
+                st.min_f, st.max_f = float(stats[0]), float(stats[1])
+            else:
+                st.min_i, st.max_i = int(stats[0]), int(stats[1])
+        check(self._lib.tdbg_column_stage(self._h, C.byref(key), int(type_), values.ctypes.data_as(C.c_void_p),
+                                          vptr, len(values), C.byref(st) if st is not None else None))
+
+    def evict(self, table_uid: int, segment_id: int, rg_idx: int, column_idx: int) -> None:
+        key = ColumnKey(table_uid, segment_id, rg_idx, column_idx)
+        check(self._lib.tdbg_column_evict(self._h, C.byref(key)))
+
+    def reserve(self, table_uid: int, rows: int) -> None:
+        check(self._lib.tdbg_table_reserve(self._h, table_uid, rows))
+
+    def drop_table(self, table_uid: int) -> None:
+        check(self._lib.tdbg_table_drop(self._h, table_uid))
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.tdbg_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ResultColumn:
+    __slots__ = ("type", "scale", "nullable", "values", "validity")
+
+    def __init__(self, type_: Type, scale: int, nullable: bool, values: np.ndarray, validity: Optional[np.ndarray]):
+        self.type = type_
+        self.scale = scale
+        self.nullable = nullable
+        self.values = values
+        self.validity = validity  # packed bitmap (bit 1 = valid) or None
+
+    def is_valid(self, row: int) -> bool:
+        if self.validity is None:
+            return True
+        return bool((self.validity[row >> 3] >> (row & 7)) & 1)
+
+
+def _column_from_view(v: ColumnView) -> ResultColumn:
+    t = Type(v.type)
+    n = int(v.n_rows)
+    if t == Type.DECIMAL128:
+        raw = np.ctypeslib.as_array(C.cast(v.data, C.POINTER(C.c_uint64)), shape=(2 * n,)).copy() if n else \
+            np.zeros(0, np.uint64)
+        lo = raw[0::2].astype(object)
+        hi = raw[1::2].view(np.int64).astype(object)
+        vals = np.array([int(h) * (1 << 64) + int(l) for l, h in zip(lo, hi)], dtype=object)
+    elif n:
+        ct = np.ctypeslib.as_ctypes_type(NUMPY_DTYPE[t])
+        vals = np.ctypeslib.as_array(C.cast(v.data, C.POINTER(ct)), shape=(n,)).copy()
+    else:
+        vals = np.zeros(0, NUMPY_DTYPE[t])
+    valid = None
+    if v.validity:
+        valid = np.ctypeslib.as_array(C.cast(v.validity, C.POINTER(C.c_uint8)), shape=((n + 7) // 8,)).copy()
+    return ResultColumn(t, int(v.scale), bool(v.nullable), vals, valid)
+
+
+class Plan:
+    """A grouped query prepared on one context (tdbg_plan)."""
+
+    def __init__(self, ctx: Context, table_uid: int, keys: Sequence[tuple], aggs: Sequence[tuple],
+                 preds: Sequence[tuple] = (), order: Sequence[tuple] = (), limit: int = 0, offset: int = 0,
+                 est_groups: int = 0):
+        """keys: (column_idx, Type); aggs: (AggOp, column_idx|-1, Type, scale, nullable);
+        preds: (column_idx, Type, PredOp, literal, nullable); order: (target, desc)."""
+        lib = load()
+        self._lib = lib
+        self.ctx = ctx
+        self._keys = (KeyPart * max(1, len(keys)))(*[KeyPart(c, int(t)) for c, t in keys])
+        self._aggs = (AggSpecC * max(1, len(aggs)))(
+            *[AggSpecC(int(op), int(c), int(t), int(sc), int(bool(nl))) for op, c, t, sc, nl in aggs])
+        pl = []
+        for c, t, op, lit, nl in preds:
+            p = PredicateC(c, int(t), int(op), int(bool(nl)))
+            if Type(t) in (Type.FLOAT, Type.DOUBLE):
+                p.f64 = float(lit)
+            else:
+                p.i64 = int(lit)
+            pl.append(p)
+        self._preds = (PredicateC * max(1, len(pl)))(*pl)
+        self._order = (OrderSpecC * max(1, len(order)))(*[OrderSpecC(t, int(bool(d))) for t, d in order])
+        desc = PlanDesc(table_uid, len(keys), len(aggs), self._keys, self._aggs, len(pl), len(order), self._preds,
+                        self._order, limit, offset, est_groups)
+        h = C.c_void_p()
+        check(lib.tdbg_plan_create(ctx.handle, C.byref(desc), C.byref(h)))
+        self._h = h
+
+    def execute(self, rgs: Optional[Sequence[tuple]] = None) -> list:
+        """Runs the plan; returns the result columns (keys, then aggregates)."""
+        out = C.c_void_p()
+        if rgs is None:
+            check(self._lib.tdbg_execute(self._h, None, 0, C.byref(out)))
+        else:
+            arr = (RgRef * max(1, len(rgs)))(*[RgRef(s, r, 0) for s, r in rgs])
+            check(self._lib.tdbg_execute(self._h, arr, len(rgs), C.byref(out)))
+        try:
+            n = self._lib.tdbg_result_num_columns(out)
+            cols = []
+            for i in range(n):
+                v = ColumnView()
+                check(self._lib.tdbg_result_column(out, i, C.byref(v)))
+                cols.append(_column_from_view(v))
+            return cols
+        finally:
+            self._lib.tdbg_result_free(out)
+
+    def stats(self) -> ExecStats:
+        st = ExecStats()
+        check(self._lib.tdbg_plan_last_stats(self._h, C.byref(st)))
+        return st
+
+    def cancel(self) -> None:
+        check(self._lib.tdbg_cancel(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.tdbg_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
