@@ -131,7 +131,7 @@ def main() -> int:
     stage_s = time.perf_counter() - t0
     log(f"staged {staged / 1e9:.2f} GB in {stage_s:.2f}s ({staged / stage_s / 1e9:.1f} GB/s incl. PCIe)")
 
-    rgs = [(rg.segment_id, rg.idx) for rg in table.row_groups]
+    rgs = q.plan.rg_array([(rg.segment_id, rg.idx) for rg in table.row_groups])
     for i in range(args.warmup):
         q.plan.execute(rgs)
     log(f"warmup {args.warmup} done")

@@ -8,7 +8,7 @@
 
 namespace tdbg {
 
-constexpr int kMaxCols = 8;     // distinct columns one plan reads
+constexpr int kMaxCols = 16;    // distinct columns one plan reads
 constexpr int kMaxPreds = 8;    // merged predicate terms
 constexpr int kMaxParts = 8;    // GROUP BY key parts (src/exec/v2_lowcard_group.zig:114)
 constexpr int kMaxAggs = 16;    // aggregates (src/exec/v2_lowcard_group.zig:115)
@@ -91,6 +91,9 @@ struct ScanParams {
   // LDS table
   uint32_t lds_mode;
   uint32_t lds_cap;        // slots
+  uint32_t lds_parts;      // LDS_DIRECT: workgroups per tile range, each owning
+                           // lds_cap consecutive keys (grid % parts == 0 and
+                           // (grid / parts) % 8 == 0 keeps partners on one XCD)
   uint32_t lds_key64;      // hash keys are u64 (else u32)
   uint32_t lds_keys_off;   // byte offsets inside dynamic LDS
   uint32_t lds_count_off;

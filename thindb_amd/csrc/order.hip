@@ -258,50 +258,72 @@ __global__ void __launch_bounds__(1024) small_sort_kernel(const uint64_t* comps,
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) pos_out[i] = sp[i];
 }
 
-// ---- per-workgroup top-k (k <= 64) ----------------------------------------------
+// ---- per-workgroup top-k (k <= kTopMaxK) -----------------------------------------
 
 constexpr uint32_t kTopThreads = 256;
 constexpr uint32_t kTopPer = 16;
 constexpr uint32_t kTopChunk = kTopThreads * kTopPer;
+constexpr uint32_t kTopMaxK = 256;
+constexpr uint32_t kNone = 0xffffffffu;
 
-__global__ void __launch_bounds__(kTopThreads) block_topk_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n,
-                                                                 uint32_t k, uint32_t* cand) {
+// Each workgroup takes kTopChunk items of the position list `pos_in`
+// (the identity when null, m items) and writes its k best, in order, to
+// cand[b*k ...].  Items stay in registers; each round is one block-wide
+// argmin under the composite order.  Every workgroup but the last holds at
+// least k items, so the candidates land contiguously.
+__global__ void __launch_bounds__(kTopThreads) block_topk_kernel(const uint64_t* comps, uint32_t n_comps,
+                                                                 uint64_t n_all, const uint32_t* pos_in,
+                                                                 uint64_t m, uint32_t k, uint32_t* cand) {
   __shared__ uint64_t rw[kTopThreads / 64];
   __shared__ uint32_t rp[kTopThreads / 64];
   const uint64_t base = (uint64_t)blockIdx.x * kTopChunk + threadIdx.x;
-  uint32_t taken = 0;  // bit j: item j already emitted
-  for (uint32_t r = 0; r < k; r++) {
-    // this thread's best remaining item
-    uint64_t bw = ~0ull;
-    uint32_t bp = 0xffffffffu;
-    for (uint32_t j = 0; j < kTopPer; j++) {
-      const uint64_t p = base + (uint64_t)j * kTopThreads;
-      if (p >= n || ((taken >> j) & 1u)) continue;
-      const uint64_t w = comps[p];
-      if (bp == 0xffffffffu || pos_less(comps, n_comps, n, (uint32_t)p, w, bp, bw)) { bw = w; bp = (uint32_t)p; }
+  uint64_t w[kTopPer];
+  uint32_t p[kTopPer];
+#pragma unroll
+  for (uint32_t j = 0; j < kTopPer; j++) {
+    const uint64_t i = base + (uint64_t)j * kTopThreads;
+    p[j] = kNone;
+    w[j] = ~0ull;
+    if (i < m) {
+      p[j] = pos_in ? pos_in[i] : (uint32_t)i;
+      w[j] = comps[p[j]];
     }
-    // wave reduction
+  }
+  for (uint32_t r = 0; r < k; r++) {
+    uint64_t bw = ~0ull;
+    uint32_t bp = kNone;
+#pragma unroll
+    for (uint32_t j = 0; j < kTopPer; j++)
+      if (p[j] != kNone && (bp == kNone || pos_less(comps, n_comps, n_all, p[j], w[j], bp, bw))) {
+        bw = w[j];
+        bp = p[j];
+      }
     for (int o = 32; o > 0; o >>= 1) {
       const uint64_t ow = __shfl_xor(bw, o);
       const uint32_t op = __shfl_xor(bp, o);
-      if (op != 0xffffffffu && (bp == 0xffffffffu || pos_less(comps, n_comps, n, op, ow, bp, bw))) { bw = ow; bp = op; }
+      if (op != kNone && (bp == kNone || pos_less(comps, n_comps, n_all, op, ow, bp, bw))) {
+        bw = ow;
+        bp = op;
+      }
     }
-    if ((threadIdx.x & 63) == 0) { rw[threadIdx.x >> 6] = bw; rp[threadIdx.x >> 6] = bp; }
+    if ((threadIdx.x & 63) == 0) {
+      rw[threadIdx.x >> 6] = bw;
+      rp[threadIdx.x >> 6] = bp;
+    }
     __syncthreads();
     uint64_t fw = rw[0];
     uint32_t fp = rp[0];
-    for (uint32_t w = 1; w < kTopThreads / 64; w++) {
-      if (rp[w] != 0xffffffffu && (fp == 0xffffffffu || pos_less(comps, n_comps, n, rp[w], rw[w], fp, fw))) {
-        fw = rw[w]; fp = rp[w];
+    for (uint32_t x = 1; x < kTopThreads / 64; x++)
+      if (rp[x] != kNone && (fp == kNone || pos_less(comps, n_comps, n_all, rp[x], rw[x], fp, fw))) {
+        fw = rw[x];
+        fp = rp[x];
       }
-    }
     __syncthreads();
-    // Every workgroup but the last holds at least k items, so the
-    // candidates land contiguously: workgroup b writes [b*k, b*k + its count).
-    if (threadIdx.x == 0 && fp != 0xffffffffu) cand[(uint64_t)blockIdx.x * k + r] = fp;
-    if (fp != 0xffffffffu && (fp % kTopThreads) == threadIdx.x && fp / kTopChunk == blockIdx.x) {
-      taken |= 1u << (uint32_t)((fp - blockIdx.x * (uint64_t)kTopChunk) / kTopThreads);
-    }
+    if (fp == kNone) break;  // uniform: the workgroup ran out of items
+    if (threadIdx.x == 0) cand[(uint64_t)blockIdx.x * k + r] = fp;
+#pragma unroll
+    for (uint32_t j = 0; j < kTopPer; j++)
+      if (p[j] == fp) p[j] = kNone;  // positions are unique: only the owner matches
   }
 }
 
@@ -372,22 +394,31 @@ __global__ void scan_chunk_sums_kernel(const uint32_t* a, uint64_t n, uint32_t* 
   __syncthreads();
   if (threadIdx.x == 0) sums[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
-__global__ void scan_apply_kernel(uint32_t* a, uint64_t n, const uint32_t* offs) {
-  __shared__ uint32_t v[kScanChunk];
-  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
-  for (uint32_t j = threadIdx.x; j < kScanChunk; j += 256) v[j] = base + j < n ? a[base + j] : 0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = offs[blockIdx.x];
-    for (uint32_t j = 0; j < kScanChunk; j++) {
-      const uint32_t c = v[j];
-      v[j] = run;
-      run += c;
-    }
+__global__ void __launch_bounds__(256) scan_apply_kernel(uint32_t* a, uint64_t n, const uint32_t* offs) {
+  constexpr uint32_t per = kScanChunk / 256;
+  __shared__ uint32_t pre[256];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * per;
+  uint32_t v[per];
+  uint32_t sum = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < per; j++) {
+    v[j] = base + j < n ? a[base + j] : 0;
+    sum += v[j];
   }
+  pre[threadIdx.x] = sum;
   __syncthreads();
-  for (uint32_t j = threadIdx.x; j < kScanChunk; j += 256)
-    if (base + j < n) a[base + j] = v[j];
+  for (uint32_t o = 1; o < 256; o <<= 1) {
+    const uint32_t x = threadIdx.x >= o ? pre[threadIdx.x - o] : 0;
+    __syncthreads();
+    pre[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = offs[blockIdx.x] + pre[threadIdx.x] - sum;
+#pragma unroll
+  for (uint32_t j = 0; j < per; j++) {
+    if (base + j < n) a[base + j] = run;
+    run += v[j];
+  }
 }
 
 // Stable scatter: round j handles items j*256 + t, so (round, wave, lane)
@@ -557,26 +588,58 @@ hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_
   SortScratch S = carve(scratch, n, p.n_comps);
   hipLaunchKernelGGL(make_comps_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, p, (const uint32_t*)gids,
                      n, S.comps);
-  if (n <= kSmallSort) {
-    hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps, n,
-                       (const uint32_t*)nullptr, (uint32_t)n, S.pos_a);
-  } else if (need <= 64) {
-    const uint32_t blocks = (uint32_t)((n + kTopChunk - 1) / kTopChunk);
-    hipLaunchKernelGGL(block_topk_kernel, dim3(blocks), dim3(kTopThreads), 0, s, (const uint64_t*)S.comps,
-                       p.n_comps, n, (uint32_t)need, S.pos_a);
-    const uint64_t last = n - (uint64_t)(blocks - 1) * kTopChunk;
-    const uint64_t m = (uint64_t)(blocks - 1) * need + (last < need ? last : need);
-    if (m <= kSmallSort) {
-      hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps, n,
-                         (const uint32_t*)S.pos_a, (uint32_t)m, S.pos_b);
-      uint32_t* t = S.pos_a; S.pos_a = S.pos_b; S.pos_b = t;
-    } else {
-      hipError_t e2 = radix_sort_positions(S, n, p.n_comps, m, s);
-      if (e2 != hipSuccess) return e2;
+  // Words equal across every group decide nothing (a DECIMAL SUM's high
+  // half is usually constant): drop them so comparisons stay on the first
+  // word that varies.
+  uint32_t n_comps = 0;
+  {
+    uint64_t mm[2 * kMaxComps];
+    for (uint32_t c = 0; c < p.n_comps; c++) {
+      hipLaunchKernelGGL(init_mm_kernel, dim3(1), dim3(1), 0, s, S.mm + 2 * c);
+      hipLaunchKernelGGL(minmax_kernel, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s,
+                         (const uint64_t*)(S.comps + (uint64_t)c * n), n, S.mm + 2 * c);
     }
+    hipError_t e = hipMemcpyAsync(mm, S.mm, 16 * p.n_comps, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    for (uint32_t c = 0; c < p.n_comps; c++) {
+      if (mm[2 * c] == mm[2 * c + 1]) continue;
+      if (c != n_comps) {
+        e = hipMemcpyAsync(S.comps + (uint64_t)n_comps * n, S.comps + (uint64_t)c * n, n * 8,
+                           hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+      }
+      n_comps++;
+    }
+  }
+  if (n_comps == 0) {
+    // every group ties on every word: slot order is the order
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, S.pos_a, n);
+  } else if (n <= kSmallSort) {
+    hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, n_comps, n,
+                       (const uint32_t*)nullptr, (uint32_t)n, S.pos_a);
+  } else if (need <= kTopMaxK) {
+    // per-workgroup top-k, repeated over the candidates until one
+    // workgroup can sort them
+    const uint32_t* cur = nullptr;
+    uint32_t* dst = S.pos_a;
+    uint32_t* other = S.pos_b;
+    uint64_t m = n;
+    while (m > kSmallSort) {
+      const uint32_t blocks = (uint32_t)((m + kTopChunk - 1) / kTopChunk);
+      hipLaunchKernelGGL(block_topk_kernel, dim3(blocks), dim3(kTopThreads), 0, s, (const uint64_t*)S.comps,
+                         n_comps, n, cur, m, (uint32_t)need, dst);
+      const uint64_t last = m - (uint64_t)(blocks - 1) * kTopChunk;
+      m = (uint64_t)(blocks - 1) * need + (last < need ? last : need);
+      cur = dst;
+      uint32_t* t = dst; dst = other; other = t;
+    }
+    hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, n_comps, n, cur,
+                       (uint32_t)m, dst);
+    S.pos_a = dst;
   } else {
     hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, S.pos_a, n);
-    hipError_t e = radix_sort_positions(S, n, p.n_comps, n, s);
+    hipError_t e = radix_sort_positions(S, n, n_comps, n, s);
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(select_gids_kernel, dim3(grid_for(need, 256, 4096)), dim3(256), 0, s, (const uint32_t*)gids,

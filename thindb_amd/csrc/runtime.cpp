@@ -343,7 +343,7 @@ int32_t validate(tdbg_plan* pl) {
     if (!is_int_type(p.type) && !is_float_type(p.type)) return fail(TDBG_ERR_UNSUPPORTED, "predicate type");
     use(p.column_idx);
   }
-  if (cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than 8 distinct columns");
+  if (cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than %d distinct columns", kMaxCols);
   if (pl->order.size() > 6) return fail(TDBG_ERR_UNSUPPORTED, "more than 6 ORDER BY items");
   for (auto& o : pl->order)
     if (o.target >= pl->keys.size() + pl->aggs.size()) return fail(TDBG_ERR_INVALID, "ORDER BY target");
@@ -554,7 +554,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
   sp.g_rec = std::max<uint32_t>(8, off);
   tmpl.resize(sp.g_rec / 8, 0);
   P.tmpl = tmpl;
-  if (P.used_cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than 8 distinct columns");
+  if (P.used_cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than %d distinct columns", kMaxCols);
   // columns
   sp.n_cols = (uint32_t)P.used_cols.size();
   for (uint32_t i = 0; i < sp.n_cols; i++) {
@@ -572,39 +572,54 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
     sp.g_cap = next_pow2(std::max<uint64_t>(1024, est + est / 2 + 1));
   }
   // launch shape and LDS table
-  const int n_cu = pl->ctx->n_cu;
-  uint32_t slot_bytes = 4;  // u32 row count
-  for (uint32_t k = 0; k < sp.n_aggs; k++) {
-    const AggDesc& d = sp.aggs[k];
-    slot_bytes += d.kind == A_COUNT_COL ? 4 : 8;
-    if (d.nullable) slot_bytes += 4;
-  }
-  // exact-sum mode needs rows per workgroup; assume one workgroup per CU
-  const uint64_t rows_per_wg = (uint64_t)((n_tiles + n_cu - 1) / std::max(n_cu, 1) + 1) * kTileRows;
-  for (uint32_t k = 0; k < sp.n_aggs; k++) {
-    AggDesc& d = sp.aggs[k];
-    if (d.kind != A_SUM_I) continue;
-    const ColumnArena& a = t->cols[kaggs[k].col];
-    const double m = a.has_stats ? std::max(std::fabs((double)a.min_i), std::fabs((double)a.max_i)) : 9.3e18;
-    d.split = ((double)rows_per_wg * m >= 4.0e18) ? 1u : 0u;
-    if (d.split) slot_bytes += 8;
-  }
-  uint32_t lds_cap = 0;
-  uint32_t key_bytes = dom < 0xffffffffull ? 4 : 8;
+  const uint32_t n_cu = (uint32_t)std::max(pl->ctx->n_cu, 8);
   const uint32_t budget = std::min<uint32_t>(kLdsBudget, (uint32_t)pl->ctx->max_lds - 1024);
-  if (dense && dom * slot_bytes <= budget) {
-    sp.lds_mode = LDS_DIRECT;
-    lds_cap = (uint32_t)dom;
-  } else {
-    const uint32_t cap = budget / (slot_bytes + key_bytes);
+  const uint32_t key_bytes = dom < 0xffffffffull ? 4 : 8;
+  // LDS bytes of one slot when a workgroup folds `rows_per_wg` rows: an
+  // integer SUM keeps one i64 partial when rows_per_wg * max|v| < 2^62,
+  // else split 32-bit halves.
+  auto slot_bytes_for = [&](uint64_t rows_per_wg) -> uint32_t {
+    uint32_t sb = 4;  // u32 row count
+    for (uint32_t k = 0; k < sp.n_aggs; k++) {
+      AggDesc& d = sp.aggs[k];
+      sb += d.kind == A_COUNT_COL ? 4 : 8;
+      if (d.nullable) sb += 4;
+      if (d.kind != A_SUM_I) continue;
+      const ColumnArena& a = t->cols[kaggs[k].col];
+      const double m = a.has_stats ? std::max(std::fabs((double)a.min_i), std::fabs((double)a.max_i)) : 9.3e18;
+      d.split = ((double)rows_per_wg * m >= 4.0e18) ? 1u : 0u;
+      if (d.split) sb += 8;
+    }
+    return sb;
+  };
+  uint32_t lds_cap = 0, parts = 1;
+  sp.lds_mode = LDS_NONE;
+  if (dense) {
+    // a direct table, split over `parts` workgroups per tile range when the
+    // key span does not fit one workgroup's LDS
+    for (uint32_t p = 1; p <= 8; p *= 2) {
+      const uint32_t groups = std::max<uint32_t>(8, n_cu / p);
+      const uint64_t rows_per_wg = ((uint64_t)n_tiles + groups - 1) / groups * kTileRows;
+      const uint32_t sb = slot_bytes_for(rows_per_wg);
+      const uint64_t cap = (dom + p - 1) / p;
+      if (cap * sb <= budget) {
+        sp.lds_mode = LDS_DIRECT;
+        lds_cap = (uint32_t)cap;
+        parts = p;
+        break;
+      }
+    }
+  }
+  if (sp.lds_mode == LDS_NONE) {
+    const uint32_t sb = slot_bytes_for(((uint64_t)n_tiles + n_cu - 1) / n_cu * kTileRows);
+    const uint32_t cap = budget / (sb + key_bytes);
     if (est <= 8ull * cap && cap >= 256) {
       sp.lds_mode = LDS_HASH;
       lds_cap = cap;
-    } else {
-      sp.lds_mode = LDS_NONE;
     }
   }
   sp.lds_cap = lds_cap;
+  sp.lds_parts = parts;
   sp.lds_key64 = key_bytes == 8;
   uint32_t lo = 0;
   auto lds_take = [&](uint32_t bytes) {
@@ -628,9 +643,16 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
     }
     sp.lds_bytes = (uint32_t)align_up(lo, 16);
     if (sp.lds_bytes > budget) return fail(TDBG_ERR_DEVICE, "LDS layout %u bytes over budget", sp.lds_bytes);
-    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(2, (uint32_t)pl->ctx->max_lds / std::max<uint32_t>(sp.lds_bytes, 1)));
-    P.block = per_cu >= 2 ? 512 : 1024;
-    P.grid = (uint32_t)std::min<uint64_t>((uint64_t)n_cu * per_cu, std::max<uint32_t>(n_tiles, 1));
+    if (parts > 1) {
+      // one 1024-thread workgroup per CU; grid/parts a multiple of 8 keeps
+      // a tile range's partners on one XCD
+      P.block = 1024;
+      P.grid = std::max<uint32_t>(8 * parts, n_cu / (8 * parts) * (8 * parts));
+    } else {
+      const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(2, (uint32_t)pl->ctx->max_lds / std::max<uint32_t>(sp.lds_bytes, 1)));
+      P.block = per_cu >= 2 ? 512 : 1024;
+      P.grid = (uint32_t)std::min<uint64_t>((uint64_t)n_cu * per_cu, std::max<uint32_t>(n_tiles, 1));
+    }
   } else {
     P.block = 256;
     P.grid = (uint32_t)std::min<uint64_t>((uint64_t)n_cu * 8, std::max<uint32_t>(n_tiles, 1) * 4ull);

@@ -287,12 +287,12 @@ __device__ void lds_init(const ScanParams& P, uint8_t* lds) {
 }
 
 // Every occupied LDS slot goes to its HBM record once.
-__device__ void lds_flush(const ScanParams& P, uint8_t* lds) {
+__device__ void lds_flush(const ScanParams& P, uint8_t* lds, uint64_t part_lo) {
   const uint32_t cap = P.lds_cap;
   for (uint32_t s = threadIdx.x; s < cap; s += blockDim.x) {
     const uint32_t cnt = lds_at<uint32_t>(lds, P.lds_count_off)[s];
     if (cnt == 0) continue;
-    uint64_t key = s;
+    uint64_t key = part_lo + s;
     if (P.lds_mode == LDS_HASH)
       key = P.lds_key64 ? lds_at<uint64_t>(lds, P.lds_keys_off)[s] : lds_at<uint32_t>(lds, P.lds_keys_off)[s];
     uint8_t* rec = g_record(P, key);
@@ -330,19 +330,189 @@ __device__ void lds_flush(const ScanParams& P, uint8_t* lds) {
 
 // ---- one strip: kVec rows of this thread -----------------------------------
 
-template <uint32_t LMODE>
-__device__ __forceinline__ void strip(const ScanParams& P, uint8_t* lds, uint64_t r, uint32_t cnt) {
+// The raw bytes of one column for this thread's kVec rows (at most 32), as
+// four 64-bit words: 64-bit kinds one row per word, narrower kinds packed
+// little-endian from w0.
+struct Raw {
+  uint64_t w0, w1, w2, w3;
+};
+
+__device__ __forceinline__ Raw load_raw(const ColDesc& c, uint64_t r) {
+  Raw x = {0, 0, 0, 0};
+  switch (c.kind) {
+    case K_I64:
+    case K_F64: {
+      const ulonglong2* p = reinterpret_cast<const ulonglong2*>(static_cast<const uint64_t*>(c.data) + r);
+      const ulonglong2 a = p[0], b = p[1];
+      x.w0 = a.x; x.w1 = a.y; x.w2 = b.x; x.w3 = b.y;
+      break;
+    }
+    case K_I32:
+    case K_F32: {
+      const ulonglong2 a = *reinterpret_cast<const ulonglong2*>(static_cast<const uint32_t*>(c.data) + r);
+      x.w0 = a.x; x.w1 = a.y;
+      break;
+    }
+    case K_I16: x.w0 = *reinterpret_cast<const uint64_t*>(static_cast<const uint16_t*>(c.data) + r); break;
+    default: x.w0 = *reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(c.data) + r); break;
+  }
+  return x;
+}
+
+__device__ __forceinline__ void decode_i(const Raw& x, uint32_t kind, int64_t v[kVec]) {
+  switch (kind) {
+    case K_I64: v[0] = (int64_t)x.w0; v[1] = (int64_t)x.w1; v[2] = (int64_t)x.w2; v[3] = (int64_t)x.w3; break;
+    case K_I32:
+      v[0] = (int32_t)(uint32_t)x.w0; v[1] = (int32_t)(uint32_t)(x.w0 >> 32);
+      v[2] = (int32_t)(uint32_t)x.w1; v[3] = (int32_t)(uint32_t)(x.w1 >> 32);
+      break;
+    case K_I16:
+      v[0] = (int16_t)(uint16_t)x.w0; v[1] = (int16_t)(uint16_t)(x.w0 >> 16);
+      v[2] = (int16_t)(uint16_t)(x.w0 >> 32); v[3] = (int16_t)(uint16_t)(x.w0 >> 48);
+      break;
+    case K_I8:
+      v[0] = (int8_t)(uint8_t)x.w0; v[1] = (int8_t)(uint8_t)(x.w0 >> 8);
+      v[2] = (int8_t)(uint8_t)(x.w0 >> 16); v[3] = (int8_t)(uint8_t)(x.w0 >> 24);
+      break;
+    default:  // K_U8
+      v[0] = (uint8_t)x.w0; v[1] = (uint8_t)(x.w0 >> 8); v[2] = (uint8_t)(x.w0 >> 16); v[3] = (uint8_t)(x.w0 >> 24);
+      break;
+  }
+}
+
+__device__ __forceinline__ void decode_f(const Raw& x, uint32_t kind, double v[kVec]) {
+  if (kind == K_F64) {
+    v[0] = __longlong_as_double((int64_t)x.w0); v[1] = __longlong_as_double((int64_t)x.w1);
+    v[2] = __longlong_as_double((int64_t)x.w2); v[3] = __longlong_as_double((int64_t)x.w3);
+  } else {
+    v[0] = __uint_as_float((uint32_t)x.w0); v[1] = __uint_as_float((uint32_t)(x.w0 >> 32));
+    v[2] = __uint_as_float((uint32_t)x.w1); v[3] = __uint_as_float((uint32_t)(x.w1 >> 32));
+  }
+}
+
+constexpr int kFastCols = 6;  // columns a strip holds in registers
+
+// The strip's columns held in registers (named fields, never an indexed
+// array, so they stay in VGPRs).
+struct RawSet {
+  Raw c0, c1, c2, c3, c4, c5;
+};
+
+// Column `c` of the strip: from the registers loaded at the strip's start
+// (FAST) or straight from memory.  `c` is uniform across the launch, so the
+// switch is a scalar branch.
+template <bool FAST>
+__device__ __forceinline__ Raw fetch(const ScanParams& P, const RawSet& raw, uint32_t c, uint64_t r) {
+  if (FAST) {
+    switch (__builtin_amdgcn_readfirstlane(c)) {
+      case 0: return raw.c0;
+      case 1: return raw.c1;
+      case 2: return raw.c2;
+      case 3: return raw.c3;
+      case 4: return raw.c4;
+      default: return raw.c5;
+    }
+  }
+  return load_raw(P.cols[c], r);
+}
+
+__device__ __forceinline__ uint32_t eval_pred_raw(const PredDesc& p, const Raw& x, uint32_t kind) {
+  uint32_t m = 0;
+  if (p.kind == P_RANGE_I || p.kind == P_NEQ_I) {
+    int64_t v[kVec];
+    decode_i(x, kind, v);
+#pragma unroll
+    for (int j = 0; j < kVec; j++) {
+      const bool ok = p.kind == P_RANGE_I ? (v[j] >= p.lo_i && v[j] <= p.hi_i) : (v[j] != p.lo_i);
+      m |= (uint32_t)ok << j;
+    }
+  } else {
+    double v[kVec];
+    decode_f(x, kind, v);
+#pragma unroll
+    for (int j = 0; j < kVec; j++) {
+      bool ok;
+      if (p.kind == P_RANGE_F) {
+        const bool lo = p.lo_incl ? v[j] >= p.lo_f : v[j] > p.lo_f;
+        const bool hi = p.hi_incl ? v[j] <= p.hi_f : v[j] < p.hi_f;
+        ok = lo && hi;
+      } else {
+        ok = v[j] != p.lo_f;
+      }
+      m |= (uint32_t)ok << j;
+    }
+  }
+  return m;
+}
+
+// LDS slots of the strip's rows, probing all rows together so their LDS
+// round trips overlap; -1 = not found within kLdsProbe probes.
+template <typename K>
+__device__ __forceinline__ void lds_find4(K* keys, uint32_t cap, const uint64_t key[kVec], uint32_t mask,
+                                          int slot[kVec]) {
+  const K empty = (K)kEmptyKey;
+  uint32_t s[kVec];
+  uint32_t todo = mask;
+#pragma unroll
+  for (int j = 0; j < kVec; j++) {
+    slot[j] = -1;
+    s[j] = __umulhi(hash32(key[j]), cap);
+  }
+#pragma nounroll
+  for (int probe = 0; probe < kLdsProbe && todo; probe++) {
+    K cur[kVec];
+#pragma unroll
+    for (int j = 0; j < kVec; j++)
+      cur[j] = ((todo >> j) & 1u) ? __hip_atomic_load(keys + s[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)
+                                  : (K)0;
+#pragma unroll
+    for (int j = 0; j < kVec; j++) {
+      if (!((todo >> j) & 1u)) continue;
+      const K k = (K)key[j];
+      bool hit = cur[j] == k;
+      if (!hit && cur[j] == empty) {
+        K expect = empty;
+        hit = __hip_atomic_compare_exchange_strong(keys + s[j], &expect, k, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP) ||
+              expect == k;
+      }
+      if (hit) {
+        slot[j] = (int)s[j];
+        todo &= ~(1u << j);
+      } else {
+        s[j] = s[j] + 1 == cap ? 0 : s[j] + 1;
+      }
+    }
+  }
+}
+
+template <uint32_t LMODE, bool FAST>
+__device__ __forceinline__ void strip(const ScanParams& P, uint8_t* lds, uint64_t r, uint32_t cnt,
+                                      uint64_t part_lo) {
+  // Every column of the strip is requested before any is used, so a
+  // thread keeps all its loads in flight at once.
+  RawSet raw = {};
+  if (FAST) {
+    const uint32_t nc = P.n_cols;
+    if (nc > 0) raw.c0 = load_raw(P.cols[0], r);
+    if (nc > 1) raw.c1 = load_raw(P.cols[1], r);
+    if (nc > 2) raw.c2 = load_raw(P.cols[2], r);
+    if (nc > 3) raw.c3 = load_raw(P.cols[3], r);
+    if (nc > 4) raw.c4 = load_raw(P.cols[4], r);
+    if (nc > 5) raw.c5 = load_raw(P.cols[5], r);
+  }
   uint32_t mask = (1u << cnt) - 1u;
   for (uint32_t i = 0; i < P.n_preds; i++) {
     const PredDesc& pd = P.preds[i];
-    mask &= eval_pred(pd, P.cols[pd.col], r);
-    if (!mask) return;
+    const ColDesc& c = P.cols[pd.col];
+    mask &= eval_pred_raw(pd, fetch<FAST>(P, raw, pd.col, r), c.kind) & valid4(c, r);
   }
+  if (!mask) return;
   uint64_t key[kVec] = {0, 0, 0, 0};
   for (uint32_t i = 0; i < P.n_parts; i++) {
     const PartDesc& pt = P.parts[i];
     int64_t v[kVec];
-    load4_i(P.cols[pt.col], r, v);
+    decode_i(fetch<FAST>(P, raw, pt.col, r), P.cols[pt.col].kind, v);
 #pragma unroll
     for (int j = 0; j < kVec; j++) {
       const uint64_t rel = (uint64_t)v[j] - (uint64_t)pt.base;
@@ -353,20 +523,25 @@ __device__ __forceinline__ void strip(const ScanParams& P, uint8_t* lds, uint64_
       key[j] += rel * pt.stride;
     }
   }
+  if (LMODE == LDS_DIRECT) {
+    // this workgroup owns keys [part_lo, part_lo + lds_cap)
+#pragma unroll
+    for (int j = 0; j < kVec; j++)
+      if (key[j] - part_lo >= P.lds_cap) mask &= ~(1u << j);
+  }
   if (!mask) return;
   int slot[kVec];
   uint8_t* rec[kVec];
+  if (LMODE == LDS_HASH) {
+    if (P.lds_key64) lds_find4(lds_at<uint64_t>(lds, P.lds_keys_off), P.lds_cap, key, mask, slot);
+    else lds_find4(lds_at<uint32_t>(lds, P.lds_keys_off), P.lds_cap, key, mask, slot);
+  }
 #pragma unroll
   for (int j = 0; j < kVec; j++) {
-    slot[j] = -1;
+    if (LMODE == LDS_DIRECT) slot[j] = (int)(key[j] - part_lo);
+    if (LMODE == LDS_NONE) slot[j] = -1;
     rec[j] = nullptr;
     if (!((mask >> j) & 1u)) continue;
-    if (LMODE == LDS_DIRECT) {
-      slot[j] = (int)key[j];
-    } else if (LMODE == LDS_HASH) {
-      slot[j] = P.lds_key64 ? lds_find(lds_at<uint64_t>(lds, P.lds_keys_off), P.lds_cap, key[j])
-                            : lds_find(lds_at<uint32_t>(lds, P.lds_keys_off), P.lds_cap, key[j]);
-    }
     if (slot[j] < 0) {
       rec[j] = g_record(P, key[j]);
       if (!rec[j]) {
@@ -386,9 +561,9 @@ __device__ __forceinline__ void strip(const ScanParams& P, uint8_t* lds, uint64_
     int64_t iv[kVec] = {0, 0, 0, 0};
     double fv[kVec] = {0, 0, 0, 0};
     if (d.kind == A_SUM_F || d.kind == A_MIN_F || d.kind == A_MAX_F) {
-      load4_f(c, r, fv);
+      decode_f(fetch<FAST>(P, raw, d.col, r), c.kind, fv);
     } else if (d.kind != A_COUNT_COL) {
-      load4_i(c, r, iv);
+      decode_i(fetch<FAST>(P, raw, d.col, r), c.kind, iv);
     }
 #pragma unroll
     for (int j = 0; j < kVec; j++) {
@@ -399,53 +574,68 @@ __device__ __forceinline__ void strip(const ScanParams& P, uint8_t* lds, uint64_
   }
 }
 
-template <uint32_t LMODE>
+template <uint32_t LMODE, bool FAST>
 __global__ void __launch_bounds__(1024) scan_agg_kernel(const ScanParams P) {
   extern __shared__ __align__(16) uint8_t lds[];
   if (LMODE != LDS_NONE) {
     lds_init(P, lds);
     __syncthreads();
   }
-  const uint32_t t0 = (uint32_t)(((uint64_t)blockIdx.x * P.n_tiles) / gridDim.x);
-  const uint32_t t1 = (uint32_t)(((uint64_t)(blockIdx.x + 1) * P.n_tiles) / gridDim.x);
+  // With a partitioned direct table, the `parts` workgroups of one group
+  // (blockIdx.x = part * groups + g; groups a multiple of 8 keeps them on
+  // one XCD, sharing its L2) scan the same tiles, each aggregating only the
+  // keys it owns, so the tiles stream from HBM once.
+  const uint32_t parts = LMODE == LDS_DIRECT ? P.lds_parts : 1u;
+  const uint32_t groups = gridDim.x / parts;
+  const uint32_t g = blockIdx.x % groups;
+  const uint64_t part_lo = (uint64_t)(blockIdx.x / groups) * P.lds_cap;
+  const uint32_t t0 = (uint32_t)(((uint64_t)g * P.n_tiles) / groups);
+  const uint32_t t1 = (uint32_t)(((uint64_t)(g + 1) * P.n_tiles) / groups);
   const uint32_t step = blockDim.x * kVec;
   for (uint32_t t = t0; t < t1; t++) {
     const Tile T = P.tiles[t];
     for (uint32_t s = threadIdx.x * kVec; s < T.n; s += step) {
       const uint32_t cnt = T.n - s < (uint32_t)kVec ? T.n - s : (uint32_t)kVec;
-      strip<LMODE>(P, lds, T.row0 + s, cnt);
+      strip<LMODE, FAST>(P, lds, T.row0 + s, cnt, part_lo);
     }
   }
   if (LMODE != LDS_NONE) {
     __syncthreads();
-    lds_flush(P, lds);
+    lds_flush(P, lds, part_lo);
   }
 }
 
-static hipError_t allow_large_lds() {
+template <uint32_t LMODE>
+static hipError_t allow_large_lds_for() {
   const int bytes = 160 * 1024;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(scan_agg_kernel<LDS_DIRECT>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(scan_agg_kernel<LMODE, true>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(scan_agg_kernel<LDS_HASH>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(scan_agg_kernel<LMODE, false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   return e;
+}
+
+static hipError_t allow_large_lds() {
+  hipError_t e = allow_large_lds_for<LDS_DIRECT>();
+  return e == hipSuccess ? allow_large_lds_for<LDS_HASH>() : e;
+}
+
+template <uint32_t LMODE>
+static void launch_mode(const ScanParams& p, uint32_t grid, uint32_t block, size_t lds, hipStream_t s) {
+  if (p.n_cols <= (uint32_t)kFastCols)
+    hipLaunchKernelGGL((scan_agg_kernel<LMODE, true>), dim3(grid), dim3(block), lds, s, p);
+  else
+    hipLaunchKernelGGL((scan_agg_kernel<LMODE, false>), dim3(grid), dim3(block), lds, s, p);
 }
 
 hipError_t launch_scan_agg(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s) {
   static const hipError_t attr = allow_large_lds();
   (void)attr;  // best effort: a launch over the limit reports its own error
-  const size_t lds = p.lds_mode == LDS_NONE ? 0 : p.lds_bytes;
   switch (p.lds_mode) {
-    case LDS_DIRECT:
-      hipLaunchKernelGGL(scan_agg_kernel<LDS_DIRECT>, dim3(grid), dim3(block), lds, s, p);
-      break;
-    case LDS_HASH:
-      hipLaunchKernelGGL(scan_agg_kernel<LDS_HASH>, dim3(grid), dim3(block), lds, s, p);
-      break;
-    default:
-      hipLaunchKernelGGL(scan_agg_kernel<LDS_NONE>, dim3(grid), dim3(block), 0, s, p);
-      break;
+    case LDS_DIRECT: launch_mode<LDS_DIRECT>(p, grid, block, p.lds_bytes, s); break;
+    case LDS_HASH: launch_mode<LDS_HASH>(p, grid, block, p.lds_bytes, s); break;
+    default: launch_mode<LDS_NONE>(p, grid, block, 0, s); break;
   }
   return hipGetLastError();
 }

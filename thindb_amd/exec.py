@@ -302,7 +302,7 @@ class GpuGroupQuery:
             return None
         self._done = True
         self.session.stage_table(self.table, self._columns)
-        cols = self.plan.execute([(rg.segment_id, rg.idx) for rg in self.table.row_groups])
+        cols = self.plan.execute(self.plan.rg_array([(rg.segment_id, rg.idx) for rg in self.table.row_groups]))
         n = len(cols[0].values) if cols else 0
         self._batch = Batch(self._schema, cols, n)
         return self._batch

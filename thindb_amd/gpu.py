@@ -348,14 +348,26 @@ class Plan:
         check(lib.tdbg_plan_create(ctx.handle, C.byref(desc), C.byref(h)))
         self._h = h
 
-    def execute(self, rgs: Optional[Sequence[tuple]] = None) -> list:
-        """Runs the plan; returns the result columns (keys, then aggregates)."""
+    @staticmethod
+    def rg_array(rgs: Sequence[tuple]):
+        """A reusable C array of (segment_id, row_group_idx) references."""
+        arr = (RgRef * max(1, len(rgs)))()
+        for i, (s, r) in enumerate(rgs):
+            arr[i].segment_id = s
+            arr[i].row_group_idx = r
+        return arr, len(rgs)
+
+    def execute(self, rgs=None) -> list:
+        """Runs the plan over `rgs` (None = every staged row group, a list of
+        (segment_id, row_group_idx), or a prepared :meth:`rg_array`); returns
+        the result columns (keys, then aggregates)."""
         out = C.c_void_p()
         if rgs is None:
             check(self._lib.tdbg_execute(self._h, None, 0, C.byref(out)))
         else:
-            arr = (RgRef * max(1, len(rgs)))(*[RgRef(s, r, 0) for s, r in rgs])
-            check(self._lib.tdbg_execute(self._h, arr, len(rgs), C.byref(out)))
+            arr, n = rgs if isinstance(rgs, tuple) and len(rgs) == 2 and isinstance(rgs[1], int) \
+                and not isinstance(rgs[0], int) else self.rg_array(rgs)
+            check(self._lib.tdbg_execute(self._h, arr, n, C.byref(out)))
         try:
             n = self._lib.tdbg_result_num_columns(out)
             cols = []
