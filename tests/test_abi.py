@@ -41,7 +41,7 @@ def test_struct_layouts():
     assert C.sizeof(gpu.OrderSpecC) == 8
     assert C.sizeof(gpu.PlanDesc) == 80
     assert C.sizeof(gpu.ColumnView) == 40
-    assert C.sizeof(gpu.ExecStats) == 48
+    assert C.sizeof(gpu.ExecStats) == 56
 
 
 def test_enums_match_header():

@@ -21,7 +21,9 @@ CSRC = PKG / "csrc"
 GPU_LIB = PKG / "libthindb_gpu.so"
 GEN_LIB = PKG / "libthindb_gen.so"
 
-HIP_SOURCES = ["scan_agg.hip", "order.hip", "runtime.cpp"]
+HIP_SOURCES = ["scan_agg.hip", "order.hip", "runtime.cpp", "jit.cpp"]
+# Device headers the run-time compiler needs, embedded as string literals.
+JIT_HEADERS = [("kJitEngineH", "engine.h"), ("kJitScanCommonH", "scan_common.h")]
 ARCH = os.environ.get("THINDB_GPU_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -49,16 +51,30 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{proc.stdout}\n{proc.stderr}")
 
 
+def write_jit_headers(objdir: Path) -> Path:
+    """Embeds the device headers as raw string literals for hipRTC."""
+    out = ["// generated by thindb_amd/_build.py: device headers for run-time compilation"]
+    for sym, name in JIT_HEADERS:
+        text = (CSRC / name).read_text()
+        if ")TDBGJIT\"" in text:
+            raise RuntimeError(f"{name} contains the raw-string delimiter")
+        out.append(f'static const char {sym}[] = R"TDBGJIT({text})TDBGJIT";')
+    path = objdir / "jit_headers.inc"
+    path.write_text("\n".join(out) + "\n")
+    return path
+
+
 def build_gpu(jobs: int = 4) -> Path:
     """Compiles every HIP source for gfx950 and links libthindb_gpu.so."""
     objdir = ROOT / "_build"
     objdir.mkdir(exist_ok=True)
+    write_jit_headers(objdir)
     hipcc = _hipcc()
     objs = []
     procs = []
     for src in HIP_SOURCES:
         obj = objdir / (src + ".o")
-        cmd = [hipcc, *HIPCC_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [hipcc, *HIPCC_FLAGS, "-I", str(objdir), "-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
         procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
         objs.append(obj)
     errors = []
@@ -69,7 +85,7 @@ def build_gpu(jobs: int = 4) -> Path:
     if errors:
         raise RuntimeError("HIP build failed:\n" + "\n".join(errors))
     tmp = GPU_LIB.with_suffix(".so.tmp")
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp), "-lhiprtc"])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
 

@@ -3,8 +3,21 @@
 // arguments (uniform across a launch, read through scalar loads).
 #pragma once
 
+#if defined(__HIPCC_RTC__)
+// Compiled at run time by hipRTC (plan-specialised kernels): the runtime
+// headers are implicit and the fixed-width types live in __hip_internal.
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::int8_t int8_t;
+typedef __hip_internal::int16_t int16_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::int64_t int64_t;
+#else
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace tdbg {
 
@@ -181,10 +194,14 @@ struct FinalParams {
   uint32_t g_rec, g_count_off, g_key_off, g_mode;
 };
 
+#if !defined(__HIPCC_RTC__)
 // ---- kernel launchers (defined in the .hip files) -------------------------
 hipError_t launch_fill_records(uint8_t* state, uint64_t n_rec, uint32_t rec_bytes, const uint64_t* tmpl,
                                hipStream_t s);
 hipError_t launch_scan_agg(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s);
+// Plan-specialised kernel (jit.cpp); hipErrorNotSupported = use launch_scan_agg.
+hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s, bool* used);
+const char* jit_last_log();
 hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_counts, uint32_t* gids,
                           uint64_t* n_out, hipStream_t s);
 hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_t need, void* scratch,
@@ -193,5 +210,6 @@ size_t sort_scratch_bytes(uint64_t n, uint32_t n_comps);
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
                            hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+#endif
 
 }  // namespace tdbg

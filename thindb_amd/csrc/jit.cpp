@@ -1,0 +1,225 @@
+// jit.cpp — plan-specialised scan kernels, compiled at run time with hipRTC.
+//
+// The generic kernel interprets the plan (column kinds, predicate terms, key
+// parts, aggregate kinds) on every strip; that costs about half of its
+// bandwidth.  Here a plan's *structure* becomes straight-line code: each
+// column is loaded with its kind fixed at compile time, each predicate and
+// aggregate is one inlined template instance, and only values (literals,
+// key bases, offsets, pointers) still come from the kernel argument.  The
+// code object is cached per (device, signature), so repeated queries of one
+// shape (a serving loop, a batch) compile once.  When hipRTC is unavailable
+// or THINDB_GPU_JIT=0, the runtime uses the generic kernel.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.h"
+#include "jit_headers.inc"  // kJitEngineH, kJitScanCommonH (generated by the build)
+
+namespace tdbg {
+
+namespace {
+
+struct JitKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  bool failed = false;
+};
+
+std::mutex g_mu;
+std::unordered_map<std::string, JitKernel> g_cache;
+std::string g_last_log;
+
+const char* kind_name(uint32_t k) {
+  switch (k) {
+    case K_U8: return "K_U8";
+    case K_I8: return "K_I8";
+    case K_I16: return "K_I16";
+    case K_I32: return "K_I32";
+    case K_I64: return "K_I64";
+    case K_F32: return "K_F32";
+    default: return "K_F64";
+  }
+}
+const char* pred_name(uint32_t k) {
+  switch (k) {
+    case P_RANGE_I: return "P_RANGE_I";
+    case P_RANGE_F: return "P_RANGE_F";
+    case P_NEQ_I: return "P_NEQ_I";
+    default: return "P_NEQ_F";
+  }
+}
+const char* agg_name(uint32_t k) {
+  switch (k) {
+    case A_COUNT_COL: return "A_COUNT_COL";
+    case A_SUM_I: return "A_SUM_I";
+    case A_SUM_F: return "A_SUM_F";
+    case A_MIN_I: return "A_MIN_I";
+    case A_MAX_I: return "A_MAX_I";
+    case A_MIN_F: return "A_MIN_F";
+    default: return "A_MAX_F";
+  }
+}
+const char* lmode_name(uint32_t m) {
+  return m == LDS_DIRECT ? "LDS_DIRECT" : (m == LDS_HASH ? "LDS_HASH" : "LDS_NONE");
+}
+
+bool jit_enabled() {
+  const char* e = std::getenv("THINDB_GPU_JIT");
+  return !(e && e[0] == '0');
+}
+
+}  // namespace
+
+// The structure of a plan: everything the generated code fixes.
+std::string jit_signature(const ScanParams& p) {
+  std::ostringstream s;
+  s << "m" << p.lds_mode << "k" << p.lds_key64 << "c";
+  for (uint32_t i = 0; i < p.n_cols; i++) s << p.cols[i].kind << (p.cols[i].valid ? "v" : "") << ",";
+  s << "p";
+  for (uint32_t i = 0; i < p.n_preds; i++) s << p.preds[i].kind << ":" << p.preds[i].col << ",";
+  s << "k";
+  for (uint32_t i = 0; i < p.n_parts; i++) s << p.parts[i].col << ",";
+  s << "a";
+  for (uint32_t i = 0; i < p.n_aggs; i++)
+    s << p.aggs[i].kind << ":" << p.aggs[i].col << (p.aggs[i].nullable ? "n" : "") << (p.aggs[i].split ? "s" : "")
+      << ",";
+  return s.str();
+}
+
+std::string jit_source(const ScanParams& p) {
+  std::ostringstream s;
+  s << "#include \"scan_common.h\"\n"
+       "namespace tdbg {\n"
+       "struct JitStrip {\n"
+       "  template <uint32_t LMODE>\n"
+       "  static __device__ __forceinline__ void run(const ScanParams& P, uint8_t* lds, uint64_t r, uint32_t cnt,\n"
+       "                                             uint64_t part_lo) {\n";
+  for (uint32_t i = 0; i < p.n_cols; i++)
+    s << "    const Raw x" << i << " = ld<" << kind_name(p.cols[i].kind) << ">(P.cols[" << i << "], r);\n";
+  s << "    uint32_t mask = (1u << cnt) - 1u;\n";
+  for (uint32_t i = 0; i < p.n_preds; i++) {
+    const PredDesc& d = p.preds[i];
+    s << "    mask &= pred_k<" << pred_name(d.kind) << ", " << kind_name(p.cols[d.col].kind) << ">(P.preds[" << i
+      << "], x" << d.col << ")";
+    if (p.cols[d.col].valid) s << " & valid4(P.cols[" << d.col << "], r)";
+    s << ";\n";
+  }
+  if (p.n_preds) s << "    if (!mask) return;\n";
+  s << "    uint64_t key[kVec];\n";
+  for (uint32_t i = 0; i < p.n_parts; i++) {
+    const PartDesc& d = p.parts[i];
+    s << "    { int64_t v[kVec]; dec_i<" << kind_name(p.cols[d.col].kind) << ">(x" << d.col << ", v); key_part<"
+      << (i == 0 ? "true" : "false") << ">(P, P.parts[" << i << "], v, key, mask); }\n";
+  }
+  s << "    if (!mask) return;\n"
+       "    int slot[kVec];\n"
+       "    uint8_t* rec[kVec];\n"
+       "    mask = resolve<LMODE>(P, lds, key, mask, part_lo, slot, rec);\n";
+  for (uint32_t i = 0; i < p.n_aggs; i++) {
+    const AggDesc& d = p.aggs[i];
+    s << "    { const uint32_t m = mask";
+    if (p.cols[d.col].valid) s << " & valid4(P.cols[" << d.col << "], r)";
+    s << "; if (m) agg_k<" << agg_name(d.kind) << ", " << kind_name(p.cols[d.col].kind) << ", "
+      << (d.nullable ? "true" : "false") << ", " << (d.split ? "true" : "false") << ">(P, P.aggs[" << i << "], x"
+      << d.col << ", m, slot, rec, lds); }\n";
+  }
+  s << "  }\n"
+       "};\n"
+       "}  // namespace tdbg\n"
+       "extern \"C\" __global__ void __launch_bounds__(1024) tdbg_scan_jit(const tdbg::ScanParams P) {\n"
+       "  extern __shared__ __align__(16) uint8_t lds[];\n"
+       "  tdbg::scan_body<tdbg::"
+    << lmode_name(p.lds_mode)
+    << ", tdbg::JitStrip>(P, lds);\n"
+       "}\n";
+  return s.str();
+}
+
+static bool compile(const std::string& src, std::vector<char>& code, std::string& log) {
+  hiprtcProgram prog;
+  const char* headers[] = {kJitEngineH, kJitScanCommonH};
+  const char* names[] = {"engine.h", "scan_common.h"};
+  if (hiprtcCreateProgram(&prog, src.c_str(), "tdbg_scan_jit.hip", 2, headers, names) != HIPRTC_SUCCESS) {
+    log = "hiprtcCreateProgram failed";
+    return false;
+  }
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  hipGetDeviceProperties(&prop, dev);
+  std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+  const char* opts[] = {arch.c_str(), "-O3", "-std=c++17", "-munsafe-fp-atomics",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult rc = hiprtcCompileProgram(prog, 8, opts);
+  size_t log_size = 0;
+  hiprtcGetProgramLogSize(prog, &log_size);
+  if (log_size > 1) {
+    std::vector<char> l(log_size + 1, 0);
+    hiprtcGetProgramLog(prog, l.data());
+    log = l.data();
+  }
+  bool ok = rc == HIPRTC_SUCCESS;
+  if (ok) {
+    size_t n = 0;
+    hiprtcGetCodeSize(prog, &n);
+    code.resize(n);
+    ok = n > 0 && hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+  }
+  hiprtcDestroyProgram(&prog);
+  return ok;
+}
+
+// Launches the plan-specialised kernel; returns hipErrorNotSupported when
+// run-time compilation is unavailable so the caller takes the generic one.
+hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s, bool* used) {
+  *used = false;
+  if (!jit_enabled() || p.n_cols > 8) return hipErrorNotSupported;
+  int dev = 0;
+  hipGetDevice(&dev);
+  const std::string key = std::to_string(dev) + "|" + jit_signature(p);
+  hipFunction_t fn = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    JitKernel& k = g_cache[key];
+    if (k.failed) return hipErrorNotSupported;
+    if (!k.fn) {
+      std::vector<char> code;
+      std::string log;
+      if (!compile(jit_source(p), code, log)) {
+        k.failed = true;
+        g_last_log = log;
+        if (std::getenv("THINDB_GPU_JIT_VERBOSE")) fprintf(stderr, "[tdbg jit] compile failed: %s\n", log.c_str());
+        return hipErrorNotSupported;
+      }
+      if (hipModuleLoadData(&k.mod, code.data()) != hipSuccess ||
+          hipModuleGetFunction(&k.fn, k.mod, "tdbg_scan_jit") != hipSuccess) {
+        (void)hipGetLastError();
+        k.failed = true;
+        return hipErrorNotSupported;
+      }
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                160 * 1024);
+      (void)hipGetLastError();
+    }
+    fn = k.fn;
+  }
+  ScanParams arg = p;
+  void* args[] = {&arg};
+  const size_t lds = p.lds_mode == LDS_NONE ? 0 : p.lds_bytes;
+  hipError_t e = hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, s, args, nullptr);
+  if (e == hipSuccess) *used = true;
+  return e;
+}
+
+const char* jit_last_log() { return g_last_log.c_str(); }
+
+}  // namespace tdbg

@@ -1096,7 +1096,11 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
   HIP_TRY(launch_fill_u32(sp.err, 16, 0, s), "init flags");
   HIP_TRY(hipEventRecord(pl->ev[1], s), "event");
-  HIP_TRY(launch_scan_agg(sp, P.grid, P.block, s), "scan_agg launch");
+  bool specialized = false;
+  if (launch_scan_jit(sp, P.grid, P.block, s, &specialized) != hipSuccess || !specialized) {
+    (void)hipGetLastError();
+    HIP_TRY(launch_scan_agg(sp, P.grid, P.block, s), "scan_agg launch");
+  }
   HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
   if (pl->cancel.load()) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
   // compaction
@@ -1158,6 +1162,8 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   pl->last.n_groups = n_groups;
   pl->last.lds_mode = sp.lds_mode;
   pl->last.global_mode = sp.g_mode;
+  pl->last.specialized = specialized ? 1u : 0u;
+  pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
   return make_result(pl, n_out, host, data_off, valid_off, out);
 }
 

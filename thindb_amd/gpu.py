@@ -8,12 +8,13 @@ from __future__ import annotations
 
 import ctypes as C
 import enum
+import os
 from pathlib import Path
 from typing import Optional, Sequence
 
 import numpy as np
 
-_LIB_PATH = Path(__file__).resolve().parent / "libthindb_gpu.so"
+_LIB_PATH = Path(os.environ.get("THINDB_GPU_LIB", Path(__file__).resolve().parent / "libthindb_gpu.so"))
 
 ABI_VERSION = 1
 
@@ -164,7 +165,7 @@ class ColumnView(C.Structure):
 class ExecStats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("scan_ms", C.c_double), ("rows_scanned", C.c_uint64),
                 ("bytes_scanned", C.c_uint64), ("n_groups", C.c_uint64), ("lds_mode", C.c_uint32),
-                ("global_mode", C.c_uint32)]
+                ("global_mode", C.c_uint32), ("specialized", C.c_uint32), ("lds_parts", C.c_uint32)]
 
 
 # Every entry point include/thindb_gpu.h declares, with its ctypes signature.
