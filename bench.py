@@ -102,6 +102,9 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=0, help="override the row count (testing only)")
     ap.add_argument("--cpu-sample-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a device-to-device copy of MIB MiB (a known-size kernel that scales "
+                         "the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this engine has no multi-GPU path (asked for {args.gpus}); "
@@ -110,6 +113,17 @@ def main() -> int:
 
     from thindb_amd import gen
     from thindb_amd.exec import GpuSession, try_build_gpu_group
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n // 8, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 copies of {n} bytes")
+        del src, dst
 
     rows, bytes_per_row, desc = WORKLOADS[args.workload]
     if args.rows:
@@ -164,6 +178,13 @@ def main() -> int:
         cpu = cpu_baseline(table, req, sample, threads)
         log(f"cpu baseline {cpu['value']:.4g} rows/s")
 
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # FETCH_SIZE/WRITE_SIZE counter passes (scaled by a 1 GiB copy kernel).
+    traffic = None
+    pmc = ROOT / "profiles" / f"scan_jit_{args.workload}_200M_pmc.json"
+    if pmc.exists() and rows == WORKLOADS[args.workload][0] and st.specialized:
+        traffic = json.loads(pmc.read_text())["traffic_bytes"]
+
     out = {
         "metric": "rows/s & HBM GB/s for filtered GROUP BY; batched QPS + p50/p99 latency",
         "value": n_rows_scanned / (ms_per_step / 1e3),
@@ -183,8 +204,10 @@ def main() -> int:
         "device_ms_per_step": float(np.mean(total_ms)),
         "stage_gbps_pcie": staged / stage_s / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "scan_agg_kernel", "kernel_ms": float(np.mean(scan_ms)),
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_source": str(pmc.relative_to(ROOT)) if traffic else None,
+                     "kernel": "tdbg_scan_jit" if st.specialized else "scan_agg_kernel",
+                     "kernel_ms": float(np.mean(scan_ms)),
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

@@ -327,6 +327,88 @@ __global__ void __launch_bounds__(kTopThreads) block_topk_kernel(const uint64_t*
   }
 }
 
+// ---- one-workgroup radix select (n <= kSelect1) ---------------------------------------
+
+constexpr uint64_t kSelect1 = 1u << 16;
+
+// Exclusive prefix sum over the 1024 threads of a workgroup; *total gets the sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
+  const uint32_t t = threadIdx.x;
+  tmp[t] = v;
+  __syncthreads();
+  for (uint32_t o = 1; o < 1024; o <<= 1) {
+    const uint32_t x = t >= o ? tmp[t - o] : 0;
+    __syncthreads();
+    tmp[t] += x;
+    __syncthreads();
+  }
+  const uint32_t incl = tmp[t];
+  *total = tmp[1023];
+  __syncthreads();
+  return incl - v;
+}
+
+// The `need` best positions by the first order word: radix-selects the
+// need-th smallest word T (12-bit digits over the word's [min, max] range),
+// then gathers, in position order, every position whose word is <= T —
+// the best `need` plus any ties at T — into out[0 .. *m).
+__global__ void __launch_bounds__(1024) select1_kernel(const uint64_t* __restrict__ w0, uint32_t n, const uint64_t* mm,
+                                                       uint32_t need, uint32_t* out, uint32_t* m_out) {
+  __shared__ uint32_t hist[4096];
+  __shared__ uint32_t tmp[1024];
+  __shared__ uint32_t sh_digit, sh_before;
+  const uint32_t t = threadIdx.x;
+  const uint64_t lo = mm[0], range = mm[1] - mm[0];
+  const int bits = range == 0 ? 0 : 64 - __clzll(range);
+  uint64_t prefix = 0;
+  uint32_t remaining = need;  // 1-based rank inside the current prefix class
+  for (int shift = (bits + 11) / 12 * 12 - 12; shift >= 0; shift -= 12) {
+    for (uint32_t i = t; i < 4096; i += 1024) hist[i] = 0;
+    __syncthreads();
+    const int hs = shift + 12;
+    for (uint32_t i = t; i < n; i += 1024) {
+      const uint64_t v = w0[i] - lo;
+      if (hs >= 64 || (v >> hs) == (prefix >> hs)) atomicAdd(&hist[(v >> shift) & 4095u], 1u);
+    }
+    __syncthreads();
+    const uint32_t h0 = hist[4 * t], h1 = hist[4 * t + 1], h2 = hist[4 * t + 2], h3 = hist[4 * t + 3];
+    uint32_t total;
+    uint32_t before = block_excl_scan(h0 + h1 + h2 + h3, tmp, &total);
+    if (before < remaining && before + h0 + h1 + h2 + h3 >= remaining) {
+      uint32_t d = 4 * t;
+      if (before + h0 < remaining) {
+        before += h0;
+        d++;
+        if (before + h1 < remaining) {
+          before += h1;
+          d++;
+          if (before + h2 < remaining) {
+            before += h2;
+            d++;
+          }
+        }
+      }
+      sh_digit = d;
+      sh_before = before;
+    }
+    __syncthreads();
+    prefix |= (uint64_t)sh_digit << shift;
+    remaining -= sh_before;
+    __syncthreads();
+  }
+  const uint64_t T = prefix + lo;
+  uint32_t base = 0;
+  for (uint32_t c = 0; c < n; c += 1024) {
+    const uint32_t i = c + t;
+    const uint32_t keep = (i < n && w0[i] <= T) ? 1u : 0u;
+    uint32_t total;
+    const uint32_t at = block_excl_scan(keep, tmp, &total);
+    if (keep) out[base + at] = i;
+    base += total;
+  }
+  if (t == 0) *m_out = base;
+}
+
 // ---- stable LSD radix sort of positions by the composite order --------------------
 
 constexpr uint32_t kRsThreads = 256;
@@ -592,6 +674,7 @@ hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_
   // half is usually constant): drop them so comparisons stay on the first
   // word that varies.
   uint32_t n_comps = 0;
+  uint32_t first_active = 0;
   {
     uint64_t mm[2 * kMaxComps];
     for (uint32_t c = 0; c < p.n_comps; c++) {
@@ -604,6 +687,7 @@ hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_
     if (e != hipSuccess) return e;
     for (uint32_t c = 0; c < p.n_comps; c++) {
       if (mm[2 * c] == mm[2 * c + 1]) continue;
+      if (n_comps == 0) first_active = c;
       if (c != n_comps) {
         e = hipMemcpyAsync(S.comps + (uint64_t)n_comps * n, S.comps + (uint64_t)c * n, n * 8,
                            hipMemcpyDeviceToDevice, s);
@@ -618,6 +702,26 @@ hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_
   } else if (n <= kSmallSort) {
     hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, n_comps, n,
                        (const uint32_t*)nullptr, (uint32_t)n, S.pos_a);
+  } else if (n <= kSelect1 && need <= kSmallSort) {
+    // radix-select the first word in one workgroup, then sort the few
+    // candidates (the best `need` and the ties at the cut)
+    uint32_t* d_m = reinterpret_cast<uint32_t*>(S.mm + 2 * kMaxComps);
+    hipLaunchKernelGGL(select1_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, (uint32_t)n,
+                       (const uint64_t*)(S.mm + 2 * first_active), (uint32_t)need, S.pos_a, d_m);
+    uint32_t m = 0;
+    hipError_t e = hipMemcpyAsync(&m, d_m, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    if (m <= kSmallSort) {
+      hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, n_comps, n,
+                         (const uint32_t*)S.pos_a, m, S.pos_b);
+      uint32_t* t = S.pos_a;
+      S.pos_a = S.pos_b;
+      S.pos_b = t;
+    } else {
+      e = radix_sort_positions(S, n, n_comps, m, s);
+      if (e != hipSuccess) return e;
+    }
   } else if (need <= kTopMaxK) {
     // per-workgroup top-k, repeated over the candidates until one
     // workgroup can sort them
