@@ -207,6 +207,60 @@ def test_highcard_config_small(session, oracle_lib):
     print(f"  highcard: {len(got_all)} groups lds_mode={st.lds_mode} global_mode={st.global_mode}")
 
 
+def test_radix_and_dense_table_agree(session, monkeypatch):
+    """The radix partition path and the HBM dense table give identical rows,
+    in the same order (ties at the LIMIT cut by key in both)."""
+    t = gen.table("highcard", 1_500_000, seed=8)
+    req = Request(["user_id", "bucket"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"),
+                                          AggSpec("max", "w", "m")], [SortSpec("c", True)], 500,
+                  est_groups=100_000_000)  # makes the ~3.2e8-key domain a dense table
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("THINDB_GPU_RADIX", flag)
+        b, st = run_gpu(session, t, req)
+        out[flag] = (rows_of(b), st.global_mode)
+    assert out["0"][1] == 1 and out["1"][1] == 3
+    assert out["0"][0] == out["1"][0]
+    # every group, no ORDER BY: the radix path emits them in key order as well
+    monkeypatch.setenv("THINDB_GPU_RADIX", "0")
+    dense_all, _ = run_gpu(session, t, Request(req.group_cols, req.aggs, est_groups=100_000_000))
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    radix_all, st = run_gpu(session, t, Request(req.group_cols, req.aggs, est_groups=100_000_000))
+    assert st.global_mode == 3 and rows_of(radix_all) == rows_of(dense_all)
+
+
+def test_radix_two_levels_at_width(session, oracle_lib, monkeypatch):
+    """Two full partition levels (2^8 x 2^10 leaves) over 2M rows of the
+    high-cardinality workload."""
+    t = gen.table("highcard", 2_000_000, seed=6)
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    req = Request(["user_id", "bucket"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"),
+                                          AggSpec("max", "w", "m")], [SortSpec("c", True)], 10)
+    got, want, st = check_vs_oracle(session, oracle_lib, t, req, dop=8)
+    assert st.global_mode == 3
+    got, want, st = check_vs_oracle(session, oracle_lib, t, Request(req.group_cols, req.aggs), dop=8)
+    assert st.global_mode == 3 and st.n_groups == len(want)
+
+
+@pytest.mark.parametrize("leaf_bits", ["3", "1", "32"])
+def test_radix_partition_passes(session, oracle_lib, monkeypatch, leaf_bits):
+    """The partition passes forced onto a small typed table: one level
+    (leaf bits 3: 2^7 buckets), two levels (leaf bits 1: 2^8 x 2), and the
+    LDS-budget leaf; NULL inputs, filters, every aggregate kind, ragged row
+    groups split into several staged tiles."""
+    rng = np.random.default_rng(11)
+    t = random_table(rng, 60_000, rg_rows=7001, nulls=True)
+    where = And((Predicate("i", "gt", -2 ** 30), Predicate("s", "neq", 0)))
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    monkeypatch.setenv("THINDB_GPU_RADIX_LEAF_BITS", leaf_bits)
+    for req in [Request(["k1", "k2"], ALL_AGGS, where_filter=where),
+                Request(["k1", "k2"], ALL_AGGS, [SortSpec("c", True)], limit=25),
+                Request(["k2", "k1"], ALL_AGGS, [SortSpec("sb", True), SortSpec("k1")], limit=40, offset=3),
+                Request(["k1"], ALL_AGGS[:5], [SortSpec("k1")])]:
+        got, want, st = check_vs_oracle(session, oracle_lib, t, req)
+        assert st.global_mode == 3
+
+
 def test_hash_table_mode(session, oracle_lib):
     """Keys spread over a wide domain take the HBM hash table."""
     rng = np.random.default_rng(5)

@@ -126,6 +126,34 @@ struct ScanParams {
   uint64_t* g_inserted;    // hash mode: number of claimed records
 };
 
+// ---- radix partitioning (high cardinality) ----------------------------------
+
+// A surviving row as the partition passes move it: the relative key (u32),
+// a validity word when an aggregate input is nullable, then each kernel
+// aggregate's input value at its column's native width.
+struct RowLayout {
+  uint32_t stride;             // bytes per row (multiple of 8)
+  uint32_t flags_off;          // ~0u: no validity word
+  uint32_t val_off[kMaxAggs];  // byte offset of aggregate a's value
+  uint32_t val_bytes[kMaxAggs];
+};
+
+struct RadixParams {
+  RowLayout row;
+  uint32_t shift1, bits1;      // level-1 digit = key >> shift1 (bits1 wide)
+  uint32_t shift2, bits2;      // level-2 digit = (key >> shift2) & (2^bits2 - 1); bits2 = 0: one level
+  uint32_t leaf_bits;          // a leaf holds 2^leaf_bits consecutive keys
+  uint32_t n_leaves;
+  uint32_t tile_rows;          // rows a scatter workgroup stages in LDS at once (multiple of 256)
+  uint32_t pad;
+  uint8_t* rows1;              // level-1 partitioned rows
+  uint8_t* rows2;              // level-2 partitioned rows (== rows1 when one level)
+  uint64_t* bucket_start;      // [2^bits1 + 1] level-1 bucket starts (from the histogram)
+  uint64_t* bucket_fill;       // [2^bits1] level-1 cursors: rows written per bucket
+  uint64_t* leaf_start;        // [n_leaves + 1] leaf row ranges in the final array
+  uint64_t* out_cursor;        // groups emitted
+};
+
 // ---- ordering / emission -------------------------------------------------
 
 // One ORDER BY component value source (64-bit order-preserving words).
@@ -210,6 +238,11 @@ size_t sort_scratch_bytes(uint64_t n, uint32_t n_comps);
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
                            hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_iota(uint32_t* p, uint64_t n, hipStream_t s);
+// Radix partitioning phases (radix.hip): RX_HIST .. RX_AGG in order.
+enum RadixPhase : int { RX_HIST = 0, RX_OFFSETS, RX_SCATTER1, RX_SCATTER2, RX_AGG };
+hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid, unsigned long long* hist,
+                        hipStream_t s, int phase);
 #endif
 
 }  // namespace tdbg

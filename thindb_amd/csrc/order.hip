@@ -657,6 +657,12 @@ __global__ void iota_kernel(uint32_t* p, uint64_t n) {
     p[i] = (uint32_t)i;
 }
 
+hipError_t launch_iota(uint32_t* p, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
 __global__ void select_gids_kernel(const uint32_t* gids, const uint32_t* pos, uint64_t k, uint32_t* out) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (uint64_t)gridDim.x * blockDim.x)
     out[i] = gids[pos[i]];

@@ -269,6 +269,9 @@ struct Prepared {
   std::vector<uint64_t> tmpl;  // one initialised record
   bool empty_domain = false;   // predicates admit no row
   std::vector<uint32_t> used_cols;  // table column per ScanParams col
+  bool radix = false;          // high cardinality: radix partitioning (radix.hip)
+  RadixParams rx{};
+  uint32_t agg_grid = 0;
 };
 
 }  // namespace
@@ -291,7 +294,7 @@ struct tdbg_plan {
   struct Buf {
     void* p = nullptr;
     uint64_t bytes = 0;
-  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout;
+  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf;
   tdbg_exec_stats last{};
 };
 
@@ -618,6 +621,75 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
       lds_cap = cap;
     }
   }
+  // High cardinality over a key domain below 2^32: radix partitioning by the
+  // key's high bits until a leaf's key range fits an LDS direct table.
+  // THINDB_GPU_RADIX=0 never takes this path, =1 takes it whenever the
+  // domain allows (tests use it to cover the partition passes on small tables).
+  const char* rx_env = getenv("THINDB_GPU_RADIX");
+  if ((sp.lds_mode == LDS_NONE || (rx_env && rx_env[0] == '1')) && dom <= 0xffffffffull &&
+      !(rx_env && rx_env[0] == '0')) {
+    const uint32_t sb = slot_bytes_for(n_rows_scan);  // a leaf may hold every row
+    const uint32_t L = dom > 1 ? 64u - (uint32_t)__builtin_clzll(dom - 1) : 0u;  // key bits
+    // two leaf workgroups per CU (THINDB_GPU_RADIX_LEAF_BITS caps the table)
+    const uint32_t leaf_budget = std::min<uint32_t>(budget, (uint32_t)pl->ctx->max_lds / 2) - 4096;
+    const char* lb_env = getenv("THINDB_GPU_RADIX_LEAF_BITS");
+    const uint32_t lb_max = lb_env ? (uint32_t)atoi(lb_env) : 32u;
+    uint32_t leaf_bits = 0;
+    while (leaf_bits < L && leaf_bits < lb_max && ((uint64_t)sb << (leaf_bits + 1)) <= leaf_budget) leaf_bits++;
+    RadixParams& R = P.rx;
+    R = RadixParams{};
+    R.row.flags_off = ~0u;
+    uint32_t ro = 4;  // u32 relative key first
+    for (uint32_t k = 0; k < sp.n_aggs; k++)
+      if (sp.cols[sp.aggs[k].col].valid) R.row.flags_off = 0;
+    if (R.row.flags_off == 0) { R.row.flags_off = ro; ro += 4; }
+    for (uint32_t k = 0; k < sp.n_aggs; k++) {
+      const AggDesc& d = sp.aggs[k];
+      const uint32_t b = d.kind == A_COUNT_COL ? 0u : (uint32_t)t->cols[kaggs[k].col].elem;
+      R.row.val_bytes[k] = b;
+      int same = -1;  // aggregates over one column share its value
+      for (uint32_t j = 0; j < k; j++)
+        if (b && sp.aggs[j].col == d.col && R.row.val_bytes[j] == b) same = (int)j;
+      if (same >= 0) { R.row.val_off[k] = R.row.val_off[same]; continue; }
+      ro = (uint32_t)align_up(ro, std::max<uint32_t>(b, 1));
+      R.row.val_off[k] = ro;
+      ro += b;
+    }
+    R.row.stride = (uint32_t)align_up(ro, 8);
+    // staged scatter tile: 24 KiB of digit tables + tile_rows rows in LDS
+    R.tile_rows = std::min<uint32_t>(4096, (budget - 24 * 1024) / R.row.stride / 256 * 256);
+    const uint32_t rem = L - leaf_bits;
+    const bool forced = rx_env && rx_env[0] == '1';
+    // automatic: at least one row per four keys of the domain on average, so
+    // leaves are well filled (sparser domains stay with the HBM hash table)
+    const bool fits = leaf_bits >= 8 && dom > 65536 && n_rows_scan >= (1u << 20) && dom / 4 <= n_rows_scan;
+    if ((forced || fits) && rem <= 18 && R.tile_rows >= 256) {
+      P.radix = true;
+      R.leaf_bits = leaf_bits;
+      R.bits1 = std::min<uint32_t>(8, rem);
+      R.bits2 = rem - R.bits1;
+      R.shift2 = leaf_bits;
+      R.shift1 = leaf_bits + R.bits2;
+      R.n_leaves = 1u << rem;
+      sp.lds_mode = LDS_DIRECT;  // one leaf's table
+      lds_cap = 1u << leaf_bits;
+      parts = 1;
+      // records carry their key (the hash layout): they are emitted
+      // compacted, leaf by leaf
+      if (dense) {
+        sp.g_key_off = 0;
+        sp.g_count_off += 8;
+        for (uint32_t k = 0; k < sp.n_aggs; k++) {
+          sp.aggs[k].g_off += 8;
+          if (sp.aggs[k].g_ns_off != ~0u) sp.aggs[k].g_ns_off += 8;
+        }
+        sp.g_rec += 8;
+        P.tmpl.insert(P.tmpl.begin(), kEmptyKey);
+      }
+      sp.g_mode = G_HASH;
+      sp.g_cap = std::max<uint64_t>(1, std::min<uint64_t>(dom, n_rows_scan));
+    }
+  }
   sp.lds_cap = lds_cap;
   sp.lds_parts = parts;
   sp.lds_key64 = key_bytes == 8;
@@ -657,6 +729,15 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
     P.block = 256;
     P.grid = (uint32_t)std::min<uint64_t>((uint64_t)n_cu * 8, std::max<uint32_t>(n_tiles, 1) * 4ull);
   }
+  if (P.radix) {
+    // partition passes: one 1024-thread workgroup per CU over contiguous tile
+    // shares; leaf aggregation: persistent 512-thread workgroups, as many
+    // per CU as their LDS tables allow
+    P.block = 1024;
+    P.grid = std::max<uint32_t>(1, std::min<uint32_t>(n_cu, n_tiles));
+    const uint32_t per_cu = std::max<uint32_t>(1, std::min<uint32_t>(4, (uint32_t)pl->ctx->max_lds / (sp.lds_bytes + 4096)));
+    P.agg_grid = std::min<uint32_t>(P.rx.n_leaves, n_cu * per_cu);
+  }
   // order words and output writers
   OrderParams& op = P.op;
   FinalParams& fp = P.fp;
@@ -687,6 +768,13 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
       case TDBG_AGG_AVG: add(ao.is_float ? C_AVG_F : C_AVG_I, 0, d.g_off, d.g_ns_off); break;
       default: add(ao.is_float ? C_FORD_AT : C_I64_AT, 0, d.g_off, d.g_ns_off); break;
     }
+  }
+  if (P.radix && op.n_comps < (uint32_t)kMaxComps) {
+    // leaves emit groups in completion order: the relative key as the last
+    // word gives ties (and an unordered result) the dense table's key order
+    CompDesc c{};
+    c.src = C_U64_AT; c.off = sp.g_key_off; c.ns_off = ~0u; c.desc = 0;
+    op.comps[op.n_comps++] = c;
   }
   // finalize writers; output buffer offsets are assigned per execution
   fp.n_outs = 0;
@@ -927,7 +1015,8 @@ void tdbg_plan_destroy(tdbg_plan* pl) {
   if (!pl) return;
   (void)hipSetDevice(pl->ctx->device);
   if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-  for (auto* b : {&pl->state, &pl->tiles, &pl->err, &pl->gids, &pl->counts, &pl->sel, &pl->scratch, &pl->out, &pl->tmpl, &pl->nout})
+  for (auto* b : {&pl->state, &pl->tiles, &pl->err, &pl->gids, &pl->counts, &pl->sel, &pl->scratch, &pl->out, &pl->tmpl, &pl->nout,
+                 &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf})
     release(pl, *b);
   for (auto& e : pl->ev) if (e) (void)hipEventDestroy(e);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
@@ -1092,23 +1181,57 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
 
   HIP_TRY(hipMemcpyAsync(pl->tiles.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, s), "tiles");
   HIP_TRY(hipMemcpyAsync(pl->tmpl.p, P.tmpl.data(), P.tmpl.size() * 8, hipMemcpyHostToDevice, s), "template");
-  HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
-  HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
-  HIP_TRY(launch_fill_u32(sp.err, 16, 0, s), "init flags");
-  HIP_TRY(hipEventRecord(pl->ev[1], s), "event");
   bool specialized = false;
-  if (launch_scan_jit(sp, P.grid, P.block, s, &specialized) != hipSuccess || !specialized) {
-    (void)hipGetLastError();
-    HIP_TRY(launch_scan_agg(sp, P.grid, P.block, s), "scan_agg launch");
-  }
-  HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
-  if (pl->cancel.load()) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
-  // compaction
   const uint64_t n_rec = sp.g_cap;
   if ((rc = ensure(pl, pl->gids, n_rec * 4 + 256))) return rc;
-  if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
-  HIP_TRY(launch_compact(P.op, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->gids.p),
-                         static_cast<uint64_t*>(pl->nout.p), s), "compact");
+  if (P.radix) {
+    // partition buffers: rows (twice when two levels), level-1 histogram,
+    // bucket starts and cursors, the emitted-group cursor, leaf ranges
+    RadixParams& R = P.rx;
+    const uint64_t nb1 = 1ull << R.bits1;
+    const uint64_t row_bytes = rows * R.row.stride + 256;
+    if ((rc = ensure(pl, pl->rx_rows1, row_bytes))) return rc;
+    if (R.bits2 && (rc = ensure(pl, pl->rx_rows2, row_bytes))) return rc;
+    const uint64_t misc_words = nb1 + (nb1 + 1) + nb1 + 1;
+    if ((rc = ensure(pl, pl->rx_misc, misc_words * 8 + 256))) return rc;
+    if (R.bits2 && (rc = ensure(pl, pl->rx_leaf, 2ull * R.n_leaves * 8 + 256))) return rc;
+    uint64_t* misc = static_cast<uint64_t*>(pl->rx_misc.p);
+    R.bucket_start = misc + nb1;
+    R.bucket_fill = R.bucket_start + nb1 + 1;
+    R.out_cursor = R.bucket_fill + nb1;
+    R.rows1 = static_cast<uint8_t*>(pl->rx_rows1.p);
+    R.rows2 = R.bits2 ? static_cast<uint8_t*>(pl->rx_rows2.p) : R.rows1;
+    R.leaf_start = R.bits2 ? static_cast<uint64_t*>(pl->rx_leaf.p) : nullptr;
+    auto* hist = reinterpret_cast<unsigned long long*>(misc);
+    HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(misc), misc_words * 2, 0, s), "init histogram");
+    HIP_TRY(launch_fill_u32(sp.err, 16, 0, s), "init flags");
+    HIP_TRY(hipEventRecord(pl->ev[1], s), "event");
+    HIP_TRY(launch_radix(sp, R, std::min<uint32_t>(2 * P.grid, sp.n_tiles), hist, s, RX_HIST), "radix histogram");
+    HIP_TRY(launch_radix(sp, R, 1, hist, s, RX_OFFSETS), "radix offsets");
+    HIP_TRY(launch_radix(sp, R, P.grid, hist, s, RX_SCATTER1), "radix scatter");
+    if (R.bits2) HIP_TRY(launch_radix(sp, R, 1, hist, s, RX_SCATTER2), "radix scatter 2");
+    HIP_TRY(launch_radix(sp, R, P.agg_grid, hist, s, RX_AGG), "radix aggregate");
+    HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
+    HIP_TRY(hipMemcpyAsync(pl->nout.p, R.out_cursor, 8, hipMemcpyDeviceToDevice, s), "group count");
+  } else {
+    HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
+    HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
+    HIP_TRY(launch_fill_u32(sp.err, 16, 0, s), "init flags");
+    HIP_TRY(hipEventRecord(pl->ev[1], s), "event");
+    if (launch_scan_jit(sp, P.grid, P.block, s, &specialized) != hipSuccess || !specialized) {
+      (void)hipGetLastError();
+      HIP_TRY(launch_scan_agg(sp, P.grid, P.block, s), "scan_agg launch");
+    }
+    HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
+  }
+  if (pl->cancel.load()) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
+  // compaction (radix records are already compact: identity positions)
+  if (!P.radix) {
+    if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
+    HIP_TRY(launch_compact(P.op, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->gids.p),
+                           static_cast<uint64_t*>(pl->nout.p), s), "compact");
+  }
   uint64_t n_groups = 0;
   uint32_t err_bits = 0;
   HIP_TRY(hipMemcpyAsync(&n_groups, pl->nout.p, 8, hipMemcpyDeviceToHost, s), "group count");
@@ -1117,12 +1240,17 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   if (err_bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
   if (err_bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
   if (pl->cancel.load()) return fail(TDBG_ERR_CANCELLED, "cancelled");
+  if (P.radix) {
+    if (n_groups > n_rec) return fail(TDBG_ERR_DEVICE, "radix pass emitted %llu groups for %llu records",
+                                      (unsigned long long)n_groups, (unsigned long long)n_rec);
+    HIP_TRY(launch_iota(static_cast<uint32_t*>(pl->gids.p), n_groups, s), "group positions");
+  }
   // ORDER BY / OFFSET / LIMIT
   const uint64_t start = std::min<uint64_t>(pl->offset, n_groups);
   const uint64_t end = std::min<uint64_t>(need_lim, n_groups);
   const uint64_t n_out = end - start;
   const uint32_t* sel = static_cast<const uint32_t*>(pl->gids.p) + start;
-  if (!pl->order.empty() && n_groups > 0 && end > 0) {
+  if (P.op.n_comps > 0 && n_groups > 0 && end > 0) {
     const size_t sb = sort_scratch_bytes(n_groups, P.op.n_comps);
     if ((rc = ensure(pl, pl->scratch, sb))) return rc;
     if ((rc = ensure(pl, pl->sel, end * 4 + 256))) return rc;
@@ -1161,7 +1289,7 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   pl->last.bytes_scanned = scanned_bytes;
   pl->last.n_groups = n_groups;
   pl->last.lds_mode = sp.lds_mode;
-  pl->last.global_mode = sp.g_mode;
+  pl->last.global_mode = P.radix ? 3u : sp.g_mode;
   pl->last.specialized = specialized ? 1u : 0u;
   pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
   return make_result(pl, n_out, host, data_off, valid_off, out);
