@@ -104,6 +104,7 @@ enum tdbg_pred_op {
 typedef struct tdbg_context tdbg_context;
 typedef struct tdbg_plan tdbg_plan;
 typedef struct tdbg_result tdbg_result;
+typedef struct tdbg_batch tdbg_batch;
 typedef struct tdbg_ticket tdbg_ticket;
 
 /* Identity of one decoded column block: src/storage/cache.zig:59-70 (`Key`). */
@@ -245,6 +246,50 @@ int32_t tdbg_plan_last_stats(const tdbg_plan* plan, tdbg_exec_stats* out);
 /* Asks a running or queued execution of `plan` to stop; it then returns
  * TDBG_ERR_CANCELLED.  Cleared when the next execution starts. */
 int32_t tdbg_cancel(tdbg_plan* plan);
+
+/* ---- batched execution --------------------------------------------------
+ * No counterpart in the reference, which runs each grouped query alone on
+ * its connection thread (exec.Query per request, src/exec/exec.zig:1221-1239;
+ * net/mysql/server.zig:191).  A batch runs many plans - each over every
+ * staged row group of its table - as one persistent device launch (captured
+ * once as a HIP graph and replayed), so small and uneven queries share the
+ * GPU instead of paying a launch-and-drain each.  Plans with ORDER BY,
+ * LIMIT or OFFSET are declined (TDBG_ERR_UNSUPPORTED): run them with
+ * tdbg_execute.  Every result is the one tdbg_execute returns for the plan. */
+typedef struct tdbg_batch_stats {
+  double total_ms;      /* device time of the last execution, result copy included */
+  double scan_ms;       /* the persistent scan/aggregate launch (0 when replayed as a graph) */
+  uint64_t rows;        /* rows scanned over all plans */
+  uint64_t bytes;       /* native-width bytes of every referenced column */
+  uint64_t result_bytes;/* device-to-host bytes of the results */
+  uint32_t n_plans;
+  uint32_t n_items;     /* work items of the persistent launch */
+  uint32_t graph;       /* 1: replayed as a captured HIP graph */
+  uint32_t pad;
+} tdbg_batch_stats;
+
+/* How the batch runs one plan (chosen per (rows, groups) bucket by the
+ * autotuner, else by the single-plan planner). */
+typedef struct tdbg_batch_route {
+  uint32_t strategy;    /* 1 LDS direct table, 2 HBM atomics, 3 LDS hash table, 4 radix partitions */
+  uint32_t lds_parts;   /* direct: workgroups splitting the key range */
+  uint32_t radix_bits;  /* radix: level-1 digit bits */
+  uint32_t items;       /* work items of this plan */
+} tdbg_batch_route;
+
+int32_t tdbg_batch_create(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n_plans, tdbg_batch** out);
+/* Measures the candidate strategies on the batch's own plans, bucket by
+ * bucket of (rows, estimated groups), `reps` timed runs each, keeps the
+ * fastest per bucket in the context (later batches reuse it) and re-plans. */
+int32_t tdbg_batch_autotune(tdbg_batch* batch, uint32_t reps);
+int32_t tdbg_batch_execute(tdbg_batch* batch);
+/* Result of plan i of the last execution.  Its columns are views into host
+ * memory the result keeps alive; they are not overwritten by later
+ * executions of the batch. */
+int32_t tdbg_batch_result(tdbg_batch* batch, uint32_t i, tdbg_result** out);
+int32_t tdbg_batch_last_stats(const tdbg_batch* batch, tdbg_batch_stats* out);
+int32_t tdbg_batch_plan_info(const tdbg_batch* batch, uint32_t i, tdbg_batch_route* out);
+void tdbg_batch_destroy(tdbg_batch* batch);
 
 uint64_t tdbg_result_num_rows(const tdbg_result* r);
 uint32_t tdbg_result_num_columns(const tdbg_result* r);

@@ -3,6 +3,7 @@ include/thindb_gpu.h declares, and the struct layouts the bindings use match
 the header.  No compute call is made (no GPU here)."""
 import ctypes as C
 import re
+import subprocess
 from pathlib import Path
 
 from thindb_amd import gpu
@@ -30,18 +31,33 @@ def test_abi_version_handshake():
     assert lib.tdbg_abi_version() == gpu.ABI_VERSION == 1
 
 
-def test_struct_layouts():
-    # sizes follow from the header's field order (natural alignment)
-    assert C.sizeof(gpu.ColumnKey) == 24
-    assert C.sizeof(gpu.BlockStats) == 40
-    assert C.sizeof(gpu.RgRef) == 16
-    assert C.sizeof(gpu.KeyPart) == 8
-    assert C.sizeof(gpu.AggSpecC) == 16
-    assert C.sizeof(gpu.PredicateC) == 32
-    assert C.sizeof(gpu.OrderSpecC) == 8
-    assert C.sizeof(gpu.PlanDesc) == 80
-    assert C.sizeof(gpu.ColumnView) == 40
-    assert C.sizeof(gpu.ExecStats) == 56
+STRUCTS = {  # header type -> ctypes binding
+    "tdbg_column_key": gpu.ColumnKey, "tdbg_block_stats": gpu.BlockStats, "tdbg_rg_ref": gpu.RgRef,
+    "tdbg_key_part": gpu.KeyPart, "tdbg_agg_spec": gpu.AggSpecC, "tdbg_predicate": gpu.PredicateC,
+    "tdbg_order_spec": gpu.OrderSpecC, "tdbg_plan_desc": gpu.PlanDesc, "tdbg_column_view": gpu.ColumnView,
+    "tdbg_exec_stats": gpu.ExecStats, "tdbg_batch_stats": gpu.BatchStats, "tdbg_batch_route": gpu.BatchRoute,
+}
+
+
+def test_struct_layouts(tmp_path):
+    """Every struct's size and field offsets as the C compiler lays out the
+    header equal the ctypes bindings'."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    for cname, cls in STRUCTS.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)
 
 
 def test_enums_match_header():

@@ -1,0 +1,559 @@
+// batch.cpp — host side of batched execution (batch.hip): many plans become
+// one queue of work items over one device layout, launched as a short fixed
+// kernel sequence (record init -> persistent scan/aggregate -> compaction ->
+// finalize) that is captured once as a HIP graph and replayed; results come
+// back in one device-to-host copy into pinned memory.
+//
+// The autotuner times the candidate strategies (LDS direct table, HBM
+// atomics, LDS hash table, radix partitions at two leaf sizes) on the batch's
+// own plans, per bucket of (rows, estimated groups) in powers of four, and
+// keeps the fastest per bucket in the context.
+#include <algorithm>
+#include <cstring>
+
+#include "runtime.h"
+
+using namespace tdbg;
+using namespace tdbg::rt;
+
+namespace {
+
+constexpr uint64_t kItemRows = 1ull << 17;     // rows per scan / partition work item
+constexpr uint32_t kSpinLimit = 1u << 21;      // polls (~seconds) before a dependency wait gives up
+constexpr uint32_t kBatchLds = 78u * 1024u;    // LDS per workgroup: two workgroups per CU
+constexpr uint32_t kTuneSample = 4;            // plans per bucket the autotuner times
+
+uint32_t ilog2(uint64_t x) { return x ? 63u - (uint32_t)__builtin_clzll(x) : 0u; }
+uint32_t bucket_of(uint64_t rows, uint64_t groups) { return (ilog2(rows) / 2) << 8 | (ilog2(groups) / 2); }
+
+struct DevBuf {
+  void* p = nullptr;
+  uint64_t bytes = 0;
+};
+
+// Estimated groups of a plan: the hint, else the key domain, at most the rows.
+uint64_t estimate_groups(const tdbg_plan* pl, Table* t, uint64_t rows) {
+  if (pl->est_groups) return std::max<uint64_t>(1, std::min<uint64_t>(pl->est_groups, rows));
+  unsigned __int128 dom = 1;
+  for (auto& k : pl->keys) {
+    auto it = t->cols.find(k.column_idx);
+    if (it == t->cols.end() || !it->second.has_stats) return std::max<uint64_t>(1, rows);
+    dom *= (unsigned __int128)((uint64_t)it->second.max_i - (uint64_t)it->second.min_i + 1);
+    if (dom > rows) return std::max<uint64_t>(1, rows);
+  }
+  return std::max<uint64_t>(1, (uint64_t)dom);
+}
+
+PlanHint batch_hint(PlanHint h) {
+  h.lds_budget = kBatchLds;
+  h.max_bits1 = 10;
+  h.single_level = true;
+  h.dense_out = true;
+  return h;
+}
+
+}  // namespace
+
+struct tdbg_batch {
+  tdbg_context* ctx = nullptr;
+  std::vector<tdbg_plan*> plans;
+  std::vector<PlanHint> forced;  // autotuner trial: this hint for every plan
+  bool use_graph = true;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::mutex mu;
+  // built state
+  bool built = false;
+  std::vector<std::pair<uint64_t, uint64_t>> versions;  // table uid and version the build saw
+  std::vector<tdbg_batch_route> info;
+  std::vector<std::vector<uint64_t>> data_off, valid_off;
+  uint64_t rows = 0, bytes = 0;
+  uint32_t n_items = 0, n_cblocks = 0, n_fblocks = 0, grid = 0, lds_bytes = 0;
+  uint64_t zero_words = 0, out_bytes = 0, err_off = 0, total_off = 0;
+  DevBuf d_q, d_items, d_tiles, d_rec, d_rows, d_misc, d_zero, d_cblocks, d_fblocks, d_counts, d_gids, d_out;
+  std::shared_ptr<uint8_t> h_out;  // pinned; results keep the buffer they view
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  tdbg_batch_stats last{};
+};
+
+namespace {
+
+int32_t dev_take(tdbg_batch* b, DevBuf& d, uint64_t bytes) {
+  dev_free(b->ctx, d.p, d.bytes);
+  d = DevBuf{};
+  const uint64_t want = std::max<uint64_t>(bytes, 256);
+  int32_t rc = dev_alloc(b->ctx, &d.p, want);
+  if (rc) return rc;
+  d.bytes = want;
+  return TDBG_OK;
+}
+
+void release_built(tdbg_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (DevBuf* d : {&b->d_q, &b->d_items, &b->d_tiles, &b->d_rec, &b->d_rows, &b->d_misc, &b->d_zero, &b->d_cblocks,
+                    &b->d_fblocks, &b->d_counts, &b->d_gids, &b->d_out}) {
+    dev_free(b->ctx, d->p, d->bytes);
+    *d = DevBuf{};
+  }
+  b->built = false;
+}
+
+template <typename T>
+T* at(const DevBuf& d, uint64_t off) {
+  return reinterpret_cast<T*>(static_cast<uint8_t*>(d.p) + off);
+}
+
+// The kernel sequence of one execution (captured into the graph).
+int32_t enqueue(tdbg_batch* b, bool timed) {
+  hipStream_t s = b->stream;
+  HIP_TRY(launch_fill_u32(static_cast<uint32_t*>(b->d_zero.p), b->zero_words, 0, s), "batch counters");
+  HIP_TRY(launch_fill_u32(at<uint32_t>(b->d_out, b->err_off), 16, 0, s), "batch error flags");
+  HIP_TRY(launch_batch_init(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchBlock*>(b->d_cblocks.p),
+                            b->n_cblocks, s), "batch record init");
+  if (timed) HIP_TRY(hipEventRecord(b->ev[1], s), "event");
+  HIP_TRY(launch_batch_scan(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchItem*>(b->d_items.p),
+                            b->n_items, static_cast<uint32_t*>(b->d_zero.p), at<uint32_t>(b->d_out, b->err_off),
+                            kSpinLimit, b->grid, b->lds_bytes, s), "batch scan");
+  if (timed) HIP_TRY(hipEventRecord(b->ev[2], s), "event");
+  uint32_t* counts = static_cast<uint32_t*>(b->d_counts.p);
+  uint64_t* total = at<uint64_t>(b->d_counts, b->total_off);
+  HIP_TRY(launch_batch_compact(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchBlock*>(b->d_cblocks.p),
+                               b->n_cblocks, counts, total, static_cast<uint32_t*>(b->d_gids.p), s), "batch compaction");
+  HIP_TRY(launch_batch_finalize(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchBlock*>(b->d_fblocks.p),
+                                b->n_fblocks, counts, total, b->n_cblocks, static_cast<const uint32_t*>(b->d_gids.p),
+                                static_cast<uint8_t*>(b->d_out.p), s), "batch finalize");
+  return TDBG_OK;
+}
+
+// Plans every member, lays out the device buffers and the work queue, and
+// captures the graph.  Caller holds b->mu.
+int32_t build(tdbg_batch* b) {
+  release_built(b);
+  tdbg_context* ctx = b->ctx;
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  std::shared_lock<std::shared_mutex> lk(ctx->mu);
+  const uint32_t n = (uint32_t)b->plans.size();
+  std::vector<BatchQuery> Q(n);
+  std::vector<ScanSet> sets(n);
+  std::vector<uint64_t> gmax(n);  // output rows upper bound per plan
+  b->versions.clear();
+  b->info.assign(n, tdbg_batch_route{});
+  b->rows = b->bytes = 0;
+  uint32_t lds_need = 4096;
+  for (uint32_t i = 0; i < n; i++) {
+    tdbg_plan* pl = b->plans[i];
+    auto tit = ctx->tables.find(pl->table_uid);
+    if (tit == ctx->tables.end())
+      return fail(TDBG_ERR_NOT_STAGED, "plan %u: table %llu has no staged columns", i, (unsigned long long)pl->table_uid);
+    Table* t = tit->second.get();
+    b->versions.emplace_back(pl->table_uid, t->version);
+    ScanSet& S = sets[i];
+    int32_t rc = collect_scan(pl, t, nullptr, 0, S);
+    if (rc) return rc;
+    PlanHint h;
+    // THINDB_GPU_BATCH_STRATEGY=direct|global|hash|radix forces one strategy
+    // for every plan (tests cover each path with it)
+    const char* senv = getenv("THINDB_GPU_BATCH_STRATEGY");
+    if (!b->forced.empty()) {
+      h = b->forced[0];
+    } else if (senv && senv[0]) {
+      const std::string v(senv);
+      h.strategy = v == "direct" ? S_DIRECT : v == "global" ? S_GLOBAL : v == "hash" ? S_LDS_HASH
+                 : v == "radix" ? S_RADIX : S_AUTO;
+    } else {
+      std::lock_guard<std::mutex> tk(ctx->tune_mu);
+      auto it = ctx->tuned.find(bucket_of(S.rows, estimate_groups(pl, t, S.rows)));
+      if (it != ctx->tuned.end()) h = it->second;
+    }
+    h = batch_hint(h);
+    if (S.rows == 0) h.strategy = S_GLOBAL;
+    std::lock_guard<std::mutex> plk(pl->mu);
+    if ((rc = prepare(pl, t, S, h))) return rc;
+    if (pl->prep.radix && !pl->prep.rx.dense_out) {
+      // a sparse key domain: compacted leaf output is not supported in a batch
+      h.strategy = S_GLOBAL;
+      if ((rc = prepare(pl, t, S, h))) return rc;
+    }
+    const Prepared& P = pl->prep;
+    if (P.tmpl.size() > (size_t)kTmplWords) return fail(TDBG_ERR_UNSUPPORTED, "plan %u: record wider than a batch allows", i);
+    BatchQuery& q = Q[i];
+    memset(&q, 0, sizeof q);
+    q.sp = P.sp;
+    q.rx = P.rx;
+    q.op = P.op;
+    q.fp = P.fp;
+    for (size_t w = 0; w < P.tmpl.size(); w++) q.tmpl[w] = P.tmpl[w];
+    q.init_records = P.radix ? 0u : 1u;
+    if (P.radix) q.rx.tile_rows = std::min<uint32_t>(q.rx.tile_rows, kBatchThreads * kVec);
+    gmax[i] = std::max<uint64_t>(1, std::min<uint64_t>(q.sp.g_cap, S.rows));
+    tdbg_batch_route& inf = b->info[i];
+    inf.strategy = P.radix ? 4u : (q.sp.lds_mode == LDS_DIRECT ? 1u : (q.sp.lds_mode == LDS_HASH ? 3u : 2u));
+    inf.lds_parts = q.sp.lds_mode == LDS_DIRECT ? q.sp.lds_parts : 1u;
+    inf.radix_bits = P.radix ? q.rx.bits1 : 0u;
+    if (q.sp.lds_mode != LDS_NONE) lds_need = std::max(lds_need, q.sp.lds_bytes);
+    if (P.radix) lds_need = std::max<uint32_t>(lds_need, kRxStageOff + q.rx.tile_rows * q.rx.row.stride);
+    b->rows += S.rows;
+    b->bytes += S.bytes;
+  }
+  if (lds_need > kBatchLds) return fail(TDBG_ERR_DEVICE, "batch LDS layout of %u bytes", lds_need);
+  b->lds_bytes = lds_need;
+  // device layout
+  uint64_t rec_b = 0, tile_b = 0, rows_b = 0, misc_b = 0;
+  std::vector<uint64_t> rec_o(n), tile_o(n), rows_o(n), misc_o(n);
+  for (uint32_t i = 0; i < n; i++) {
+    const BatchQuery& q = Q[i];
+    rec_o[i] = rec_b;
+    rec_b += align_up(q.sp.g_cap * q.sp.g_rec, 256);
+    tile_o[i] = tile_b;
+    tile_b += align_up(sets[i].tiles.size() * sizeof(Tile), 256);
+    if (b->info[i].strategy == 4u) {
+      rows_o[i] = rows_b;
+      rows_b += align_up(sets[i].rows * q.rx.row.stride + 256, 256);
+      misc_o[i] = misc_b;
+      misc_b += align_up(((2ull << q.rx.bits1) + 2) * 8, 256);
+    }
+  }
+  // zeroed counters: queue head, per-plan sync words, radix histograms
+  uint64_t zero_b = 64 + align_up(16ull * n, 256);
+  std::vector<uint64_t> hist_o(n, 0);
+  for (uint32_t i = 0; i < n; i++)
+    if (b->info[i].strategy == 4u) {
+      hist_o[i] = zero_b;
+      zero_b += align_up((8ull << Q[i].rx.bits1), 256);
+    }
+  // output buffer: group counts, error flags, then each plan's columns
+  b->err_off = align_up(8ull * n, 64);
+  uint64_t out_b = align_up(b->err_off + 64, 256);
+  b->data_off.assign(n, {});
+  b->valid_off.assign(n, {});
+  for (uint32_t i = 0; i < n; i++) {
+    FinalParams& fp = Q[i].fp;
+    out_b = align_up(out_b, 256);
+    b->data_off[i].assign(fp.n_outs, 0);
+    b->valid_off[i].assign(fp.n_outs, ~0ull);
+    for (uint32_t c = 0; c < fp.n_outs; c++) {
+      OutDesc& o = fp.outs[c];
+      out_b = align_up(out_b, 16);
+      o.data_off = out_b;
+      b->data_off[i][c] = out_b;
+      out_b += gmax[i] * o.bytes;
+      if (o.valid_off != ~0ull) {
+        out_b = align_up(out_b, 16);
+        o.valid_off = out_b;
+        b->valid_off[i][c] = out_b;
+        out_b += gmax[i];
+      }
+    }
+  }
+  b->out_bytes = out_b;
+  // compaction and finalize blocks, work items
+  std::vector<BatchBlock> cblocks, fblocks;
+  std::vector<BatchItem> hist_items, scan_items, scatter_items, agg_items;
+  for (uint32_t i = 0; i < n; i++) {
+    BatchQuery& q = Q[i];
+    q.blk0 = (uint32_t)cblocks.size();
+    for (uint64_t r = 0; r < q.sp.g_cap; r += kBatchChunk) cblocks.push_back(BatchBlock{i, 0, r});
+    q.blk1 = (uint32_t)cblocks.size();
+    for (uint64_t r = 0; r < gmax[i]; r += kBatchChunk) fblocks.push_back(BatchBlock{i, 0, r});
+    const uint32_t n_tiles = (uint32_t)sets[i].tiles.size();
+    const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, kItemRows / kTileRows);
+    const uint32_t before = (uint32_t)(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
+    if (b->info[i].strategy == 4u) {
+      for (uint32_t t0 = 0; t0 < n_tiles; t0 += chunk) {
+        const uint32_t t1 = std::min(n_tiles, t0 + chunk);
+        hist_items.push_back(BatchItem{BI_RX_HIST, i, t0, t1, 0, 0});
+        scatter_items.push_back(BatchItem{BI_RX_SCATTER, i, t0, t1, 0, 0});
+      }
+      const uint64_t per = std::max<uint64_t>(1, (uint64_t)q.rx.n_leaves * kItemRows / std::max<uint64_t>(sets[i].rows, 1));
+      for (uint64_t l0 = 0; l0 < q.rx.n_leaves; l0 += per)
+        agg_items.push_back(BatchItem{BI_RX_AGG, i, (uint32_t)l0, (uint32_t)std::min<uint64_t>(q.rx.n_leaves, l0 + per), 0, 0});
+    } else {
+      const uint32_t parts = q.sp.lds_mode == LDS_DIRECT ? q.sp.lds_parts : 1u;
+      for (uint32_t t0 = 0; t0 < n_tiles; t0 += chunk)
+        for (uint32_t p = 0; p < parts; p++)
+          scan_items.push_back(BatchItem{BI_SCAN, i, t0, std::min(n_tiles, t0 + chunk), p, 0});
+    }
+    const uint32_t after = (uint32_t)(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
+    b->info[i].items = after - before;
+  }
+  // per-plan radix item counts
+  for (uint32_t i = 0; i < n; i++) {
+    Q[i].n_hist = 0;
+    Q[i].n_scatter = 0;
+  }
+  for (auto& it : hist_items) Q[it.q].n_hist++;
+  for (auto& it : scatter_items) Q[it.q].n_scatter++;
+  // the largest scans first, so the tail of the queue is short items
+  std::stable_sort(scan_items.begin(), scan_items.end(), [](const BatchItem& x, const BatchItem& y) {
+    return (x.a1 - x.a0) > (y.a1 - y.a0);
+  });
+  std::vector<BatchItem> items;
+  items.reserve(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
+  for (auto* v : {&hist_items, &scan_items, &scatter_items, &agg_items}) items.insert(items.end(), v->begin(), v->end());
+  b->n_items = (uint32_t)items.size();
+  b->n_cblocks = (uint32_t)cblocks.size();
+  b->n_fblocks = (uint32_t)fblocks.size();
+  const uint32_t n_cu = (uint32_t)std::max(ctx->n_cu, 8);
+  b->grid = std::max<uint32_t>(1, std::min<uint32_t>(n_cu * 2, b->n_items));
+  // allocations
+  int32_t rc;
+  uint64_t gids_b = 0;
+  for (uint32_t i = 0; i < n; i++) gids_b += gmax[i] * 4;
+  b->total_off = align_up((uint64_t)b->n_cblocks * 4, 8);
+  if ((rc = dev_take(b, b->d_q, sizeof(BatchQuery) * n)) || (rc = dev_take(b, b->d_items, sizeof(BatchItem) * items.size())) ||
+      (rc = dev_take(b, b->d_tiles, tile_b)) || (rc = dev_take(b, b->d_rec, rec_b)) || (rc = dev_take(b, b->d_rows, rows_b)) ||
+      (rc = dev_take(b, b->d_misc, misc_b)) || (rc = dev_take(b, b->d_zero, zero_b)) ||
+      (rc = dev_take(b, b->d_cblocks, sizeof(BatchBlock) * cblocks.size())) ||
+      (rc = dev_take(b, b->d_fblocks, sizeof(BatchBlock) * fblocks.size())) ||
+      (rc = dev_take(b, b->d_counts, b->total_off + 64)) || (rc = dev_take(b, b->d_gids, gids_b + 256)) ||
+      (rc = dev_take(b, b->d_out, b->out_bytes))) {
+    release_built(b);
+    return rc;
+  }
+  b->zero_words = zero_b / 4;
+  // device pointers into the layout
+  for (uint32_t i = 0; i < n; i++) {
+    BatchQuery& q = Q[i];
+    q.sp.g_state = at<uint8_t>(b->d_rec, rec_o[i]);
+    q.sp.tiles = at<const Tile>(b->d_tiles, tile_o[i]);
+    q.sp.n_tiles = (uint32_t)sets[i].tiles.size();
+    q.sp.err = at<uint32_t>(b->d_out, b->err_off);
+    q.op.g_state = q.sp.g_state;
+    q.fp.g_state = q.sp.g_state;
+    q.sync = at<uint32_t>(b->d_zero, 64 + 16ull * i);
+    q.n_out = at<uint64_t>(b->d_out, 8ull * i);
+    if (b->info[i].strategy == 4u) {
+      const uint64_t nb = 1ull << q.rx.bits1;
+      q.hist = at<unsigned long long>(b->d_zero, hist_o[i]);
+      q.rx.rows1 = at<uint8_t>(b->d_rows, rows_o[i]);
+      q.rx.rows2 = q.rx.rows1;
+      q.rx.bucket_start = at<uint64_t>(b->d_misc, misc_o[i]);
+      q.rx.bucket_fill = q.rx.bucket_start + nb + 1;
+      q.rx.leaf_start = nullptr;
+      q.rx.out_cursor = q.rx.bucket_fill + nb;
+    }
+  }
+  for (uint32_t i = 0; i < n; i++)
+    if (!sets[i].tiles.empty())
+      HIP_TRY(hipMemcpy(at<void>(b->d_tiles, tile_o[i]), sets[i].tiles.data(), sets[i].tiles.size() * sizeof(Tile),
+                        hipMemcpyHostToDevice), "batch tiles");
+  HIP_TRY(hipMemcpy(b->d_q.p, Q.data(), sizeof(BatchQuery) * n, hipMemcpyHostToDevice), "batch plans");
+  if (!items.empty())
+    HIP_TRY(hipMemcpy(b->d_items.p, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice), "batch items");
+  HIP_TRY(hipMemcpy(b->d_cblocks.p, cblocks.data(), sizeof(BatchBlock) * cblocks.size(), hipMemcpyHostToDevice), "blocks");
+  HIP_TRY(hipMemcpy(b->d_fblocks.p, fblocks.data(), sizeof(BatchBlock) * fblocks.size(), hipMemcpyHostToDevice), "blocks");
+  b->built = true;
+  // kernel attributes are set outside any capture
+  HIP_TRY(launch_batch_scan(nullptr, nullptr, 0, nullptr, nullptr, kSpinLimit, 1, 0, b->stream), "batch kernel setup");
+  const char* genv = getenv("THINDB_GPU_GRAPH");
+  if (b->use_graph && !(genv && genv[0] == '0')) {
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+    rc = enqueue(b, false);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    HIP_TRY(ce, "graph capture");
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+  }
+  return TDBG_OK;
+}
+
+// A member table was restaged, evicted or dropped since the build.  Caller
+// holds ctx->mu.
+bool stale(const tdbg_batch* b) {
+  for (auto& v : b->versions) {
+    auto it = b->ctx->tables.find(v.first);
+    if (it == b->ctx->tables.end() || it->second->version != v.second) return true;
+  }
+  return false;
+}
+
+// One execution into the pinned result buffer.  Caller holds b->mu.
+int32_t run(tdbg_batch* b) {
+  int32_t rc;
+  {
+    std::shared_lock<std::shared_mutex> lk(b->ctx->mu);
+    if (b->built && stale(b)) b->built = false;
+  }
+  if (!b->built && (rc = build(b))) return rc;
+  HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
+  if (!b->h_out || b->h_out.use_count() > 1) {
+    // results of an earlier execution still view the old buffer: keep it
+    void* h = nullptr;
+    HIP_TRY(hipHostMalloc(&h, b->out_bytes, hipHostMallocDefault), "pinned result buffer");
+    b->h_out.reset(static_cast<uint8_t*>(h), [](uint8_t* p) { (void)hipHostFree(p); });
+  }
+  hipStream_t s = b->stream;
+  std::shared_lock<std::shared_mutex> lk(b->ctx->mu);  // staged columns stay put while the kernels read them
+  if (stale(b)) return fail(TDBG_ERR_NOT_STAGED, "batch: a member table changed while the batch was planned");
+  HIP_TRY(hipEventRecord(b->ev[0], s), "event");
+  const bool graph = b->exec != nullptr;
+  if (graph) HIP_TRY(hipGraphLaunch(b->exec, s), "graph launch");
+  else if ((rc = enqueue(b, true))) return rc;
+  HIP_TRY(hipMemcpyAsync(b->h_out.get(), b->d_out.p, b->out_bytes, hipMemcpyDeviceToHost, s), "result copy");
+  HIP_TRY(hipEventRecord(b->ev[3], s), "event");
+  HIP_TRY(hipStreamSynchronize(s), "batch");
+  const uint32_t* err = reinterpret_cast<const uint32_t*>(b->h_out.get() + b->err_off);
+  uint32_t bits = 0;
+  for (int i = 0; i < 16; i++) bits |= err[i];
+  if (bits & ERR_BATCH_WAIT) return fail(TDBG_ERR_DEVICE, "batch: a dependency wait gave up");
+  if (bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "batch: a key value lies outside the staged zone-map statistics");
+  if (bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "batch: a group table is full (est_groups too small)");
+  float total = 0, scan = 0;
+  hipEventElapsedTime(&total, b->ev[0], b->ev[3]);
+  if (!graph) hipEventElapsedTime(&scan, b->ev[1], b->ev[2]);
+  b->last = tdbg_batch_stats{};
+  b->last.total_ms = total;
+  b->last.scan_ms = scan;
+  b->last.rows = b->rows;
+  b->last.bytes = b->bytes;
+  b->last.result_bytes = b->out_bytes;
+  b->last.n_plans = (uint32_t)b->plans.size();
+  b->last.n_items = b->n_items;
+  b->last.graph = graph ? 1u : 0u;
+  return TDBG_OK;
+}
+
+int32_t make_batch(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n, hipStream_t stream, tdbg_batch** out) {
+  std::unique_ptr<tdbg_batch> b(new tdbg_batch());
+  b->ctx = ctx;
+  b->plans.assign(plans, plans + n);
+  HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  if (stream) {
+    b->stream = stream;
+  } else {
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+    b->own_stream = true;
+  }
+  for (auto& e : b->ev) HIP_TRY(hipEventCreate(&e), "event");
+  *out = b.release();
+  return TDBG_OK;
+}
+
+void free_batch(tdbg_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  release_built(b);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->own_stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t tdbg_batch_create(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n_plans, tdbg_batch** out) {
+  if (!ctx || !out || (n_plans && !plans)) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (n_plans == 0) return fail(TDBG_ERR_INVALID, "empty batch");
+  for (uint32_t i = 0; i < n_plans; i++) {
+    const tdbg_plan* pl = plans[i];
+    if (!pl || pl->ctx != ctx) return fail(TDBG_ERR_INVALID, "plan %u is null or of another context", i);
+    if (!pl->order.empty() || pl->limit || pl->offset)
+      return fail(TDBG_ERR_UNSUPPORTED, "plan %u has ORDER BY / LIMIT / OFFSET: run it alone", i);
+  }
+  return make_batch(ctx, plans, n_plans, nullptr, out);
+}
+
+int32_t tdbg_batch_execute(tdbg_batch* b) {
+  if (!b) return fail(TDBG_ERR_INVALID, "null batch");
+  std::lock_guard<std::mutex> lk(b->mu);
+  return run(b);
+}
+
+int32_t tdbg_batch_autotune(tdbg_batch* b, uint32_t reps) {
+  if (!b) return fail(TDBG_ERR_INVALID, "null batch");
+  std::lock_guard<std::mutex> lk(b->mu);
+  tdbg_context* ctx = b->ctx;
+  reps = std::max<uint32_t>(1, reps);
+  // the batch's buckets
+  std::map<uint32_t, std::vector<tdbg_plan*>> buckets;
+  {
+    std::shared_lock<std::shared_mutex> clk(ctx->mu);
+    for (tdbg_plan* pl : b->plans) {
+      auto tit = ctx->tables.find(pl->table_uid);
+      if (tit == ctx->tables.end()) return fail(TDBG_ERR_NOT_STAGED, "table %llu has no staged columns",
+                                                (unsigned long long)pl->table_uid);
+      ScanSet S;
+      int32_t rc = collect_scan(pl, tit->second.get(), nullptr, 0, S);
+      if (rc) return rc;
+      buckets[bucket_of(S.rows, estimate_groups(pl, tit->second.get(), S.rows))].push_back(pl);
+    }
+  }
+  const PlanHint cands[] = {
+      PlanHint{S_DIRECT}, PlanHint{S_GLOBAL}, PlanHint{S_LDS_HASH}, PlanHint{S_RADIX, 0}, PlanHint{S_RADIX, 10},
+  };
+  for (auto& kv : buckets) {
+    {
+      std::lock_guard<std::mutex> tk(ctx->tune_mu);
+      if (ctx->tuned.count(kv.first)) continue;
+    }
+    std::vector<tdbg_plan*> sample(kv.second.begin(),
+                                   kv.second.begin() + std::min<size_t>(kv.second.size(), kTuneSample));
+    double best = 1e30;
+    PlanHint pick{};
+    for (const PlanHint& c : cands) {
+      tdbg_batch* sub = nullptr;
+      if (make_batch(ctx, sample.data(), (uint32_t)sample.size(), b->stream, &sub)) continue;
+      sub->forced = {c};
+      sub->use_graph = false;
+      double t = 1e30;
+      bool ok = true;
+      for (uint32_t r = 0; r <= reps && ok; r++) {
+        if (run(sub)) {
+          ok = false;
+          break;
+        }
+        if (r > 0) t = std::min(t, sub->last.total_ms);  // the first run warms up
+      }
+      free_batch(sub);
+      if (ok && t < best) {
+        best = t;
+        pick = c;
+      }
+    }
+    std::lock_guard<std::mutex> tk(ctx->tune_mu);
+    ctx->tuned[kv.first] = pick;
+  }
+  release_built(b);  // re-plans with the tuned strategies at the next execution
+  return TDBG_OK;
+}
+
+int32_t tdbg_batch_result(tdbg_batch* b, uint32_t i, tdbg_result** out) {
+  if (!b || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (!b->built || !b->h_out || i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
+  const uint64_t n_out = reinterpret_cast<const uint64_t*>(b->h_out.get())[i];
+  std::shared_ptr<void> keep(b->h_out, b->h_out.get());
+  return make_result(b->plans[i], n_out, b->h_out.get(), b->data_off[i], b->valid_off[i], keep, out);
+}
+
+int32_t tdbg_batch_last_stats(const tdbg_batch* b, tdbg_batch_stats* out) {
+  if (!b || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = b->last;
+  return TDBG_OK;
+}
+
+int32_t tdbg_batch_plan_info(const tdbg_batch* b, uint32_t i, tdbg_batch_route* out) {
+  if (!b || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  if (!b->built || i >= b->info.size()) return fail(TDBG_ERR_INVALID, "plan %u not planned", i);
+  *out = b->info[i];
+  return TDBG_OK;
+}
+
+void tdbg_batch_destroy(tdbg_batch* b) { free_batch(b); }
+
+}  // extern "C"

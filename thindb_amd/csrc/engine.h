@@ -145,7 +145,8 @@ struct RadixParams {
   uint32_t leaf_bits;          // a leaf holds 2^leaf_bits consecutive keys
   uint32_t n_leaves;
   uint32_t tile_rows;          // rows a scatter workgroup stages in LDS at once (multiple of 256)
-  uint32_t pad;
+  uint32_t dense_out;          // leaves write every key of their range at record index key
+                               // (G_DENSE table) instead of compacted keyed records
   uint8_t* rows1;              // level-1 partitioned rows
   uint8_t* rows2;              // level-2 partitioned rows (== rows1 when one level)
   uint64_t* bucket_start;      // [2^bits1 + 1] level-1 bucket starts (from the histogram)
@@ -222,6 +223,51 @@ struct FinalParams {
   uint32_t g_rec, g_count_off, g_key_off, g_mode;
 };
 
+// ---- batched execution (one persistent launch over many plans) ------------
+
+// A work item of the persistent kernel: tiles [a0, a1) of plan q for a scan
+// or a radix partition pass, or leaves [a0, a1) for a radix leaf pass.
+enum BatchItemKind : uint32_t { BI_SCAN = 0, BI_RX_HIST, BI_RX_SCATTER, BI_RX_AGG };
+struct BatchItem {
+  uint32_t kind;
+  uint32_t q;
+  uint32_t a0, a1;
+  uint32_t part;   // BI_SCAN over a partitioned direct table: the key range owned
+  uint32_t pad;
+};
+
+constexpr int kTmplWords = 64;  // record template words (records up to 512 B)
+struct BatchQuery {
+  ScanParams sp;
+  RadixParams rx;
+  OrderParams op;
+  FinalParams fp;                // data/valid offsets are absolute in the output buffer
+  uint64_t tmpl[kTmplWords];     // one initialised record
+  uint32_t blk0, blk1;           // compaction blocks [blk0, blk1) over this plan's records
+  uint32_t n_hist, n_scatter;    // radix partition items of this plan
+  uint32_t init_records;         // records start from tmpl (else the leaf pass writes them all)
+  uint32_t pad;
+  uint32_t* sync;                // [0] histogram items done, [1] bucket starts published,
+                                 // [2] scatter items done
+  unsigned long long* hist;      // level-1 histogram (radix)
+  uint64_t* n_out;               // groups emitted (output buffer header)
+};
+
+// kBatchChunk consecutive records (compaction, record init) or output rows
+// (finalize) of plan q starting at r0.
+constexpr uint32_t kBatchChunk = 4096;
+struct BatchBlock {
+  uint32_t q;
+  uint32_t pad;
+  uint64_t r0;
+};
+
+constexpr uint32_t kBatchThreads = 512;
+// LDS of the radix scatter: digit tables (counts, offsets, cursors, scan
+// temporaries, bucket bases for up to 1024 digits) then the staged rows.
+constexpr uint32_t kRxStageOff = 24 * 1024;
+enum BatchErr : uint32_t { ERR_BATCH_WAIT = 4u };  // a dependency wait gave up (ErrBit space)
+
 #if !defined(__HIPCC_RTC__)
 // ---- kernel launchers (defined in the .hip files) -------------------------
 hipError_t launch_fill_records(uint8_t* state, uint64_t n_rec, uint32_t rec_bytes, const uint64_t* tmpl,
@@ -243,6 +289,17 @@ hipError_t launch_iota(uint32_t* p, uint64_t n, hipStream_t s);
 enum RadixPhase : int { RX_HIST = 0, RX_OFFSETS, RX_SCATTER1, RX_SCATTER2, RX_AGG };
 hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid, unsigned long long* hist,
                         hipStream_t s, int phase);
+// Batched execution (batch.hip): record init over `blocks`, the persistent
+// kernel over `items` (grid workgroups of kBatchThreads, lds_bytes dynamic
+// LDS each; *head must start at 0), compaction and finalize.
+hipError_t launch_batch_init(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks, hipStream_t s);
+hipError_t launch_batch_scan(const BatchQuery* q, const BatchItem* items, uint32_t n_items, uint32_t* head,
+                             uint32_t* err, uint32_t spin_limit, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
+hipError_t launch_batch_compact(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks, uint32_t* counts,
+                                uint64_t* total, uint32_t* gids, hipStream_t s);
+hipError_t launch_batch_finalize(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks,
+                                 const uint32_t* offs, const uint64_t* total, uint32_t n_blocks_compact,
+                                 const uint32_t* gids, uint8_t* out, hipStream_t s);
 #endif
 
 }  // namespace tdbg

@@ -769,10 +769,9 @@ __device__ __forceinline__ void put(uint8_t* out, uint64_t off, uint32_t bytes, 
   }
 }
 
-__global__ void finalize_kernel(const FinalParams P, const uint32_t* __restrict__ sel, uint64_t n_out,
-                                uint8_t* out) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_out; i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t gid = sel[i];
+// Output row i from record gid: every output column at its offset in `out`.
+__device__ __forceinline__ void finalize_row(const FinalParams& P, uint32_t gid, uint64_t i, uint8_t* out) {
+  {
     const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
     const uint64_t count = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off);
     for (uint32_t c = 0; c < P.n_outs; c++) {
@@ -836,9 +835,103 @@ __global__ void finalize_kernel(const FinalParams P, const uint32_t* __restrict_
   }
 }
 
+__global__ void finalize_kernel(const FinalParams P, const uint32_t* __restrict__ sel, uint64_t n_out,
+                                uint8_t* out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_out; i += (uint64_t)gridDim.x * blockDim.x)
+    finalize_row(P, sel[i], i, out);
+}
+
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out, hipStream_t s) {
   if (n_out == 0) return hipSuccess;
   hipLaunchKernelGGL(finalize_kernel, dim3(grid_for(n_out, 256, 8192)), dim3(256), 0, s, p, sel, n_out, out);
+  return hipGetLastError();
+}
+
+// ---- batched compaction and finalize ---------------------------------------------
+// Blocks of kBatchChunk (= kCompactChunk) records or output rows of one plan
+// each; a plan's compacted positions are contiguous in block order.
+
+__global__ void __launch_bounds__(kCompactThreads) batch_compact_count_kernel(const BatchQuery* __restrict__ Q,
+                                                                              const BatchBlock* __restrict__ blocks,
+                                                                              uint32_t* block_counts) {
+  __shared__ uint32_t red[kCompactThreads / 64];
+  const BatchBlock B = blocks[blockIdx.x];
+  const OrderParams& P = Q[B.q].op;
+  const uint64_t n_rec = Q[B.q].sp.g_cap;
+  const uint64_t base = B.r0 + (uint64_t)threadIdx.x * kCompactPer;
+  uint32_t c = 0;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if (base + j < n_rec && occupied(P, base + j)) c++;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kCompactThreads / 64; w++) t += red[w];
+    block_counts[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kCompactThreads) batch_compact_write_kernel(const BatchQuery* __restrict__ Q,
+                                                                              const BatchBlock* __restrict__ blocks,
+                                                                              const uint32_t* block_offs,
+                                                                              uint32_t* gids) {
+  __shared__ uint32_t pre[kCompactThreads];
+  const BatchBlock B = blocks[blockIdx.x];
+  const OrderParams& P = Q[B.q].op;
+  const uint64_t n_rec = Q[B.q].sp.g_cap;
+  const uint64_t base = B.r0 + (uint64_t)threadIdx.x * kCompactPer;
+  uint32_t bits = 0, c = 0;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if (base + j < n_rec && occupied(P, base + j)) { bits |= 1u << j; c++; }
+  pre[threadIdx.x] = c;
+  __syncthreads();
+  for (uint32_t o = 1; o < kCompactThreads; o <<= 1) {
+    uint32_t v = threadIdx.x >= o ? pre[threadIdx.x - o] : 0;
+    __syncthreads();
+    pre[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t out = block_offs[blockIdx.x] + pre[threadIdx.x] - c;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if ((bits >> j) & 1u) gids[out++] = (uint32_t)(base + j);  // record index within the plan
+}
+
+hipError_t launch_batch_compact(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks, uint32_t* counts,
+                                uint64_t* total, uint32_t* gids, hipStream_t s) {
+  static_assert(kBatchChunk == kCompactChunk, "one compaction block per batch block");
+  if (n_blocks == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_compact_count_kernel, dim3(n_blocks), dim3(kCompactThreads), 0, s, q, blocks, counts);
+  hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, counts, n_blocks, total);
+  hipLaunchKernelGGL(batch_compact_write_kernel, dim3(n_blocks), dim3(kCompactThreads), 0, s, q, blocks,
+                     (const uint32_t*)counts, gids);
+  return hipGetLastError();
+}
+
+// Output rows [r0, r0 + kBatchChunk) of one plan; the block holding row 0
+// publishes the plan's group count.
+__global__ void __launch_bounds__(256) batch_finalize_kernel(const BatchQuery* __restrict__ Q,
+                                                             const BatchBlock* __restrict__ blocks,
+                                                             const uint32_t* __restrict__ offs,
+                                                             const uint64_t* __restrict__ total,
+                                                             uint32_t n_cblocks, const uint32_t* __restrict__ gids,
+                                                             uint8_t* out) {
+  const BatchBlock B = blocks[blockIdx.x];
+  const BatchQuery& q = Q[B.q];
+  const uint64_t start = offs[q.blk0];
+  const uint64_t end = q.blk1 < n_cblocks ? offs[q.blk1] : *total;
+  const uint64_t n = end - start;
+  if (B.r0 == 0 && threadIdx.x == 0) *q.n_out = n;
+  const uint64_t hi = B.r0 + kBatchChunk < n ? B.r0 + kBatchChunk : n;
+  for (uint64_t i = B.r0 + threadIdx.x; i < hi; i += blockDim.x) finalize_row(q.fp, gids[start + i], i, out);
+}
+
+hipError_t launch_batch_finalize(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks,
+                                 const uint32_t* offs, const uint64_t* total, uint32_t n_blocks_compact,
+                                 const uint32_t* gids, uint8_t* out, hipStream_t s) {
+  if (n_blocks == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_finalize_kernel, dim3(n_blocks), dim3(256), 0, s, q, blocks, offs, total, n_blocks_compact,
+                     gids, out);
   return hipGetLastError();
 }
 

@@ -15,31 +15,23 @@
 //    LDS table shape, exact-sum mode) and run as one stream of kernels:
 //    record init -> fused scan/filter/aggregate -> compaction -> ORDER BY
 //    selection -> finalize -> one device-to-host copy.
-#include <hip/hip_runtime.h>
+#include "runtime.h"
 
 #include <algorithm>
-#include <atomic>
 #include <chrono>
-#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <shared_mutex>
-#include <string>
-#include <unordered_map>
-#include <vector>
-
-#include "../../include/thindb_gpu.h"
-#include "engine.h"
 
 using namespace tdbg;
+using namespace tdbg::rt;
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+namespace tdbg {
+namespace rt {
 
 int32_t fail(int32_t code, const char* fmt, ...) {
   char buf[512];
@@ -56,12 +48,6 @@ int32_t hip_fail(hipError_t e, const char* what) {
   return fail(TDBG_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr, what)                   \
-  do {                                        \
-    hipError_t e_ = (expr);                   \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
-
 int type_bytes(uint32_t t) {
   switch (t) {
     case TDBG_BOOLEAN: case TDBG_TINYINT: return 1;
@@ -72,8 +58,6 @@ int type_bytes(uint32_t t) {
     default: return 0;
   }
 }
-bool is_int_type(uint32_t t) { return t >= TDBG_BOOLEAN && t <= TDBG_DECIMAL64; }
-bool is_float_type(uint32_t t) { return t == TDBG_FLOAT || t == TDBG_DOUBLE; }
 uint32_t kind_of(uint32_t t) {
   switch (t) {
     case TDBG_BOOLEAN: return K_U8;
@@ -85,64 +69,6 @@ uint32_t kind_of(uint32_t t) {
     default: return K_I64;
   }
 }
-
-uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
-uint64_t next_pow2(uint64_t x) {
-  uint64_t p = 1;
-  while (p < x) p <<= 1;
-  return p;
-}
-
-struct ColumnArena {
-  uint32_t type = 0;
-  uint32_t elem = 0;
-  uint8_t* d_data = nullptr;
-  uint32_t* d_valid = nullptr;  // allocated once any block is nullable
-  bool has_stats = false;
-  int64_t min_i = 0, max_i = 0;
-  double min_f = 0, max_f = 0;
-};
-
-struct BlockInfo {
-  bool staged = false;
-  bool has_stats = false;
-  tdbg_block_stats stats{};
-};
-
-struct RowGroupEntry {
-  uint64_t segment_id = 0;
-  uint32_t rg_idx = 0;
-  uint32_t n_rows = 0;
-  uint64_t row_off = 0;
-  std::map<uint32_t, BlockInfo> blocks;
-};
-
-struct Table {
-  uint64_t uid = 0;
-  uint64_t cap_rows = 0;
-  uint64_t next_row = 0;
-  uint64_t reserve = 0;
-  uint64_t version = 0;
-  std::map<uint32_t, ColumnArena> cols;
-  std::vector<RowGroupEntry> rgs;
-  std::unordered_map<uint64_t, std::vector<size_t>> rg_index;  // hash(seg, rg) -> entries
-};
-
-uint64_t rg_hash(uint64_t seg, uint32_t rg) { return seg * 0x9e3779b97f4a7c15ull ^ rg; }
-
-}  // namespace
-
-struct tdbg_context {
-  int device = 0;
-  uint64_t budget = 0;
-  std::atomic<uint64_t> in_use{0};
-  std::shared_mutex mu;
-  std::unordered_map<uint64_t, std::unique_ptr<Table>> tables;
-  int n_cu = 256;
-  int max_lds = 160 * 1024;
-};
-
-namespace {
 
 int32_t dev_alloc(tdbg_context* ctx, void** p, uint64_t bytes) {
   *p = nullptr;
@@ -169,6 +95,29 @@ void dev_free(tdbg_context* ctx, void* p, uint64_t bytes) {
   (void)hipFree(p);
   ctx->in_use.fetch_sub(bytes);
 }
+
+int32_t ensure(tdbg_plan* pl, tdbg_plan::Buf& b, uint64_t bytes) {
+  if (b.bytes >= bytes && b.p) return TDBG_OK;
+  dev_free(pl->ctx, b.p, b.bytes);
+  b.p = nullptr;
+  b.bytes = 0;
+  uint64_t want = std::max<uint64_t>(bytes, 256);
+  int32_t rc = dev_alloc(pl->ctx, &b.p, want);
+  if (rc) return rc;
+  b.bytes = want;
+  return TDBG_OK;
+}
+
+void release(tdbg_plan* pl, tdbg_plan::Buf& b) {
+  dev_free(pl->ctx, b.p, b.bytes);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+}  // namespace rt
+}  // namespace tdbg
+
+namespace {
 
 uint64_t arena_bytes(uint64_t cap_rows, uint32_t elem) { return cap_rows * elem + 64; }
 uint64_t valid_bytes(uint64_t cap_rows) { return cap_rows / 8 + 64; }
@@ -238,85 +187,7 @@ void host_stats(uint32_t type, const void* v, const uint8_t* valid, uint64_t n, 
 
 }  // namespace
 
-// ---- result ----------------------------------------------------------------
-
-struct tdbg_result {
-  uint64_t n_rows = 0;
-  std::vector<tdbg_column_view> cols;
-  std::vector<std::vector<uint8_t>> data;
-  std::vector<std::vector<uint8_t>> valid;
-};
-
-// ---- plan --------------------------------------------------------------------
-
 namespace {
-
-struct AggOut {
-  uint32_t op;          // tdbg_agg_op
-  int kagg;             // index into kernel aggs, -1 when served by the row count
-  uint32_t in_type, out_type, scale;
-  bool is_float;
-  bool nullable;
-};
-
-struct Prepared {
-  uint64_t version = ~0ull;
-  uint64_t rows_hint = 0;
-  ScanParams sp{};
-  OrderParams op{};
-  FinalParams fp{};
-  uint32_t grid = 0, block = 0;
-  std::vector<uint64_t> tmpl;  // one initialised record
-  bool empty_domain = false;   // predicates admit no row
-  std::vector<uint32_t> used_cols;  // table column per ScanParams col
-  bool radix = false;          // high cardinality: radix partitioning (radix.hip)
-  RadixParams rx{};
-  uint32_t agg_grid = 0;
-};
-
-}  // namespace
-
-struct tdbg_plan {
-  tdbg_context* ctx = nullptr;
-  uint64_t table_uid = 0;
-  std::vector<tdbg_key_part> keys;
-  std::vector<tdbg_agg_spec> aggs;
-  std::vector<tdbg_predicate> preds;
-  std::vector<tdbg_order_spec> order;
-  uint64_t limit = 0, offset = 0, est_groups = 0;
-  std::vector<AggOut> outs;
-  std::mutex mu;
-  std::atomic<bool> cancel{false};
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  Prepared prep;
-  // device buffers, grown on demand
-  struct Buf {
-    void* p = nullptr;
-    uint64_t bytes = 0;
-  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf;
-  tdbg_exec_stats last{};
-};
-
-namespace {
-
-int32_t ensure(tdbg_plan* pl, tdbg_plan::Buf& b, uint64_t bytes) {
-  if (b.bytes >= bytes && b.p) return TDBG_OK;
-  dev_free(pl->ctx, b.p, b.bytes);
-  b.p = nullptr;
-  b.bytes = 0;
-  uint64_t want = std::max<uint64_t>(bytes, 256);
-  int32_t rc = dev_alloc(pl->ctx, &b.p, want);
-  if (rc) return rc;
-  b.bytes = want;
-  return TDBG_OK;
-}
-
-void release(tdbg_plan* pl, tdbg_plan::Buf& b) {
-  dev_free(pl->ctx, b.p, b.bytes);
-  b.p = nullptr;
-  b.bytes = 0;
-}
 
 // Validates the request; TDBG_ERR_UNSUPPORTED = decline to the CPU path
 // (the gates of src/exec/v2_lowcard_group.zig:234-373 and
@@ -371,15 +242,6 @@ int32_t validate(tdbg_plan* pl) {
   }
   return TDBG_OK;
 }
-
-struct MergedPred {
-  uint32_t col;
-  bool is_float;
-  bool empty = false;
-  int64_t lo_i = INT64_MIN, hi_i = INT64_MAX;
-  double lo_f = -HUGE_VAL, hi_f = HUGE_VAL;
-  bool lo_incl = true, hi_incl = true;
-};
 
 // The WHERE's `col op literal` leaves merged per column into one range term
 // (plus NEQ terms).  Returns false when some column's range is empty.
@@ -441,12 +303,77 @@ bool block_can_match(const MergedPred& m, const BlockInfo& b) {
   return true;
 }
 
-constexpr uint32_t kTileRows = 16384;
-constexpr uint64_t kDenseMax = 1ull << 24;
+}  // namespace
 
-// Specialises the plan to the table's statistics and the scanned row count.
-int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles, const std::vector<MergedPred>& ranges,
-                const std::vector<tdbg_predicate>& neqs) {
+namespace tdbg {
+namespace rt {
+
+int32_t collect_scan(tdbg_plan* pl, Table* t, const tdbg_rg_ref* rgs, uint32_t n_rgs, ScanSet& S) {
+  S = ScanSet{};
+  // the predicate terms and the columns every row group must have staged
+  const bool satisfiable = merge_preds(pl->preds, S.ranges, S.neqs);
+  std::vector<uint32_t> need_cols;
+  for (auto& k : pl->keys) need_cols.push_back(k.column_idx);
+  for (auto& a : pl->aggs) if (a.op != TDBG_AGG_COUNT_STAR) need_cols.push_back((uint32_t)a.column_idx);
+  for (auto& p : pl->preds) need_cols.push_back(p.column_idx);
+  std::vector<const RowGroupEntry*> scan;
+  auto consider = [&](const RowGroupEntry& rg) -> int32_t {
+    for (uint32_t c : need_cols) {
+      auto bi = rg.blocks.find(c);
+      if (bi == rg.blocks.end() || !bi->second.staged)
+        return fail(TDBG_ERR_NOT_STAGED, "segment %llu row group %u column %u not staged",
+                    (unsigned long long)rg.segment_id, rg.rg_idx, c);
+    }
+    for (auto& m : S.ranges) {
+      auto bi = rg.blocks.find(m.col);
+      if (!block_can_match(m, bi->second)) return TDBG_OK;  // zone map prunes it
+    }
+    scan.push_back(&rg);
+    return TDBG_OK;
+  };
+  if (rgs) {
+    for (uint32_t i = 0; i < n_rgs; i++) {
+      auto b = t->rg_index.find(rg_hash(rgs[i].segment_id, rgs[i].row_group_idx));
+      const RowGroupEntry* found = nullptr;
+      if (b != t->rg_index.end())
+        for (size_t j : b->second)
+          if (t->rgs[j].segment_id == rgs[i].segment_id && t->rgs[j].rg_idx == rgs[i].row_group_idx) found = &t->rgs[j];
+      if (!found) return fail(TDBG_ERR_NOT_STAGED, "segment %llu row group %u not staged",
+                              (unsigned long long)rgs[i].segment_id, rgs[i].row_group_idx);
+      int32_t rc = consider(*found);
+      if (rc) return rc;
+    }
+  } else {
+    for (auto& rg : t->rgs) {
+      int32_t rc = consider(rg);
+      if (rc) return rc;
+    }
+  }
+  if (!satisfiable) return TDBG_OK;
+  for (const RowGroupEntry* rg : scan) {
+    for (uint32_t o = 0; o < rg->n_rows; o += kTileRows) {
+      Tile tl{};
+      tl.row0 = rg->row_off + o;
+      tl.n = std::min<uint32_t>(kTileRows, rg->n_rows - o);
+      S.tiles.push_back(tl);
+    }
+    S.rows += rg->n_rows;
+  }
+  std::vector<uint32_t> cols;
+  for (uint32_t c : need_cols)
+    if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
+  for (uint32_t c : cols) {
+    auto it = t->cols.find(c);
+    if (it != t->cols.end()) S.bytes += S.rows * (uint64_t)it->second.elem;
+  }
+  return TDBG_OK;
+}
+
+int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint) {
+  const uint64_t n_rows_scan = S.rows;
+  const uint32_t n_tiles = (uint32_t)S.tiles.size();
+  const std::vector<MergedPred>& ranges = S.ranges;
+  const std::vector<tdbg_predicate>& neqs = S.neqs;
   Prepared& P = pl->prep;
   P = Prepared{};
   ScanParams& sp = P.sp;
@@ -576,7 +503,8 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
   }
   // launch shape and LDS table
   const uint32_t n_cu = (uint32_t)std::max(pl->ctx->n_cu, 8);
-  const uint32_t budget = std::min<uint32_t>(kLdsBudget, (uint32_t)pl->ctx->max_lds - 1024);
+  uint32_t budget = std::min<uint32_t>(kLdsBudget, (uint32_t)pl->ctx->max_lds - 1024);
+  if (hint.lds_budget) budget = std::min(budget, hint.lds_budget);
   const uint32_t key_bytes = dom < 0xffffffffull ? 4 : 8;
   // LDS bytes of one slot when a workgroup folds `rows_per_wg` rows: an
   // integer SUM keeps one i64 partial when rows_per_wg * max|v| < 2^62,
@@ -597,10 +525,12 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
   };
   uint32_t lds_cap = 0, parts = 1;
   sp.lds_mode = LDS_NONE;
-  if (dense) {
+  const uint32_t strat = hint.strategy;
+  if (dense && (strat == S_AUTO || strat == S_DIRECT)) {
     // a direct table, split over `parts` workgroups per tile range when the
     // key span does not fit one workgroup's LDS
-    for (uint32_t p = 1; p <= 8; p *= 2) {
+    const uint32_t max_parts = strat == S_DIRECT ? 64u : 8u;
+    for (uint32_t p = 1; p <= max_parts; p *= 2) {
       const uint32_t groups = std::max<uint32_t>(8, n_cu / p);
       const uint64_t rows_per_wg = ((uint64_t)n_tiles + groups - 1) / groups * kTileRows;
       const uint32_t sb = slot_bytes_for(rows_per_wg);
@@ -613,10 +543,10 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
       }
     }
   }
-  if (sp.lds_mode == LDS_NONE) {
+  if (sp.lds_mode == LDS_NONE && (strat == S_AUTO || strat == S_LDS_HASH)) {
     const uint32_t sb = slot_bytes_for(((uint64_t)n_tiles + n_cu - 1) / n_cu * kTileRows);
     const uint32_t cap = budget / (sb + key_bytes);
-    if (est <= 8ull * cap && cap >= 256) {
+    if ((strat == S_LDS_HASH || est <= 8ull * cap) && cap >= 256) {
       sp.lds_mode = LDS_HASH;
       lds_cap = cap;
     }
@@ -626,14 +556,15 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
   // THINDB_GPU_RADIX=0 never takes this path, =1 takes it whenever the
   // domain allows (tests use it to cover the partition passes on small tables).
   const char* rx_env = getenv("THINDB_GPU_RADIX");
-  if ((sp.lds_mode == LDS_NONE || (rx_env && rx_env[0] == '1')) && dom <= 0xffffffffull &&
-      !(rx_env && rx_env[0] == '0')) {
+  const bool forced = strat == S_RADIX || (strat == S_AUTO && rx_env && rx_env[0] == '1');
+  const bool rx_off = (strat != S_AUTO && strat != S_RADIX) || (strat == S_AUTO && rx_env && rx_env[0] == '0');
+  if ((sp.lds_mode == LDS_NONE || forced) && dom <= 0xffffffffull && !rx_off) {
     const uint32_t sb = slot_bytes_for(n_rows_scan);  // a leaf may hold every row
     const uint32_t L = dom > 1 ? 64u - (uint32_t)__builtin_clzll(dom - 1) : 0u;  // key bits
     // two leaf workgroups per CU (THINDB_GPU_RADIX_LEAF_BITS caps the table)
     const uint32_t leaf_budget = std::min<uint32_t>(budget, (uint32_t)pl->ctx->max_lds / 2) - 4096;
     const char* lb_env = getenv("THINDB_GPU_RADIX_LEAF_BITS");
-    const uint32_t lb_max = lb_env ? (uint32_t)atoi(lb_env) : 32u;
+    const uint32_t lb_max = hint.leaf_bits ? hint.leaf_bits : (lb_env ? (uint32_t)atoi(lb_env) : 32u);
     uint32_t leaf_bits = 0;
     while (leaf_bits < L && leaf_bits < lb_max && ((uint64_t)sb << (leaf_bits + 1)) <= leaf_budget) leaf_bits++;
     RadixParams& R = P.rx;
@@ -659,14 +590,15 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
     // staged scatter tile: 24 KiB of digit tables + tile_rows rows in LDS
     R.tile_rows = std::min<uint32_t>(4096, (budget - 24 * 1024) / R.row.stride / 256 * 256);
     const uint32_t rem = L - leaf_bits;
-    const bool forced = rx_env && rx_env[0] == '1';
+    const uint32_t max_bits1 = std::min<uint32_t>(hint.max_bits1 ? hint.max_bits1 : 8u, 10u);
+    const uint32_t max_rem = max_bits1 + (hint.single_level ? 0u : 10u);
     // automatic: at least one row per four keys of the domain on average, so
     // leaves are well filled (sparser domains stay with the HBM hash table)
     const bool fits = leaf_bits >= 8 && dom > 65536 && n_rows_scan >= (1u << 20) && dom / 4 <= n_rows_scan;
-    if ((forced || fits) && rem <= 18 && R.tile_rows >= 256) {
+    if ((forced || fits) && rem <= max_rem && R.tile_rows >= 256) {
       P.radix = true;
       R.leaf_bits = leaf_bits;
-      R.bits1 = std::min<uint32_t>(8, rem);
+      R.bits1 = std::min<uint32_t>(max_bits1, rem);
       R.bits2 = rem - R.bits1;
       R.shift2 = leaf_bits;
       R.shift1 = leaf_bits + R.bits2;
@@ -674,20 +606,25 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
       sp.lds_mode = LDS_DIRECT;  // one leaf's table
       lds_cap = 1u << leaf_bits;
       parts = 1;
-      // records carry their key (the hash layout): they are emitted
-      // compacted, leaf by leaf
-      if (dense) {
-        sp.g_key_off = 0;
-        sp.g_count_off += 8;
-        for (uint32_t k = 0; k < sp.n_aggs; k++) {
-          sp.aggs[k].g_off += 8;
-          if (sp.aggs[k].g_ns_off != ~0u) sp.aggs[k].g_ns_off += 8;
+      if (hint.dense_out && dense) {
+        // leaves write every key of their range into the dense table
+        R.dense_out = 1;
+      } else {
+        // records carry their key (the hash layout): they are emitted
+        // compacted, leaf by leaf
+        if (dense) {
+          sp.g_key_off = 0;
+          sp.g_count_off += 8;
+          for (uint32_t k = 0; k < sp.n_aggs; k++) {
+            sp.aggs[k].g_off += 8;
+            if (sp.aggs[k].g_ns_off != ~0u) sp.aggs[k].g_ns_off += 8;
+          }
+          sp.g_rec += 8;
+          P.tmpl.insert(P.tmpl.begin(), kEmptyKey);
         }
-        sp.g_rec += 8;
-        P.tmpl.insert(P.tmpl.begin(), kEmptyKey);
+        sp.g_mode = G_HASH;
+        sp.g_cap = std::max<uint64_t>(1, std::min<uint64_t>(dom, n_rows_scan));
       }
-      sp.g_mode = G_HASH;
-      sp.g_cap = std::max<uint64_t>(1, std::min<uint64_t>(dom, n_rows_scan));
     }
   }
   sp.lds_cap = lds_cap;
@@ -769,7 +706,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
       default: add(ao.is_float ? C_FORD_AT : C_I64_AT, 0, d.g_off, d.g_ns_off); break;
     }
   }
-  if (P.radix && op.n_comps < (uint32_t)kMaxComps) {
+  if (P.radix && !P.rx.dense_out && op.n_comps < (uint32_t)kMaxComps) {
     // leaves emit groups in completion order: the relative key as the last
     // word gives ties (and an unordered result) the dense table's key order
     CompDesc c{};
@@ -786,7 +723,9 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
   }
   for (auto& ao : pl->outs) {
     OutDesc o{};
-    o.ns_off = ~0u; o.valid_off = 0; o.scale_div = 1.0;
+    // SUM/AVG/MIN/MAX are NULL only for a group without a non-null input,
+    // which a non-nullable input column cannot produce: no validity bytes
+    o.ns_off = ~0u; o.valid_off = ao.nullable ? 0 : ~0ull; o.scale_div = 1.0;
     o.bytes = (uint32_t)type_bytes(ao.out_type);
     const AggDesc* d = ao.kagg >= 0 ? &sp.aggs[ao.kagg] : nullptr;
     if (d) { o.off = d->g_off; o.ns_off = d->g_ns_off; }
@@ -814,6 +753,11 @@ int32_t prepare(tdbg_plan* pl, Table* t, uint64_t n_rows_scan, uint32_t n_tiles,
   P.rows_hint = n_rows_scan;
   return TDBG_OK;
 }
+
+}  // namespace rt
+}  // namespace tdbg
+
+namespace {
 
 int32_t sync_check(tdbg_plan* pl, const char* what) {
   hipError_t e = hipStreamSynchronize(pl->stream);
@@ -1035,10 +979,12 @@ int32_t tdbg_plan_last_stats(const tdbg_plan* pl, tdbg_exec_stats* out) {
   return TDBG_OK;
 }
 
-static int32_t make_result(tdbg_plan* pl, uint64_t n_out, const std::vector<uint8_t>& host,
-                           const std::vector<uint64_t>& data_off, const std::vector<uint64_t>& valid_off,
-                           tdbg_result** out) {
+}  // extern "C"
+
+int32_t tdbg::rt::make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host, const std::vector<uint64_t>& data_off,
+                              const std::vector<uint64_t>& valid_off, std::shared_ptr<void> keep, tdbg_result** out) {
   auto* r = new tdbg_result();
+  r->keep = keep;
   r->n_rows = n_out;
   const size_t nk = pl->keys.size();
   const size_t nc = nk + pl->outs.size();
@@ -1057,13 +1003,15 @@ static int32_t make_result(tdbg_plan* pl, uint64_t n_out, const std::vector<uint
       nullable = o.op != TDBG_AGG_COUNT_STAR && o.op != TDBG_AGG_COUNT;
     }
     const uint64_t bytes = n_out * (uint64_t)type_bytes(type);
-    r->data[c].resize(bytes + 16);
-    if (n_out) memcpy(r->data[c].data(), host.data() + data_off[c], bytes);
+    if (!keep) {
+      r->data[c].resize(bytes + 16);
+      if (n_out) memcpy(r->data[c].data(), host + data_off[c], bytes);
+    }
     bool any_null = false;
     if (valid_off[c] != ~0ull && n_out) {
       std::vector<uint8_t>& bm = r->valid[c];
       bm.assign((n_out + 7) / 8, 0);
-      const uint8_t* vb = host.data() + valid_off[c];
+      const uint8_t* vb = host + valid_off[c];
       for (uint64_t i = 0; i < n_out; i++) {
         if (vb[i]) bm[i >> 3] |= (uint8_t)(1u << (i & 7));
         else any_null = true;
@@ -1073,13 +1021,15 @@ static int32_t make_result(tdbg_plan* pl, uint64_t n_out, const std::vector<uint
     rc.type = type;
     rc.scale = scale;
     rc.nullable = nullable;
-    rc.data = r->data[c].data();
+    rc.data = keep ? host + data_off[c] : r->data[c].data();
     rc.validity = any_null ? r->valid[c].data() : nullptr;
     rc.n_rows = n_out;
   }
   *out = r;
   return TDBG_OK;
 }
+
+extern "C" {
 
 int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out) {
   if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
@@ -1093,63 +1043,12 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   auto tit = ctx->tables.find(pl->table_uid);
   if (tit == ctx->tables.end()) return fail(TDBG_ERR_NOT_STAGED, "table %llu has no staged columns", (unsigned long long)pl->table_uid);
   Table* t = tit->second.get();
-  // the predicate terms and the columns every row group must have staged
-  std::vector<MergedPred> ranges;
-  std::vector<tdbg_predicate> neqs;
-  const bool satisfiable = merge_preds(pl->preds, ranges, neqs);
-  std::vector<uint32_t> need_cols;
-  for (auto& k : pl->keys) need_cols.push_back(k.column_idx);
-  for (auto& a : pl->aggs) if (a.op != TDBG_AGG_COUNT_STAR) need_cols.push_back((uint32_t)a.column_idx);
-  for (auto& p : pl->preds) need_cols.push_back(p.column_idx);
-  std::vector<const RowGroupEntry*> scan;
-  auto consider = [&](const RowGroupEntry& rg) -> int32_t {
-    for (uint32_t c : need_cols) {
-      auto bi = rg.blocks.find(c);
-      if (bi == rg.blocks.end() || !bi->second.staged)
-        return fail(TDBG_ERR_NOT_STAGED, "segment %llu row group %u column %u not staged",
-                    (unsigned long long)rg.segment_id, rg.rg_idx, c);
-    }
-    for (auto& m : ranges) {
-      auto bi = rg.blocks.find(m.col);
-      if (!block_can_match(m, bi->second)) return TDBG_OK;  // zone map prunes it
-    }
-    scan.push_back(&rg);
-    return TDBG_OK;
-  };
-  if (rgs) {
-    for (uint32_t i = 0; i < n_rgs; i++) {
-      auto b = t->rg_index.find(rg_hash(rgs[i].segment_id, rgs[i].row_group_idx));
-      const RowGroupEntry* found = nullptr;
-      if (b != t->rg_index.end())
-        for (size_t j : b->second)
-          if (t->rgs[j].segment_id == rgs[i].segment_id && t->rgs[j].rg_idx == rgs[i].row_group_idx) found = &t->rgs[j];
-      if (!found) return fail(TDBG_ERR_NOT_STAGED, "segment %llu row group %u not staged",
-                              (unsigned long long)rgs[i].segment_id, rgs[i].row_group_idx);
-      int32_t rc = consider(*found);
-      if (rc) return rc;
-    }
-  } else {
-    for (auto& rg : t->rgs) {
-      int32_t rc = consider(rg);
-      if (rc) return rc;
-    }
-  }
-  // tiles
-  std::vector<Tile> tiles;
-  uint64_t rows = 0;
-  if (satisfiable) {
-    for (const RowGroupEntry* rg : scan) {
-      for (uint32_t o = 0; o < rg->n_rows; o += kTileRows) {
-        Tile tl{};
-        tl.row0 = rg->row_off + o;
-        tl.n = std::min<uint32_t>(kTileRows, rg->n_rows - o);
-        tiles.push_back(tl);
-      }
-      rows += rg->n_rows;
-    }
-  }
-  int32_t rc = prepare(pl, t, rows, (uint32_t)tiles.size(), ranges, neqs);
+  ScanSet S;
+  int32_t rc = collect_scan(pl, t, rgs, n_rgs, S);
   if (rc) return rc;
+  const std::vector<Tile>& tiles = S.tiles;
+  const uint64_t rows = S.rows;
+  if ((rc = prepare(pl, t, S, PlanHint{}))) return rc;
   Prepared& P = pl->prep;
   ScanParams& sp = P.sp;
   // offsets of the output buffer
@@ -1158,9 +1057,8 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   std::vector<uint64_t> data_off(nc, 0), valid_off(nc, ~0ull);
   const uint64_t need_lim = pl->limit == 0 ? ~0ull : pl->offset + pl->limit;
   if (tiles.empty()) {
-    std::vector<uint8_t> none;
     pl->last.rows_scanned = 0;
-    return make_result(pl, 0, none, data_off, valid_off, out);
+    return make_result(pl, 0, nullptr, data_off, valid_off, nullptr, out);
   }
   hipStream_t s = pl->stream;
   // buffers
@@ -1176,8 +1074,7 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   sp.err = static_cast<uint32_t*>(pl->err.p);
   P.op.g_state = sp.g_state;
   P.fp.g_state = sp.g_state;
-  uint64_t scanned_bytes = 0;
-  for (uint32_t i = 0; i < sp.n_cols; i++) scanned_bytes += rows * (uint64_t)t->cols[P.used_cols[i]].elem;
+  const uint64_t scanned_bytes = S.bytes;
 
   HIP_TRY(hipMemcpyAsync(pl->tiles.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, s), "tiles");
   HIP_TRY(hipMemcpyAsync(pl->tmpl.p, P.tmpl.data(), P.tmpl.size() * 8, hipMemcpyHostToDevice, s), "template");
@@ -1292,7 +1189,7 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   pl->last.global_mode = P.radix ? 3u : sp.g_mode;
   pl->last.specialized = specialized ? 1u : 0u;
   pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
-  return make_result(pl, n_out, host, data_off, valid_off, out);
+  return make_result(pl, n_out, host.data(), data_off, valid_off, nullptr, out);
 }
 
 uint64_t tdbg_result_num_rows(const tdbg_result* r) { return r ? r->n_rows : 0; }

@@ -1,0 +1,213 @@
+// runtime.h — host-side internals of libthindb_gpu.so shared by runtime.cpp
+// (context, HBM column store, single-query plans) and batch.cpp (batched and
+// served execution of many plans).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/thindb_gpu.h"
+#include "engine.h"
+
+namespace tdbg {
+namespace rt {
+
+// Records `fmt` as the calling thread's last error and returns `code`.
+int32_t fail(int32_t code, const char* fmt, ...);
+int32_t hip_fail(hipError_t e, const char* what);
+
+#define HIP_TRY(expr, what)                              \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return ::tdbg::rt::hip_fail(e_, what); \
+  } while (0)
+
+int type_bytes(uint32_t t);
+inline bool is_int_type(uint32_t t) { return t >= TDBG_BOOLEAN && t <= TDBG_DECIMAL64; }
+inline bool is_float_type(uint32_t t) { return t == TDBG_FLOAT || t == TDBG_DOUBLE; }
+uint32_t kind_of(uint32_t t);
+inline uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+inline uint64_t next_pow2(uint64_t x) {
+  uint64_t p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+constexpr uint32_t kTileRows = 16384;        // rows per scan tile
+constexpr uint64_t kDenseMax = 1ull << 24;   // key spans up to this take a dense HBM table
+
+// ---- HBM column store ------------------------------------------------------
+
+struct ColumnArena {
+  uint32_t type = 0;
+  uint32_t elem = 0;
+  uint8_t* d_data = nullptr;
+  uint32_t* d_valid = nullptr;  // allocated once any block is nullable
+  bool has_stats = false;
+  int64_t min_i = 0, max_i = 0;
+  double min_f = 0, max_f = 0;
+};
+
+struct BlockInfo {
+  bool staged = false;
+  bool has_stats = false;
+  tdbg_block_stats stats{};
+};
+
+struct RowGroupEntry {
+  uint64_t segment_id = 0;
+  uint32_t rg_idx = 0;
+  uint32_t n_rows = 0;
+  uint64_t row_off = 0;
+  std::map<uint32_t, BlockInfo> blocks;
+};
+
+struct Table {
+  uint64_t uid = 0;
+  uint64_t cap_rows = 0;
+  uint64_t next_row = 0;
+  uint64_t reserve = 0;
+  uint64_t version = 0;
+  std::map<uint32_t, ColumnArena> cols;
+  std::vector<RowGroupEntry> rgs;
+  std::unordered_map<uint64_t, std::vector<size_t>> rg_index;  // hash(seg, rg) -> entries
+};
+
+inline uint64_t rg_hash(uint64_t seg, uint32_t rg) { return seg * 0x9e3779b97f4a7c15ull ^ rg; }
+
+// ---- plans -----------------------------------------------------------------
+
+struct AggOut {
+  uint32_t op;          // tdbg_agg_op
+  int kagg;             // index into kernel aggs, -1 when served by the row count
+  uint32_t in_type, out_type, scale;
+  bool is_float;
+  bool nullable;
+};
+
+// How prepare() shapes the group table.  S_AUTO lets it choose; the batch
+// autotuner passes the strategy it measured fastest for the plan's
+// (rows, estimated groups) bucket.
+enum Strategy : uint32_t { S_AUTO = 0, S_DIRECT, S_GLOBAL, S_LDS_HASH, S_RADIX };
+struct PlanHint {
+  uint32_t strategy = S_AUTO;
+  uint32_t leaf_bits = 0;       // S_RADIX: log2 keys per leaf (0: largest that fits)
+  uint32_t lds_budget = 0;      // LDS bytes per workgroup (0: the device maximum)
+  uint32_t max_bits1 = 8;       // S_RADIX: widest level-1 digit
+  bool single_level = false;    // S_RADIX: one scatter level only
+  bool dense_out = false;       // S_RADIX: leaves write a dense record array
+};
+
+struct Prepared {
+  uint64_t version = ~0ull;
+  uint64_t rows_hint = 0;
+  ScanParams sp{};
+  OrderParams op{};
+  FinalParams fp{};
+  uint32_t grid = 0, block = 0;
+  std::vector<uint64_t> tmpl;  // one initialised record
+  bool empty_domain = false;   // predicates admit no row
+  std::vector<uint32_t> used_cols;  // table column per ScanParams col
+  bool radix = false;          // high cardinality: radix partitioning (radix.hip)
+  RadixParams rx{};
+  uint32_t agg_grid = 0;
+};
+
+struct MergedPred {
+  uint32_t col;
+  bool is_float;
+  bool empty = false;
+  int64_t lo_i = INT64_MIN, hi_i = INT64_MAX;
+  double lo_f = -HUGE_VAL, hi_f = HUGE_VAL;
+  bool lo_incl = true, hi_incl = true;
+};
+
+// The row groups one execution scans, as tiles, after zone-map pruning.
+struct ScanSet {
+  std::vector<MergedPred> ranges;
+  std::vector<tdbg_predicate> neqs;
+  std::vector<Tile> tiles;
+  uint64_t rows = 0;
+  uint64_t bytes = 0;  // native-width bytes of the referenced columns
+};
+
+}  // namespace rt
+}  // namespace tdbg
+
+struct tdbg_context {
+  int device = 0;
+  uint64_t budget = 0;
+  std::atomic<uint64_t> in_use{0};
+  std::shared_mutex mu;
+  std::unordered_map<uint64_t, std::unique_ptr<tdbg::rt::Table>> tables;
+  int n_cu = 256;
+  int max_lds = 160 * 1024;
+  // batch autotuner: (rows bucket, groups bucket) -> measured best hint
+  std::mutex tune_mu;
+  std::map<uint32_t, tdbg::rt::PlanHint> tuned;
+};
+
+struct tdbg_result {
+  uint64_t n_rows = 0;
+  std::vector<tdbg_column_view> cols;
+  std::vector<std::vector<uint8_t>> data;   // owned column copies
+  std::vector<std::vector<uint8_t>> valid;
+  std::shared_ptr<void> keep;               // host memory the views borrow (batch results)
+};
+
+struct tdbg_plan {
+  tdbg_context* ctx = nullptr;
+  uint64_t table_uid = 0;
+  std::vector<tdbg_key_part> keys;
+  std::vector<tdbg_agg_spec> aggs;
+  std::vector<tdbg_predicate> preds;
+  std::vector<tdbg_order_spec> order;
+  uint64_t limit = 0, offset = 0, est_groups = 0;
+  std::vector<tdbg::rt::AggOut> outs;
+  std::mutex mu;
+  std::atomic<bool> cancel{false};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  tdbg::rt::Prepared prep;
+  // device buffers, grown on demand
+  struct Buf {
+    void* p = nullptr;
+    uint64_t bytes = 0;
+  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf;
+  tdbg_exec_stats last{};
+};
+
+namespace tdbg {
+namespace rt {
+
+int32_t dev_alloc(tdbg_context* ctx, void** p, uint64_t bytes);
+void dev_free(tdbg_context* ctx, void* p, uint64_t bytes);
+int32_t ensure(tdbg_plan* pl, tdbg_plan::Buf& b, uint64_t bytes);
+void release(tdbg_plan* pl, tdbg_plan::Buf& b);
+
+// Resolves the row groups a plan scans (all staged ones when rgs is null),
+// checks that every referenced block is staged, prunes by zone maps and cuts
+// the survivors into tiles.  Caller holds ctx->mu shared.
+int32_t collect_scan(tdbg_plan* pl, Table* t, const tdbg_rg_ref* rgs, uint32_t n_rgs, ScanSet& out);
+
+// Specialises the plan to the table's statistics and the scanned row count.
+int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& s, const PlanHint& hint);
+
+// A result of n_out rows whose column c sits at host + data_off[c] (validity
+// bytes at host + valid_off[c], or ~0 for none): copied out of `host`, or,
+// with `keep` set, viewed in place while the result holds `keep`.
+int32_t make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host, const std::vector<uint64_t>& data_off,
+                    const std::vector<uint64_t>& valid_off, std::shared_ptr<void> keep, tdbg_result** out);
+
+}  // namespace rt
+}  // namespace tdbg

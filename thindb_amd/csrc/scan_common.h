@@ -526,24 +526,16 @@ __device__ __forceinline__ void agg_rt(const ScanParams& P, const AggDesc& a, ui
 
 // ---- the kernel body ----------------------------------------------------------------------
 
-// Tiles, key partitions and the LDS table's life cycle around a strip
-// routine STRIP::run<LMODE>(P, lds, row, count, part_lo).
+// The LDS table's life cycle around a strip routine
+// STRIP::run<LMODE>(P, lds, row, count, part_lo) over tiles [t0, t1); with a
+// direct table the workgroup owns keys [part_lo, part_lo + lds_cap).
 template <uint32_t LMODE, class STRIP>
-__device__ __forceinline__ void scan_body(const ScanParams& P, uint8_t* lds) {
+__device__ __forceinline__ void scan_range(const ScanParams& P, uint8_t* lds, uint32_t t0, uint32_t t1,
+                                           uint64_t part_lo) {
   if (LMODE != LDS_NONE) {
     lds_init(P, lds);
     __syncthreads();
   }
-  // With a partitioned direct table, the `parts` workgroups of one group
-  // (blockIdx.x = part * groups + g; groups a multiple of 8 keeps them on
-  // one XCD, sharing its L2) scan the same tiles, each aggregating only the
-  // keys it owns, so the tiles stream from HBM once.
-  const uint32_t parts = LMODE == LDS_DIRECT ? P.lds_parts : 1u;
-  const uint32_t groups = gridDim.x / parts;
-  const uint32_t g = blockIdx.x % groups;
-  const uint64_t part_lo = (uint64_t)(blockIdx.x / groups) * P.lds_cap;
-  const uint32_t t0 = (uint32_t)(((uint64_t)g * P.n_tiles) / groups);
-  const uint32_t t1 = (uint32_t)(((uint64_t)(g + 1) * P.n_tiles) / groups);
   const uint32_t step = blockDim.x * kVec;
   for (uint32_t t = t0; t < t1; t++) {
     const Tile T = P.tiles[t];
@@ -556,6 +548,21 @@ __device__ __forceinline__ void scan_body(const ScanParams& P, uint8_t* lds) {
     __syncthreads();
     lds_flush(P, lds, part_lo);
   }
+}
+
+// One launch over all of P's tiles.  With a partitioned direct table, the
+// `parts` workgroups of one group (blockIdx.x = part * groups + g; groups a
+// multiple of 8 keeps them on one XCD, sharing its L2) scan the same tiles,
+// each aggregating only the keys it owns, so the tiles stream from HBM once.
+template <uint32_t LMODE, class STRIP>
+__device__ __forceinline__ void scan_body(const ScanParams& P, uint8_t* lds) {
+  const uint32_t parts = LMODE == LDS_DIRECT ? P.lds_parts : 1u;
+  const uint32_t groups = gridDim.x / parts;
+  const uint32_t g = blockIdx.x % groups;
+  const uint64_t part_lo = (uint64_t)(blockIdx.x / groups) * P.lds_cap;
+  const uint32_t t0 = (uint32_t)(((uint64_t)g * P.n_tiles) / groups);
+  const uint32_t t1 = (uint32_t)(((uint64_t)(g + 1) * P.n_tiles) / groups);
+  scan_range<LMODE, STRIP>(P, lds, t0, t1, part_lo);
 }
 
 }  // namespace tdbg

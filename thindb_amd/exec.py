@@ -413,3 +413,44 @@ def try_build_gpu_group(session: GpuSession, table: Table, request: Request) -> 
     except Declined:
         return None
     return GpuGroupQuery(session, table, request, plan, cols, output_schema(table, request))
+
+
+class GpuBatchRun:
+    """Many grouped requests over (possibly different) tables run together in
+    one persistent device launch (gpu.Batch); each result equals what the
+    request returns alone.  ORDER BY / LIMIT / OFFSET requests are declined:
+    run those through try_build_gpu_group."""
+
+    def __init__(self, session: GpuSession, pairs: Sequence[Tuple[Table, Request]]):
+        self.session = session
+        self.items = []
+        plans = []
+        try:
+            for t, r in pairs:
+                if r.order_specs or r.limit or r.offset:
+                    raise Declined("ORDER BY / LIMIT / OFFSET requests run alone")
+                plan, cols = build_plan(session, t, r)
+                self.items.append((t, r, plan, output_schema(t, r)))
+                plans.append(plan)
+                session.stage_table(t, cols)
+            self.batch = gpu.Batch(session.ctx, plans)
+        except Exception:
+            for p in plans:
+                p.close()
+            raise
+
+    def autotune(self, reps: int = 2) -> None:
+        self.batch.autotune(reps)
+
+    def execute(self) -> List[Batch]:
+        self.batch.execute()
+        return [self.result(i) for i in range(len(self.items))]
+
+    def result(self, i: int) -> Batch:
+        cols = self.batch.result(i)
+        return Batch(self.items[i][3], cols, len(cols[0].values) if cols else 0)
+
+    def close(self) -> None:
+        self.batch.close()
+        for _, _, p, _ in self.items:
+            p.close()

@@ -130,3 +130,26 @@ def table(kind: str, n: int, seed: int = None, **kw) -> Table:
     if kind == "highcard":
         return Table(HIGHCARD_SCHEMA, highcard(n, seed if seed is not None else 3), **kw)
     raise ValueError(kind)
+
+
+BATCH_FILTER_LO = -500_000_000   # optional filter: value >= -5e8 AND value < 5e8 (50% of rows)
+BATCH_FILTER_HI = 500_000_000
+
+
+def batched_table(seed: int, t: int, max_rows: int = 0) -> Table:
+    """Batched-workload table t (rows log-uniform in [1e4, 2e7], key
+    cardinality log-uniform in [4, 1e6]); max_rows > 0 truncates it."""
+    rows, _, _ = batched_shape(seed, t)
+    n = min(rows, max_rows) if max_rows else rows
+    return Table(BATCHED_SCHEMA, batched(seed, t, n))
+
+
+def batched_request(seed: int, t: int):
+    """GROUP BY key with COUNT/SUM/MIN/MAX of value, with the 50%-selective
+    range filter on value for the tables whose coin flip says so."""
+    from .exec import AggSpec, And, Predicate, Request
+    _, _, filtered_ = batched_shape(seed, t)
+    where = And((Predicate("value", "gte", BATCH_FILTER_LO), Predicate("value", "lt", BATCH_FILTER_HI))) \
+        if filtered_ else None
+    return Request(["key"], [AggSpec("count", None, "c"), AggSpec("sum", "value", "s"), AggSpec("min", "value", "mn"),
+                             AggSpec("max", "value", "mx")], where_filter=where)

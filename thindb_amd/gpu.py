@@ -168,6 +168,17 @@ class ExecStats(C.Structure):
                 ("global_mode", C.c_uint32), ("specialized", C.c_uint32), ("lds_parts", C.c_uint32)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("scan_ms", C.c_double), ("rows", C.c_uint64), ("bytes", C.c_uint64),
+                ("result_bytes", C.c_uint64), ("n_plans", C.c_uint32), ("n_items", C.c_uint32),
+                ("graph", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class BatchRoute(C.Structure):
+    _fields_ = [("strategy", C.c_uint32), ("lds_parts", C.c_uint32), ("radix_bits", C.c_uint32),
+                ("items", C.c_uint32)]
+
+
 # Every entry point include/thindb_gpu.h declares, with its ctypes signature.
 EXPORTS = {
     "tdbg_abi_version": (C.c_uint32, []),
@@ -185,6 +196,13 @@ EXPORTS = {
     "tdbg_execute": (C.c_int32, [C.c_void_p, C.POINTER(RgRef), C.c_uint32, C.POINTER(C.c_void_p)]),
     "tdbg_plan_last_stats": (C.c_int32, [C.c_void_p, C.POINTER(ExecStats)]),
     "tdbg_cancel": (C.c_int32, [C.c_void_p]),
+    "tdbg_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tdbg_batch_autotune": (C.c_int32, [C.c_void_p, C.c_uint32]),
+    "tdbg_batch_execute": (C.c_int32, [C.c_void_p]),
+    "tdbg_batch_result": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tdbg_batch_last_stats": (C.c_int32, [C.c_void_p, C.POINTER(BatchStats)]),
+    "tdbg_batch_plan_info": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(BatchRoute)]),
+    "tdbg_batch_destroy": (None, [C.c_void_p]),
     "tdbg_result_num_rows": (C.c_uint64, [C.c_void_p]),
     "tdbg_result_num_columns": (C.c_uint32, [C.c_void_p]),
     "tdbg_result_column": (C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(ColumnView)]),
@@ -369,16 +387,11 @@ class Plan:
             arr, n = rgs if isinstance(rgs, tuple) and len(rgs) == 2 and isinstance(rgs[1], int) \
                 and not isinstance(rgs[0], int) else self.rg_array(rgs)
             check(self._lib.tdbg_execute(self._h, arr, n, C.byref(out)))
-        try:
-            n = self._lib.tdbg_result_num_columns(out)
-            cols = []
-            for i in range(n):
-                v = ColumnView()
-                check(self._lib.tdbg_result_column(out, i, C.byref(v)))
-                cols.append(_column_from_view(v))
-            return cols
-        finally:
-            self._lib.tdbg_result_free(out)
+        return _result_columns(self._lib, out)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
 
     def stats(self) -> ExecStats:
         st = ExecStats()
@@ -391,6 +404,71 @@ class Plan:
     def close(self) -> None:
         if self._h:
             self._lib.tdbg_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _result_columns(lib, out) -> list:
+    try:
+        cols = []
+        for i in range(lib.tdbg_result_num_columns(out)):
+            v = ColumnView()
+            check(lib.tdbg_result_column(out, i, C.byref(v)))
+            cols.append(_column_from_view(v))
+        return cols
+    finally:
+        lib.tdbg_result_free(out)
+
+
+class Batch:
+    """Many plans executed together in one persistent launch (tdbg_batch)."""
+
+    STRATEGIES = {1: "lds_direct", 2: "hbm_atomics", 3: "lds_hash", 4: "radix"}
+
+    def __init__(self, ctx: Context, plans: Sequence["Plan"]):
+        lib = load()
+        self._lib = lib
+        self.ctx = ctx
+        self.plans = list(plans)
+        arr = (C.c_void_p * len(self.plans))(*[p.handle.value for p in self.plans])
+        h = C.c_void_p()
+        check(lib.tdbg_batch_create(ctx.handle, arr, len(self.plans), C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def autotune(self, reps: int = 2) -> None:
+        check(self._lib.tdbg_batch_autotune(self._h, reps))
+
+    def execute(self) -> None:
+        check(self._lib.tdbg_batch_execute(self._h))
+
+    def result(self, i: int) -> list:
+        """Result columns of plan i of the last execution (keys, then aggregates)."""
+        out = C.c_void_p()
+        check(self._lib.tdbg_batch_result(self._h, i, C.byref(out)))
+        return _result_columns(self._lib, out)
+
+    def stats(self) -> BatchStats:
+        st = BatchStats()
+        check(self._lib.tdbg_batch_last_stats(self._h, C.byref(st)))
+        return st
+
+    def route(self, i: int) -> BatchRoute:
+        r = BatchRoute()
+        check(self._lib.tdbg_batch_plan_info(self._h, i, C.byref(r)))
+        return r
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.tdbg_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):
