@@ -291,6 +291,59 @@ int32_t tdbg_batch_last_stats(const tdbg_batch* batch, tdbg_batch_stats* out);
 int32_t tdbg_batch_plan_info(const tdbg_batch* batch, uint32_t i, tdbg_batch_route* out);
 void tdbg_batch_destroy(tdbg_batch* batch);
 
+/* ---- serving ----------------------------------------------------------------
+ * No counterpart in the reference: thindb-server runs each query on its
+ * connection thread (net/mysql/server.zig:191) with a process-wide CPU
+ * scheduler (util/core_scheduler.zig:1-28).  A server is the device-side
+ * request loop: connection threads submit plans (tdbg_server_submit replaces
+ * a direct tdbg_execute) and wait on tickets; `lanes` worker threads each
+ * keep one unit of work on the GPU (own stream), so units overlap.  A unit is
+ * either one large request (many rows, or ORDER BY/LIMIT/OFFSET) run alone,
+ * or the small requests queued at that moment (up to max_batch, FIFO) run as
+ * one batched persistent launch: continuous batching.  Requests are admitted
+ * in arrival order while the estimated device bytes of the running units
+ * (group tables, partition scratch, results) stay within admit_bytes. */
+typedef struct tdbg_server tdbg_server;
+typedef struct tdbg_ticket tdbg_ticket;
+typedef struct tdbg_server_config {
+  uint32_t lanes;        /* units on the device at once (0 = 3) */
+  uint32_t max_batch;    /* requests per batched launch (0 = 64) */
+  uint64_t admit_bytes;  /* estimated device bytes of running units (0 = a quarter of the budget) */
+  uint64_t large_rows;   /* requests scanning this many rows run alone (0 = 16M) */
+} tdbg_server_config;
+typedef struct tdbg_ticket_report {
+  int64_t submit_ns;     /* steady-clock times of submission, dispatch and completion */
+  int64_t start_ns;
+  int64_t done_ns;
+  uint64_t est_bytes;    /* admission estimate */
+  uint32_t batch_size;   /* requests that shared its launch (1 = ran alone) */
+  uint32_t lane;
+  int32_t status;        /* TDBG_PENDING until done */
+  uint32_t pad;
+} tdbg_ticket_report;
+typedef struct tdbg_server_counters {
+  uint64_t submitted;
+  uint64_t completed;
+  uint64_t units;            /* launches (a batch counts once) */
+  uint64_t batched;          /* requests that ran in batches of two or more */
+  uint64_t peak_admitted;    /* highest estimated bytes of running units */
+  uint64_t admission_waits;  /* times the queue head waited for memory */
+} tdbg_server_counters;
+
+int32_t tdbg_server_start(tdbg_context* ctx, const tdbg_server_config* cfg, tdbg_server** out);
+/* Queues one execution of `plan` over every staged row group of its table.
+ * The plan must outlive the ticket's completion. */
+int32_t tdbg_server_submit(tdbg_server* server, tdbg_plan* plan, tdbg_ticket** out);
+/* TDBG_PENDING while queued or running, else the execution's status. */
+int32_t tdbg_ticket_poll(tdbg_ticket* ticket);
+/* Blocks until done; on TDBG_OK hands over the result (caller frees it). */
+int32_t tdbg_ticket_wait(tdbg_ticket* ticket, tdbg_result** out);
+int32_t tdbg_ticket_info(const tdbg_ticket* ticket, tdbg_ticket_report* out);
+void tdbg_ticket_free(tdbg_ticket* ticket);
+int32_t tdbg_server_stats(const tdbg_server* server, tdbg_server_counters* out);
+/* Runs what is queued, then stops the lanes. */
+void tdbg_server_stop(tdbg_server* server);
+
 uint64_t tdbg_result_num_rows(const tdbg_result* r);
 uint32_t tdbg_result_num_columns(const tdbg_result* r);
 int32_t tdbg_result_column(const tdbg_result* r, uint32_t i, tdbg_column_view* out);

@@ -36,6 +36,8 @@ STRUCTS = {  # header type -> ctypes binding
     "tdbg_key_part": gpu.KeyPart, "tdbg_agg_spec": gpu.AggSpecC, "tdbg_predicate": gpu.PredicateC,
     "tdbg_order_spec": gpu.OrderSpecC, "tdbg_plan_desc": gpu.PlanDesc, "tdbg_column_view": gpu.ColumnView,
     "tdbg_exec_stats": gpu.ExecStats, "tdbg_batch_stats": gpu.BatchStats, "tdbg_batch_route": gpu.BatchRoute,
+    "tdbg_server_config": gpu.ServerConfig, "tdbg_ticket_report": gpu.TicketReport,
+    "tdbg_server_counters": gpu.ServerCounters,
 }
 
 

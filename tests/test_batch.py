@@ -78,14 +78,15 @@ def test_each_strategy_vs_oracle(session, oracle_lib, monkeypatch, strategy):
 
 
 def test_same_as_alone(session):
-    """Each plan's batched result is its single-query result, row for row."""
+    """Each plan's batched result is its single-query result: integer columns
+    bit-exact, DOUBLE SUM/AVG within 1e-12 (their addition order differs)."""
     pairs = typed_pairs()[:6] + batched_pairs(5, 6, 80_000)
     res, _, _ = run_batch(session, pairs)
     for (t, r), b in zip(pairs, res):
         q = try_build_gpu_group(session, t, r)
         alone = rows_of(q.next())
         q.deinit()
-        assert sorted(rows_of(b), key=repr) == sorted(alone, key=repr)
+        assert_same_rows(rows_of(b), alone, float_cols(t, r), REL, len(r.group_cols), None)
 
 
 def test_graph_replays(session, oracle_lib, monkeypatch):
@@ -101,8 +102,8 @@ def test_graph_replays(session, oracle_lib, monkeypatch):
         for rep in range(3):
             res = run.execute()
             assert run.batch.stats().graph == 1
-            for a, b in zip(res, eager):
-                assert rows_of(a) == rows_of(b), rep
+            for (t, r), a, b in zip(pairs, res, eager):  # DOUBLE sums add in run-dependent order
+                assert_same_rows(rows_of(a), rows_of(b), float_cols(t, r), REL, len(r.group_cols), None)
             q = try_build_gpu_group(session, *typed_pairs()[1])  # other work between replays
             q.next()
             q.deinit()

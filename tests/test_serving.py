@@ -1,0 +1,112 @@
+"""The device request loop (tdbg_server_*), through the C-ABI: every request's
+result is the same alone and in a concurrent mix that the server batches,
+overlaps on its lanes and admits by estimated device bytes."""
+import numpy as np
+import pytest
+
+from helpers import assert_same_rows, rows_of
+from thindb_amd import gen, gpu
+from thindb_amd.exec import And, AggSpec, Batch, Predicate, Request, SortSpec, build_plan, output_schema
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+def mix(session):
+    """Small batched-workload tables, two medium tables and one large filtered
+    query with ORDER BY ... LIMIT (runs alone), with their plans."""
+    items = []
+    for t in range(24):
+        table = gen.batched_table(21, t, 60_000)
+        items.append((table, gen.batched_request(21, t)))
+    for t in (2, 5):
+        table = gen.batched_table(22, t, 2_000_000)
+        items.append((table, gen.batched_request(22, t)))
+    big = gen.table("filtered", 3_000_000, seed=9)
+    where = And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI)))
+    items.append((big, Request(["region"], [AggSpec("count", None, "c"), AggSpec("sum", "amount", "s"),
+                                            AggSpec("avg", "price", "a")], [SortSpec("s", True)], 100,
+                               where_filter=where)))
+    plans = []
+    for table, req in items:
+        plan, cols = build_plan(session, table, req)
+        session.stage_table(table, cols)
+        plans.append(plan)
+    return items, plans
+
+
+def alone(items, plans):
+    out = []
+    for (table, req), p in zip(items, plans):
+        cols = p.execute()
+        out.append(rows_of(Batch(output_schema(table, req), cols, len(cols[0].values))))
+    return out
+
+
+def float_cols(table, req):
+    from test_gpu_parity import float_cols_of
+    return float_cols_of(table, req)
+
+
+def compare(items, got, want):
+    for (table, req), g, w in zip(items, got, want):
+        names = [c.name.lower() for c in output_schema(table, req)]
+        order = [names.index(s.col.lower()) for s in req.order_specs] or None
+        assert_same_rows(g, w, float_cols(table, req), REL, len(req.group_cols), order)
+
+
+def test_same_alone_and_in_the_mix(session):
+    items, plans = mix(session)
+    want = alone(items, plans)
+    print(f"[gpu] serving mix of {len(items)} requests, 3 rounds", flush=True)
+    srv = gpu.Server(session.ctx, lanes=3, max_batch=16)
+    try:
+        for rnd in range(3):
+            order = np.random.default_rng(rnd).permutation(len(items))
+            tickets = [(i, srv.submit(plans[i])) for i in order]  # all at once: queued behind running units
+            got = [None] * len(items)
+            sizes = []
+            for i, t in tickets:
+                cols = t.wait()
+                table, req = items[i]
+                got[i] = rows_of(Batch(output_schema(table, req), cols, len(cols[0].values)))
+                rep = t.report()
+                assert rep.status == 0 and rep.done_ns >= rep.start_ns >= rep.submit_ns
+                sizes.append(rep.batch_size)
+                t.close()
+            compare(items, got, want)
+            print(f"  round {rnd}: batch sizes {sorted(set(sizes))}", flush=True)
+        c = srv.counters()
+        assert c.completed == c.submitted == 3 * len(items)
+        assert c.batched > 0, "queued small requests share launches"
+    finally:
+        srv.stop()
+        for p in plans:
+            p.close()
+
+
+def test_admission_by_device_bytes(session):
+    items, plans = mix(session)
+    want = alone(items, plans)
+    # room for about one medium request at a time: later arrivals wait for memory
+    est = []
+    srv = gpu.Server(session.ctx, lanes=3, max_batch=8, admit_bytes=64 << 20)
+    try:
+        tickets = [srv.submit(p) for p in plans]
+        got = []
+        for (table, req), t in zip(items, tickets):
+            cols = t.wait()
+            est.append(t.report().est_bytes)
+            got.append(rows_of(Batch(output_schema(table, req), cols, len(cols[0].values))))
+            t.close()
+        compare(items, got, want)
+        c = srv.counters()
+        print(f"[gpu] admission: peak {c.peak_admitted >> 20} MiB of 64, waits {c.admission_waits}, "
+              f"largest estimate {max(est) >> 20} MiB", flush=True)
+        assert c.admission_waits > 0
+        assert c.peak_admitted <= max(64 << 20, max(est))
+    finally:
+        srv.stop()
+        for p in plans:
+            p.close()

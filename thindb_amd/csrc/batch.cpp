@@ -73,6 +73,7 @@ struct tdbg_batch {
   uint64_t zero_words = 0, out_bytes = 0, err_off = 0, total_off = 0;
   DevBuf d_q, d_items, d_tiles, d_rec, d_rows, d_misc, d_zero, d_cblocks, d_fblocks, d_counts, d_gids, d_out;
   std::shared_ptr<uint8_t> h_out;  // pinned; results keep the buffer they view
+  uint64_t h_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   tdbg_batch_stats last{};
@@ -80,7 +81,9 @@ struct tdbg_batch {
 
 namespace {
 
+// Grows `d` to at least `bytes` (buffers are reused across builds).
 int32_t dev_take(tdbg_batch* b, DevBuf& d, uint64_t bytes) {
+  if (d.p && d.bytes >= bytes) return TDBG_OK;
   dev_free(b->ctx, d.p, d.bytes);
   d = DevBuf{};
   const uint64_t want = std::max<uint64_t>(bytes, 256);
@@ -90,11 +93,17 @@ int32_t dev_take(tdbg_batch* b, DevBuf& d, uint64_t bytes) {
   return TDBG_OK;
 }
 
+// Drops the captured graph (the plan must be rebuilt); buffers stay.
 void release_built(tdbg_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
+  b->built = false;
+}
+
+void release_buffers(tdbg_batch* b) {
+  release_built(b);
   for (DevBuf* d : {&b->d_q, &b->d_items, &b->d_tiles, &b->d_rec, &b->d_rows, &b->d_misc, &b->d_zero, &b->d_cblocks,
                     &b->d_fblocks, &b->d_counts, &b->d_gids, &b->d_out}) {
     dev_free(b->ctx, d->p, d->bytes);
@@ -312,7 +321,7 @@ int32_t build(tdbg_batch* b) {
       (rc = dev_take(b, b->d_fblocks, sizeof(BatchBlock) * fblocks.size())) ||
       (rc = dev_take(b, b->d_counts, b->total_off + 64)) || (rc = dev_take(b, b->d_gids, gids_b + 256)) ||
       (rc = dev_take(b, b->d_out, b->out_bytes))) {
-    release_built(b);
+    release_buffers(b);
     return rc;
   }
   b->zero_words = zero_b / 4;
@@ -386,11 +395,13 @@ int32_t run(tdbg_batch* b) {
   }
   if (!b->built && (rc = build(b))) return rc;
   HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
-  if (!b->h_out || b->h_out.use_count() > 1) {
+  if (!b->h_out || b->h_out.use_count() > 1 || b->h_cap < b->out_bytes) {
     // results of an earlier execution still view the old buffer: keep it
     void* h = nullptr;
-    HIP_TRY(hipHostMalloc(&h, b->out_bytes, hipHostMallocDefault), "pinned result buffer");
+    const uint64_t cap = std::max<uint64_t>(b->out_bytes, b->h_out ? b->h_cap : 0);
+    HIP_TRY(hipHostMalloc(&h, cap, hipHostMallocDefault), "pinned result buffer");
     b->h_out.reset(static_cast<uint8_t*>(h), [](uint8_t* p) { (void)hipHostFree(p); });
+    b->h_cap = cap;
   }
   hipStream_t s = b->stream;
   std::shared_lock<std::shared_mutex> lk(b->ctx->mu);  // staged columns stay put while the kernels read them
@@ -443,7 +454,7 @@ void free_batch(tdbg_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->ctx->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  release_built(b);
+  release_buffers(b);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);
@@ -451,6 +462,35 @@ void free_batch(tdbg_batch* b) {
 }
 
 }  // namespace
+
+namespace tdbg {
+namespace rt {
+
+int32_t batch_open(tdbg_context* ctx, tdbg_batch** out) {
+  tdbg_plan* none = nullptr;
+  int32_t rc = make_batch(ctx, &none, 0, nullptr, out);
+  if (rc == TDBG_OK) (*out)->use_graph = false;
+  return rc;
+}
+
+int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n) {
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->plans.assign(plans, plans + n);
+  release_built(b);
+  return run(b);
+}
+
+int32_t batch_copy_result(tdbg_batch* b, uint32_t i, tdbg_result** out) {
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (!b->built || !b->h_out || i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
+  const uint64_t n_out = reinterpret_cast<const uint64_t*>(b->h_out.get())[i];
+  return make_result(b->plans[i], n_out, b->h_out.get(), b->data_off[i], b->valid_off[i], nullptr, out);
+}
+
+void batch_close(tdbg_batch* b) { free_batch(b); }
+
+}  // namespace rt
+}  // namespace tdbg
 
 extern "C" {
 

@@ -70,10 +70,17 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kernel(const BatchQuery* 
     const ScanParams& P = B.sp;
     switch (it.kind) {
       case BI_SCAN: {
+        // columns held in registers when the plan reads at most kFastCols
         const uint64_t part_lo = (uint64_t)it.part * P.lds_cap;
-        if (P.lds_mode == LDS_DIRECT) scan_range<LDS_DIRECT, GenericStrip<true>>(P, lds, it.a0, it.a1, part_lo);
-        else if (P.lds_mode == LDS_HASH) scan_range<LDS_HASH, GenericStrip<true>>(P, lds, it.a0, it.a1, 0);
-        else scan_range<LDS_NONE, GenericStrip<true>>(P, lds, it.a0, it.a1, 0);
+        if (P.n_cols <= (uint32_t)kFastCols) {
+          if (P.lds_mode == LDS_DIRECT) scan_range<LDS_DIRECT, GenericStrip<true>>(P, lds, it.a0, it.a1, part_lo);
+          else if (P.lds_mode == LDS_HASH) scan_range<LDS_HASH, GenericStrip<true>>(P, lds, it.a0, it.a1, 0);
+          else scan_range<LDS_NONE, GenericStrip<true>>(P, lds, it.a0, it.a1, 0);
+        } else {
+          if (P.lds_mode == LDS_DIRECT) scan_range<LDS_DIRECT, GenericStrip<false>>(P, lds, it.a0, it.a1, part_lo);
+          else if (P.lds_mode == LDS_HASH) scan_range<LDS_HASH, GenericStrip<false>>(P, lds, it.a0, it.a1, 0);
+          else scan_range<LDS_NONE, GenericStrip<false>>(P, lds, it.a0, it.a1, 0);
+        }
         break;
       }
       case BI_RX_HIST: {

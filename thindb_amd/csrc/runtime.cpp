@@ -535,7 +535,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
       const uint64_t rows_per_wg = ((uint64_t)n_tiles + groups - 1) / groups * kTileRows;
       const uint32_t sb = slot_bytes_for(rows_per_wg);
       const uint64_t cap = (dom + p - 1) / p;
-      if (cap * sb <= budget) {
+      if (cap * sb + 8 * (2 * sp.n_aggs + 3) + 16 <= budget) {  // + alignment of each LDS array
         sp.lds_mode = LDS_DIRECT;
         lds_cap = (uint32_t)cap;
         parts = p;
@@ -545,7 +545,9 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
   }
   if (sp.lds_mode == LDS_NONE && (strat == S_AUTO || strat == S_LDS_HASH)) {
     const uint32_t sb = slot_bytes_for(((uint64_t)n_tiles + n_cu - 1) / n_cu * kTileRows);
-    const uint32_t cap = budget / (sb + key_bytes);
+    // every LDS array starts 8-byte aligned and the layout ends 16-byte aligned
+    const uint32_t pad = 8 * (2 * sp.n_aggs + 3) + 16;
+    const uint32_t cap = (budget - pad) / (sb + key_bytes);
     if ((strat == S_LDS_HASH || est <= 8ull * cap) && cap >= 256) {
       sp.lds_mode = LDS_HASH;
       lds_cap = cap;
@@ -1034,11 +1036,12 @@ extern "C" {
 int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out) {
   if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
-  std::lock_guard<std::mutex> plk(pl->mu);
-  pl->cancel.store(false);
   tdbg_context* ctx = pl->ctx;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
+  // lock order everywhere: the context's table lock, then the plan's
   std::shared_lock<std::shared_mutex> lk(ctx->mu);
+  std::lock_guard<std::mutex> plk(pl->mu);
+  pl->cancel.store(false);
   pl->last = tdbg_exec_stats{};
   auto tit = ctx->tables.find(pl->table_uid);
   if (tit == ctx->tables.end()) return fail(TDBG_ERR_NOT_STAGED, "table %llu has no staged columns", (unsigned long long)pl->table_uid);

@@ -209,5 +209,12 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& s, const PlanHint& hint)
 int32_t make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host, const std::vector<uint64_t>& data_off,
                     const std::vector<uint64_t>& valid_off, std::shared_ptr<void> keep, tdbg_result** out);
 
+// A reusable batch for the serving loop (batch.cpp): eager launches, device
+// and pinned buffers kept and grown across runs of different plan sets.
+int32_t batch_open(tdbg_context* ctx, tdbg_batch** out);
+int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n);
+int32_t batch_copy_result(tdbg_batch* b, uint32_t i, tdbg_result** out);  // an owned copy
+void batch_close(tdbg_batch* b);
+
 }  // namespace rt
 }  // namespace tdbg

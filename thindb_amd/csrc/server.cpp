@@ -1,0 +1,289 @@
+// server.cpp — the device request loop (tdbg_server_*): admission by
+// estimated device bytes, continuous batching of small requests into the
+// persistent batched launch, and lanes that keep several units of work on the
+// GPU at once.
+//
+// Lane loop: take the queue head; if it is large (many rows, or ORDER BY /
+// LIMIT / OFFSET) run it alone through the single-query operator, else take
+// every small request queued behind it (up to max_batch, stopping at a large
+// one) and run them as one batch.  A unit is admitted only while the
+// estimated bytes of the running units stay within admit_bytes (a unit always
+// runs when nothing else does, so an oversized request cannot starve).
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
+
+#include "runtime.h"
+
+using namespace tdbg;
+using namespace tdbg::rt;
+
+namespace {
+
+int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace
+
+struct tdbg_ticket {
+  tdbg_plan* plan = nullptr;
+  uint64_t est = 0;
+  bool large = false;
+  std::mutex m;
+  std::condition_variable cv;
+  bool done = false;
+  int32_t rc = TDBG_PENDING;
+  std::string err;
+  tdbg_result* result = nullptr;
+  int64_t t_submit = 0, t_start = 0, t_done = 0;
+  uint32_t batch_size = 0, lane = 0;
+  std::atomic<int> refs{2};  // the caller's handle and the server's
+};
+
+struct tdbg_server {
+  tdbg_context* ctx = nullptr;
+  tdbg_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<tdbg_ticket*> queue;
+  uint64_t admitted = 0;
+  bool stopping = false;
+  std::vector<std::thread> lanes;
+  std::vector<tdbg_batch*> batches;  // one reusable batch per lane
+  tdbg_server_counters stats{};
+};
+
+namespace {
+
+void unref(tdbg_ticket* t) {
+  if (t->refs.fetch_sub(1) == 1) {
+    if (t->result) tdbg_result_free(t->result);
+    delete t;
+  }
+}
+
+void finish(tdbg_ticket* t, int32_t rc, tdbg_result* r, uint32_t batch_size, uint32_t lane) {
+  {
+    std::lock_guard<std::mutex> lk(t->m);
+    t->rc = rc;
+    if (rc != TDBG_OK) t->err = tdbg_last_error();
+    t->result = r;
+    t->batch_size = batch_size;
+    t->lane = lane;
+    t->t_done = now_ns();
+    t->done = true;
+  }
+  t->cv.notify_all();
+  unref(t);
+}
+
+// Estimated device bytes of one execution: the group table, radix partition
+// rows when the key domain is wide and well filled, tiles and the result.
+uint64_t estimate(tdbg_context* ctx, const tdbg_plan* pl, uint64_t* rows_out) {
+  std::shared_lock<std::shared_mutex> lk(ctx->mu);
+  *rows_out = 0;
+  auto it = ctx->tables.find(pl->table_uid);
+  if (it == ctx->tables.end()) return 0;
+  const Table* t = it->second.get();
+  uint64_t rows = 0;
+  for (auto& rg : t->rgs) rows += rg.n_rows;
+  *rows_out = rows;
+  unsigned __int128 dom = 1;
+  for (auto& k : pl->keys) {
+    auto c = t->cols.find(k.column_idx);
+    const uint64_t span = c != t->cols.end() && c->second.has_stats
+                              ? (uint64_t)c->second.max_i - (uint64_t)c->second.min_i + 1
+                              : ~0ull;
+    dom *= span;
+    if (dom > ((unsigned __int128)1 << 62)) dom = (unsigned __int128)1 << 62;
+  }
+  const uint64_t d = (uint64_t)dom;
+  const uint64_t groups = std::max<uint64_t>(1, std::min<uint64_t>(pl->est_groups ? pl->est_groups : d, rows));
+  const uint64_t rec = 16 + 24 * pl->aggs.size();
+  const uint64_t table = (d <= kDenseMax ? d : next_pow2(groups + groups / 2 + 1)) * rec;
+  const uint64_t radix = (d > 65536 && rows >= (1u << 20) && d / 4 <= rows) ? rows * 32 : 0;
+  const uint64_t out = groups * 9 * (pl->keys.size() + pl->aggs.size());
+  return table + radix + out + (rows / kTileRows + 1) * sizeof(Tile);
+}
+
+void lane_main(tdbg_server* s, uint32_t lane) {
+  (void)hipSetDevice(s->ctx->device);
+  std::vector<tdbg_ticket*> unit;
+  std::vector<tdbg_plan*> plans;
+  for (;;) {
+    uint64_t est = 0;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      for (;;) {
+        if (!s->queue.empty()) {
+          tdbg_ticket* head = s->queue.front();
+          if (s->admitted != 0 && s->admitted + head->est > s->cfg.admit_bytes) {
+            s->stats.admission_waits++;
+            s->cv.wait(lk);  // a running unit frees memory
+            continue;
+          }
+          unit.clear();
+          if (head->large) {
+            unit.push_back(head);
+            est = head->est;
+            s->queue.pop_front();
+          } else {
+            while (!s->queue.empty() && unit.size() < s->cfg.max_batch) {
+              tdbg_ticket* t = s->queue.front();
+              if (t->large || (!unit.empty() && s->admitted + est + t->est > s->cfg.admit_bytes)) break;
+              unit.push_back(t);
+              est += t->est;
+              s->queue.pop_front();
+            }
+          }
+          s->admitted += est;
+          s->stats.peak_admitted = std::max(s->stats.peak_admitted, s->admitted);
+          break;
+        }
+        if (s->stopping) return;
+        s->cv.wait(lk);
+      }
+    }
+    const int64_t t0 = now_ns();
+    for (tdbg_ticket* t : unit) t->t_start = t0;
+    if (unit.size() == 1) {
+      tdbg_result* r = nullptr;
+      const int32_t rc = tdbg_execute(unit[0]->plan, nullptr, 0, &r);
+      finish(unit[0], rc, rc == TDBG_OK ? r : nullptr, 1, lane);
+    } else {
+      plans.clear();
+      for (tdbg_ticket* t : unit) plans.push_back(t->plan);
+      int32_t rc = batch_run_plans(s->batches[lane], plans.data(), (uint32_t)plans.size());
+      for (size_t i = 0; i < unit.size(); i++) {
+        tdbg_result* r = nullptr;
+        int32_t ri = rc;
+        if (rc == TDBG_OK) ri = batch_copy_result(s->batches[lane], (uint32_t)i, &r);
+        finish(unit[i], ri, ri == TDBG_OK ? r : nullptr, (uint32_t)unit.size(), lane);
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->admitted -= est;
+      s->stats.completed += unit.size();
+      s->stats.units++;
+      if (unit.size() > 1) s->stats.batched += unit.size();
+    }
+    s->cv.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t tdbg_server_start(tdbg_context* ctx, const tdbg_server_config* cfg, tdbg_server** out) {
+  if (!ctx || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::unique_ptr<tdbg_server> s(new tdbg_server());
+  s->ctx = ctx;
+  if (cfg) s->cfg = *cfg;
+  if (!s->cfg.lanes) s->cfg.lanes = 3;
+  if (!s->cfg.max_batch) s->cfg.max_batch = 64;
+  if (!s->cfg.admit_bytes) s->cfg.admit_bytes = ctx->budget / 4;
+  if (!s->cfg.large_rows) s->cfg.large_rows = 16u << 20;
+  if (s->cfg.lanes > 16) return fail(TDBG_ERR_INVALID, "%u lanes (at most 16)", s->cfg.lanes);
+  for (uint32_t i = 0; i < s->cfg.lanes; i++) {
+    tdbg_batch* b = nullptr;
+    int32_t rc = batch_open(ctx, &b);
+    if (rc) {
+      for (tdbg_batch* x : s->batches) batch_close(x);
+      return rc;
+    }
+    s->batches.push_back(b);
+  }
+  for (uint32_t i = 0; i < s->cfg.lanes; i++) s->lanes.emplace_back(lane_main, s.get(), i);
+  *out = s.release();
+  return TDBG_OK;
+}
+
+int32_t tdbg_server_submit(tdbg_server* s, tdbg_plan* plan, tdbg_ticket** out) {
+  if (!s || !plan || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (plan->ctx != s->ctx) return fail(TDBG_ERR_INVALID, "plan of another context");
+  auto* t = new tdbg_ticket();
+  t->plan = plan;
+  t->t_submit = now_ns();
+  uint64_t rows = 0;
+  t->est = estimate(s->ctx, plan, &rows);
+  t->large = rows >= s->cfg.large_rows || !plan->order.empty() || plan->limit || plan->offset;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stopping) {
+      delete t;
+      return fail(TDBG_ERR_INVALID, "server is stopping");
+    }
+    s->queue.push_back(t);
+    s->stats.submitted++;
+  }
+  s->cv.notify_one();
+  *out = t;
+  return TDBG_OK;
+}
+
+int32_t tdbg_ticket_poll(tdbg_ticket* t) {
+  if (!t) return fail(TDBG_ERR_INVALID, "null ticket");
+  std::lock_guard<std::mutex> lk(t->m);
+  if (!t->done) return TDBG_PENDING;
+  if (t->rc != TDBG_OK) fail(t->rc, "%s", t->err.c_str());
+  return t->rc;
+}
+
+int32_t tdbg_ticket_wait(tdbg_ticket* t, tdbg_result** out) {
+  if (!t || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::unique_lock<std::mutex> lk(t->m);
+  t->cv.wait(lk, [&] { return t->done; });
+  if (t->rc != TDBG_OK) return fail(t->rc, "%s", t->err.c_str());
+  *out = t->result;
+  t->result = nullptr;
+  return TDBG_OK;
+}
+
+int32_t tdbg_ticket_info(const tdbg_ticket* t, tdbg_ticket_report* out) {
+  if (!t || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  auto* mt = const_cast<tdbg_ticket*>(t);
+  std::lock_guard<std::mutex> lk(mt->m);
+  *out = tdbg_ticket_report{};
+  out->submit_ns = t->t_submit;
+  out->start_ns = t->t_start;
+  out->done_ns = t->t_done;
+  out->est_bytes = t->est;
+  out->batch_size = t->batch_size;
+  out->lane = t->lane;
+  out->status = t->done ? t->rc : TDBG_PENDING;
+  return TDBG_OK;
+}
+
+void tdbg_ticket_free(tdbg_ticket* t) {
+  if (t) unref(t);
+}
+
+int32_t tdbg_server_stats(const tdbg_server* s, tdbg_server_counters* out) {
+  if (!s || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  auto* ms = const_cast<tdbg_server*>(s);
+  std::lock_guard<std::mutex> lk(ms->mu);
+  *out = s->stats;
+  return TDBG_OK;
+}
+
+void tdbg_server_stop(tdbg_server* s) {
+  if (!s) return;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stopping = true;
+  }
+  s->cv.notify_all();
+  for (auto& th : s->lanes) th.join();
+  for (tdbg_batch* b : s->batches) batch_close(b);
+  delete s;
+}
+
+}  // extern "C"

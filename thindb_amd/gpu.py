@@ -476,3 +476,101 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("lanes", C.c_uint32), ("max_batch", C.c_uint32), ("admit_bytes", C.c_uint64),
+                ("large_rows", C.c_uint64)]
+
+
+class TicketReport(C.Structure):
+    _fields_ = [("submit_ns", C.c_int64), ("start_ns", C.c_int64), ("done_ns", C.c_int64), ("est_bytes", C.c_uint64),
+                ("batch_size", C.c_uint32), ("lane", C.c_uint32), ("status", C.c_int32), ("pad", C.c_uint32)]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("units", C.c_uint64),
+                ("batched", C.c_uint64), ("peak_admitted", C.c_uint64), ("admission_waits", C.c_uint64)]
+
+
+EXPORTS.update({
+    "tdbg_server_start": (C.c_int32, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "tdbg_server_submit": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "tdbg_ticket_poll": (C.c_int32, [C.c_void_p]),
+    "tdbg_ticket_wait": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "tdbg_ticket_info": (C.c_int32, [C.c_void_p, C.POINTER(TicketReport)]),
+    "tdbg_ticket_free": (None, [C.c_void_p]),
+    "tdbg_server_stats": (C.c_int32, [C.c_void_p, C.POINTER(ServerCounters)]),
+    "tdbg_server_stop": (None, [C.c_void_p]),
+})
+
+
+class Ticket:
+    """One submitted request (tdbg_ticket)."""
+
+    def __init__(self, lib, h: C.c_void_p):
+        self._lib = lib
+        self._h = h
+
+    def done(self) -> bool:
+        rc = self._lib.tdbg_ticket_poll(self._h)
+        if rc == Status.PENDING:
+            return False
+        check(rc)
+        return True
+
+    def wait(self) -> list:
+        """Blocks until the request has run; returns its result columns."""
+        out = C.c_void_p()
+        check(self._lib.tdbg_ticket_wait(self._h, C.byref(out)))
+        return _result_columns(self._lib, out)
+
+    def report(self) -> TicketReport:
+        r = TicketReport()
+        check(self._lib.tdbg_ticket_info(self._h, C.byref(r)))
+        return r
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.tdbg_ticket_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The device request loop (tdbg_server): admission, continuous batching, lanes."""
+
+    def __init__(self, ctx: Context, lanes: int = 0, max_batch: int = 0, admit_bytes: int = 0, large_rows: int = 0):
+        lib = load()
+        self._lib = lib
+        self.ctx = ctx
+        cfg = ServerConfig(lanes, max_batch, admit_bytes, large_rows)
+        h = C.c_void_p()
+        check(lib.tdbg_server_start(ctx.handle, C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def submit(self, plan: Plan) -> Ticket:
+        t = C.c_void_p()
+        check(self._lib.tdbg_server_submit(self._h, plan.handle, C.byref(t)))
+        return Ticket(self._lib, t)
+
+    def counters(self) -> ServerCounters:
+        c = ServerCounters()
+        check(self._lib.tdbg_server_stats(self._h, C.byref(c)))
+        return c
+
+    def stop(self) -> None:
+        if self._h:
+            self._lib.tdbg_server_stop(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass
