@@ -49,6 +49,7 @@ WORKLOADS = {
     "plain": (4_000_000, 16, "GROUP BY k COUNT/SUM/MIN/MAX, 4M rows"),
     "highcard": (200_000_000, 21, "GROUP BY (user_id, bucket) COUNT/SUM/MAX top-10 by COUNT, 200M rows"),
     "batched": (0, 12, "512 GROUP BY key COUNT/SUM/MIN/MAX queries over 512 uneven tables, one batch"),
+    "serving": (0, 0, "serving loop under Poisson arrivals"),
 }
 BATCH_SEED = 7
 
@@ -160,19 +161,37 @@ def run_batched(args) -> int:
         f"device {st.total_ms:.2f} ms, {st.n_items} items, graph {st.graph}, results {st.result_bytes / 1e6:.1f} MB")
 
     # the persistent kernel alone: the same plans as an eagerly launched batch
-    os.environ["THINDB_GPU_GRAPH"] = "0"
+    os.environ["THINDB_GPU_GRAPH"] = "0"  # read when the batch is planned, at its first execution
     eager = gpu.Batch(sess.ctx, plans)
-    del os.environ["THINDB_GPU_GRAPH"]
     kern = []
     for _ in range(3):
         eager.execute()
         kern.append(eager.stats().scan_ms)
-    est = eager.stats()
+    del os.environ["THINDB_GPU_GRAPH"]
     eager.close()
     kernel_ms = float(np.median(kern))
     alg_bytes = int(st.bytes)
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     log(f"persistent kernel {kernel_ms:.3f} ms for {alg_bytes / 1e9:.2f} GB ({achieved:.0f} GB/s)")
+    # the same kernel over each strategy's plans alone
+    breakdown = {}
+    for k, name in gpu.Batch.STRATEGIES.items():
+        sub = [p for p, r in zip(plans, routes) if r.strategy == k]
+        if not sub:
+            continue
+        os.environ["THINDB_GPU_GRAPH"] = "0"
+        b = gpu.Batch(sess.ctx, sub)
+        ms = []
+        for _ in range(3):
+            b.execute()
+            ms.append(b.stats().scan_ms)
+        del os.environ["THINDB_GPU_GRAPH"]
+        bs = b.stats()
+        b.close()
+        breakdown[name] = {"plans": len(sub), "rows": int(bs.rows), "kernel_ms": float(np.median(ms)),
+                           "gbps": bs.bytes / (float(np.median(ms)) / 1e3) / 1e9}
+        log(f"  {name}: {len(sub)} plans, {bs.rows / 1e6:.1f}M rows, kernel {np.median(ms):.3f} ms "
+            f"({breakdown[name]['gbps']:.0f} GB/s)")
 
     cpu = None
     if not args.no_cpu:
@@ -207,12 +226,159 @@ def run_batched(args) -> int:
         "device_ms_per_step": st.total_ms,
         "result_bytes": int(st.result_bytes),
         "autotune_s": tune_s,
+        "kernel_by_strategy": breakdown,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "batch_kernel",
                      "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
     run.close()
+    sess.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+SERVE_RATES = (100, 250, 500, 1000, 2500, 5000)
+
+
+def run_serving(args) -> int:
+    """The serving loop under Poisson arrivals: 70% small (batched-workload
+    tables of 1e4-1e6 rows), 25% medium (1e7 rows, 1e5 groups), 5% large (the
+    200M-row filtered query).  The served form (tdbg_server: continuous
+    batching, 3 lanes, admission by estimated bytes) against the plain
+    operator behind a FIFO queue (one lane, no batching), both through the
+    same C-ABI; latency = completion - arrival (steady clock)."""
+    import threading
+
+    from thindb_amd import gen, gpu
+    from thindb_amd.exec import And, AggSpec, GpuSession, Predicate, Request, SortSpec, build_plan
+
+    rng = np.random.default_rng(41)
+    sess = GpuSession(0)
+    t0 = time.perf_counter()
+    classes = {"small": [], "medium": [], "large": []}
+    for t in range(48):
+        n = int(round(np.exp(rng.uniform(np.log(1e4), np.log(1e6)))))
+        classes["small"].append((gen.batched_table(41, t, n), gen.batched_request(41, t)))
+    for t in range(6):
+        classes["medium"].append((gen.keyed_table(100 + t, 10_000_000, 100_000), gen.batched_request(41, t)))
+    big = gen.table("filtered", args.rows or 200_000_000)
+    classes["large"].append((big, make_request("filtered")))
+    plans = {k: [] for k in classes}
+    staged = 0
+    for k, pairs in classes.items():
+        for table, req in pairs:
+            plan, cols = build_plan(sess, table, req)
+            staged += sess.stage_table(table, cols)
+            plans[k].append(plan)
+    log(f"resident: {staged / 1e9:.2f} GB in {sum(len(v) for v in plans.values())} tables, "
+        f"{time.perf_counter() - t0:.1f}s")
+
+    def draw(n, seed):
+        r = np.random.default_rng(seed)
+        u = r.random(n)
+        out = []
+        for x in u:
+            k = "small" if x < 0.70 else ("medium" if x < 0.95 else "large")
+            out.append(plans[k][r.integers(len(plans[k]))])
+        return out
+
+    # single-query latency of each class, alone on an idle GPU
+    # (submission to completion through the plain operator, one request at a time)
+    single = {}
+    solo = gpu.Server(sess.ctx, lanes=1, max_batch=1)
+    for k, ps in plans.items():
+        lat = []
+        for p in ps[:8]:
+            for rep in range(4):
+                t = solo.submit(p)
+                t.wait()
+                r = t.report()
+                t.close()
+                if rep:
+                    lat.append((r.done_ns - r.submit_ns) / 1e9)
+        single[k] = float(np.median(lat)) * 1e3
+    solo.stop()
+    mix_single = 0.70 * single["small"] + 0.25 * single["medium"] + 0.05 * single["large"]
+    log("single-query latency ms: " + ", ".join(f"{k} {v:.3f}" for k, v in single.items()) +
+        f"; mix-weighted {mix_single:.3f}")
+
+    def drive(server, reqs, rate):
+        """Submits reqs at Poisson arrival times (rate/s; 0 = all at once)."""
+        r = np.random.default_rng(int(rate) + 7)
+        gaps = r.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+        times = np.cumsum(gaps)
+        tickets = []
+        start = time.perf_counter()
+        for p, at in zip(reqs, times):
+            while time.perf_counter() - start < at:
+                pass
+            tickets.append(server.submit(p))
+        lat, first, last = [], None, None
+        units = {}
+        for t in tickets:
+            t.wait()
+            rep = t.report()
+            units[(rep.lane, rep.start_ns)] = (rep.batch_size, (rep.done_ns - rep.start_ns) / 1e6)
+            lat.append((rep.done_ns - rep.submit_ns) / 1e6)
+            first = rep.submit_ns if first is None else min(first, rep.submit_ns)
+            last = rep.done_ns if last is None else max(last, rep.done_ns)
+            t.close()
+        lat = np.array(lat)
+        # launches by size: how many, mean duration (ms)
+        by = {}
+        for size, ms in units.values():
+            k = "1" if size == 1 else ("2-8" if size <= 8 else ("9-32" if size <= 32 else "33+"))
+            n, tot = by.get(k, (0, 0.0))
+            by[k] = (n + 1, tot + ms)
+        return {"offered_qps": rate, "throughput_qps": len(reqs) / ((last - first) / 1e9),
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()),
+                "launches": {k: {"count": n, "mean_ms": tot / n} for k, (n, tot) in sorted(by.items())}}
+
+    results = {"served": [], "plain": []}
+    for name, cfg in (("served", dict(lanes=3, max_batch=64)), ("plain", dict(lanes=1, max_batch=1))):
+        srv = gpu.Server(sess.ctx, **cfg)
+        drive(srv, draw(200, 1), 0)  # warm-up: plans, buffers, kernels
+        for rate in SERVE_RATES:
+            n = int(min(max(rate * args.serve_seconds, 200), 6000))
+            res = drive(srv, draw(n, rate), rate)
+            res["requests"] = n
+            results[name].append(res)
+            log(f"{name} {rate}/s: {res['throughput_qps']:.0f} q/s, p50 {res['p50_ms']:.2f} ms, "
+                f"p99 {res['p99_ms']:.2f} ms")
+        sat = drive(srv, draw(3000, 99), 0)
+        sat["requests"] = 3000
+        results[name + "_saturated"] = sat
+        c = srv.counters()
+        log(f"{name} saturated: {sat['throughput_qps']:.0f} q/s, p50 {sat['p50_ms']:.2f} ms, p99 {sat['p99_ms']:.2f} ms; "
+            f"units {c.units}, batched {c.batched}, admission waits {c.admission_waits}; launches {sat['launches']}")
+        srv.stop()
+    served, plain = results["served_saturated"], results["plain_saturated"]
+    out = {
+        "metric": "rows/s & HBM GB/s for filtered GROUP BY; batched QPS + p50/p99 latency",
+        "value": served["throughput_qps"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": 1e3 / served["throughput_qps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded counter-based generator)",
+        "config": {"workload": "serving: Poisson arrivals, 70% small / 25% medium / 5% large (200M-row filtered)",
+                   "resident_gb": staged / 1e9, "rates": list(SERVE_RATES)},
+        "single_query_ms": single,
+        "sweep": results,
+        "speedup_vs_plain_saturated": served["throughput_qps"] / plain["throughput_qps"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    for ps in plans.values():
+        for p in ps:
+            p.close()
     sess.close()
     print(json.dumps(out), flush=True)
     return 0
@@ -227,6 +393,7 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=0,
                     help="override the row count (batched: cap per table; testing only)")
     ap.add_argument("--tables", type=int, default=512, help="batched: number of tables/queries")
+    ap.add_argument("--serve-seconds", type=float, default=1.5, help="serving: seconds of arrivals per rate")
     ap.add_argument("--cpu-sample-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -243,6 +410,8 @@ def main() -> int:
 
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "serving":
+        return run_serving(args)
 
     if args.copy_calibration:
         import torch

@@ -242,17 +242,20 @@ def test_radix_two_levels_at_width(session, oracle_lib, monkeypatch):
     assert st.global_mode == 3 and st.n_groups == len(want)
 
 
-@pytest.mark.parametrize("leaf_bits", ["3", "1", "32"])
-def test_radix_partition_passes(session, oracle_lib, monkeypatch, leaf_bits):
-    """The partition passes forced onto a small typed table: one level
-    (leaf bits 3: 2^7 buckets), two levels (leaf bits 1: 2^8 x 2), and the
-    LDS-budget leaf; NULL inputs, filters, every aggregate kind, ragged row
-    groups split into several staged tiles."""
+@pytest.mark.parametrize("leaf_bits,bits1", [("3", ""), ("1", "5"), ("32", "")])
+def test_radix_partition_passes(session, oracle_lib, monkeypatch, leaf_bits, bits1):
+    """The partition passes forced onto a small typed table (a 700-key
+    domain): one level (leaf bits 3: 2^7 buckets), two levels (leaf bits 1,
+    level-1 digit of 5 bits: 2^5 x 2^4), and the default leaf; NULL inputs,
+    filters, every aggregate kind, ragged row groups split into several
+    staged tiles."""
     rng = np.random.default_rng(11)
     t = random_table(rng, 60_000, rg_rows=7001, nulls=True)
     where = And((Predicate("i", "gt", -2 ** 30), Predicate("s", "neq", 0)))
     monkeypatch.setenv("THINDB_GPU_RADIX", "1")
     monkeypatch.setenv("THINDB_GPU_RADIX_LEAF_BITS", leaf_bits)
+    if bits1:
+        monkeypatch.setenv("THINDB_GPU_RADIX_BITS1", bits1)
     for req in [Request(["k1", "k2"], ALL_AGGS, where_filter=where),
                 Request(["k1", "k2"], ALL_AGGS, [SortSpec("c", True)], limit=25),
                 Request(["k2", "k1"], ALL_AGGS, [SortSpec("sb", True), SortSpec("k1")], limit=40, offset=3),

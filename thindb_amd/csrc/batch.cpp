@@ -21,7 +21,7 @@ namespace {
 constexpr uint64_t kItemRows = 1ull << 17;     // rows per scan / partition work item
 constexpr uint32_t kSpinLimit = 1u << 21;      // polls (~seconds) before a dependency wait gives up
 constexpr uint32_t kBatchLds = 78u * 1024u;    // LDS per workgroup: two workgroups per CU
-constexpr uint32_t kTuneSample = 4;            // plans per bucket the autotuner times
+constexpr uint32_t kTuneSample = 64;           // plans per bucket the autotuner times together
 
 uint32_t ilog2(uint64_t x) { return x ? 63u - (uint32_t)__builtin_clzll(x) : 0u; }
 uint32_t bucket_of(uint64_t rows, uint64_t groups) { return (ilog2(rows) / 2) << 8 | (ilog2(groups) / 2); }
@@ -74,6 +74,8 @@ struct tdbg_batch {
   DevBuf d_q, d_items, d_tiles, d_rec, d_rows, d_misc, d_zero, d_cblocks, d_fblocks, d_counts, d_gids, d_out;
   std::shared_ptr<uint8_t> h_out;  // pinned; results keep the buffer they view
   uint64_t h_cap = 0;
+  uint8_t* h_desc = nullptr;       // pinned staging of the descriptors
+  uint64_t h_desc_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   tdbg_batch_stats last{};
@@ -183,8 +185,9 @@ int32_t build(tdbg_batch* b) {
     if (S.rows == 0) h.strategy = S_GLOBAL;
     std::lock_guard<std::mutex> plk(pl->mu);
     if ((rc = prepare(pl, t, S, h))) return rc;
-    if (pl->prep.radix && !pl->prep.rx.dense_out) {
-      // a sparse key domain: compacted leaf output is not supported in a batch
+    if (pl->prep.radix && (!pl->prep.rx.dense_out || S.rows >= (1ull << 32))) {
+      // a sparse key domain (compacted leaf output) or a table beyond 32-bit
+      // row offsets: not partitioned in a batch
       h.strategy = S_GLOBAL;
       if ((rc = prepare(pl, t, S, h))) return rc;
     }
@@ -223,18 +226,15 @@ int32_t build(tdbg_batch* b) {
     if (b->info[i].strategy == 4u) {
       rows_o[i] = rows_b;
       rows_b += align_up(sets[i].rows * q.rx.row.stride + 256, 256);
+      // bucket starts and ends, then per-item digit counts (one item per
+      // kItemRows rows of tiles)
+      const uint64_t n_hist = (sets[i].tiles.size() + kItemRows / kTileRows - 1) / (kItemRows / kTileRows);
       misc_o[i] = misc_b;
-      misc_b += align_up(((2ull << q.rx.bits1) + 2) * 8, 256);
+      misc_b += align_up(((2ull << q.rx.bits1) + 2) * 8, 256) + align_up(n_hist * (4ull << q.rx.bits1), 256);
     }
   }
-  // zeroed counters: queue head, per-plan sync words, radix histograms
-  uint64_t zero_b = 64 + align_up(16ull * n, 256);
-  std::vector<uint64_t> hist_o(n, 0);
-  for (uint32_t i = 0; i < n; i++)
-    if (b->info[i].strategy == 4u) {
-      hist_o[i] = zero_b;
-      zero_b += align_up((8ull << Q[i].rx.bits1), 256);
-    }
+  // zeroed counters: queue head, per-plan sync words
+  const uint64_t zero_b = 64 + align_up(16ull * n, 256);
   // output buffer: group counts, error flags, then each plan's columns
   b->err_off = align_up(8ull * n, 64);
   uint64_t out_b = align_up(b->err_off + 64, 256);
@@ -270,22 +270,28 @@ int32_t build(tdbg_batch* b) {
     q.blk1 = (uint32_t)cblocks.size();
     for (uint64_t r = 0; r < gmax[i]; r += kBatchChunk) fblocks.push_back(BatchBlock{i, 0, r});
     const uint32_t n_tiles = (uint32_t)sets[i].tiles.size();
-    const uint32_t chunk = (uint32_t)std::max<uint64_t>(1, kItemRows / kTileRows);
+    const uint32_t base_chunk = (uint32_t)std::max<uint64_t>(1, kItemRows / kTileRows);
+    const uint32_t chunk = base_chunk;
     const uint32_t before = (uint32_t)(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
     if (b->info[i].strategy == 4u) {
       for (uint32_t t0 = 0; t0 < n_tiles; t0 += chunk) {
         const uint32_t t1 = std::min(n_tiles, t0 + chunk);
-        hist_items.push_back(BatchItem{BI_RX_HIST, i, t0, t1, 0, 0});
-        scatter_items.push_back(BatchItem{BI_RX_SCATTER, i, t0, t1, 0, 0});
+        hist_items.push_back(BatchItem{BI_RX_HIST, i, t0, t1, t0 / chunk, 0});  // part: the item's index
+        scatter_items.push_back(BatchItem{BI_RX_SCATTER, i, t0, t1, t0 / chunk, 0});
       }
       const uint64_t per = std::max<uint64_t>(1, (uint64_t)q.rx.n_leaves * kItemRows / std::max<uint64_t>(sets[i].rows, 1));
       for (uint64_t l0 = 0; l0 < q.rx.n_leaves; l0 += per)
         agg_items.push_back(BatchItem{BI_RX_AGG, i, (uint32_t)l0, (uint32_t)std::min<uint64_t>(q.rx.n_leaves, l0 + per), 0, 0});
     } else {
       const uint32_t parts = q.sp.lds_mode == LDS_DIRECT ? q.sp.lds_parts : 1u;
-      for (uint32_t t0 = 0; t0 < n_tiles; t0 += chunk)
+      // an LDS table item ends with a flush of every slot to HBM: give it at
+      // least 32 rows per slot so the flush stays a small part of the item
+      uint32_t scan_chunk = base_chunk;
+      if (q.sp.lds_mode != LDS_NONE)
+        scan_chunk = (uint32_t)std::max<uint64_t>(scan_chunk, (32ull * q.sp.lds_cap + kTileRows - 1) / kTileRows);
+      for (uint32_t t0 = 0; t0 < n_tiles; t0 += scan_chunk)
         for (uint32_t p = 0; p < parts; p++)
-          scan_items.push_back(BatchItem{BI_SCAN, i, t0, std::min(n_tiles, t0 + chunk), p, 0});
+          scan_items.push_back(BatchItem{BI_SCAN, i, t0, std::min(n_tiles, t0 + scan_chunk), p, 0});
     }
     const uint32_t after = (uint32_t)(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
     b->info[i].items = after - before;
@@ -338,7 +344,7 @@ int32_t build(tdbg_batch* b) {
     q.n_out = at<uint64_t>(b->d_out, 8ull * i);
     if (b->info[i].strategy == 4u) {
       const uint64_t nb = 1ull << q.rx.bits1;
-      q.hist = at<unsigned long long>(b->d_zero, hist_o[i]);
+      q.item_hist = at<uint32_t>(b->d_misc, misc_o[i] + align_up(((2ull << q.rx.bits1) + 2) * 8, 256));
       q.rx.rows1 = at<uint8_t>(b->d_rows, rows_o[i]);
       q.rx.rows2 = q.rx.rows1;
       q.rx.bucket_start = at<uint64_t>(b->d_misc, misc_o[i]);
@@ -347,20 +353,41 @@ int32_t build(tdbg_batch* b) {
       q.rx.out_cursor = q.rx.bucket_fill + nb;
     }
   }
+  // descriptors: staged back to back in pinned memory, copied asynchronously
+  // on the batch's stream ahead of the kernels that read them
+  const uint64_t q_b = sizeof(BatchQuery) * n, it_b = sizeof(BatchItem) * items.size();
+  const uint64_t cb_b = sizeof(BatchBlock) * cblocks.size(), fb_b = sizeof(BatchBlock) * fblocks.size();
+  const uint64_t desc_b = align_up(tile_b, 256) + align_up(q_b, 256) + align_up(it_b, 256) + align_up(cb_b, 256) + fb_b;
+  if (b->h_desc_cap < desc_b) {
+    if (b->h_desc) (void)hipHostFree(b->h_desc);
+    b->h_desc = nullptr;
+    b->h_desc_cap = 0;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->h_desc), desc_b, hipHostMallocDefault), "pinned descriptors");
+    b->h_desc_cap = desc_b;
+  }
+  uint8_t* h = b->h_desc;
   for (uint32_t i = 0; i < n; i++)
-    if (!sets[i].tiles.empty())
-      HIP_TRY(hipMemcpy(at<void>(b->d_tiles, tile_o[i]), sets[i].tiles.data(), sets[i].tiles.size() * sizeof(Tile),
-                        hipMemcpyHostToDevice), "batch tiles");
-  HIP_TRY(hipMemcpy(b->d_q.p, Q.data(), sizeof(BatchQuery) * n, hipMemcpyHostToDevice), "batch plans");
-  if (!items.empty())
-    HIP_TRY(hipMemcpy(b->d_items.p, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice), "batch items");
-  HIP_TRY(hipMemcpy(b->d_cblocks.p, cblocks.data(), sizeof(BatchBlock) * cblocks.size(), hipMemcpyHostToDevice), "blocks");
-  HIP_TRY(hipMemcpy(b->d_fblocks.p, fblocks.data(), sizeof(BatchBlock) * fblocks.size(), hipMemcpyHostToDevice), "blocks");
+    if (!sets[i].tiles.empty()) memcpy(h + tile_o[i], sets[i].tiles.data(), sets[i].tiles.size() * sizeof(Tile));
+  uint64_t o = align_up(tile_b, 256);
+  auto put = [&](DevBuf& d, const void* src, uint64_t bytes, uint64_t at_off) -> hipError_t {
+    if (!bytes) return hipSuccess;
+    if (src) memcpy(h + at_off, src, bytes);
+    return hipMemcpyAsync(d.p, h + at_off, bytes, hipMemcpyHostToDevice, b->stream);
+  };
+  HIP_TRY(put(b->d_tiles, nullptr, tile_b, 0), "batch tiles");
+  HIP_TRY(put(b->d_q, Q.data(), q_b, o), "batch plans");
+  o += align_up(q_b, 256);
+  HIP_TRY(put(b->d_items, items.data(), it_b, o), "batch items");
+  o += align_up(it_b, 256);
+  HIP_TRY(put(b->d_cblocks, cblocks.data(), cb_b, o), "batch blocks");
+  o += align_up(cb_b, 256);
+  HIP_TRY(put(b->d_fblocks, fblocks.data(), fb_b, o), "batch blocks");
   b->built = true;
   // kernel attributes are set outside any capture
   HIP_TRY(launch_batch_scan(nullptr, nullptr, 0, nullptr, nullptr, kSpinLimit, 1, 0, b->stream), "batch kernel setup");
   const char* genv = getenv("THINDB_GPU_GRAPH");
   if (b->use_graph && !(genv && genv[0] == '0')) {
+    HIP_TRY(hipStreamSynchronize(b->stream), "descriptors");  // the capture starts on an idle stream
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
     rc = enqueue(b, false);
     hipGraph_t g = nullptr;
@@ -455,6 +482,7 @@ void free_batch(tdbg_batch* b) {
   (void)hipSetDevice(b->ctx->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   release_buffers(b);
+  if (b->h_desc) (void)hipHostFree(b->h_desc);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);

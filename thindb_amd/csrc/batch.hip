@@ -7,9 +7,11 @@
 // workgroup runs the same routines as the single-plan kernels:
 //   BI_SCAN        scan/filter/aggregate tiles [a0, a1) into an LDS table
 //                  (direct: one key range `part`; hash) or HBM atomics
-//   BI_RX_HIST     radix level-1 histogram of tiles [a0, a1); the plan's last
-//                  histogram item publishes the bucket starts
-//   BI_RX_SCATTER  radix scatter of tiles [a0, a1), after the bucket starts
+//   BI_RX_HIST     radix level-1 digit counts of the filtered rows of tiles
+//                  [a0, a1); the plan's last histogram item turns every
+//                  item's counts into its write offsets and the bucket ranges
+//   BI_RX_SCATTER  radix scatter of tiles [a0, a1) at its item's offsets (no
+//                  device atomics), after the offsets are published
 //   BI_RX_AGG      radix leaves [a0, a1), after all of the plan's scatters,
 //                  into the plan's dense record array
 // The queue lists every histogram item first, then scans, then scatters, then
@@ -84,13 +86,14 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kernel(const BatchQuery* 
         break;
       }
       case BI_RX_HIST: {
-        rx_hist_range(P, B.rx, it.a0, it.a1, reinterpret_cast<uint32_t*>(lds), B.hist);
+        const uint32_t nb = 1u << B.rx.bits1;
+        rx_item_hist(P, B.rx, it.a0, it.a1, reinterpret_cast<uint32_t*>(lds), B.item_hist + (uint64_t)it.part * nb);
         const uint32_t prev = batch_signal(&B.sync[0]);
         if (threadIdx.x == 0) s_ok = prev + 1 == B.n_hist ? 1u : 0u;
         __syncthreads();
-        if (s_ok) {  // uniform: the plan's last histogram item publishes the bucket starts
+        if (s_ok) {  // uniform: the plan's last histogram item turns counts into offsets
           __threadfence();
-          rx_offsets(B.rx, B.hist, reinterpret_cast<uint64_t*>(lds));
+          rx_item_offsets(B.rx, B.item_hist, B.n_hist, reinterpret_cast<uint64_t*>(lds));
           __threadfence();
           __syncthreads();
           if (threadIdx.x == 0) __hip_atomic_store(&B.sync[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -99,7 +102,8 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kernel(const BatchQuery* 
       }
       case BI_RX_SCATTER: {
         if (batch_wait(&B.sync[1], 1u, spin_limit, err, &s_ok)) {
-          rx_scatter_range(P, B.rx, it.a0, it.a1, lds);
+          const uint32_t nb = 1u << B.rx.bits1;
+          rx_scatter_range(P, B.rx, it.a0, it.a1, lds, B.item_hist + (uint64_t)it.part * nb);
           batch_signal(&B.sync[2]);
         }
         break;

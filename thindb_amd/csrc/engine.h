@@ -232,7 +232,8 @@ struct BatchItem {
   uint32_t kind;
   uint32_t q;
   uint32_t a0, a1;
-  uint32_t part;   // BI_SCAN over a partitioned direct table: the key range owned
+  uint32_t part;   // BI_SCAN over a partitioned direct table: the key range owned;
+                   // BI_RX_HIST / BI_RX_SCATTER: the item's index among the plan's
   uint32_t pad;
 };
 
@@ -249,7 +250,7 @@ struct BatchQuery {
   uint32_t pad;
   uint32_t* sync;                // [0] histogram items done, [1] bucket starts published,
                                  // [2] scatter items done
-  unsigned long long* hist;      // level-1 histogram (radix)
+  uint32_t* item_hist;           // radix: [n_hist][2^bits1] per-item digit counts, then offsets
   uint64_t* n_out;               // groups emitted (output buffer header)
 };
 

@@ -201,6 +201,14 @@ void tdbg_gen_batched(uint64_t seed, uint32_t t, uint64_t row0, uint64_t n, int3
   run(&j, threads);
 }
 
+/* A batched-workload-shaped table with a given key cardinality: key INT
+ * U[0, card), value BIGINT U[-1e9, 1e9]. */
+void tdbg_gen_keyed(uint64_t seed, int64_t card, uint64_t row0, uint64_t n, int32_t* key, int64_t* value,
+                    int threads) {
+  job_t j = {4, mix64(seed + 0x2000), row0, n, {key, value, 0, 0}, 0, card, 0};
+  run(&j, threads);
+}
+
 /* The Zipf probability mass of rank k (1-based), for distribution tests. */
 double tdbg_gen_zipf_pmf(uint32_t n, double s, uint32_t k) {
   double h = 0;

@@ -432,7 +432,18 @@ __global__ void minmax_kernel(const uint64_t* keys, uint64_t n, uint64_t* mm) {
     lo = a < lo ? a : lo;
     hi = b > hi ? b : hi;
   }
+  // one device atomic pair per workgroup (the two words are contended)
+  __shared__ uint64_t wl[16], wh[16];
   if ((threadIdx.x & 63) == 0) {
+    wl[threadIdx.x >> 6] = lo;
+    wh[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t w = 1; w < (blockDim.x + 63) / 64; w++) {
+      lo = wl[w] < lo ? wl[w] : lo;
+      hi = wh[w] > hi ? wh[w] : hi;
+    }
     atomicMin((unsigned long long*)&mm[0], (unsigned long long)lo);
     atomicMax((unsigned long long*)&mm[1], (unsigned long long)hi);
   }

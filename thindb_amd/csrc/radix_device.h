@@ -38,7 +38,7 @@ __device__ __forceinline__ void rx_tile_range(const ScanParams& P, uint32_t& t0,
   t1 = (uint32_t)(((uint64_t)(blockIdx.x + 1) * P.n_tiles) / gridDim.x);
 }
 
-constexpr uint32_t kRxMaxBits1 = 8;  // level-1 fanout <= 256
+constexpr uint32_t kRxMaxBits1 = 10;  // level-1 fanout <= 1024
 
 // Level-1 bucket starts from the histogram, by the whole workgroup (at most
 // 1024 buckets, tmp: blockDim.x LDS words): bucket_start[d] = rows in buckets
@@ -69,6 +69,73 @@ __device__ __forceinline__ void rx_offsets(const RadixParams& R, const unsigned 
       R.bucket_fill[lo + i] = run;
       run += v[i];
     }
+  if (threadIdx.x == blockDim.x - 1) R.bucket_start[nb] = tmp[blockDim.x - 1];
+  __syncthreads();
+}
+
+// Per-item digit counts (batched radix): the rows of tiles [t0, t1) that
+// pass the predicates, by level-1 digit, through `h` (2^bits1 LDS words) to
+// `out`.  Counting after the filter makes the item's runs exact, so the
+// scatter can write at precomputed offsets without gaps.
+__device__ __forceinline__ void rx_item_hist(const ScanParams& P, const RadixParams& R, uint32_t t0, uint32_t t1,
+                                             uint32_t* h, uint32_t* out) {
+  const uint32_t nb = 1u << R.bits1;
+  for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x) h[d] = 0;
+  __syncthreads();
+  for (uint32_t t = t0; t < t1; t++) {
+    const Tile T = P.tiles[t];
+    for (uint32_t s = threadIdx.x * kVec; s < T.n; s += blockDim.x * kVec) {
+      const uint32_t cnt = T.n - s < (uint32_t)kVec ? T.n - s : (uint32_t)kVec;
+      uint64_t key[kVec];
+      const uint32_t mask = rx_strip_keys(P, T.row0 + s, cnt, key, true);
+#pragma unroll
+      for (int j = 0; j < kVec; j++)
+        if ((mask >> j) & 1u) atomicAdd(&h[(uint32_t)(key[j] >> R.shift1)], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x) out[d] = h[d];
+}
+
+// From the per-item counts item_hist[i * 2^bits1 + d] of n_items items (one
+// workgroup): the level-1 bucket ranges (bucket_start[d], bucket_fill[d] =
+// its end) and, in place, each item's first row per digit: bucket start plus
+// the rows of that digit in earlier items.  tmp: blockDim.x LDS words.
+__device__ __forceinline__ void rx_item_offsets(const RadixParams& R, uint32_t* item_hist, uint32_t n_items,
+                                                uint64_t* tmp) {
+  const uint32_t nb = 1u << R.bits1;
+  const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;  // <= 4 for nb <= 1024 over >= 256 threads
+  const uint32_t lo = threadIdx.x * per;
+  uint64_t tot[4] = {0, 0, 0, 0};
+  uint64_t s = 0;
+  for (uint32_t k = 0; k < per && k < 4; k++) {
+    if (lo + k >= nb) break;
+    for (uint32_t i = 0; i < n_items; i++) tot[k] += item_hist[(uint64_t)i * nb + lo + k];
+    s += tot[k];
+  }
+  tmp[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
+    const uint64_t x = threadIdx.x >= o ? tmp[threadIdx.x - o] : 0;
+    __syncthreads();
+    tmp[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint64_t run = tmp[threadIdx.x] - s;
+  for (uint32_t k = 0; k < per && k < 4; k++) {
+    const uint32_t d = lo + k;
+    if (d >= nb) break;
+    R.bucket_start[d] = run;
+    R.bucket_fill[d] = run + tot[k];
+    uint64_t c = run;
+    for (uint32_t i = 0; i < n_items; i++) {
+      uint32_t* p = item_hist + (uint64_t)i * nb + d;
+      const uint32_t v = *p;
+      *p = (uint32_t)c;
+      c += v;
+    }
+    run += tot[k];
+  }
   if (threadIdx.x == blockDim.x - 1) R.bucket_start[nb] = tmp[blockDim.x - 1];
   __syncthreads();
 }
@@ -243,14 +310,21 @@ __device__ __forceinline__ void rx_hist_range(const ScanParams& P, const RadixPa
 // R.tile_rows rows (<= blockDim.x * kVec) is staged in LDS grouped by digit,
 // and each digit's run is written contiguously after one device atomic on
 // the bucket cursor.
+//
+// With `item_off` (the item's row offset per digit, from rx_item_offsets)
+// the runs go to consecutive places from there, tracked in LDS, and no
+// device atomic is needed.
 __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const RadixParams& R, uint32_t t0, uint32_t t1,
-                                                 uint8_t* lds) {
+                                                 uint8_t* lds, const uint32_t* item_off = nullptr) {
   uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + kRxCntOff);
   uint32_t* off = reinterpret_cast<uint32_t*>(lds + kRxOffOff);
+  uint32_t* cur = reinterpret_cast<uint32_t*>(lds + kRxCurOff);
   uint32_t* tmp = reinterpret_cast<uint32_t*>(lds + kRxTmpOff);
   uint64_t* base = reinterpret_cast<uint64_t*>(lds + kRxBaseOff);
   uint8_t* stage = lds + kRxStageOff;
   const uint32_t nb = 1u << R.bits1, stride = R.row.stride;
+  if (item_off)
+    for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x) cur[d] = item_off[d];
   for (uint32_t t = t0; t < t1; t++) {
     const Tile T = P.tiles[t];
     for (uint32_t s0 = 0; s0 < T.n; s0 += R.tile_rows) {  // uniform sub-tile loop
@@ -272,10 +346,16 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
       }
       __syncthreads();
       const uint32_t total = rx_prefix(cnt, off, nb, tmp);
-      for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x)
-        base[d] = cnt[d] ? atomicAdd(reinterpret_cast<unsigned long long*>(&R.bucket_fill[d]),
-                                     (unsigned long long)cnt[d])
-                         : 0;
+      for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x) {
+        if (item_off) {  // this thread alone reads and advances cur[d]
+          base[d] = cur[d];
+          cur[d] += cnt[d];
+        } else {
+          base[d] = cnt[d] ? atomicAdd(reinterpret_cast<unsigned long long*>(&R.bucket_fill[d]),
+                                       (unsigned long long)cnt[d])
+                           : 0;
+        }
+      }
       if (mask) {
         // stage the rows: key, then each aggregate input (the loads hit the
         // cache lines the key pass just fetched), then the validity word

@@ -114,6 +114,17 @@ void release(tdbg_plan* pl, tdbg_plan::Buf& b) {
   b.bytes = 0;
 }
 
+int32_t ensure_pinned(uint8_t*& p, uint64_t& cap, uint64_t bytes) {
+  if (p && cap >= bytes) return TDBG_OK;
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  cap = 0;
+  const uint64_t want = std::max<uint64_t>(bytes, 4096);
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p), want, hipHostMallocDefault), "pinned staging");
+  cap = want;
+  return TDBG_OK;
+}
+
 }  // namespace rt
 }  // namespace tdbg
 
@@ -567,8 +578,11 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     const uint32_t leaf_budget = std::min<uint32_t>(budget, (uint32_t)pl->ctx->max_lds / 2) - 4096;
     const char* lb_env = getenv("THINDB_GPU_RADIX_LEAF_BITS");
     const uint32_t lb_max = hint.leaf_bits ? hint.leaf_bits : (lb_env ? (uint32_t)atoi(lb_env) : 32u);
-    uint32_t leaf_bits = 0;
-    while (leaf_bits < L && leaf_bits < lb_max && ((uint64_t)sb << (leaf_bits + 1)) <= leaf_budget) leaf_bits++;
+    uint32_t fit_bits = 0;  // the widest leaf table that fits
+    while (fit_bits < L && fit_bits < lb_max && ((uint64_t)sb << (fit_bits + 1)) <= leaf_budget) fit_bits++;
+    // at least 2^10 leaves where the domain allows (several per workgroup of
+    // the leaf pass), each of at least 64 keys
+    const uint32_t leaf_bits = std::min<uint32_t>(fit_bits, std::max<uint32_t>(6, L > 10 ? L - 10 : 0));
     RadixParams& R = P.rx;
     R = RadixParams{};
     R.row.flags_off = ~0u;
@@ -592,15 +606,18 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     // staged scatter tile: 24 KiB of digit tables + tile_rows rows in LDS
     R.tile_rows = std::min<uint32_t>(4096, (budget - 24 * 1024) / R.row.stride / 256 * 256);
     const uint32_t rem = L - leaf_bits;
+    // one level up to 2^10 buckets, else 2^max_bits1 x the rest
     const uint32_t max_bits1 = std::min<uint32_t>(hint.max_bits1 ? hint.max_bits1 : 8u, 10u);
-    const uint32_t max_rem = max_bits1 + (hint.single_level ? 0u : 10u);
+    const uint32_t max_rem = hint.single_level ? 10u : max_bits1 + 10u;
     // automatic: at least one row per four keys of the domain on average, so
     // leaves are well filled (sparser domains stay with the HBM hash table)
-    const bool fits = leaf_bits >= 8 && dom > 65536 && n_rows_scan >= (1u << 20) && dom / 4 <= n_rows_scan;
+    const bool fits = fit_bits >= 8 && dom > 65536 && n_rows_scan >= (1u << 20) && dom / 4 <= n_rows_scan;
     if ((forced || fits) && rem <= max_rem && R.tile_rows >= 256) {
       P.radix = true;
       R.leaf_bits = leaf_bits;
-      R.bits1 = std::min<uint32_t>(max_bits1, rem);
+      R.bits1 = rem <= 10 ? rem : std::min<uint32_t>(max_bits1, rem);
+      const char* b1_env = getenv("THINDB_GPU_RADIX_BITS1");  // tests: force a second level
+      if (b1_env && !hint.single_level) R.bits1 = std::min<uint32_t>(R.bits1, (uint32_t)atoi(b1_env));
       R.bits2 = rem - R.bits1;
       R.shift2 = leaf_bits;
       R.shift1 = leaf_bits + R.bits2;
@@ -608,8 +625,9 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
       sp.lds_mode = LDS_DIRECT;  // one leaf's table
       lds_cap = 1u << leaf_bits;
       parts = 1;
-      if (hint.dense_out && dense) {
-        // leaves write every key of their range into the dense table
+      // a mostly occupied dense domain: leaves write every key of their range
+      // into the dense table (compaction then yields key order, no sort)
+      if (dense && (hint.dense_out || dom <= 2 * est)) {
         R.dense_out = 1;
       } else {
         // records carry their key (the hash layout): they are emitted
@@ -965,6 +983,7 @@ void tdbg_plan_destroy(tdbg_plan* pl) {
                  &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf})
     release(pl, *b);
   for (auto& e : pl->ev) if (e) (void)hipEventDestroy(e);
+  if (pl->h_in) (void)hipHostFree(pl->h_in);
   if (pl->stream) (void)hipStreamDestroy(pl->stream);
   delete pl;
 }
@@ -1031,9 +1050,7 @@ int32_t tdbg::rt::make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host
   return TDBG_OK;
 }
 
-extern "C" {
-
-int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out) {
+int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out) {
   if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
   tdbg_context* ctx = pl->ctx;
@@ -1079,8 +1096,15 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   P.fp.g_state = sp.g_state;
   const uint64_t scanned_bytes = S.bytes;
 
-  HIP_TRY(hipMemcpyAsync(pl->tiles.p, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, s), "tiles");
-  HIP_TRY(hipMemcpyAsync(pl->tmpl.p, P.tmpl.data(), P.tmpl.size() * 8, hipMemcpyHostToDevice, s), "template");
+  // tiles and the record template go through the plan's pinned staging
+  // buffer, so the copies are asynchronous (the previous execution's copies
+  // completed at its final synchronisation)
+  const uint64_t tiles_b = tiles.size() * sizeof(Tile), tmpl_b = P.tmpl.size() * 8;
+  if ((rc = ensure_pinned(pl->h_in, pl->h_in_cap, align_up(tiles_b, 256) + tmpl_b))) return rc;
+  memcpy(pl->h_in, tiles.data(), tiles_b);
+  memcpy(pl->h_in + align_up(tiles_b, 256), P.tmpl.data(), tmpl_b);
+  HIP_TRY(hipMemcpyAsync(pl->tiles.p, pl->h_in, tiles_b, hipMemcpyHostToDevice, s), "tiles");
+  HIP_TRY(hipMemcpyAsync(pl->tmpl.p, pl->h_in + align_up(tiles_b, 256), tmpl_b, hipMemcpyHostToDevice, s), "template");
   bool specialized = false;
   const uint64_t n_rec = sp.g_cap;
   if ((rc = ensure(pl, pl->gids, n_rec * 4 + 256))) return rc;
@@ -1113,7 +1137,8 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
     if (R.bits2) HIP_TRY(launch_radix(sp, R, 1, hist, s, RX_SCATTER2), "radix scatter 2");
     HIP_TRY(launch_radix(sp, R, P.agg_grid, hist, s, RX_AGG), "radix aggregate");
     HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
-    HIP_TRY(hipMemcpyAsync(pl->nout.p, R.out_cursor, 8, hipMemcpyDeviceToDevice, s), "group count");
+    if (!R.dense_out)
+      HIP_TRY(hipMemcpyAsync(pl->nout.p, R.out_cursor, 8, hipMemcpyDeviceToDevice, s), "group count");
   } else {
     HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
     HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
@@ -1126,8 +1151,8 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
     HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
   }
   if (pl->cancel.load()) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
-  // compaction (radix records are already compact: identity positions)
-  if (!P.radix) {
+  // compaction (compacted radix records are already in place: identity positions)
+  if (!P.radix || P.rx.dense_out) {
     if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
     HIP_TRY(launch_compact(P.op, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->gids.p),
                            static_cast<uint64_t*>(pl->nout.p), s), "compact");
@@ -1140,7 +1165,7 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   if (err_bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
   if (err_bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
   if (pl->cancel.load()) return fail(TDBG_ERR_CANCELLED, "cancelled");
-  if (P.radix) {
+  if (P.radix && !P.rx.dense_out) {
     if (n_groups > n_rec) return fail(TDBG_ERR_DEVICE, "radix pass emitted %llu groups for %llu records",
                                       (unsigned long long)n_groups, (unsigned long long)n_rec);
     HIP_TRY(launch_iota(static_cast<uint32_t*>(pl->gids.p), n_groups, s), "group positions");
@@ -1177,8 +1202,16 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   if ((rc = ensure(pl, pl->out, ob + 256))) return rc;
   HIP_TRY(launch_finalize(P.fp, sel, n_out, static_cast<uint8_t*>(pl->out.p), s), "finalize");
   HIP_TRY(hipEventRecord(pl->ev[3], s), "event");
-  std::vector<uint8_t> host(ob + 16);
-  if (n_out) HIP_TRY(hipMemcpyAsync(host.data(), pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
+  // the result lands in pinned memory its columns then view in place; a
+  // buffer a live earlier result still views is left to it
+  if (!pl->h_out || pl->h_out.use_count() > 1 || pl->h_out_cap < ob + 16) {
+    void* h = nullptr;
+    const uint64_t cap = std::max<uint64_t>(ob + 16, pl->h_out ? pl->h_out_cap : 0);
+    HIP_TRY(hipHostMalloc(&h, cap, hipHostMallocDefault), "pinned result buffer");
+    pl->h_out.reset(static_cast<uint8_t*>(h), [](uint8_t* p) { (void)hipHostFree(p); });
+    pl->h_out_cap = cap;
+  }
+  if (n_out) HIP_TRY(hipMemcpyAsync(pl->h_out.get(), pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
   if ((rc = sync_check(pl, "finalize"))) return rc;
   float ms_total = 0, ms_scan = 0;
   hipEventElapsedTime(&ms_total, pl->ev[0], pl->ev[3]);
@@ -1192,7 +1225,14 @@ int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg
   pl->last.global_mode = P.radix ? 3u : sp.g_mode;
   pl->last.specialized = specialized ? 1u : 0u;
   pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
-  return make_result(pl, n_out, host.data(), data_off, valid_off, nullptr, out);
+  return make_result(pl, n_out, pl->h_out.get(), data_off, valid_off,
+                     views ? std::shared_ptr<void>(pl->h_out, pl->h_out.get()) : nullptr, out);
+}
+
+extern "C" {
+
+int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out) {
+  return execute_plan(pl, rgs, n_rgs, true, out);
 }
 
 uint64_t tdbg_result_num_rows(const tdbg_result* r) { return r ? r->n_rows : 0; }

@@ -184,6 +184,12 @@ struct tdbg_plan {
     void* p = nullptr;
     uint64_t bytes = 0;
   } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf;
+  // pinned host staging: inputs (tiles, template) and the result the
+  // columns of a returned tdbg_result view
+  uint8_t* h_in = nullptr;
+  uint64_t h_in_cap = 0;
+  std::shared_ptr<uint8_t> h_out;
+  uint64_t h_out_cap = 0;
   tdbg_exec_stats last{};
 };
 
@@ -193,6 +199,8 @@ namespace rt {
 int32_t dev_alloc(tdbg_context* ctx, void** p, uint64_t bytes);
 void dev_free(tdbg_context* ctx, void* p, uint64_t bytes);
 int32_t ensure(tdbg_plan* pl, tdbg_plan::Buf& b, uint64_t bytes);
+// Grows a pinned host buffer to at least `bytes`.
+int32_t ensure_pinned(uint8_t*& p, uint64_t& cap, uint64_t bytes);
 void release(tdbg_plan* pl, tdbg_plan::Buf& b);
 
 // Resolves the row groups a plan scans (all staged ones when rgs is null),
@@ -208,6 +216,11 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& s, const PlanHint& hint)
 // with `keep` set, viewed in place while the result holds `keep`.
 int32_t make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host, const std::vector<uint64_t>& data_off,
                     const std::vector<uint64_t>& valid_off, std::shared_ptr<void> keep, tdbg_result** out);
+
+// One execution of a plan (tdbg_execute): the result's columns view the
+// plan's pinned result buffer (`views`) or are copied out of it, which leaves
+// the buffer free for the plan's next execution (the serving loop).
+int32_t execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out);
 
 // A reusable batch for the serving loop (batch.cpp): eager launches, device
 // and pinned buffers kept and grown across runs of different plan sets.

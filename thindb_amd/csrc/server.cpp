@@ -151,7 +151,7 @@ void lane_main(tdbg_server* s, uint32_t lane) {
     for (tdbg_ticket* t : unit) t->t_start = t0;
     if (unit.size() == 1) {
       tdbg_result* r = nullptr;
-      const int32_t rc = tdbg_execute(unit[0]->plan, nullptr, 0, &r);
+      const int32_t rc = execute_plan(unit[0]->plan, nullptr, 0, false, &r);
       finish(unit[0], rc, rc == TDBG_OK ? r : nullptr, 1, lane);
     } else {
       plans.clear();

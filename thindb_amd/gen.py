@@ -45,6 +45,8 @@ def lib():
                                              C.POINTER(C.c_int32)]
         L.tdbg_gen_batched.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(C.c_int32), p64,
                                        C.c_int]
+        L.tdbg_gen_keyed.argtypes = [C.c_uint64, C.c_int64, C.c_uint64, C.c_uint64, C.POINTER(C.c_int32), p64,
+                                     C.c_int]
         L.tdbg_gen_zipf_pmf.argtypes = [C.c_uint32, C.c_double, C.c_uint32]
         L.tdbg_gen_zipf_pmf.restype = C.c_double
         _lib = L
@@ -153,3 +155,11 @@ def batched_request(seed: int, t: int):
         if filtered_ else None
     return Request(["key"], [AggSpec("count", None, "c"), AggSpec("sum", "value", "s"), AggSpec("min", "value", "mn"),
                              AggSpec("max", "value", "mx")], where_filter=where)
+
+
+def keyed_table(seed: int, n: int, card: int) -> Table:
+    """n rows of the batched-workload shape with key cardinality `card`."""
+    key = np.empty(n, np.int32)
+    value = np.empty(n, np.int64)
+    lib().tdbg_gen_keyed(seed, card, 0, n, _ptr(key, C.c_int32), _ptr(value, C.c_int64), _threads())
+    return Table(BATCHED_SCHEMA, {"key": key, "value": value})
