@@ -72,8 +72,7 @@ struct tdbg_batch {
   uint32_t n_items = 0, n_cblocks = 0, n_fblocks = 0, grid = 0, lds_bytes = 0;
   uint64_t zero_words = 0, out_bytes = 0, err_off = 0, total_off = 0;
   DevBuf d_q, d_items, d_tiles, d_rec, d_rows, d_misc, d_zero, d_cblocks, d_fblocks, d_counts, d_gids, d_out;
-  std::shared_ptr<uint8_t> h_out;  // pinned; results keep the buffer they view
-  uint64_t h_cap = 0;
+  PinnedBuf h_out;                 // results keep the buffer they view
   uint8_t* h_desc = nullptr;       // pinned staging of the descriptors
   uint64_t h_desc_cap = 0;
   hipGraph_t graph = nullptr;
@@ -422,13 +421,10 @@ int32_t run(tdbg_batch* b) {
   }
   if (!b->built && (rc = build(b))) return rc;
   HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
-  if (!b->h_out || b->h_out.use_count() > 1 || b->h_cap < b->out_bytes) {
+  if (!b->h_out.p || b->h_out.p.use_count() > 1 || b->h_out.cap < b->out_bytes) {
     // results of an earlier execution still view the old buffer: keep it
-    void* h = nullptr;
-    const uint64_t cap = std::max<uint64_t>(b->out_bytes, b->h_out ? b->h_cap : 0);
-    HIP_TRY(hipHostMalloc(&h, cap, hipHostMallocDefault), "pinned result buffer");
-    b->h_out.reset(static_cast<uint8_t*>(h), [](uint8_t* p) { (void)hipHostFree(p); });
-    b->h_cap = cap;
+    b->h_out = PinnedBuf{};
+    if ((rc = pinned_acquire(b->ctx->pinned, b->out_bytes, b->h_out))) return rc;
   }
   hipStream_t s = b->stream;
   std::shared_lock<std::shared_mutex> lk(b->ctx->mu);  // staged columns stay put while the kernels read them
@@ -437,10 +433,10 @@ int32_t run(tdbg_batch* b) {
   const bool graph = b->exec != nullptr;
   if (graph) HIP_TRY(hipGraphLaunch(b->exec, s), "graph launch");
   else if ((rc = enqueue(b, true))) return rc;
-  HIP_TRY(hipMemcpyAsync(b->h_out.get(), b->d_out.p, b->out_bytes, hipMemcpyDeviceToHost, s), "result copy");
+  HIP_TRY(hipMemcpyAsync(b->h_out.p.get(), b->d_out.p, b->out_bytes, hipMemcpyDeviceToHost, s), "result copy");
   HIP_TRY(hipEventRecord(b->ev[3], s), "event");
   HIP_TRY(hipStreamSynchronize(s), "batch");
-  const uint32_t* err = reinterpret_cast<const uint32_t*>(b->h_out.get() + b->err_off);
+  const uint32_t* err = reinterpret_cast<const uint32_t*>(b->h_out.p.get() + b->err_off);
   uint32_t bits = 0;
   for (int i = 0; i < 16; i++) bits |= err[i];
   if (bits & ERR_BATCH_WAIT) return fail(TDBG_ERR_DEVICE, "batch: a dependency wait gave up");
@@ -508,11 +504,13 @@ int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n) {
   return run(b);
 }
 
-int32_t batch_copy_result(tdbg_batch* b, uint32_t i, tdbg_result** out) {
+int32_t batch_take_result(tdbg_batch* b, uint32_t i, tdbg_result** out) {
   std::lock_guard<std::mutex> lk(b->mu);
-  if (!b->built || !b->h_out || i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
-  const uint64_t n_out = reinterpret_cast<const uint64_t*>(b->h_out.get())[i];
-  return make_result(b->plans[i], n_out, b->h_out.get(), b->data_off[i], b->valid_off[i], nullptr, out);
+  if (!b->built || !b->h_out.p || i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
+  const uint8_t* h = b->h_out.p.get();
+  const uint64_t n_out = reinterpret_cast<const uint64_t*>(h)[i];
+  return make_result(b->plans[i], n_out, h, b->data_off[i], b->valid_off[i],
+                     std::shared_ptr<void>(b->h_out.p, b->h_out.p.get()), out);
 }
 
 void batch_close(tdbg_batch* b) { free_batch(b); }
@@ -603,10 +601,10 @@ int32_t tdbg_batch_result(tdbg_batch* b, uint32_t i, tdbg_result** out) {
   if (!b || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
   std::lock_guard<std::mutex> lk(b->mu);
-  if (!b->built || !b->h_out || i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
-  const uint64_t n_out = reinterpret_cast<const uint64_t*>(b->h_out.get())[i];
-  std::shared_ptr<void> keep(b->h_out, b->h_out.get());
-  return make_result(b->plans[i], n_out, b->h_out.get(), b->data_off[i], b->valid_off[i], keep, out);
+  if (!b->built || !b->h_out.p || i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
+  const uint64_t n_out = reinterpret_cast<const uint64_t*>(b->h_out.p.get())[i];
+  std::shared_ptr<void> keep(b->h_out.p, b->h_out.p.get());
+  return make_result(b->plans[i], n_out, b->h_out.p.get(), b->data_off[i], b->valid_off[i], keep, out);
 }
 
 int32_t tdbg_batch_last_stats(const tdbg_batch* b, tdbg_batch_stats* out) {

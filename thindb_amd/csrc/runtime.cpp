@@ -114,6 +114,35 @@ void release(tdbg_plan* pl, tdbg_plan::Buf& b) {
   b.bytes = 0;
 }
 
+PinnedPool::~PinnedPool() {
+  for (auto& kv : free) (void)hipHostFree(kv.second);
+}
+
+int32_t pinned_acquire(const std::shared_ptr<PinnedPool>& pool, uint64_t bytes, PinnedBuf& out) {
+  uint8_t* p = nullptr;
+  uint64_t cap = 0;
+  {
+    std::lock_guard<std::mutex> lk(pool->mu);
+    auto it = pool->free.lower_bound(bytes);
+    if (it != pool->free.end() && it->first <= 4 * bytes + (1u << 20)) {  // not a far larger one
+      cap = it->first;
+      p = it->second;
+      pool->free.erase(it);
+    }
+  }
+  if (!p) {
+    cap = std::max<uint64_t>(bytes, 4096);
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&p), cap, hipHostMallocDefault), "pinned result buffer");
+  }
+  std::shared_ptr<PinnedPool> keep = pool;
+  out.p.reset(p, [keep, cap](uint8_t* q) {
+    std::lock_guard<std::mutex> lk(keep->mu);
+    keep->free.emplace(cap, q);
+  });
+  out.cap = cap;
+  return TDBG_OK;
+}
+
 int32_t ensure_pinned(uint8_t*& p, uint64_t& cap, uint64_t bytes) {
   if (p && cap >= bytes) return TDBG_OK;
   if (p) (void)hipHostFree(p);
@@ -1204,14 +1233,11 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   HIP_TRY(hipEventRecord(pl->ev[3], s), "event");
   // the result lands in pinned memory its columns then view in place; a
   // buffer a live earlier result still views is left to it
-  if (!pl->h_out || pl->h_out.use_count() > 1 || pl->h_out_cap < ob + 16) {
-    void* h = nullptr;
-    const uint64_t cap = std::max<uint64_t>(ob + 16, pl->h_out ? pl->h_out_cap : 0);
-    HIP_TRY(hipHostMalloc(&h, cap, hipHostMallocDefault), "pinned result buffer");
-    pl->h_out.reset(static_cast<uint8_t*>(h), [](uint8_t* p) { (void)hipHostFree(p); });
-    pl->h_out_cap = cap;
+  if (!pl->h_out.p || pl->h_out.p.use_count() > 1 || pl->h_out.cap < ob + 16) {
+    pl->h_out = PinnedBuf{};  // back to the pool unless a result still views it
+    if ((rc = pinned_acquire(ctx->pinned, ob + 16, pl->h_out))) return rc;
   }
-  if (n_out) HIP_TRY(hipMemcpyAsync(pl->h_out.get(), pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
+  if (n_out) HIP_TRY(hipMemcpyAsync(pl->h_out.p.get(), pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
   if ((rc = sync_check(pl, "finalize"))) return rc;
   float ms_total = 0, ms_scan = 0;
   hipEventElapsedTime(&ms_total, pl->ev[0], pl->ev[3]);
@@ -1225,8 +1251,8 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   pl->last.global_mode = P.radix ? 3u : sp.g_mode;
   pl->last.specialized = specialized ? 1u : 0u;
   pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
-  return make_result(pl, n_out, pl->h_out.get(), data_off, valid_off,
-                     views ? std::shared_ptr<void>(pl->h_out, pl->h_out.get()) : nullptr, out);
+  return make_result(pl, n_out, pl->h_out.p.get(), data_off, valid_off,
+                     views ? std::shared_ptr<void>(pl->h_out.p, pl->h_out.p.get()) : nullptr, out);
 }
 
 extern "C" {

@@ -141,6 +141,19 @@ struct ScanSet {
   uint64_t bytes = 0;  // native-width bytes of the referenced columns
 };
 
+// Pinned host buffers for results, recycled: a buffer returns to the pool
+// when the last result viewing it (and its plan or batch) lets it go.
+struct PinnedPool {
+  std::mutex mu;
+  std::multimap<uint64_t, uint8_t*> free;  // capacity -> buffer
+  ~PinnedPool();
+};
+struct PinnedBuf {
+  std::shared_ptr<uint8_t> p;
+  uint64_t cap = 0;
+};
+int32_t pinned_acquire(const std::shared_ptr<PinnedPool>& pool, uint64_t bytes, PinnedBuf& out);
+
 }  // namespace rt
 }  // namespace tdbg
 
@@ -155,6 +168,7 @@ struct tdbg_context {
   // batch autotuner: (rows bucket, groups bucket) -> measured best hint
   std::mutex tune_mu;
   std::map<uint32_t, tdbg::rt::PlanHint> tuned;
+  std::shared_ptr<tdbg::rt::PinnedPool> pinned = std::make_shared<tdbg::rt::PinnedPool>();
 };
 
 struct tdbg_result {
@@ -188,8 +202,7 @@ struct tdbg_plan {
   // columns of a returned tdbg_result view
   uint8_t* h_in = nullptr;
   uint64_t h_in_cap = 0;
-  std::shared_ptr<uint8_t> h_out;
-  uint64_t h_out_cap = 0;
+  tdbg::rt::PinnedBuf h_out;
   tdbg_exec_stats last{};
 };
 
@@ -226,7 +239,7 @@ int32_t execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool
 // and pinned buffers kept and grown across runs of different plan sets.
 int32_t batch_open(tdbg_context* ctx, tdbg_batch** out);
 int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n);
-int32_t batch_copy_result(tdbg_batch* b, uint32_t i, tdbg_result** out);  // an owned copy
+int32_t batch_take_result(tdbg_batch* b, uint32_t i, tdbg_result** out);  // views pooled pinned memory
 void batch_close(tdbg_batch* b);
 
 }  // namespace rt

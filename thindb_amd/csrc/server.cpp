@@ -151,7 +151,7 @@ void lane_main(tdbg_server* s, uint32_t lane) {
     for (tdbg_ticket* t : unit) t->t_start = t0;
     if (unit.size() == 1) {
       tdbg_result* r = nullptr;
-      const int32_t rc = execute_plan(unit[0]->plan, nullptr, 0, false, &r);
+      const int32_t rc = execute_plan(unit[0]->plan, nullptr, 0, true, &r);
       finish(unit[0], rc, rc == TDBG_OK ? r : nullptr, 1, lane);
     } else {
       plans.clear();
@@ -160,7 +160,7 @@ void lane_main(tdbg_server* s, uint32_t lane) {
       for (size_t i = 0; i < unit.size(); i++) {
         tdbg_result* r = nullptr;
         int32_t ri = rc;
-        if (rc == TDBG_OK) ri = batch_copy_result(s->batches[lane], (uint32_t)i, &r);
+        if (rc == TDBG_OK) ri = batch_take_result(s->batches[lane], (uint32_t)i, &r);
         finish(unit[i], ri, ri == TDBG_OK ? r : nullptr, (uint32_t)unit.size(), lane);
       }
     }
