@@ -264,6 +264,39 @@ def test_radix_partition_passes(session, oracle_lib, monkeypatch, leaf_bits, bit
         assert st.global_mode == 3
 
 
+def test_topn_selection_many_groups(session, oracle_lib, monkeypatch):
+    """ORDER BY ... LIMIT over more than 2^16 groups takes the selection of
+    the first order word's cut: the same rows, in the same order, as
+    ordering every group (THINDB_GPU_TOPN_SELECT=0), and the oracle's ranks.
+    Cases: distinct integer and DOUBLE words, a DECIMAL SUM (constant high
+    word), ascending and descending, a second ORDER BY item deciding ties at
+    the cut, heavy ties (COUNT: more candidates than one workgroup sorts),
+    OFFSET, and LIMIT 4096."""
+    rng = np.random.default_rng(21)
+    base = random_table(rng, 700_000, rg_rows=65_536, nulls=True)
+    cols = dict(base.columns)
+    cols["k1"] = rng.integers(0, 400_000, base.n_rows).astype(np.int32)
+    t = Table(base.schema, cols, base.validity, rg_rows=65_536)
+    reqs = [Request(["k1"], ALL_AGGS, [SortSpec("sb", True)], limit=100),
+            Request(["k1"], ALL_AGGS, [SortSpec("ad")], limit=37, offset=5),
+            Request(["k1"], ALL_AGGS, [SortSpec("sm", True)], limit=4096),
+            Request(["k1"], ALL_AGGS, [SortSpec("c", True), SortSpec("mins")], limit=64),
+            Request(["k1"], ALL_AGGS, [SortSpec("c")], limit=10),
+            Request(["k1", "k2"], ALL_AGGS[:5], [SortSpec("cs", True)], limit=300)]
+    for req in reqs:
+        monkeypatch.setenv("THINDB_GPU_TOPN_SELECT", "1")
+        got, want, st = check_vs_oracle(session, oracle_lib, t, req)
+        assert st.n_groups > 65_536
+        monkeypatch.setenv("THINDB_GPU_TOPN_SELECT", "0")
+        b, _ = run_gpu(session, t, req)
+        every = rows_of(b)
+        nk = len(req.group_cols)
+        # same groups in the same order; DOUBLE sums differ in the last
+        # place between runs (the order of float atomics)
+        assert [r[:nk] for r in every] == [r[:nk] for r in got]
+        assert_same_rows(every, got, float_cols_of(t, req), REL, nk)
+
+
 def test_hash_table_mode(session, oracle_lib):
     """Keys spread over a wide domain take the HBM hash table."""
     rng = np.random.default_rng(5)

@@ -13,6 +13,9 @@
 // reference breaks ties by gid (first-seen order), which its parallel scan
 // leaves unspecified; slot order is one admissible choice and is
 // deterministic here.
+#include <cstdlib>
+#include <vector>
+
 #include "engine.h"
 
 namespace tdbg {
@@ -679,12 +682,298 @@ __global__ void select_gids_kernel(const uint32_t* gids, const uint32_t* pos, ui
     out[i] = gids[pos[i]];
 }
 
+// ---- selection over many groups: the cut of the first word -------------------------
+// For ORDER BY ... LIMIT over more groups than one workgroup handles: only
+// the first order word is built for every group; the value of the
+// need-th smallest word is found 12 bits at a time (multi-workgroup
+// histograms), every group at or below it is gathered in slot order, and
+// only those candidates get their full order words and a sort.
+
+constexpr uint32_t kSelBins = 4096;
+constexpr uint32_t kSelBlocks = 512;
+
+// w[i] = order word c of group gids[i]; its min and max into mm (which
+// init_mm_kernel set).
+__global__ void __launch_bounds__(256) sel_word_kernel(const OrderParams P, uint32_t c,
+                                                       const uint32_t* __restrict__ gids, uint64_t n, uint64_t* w,
+                                                       uint64_t* mm) {
+  uint64_t lo = ~0ull, hi = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t v = comp_word(P, P.comps[c], gids[i]);
+    w[i] = v;
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  __shared__ uint64_t wl[4], wh[4];
+  if ((threadIdx.x & 63) == 0) {
+    wl[threadIdx.x >> 6] = lo;
+    wh[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t k = 1; k < 4; k++) {
+      lo = wl[k] < lo ? wl[k] : lo;
+      hi = wh[k] > hi ? wh[k] : hi;
+    }
+    atomicMin((unsigned long long*)&mm[0], (unsigned long long)lo);
+    atomicMax((unsigned long long*)&mm[1], (unsigned long long)hi);
+  }
+}
+
+// Per-workgroup histogram of digit (rel >> shift) & mask over the words
+// whose bits at and above `above` equal `prefix` (rel = w - min).  A thread
+// adds a run of equal digits with one LDS atomic, so a skewed digit (most
+// groups share one value) does not serialise on one counter.
+__global__ void __launch_bounds__(256) sel_hist_kernel(const uint64_t* __restrict__ w, uint64_t n,
+                                                       const uint64_t* __restrict__ mm, uint64_t prefix,
+                                                       uint32_t above, uint32_t shift, uint32_t mask,
+                                                       uint32_t* hist) {
+  __shared__ uint32_t h[kSelBins];
+  for (uint32_t d = threadIdx.x; d < kSelBins; d += blockDim.x) h[d] = 0;
+  __syncthreads();
+  const uint64_t lo = mm[0];
+  uint32_t run_d = 0, run_n = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t rel = w[i] - lo;
+    if (above >= 64 || (rel >> above) == prefix) {
+      const uint32_t d = (uint32_t)(rel >> shift) & mask;
+      if (d != run_d && run_n) {
+        atomicAdd(&h[run_d], run_n);
+        run_n = 0;
+      }
+      run_d = d;
+      run_n++;
+    }
+  }
+  if (run_n) atomicAdd(&h[run_d], run_n);
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < kSelBins; d += blockDim.x) hist[(uint64_t)blockIdx.x * kSelBins + d] = h[d];
+}
+
+__global__ void sel_reduce_kernel(const uint32_t* __restrict__ hist, uint32_t blocks, uint32_t* out) {
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= kSelBins) return;
+  uint32_t s = 0;
+  for (uint32_t b = 0; b < blocks; b++) s += hist[(uint64_t)b * kSelBins + d];
+  out[d] = s;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  const uint32_t lane = threadIdx.x & 63;
+  return (uint32_t)__popcll(mask & ((1ull << lane) - 1));
+}
+
+// Few candidates: the index i of every word with w[i] - min <= cut, in any
+// order (one device atomic per wavefront).  *count starts at zero.
+__global__ void __launch_bounds__(256) sel_take_kernel(const uint64_t* __restrict__ w, uint64_t n,
+                                                       const uint64_t* __restrict__ mm, uint64_t cut, uint32_t cap,
+                                                       uint32_t* idx, uint32_t* count) {
+  const uint64_t lo = mm[0];
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x; b < n; b += stride) {  // uniform per wavefront
+    const uint64_t i = b + threadIdx.x;
+    const bool take = i < n && w[i] - lo <= cut;
+    const uint64_t m = __ballot(take);
+    if (m == 0) continue;
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == (uint32_t)(__ffsll((long long)m) - 1)) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1);
+    const uint32_t at = base + lanes_below(m);
+    if (take && at < cap) idx[at] = (uint32_t)i;
+  }
+}
+
+// Order words of the candidates plus a last word, the candidate's index in
+// slot order, so that ties break by slot whatever order they were taken in.
+__global__ void sel_comps_kernel(const OrderParams P, const uint32_t* __restrict__ gids, uint64_t n,
+                                 const uint32_t* __restrict__ idx, uint32_t m, uint64_t* comps) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < m; j += gridDim.x * blockDim.x) {
+    const uint32_t gid = gids[idx[j] < n ? idx[j] : 0];  // idx[j] < n: taken from [0, n)
+    for (uint32_t c = 0; c < P.n_comps; c++) comps[(uint64_t)c * m + j] = comp_word(P, P.comps[c], gid);
+    comps[(uint64_t)P.n_comps * m + j] = idx[j];
+  }
+}
+
+__global__ void sel_out_kernel(const uint32_t* __restrict__ gids, uint64_t n, const uint32_t* __restrict__ idx,
+                               const uint32_t* __restrict__ pos, uint64_t k, uint32_t* out) {
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < k; r += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t ix = idx[pos[r]];
+    out[r] = gids[ix < n ? ix : 0];
+  }
+}
+
+// Many candidates: ordered gather of gids[i] for every w[i] - min <= cut.
+// Workgroup b covers items [b * kCompactChunk, (b + 1) * kCompactChunk), read
+// as kCompactPer coalesced rows of kCompactThreads items.
+__global__ void __launch_bounds__(kCompactThreads) sel_count_kernel(const uint64_t* __restrict__ w, uint64_t n,
+                                                                    const uint64_t* __restrict__ mm, uint64_t cut,
+                                                                    uint32_t* block_counts) {
+  __shared__ uint32_t red[kCompactThreads / 64];
+  const uint64_t lo = mm[0];
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + threadIdx.x;
+  uint32_t c = 0;
+  for (uint32_t k = 0; k < kCompactPer; k++) {
+    const uint64_t i = base + (uint64_t)k * kCompactThreads;
+    if (i < n && w[i] - lo <= cut) c++;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t k = 0; k < kCompactThreads / 64; k++) t += red[k];
+    block_counts[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kCompactThreads) sel_write_kernel(const uint64_t* __restrict__ w, uint64_t n,
+                                                                    const uint64_t* __restrict__ mm, uint64_t cut,
+                                                                    const uint32_t* block_offs,
+                                                                    const uint32_t* __restrict__ gids, uint32_t* out) {
+  constexpr uint32_t kWaves = kCompactThreads / 64;
+  __shared__ uint32_t cnt[kCompactPer * kWaves];
+  const uint64_t lo = mm[0];
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + threadIdx.x;
+  const uint32_t wave = threadIdx.x >> 6;
+  uint64_t masks[kCompactPer];
+  for (uint32_t k = 0; k < kCompactPer; k++) {
+    const uint64_t i = base + (uint64_t)k * kCompactThreads;
+    masks[k] = __ballot(i < n && w[i] - lo <= cut);
+    if ((threadIdx.x & 63) == 0) cnt[k * kWaves + wave] = (uint32_t)__popcll(masks[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // exclusive scan in item order: row k, then wavefront
+    uint32_t run = block_offs[blockIdx.x];
+    for (uint32_t e = 0; e < kCompactPer * kWaves; e++) {
+      const uint32_t v = cnt[e];
+      cnt[e] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  for (uint32_t k = 0; k < kCompactPer; k++) {
+    if ((masks[k] >> (threadIdx.x & 63)) & 1ull)
+      out[cnt[k * kWaves + wave] + lanes_below(masks[k])] = gids[base + (uint64_t)k * kCompactThreads];
+  }
+}
+
+static hipError_t sort_general(const OrderParams& p, const uint32_t* gids, uint64_t n, uint64_t need, SortScratch& S,
+                               uint32_t* out_sel, hipStream_t s);
+
+// The selection path; leaves *done false (nothing written) when it does not
+// apply: every word constant, or so many candidates that the scratch below
+// them cannot order them.
+static hipError_t select_large(const OrderParams& p, const uint32_t* gids, uint64_t n, uint64_t need,
+                               SortScratch& S, uint32_t* out_sel, hipStream_t s, bool* done) {
+  *done = false;
+  uint64_t* w = S.keys_a;
+  // the first word that varies selects (constant words decide nothing)
+  uint64_t mm[2] = {0, 0};
+  for (uint32_t c = 0; c < p.n_comps && mm[0] == mm[1]; c++) {
+    hipLaunchKernelGGL(init_mm_kernel, dim3(1), dim3(1), 0, s, S.mm);
+    hipLaunchKernelGGL(sel_word_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, p, c, gids, n, w, S.mm);
+    hipError_t e = hipMemcpyAsync(mm, S.mm, sizeof mm, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+  }
+  if (mm[0] == mm[1]) return hipSuccess;
+  const uint32_t bits = 64 - __builtin_clzll(mm[1] - mm[0]);
+  // the need-th smallest relative word, 12 bits at a time from the top;
+  // per-workgroup histograms in keys_b (n u64 words hold 2n counts)
+  const uint64_t hb = 2 * n / kSelBins;
+  const uint32_t hblocks = (uint32_t)(hb < kSelBlocks ? hb : kSelBlocks);
+  uint32_t* hist = reinterpret_cast<uint32_t*>(S.keys_b);
+  uint32_t* sums = S.pos_a;  // kSelBins counts (n > kSelBins)
+  uint64_t prefix = 0, remaining = need, m = 0;
+  uint32_t done_bits = 0;
+  std::vector<uint32_t> h(kSelBins);
+  while (done_bits < bits) {
+    const uint32_t width = bits - done_bits < 12 ? bits - done_bits : 12;
+    const uint32_t shift = bits - done_bits - width;
+    hipLaunchKernelGGL(sel_hist_kernel, dim3(hblocks), dim3(256), 0, s, (const uint64_t*)w, n,
+                       (const uint64_t*)S.mm, prefix, shift + width, shift, (1u << width) - 1, hist);
+    hipLaunchKernelGGL(sel_reduce_kernel, dim3(kSelBins / 256), dim3(256), 0, s, (const uint32_t*)hist, hblocks,
+                       sums);
+    hipError_t e = hipMemcpyAsync(h.data(), sums, (4u << width), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    uint64_t below = 0;
+    uint32_t v = 0;
+    for (; v < (1u << width); v++) {
+      if (below + h[v] >= remaining) break;
+      below += h[v];
+    }
+    if (v == (1u << width)) return hipErrorUnknown;  // fewer than `need` words: need <= n rules it out
+    remaining -= below;
+    m += below;  // every word counted below the digit lies below the cut
+    prefix = (prefix << width) | v;
+    done_bits += width;
+    if (done_bits == bits) m += h[v];  // and the words equal to it
+  }
+  const uint64_t cut = prefix;  // m words lie at or below it
+  if (m < need) return hipErrorUnknown;
+  if (m <= kSmallSort) {
+    // take the candidates in any order, then sort them with their slot
+    // index as the last word
+    uint32_t* idx = S.pos_b;
+    uint32_t* d_count = reinterpret_cast<uint32_t*>(S.total);
+    hipError_t e = launch_fill_u32(d_count, 1, 0, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sel_take_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, (const uint64_t*)w, n,
+                       (const uint64_t*)S.mm, cut, (uint32_t)m, idx, d_count);
+    hipLaunchKernelGGL(sel_comps_kernel, dim3(grid_for(m, 256, 16)), dim3(256), 0, s, p, gids, n, (const uint32_t*)idx,
+                       (uint32_t)m, S.comps);
+    hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps + 1, m,
+                       (const uint32_t*)nullptr, (uint32_t)m, S.pos_a);
+    hipLaunchKernelGGL(sel_out_kernel, dim3(grid_for(need, 256, 16)), dim3(256), 0, s, gids, n, (const uint32_t*)idx,
+                       (const uint32_t*)S.pos_a, need, out_sel);
+    *done = true;
+    return hipGetLastError();
+  }
+  // many ties at the cut: gather the candidates in slot order (ties still
+  // break by slot) and order them with the general path, in scratch below them
+  const uint32_t blocks = (uint32_t)((n + kCompactChunk - 1) / kCompactChunk);
+  uint32_t* counts = S.pos_a;
+  uint32_t* cg = S.pos_b;
+  const size_t below = (size_t)(reinterpret_cast<uint8_t*>(cg) - reinterpret_cast<uint8_t*>(S.comps));
+  if (sort_scratch_bytes(m, p.n_comps) > below) return hipSuccess;
+  hipLaunchKernelGGL(sel_count_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, (const uint64_t*)w, n,
+                     (const uint64_t*)S.mm, cut, counts);
+  hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, counts, blocks, S.total);
+  hipLaunchKernelGGL(sel_write_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, (const uint64_t*)w, n,
+                     (const uint64_t*)S.mm, cut, (const uint32_t*)counts, gids, cg);
+  SortScratch C = carve(S.comps, m, p.n_comps);
+  hipError_t e = sort_general(p, cg, m, need, C, out_sel, s);
+  if (e == hipSuccess) *done = true;
+  return e;
+}
+
 // Writes the first `need` gids in ORDER BY order into out_sel.
 hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_t need, void* scratch,
                        size_t scratch_bytes, uint32_t* out_sel, hipStream_t s) {
   if (n == 0 || need == 0) return hipSuccess;
   if (scratch_bytes < sort_scratch_bytes(n, p.n_comps)) return hipErrorInvalidValue;
   SortScratch S = carve(scratch, n, p.n_comps);
+  const char* sel_env = getenv("THINDB_GPU_TOPN_SELECT");  // "0": always order every group
+  const bool select_on = !(sel_env && sel_env[0] == '0');
+  if (select_on && n > kSelect1 && need <= kSmallSort && p.n_comps > 0) {
+    bool done = false;
+    hipError_t e = select_large(p, gids, n, need, S, out_sel, s, &done);
+    if (e != hipSuccess || done) return e;
+  }
+  return sort_general(p, gids, n, need, S, out_sel, s);
+}
+
+// Every order word of all n groups, then one of: slot order (every word
+// constant), a one-workgroup sort, a one-workgroup select of the first word,
+// repeated per-workgroup top-k, or an LSD radix sort of all positions.
+static hipError_t sort_general(const OrderParams& p, const uint32_t* gids, uint64_t n, uint64_t need, SortScratch& S,
+                               uint32_t* out_sel, hipStream_t s) {
   hipLaunchKernelGGL(make_comps_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, p, (const uint32_t*)gids,
                      n, S.comps);
   // Words equal across every group decide nothing (a DECIMAL SUM's high
