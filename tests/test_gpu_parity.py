@@ -256,10 +256,15 @@ def test_radix_partition_passes(session, oracle_lib, monkeypatch, leaf_bits, bit
     monkeypatch.setenv("THINDB_GPU_RADIX_LEAF_BITS", leaf_bits)
     if bits1:
         monkeypatch.setenv("THINDB_GPU_RADIX_BITS1", bits1)
+    # est_groups=100 (under half the domain) makes the leaves emit compacted
+    # keyed records instead of every key of the domain
     for req in [Request(["k1", "k2"], ALL_AGGS, where_filter=where),
                 Request(["k1", "k2"], ALL_AGGS, [SortSpec("c", True)], limit=25),
                 Request(["k2", "k1"], ALL_AGGS, [SortSpec("sb", True), SortSpec("k1")], limit=40, offset=3),
-                Request(["k1"], ALL_AGGS[:5], [SortSpec("k1")])]:
+                Request(["k1"], ALL_AGGS[:5], [SortSpec("k1")]),
+                Request(["k1", "k2"], ALL_AGGS, where_filter=where, est_groups=100),
+                Request(["k2", "k1"], ALL_AGGS, [SortSpec("sb", True), SortSpec("k1")], limit=40, offset=3,
+                        est_groups=100)]:
         got, want, st = check_vs_oracle(session, oracle_lib, t, req)
         assert st.global_mode == 3
 

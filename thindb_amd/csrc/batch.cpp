@@ -349,7 +349,6 @@ int32_t build(tdbg_batch* b) {
       q.rx.bucket_start = at<uint64_t>(b->d_misc, misc_o[i]);
       q.rx.bucket_fill = q.rx.bucket_start + nb + 1;
       q.rx.leaf_start = nullptr;
-      q.rx.out_cursor = q.rx.bucket_fill + nb;
     }
   }
   // descriptors: staged back to back in pinned memory, copied asynchronously

@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(kBatchThreads) batch_kernel(const BatchQuery* 
       }
       default: {  // BI_RX_AGG
         if (batch_wait(&B.sync[2], B.n_scatter, spin_limit, err, &s_ok))
-          for (uint32_t leaf = it.a0; leaf < it.a1; leaf++) rx_agg_leaf<true>(P, B.rx, leaf, lds, nullptr, nullptr);
+          for (uint32_t leaf = it.a0; leaf < it.a1; leaf++) rx_agg_leaf<true>(P, B.rx, leaf, lds, nullptr);
         break;
       }
     }

@@ -147,12 +147,16 @@ struct RadixParams {
   uint32_t tile_rows;          // rows a scatter workgroup stages in LDS at once (multiple of 256)
   uint32_t dense_out;          // leaves write every key of their range at record index key
                                // (G_DENSE table) instead of compacted keyed records
+  uint32_t chunks2;            // level-2 workgroups per level-1 bucket (<= 64)
+  uint32_t tile_rows2;         // rows a level-2 scatter workgroup stages at once (multiple of 256, <= 8192)
+  uint32_t* hist2;             // [2^bits1 * chunks2 * 2^bits2] chunk digit counts, then offsets
   uint8_t* rows1;              // level-1 partitioned rows
   uint8_t* rows2;              // level-2 partitioned rows (== rows1 when one level)
   uint64_t* bucket_start;      // [2^bits1 + 1] level-1 bucket starts (from the histogram)
   uint64_t* bucket_fill;       // [2^bits1] level-1 cursors: rows written per bucket
   uint64_t* leaf_start;        // [n_leaves + 1] leaf row ranges in the final array
-  uint64_t* out_cursor;        // groups emitted
+  uint32_t* leaf_groups;       // [n_leaves] groups each leaf wrote (compacted records from its first row),
+                               // then scratch for their positions: [1 + n_leaves / 256 + 1] words
 };
 
 // ---- ordering / emission -------------------------------------------------
@@ -290,6 +294,9 @@ hipError_t launch_iota(uint32_t* p, uint64_t n, hipStream_t s);
 enum RadixPhase : int { RX_HIST = 0, RX_OFFSETS, RX_SCATTER1, RX_SCATTER2, RX_AGG };
 hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid, unsigned long long* hist,
                         hipStream_t s, int phase);
+// After RX_AGG of compacted leaves: the group count to *n_out and the
+// groups' record positions, in key order, to gids.
+hipError_t launch_radix_gids(const RadixParams& r, uint32_t* gids, uint64_t* n_out, hipStream_t s);
 // Batched execution (batch.hip): record init over `blocks`, the persistent
 // kernel over `items` (grid workgroups of kBatchThreads, lds_bytes dynamic
 // LDS each; *head must start at 0), compaction and finalize.

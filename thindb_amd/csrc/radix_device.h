@@ -140,29 +140,52 @@ __device__ __forceinline__ void rx_item_offsets(const RadixParams& R, uint32_t* 
   __syncthreads();
 }
 
+// Exclusive prefix of one value per thread over the workgroup (blockDim.x a
+// multiple of 64): wavefront scans by shuffles, one pass over the wavefront
+// totals.  *total gets the sum; tmp: blockDim.x / 64 + 1 LDS words.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* tmp, uint32_t* total) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) tmp[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    const uint32_t t = lane < nw ? tmp[lane] : 0u;
+    uint32_t u = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(u, o);
+      if (lane >= (uint32_t)o) u += y;
+    }
+    if (lane < nw) tmp[lane] = u - t;
+    if (lane == nw - 1) tmp[nw] = u;
+  }
+  __syncthreads();
+  const uint32_t r = tmp[wave] + x - v;
+  *total = tmp[nw];
+  __syncthreads();  // tmp is free again
+  return r;
+}
+
 // Exclusive prefix over nb (<= blockDim.x * per) LDS counters into `off`;
-// returns the total.
+// returns the total.  tmp: blockDim.x / 64 + 1 LDS words.
 __device__ __forceinline__ uint32_t rx_prefix(const uint32_t* cnt, uint32_t* off, uint32_t nb, uint32_t* tmp) {
   const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
   const uint32_t lo = threadIdx.x * per;
   uint32_t s = 0;
   for (uint32_t i = 0; i < per; i++)
     if (lo + i < nb) s += cnt[lo + i];
-  tmp[threadIdx.x] = s;
-  __syncthreads();
-  for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
-    const uint32_t x = threadIdx.x >= o ? tmp[threadIdx.x - o] : 0;
-    __syncthreads();
-    tmp[threadIdx.x] += x;
-    __syncthreads();
-  }
-  uint32_t run = tmp[threadIdx.x] - s;
+  uint32_t total;
+  uint32_t run = block_excl_scan(s, tmp, &total);
   for (uint32_t i = 0; i < per; i++)
     if (lo + i < nb) {
       off[lo + i] = run;
       run += cnt[lo + i];
     }
-  const uint32_t total = tmp[blockDim.x - 1];
   __syncthreads();
   return total;
 }
@@ -398,70 +421,86 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
 // Aggregates leaf `leaf` in an LDS direct table and writes its groups: with
 // DENSE_OUT every key of the leaf's range below g_cap at record index key
 // (empty keys get a zero count), else the occupied slots in slot order to
-// consecutive records claimed from R.out_cursor (tmp: blockDim.x LDS words).
+// consecutive records from the leaf's first row index (a leaf holds at least
+// as many rows as groups, so leaves never overlap), and the number of
+// groups to R.leaf_groups[leaf].  tmp: blockDim.x / 64 + 1 + cap / 64 LDS
+// words (unused with DENSE_OUT).
 template <bool DENSE_OUT>
 __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint8_t* lds,
-                                            uint32_t* tmp, uint64_t* out_base) {
+                                            uint32_t* tmp) {
   const uint32_t cap = P.lds_cap, stride = R.row.stride;
   const uint64_t mask = (uint64_t)cap - 1u;
-  {
-    uint64_t s, e;
-    if (R.bits2) {
-      s = R.leaf_start[2 * (uint64_t)leaf];
-      e = R.leaf_start[2 * (uint64_t)leaf + 1];
-    } else {
-      s = R.bucket_start[leaf];
-      e = R.bucket_fill[leaf];
-    }
-    const uint64_t leaf_key0 = (uint64_t)leaf << R.leaf_bits;
-    if (s == e) {  // uniform
-      if (DENSE_OUT)
-        for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x)
-          *reinterpret_cast<uint64_t*>(P.g_state + (leaf_key0 + slot) * P.g_rec + P.g_count_off) = 0;
-      return;
-    }
-    lds_init(P, lds);
-    __syncthreads();
-    for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
-      const uint8_t* row = R.rows2 + i * stride;
-      const uint32_t key = *reinterpret_cast<const uint32_t*>(row);
-      rx_fold_row(P, R, lds, row, (uint32_t)(key & mask));
-    }
-    __syncthreads();
-    if (DENSE_OUT) {
-      for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x) {
-        uint8_t* rec = P.g_state + (leaf_key0 + slot) * P.g_rec;
-        if (lds_at<uint32_t>(lds, P.lds_count_off)[slot]) rx_write_record(P, lds, slot, leaf_key0 + slot, rec);
-        else *reinterpret_cast<uint64_t*>(rec + P.g_count_off) = 0;
-      }
-      __syncthreads();
-      return;
-    }
-    // occupied slots, in slot order, to consecutive records
-    const uint32_t per = (cap + blockDim.x - 1) / blockDim.x;
-    const uint32_t lo = threadIdx.x * per;
-    uint32_t mine = 0;
-    for (uint32_t k = 0; k < per; k++)
-      if (lo + k < cap && lds_at<uint32_t>(lds, P.lds_count_off)[lo + k]) mine++;
-    tmp[threadIdx.x] = mine;
-    __syncthreads();
-    for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
-      const uint32_t x = threadIdx.x >= o ? tmp[threadIdx.x - o] : 0;
-      __syncthreads();
-      tmp[threadIdx.x] += x;
-      __syncthreads();
-    }
-    if (threadIdx.x == blockDim.x - 1)
-      *out_base = atomicAdd(reinterpret_cast<unsigned long long*>(R.out_cursor), (unsigned long long)tmp[blockDim.x - 1]);
-    __syncthreads();
-    uint64_t at = *out_base + tmp[threadIdx.x] - mine;
-    for (uint32_t k = 0; k < per; k++) {
-      const uint32_t slot = lo + k;
-      if (slot < cap && lds_at<uint32_t>(lds, P.lds_count_off)[slot])
-        rx_write_record(P, lds, slot, leaf_key0 + slot, P.g_state + (at++) * P.g_rec);
-    }
-    __syncthreads();
+  uint64_t s, e;
+  if (R.bits2) {
+    s = R.leaf_start[2 * (uint64_t)leaf];
+    e = R.leaf_start[2 * (uint64_t)leaf + 1];
+  } else {
+    s = R.bucket_start[leaf];
+    e = R.bucket_fill[leaf];
   }
+  const uint64_t leaf_key0 = (uint64_t)leaf << R.leaf_bits;
+  if (s == e) {  // uniform
+    if (DENSE_OUT) {
+      for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x)
+        *reinterpret_cast<uint64_t*>(P.g_state + (leaf_key0 + slot) * P.g_rec + P.g_count_off) = 0;
+    } else if (threadIdx.x == 0) {
+      R.leaf_groups[leaf] = 0;
+    }
+    return;
+  }
+  lds_init(P, lds);
+  __syncthreads();
+  for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+    const uint8_t* row = R.rows2 + i * stride;
+    const uint32_t key = *reinterpret_cast<const uint32_t*>(row);
+    rx_fold_row(P, R, lds, row, (uint32_t)(key & mask));
+  }
+  __syncthreads();
+  const uint32_t* count = lds_at<uint32_t>(lds, P.lds_count_off);
+  if (DENSE_OUT) {
+    for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x) {
+      uint8_t* rec = P.g_state + (leaf_key0 + slot) * P.g_rec;
+      if (count[slot]) rx_write_record(P, lds, slot, leaf_key0 + slot, rec);
+      else *reinterpret_cast<uint64_t*>(rec + P.g_count_off) = 0;
+    }
+    __syncthreads();
+    return;
+  }
+  // occupied slots in slot order: slot k * blockDim.x + t is handled by
+  // thread t in round k, so each record store is coalesced; per (round,
+  // wavefront) counts are scanned in that order
+  const uint32_t nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
+  const uint32_t rounds = (cap + blockDim.x - 1) / blockDim.x;
+  uint32_t* scan_tmp = tmp;
+  uint32_t* wcnt = tmp + nw + 1;  // rounds * nw entries
+  for (uint32_t k = 0; k < rounds; k++) {
+    const uint32_t slot = k * blockDim.x + threadIdx.x;
+    const uint64_t m = __ballot(slot < cap && count[slot] != 0);
+    if ((threadIdx.x & 63) == 0) wcnt[k * nw + wave] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  uint32_t total = 0;
+  for (uint32_t b = 0; b < rounds * nw; b += blockDim.x) {  // uniform; one pass while rounds * nw <= blockDim.x
+    const uint32_t t = b + threadIdx.x;
+    const uint32_t v = t < rounds * nw ? wcnt[t] : 0u;
+    uint32_t part;
+    const uint32_t ex = block_excl_scan(v, scan_tmp, &part);
+    if (t < rounds * nw) wcnt[t] = total + ex;
+    total += part;
+  }
+  __syncthreads();
+  for (uint32_t k = 0; k < rounds; k++) {
+    const uint32_t slot = k * blockDim.x + threadIdx.x;
+    const bool occ = slot < cap && count[slot] != 0;
+    const uint64_t m = __ballot(occ);
+    if (occ) {
+      const uint32_t lane = threadIdx.x & 63;
+      const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      rx_write_record(P, lds, slot, leaf_key0 + slot, P.g_state + at * P.g_rec);
+    }
+  }
+  if (threadIdx.x == 0) R.leaf_groups[leaf] = total;
+  __syncthreads();
 }
 
 }  // namespace tdbg

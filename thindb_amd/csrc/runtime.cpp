@@ -634,6 +634,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     R.row.stride = (uint32_t)align_up(ro, 8);
     // staged scatter tile: 24 KiB of digit tables + tile_rows rows in LDS
     R.tile_rows = std::min<uint32_t>(4096, (budget - 24 * 1024) / R.row.stride / 256 * 256);
+    R.tile_rows2 = std::min<uint32_t>(8192, (budget - 24 * 1024) / R.row.stride / 256 * 256);
     const uint32_t rem = L - leaf_bits;
     // one level up to 2^10 buckets, else 2^max_bits1 x the rest
     const uint32_t max_bits1 = std::min<uint32_t>(hint.max_bits1 ? hint.max_bits1 : 8u, 10u);
@@ -672,7 +673,8 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
           P.tmpl.insert(P.tmpl.begin(), kEmptyKey);
         }
         sp.g_mode = G_HASH;
-        sp.g_cap = std::max<uint64_t>(1, std::min<uint64_t>(dom, n_rows_scan));
+        // a leaf's records start at its first row index: one per row
+        sp.g_cap = std::max<uint64_t>(1, n_rows_scan);
       }
     }
   }
@@ -1145,16 +1147,21 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     const uint64_t row_bytes = rows * R.row.stride + 256;
     if ((rc = ensure(pl, pl->rx_rows1, row_bytes))) return rc;
     if (R.bits2 && (rc = ensure(pl, pl->rx_rows2, row_bytes))) return rc;
-    const uint64_t misc_words = nb1 + (nb1 + 1) + nb1 + 1;
+    // leaf_groups: n_leaves counts, a total, then one sum per 256 leaves
+    const uint64_t misc_words = nb1 + (nb1 + 1) + nb1 + (R.n_leaves + 1 + R.n_leaves / 256 + 2) / 2 + 1;
     if ((rc = ensure(pl, pl->rx_misc, misc_words * 8 + 256))) return rc;
-    if (R.bits2 && (rc = ensure(pl, pl->rx_leaf, 2ull * R.n_leaves * 8 + 256))) return rc;
+    // level 2: about four workgroups per CU over all buckets
+    R.chunks2 = R.bits2 ? (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, 4ull * ctx->n_cu / nb1)) : 0;
+    const uint64_t hist2_b = R.bits2 ? nb1 * R.chunks2 * (1ull << R.bits2) * 4 : 0;
+    if (R.bits2 && (rc = ensure(pl, pl->rx_leaf, 2ull * R.n_leaves * 8 + 256 + hist2_b))) return rc;
     uint64_t* misc = static_cast<uint64_t*>(pl->rx_misc.p);
     R.bucket_start = misc + nb1;
     R.bucket_fill = R.bucket_start + nb1 + 1;
-    R.out_cursor = R.bucket_fill + nb1;
+    R.leaf_groups = reinterpret_cast<uint32_t*>(R.bucket_fill + nb1);
     R.rows1 = static_cast<uint8_t*>(pl->rx_rows1.p);
     R.rows2 = R.bits2 ? static_cast<uint8_t*>(pl->rx_rows2.p) : R.rows1;
     R.leaf_start = R.bits2 ? static_cast<uint64_t*>(pl->rx_leaf.p) : nullptr;
+    R.hist2 = R.bits2 ? reinterpret_cast<uint32_t*>(R.leaf_start + 2ull * R.n_leaves + 32) : nullptr;
     auto* hist = reinterpret_cast<unsigned long long*>(misc);
     HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(misc), misc_words * 2, 0, s), "init histogram");
@@ -1167,7 +1174,8 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     HIP_TRY(launch_radix(sp, R, P.agg_grid, hist, s, RX_AGG), "radix aggregate");
     HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
     if (!R.dense_out)
-      HIP_TRY(hipMemcpyAsync(pl->nout.p, R.out_cursor, 8, hipMemcpyDeviceToDevice, s), "group count");
+      HIP_TRY(launch_radix_gids(R, static_cast<uint32_t*>(pl->gids.p), static_cast<uint64_t*>(pl->nout.p), s),
+              "group positions");
   } else {
     HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
     HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
@@ -1180,7 +1188,7 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
   }
   if (pl->cancel.load()) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
-  // compaction (compacted radix records are already in place: identity positions)
+  // compaction (compacted radix leaves listed their groups' positions above)
   if (!P.radix || P.rx.dense_out) {
     if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
     HIP_TRY(launch_compact(P.op, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->gids.p),
@@ -1194,11 +1202,9 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   if (err_bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
   if (err_bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
   if (pl->cancel.load()) return fail(TDBG_ERR_CANCELLED, "cancelled");
-  if (P.radix && !P.rx.dense_out) {
-    if (n_groups > n_rec) return fail(TDBG_ERR_DEVICE, "radix pass emitted %llu groups for %llu records",
-                                      (unsigned long long)n_groups, (unsigned long long)n_rec);
-    HIP_TRY(launch_iota(static_cast<uint32_t*>(pl->gids.p), n_groups, s), "group positions");
-  }
+  if (P.radix && !P.rx.dense_out && n_groups > n_rec)
+    return fail(TDBG_ERR_DEVICE, "radix pass emitted %llu groups for %llu records", (unsigned long long)n_groups,
+                (unsigned long long)n_rec);
   // ORDER BY / OFFSET / LIMIT
   const uint64_t start = std::min<uint64_t>(pl->offset, n_groups);
   const uint64_t end = std::min<uint64_t>(need_lim, n_groups);
