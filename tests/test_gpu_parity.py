@@ -300,6 +300,18 @@ def test_topn_selection_many_groups(session, oracle_lib, monkeypatch):
         # place between runs (the order of float atomics)
         assert [r[:nk] for r in every] == [r[:nk] for r in got]
         assert_same_rows(every, got, float_cols_of(t, req), REL, nk)
+    # the radix path's compacted leaves hand the selection its first order
+    # word: (k1, k2) spans 2.8M keys, over twice the groups, so the leaves
+    # emit compacted records
+    monkeypatch.setenv("THINDB_GPU_TOPN_SELECT", "1")
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    for req in [Request(["k1", "k2"], ALL_AGGS, [SortSpec("sb", True)], limit=100),
+                Request(["k1", "k2"], ALL_AGGS, [SortSpec("ad"), SortSpec("k2", True)], limit=37, offset=5),
+                Request(["k2", "k1"], ALL_AGGS, [SortSpec("sm", True)], limit=4096),
+                Request(["k1", "k2"], ALL_AGGS, [SortSpec("c", True), SortSpec("mins")], limit=64),
+                Request(["k1", "k2"], ALL_AGGS, [SortSpec("c")], limit=10)]:
+        got, want, st = check_vs_oracle(session, oracle_lib, t, req)
+        assert st.global_mode == 3 and st.n_groups > 65_536
 
 
 def test_hash_table_mode(session, oracle_lib):

@@ -150,6 +150,8 @@ struct RadixParams {
   uint32_t chunks2;            // level-2 workgroups per level-1 bucket (<= 64)
   uint32_t tile_rows2;         // rows a level-2 scatter workgroup stages at once (multiple of 256, <= 8192)
   uint32_t* hist2;             // [2^bits1 * chunks2 * 2^bits2] chunk digit counts, then offsets
+  uint32_t* counts_out;        // compacted leaves: each group's row count by record index, or null
+  uint32_t* count_hist;        // [kRxCountBins] groups by min(row count, kRxCountBins - 1) (zeroed)
   uint8_t* rows1;              // level-1 partitioned rows
   uint8_t* rows2;              // level-2 partitioned rows (== rows1 when one level)
   uint64_t* bucket_start;      // [2^bits1 + 1] level-1 bucket starts (from the histogram)
@@ -285,6 +287,13 @@ hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_
                           uint64_t* n_out, hipStream_t s);
 hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_t need, void* scratch,
                        size_t scratch_bytes, uint32_t* out_sel, hipStream_t s);
+// The first `need` of m candidate groups gids[idx[0..m)] (any order, m <=
+// kSortCandidatesMax, need <= m) in ORDER BY order into out_sel; ties
+// break by position in gids.  scratch: sort_candidates_scratch_bytes(m).
+constexpr uint32_t kSortCandidatesMax = 4096;
+size_t sort_candidates_scratch_bytes(uint32_t m, uint32_t n_comps);
+hipError_t launch_sort_candidates(const OrderParams& p, const uint32_t* gids, uint64_t n, const uint32_t* idx,
+                                  uint32_t m, uint64_t need, void* scratch, uint32_t* out_sel, hipStream_t s);
 size_t sort_scratch_bytes(uint64_t n, uint32_t n_comps);
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
                            hipStream_t s);
@@ -294,6 +303,13 @@ hipError_t launch_iota(uint32_t* p, uint64_t n, hipStream_t s);
 enum RadixPhase : int { RX_HIST = 0, RX_OFFSETS, RX_SCATTER1, RX_SCATTER2, RX_AGG };
 hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid, unsigned long long* hist,
                         hipStream_t s, int phase);
+// ORDER BY COUNT(*) over compacted leaves (r.counts_out set): the indexes
+// into the group positions of every group whose count is >= cut (desc) or
+// <= cut (asc), in any order, to idx (cap entries); their number to *n_idx
+// (zeroed here).
+constexpr uint32_t kRxCountBins = 256;
+hipError_t launch_radix_count_take(const RadixParams& r, uint32_t cut, bool desc, uint32_t* idx, uint32_t cap,
+                                   uint32_t* n_idx, hipStream_t s);
 // After RX_AGG of compacted leaves: the group count to *n_out and the
 // groups' record positions, in key order, to gids.
 hipError_t launch_radix_gids(const RadixParams& r, uint32_t* gids, uint64_t* n_out, hipStream_t s);

@@ -16,102 +16,9 @@
 #include <cstdlib>
 #include <vector>
 
-#include "engine.h"
+#include "order_device.h"
 
 namespace tdbg {
-
-// ---- shared helpers ----------------------------------------------------------
-
-__device__ __forceinline__ uint64_t ord_i64(int64_t v) { return (uint64_t)v ^ 0x8000000000000000ull; }
-
-// types.floatOrder (src/types.zig:579-587): -0 == +0, NaN largest.
-__device__ __forceinline__ uint64_t ord_f64(double x) {
-  if (x != x) return ~0ull;
-  if (x == 0.0) x = 0.0;
-  uint64_t b = (uint64_t)__double_as_longlong(x);
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
-// Inverse of the order-preserving i64 map used for float MIN/MAX states.
-__device__ __forceinline__ double unord_f64(int64_t k) {
-  int64_t s = k >= 0 ? k : (k ^ 0x7fffffffffffffffll);
-  return __longlong_as_double(s);
-}
-
-// i128 -> f64 with round-to-nearest-even (Zig's @floatFromInt).
-__device__ __forceinline__ double i128_to_f64(__int128 v) {
-  const bool neg = v < 0;
-  unsigned __int128 a = neg ? (unsigned __int128)(-(v + 1)) + 1u : (unsigned __int128)v;
-  const uint64_t hi = (uint64_t)(a >> 64);
-  double r;
-  if (hi == 0) {
-    r = (double)(uint64_t)a;  // u64 -> f64 is correctly rounded
-  } else {
-    const int lz = __clzll(hi);
-    const int drop = 64 - lz;  // bits beyond a 64-bit window
-    uint64_t m = (uint64_t)(a >> drop);
-    const unsigned __int128 rest = a & ((((unsigned __int128)1) << drop) - 1u);
-    if (rest != 0) m |= 1u;  // sticky bit below the rounding position
-    r = ldexp((double)m, drop);
-  }
-  return neg ? -r : r;
-}
-
-__device__ __forceinline__ __int128 load_i128(const uint8_t* p) {
-  const uint64_t lo = *reinterpret_cast<const uint64_t*>(p);
-  const int64_t hi = *reinterpret_cast<const int64_t*>(p + 8);
-  return (__int128)(((unsigned __int128)(uint64_t)hi << 64) | lo);
-}
-
-__device__ __forceinline__ uint64_t group_key(const uint8_t* rec, uint32_t gid, uint32_t g_mode, uint32_t key_off) {
-  return g_mode == G_DENSE ? (uint64_t)gid : *reinterpret_cast<const uint64_t*>(rec + key_off);
-}
-
-__device__ __forceinline__ int64_t part_value(const PartDesc& pt, uint64_t key) {
-  const uint64_t rel = (key / pt.stride) % pt.span;
-  return (int64_t)((uint64_t)pt.base + rel);
-}
-
-__device__ __forceinline__ uint64_t rec_ns(const uint8_t* rec, uint32_t ns_off, uint32_t count_off) {
-  return *reinterpret_cast<const uint64_t*>(rec + (ns_off != ~0u ? ns_off : count_off));
-}
-
-// One ORDER BY word of one group.
-__device__ uint64_t comp_word(const OrderParams& P, const CompDesc& c, uint32_t gid) {
-  const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
-  uint64_t w = 0;
-  switch (c.src) {
-    case C_KEY: w = ord_i64(part_value(P.parts[c.idx], group_key(rec, gid, P.g_mode, P.g_key_off))); break;
-    case C_COUNT: w = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off); break;
-    case C_U64_AT: w = *reinterpret_cast<const uint64_t*>(rec + c.off); break;
-    case C_I64_AT:
-    case C_I128_HI:
-    case C_I128_LO:
-    case C_F64_AT:
-    case C_FORD_AT: {
-      // A group with no non-null input compares as a zero slot
-      // (compareBySpec reads the untouched slot).
-      const bool empty = rec_ns(rec, c.ns_off, P.g_count_off) == 0;
-      if (c.src == C_I64_AT) w = ord_i64(empty ? 0 : *reinterpret_cast<const int64_t*>(rec + c.off));
-      else if (c.src == C_I128_HI) w = ord_i64(empty ? 0 : *reinterpret_cast<const int64_t*>(rec + c.off + 8));
-      else if (c.src == C_I128_LO) w = empty ? 0 : *reinterpret_cast<const uint64_t*>(rec + c.off);
-      else if (c.src == C_F64_AT) w = ord_f64(empty ? 0.0 : *reinterpret_cast<const double*>(rec + c.off));
-      else w = ord_f64(empty ? 0.0 : unord_f64(*reinterpret_cast<const int64_t*>(rec + c.off)));
-      break;
-    }
-    case C_AVG_I: {
-      const uint64_t n = rec_ns(rec, c.ns_off, P.g_count_off);
-      w = ord_f64(n == 0 ? 0.0 : i128_to_f64(load_i128(rec + c.off)) / (double)n);
-      break;
-    }
-    default: {  // C_AVG_F
-      const uint64_t n = rec_ns(rec, c.ns_off, P.g_count_off);
-      w = ord_f64(n == 0 ? 0.0 : *reinterpret_cast<const double*>(rec + c.off) / (double)n);
-      break;
-    }
-  }
-  return c.desc ? ~w : w;
-}
 
 // ---- compaction: occupied records in slot order --------------------------------
 
@@ -862,6 +769,27 @@ __global__ void __launch_bounds__(kCompactThreads) sel_write_kernel(const uint64
   }
 }
 
+static_assert(kSortCandidatesMax == kSmallSort, "candidates are ordered by one workgroup");
+
+size_t sort_candidates_scratch_bytes(uint32_t m, uint32_t n_comps) {
+  return ((size_t)(n_comps + 1) * m * 8 + 255) / 256 * 256 + (size_t)kSmallSort * 4;
+}
+
+hipError_t launch_sort_candidates(const OrderParams& p, const uint32_t* gids, uint64_t n, const uint32_t* idx,
+                                  uint32_t m, uint64_t need, void* scratch, uint32_t* out_sel, hipStream_t s) {
+  if (m == 0 || need == 0) return hipSuccess;
+  if (m > kSmallSort || need > m) return hipErrorInvalidValue;
+  uint64_t* comps = static_cast<uint64_t*>(scratch);
+  uint32_t* pos = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(scratch) +
+                                              ((size_t)(p.n_comps + 1) * m * 8 + 255) / 256 * 256);
+  hipLaunchKernelGGL(sel_comps_kernel, dim3(grid_for(m, 256, 16)), dim3(256), 0, s, p, gids, n, idx, m, comps);
+  hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)comps, p.n_comps + 1, (uint64_t)m,
+                     (const uint32_t*)nullptr, m, pos);
+  hipLaunchKernelGGL(sel_out_kernel, dim3(grid_for(need, 256, 16)), dim3(256), 0, s, gids, n, idx,
+                     (const uint32_t*)pos, need, out_sel);
+  return hipGetLastError();
+}
+
 static hipError_t sort_general(const OrderParams& p, const uint32_t* gids, uint64_t n, uint64_t need, SortScratch& S,
                                uint32_t* out_sel, hipStream_t s);
 
@@ -926,14 +854,9 @@ static hipError_t select_large(const OrderParams& p, const uint32_t* gids, uint6
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sel_take_kernel, dim3(grid_for(n, 256, 8192)), dim3(256), 0, s, (const uint64_t*)w, n,
                        (const uint64_t*)S.mm, cut, (uint32_t)m, idx, d_count);
-    hipLaunchKernelGGL(sel_comps_kernel, dim3(grid_for(m, 256, 16)), dim3(256), 0, s, p, gids, n, (const uint32_t*)idx,
-                       (uint32_t)m, S.comps);
-    hipLaunchKernelGGL(small_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps + 1, m,
-                       (const uint32_t*)nullptr, (uint32_t)m, S.pos_a);
-    hipLaunchKernelGGL(sel_out_kernel, dim3(grid_for(need, 256, 16)), dim3(256), 0, s, gids, n, (const uint32_t*)idx,
-                       (const uint32_t*)S.pos_a, need, out_sel);
-    *done = true;
-    return hipGetLastError();
+    e = launch_sort_candidates(p, gids, n, idx, (uint32_t)m, need, S.comps, out_sel, s);
+    if (e == hipSuccess) *done = true;
+    return e;
   }
   // many ties at the cut: gather the candidates in slot order (ties still
   // break by slot) and order them with the general path, in scratch below them

@@ -235,17 +235,28 @@ constexpr uint32_t kRxAggTmp = 512;  // LDS words for compaction counts: 9 + 2^l
 __global__ void __launch_bounds__(512) rx_agg_kernel(const ScanParams P, const RadixParams R) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t tmp[kRxAggTmp];
+  __shared__ uint32_t chist[kRxCountBins];
+  if (R.counts_out) {  // uniform
+    for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x) chist[b] = 0;
+    __syncthreads();
+  }
   for (uint32_t leaf = blockIdx.x; leaf < R.n_leaves; leaf += gridDim.x) {
     if (R.dense_out) rx_agg_leaf<true>(P, R, leaf, lds, tmp);
-    else rx_agg_leaf<false>(P, R, leaf, lds, tmp);
+    else rx_agg_leaf<false>(P, R, leaf, lds, tmp, chist);
+  }
+  if (R.counts_out) {
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x)
+      if (chist[b]) atomicAdd(&R.count_hist[b], chist[b]);
   }
 }
 
 // Compacted leaves: group positions, leaf by leaf.  The per-leaf counts
 // are summed per block of 256 leaves (rx_gsum), the block sums scanned
-// (rx_gscan: the total to *n_out), and each block then scans its own counts
-// and writes its leaves' positions, one wavefront per leaf (rx_gids).  The
-// block sums sit after leaf_groups[n_leaves].
+// (rx_gscan: the total to *n_out and leaf_groups[n_leaves]), and each block
+// then scans its own counts, turns them into the leaves' first positions in
+// place and writes its leaves' positions, one wavefront per leaf (rx_gids).
+// The block sums sit after leaf_groups[n_leaves].
 constexpr uint32_t kRxGidsBlock = 256;
 
 __global__ void __launch_bounds__(kRxGidsBlock) rx_gsum_kernel(const RadixParams R) {
@@ -278,7 +289,10 @@ __global__ void __launch_bounds__(1024) rx_gscan_kernel(const RadixParams R, uin
       sums[lo + i] = run;
       run += c;
     }
-  if (threadIdx.x == 0) *n_out = total;
+  if (threadIdx.x == 0) {
+    *n_out = total;
+    R.leaf_groups[R.n_leaves] = total;
+  }
 }
 
 __global__ void __launch_bounds__(kRxGidsBlock) rx_gids_kernel(const RadixParams R, uint32_t* gids) {
@@ -291,6 +305,7 @@ __global__ void __launch_bounds__(kRxGidsBlock) rx_gids_kernel(const RadixParams
   const uint32_t ex = block_excl_scan(c, tmp, &total);
   s_off[threadIdx.x] = R.leaf_groups[R.n_leaves + 1 + blockIdx.x] + ex;
   s_cnt[threadIdx.x] = c;
+  if (leaf < R.n_leaves) R.leaf_groups[leaf] = s_off[threadIdx.x];  // counts become offsets
   s_row[threadIdx.x] = leaf < R.n_leaves ? (R.bits2 ? R.leaf_start[2 * (uint64_t)leaf] : R.bucket_start[leaf]) : 0;
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63, w0 = (threadIdx.x >> 6) * 64;
@@ -299,6 +314,43 @@ __global__ void __launch_bounds__(kRxGidsBlock) rx_gids_kernel(const RadixParams
     const uint64_t r = s_row[k];
     for (uint32_t j = lane; j < n; j += 64) gids[(uint64_t)o + j] = (uint32_t)(r + j);
   }
+}
+
+// ORDER BY COUNT(*): one wavefront per leaf (after rx_gids) appends the
+// position index of each group whose count passes the cut.
+__global__ void __launch_bounds__(256) rx_count_take_kernel(const RadixParams R, uint32_t cut, uint32_t desc,
+                                                            uint32_t* idx, uint32_t cap, uint32_t* n_idx) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t leaf = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); leaf < R.n_leaves; leaf += waves) {
+    const uint32_t o = R.leaf_groups[leaf];
+    const uint32_t n = R.leaf_groups[leaf + 1] - o;  // offsets, and the total at n_leaves (rx_gids)
+    const uint64_t r = R.bits2 ? R.leaf_start[2 * (uint64_t)leaf] : R.bucket_start[leaf];
+    for (uint32_t j0 = 0; j0 < n; j0 += 64) {  // uniform per wavefront
+      const uint32_t j = j0 + lane;
+      const uint32_t c = j < n ? R.counts_out[r + j] : 0u;
+      const bool take = j < n && (desc ? c >= cut : c <= cut);
+      const uint64_t m = __ballot(take);
+      if (m == 0) continue;
+      const int first = __ffsll((long long)m) - 1;
+      uint32_t base = 0;
+      if ((int)lane == first) base = atomicAdd(n_idx, (uint32_t)__popcll(m));
+      base = __shfl(base, first);
+      const uint32_t at = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      if (take && at < cap) idx[at] = o + j;
+    }
+  }
+}
+
+__global__ void rx_zero_kernel(uint32_t* p) { *p = 0; }
+
+hipError_t launch_radix_count_take(const RadixParams& r, uint32_t cut, bool desc, uint32_t* idx, uint32_t cap,
+                                   uint32_t* n_idx, hipStream_t s) {
+  if (r.dense_out || !r.counts_out || !r.leaf_groups || r.n_leaves == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(rx_zero_kernel, dim3(1), dim3(1), 0, s, n_idx);
+  const uint32_t grid = (r.n_leaves + 3) / 4 < 16384 ? (r.n_leaves + 3) / 4 : 16384;
+  hipLaunchKernelGGL(rx_count_take_kernel, dim3(grid), dim3(256), 0, s, r, cut, desc ? 1u : 0u, idx, cap, n_idx);
+  return hipGetLastError();
 }
 
 hipError_t launch_radix_gids(const RadixParams& r, uint32_t* gids, uint64_t* n_out, hipStream_t s) {
@@ -346,7 +398,10 @@ hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid
       hipLaunchKernelGGL(rx_scatter2_kernel, g, dim3(kRxThreads), rx_scatter2_lds_bytes(r), s, r);
       break;
     }
-    case RX_AGG: hipLaunchKernelGGL(rx_agg_kernel, dim3(grid), dim3(512), p.lds_bytes, s, p, r); break;
+    case RX_AGG:
+      if (r.counts_out && (r.dense_out || !r.count_hist)) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(rx_agg_kernel, dim3(grid), dim3(512), p.lds_bytes, s, p, r);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -425,9 +425,13 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
 // as many rows as groups, so leaves never overlap), and the number of
 // groups to R.leaf_groups[leaf].  tmp: blockDim.x / 64 + 1 + cap / 64 LDS
 // words (unused with DENSE_OUT).
+//
+// With R.counts_out (compacted only) each written group's row count also
+// goes to R.counts_out[record] and into chist (kRxCountBins LDS counters of
+// min(count, kRxCountBins - 1)).
 template <bool DENSE_OUT>
 __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint8_t* lds,
-                                            uint32_t* tmp) {
+                                            uint32_t* tmp, uint32_t* chist = nullptr) {
   const uint32_t cap = P.lds_cap, stride = R.row.stride;
   const uint64_t mask = (uint64_t)cap - 1u;
   uint64_t s, e;
@@ -489,14 +493,23 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
     total += part;
   }
   __syncthreads();
+  const uint32_t lane = threadIdx.x & 63;
   for (uint32_t k = 0; k < rounds; k++) {
     const uint32_t slot = k * blockDim.x + threadIdx.x;
     const bool occ = slot < cap && count[slot] != 0;
     const uint64_t m = __ballot(occ);
-    if (occ) {
-      const uint32_t lane = threadIdx.x & 63;
-      const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-      rx_write_record(P, lds, slot, leaf_key0 + slot, P.g_state + at * P.g_rec);
+    const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    if (occ) rx_write_record(P, lds, slot, leaf_key0 + slot, P.g_state + at * P.g_rec);
+    if (R.counts_out) {  // uniform
+      const uint32_t c = occ ? count[slot] : 0u;
+      if (occ) R.counts_out[at] = c;
+      const uint32_t b = c < kRxCountBins ? c : kRxCountBins - 1;
+      // the common small counts: one LDS atomic per wavefront and value
+      for (uint32_t v = 1; v <= 3; v++) {
+        const uint64_t mv = __ballot(occ && b == v);
+        if (lane == 0 && mv) atomicAdd(&chist[v], (uint32_t)__popcll(mv));
+      }
+      if (occ && b > 3) atomicAdd(&chist[b], 1u);
     }
   }
   if (threadIdx.x == 0) R.leaf_groups[leaf] = total;

@@ -1011,7 +1011,7 @@ void tdbg_plan_destroy(tdbg_plan* pl) {
   (void)hipSetDevice(pl->ctx->device);
   if (pl->stream) (void)hipStreamSynchronize(pl->stream);
   for (auto* b : {&pl->state, &pl->tiles, &pl->err, &pl->gids, &pl->counts, &pl->sel, &pl->scratch, &pl->out, &pl->tmpl, &pl->nout,
-                 &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf})
+                 &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf, &pl->rx_words})
     release(pl, *b);
   for (auto& e : pl->ev) if (e) (void)hipEventDestroy(e);
   if (pl->h_in) (void)hipHostFree(pl->h_in);
@@ -1147,8 +1147,10 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     const uint64_t row_bytes = rows * R.row.stride + 256;
     if ((rc = ensure(pl, pl->rx_rows1, row_bytes))) return rc;
     if (R.bits2 && (rc = ensure(pl, pl->rx_rows2, row_bytes))) return rc;
-    // leaf_groups: n_leaves counts, a total, then one sum per 256 leaves
-    const uint64_t misc_words = nb1 + (nb1 + 1) + nb1 + (R.n_leaves + 1 + R.n_leaves / 256 + 2) / 2 + 1;
+    // leaf_groups: n_leaves counts, a total, then one sum per 256 leaves;
+    // then the groups' count histogram
+    const uint64_t lg_words = (R.n_leaves + 1 + R.n_leaves / 256 + 2) / 2 + 1;
+    const uint64_t misc_words = nb1 + (nb1 + 1) + nb1 + lg_words + kRxCountBins / 2;
     if ((rc = ensure(pl, pl->rx_misc, misc_words * 8 + 256))) return rc;
     // level 2: about four workgroups per CU over all buckets
     R.chunks2 = R.bits2 ? (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, 4ull * ctx->n_cu / nb1)) : 0;
@@ -1162,6 +1164,17 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     R.rows2 = R.bits2 ? static_cast<uint8_t*>(pl->rx_rows2.p) : R.rows1;
     R.leaf_start = R.bits2 ? static_cast<uint64_t*>(pl->rx_leaf.p) : nullptr;
     R.hist2 = R.bits2 ? reinterpret_cast<uint32_t*>(R.leaf_start + 2ull * R.n_leaves + 32) : nullptr;
+    // ORDER BY COUNT(*) ... LIMIT over compacted leaves (the reference's
+    // native COUNT top-N, v2_shape_group_topn.zig:846-870): the leaves also
+    // write each group's count and a histogram of them, which fixes the cut
+    // without another pass over the groups
+    R.counts_out = R.count_hist = nullptr;
+    if (!R.dense_out && P.op.n_comps > 0 && P.op.comps[0].src == C_COUNT && need_lim <= kSortCandidatesMax &&
+        rows > 65536) {
+      if ((rc = ensure(pl, pl->rx_words, rows * 4 + 256))) return rc;
+      R.counts_out = static_cast<uint32_t*>(pl->rx_words.p);
+      R.count_hist = reinterpret_cast<uint32_t*>(R.bucket_fill + nb1 + lg_words);
+    }
     auto* hist = reinterpret_cast<unsigned long long*>(misc);
     HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
     HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(misc), misc_words * 2, 0, s), "init histogram");
@@ -1196,8 +1209,12 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   }
   uint64_t n_groups = 0;
   uint32_t err_bits = 0;
+  uint32_t count_hist[kRxCountBins];
+  const bool count_topn = P.radix && P.rx.counts_out;
   HIP_TRY(hipMemcpyAsync(&n_groups, pl->nout.p, 8, hipMemcpyDeviceToHost, s), "group count");
   HIP_TRY(hipMemcpyAsync(&err_bits, sp.err, 4, hipMemcpyDeviceToHost, s), "error flags");
+  if (count_topn)
+    HIP_TRY(hipMemcpyAsync(count_hist, P.rx.count_hist, sizeof count_hist, hipMemcpyDeviceToHost, s), "count histogram");
   if ((rc = sync_check(pl, "scan"))) return rc;
   if (err_bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
   if (err_bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
@@ -1211,11 +1228,40 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   const uint64_t n_out = end - start;
   const uint32_t* sel = static_cast<const uint32_t*>(pl->gids.p) + start;
   if (P.op.n_comps > 0 && n_groups > 0 && end > 0) {
-    const size_t sb = sort_scratch_bytes(n_groups, P.op.n_comps);
-    if ((rc = ensure(pl, pl->scratch, sb))) return rc;
     if ((rc = ensure(pl, pl->sel, end * 4 + 256))) return rc;
-    HIP_TRY(launch_sort(P.op, static_cast<uint32_t*>(pl->gids.p), n_groups, end, pl->scratch.p, pl->scratch.bytes,
-                        static_cast<uint32_t*>(pl->sel.p), s), "order");
+    // COUNT(*) first: the cut is the count at which the histogram, walked
+    // from the best end, reaches `end` groups; counts in the last bin are
+    // not resolved, so a cut there takes the general path
+    bool done = false;
+    if (count_topn) {
+      const bool desc = P.op.comps[0].desc != 0;
+      uint64_t cum = 0;
+      uint32_t cut = 0;
+      for (uint32_t k = 1; k < kRxCountBins; k++) {
+        const uint32_t b = desc ? kRxCountBins - k : k;
+        cum += count_hist[b];
+        if (cum >= end) {
+          cut = b;
+          break;
+        }
+      }
+      if (cut != 0 && cut != kRxCountBins - 1 && cum <= kSortCandidatesMax) {
+        const uint32_t m = (uint32_t)cum;
+        const size_t cb = align_up(sort_candidates_scratch_bytes(m, P.op.n_comps), 256);
+        if ((rc = ensure(pl, pl->scratch, cb + (uint64_t)m * 4 + 256))) return rc;
+        uint32_t* idx = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(pl->scratch.p) + cb);
+        HIP_TRY(launch_radix_count_take(P.rx, cut, desc, idx, m, reinterpret_cast<uint32_t*>(pl->nout.p) + 2, s),
+                "count cut");
+        HIP_TRY(launch_sort_candidates(P.op, static_cast<const uint32_t*>(pl->gids.p), n_groups, idx, m, end,
+                                       pl->scratch.p, static_cast<uint32_t*>(pl->sel.p), s), "order");
+        done = true;
+      }
+    }
+    if (!done) {
+      if ((rc = ensure(pl, pl->scratch, sort_scratch_bytes(n_groups, P.op.n_comps)))) return rc;
+      HIP_TRY(launch_sort(P.op, static_cast<uint32_t*>(pl->gids.p), n_groups, end, pl->scratch.p, pl->scratch.bytes,
+                          static_cast<uint32_t*>(pl->sel.p), s), "order");
+    }
     sel = static_cast<const uint32_t*>(pl->sel.p) + start;
   }
   // finalize into one output buffer

@@ -197,7 +197,7 @@ struct tdbg_plan {
   struct Buf {
     void* p = nullptr;
     uint64_t bytes = 0;
-  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf;
+  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words;
   // pinned host staging: inputs (tiles, template) and the result the
   // columns of a returned tdbg_result view
   uint8_t* h_in = nullptr;
