@@ -13,7 +13,9 @@ A step is one execution of the query over the HBM-resident table (columns
 staged once, before timing).  `value` is rows scanned per second.  The
 roofline block prices the dominant kernel (the fused scan/filter/aggregate)
 at 28 algorithmic bytes per row (4+8+8+8) against 8 TB/s, with its duration
-from HIP events on the plan's stream.  The CPU baseline times the oracle
+from HIP events on the plan's stream.  For `highcard` it prices the radix
+partition/aggregate kernels together (events around the sequence) at 21
+bytes per row plus 33 bytes of group state per group.  The CPU baseline times the oracle
 (the C restatement of thinDB's low-cardinality grouped aggregate) on a
 bounded sample of the same table on the host cores.
 
@@ -52,6 +54,7 @@ WORKLOADS = {
     "serving": (0, 0, "serving loop under Poisson arrivals"),
 }
 BATCH_SEED = 7
+HIGHCARD_GROUP_BYTES = 33  # key 9 + COUNT 8 + SUM 8 + MAX 8 per group
 
 
 def log(msg: str) -> None:
@@ -463,6 +466,9 @@ def main() -> int:
     st = q.plan.stats()
     n_rows_scanned = int(st.rows_scanned)
     alg_bytes = n_rows_scanned * bytes_per_row
+    if args.workload == "highcard":
+        # plus the compulsory group state: key 9 + count 8 + sum 8 + max 8 bytes per group
+        alg_bytes += int(st.n_groups) * HIGHCARD_GROUP_BYTES
     avg_scan_s = float(np.mean(scan_ms)) / 1e3
     achieved = alg_bytes / avg_scan_s / 1e9
     log(f"{ms_per_step:.3f} ms/step wall, scan kernel {np.mean(scan_ms):.3f} ms avg "
@@ -480,8 +486,10 @@ def main() -> int:
     # HBM bytes per launch of the dominant kernel, from the committed
     # FETCH_SIZE/WRITE_SIZE counter passes (scaled by a 1 GiB copy kernel).
     traffic = None
-    pmc = ROOT / "profiles" / f"scan_jit_{args.workload}_200M_pmc.json"
-    if pmc.exists() and rows == WORKLOADS[args.workload][0] and st.specialized:
+    radix = int(st.global_mode) == 3
+    pmc = ROOT / "profiles" / (f"radix_{args.workload}_200M_pmc.json" if radix
+                               else f"scan_jit_{args.workload}_200M_pmc.json")
+    if pmc.exists() and rows == WORKLOADS[args.workload][0] and (st.specialized or radix):
         traffic = json.loads(pmc.read_text())["traffic_bytes"]
 
     out = {
@@ -505,7 +513,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "traffic_source": str(pmc.relative_to(ROOT)) if traffic else None,
-                     "kernel": "tdbg_scan_jit" if st.specialized else "scan_agg_kernel",
+                     "kernel": ("radix pipeline rx_hist+rx_scatter1+rx_hist2+rx_scatter2+rx_agg" if radix
+                                else "tdbg_scan_jit" if st.specialized else "scan_agg_kernel"),
                      "kernel_ms": float(np.mean(scan_ms)),
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
