@@ -255,7 +255,10 @@ int32_t tdbg_cancel(tdbg_plan* plan);
  * once as a HIP graph and replayed), so small and uneven queries share the
  * GPU instead of paying a launch-and-drain each.  Plans with ORDER BY,
  * LIMIT or OFFSET are declined (TDBG_ERR_UNSUPPORTED): run them with
- * tdbg_execute.  Every result is the one tdbg_execute returns for the plan. */
+ * tdbg_execute.  Every result is the one tdbg_execute returns for the plan.
+ * A batch of 64 plans or more runs as up to 4 waves (THINDB_GPU_BATCH_WAVES
+ * sets the number; at least 32 plans each) so that the result copy of each
+ * wave overlaps the kernels of the next. */
 typedef struct tdbg_batch_stats {
   double total_ms;      /* device time of the last execution, result copy included */
   double scan_ms;       /* the persistent scan/aggregate launch (0 when replayed as a graph) */
@@ -265,7 +268,8 @@ typedef struct tdbg_batch_stats {
   uint32_t n_plans;
   uint32_t n_items;     /* work items of the persistent launch */
   uint32_t graph;       /* 1: replayed as a captured HIP graph */
-  uint32_t pad;
+  uint32_t waves;       /* sub-batches run one after another, each one's result copy overlapping the
+                           kernels of the next (0: one launch sequence) */
 } tdbg_batch_stats;
 
 /* How the batch runs one plan (chosen per (rows, groups) bucket by the

@@ -124,3 +124,26 @@ def test_declines_ordered_requests(session):
     r = Request(["key"], [AggSpec("count", None, "c")], [SortSpec("c", True)], 5)
     with pytest.raises(Declined):
         GpuBatchRun(session, [(t, r)])
+
+
+def test_waves_same_results(session, oracle_lib, monkeypatch):
+    """A batch large enough to run as waves (result copies overlapping the
+    next wave's kernels) gives every plan the result of the one-launch batch
+    and of the oracle, on two executions in a row."""
+    pairs = batched_pairs(17, 96, 30_000)
+    monkeypatch.setenv("THINDB_GPU_BATCH_WAVES", "1")
+    single, _, st1 = run_batch(session, pairs)
+    assert st1.waves == 0
+    monkeypatch.setenv("THINDB_GPU_BATCH_WAVES", "3")
+    run = GpuBatchRun(session, pairs)
+    try:
+        for rep in range(2):
+            res = run.execute()
+            st = run.batch.stats()
+            assert st.waves == 3 and st.n_plans == 96 and st.graph == 1
+            assert [run.batch.route(i).items > 0 for i in range(len(pairs))] == [True] * len(pairs)
+            for (t, r), a, b in zip(pairs, res, single):
+                assert_same_rows(rows_of(a), rows_of(b), float_cols(t, r), REL, len(r.group_cols), None)
+    finally:
+        run.close()
+    check(oracle_lib, pairs[::8], res[::8])

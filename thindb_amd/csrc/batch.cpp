@@ -22,6 +22,9 @@ constexpr uint64_t kItemRows = 1ull << 17;     // rows per scan / partition work
 constexpr uint32_t kSpinLimit = 1u << 21;      // polls (~seconds) before a dependency wait gives up
 constexpr uint32_t kBatchLds = 78u * 1024u;    // LDS per workgroup: two workgroups per CU
 constexpr uint32_t kTuneSample = 64;           // plans per bucket the autotuner times together
+constexpr uint32_t kRowsPerSlot = 256;         // scan rows per LDS slot flushed, at least, per item
+constexpr uint32_t kWaves = 4;                 // waves of a large batch (result copies overlap later waves)
+constexpr uint32_t kWavePlansMin = 32;         // plans per wave, at least
 
 uint32_t ilog2(uint64_t x) { return x ? 63u - (uint32_t)__builtin_clzll(x) : 0u; }
 uint32_t bucket_of(uint64_t rows, uint64_t groups) { return (ilog2(rows) / 2) << 8 | (ilog2(groups) / 2); }
@@ -78,6 +81,13 @@ struct tdbg_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   tdbg_batch_stats last{};
+  // waves: a large batch runs as sub-batches over contiguous plan ranges on
+  // the same stream; each wave's result copy, on copy_stream, overlaps the
+  // kernels of the waves after it
+  std::vector<tdbg_batch*> waves;
+  std::vector<uint32_t> wave_first;  // first plan of each wave, then the plan count
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t kernels_done = nullptr;
 };
 
 namespace {
@@ -283,11 +293,13 @@ int32_t build(tdbg_batch* b) {
         agg_items.push_back(BatchItem{BI_RX_AGG, i, (uint32_t)l0, (uint32_t)std::min<uint64_t>(q.rx.n_leaves, l0 + per), 0, 0});
     } else {
       const uint32_t parts = q.sp.lds_mode == LDS_DIRECT ? q.sp.lds_parts : 1u;
-      // an LDS table item ends with a flush of every slot to HBM: give it at
-      // least 32 rows per slot so the flush stays a small part of the item
+      // an LDS table item ends with a flush of every slot to HBM (about five
+      // device atomics each): give it at least kRowsPerSlot rows per slot so
+      // the flush stays a small part of the item
       uint32_t scan_chunk = base_chunk;
       if (q.sp.lds_mode != LDS_NONE)
-        scan_chunk = (uint32_t)std::max<uint64_t>(scan_chunk, (32ull * q.sp.lds_cap + kTileRows - 1) / kTileRows);
+        scan_chunk = (uint32_t)std::max<uint64_t>(scan_chunk,
+                                                  ((uint64_t)kRowsPerSlot * q.sp.lds_cap + kTileRows - 1) / kTileRows);
       for (uint32_t t0 = 0; t0 < n_tiles; t0 += scan_chunk)
         for (uint32_t p = 0; p < parts; p++)
           scan_items.push_back(BatchItem{BI_SCAN, i, t0, std::min(n_tiles, t0 + scan_chunk), p, 0});
@@ -411,36 +423,50 @@ bool stale(const tdbg_batch* b) {
   return false;
 }
 
-// One execution into the pinned result buffer.  Caller holds b->mu.
-int32_t run(tdbg_batch* b) {
+// Builds the batch if it is stale or new and makes sure its pinned result
+// buffer is free.  Caller holds b->mu (or that of the batch it is a wave of).
+int32_t prepare_run(tdbg_batch* b) {
   int32_t rc;
   {
     std::shared_lock<std::shared_mutex> lk(b->ctx->mu);
     if (b->built && stale(b)) b->built = false;
   }
   if (!b->built && (rc = build(b))) return rc;
-  HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
   if (!b->h_out.p || b->h_out.p.use_count() > 1 || b->h_out.cap < b->out_bytes) {
     // results of an earlier execution still view the old buffer: keep it
     b->h_out = PinnedBuf{};
     if ((rc = pinned_acquire(b->ctx->pinned, b->out_bytes, b->h_out))) return rc;
   }
+  return TDBG_OK;
+}
+
+// Enqueues the kernels on b->stream and the result copy on `cs` (after the
+// kernels).  Caller holds ctx->mu shared until the copy completed.
+int32_t launch(tdbg_batch* b, hipStream_t cs) {
+  int32_t rc;
   hipStream_t s = b->stream;
-  std::shared_lock<std::shared_mutex> lk(b->ctx->mu);  // staged columns stay put while the kernels read them
   if (stale(b)) return fail(TDBG_ERR_NOT_STAGED, "batch: a member table changed while the batch was planned");
   HIP_TRY(hipEventRecord(b->ev[0], s), "event");
-  const bool graph = b->exec != nullptr;
-  if (graph) HIP_TRY(hipGraphLaunch(b->exec, s), "graph launch");
+  if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, s), "graph launch");
   else if ((rc = enqueue(b, true))) return rc;
-  HIP_TRY(hipMemcpyAsync(b->h_out.p.get(), b->d_out.p, b->out_bytes, hipMemcpyDeviceToHost, s), "result copy");
-  HIP_TRY(hipEventRecord(b->ev[3], s), "event");
-  HIP_TRY(hipStreamSynchronize(s), "batch");
+  if (cs != s) {
+    HIP_TRY(hipEventRecord(b->kernels_done, s), "event");
+    HIP_TRY(hipStreamWaitEvent(cs, b->kernels_done, 0), "copy stream wait");
+  }
+  HIP_TRY(hipMemcpyAsync(b->h_out.p.get(), b->d_out.p, b->out_bytes, hipMemcpyDeviceToHost, cs), "result copy");
+  HIP_TRY(hipEventRecord(b->ev[3], cs), "event");
+  return TDBG_OK;
+}
+
+// After the copy: the error flags, then the statistics.
+int32_t finish(tdbg_batch* b) {
   const uint32_t* err = reinterpret_cast<const uint32_t*>(b->h_out.p.get() + b->err_off);
   uint32_t bits = 0;
   for (int i = 0; i < 16; i++) bits |= err[i];
   if (bits & ERR_BATCH_WAIT) return fail(TDBG_ERR_DEVICE, "batch: a dependency wait gave up");
   if (bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "batch: a key value lies outside the staged zone-map statistics");
   if (bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "batch: a group table is full (est_groups too small)");
+  const bool graph = b->exec != nullptr;
   float total = 0, scan = 0;
   hipEventElapsedTime(&total, b->ev[0], b->ev[3]);
   if (!graph) hipEventElapsedTime(&scan, b->ev[1], b->ev[2]);
@@ -456,6 +482,62 @@ int32_t run(tdbg_batch* b) {
   return TDBG_OK;
 }
 
+// One execution into the pinned result buffer(s).  Caller holds b->mu.
+int32_t run(tdbg_batch* b) {
+  int32_t rc;
+  if (b->waves.empty()) {
+    if ((rc = prepare_run(b))) return rc;
+    HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
+    std::shared_lock<std::shared_mutex> lk(b->ctx->mu);  // staged columns stay put while the kernels read them
+    if ((rc = launch(b, b->stream))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream), "batch");
+    return finish(b);
+  }
+  for (tdbg_batch* w : b->waves)
+    if ((rc = prepare_run(w))) return rc;
+  HIP_TRY(hipSetDevice(b->ctx->device), "hipSetDevice");
+  {
+    std::shared_lock<std::shared_mutex> lk(b->ctx->mu);
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream), "event");
+    for (tdbg_batch* w : b->waves)
+      if ((rc = launch(w, b->copy_stream))) {
+        (void)hipStreamSynchronize(b->copy_stream);
+        (void)hipStreamSynchronize(b->stream);
+        return rc;
+      }
+    HIP_TRY(hipEventRecord(b->ev[3], b->copy_stream), "event");
+    HIP_TRY(hipStreamSynchronize(b->copy_stream), "batch copies");
+    HIP_TRY(hipStreamSynchronize(b->stream), "batch");
+  }
+  b->last = tdbg_batch_stats{};
+  b->last.graph = 1u;
+  b->last.waves = (uint32_t)b->waves.size();
+  for (tdbg_batch* w : b->waves) {
+    if ((rc = finish(w))) return rc;
+    b->last.scan_ms += w->last.scan_ms;
+    b->last.rows += w->last.rows;
+    b->last.bytes += w->last.bytes;
+    b->last.result_bytes += w->last.result_bytes;
+    b->last.n_plans += w->last.n_plans;
+    b->last.n_items += w->last.n_items;
+    b->last.graph &= w->last.graph;
+  }
+  float total = 0;
+  hipEventElapsedTime(&total, b->ev[0], b->ev[3]);
+  b->last.total_ms = total;
+  b->built = true;
+  return TDBG_OK;
+}
+
+// The wave holding plan i (and i's index in it), or b itself.
+tdbg_batch* wave_of(tdbg_batch* b, uint32_t& i) {
+  if (b->waves.empty()) return b;
+  const auto it = std::upper_bound(b->wave_first.begin(), b->wave_first.end(), i);
+  const size_t w = (size_t)(it - b->wave_first.begin()) - 1;
+  i -= b->wave_first[w];
+  return b->waves[w];
+}
+
 int32_t make_batch(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n, hipStream_t stream, tdbg_batch** out) {
   std::unique_ptr<tdbg_batch> b(new tdbg_batch());
   b->ctx = ctx;
@@ -468,7 +550,48 @@ int32_t make_batch(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n, hipSt
     b->own_stream = true;
   }
   for (auto& e : b->ev) HIP_TRY(hipEventCreate(&e), "event");
+  HIP_TRY(hipEventCreateWithFlags(&b->kernels_done, hipEventDisableTiming), "event");
   *out = b.release();
+  return TDBG_OK;
+}
+
+void free_batch(tdbg_batch* b);
+
+// Splits a large batch into waves of about equal staged rows (contiguous
+// plan ranges).  THINDB_GPU_BATCH_WAVES sets their number (1: no waves).
+int32_t split_waves(tdbg_batch* b) {
+  const char* env = getenv("THINDB_GPU_BATCH_WAVES");
+  const uint32_t n = (uint32_t)b->plans.size();
+  uint32_t w = env && env[0] ? (uint32_t)std::max(1, atoi(env)) : kWaves;
+  w = std::min<uint32_t>(w, n / kWavePlansMin);
+  if (w <= 1) return TDBG_OK;
+  std::vector<uint64_t> rows(n, 0);
+  uint64_t total = 0;
+  {
+    std::shared_lock<std::shared_mutex> lk(b->ctx->mu);
+    for (uint32_t i = 0; i < n; i++) {
+      auto it = b->ctx->tables.find(b->plans[i]->table_uid);
+      rows[i] = it == b->ctx->tables.end() ? 0 : it->second->next_row;
+      total += rows[i];
+    }
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&b->copy_stream, hipStreamNonBlocking), "copy stream");
+  uint64_t acc = 0;
+  uint32_t first = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    acc += rows[i];
+    const bool cut = (uint64_t)(b->waves.size() + 1) * total <= acc * w || i == n - 1;
+    if (cut && b->waves.size() + 1 < w && i + 1 < n || i == n - 1) {
+      tdbg_batch* sub = nullptr;
+      int32_t rc = make_batch(b->ctx, b->plans.data() + first, i + 1 - first, b->stream, &sub);
+      if (rc) return rc;
+      sub->use_graph = b->use_graph;
+      b->waves.push_back(sub);
+      b->wave_first.push_back(first);
+      first = i + 1;
+    }
+  }
+  b->wave_first.push_back(n);
   return TDBG_OK;
 }
 
@@ -476,6 +599,10 @@ void free_batch(tdbg_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->ctx->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
+  for (tdbg_batch* w : b->waves) free_batch(w);
+  if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
+  if (b->kernels_done) (void)hipEventDestroy(b->kernels_done);
   release_buffers(b);
   if (b->h_desc) (void)hipHostFree(b->h_desc);
   for (auto& e : b->ev)
@@ -529,7 +656,12 @@ int32_t tdbg_batch_create(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n
     if (!pl->order.empty() || pl->limit || pl->offset)
       return fail(TDBG_ERR_UNSUPPORTED, "plan %u has ORDER BY / LIMIT / OFFSET: run it alone", i);
   }
-  return make_batch(ctx, plans, n_plans, nullptr, out);
+  int32_t rc = make_batch(ctx, plans, n_plans, nullptr, out);
+  if (rc == TDBG_OK && (rc = split_waves(*out))) {
+    free_batch(*out);
+    *out = nullptr;
+  }
+  return rc;
 }
 
 int32_t tdbg_batch_execute(tdbg_batch* b) {
@@ -593,6 +725,7 @@ int32_t tdbg_batch_autotune(tdbg_batch* b, uint32_t reps) {
     ctx->tuned[kv.first] = pick;
   }
   release_built(b);  // re-plans with the tuned strategies at the next execution
+  for (tdbg_batch* w : b->waves) release_built(w);
   return TDBG_OK;
 }
 
@@ -600,10 +733,12 @@ int32_t tdbg_batch_result(tdbg_batch* b, uint32_t i, tdbg_result** out) {
   if (!b || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
   std::lock_guard<std::mutex> lk(b->mu);
-  if (!b->built || !b->h_out.p || i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
-  const uint64_t n_out = reinterpret_cast<const uint64_t*>(b->h_out.p.get())[i];
-  std::shared_ptr<void> keep(b->h_out.p, b->h_out.p.get());
-  return make_result(b->plans[i], n_out, b->h_out.p.get(), b->data_off[i], b->valid_off[i], keep, out);
+  if (i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "no result %u", i);
+  tdbg_batch* w = wave_of(b, i);
+  if (!w->built || !w->h_out.p) return fail(TDBG_ERR_INVALID, "no result %u", i);
+  const uint64_t n_out = reinterpret_cast<const uint64_t*>(w->h_out.p.get())[i];
+  std::shared_ptr<void> keep(w->h_out.p, w->h_out.p.get());
+  return make_result(w->plans[i], n_out, w->h_out.p.get(), w->data_off[i], w->valid_off[i], keep, out);
 }
 
 int32_t tdbg_batch_last_stats(const tdbg_batch* b, tdbg_batch_stats* out) {
@@ -614,8 +749,10 @@ int32_t tdbg_batch_last_stats(const tdbg_batch* b, tdbg_batch_stats* out) {
 
 int32_t tdbg_batch_plan_info(const tdbg_batch* b, uint32_t i, tdbg_batch_route* out) {
   if (!b || !out) return fail(TDBG_ERR_INVALID, "null argument");
-  if (!b->built || i >= b->info.size()) return fail(TDBG_ERR_INVALID, "plan %u not planned", i);
-  *out = b->info[i];
+  if (i >= b->plans.size()) return fail(TDBG_ERR_INVALID, "plan %u not planned", i);
+  const tdbg_batch* w = wave_of(const_cast<tdbg_batch*>(b), i);
+  if (!w->built || i >= w->info.size()) return fail(TDBG_ERR_INVALID, "plan %u not planned", i);
+  *out = w->info[i];
   return TDBG_OK;
 }
 

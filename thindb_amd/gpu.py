@@ -171,7 +171,7 @@ class ExecStats(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("scan_ms", C.c_double), ("rows", C.c_uint64), ("bytes", C.c_uint64),
                 ("result_bytes", C.c_uint64), ("n_plans", C.c_uint32), ("n_items", C.c_uint32),
-                ("graph", C.c_uint32), ("pad", C.c_uint32)]
+                ("graph", C.c_uint32), ("waves", C.c_uint32)]
 
 
 class BatchRoute(C.Structure):
