@@ -256,9 +256,9 @@ int32_t tdbg_cancel(tdbg_plan* plan);
  * GPU instead of paying a launch-and-drain each.  Plans with ORDER BY,
  * LIMIT or OFFSET are declined (TDBG_ERR_UNSUPPORTED): run them with
  * tdbg_execute.  Every result is the one tdbg_execute returns for the plan.
- * A batch of 64 plans or more runs as up to 4 waves (THINDB_GPU_BATCH_WAVES
- * sets the number; at least 32 plans each) so that the result copy of each
- * wave overlaps the kernels of the next. */
+ * A batch of 64 plans or more runs as 2 waves (THINDB_GPU_BATCH_WAVES sets
+ * the number; at least 32 plans each) so that the result copy of each wave
+ * overlaps the kernels of the next. */
 typedef struct tdbg_batch_stats {
   double total_ms;      /* device time of the last execution, result copy included */
   double scan_ms;       /* the persistent scan/aggregate launch (0 when replayed as a graph) */

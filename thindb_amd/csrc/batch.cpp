@@ -23,7 +23,7 @@ constexpr uint32_t kSpinLimit = 1u << 21;      // polls (~seconds) before a depe
 constexpr uint32_t kBatchLds = 78u * 1024u;    // LDS per workgroup: two workgroups per CU
 constexpr uint32_t kTuneSample = 64;           // plans per bucket the autotuner times together
 constexpr uint32_t kRowsPerSlot = 256;         // scan rows per LDS slot flushed, at least, per item
-constexpr uint32_t kWaves = 4;                 // waves of a large batch (result copies overlap later waves)
+constexpr uint32_t kWaves = 2;                 // waves of a large batch (result copies overlap later waves)
 constexpr uint32_t kWavePlansMin = 32;         // plans per wave, at least
 
 uint32_t ilog2(uint64_t x) { return x ? 63u - (uint32_t)__builtin_clzll(x) : 0u; }
@@ -88,6 +88,8 @@ struct tdbg_batch {
   std::vector<uint32_t> wave_first;  // first plan of each wave, then the plan count
   hipStream_t copy_stream = nullptr;
   hipEvent_t kernels_done = nullptr;
+  hipEvent_t ev_kernels = nullptr;   // timing: the kernels ended (before the result copy)
+  float kernels_ms = 0;              // last execution: device time before the result copy
 };
 
 namespace {
@@ -449,6 +451,7 @@ int32_t launch(tdbg_batch* b, hipStream_t cs) {
   HIP_TRY(hipEventRecord(b->ev[0], s), "event");
   if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, s), "graph launch");
   else if ((rc = enqueue(b, true))) return rc;
+  HIP_TRY(hipEventRecord(b->ev_kernels, s), "event");
   if (cs != s) {
     HIP_TRY(hipEventRecord(b->kernels_done, s), "event");
     HIP_TRY(hipStreamWaitEvent(cs, b->kernels_done, 0), "copy stream wait");
@@ -469,6 +472,7 @@ int32_t finish(tdbg_batch* b) {
   const bool graph = b->exec != nullptr;
   float total = 0, scan = 0;
   hipEventElapsedTime(&total, b->ev[0], b->ev[3]);
+  hipEventElapsedTime(&b->kernels_ms, b->ev[0], b->ev_kernels);
   if (!graph) hipEventElapsedTime(&scan, b->ev[1], b->ev[2]);
   b->last = tdbg_batch_stats{};
   b->last.total_ms = total;
@@ -551,6 +555,7 @@ int32_t make_batch(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n, hipSt
   }
   for (auto& e : b->ev) HIP_TRY(hipEventCreate(&e), "event");
   HIP_TRY(hipEventCreateWithFlags(&b->kernels_done, hipEventDisableTiming), "event");
+  HIP_TRY(hipEventCreate(&b->ev_kernels), "event");
   *out = b.release();
   return TDBG_OK;
 }
@@ -603,6 +608,7 @@ void free_batch(tdbg_batch* b) {
   for (tdbg_batch* w : b->waves) free_batch(w);
   if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
   if (b->kernels_done) (void)hipEventDestroy(b->kernels_done);
+  if (b->ev_kernels) (void)hipEventDestroy(b->ev_kernels);
   release_buffers(b);
   if (b->h_desc) (void)hipHostFree(b->h_desc);
   for (auto& e : b->ev)
@@ -713,7 +719,9 @@ int32_t tdbg_batch_autotune(tdbg_batch* b, uint32_t reps) {
           ok = false;
           break;
         }
-        if (r > 0) t = std::min(t, sub->last.total_ms);  // the first run warms up
+        // the first run warms up; the result copy is the same for every
+        // strategy, so the device time before it decides
+        if (r > 0) t = std::min<double>(t, sub->kernels_ms);
       }
       free_batch(sub);
       if (ok && t < best) {

@@ -521,6 +521,15 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     d.g_ns_off = ~0u;
     if (d.nullable) { d.g_ns_off = off; tmpl_put(off, 0); off += 8; }
   }
+  // an integer SUM state read only as a wrapped BIGINT (no AVG, no DECIMAL
+  // SUM over it) needs just its low word: single wrapping HBM adds
+  for (size_t k = 0; k < kaggs.size(); k++) {
+    if (sp.aggs[k].kind != A_SUM_I) continue;
+    bool lo = true;
+    for (auto& o : pl->outs)
+      if (o.kagg == (int)k && (o.op != TDBG_AGG_SUM || o.out_type == TDBG_DECIMAL128)) lo = false;
+    sp.aggs[k].lo_only = lo ? 1u : 0u;
+  }
   sp.g_rec = std::max<uint32_t>(8, off);
   tmpl.resize(sp.g_rec / 8, 0);
   P.tmpl = tmpl;

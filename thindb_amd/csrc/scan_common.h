@@ -245,7 +245,10 @@ __device__ __forceinline__ void g_apply(const AggDesc& a, uint8_t* rec, int64_t 
     g_add_u64(rec + a.g_off, 1);
     return;
   }
-  if (AK == A_SUM_I) g_add_i128(rec + a.g_off, (uint64_t)iv, iv < 0 ? ~0ull : 0ull);
+  if (AK == A_SUM_I) {
+    if (a.lo_only) g_add_u64(rec + a.g_off, (uint64_t)iv);
+    else g_add_i128(rec + a.g_off, (uint64_t)iv, iv < 0 ? ~0ull : 0ull);
+  }
   if (AK == A_SUM_F) g_add_f64(rec + a.g_off, fv);
   if (AK == A_MIN_I) g_min_i64(rec + a.g_off, iv);
   if (AK == A_MAX_I) g_max_i64(rec + a.g_off, iv);
@@ -394,7 +397,10 @@ __device__ __forceinline__ void lds_flush(const ScanParams& P, uint8_t* lds, uin
             // sum = hi_sum * 2^32 + lo_sum, as i128
             const int64_t hs = (int64_t)lds_at<uint64_t>(lds, d.lds_off)[cap + s];
             const __int128 v = (__int128)hs * ((__int128)1 << 32) + (__int128)w;
-            g_add_i128(rec + d.g_off, (uint64_t)v, (uint64_t)(v >> 64));
+            if (d.lo_only) g_add_u64(rec + d.g_off, (uint64_t)v);
+            else g_add_i128(rec + d.g_off, (uint64_t)v, (uint64_t)(v >> 64));
+          } else if (d.lo_only) {
+            g_add_u64(rec + d.g_off, w);
           } else {
             g_add_i128(rec + d.g_off, w, (int64_t)w < 0 ? ~0ull : 0ull);
           }
