@@ -155,15 +155,17 @@ def test_random_tables_vs_oracle(session, oracle_lib, seed):
 
 
 def test_generic_and_specialized_kernels_agree(session, oracle_lib, monkeypatch):
-    """The generic kernel (THINDB_GPU_JIT=0) and the plan-specialised one
-    both match the oracle on the same plans."""
+    """The generic kernel (THINDB_GPU_JIT=0) and the plan-specialised one,
+    with and without loading the next strip ahead (THINDB_GPU_JIT_PREFETCH),
+    all match the oracle on the same plans."""
     rng = np.random.default_rng(4)
     t = random_table(rng, 30_000, rg_rows=999)
     where = And((Predicate("i", "gt", -2 ** 30), Predicate("s", "neq", 0)))
     reqs = [Request(["k1", "k2"], ALL_AGGS, [SortSpec("sb", True)], 20, where_filter=where),
             Request(["k2"], ALL_AGGS[:6])]
-    for flag, want_spec in (("0", 0), ("1", 1)):
+    for flag, pf, want_spec in (("0", "1", 0), ("1", "0", 1), ("1", "1", 1)):
         monkeypatch.setenv("THINDB_GPU_JIT", flag)
+        monkeypatch.setenv("THINDB_GPU_JIT_PREFETCH", pf)
         for req in reqs:
             _, _, st = check_vs_oracle(session, oracle_lib, t, req)
             if flag == "0" or len({a.col for a in req.aggs} | set(req.group_cols) | {"i", "s"}) <= 8:

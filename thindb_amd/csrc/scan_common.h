@@ -556,6 +556,61 @@ __device__ __forceinline__ void scan_range(const ScanParams& P, uint8_t* lds, ui
   }
 }
 
+// scan_range with the next strip's column loads issued before the current
+// strip is filtered and aggregated, so each thread keeps a strip's loads in
+// flight while it works (STRIP provides Data, load() and process<LMODE>()).
+// A thread walks its strips of tiles [t0, t1) in the same order as
+// scan_range and loads only strips it then processes.
+template <uint32_t LMODE, class STRIP>
+__device__ __forceinline__ void scan_range_pf(const ScanParams& P, uint8_t* lds, uint32_t t0, uint32_t t1,
+                                              uint64_t part_lo) {
+  if (LMODE != LDS_NONE) {
+    lds_init(P, lds);
+    __syncthreads();
+  }
+  const uint32_t step = blockDim.x * kVec, s0 = threadIdx.x * kVec;
+  uint32_t t = t0, s = s0;
+  Tile T{};
+  while (t < t1) {  // this thread's first strip
+    T = P.tiles[t];
+    if (s < T.n) break;
+    t++;
+  }
+  typename STRIP::Data cur, nxt;
+  if (t < t1) STRIP::load(P, T.row0 + s, cur);
+  while (t < t1) {
+    uint32_t nt = t, ns = s + step;
+    Tile NT = T;
+    while (nt < t1 && ns >= NT.n) {
+      nt++;
+      ns = s0;
+      if (nt < t1) NT = P.tiles[nt];
+    }
+    if (nt < t1) STRIP::load(P, NT.row0 + ns, nxt);
+    const uint32_t cnt = T.n - s < (uint32_t)kVec ? T.n - s : (uint32_t)kVec;
+    STRIP::template process<LMODE>(P, lds, T.row0 + s, cnt, part_lo, cur);
+    t = nt;
+    s = ns;
+    T = NT;
+    cur = nxt;
+  }
+  if (LMODE != LDS_NONE) {
+    __syncthreads();
+    lds_flush(P, lds, part_lo);
+  }
+}
+
+template <uint32_t LMODE, class STRIP>
+__device__ __forceinline__ void scan_body_pf(const ScanParams& P, uint8_t* lds) {
+  const uint32_t parts = LMODE == LDS_DIRECT ? P.lds_parts : 1u;
+  const uint32_t groups = gridDim.x / parts;
+  const uint32_t g = blockIdx.x % groups;
+  const uint64_t part_lo = (uint64_t)(blockIdx.x / groups) * P.lds_cap;
+  const uint32_t t0 = (uint32_t)(((uint64_t)g * P.n_tiles) / groups);
+  const uint32_t t1 = (uint32_t)(((uint64_t)(g + 1) * P.n_tiles) / groups);
+  scan_range_pf<LMODE, STRIP>(P, lds, t0, t1, part_lo);
+}
+
 // One launch over all of P's tiles.  With a partitioned direct table, the
 // `parts` workgroups of one group (blockIdx.x = part * groups + g; groups a
 // multiple of 8 keeps them on one XCD, sharing its L2) scan the same tiles,
