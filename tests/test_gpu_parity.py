@@ -271,6 +271,30 @@ def test_radix_partition_passes(session, oracle_lib, monkeypatch, leaf_bits, bit
         assert st.global_mode == 3
 
 
+def test_topn_chunks(session, oracle_lib, monkeypatch):
+    """ORDER BY ... LIMIT <= 256 over 4K-64K groups sorts 4096-group chunks in
+    parallel and then their best candidates: the same rows, in the same
+    order, as the general path (THINDB_GPU_TOPN_CHUNKS=0), and the oracle's
+    ranks; one chunk short of `need`, heavy ties, OFFSET, several words."""
+    rng = np.random.default_rng(23)
+    base = random_table(rng, 200_000, rg_rows=20_000, nulls=True)
+    cols = dict(base.columns)
+    cols["k1"] = rng.integers(0, 8_300, base.n_rows).astype(np.int32)  # 8,300 keys: a last chunk of 108 groups
+    t = Table(base.schema, cols, base.validity, rg_rows=20_000)
+    reqs = [Request(["k1"], ALL_AGGS, [SortSpec("sb", True)], limit=100),
+            Request(["k1"], ALL_AGGS, [SortSpec("c"), SortSpec("maxd", True)], limit=200, offset=56),
+            Request(["k1"], ALL_AGGS, [SortSpec("sm", True)], limit=1),
+            Request(["k1", "k2"], ALL_AGGS[:6], [SortSpec("c", True)], limit=256)]
+    for req in reqs:
+        monkeypatch.setenv("THINDB_GPU_TOPN_CHUNKS", "1")
+        got, want, st = check_vs_oracle(session, oracle_lib, t, req)
+        assert 4096 < st.n_groups <= 65536
+        monkeypatch.setenv("THINDB_GPU_TOPN_CHUNKS", "0")
+        b, _ = run_gpu(session, t, req)
+        nk = len(req.group_cols)
+        assert [r[:nk] for r in rows_of(b)] == [r[:nk] for r in got]
+
+
 def test_topn_selection_many_groups(session, oracle_lib, monkeypatch):
     """ORDER BY ... LIMIT over more than 2^16 groups takes the selection of
     the first order word's cut: the same rows, in the same order, as

@@ -128,17 +128,30 @@ __device__ __forceinline__ bool pos_less(const uint64_t* comps, uint32_t n_comps
 
 constexpr uint32_t kSmallSort = 4096;
 
-__global__ void __launch_bounds__(1024) small_sort_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n_all,
-                                                          const uint32_t* pos_in, uint32_t n, uint32_t* pos_out) {
-  __shared__ uint32_t sp[kSmallSort];
-  __shared__ uint64_t sw[kSmallSort];
+// Composite order from word `first` on (the words before it are equal for
+// every group): wa/wb are the two positions' word `first`.
+__device__ __forceinline__ bool pos_less_from(const uint64_t* comps, uint32_t first, uint32_t n_comps, uint64_t n,
+                                              uint32_t a, uint64_t wa, uint32_t b, uint64_t wb) {
+  if (wa != wb) return wa < wb;
+  for (uint32_t c = first + 1; c < n_comps; c++) {
+    const uint64_t xa = comps[c * n + a], xb = comps[c * n + b];
+    if (xa != xb) return xa < xb;
+  }
+  return a < b;
+}
+
+// Sorts n <= kSmallSort positions (pos_in[i], or base + i) by the composite
+// order in LDS, from word `first` on; writes the first n_out to pos_out.
+__device__ __forceinline__ void sort_block(const uint64_t* comps, uint32_t first, uint32_t n_comps, uint64_t n_all,
+                                          const uint32_t* pos_in, uint32_t base, uint32_t n, uint32_t* pos_out,
+                                          uint32_t n_out, uint32_t* sp, uint64_t* sw) {
   uint32_t m = 1;
   while (m < n) m <<= 1;
   for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
     if (i < n) {
-      const uint32_t p = pos_in ? pos_in[i] : i;
+      const uint32_t p = pos_in ? pos_in[i] : base + i;
       sp[i] = p;
-      sw[i] = comps[p];
+      sw[i] = first < n_comps ? comps[(uint64_t)first * n_all + p] : 0ull;
     } else {
       sp[i] = 0xffffffffu;
       sw[i] = ~0ull;
@@ -155,7 +168,7 @@ __global__ void __launch_bounds__(1024) small_sort_kernel(const uint64_t* comps,
           bool less;  // element l < element i
           if (pb == 0xffffffffu) less = false;
           else if (pa == 0xffffffffu) less = true;
-          else less = pos_less(comps, n_comps, n_all, pb, sw[l], pa, sw[i]);
+          else less = pos_less_from(comps, first, n_comps, n_all, pb, sw[l], pa, sw[i]);
           if (less == up) {
             sp[i] = pb; sp[l] = pa;
             const uint64_t t = sw[i]; sw[i] = sw[l]; sw[l] = t;
@@ -165,7 +178,44 @@ __global__ void __launch_bounds__(1024) small_sort_kernel(const uint64_t* comps,
       __syncthreads();
     }
   }
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) pos_out[i] = sp[i];
+  for (uint32_t i = threadIdx.x; i < n_out; i += blockDim.x) pos_out[i] = sp[i];
+}
+
+__global__ void __launch_bounds__(1024) small_sort_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n_all,
+                                                          const uint32_t* pos_in, uint32_t n, uint32_t* pos_out) {
+  __shared__ uint32_t sp[kSmallSort];
+  __shared__ uint64_t sw[kSmallSort];
+  sort_block(comps, 0, n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
+}
+
+// The first word whose min and max (mm[2c], mm[2c + 1]) differ, or n_comps.
+__device__ __forceinline__ uint32_t first_varying(const uint64_t* mm, uint32_t n_comps) {
+  uint32_t c = 0;
+  while (c < n_comps && mm[2 * c] == mm[2 * c + 1]) c++;
+  return c;
+}
+
+// Workgroup b sorts positions [b * chunk, min(n, (b + 1) * chunk)) (chunk
+// <= kSmallSort) from the first varying word on (mm: every word's min and
+// max) and keeps its first `keep` at out[b * keep ...].
+__global__ void __launch_bounds__(1024) chunk_sort_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n_all,
+                                                          const uint64_t* mm, uint32_t n, uint32_t chunk,
+                                                          uint32_t keep, uint32_t* out) {
+  __shared__ uint32_t sp[kSmallSort];
+  __shared__ uint64_t sw[kSmallSort];
+  const uint32_t base = blockIdx.x * chunk;
+  const uint32_t len = n - base < chunk ? n - base : chunk;
+  sort_block(comps, first_varying(mm, n_comps), n_comps, n_all, nullptr, base, len,
+             out + (uint64_t)blockIdx.x * keep, keep < len ? keep : len, sp, sw);
+}
+
+// The candidates pos_in[0..n) sorted the same way.
+__global__ void __launch_bounds__(1024) cand_sort_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n_all,
+                                                         const uint64_t* mm, const uint32_t* pos_in, uint32_t n,
+                                                         uint32_t* pos_out) {
+  __shared__ uint32_t sp[kSmallSort];
+  __shared__ uint64_t sw[kSmallSort];
+  sort_block(comps, first_varying(mm, n_comps), n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
 }
 
 // ---- per-workgroup top-k (k <= kTopMaxK) -----------------------------------------
@@ -174,6 +224,7 @@ constexpr uint32_t kTopThreads = 256;
 constexpr uint32_t kTopPer = 16;
 constexpr uint32_t kTopChunk = kTopThreads * kTopPer;
 constexpr uint32_t kTopMaxK = 256;
+constexpr uint32_t kChunkMax = 16;  // chunk path: up to 16 chunks of kSmallSort groups (16 x kTopMaxK candidates)
 constexpr uint32_t kNone = 0xffffffffu;
 
 // Each workgroup takes kTopChunk items of the position list `pos_in`
@@ -888,6 +939,35 @@ hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_
     bool done = false;
     hipError_t e = select_large(p, gids, n, need, S, out_sel, s, &done);
     if (e != hipSuccess || done) return e;
+  }
+  const char* ch_env = getenv("THINDB_GPU_TOPN_CHUNKS");  // "0": not by chunks
+  if (!(ch_env && ch_env[0] == '0') && n > kSmallSort && n <= (uint64_t)kSmallSort * kChunkMax && need <= kTopMaxK &&
+      p.n_comps > 0) {
+    // every chunk of 4096 groups sorted by its own workgroup, its best
+    // `need` kept, those (<= 4096) sorted again: no host round trip
+    hipLaunchKernelGGL(make_comps_kernel, dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, p, (const uint32_t*)gids,
+                       n, S.comps);
+    // words equal for every group decide nothing: the sorts start at the
+    // first that varies (found on the device, no host round trip)
+    for (uint32_t c = 0; c < p.n_comps; c++) {
+      hipLaunchKernelGGL(init_mm_kernel, dim3(1), dim3(1), 0, s, S.mm + 2 * c);
+      hipLaunchKernelGGL(minmax_kernel, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s,
+                         (const uint64_t*)(S.comps + (uint64_t)c * n), n, S.mm + 2 * c);
+    }
+    // the smallest chunk (>= 1024: one element per thread) whose kept
+    // candidates still fit one workgroup's sort
+    uint32_t chunk = 1024;
+    while ((n + chunk - 1) / chunk * need > kSmallSort) chunk <<= 1;  // ends at kSmallSort at the latest
+    const uint32_t chunks = (uint32_t)((n + chunk - 1) / chunk);
+    const uint64_t last = n - (uint64_t)(chunks - 1) * chunk;
+    const uint32_t m = (uint32_t)((chunks - 1) * need + (last < need ? last : need));
+    hipLaunchKernelGGL(chunk_sort_kernel, dim3(chunks), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps, n,
+                       (const uint64_t*)S.mm, (uint32_t)n, chunk, (uint32_t)need, S.pos_b);
+    hipLaunchKernelGGL(cand_sort_kernel, dim3(1), dim3(1024), 0, s, (const uint64_t*)S.comps, p.n_comps, n,
+                       (const uint64_t*)S.mm, (const uint32_t*)S.pos_b, m, S.pos_a);
+    hipLaunchKernelGGL(select_gids_kernel, dim3(grid_for(need, 256, 4096)), dim3(256), 0, s, (const uint32_t*)gids,
+                       (const uint32_t*)S.pos_a, need, out_sel);
+    return hipGetLastError();
   }
   return sort_general(p, gids, n, need, S, out_sel, s);
 }
