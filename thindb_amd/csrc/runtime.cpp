@@ -872,7 +872,7 @@ int32_t tdbg_table_reserve(tdbg_context* ctx, uint64_t table_uid, uint64_t rows)
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   std::unique_lock<std::shared_mutex> lk(ctx->mu);
   auto& tp = ctx->tables[table_uid];
-  if (!tp) { tp.reset(new Table()); tp->uid = table_uid; }
+  if (!tp) { tp.reset(new Table()); tp->uid = table_uid; tp->version = ++ctx->clock; }
   tp->reserve = std::max(tp->reserve, align_up(rows + kRowAlign * 64, kRowAlign));
   if (tp->cap_rows < tp->reserve && !tp->cols.empty()) return table_grow(ctx, tp.get(), tp->reserve);
   return TDBG_OK;
@@ -897,7 +897,7 @@ int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   std::unique_lock<std::shared_mutex> lk(ctx->mu);
   auto& tp = ctx->tables[key->table_uid];
-  if (!tp) { tp.reset(new Table()); tp->uid = key->table_uid; }
+  if (!tp) { tp.reset(new Table()); tp->uid = key->table_uid; tp->version = ++ctx->clock; }
   Table* t = tp.get();
   // row group entry
   RowGroupEntry* rg = nullptr;
@@ -969,7 +969,7 @@ int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_
       a.min_f = std::min(a.min_f, bi.stats.min_f); a.max_f = std::max(a.max_f, bi.stats.max_f);
     }
   }
-  t->version++;
+  t->version = ++ctx->clock;  // unique across tables and drops: plans cache by it
   return TDBG_OK;
 }
 
@@ -987,7 +987,7 @@ int32_t tdbg_column_evict(tdbg_context* ctx, const tdbg_column_key* key) {
     auto bi = rg.blocks.find(key->column_idx);
     if (bi != rg.blocks.end()) bi->second.staged = false;
   }
-  t->version++;
+  t->version = ++ctx->clock;  // unique across tables and drops: plans cache by it
   return TDBG_OK;
 }
 
@@ -1103,9 +1103,24 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   auto tit = ctx->tables.find(pl->table_uid);
   if (tit == ctx->tables.end()) return fail(TDBG_ERR_NOT_STAGED, "table %llu has no staged columns", (unsigned long long)pl->table_uid);
   Table* t = tit->second.get();
-  ScanSet S;
-  int32_t rc = collect_scan(pl, t, rgs, n_rgs, S);
-  if (rc) return rc;
+  // the row groups' tiles, reused while the table and the row-group list
+  // are unchanged (resolving thousands of row groups per query is host time)
+  int32_t rc;
+  tdbg_plan::ScanCache& C = pl->scan_cache;
+  const bool hit = C.valid && C.table == t && C.version == t->version && C.all == (rgs == nullptr) &&
+                   (rgs == nullptr || (C.rgs.size() == n_rgs &&
+                                       (n_rgs == 0 || memcmp(C.rgs.data(), rgs, n_rgs * sizeof(tdbg_rg_ref)) == 0)));
+  if (!hit) {
+    C.valid = false;
+    if ((rc = collect_scan(pl, t, rgs, n_rgs, C.S))) return rc;
+    C.table = t;
+    C.version = t->version;
+    C.all = rgs == nullptr;
+    if (rgs) C.rgs.assign(rgs, rgs + n_rgs);
+    else C.rgs.clear();
+    C.valid = true;
+  }
+  const ScanSet& S = C.S;
   const std::vector<Tile>& tiles = S.tiles;
   const uint64_t rows = S.rows;
   if ((rc = prepare(pl, t, S, PlanHint{}))) return rc;

@@ -161,6 +161,7 @@ struct tdbg_context {
   int device = 0;
   uint64_t budget = 0;
   std::atomic<uint64_t> in_use{0};
+  std::atomic<uint64_t> clock{0};  // table versions: each change takes the next value
   std::shared_mutex mu;
   std::unordered_map<uint64_t, std::unique_ptr<tdbg::rt::Table>> tables;
   int n_cu = 256;
@@ -204,6 +205,16 @@ struct tdbg_plan {
   uint64_t h_in_cap = 0;
   tdbg::rt::PinnedBuf h_out;
   tdbg_exec_stats last{};
+  // the last execution's scan set, with the table version and row groups it
+  // was resolved for
+  struct ScanCache {
+    bool valid = false;
+    const tdbg::rt::Table* table = nullptr;
+    uint64_t version = 0;
+    bool all = false;
+    std::vector<tdbg_rg_ref> rgs;
+    tdbg::rt::ScanSet S;
+  } scan_cache;
 };
 
 namespace tdbg {
