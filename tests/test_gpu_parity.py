@@ -398,6 +398,51 @@ def test_plan_reuse_same_result(session):
     q.deinit()
 
 
+def test_scan_cache_follows_staging(session):
+    """A plan reuses its resolved row groups between executions; an evicted
+    block, a restaged block and a dropped-then-restaged table are all seen
+    by the next execution."""
+    from thindb_amd.gpu import NotStaged
+
+    def want_sums(cols):
+        k, v = cols["k"], cols["v"]
+        out = {}
+        for key in np.unique(k):
+            out[int(key)] = int(v[k == key].sum())
+        return out
+
+    t = gen.table("plain", 300_000, seed=9, rg_rows=65_536)
+    req = Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s")])
+    q = try_build_gpu_group(session, t, req)
+    got = {r[0]: r[2] for r in rows_of(q.next())}
+    assert got == want_sums(t.columns)
+    rgs = q.plan.rg_array([(rg.segment_id, rg.idx) for rg in t.row_groups])
+    q.plan.execute(rgs)  # cached from here on
+    rg = t.row_groups[1]
+    vi = t.column_index("v")
+    session.ctx.evict(t.uid, rg.segment_id, rg.idx, vi)
+    with pytest.raises(NotStaged):
+        q.plan.execute(rgs)
+    # the block back, with its values doubled
+    lo = rg.start
+    doubled = dict(t.columns)
+    doubled["v"] = t.columns["v"].copy()
+    doubled["v"][lo:lo + rg.n_rows] *= 2
+    session.ctx.stage(t.uid, rg.segment_id, rg.idx, vi, Type.BIGINT, doubled["v"][lo:lo + rg.n_rows])
+    cols = q.plan.execute(rgs)
+    got = {k: s for k, s in zip(cols[0].values.tolist(), cols[2].values.tolist())}
+    assert got == want_sums(doubled)
+    # the table dropped and staged again with other values under the same uid
+    session.evict_table(t)
+    t2 = Table(t.schema, {"k": t.columns["k"], "v": -t.columns["v"]}, rg_rows=65_536, uid=t.uid)
+    session.stage_table(t2, ["k", "v"])
+    cols = q.plan.execute(rgs)
+    got = {k: s for k, s in zip(cols[0].values.tolist(), cols[2].values.tolist())}
+    assert got == want_sums(t2.columns)
+    q.deinit()
+    session.evict_table(t2)
+
+
 # ---- full-size properties (config sizes; no oracle run) ------------------------------------
 
 @pytest.mark.slow
