@@ -22,6 +22,8 @@ constexpr uint64_t kItemRows = 1ull << 17;     // rows per scan / partition work
 constexpr uint32_t kSpinLimit = 1u << 21;      // polls (~seconds) before a dependency wait gives up
 constexpr uint32_t kBatchLds = 78u * 1024u;    // LDS per workgroup: two workgroups per CU
 constexpr uint32_t kTuneSample = 64;           // plans per bucket the autotuner times together
+constexpr uint64_t kTuneRows = 128ull << 20;   // rows a tuning trial runs (the sample repeated)
+constexpr uint32_t kTuneCopies = 16;           // repeats of the sample, at most
 constexpr uint32_t kRowsPerSlot = 256;         // scan rows per LDS slot flushed, at least, per item
 constexpr uint32_t kWaves = 2;                 // waves of a large batch (result copies overlap later waves)
 constexpr uint32_t kWavePlansMin = 32;         // plans per wave, at least
@@ -683,6 +685,7 @@ int32_t tdbg_batch_autotune(tdbg_batch* b, uint32_t reps) {
   reps = std::max<uint32_t>(1, reps);
   // the batch's buckets
   std::map<uint32_t, std::vector<tdbg_plan*>> buckets;
+  std::map<const tdbg_plan*, uint64_t> plan_rows;
   {
     std::shared_lock<std::shared_mutex> clk(ctx->mu);
     for (tdbg_plan* pl : b->plans) {
@@ -693,6 +696,7 @@ int32_t tdbg_batch_autotune(tdbg_batch* b, uint32_t reps) {
       int32_t rc = collect_scan(pl, tit->second.get(), nullptr, 0, S);
       if (rc) return rc;
       buckets[bucket_of(S.rows, estimate_groups(pl, tit->second.get(), S.rows))].push_back(pl);
+      plan_rows[pl] = S.rows;
     }
   }
   const PlanHint cands[] = {
@@ -705,6 +709,14 @@ int32_t tdbg_batch_autotune(tdbg_batch* b, uint32_t reps) {
     }
     std::vector<tdbg_plan*> sample(kv.second.begin(),
                                    kv.second.begin() + std::min<size_t>(kv.second.size(), kTuneSample));
+    // the sample repeated until it fills the GPU (about kTuneRows rows): in
+    // a batch the plans of a bucket run among many others, so throughput
+    // under load decides, not one small sample's latency
+    uint64_t sample_rows = 0;
+    for (tdbg_plan* pl : sample) sample_rows += plan_rows[pl];
+    const size_t copies = (size_t)std::min<uint64_t>(kTuneCopies, std::max<uint64_t>(1, kTuneRows / std::max<uint64_t>(sample_rows, 1)));
+    const size_t n_sample = sample.size();
+    for (size_t c = 1; c < copies; c++) sample.insert(sample.end(), sample.begin(), sample.begin() + n_sample);
     double best = 1e30;
     PlanHint pick{};
     for (const PlanHint& c : cands) {
