@@ -230,6 +230,28 @@ uint64_t tdbg_context_bytes_in_use(const tdbg_context* ctx);
 int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type,
                           const void* host_values, const uint8_t* host_validity,
                           uint64_t n_rows, const tdbg_block_stats* stats);
+/* Stages one block as the decompressed payload thinDB's block cache holds it
+ * (src/storage/cache.zig:1-8): the validity bitmap first when `has_nulls`
+ * ((n_rows + 7) / 8 bytes, bit 1 = valid), then the body of its encoding,
+ * expanded in HBM by the library — decodeColumnPayload and its raw/FOR/RLE
+ * cases (src/storage/segment_reader.zig:1727-1760, :1989-2075, :2133-2247):
+ *   TDBG_ENC_RAW  n_rows native values
+ *   TDBG_ENC_FOR  [base i128 LE][width u8 = 1|2|4|8][3 pad][n_rows codes of
+ *                 `width` bytes LE]; value = base + code, wrapping at the
+ *                 column's width (forBlockOf, :1809-1815)
+ *   TDBG_ENC_RLE  [n_runs u32][value width u8 = the column's][3 pad]
+ *                 [n_runs values][n_runs u32 lengths]; runs past n_rows are
+ *                 clamped, rows past the last run read 0 (rleBlockOf,
+ *                 :2098-2109)
+ * Only the encoded body crosses PCIe.  FOR/RLE hold integer-family columns
+ * (the writer encodes no others); dict/FSST blocks carry strings and are
+ * TDBG_ERR_UNSUPPORTED.  `stats` as in tdbg_column_stage (the block's
+ * footer Stats, format.zig:265-270); without them the library derives them
+ * on the host. */
+enum { TDBG_ENC_RAW = 0, TDBG_ENC_FOR = 1, TDBG_ENC_DICT = 2, TDBG_ENC_RLE = 3, TDBG_ENC_FSST = 4 };
+int32_t tdbg_column_stage_encoded(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type, uint32_t encoding,
+                                  const void* payload, uint64_t payload_bytes, uint64_t n_rows, uint32_t has_nulls,
+                                  const tdbg_block_stats* stats);
 int32_t tdbg_column_evict(tdbg_context* ctx, const tdbg_column_key* key);
 /* Pre-sizes a table's HBM arenas for `rows` rows (a hint; arenas grow). */
 int32_t tdbg_table_reserve(tdbg_context* ctx, uint64_t table_uid, uint64_t rows);

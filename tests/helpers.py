@@ -297,3 +297,75 @@ ALL_AGGS = [AggSpec("count", None, "c"), AggSpec("count", "s", "cs"), AggSpec("s
             AggSpec("max", "b", "maxb"), AggSpec("sum", "d", "sd"), AggSpec("avg", "d", "ad"),
             AggSpec("min", "f", "minf"), AggSpec("max", "d", "maxd"), AggSpec("sum", "m", "sm"),
             AggSpec("avg", "m", "am"), AggSpec("min", "t", "mint")]
+
+
+# ---- encoded blocks as thinDB's writer lays them out ----------------------------------------
+# src/storage/segment_writer.zig: forEncode (:1410-1470: base = min non-null value, codes
+# value - base, NULL or out-of-range placeholders clamped into [0, span]) and rleEncode
+# (:1290-1370: runs over the stored stream, NULL placeholders included).  The widths here
+# are forced by the caller so that every width the reader accepts is exercised.
+
+def bitmap_bytes(validity) -> bytes:
+    """Validity as the reference's bitmap: bit r of byte r / 8 set = row r valid."""
+    return np.packbits(np.asarray(validity, dtype=bool), bitorder="little").tobytes()
+
+
+def for_payload(values: np.ndarray, validity=None, width: int = 0) -> bytes:
+    v = np.asarray(values).astype(np.int64)
+    ok = np.ones(len(v), bool) if validity is None else np.asarray(validity, bool)
+    lo = int(v[ok].min()) if ok.any() else 0
+    hi = int(v[ok].max()) if ok.any() else 0
+    span = hi - lo
+    if not width:
+        width = 1 if span < 1 << 8 else 2 if span < 1 << 16 else 4 if span < 1 << 32 else 8
+    d = np.where(~ok | (v < lo), 0, np.where(v > hi, span, v.astype(object) - lo)).astype(object)
+    codes = b"".join(int(x).to_bytes(width, "little") for x in d)
+    head = lo.to_bytes(16, "little", signed=True) + bytes([width, 0, 0, 0])
+    return (bitmap_bytes(ok) if validity is not None else b"") + head + codes
+
+
+def rle_payload(values: np.ndarray, validity=None, lengths=None) -> bytes:
+    """`lengths` (optional) replaces the run lengths, to build blocks whose runs
+    do not sum to the row count."""
+    v = np.asarray(values)
+    runs_v, runs_n = [], []
+    for x in v.tolist():
+        if runs_v and runs_v[-1] == x:
+            runs_n[-1] += 1
+        else:
+            runs_v.append(x)
+            runs_n.append(1)
+    if lengths is not None:
+        runs_n = list(lengths)
+    w = v.dtype.itemsize
+    body = len(runs_v).to_bytes(4, "little") + bytes([w, 0, 0, 0])
+    body += np.asarray(runs_v, dtype=v.dtype).tobytes() + np.asarray(runs_n, dtype="<u4").tobytes()
+    return (bitmap_bytes(validity) if validity is not None else b"") + body
+
+
+def decode_payload(kind: str, payload: bytes, n: int, dtype, has_nulls: bool) -> np.ndarray:
+    """Pure-Python decode following decodeForColumn / fillRuns
+    (src/storage/segment_reader.zig:1989-2075, :2227-2247): the test's own
+    check of the encoders."""
+    p = payload[(n + 7) // 8:] if has_nulls else payload
+    out = np.zeros(n, dtype=dtype)
+    bits = 8 * np.dtype(dtype).itemsize
+    if kind == "for":
+        base = int.from_bytes(p[0:16], "little", signed=True)
+        w = p[16]
+        for r in range(n):
+            x = (base + int.from_bytes(p[20 + r * w:20 + (r + 1) * w], "little")) % (1 << bits)
+            out[r] = x - (1 << bits) if (np.issubdtype(dtype, np.signedinteger) and x >= 1 << (bits - 1)) else x
+        return out
+    n_runs = int.from_bytes(p[0:4], "little")
+    w = p[4]
+    vals = np.frombuffer(p[8:8 + n_runs * w], dtype=dtype)
+    lens = np.frombuffer(p[8 + n_runs * w:8 + n_runs * (w + 4)], dtype="<u4")
+    pos = 0
+    for x, ln in zip(vals, lens):
+        ln = min(int(ln), n - pos)
+        out[pos:pos + ln] = x
+        pos += ln
+        if pos == n:
+            break
+    return out

@@ -306,6 +306,13 @@ hipError_t launch_iota(uint32_t* p, uint64_t n, hipStream_t s);
 enum RadixPhase : int { RX_HIST = 0, RX_OFFSETS, RX_SCATTER1, RX_SCATTER2, RX_AGG };
 hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid, unsigned long long* hist,
                         hipStream_t s, int phase);
+// Encoded blocks (decode.hip): out[i] = base + codes[i] (codes `width` bytes,
+// the add wrapping at `elem` bytes); out[i] = values[run of row i] for runs
+// starting at starts[0..n_runs), rows from `covered` on 0.
+hipError_t launch_for_expand(const void* codes, uint32_t width, int64_t base, uint32_t elem, uint64_t n, void* out,
+                             hipStream_t s);
+hipError_t launch_rle_expand(const void* values, const uint32_t* starts, uint32_t n_runs, uint32_t covered,
+                             uint32_t elem, uint64_t n, void* out, hipStream_t s);
 // ORDER BY COUNT(*) over compacted leaves (r.counts_out set): the indexes
 // into the group positions of every group whose count is >= cut (desc) or
 // <= cut (asc), in any order, to idx (cap entries); their number to *n_idx

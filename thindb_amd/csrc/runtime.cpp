@@ -15,6 +15,8 @@
 //    LDS table shape, exact-sum mode) and run as one stream of kernels:
 //    record init -> fused scan/filter/aggregate -> compaction -> ORDER BY
 //    selection -> finalize -> one device-to-host copy.
+#include <functional>
+
 #include "runtime.h"
 
 #include <algorithm>
@@ -861,6 +863,8 @@ void tdbg_context_destroy(tdbg_context* ctx) {
     std::unique_lock<std::shared_mutex> lk(ctx->mu);
     for (auto& kv : ctx->tables) free_table(ctx, kv.second.get());
     ctx->tables.clear();
+    dev_free(ctx, ctx->d_stage, ctx->d_stage_bytes);
+    ctx->d_stage = nullptr;
   }
   delete ctx;
 }
@@ -888,12 +892,17 @@ int32_t tdbg_table_drop(tdbg_context* ctx, uint64_t table_uid) {
   return TDBG_OK;
 }
 
-int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type, const void* host_values,
-                          const uint8_t* host_validity, uint64_t n_rows, const tdbg_block_stats* stats) {
-  if (!ctx || !key) return fail(TDBG_ERR_INVALID, "null argument");
-  if (!is_int_type(type) && !is_float_type(type)) return fail(TDBG_ERR_UNSUPPORTED, "column type %u", type);
-  if (n_rows == 0 || n_rows > 0xffffffffull) return fail(TDBG_ERR_INVALID, "row count %llu", (unsigned long long)n_rows);
-  if (!host_values) return fail(TDBG_ERR_INVALID, "null values");
+}  // extern "C"
+
+namespace {
+
+// One block into its row group's rows of the column's arena: finds or
+// creates both, lets `fill` write the values at the row group's offset
+// (validity from `host_validity`, else all valid), then records the block's
+// zone-map statistics.  Serialised by the context lock.
+int32_t stage_block(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type, const uint8_t* host_validity,
+                    uint64_t n_rows, const tdbg_block_stats& stats,
+                    const std::function<int32_t(uint8_t* dst, uint32_t elem)>& fill) {
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   std::unique_lock<std::shared_mutex> lk(ctx->mu);
   auto& tp = ctx->tables[key->table_uid];
@@ -948,7 +957,8 @@ int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_
     HIP_TRY(hipMemset(p, 0xff, valid_bytes(t->cap_rows)), "validity init");
     a.d_valid = static_cast<uint32_t*>(p);
   }
-  HIP_TRY(hipMemcpy(a.d_data + rg->row_off * a.elem, host_values, n_rows * a.elem, hipMemcpyHostToDevice), "stage values");
+  int32_t rc = fill(a.d_data + rg->row_off * a.elem, a.elem);
+  if (rc) return rc;
   if (a.d_valid) {
     uint8_t* vb = reinterpret_cast<uint8_t*>(a.d_valid) + rg->row_off / 8;
     if (host_validity) HIP_TRY(hipMemcpy(vb, host_validity, (n_rows + 7) / 8, hipMemcpyHostToDevice), "stage validity");
@@ -956,8 +966,7 @@ int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_
   }
   BlockInfo& bi = rg->blocks[key->column_idx];
   bi.staged = true;
-  if (stats && stats->has_minmax) bi.stats = *stats;
-  else host_stats(type, host_values, host_validity, n_rows, &bi.stats);
+  bi.stats = stats;
   bi.has_stats = bi.stats.has_minmax != 0;
   if (bi.has_stats) {
     if (!a.has_stats) {
@@ -971,6 +980,144 @@ int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_
   }
   t->version = ++ctx->clock;  // unique across tables and drops: plans cache by it
   return TDBG_OK;
+}
+
+// The device staging buffer of encoded bodies (caller holds ctx->mu).
+int32_t stage_scratch(tdbg_context* ctx, uint64_t bytes, uint8_t** out) {
+  if (ctx->d_stage_bytes < bytes) {
+    dev_free(ctx, ctx->d_stage, ctx->d_stage_bytes);
+    ctx->d_stage = nullptr;
+    ctx->d_stage_bytes = 0;
+    void* p = nullptr;
+    const uint64_t want = std::max<uint64_t>(align_up(bytes, 1 << 20), 1 << 20);
+    int32_t rc = dev_alloc(ctx, &p, want);
+    if (rc) return rc;
+    ctx->d_stage = p;
+    ctx->d_stage_bytes = want;
+  }
+  *out = static_cast<uint8_t*>(ctx->d_stage);
+  return TDBG_OK;
+}
+
+uint64_t rd_le(const uint8_t* p, uint32_t bytes) {
+  uint64_t v = 0;
+  for (uint32_t b = 0; b < bytes; b++) v |= (uint64_t)p[b] << (8 * b);
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t tdbg_column_stage(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type, const void* host_values,
+                          const uint8_t* host_validity, uint64_t n_rows, const tdbg_block_stats* stats) {
+  if (!ctx || !key) return fail(TDBG_ERR_INVALID, "null argument");
+  if (!is_int_type(type) && !is_float_type(type)) return fail(TDBG_ERR_UNSUPPORTED, "column type %u", type);
+  if (n_rows == 0 || n_rows > 0xffffffffull) return fail(TDBG_ERR_INVALID, "row count %llu", (unsigned long long)n_rows);
+  if (!host_values) return fail(TDBG_ERR_INVALID, "null values");
+  tdbg_block_stats st{};
+  if (stats && stats->has_minmax) st = *stats;
+  else host_stats(type, host_values, host_validity, n_rows, &st);
+  return stage_block(ctx, key, type, host_validity, n_rows, st, [&](uint8_t* dst, uint32_t elem) -> int32_t {
+    HIP_TRY(hipMemcpy(dst, host_values, n_rows * elem, hipMemcpyHostToDevice), "stage values");
+    return TDBG_OK;
+  });
+}
+
+int32_t tdbg_column_stage_encoded(tdbg_context* ctx, const tdbg_column_key* key, uint32_t type, uint32_t encoding,
+                                  const void* payload, uint64_t payload_bytes, uint64_t n_rows, uint32_t has_nulls,
+                                  const tdbg_block_stats* stats) {
+  if (!ctx || !key || !payload) return fail(TDBG_ERR_INVALID, "null argument");
+  if (!is_int_type(type) && !is_float_type(type)) return fail(TDBG_ERR_UNSUPPORTED, "column type %u", type);
+  if (n_rows == 0 || n_rows > 0xffffffffull) return fail(TDBG_ERR_INVALID, "row count %llu", (unsigned long long)n_rows);
+  const uint8_t* p = static_cast<const uint8_t*>(payload);
+  const uint64_t bm = has_nulls ? (n_rows + 7) / 8 : 0;  // column.bitmapBytes
+  if (payload_bytes < bm) return fail(TDBG_ERR_INVALID, "payload of %llu bytes shorter than its bitmap", (unsigned long long)payload_bytes);
+  const uint8_t* valid = has_nulls ? p : nullptr;
+  const uint8_t* body = p + bm;
+  const uint64_t body_bytes = payload_bytes - bm;
+  const uint32_t elem = (uint32_t)type_bytes(type);
+  if (encoding == TDBG_ENC_RAW) {
+    if (body_bytes < n_rows * elem) return fail(TDBG_ERR_INVALID, "raw body of %llu bytes for %llu rows", (unsigned long long)body_bytes, (unsigned long long)n_rows);
+    return tdbg_column_stage(ctx, key, type, body, valid, n_rows, stats);
+  }
+  if (encoding != TDBG_ENC_FOR && encoding != TDBG_ENC_RLE)
+    return fail(TDBG_ERR_UNSUPPORTED, "encoding %u (dict/fsst carry strings, not on this path)", encoding);
+  if (is_float_type(type)) return fail(TDBG_ERR_UNSUPPORTED, "FOR/RLE blocks hold integer-family columns only");
+  // host view of the values, for zone-map statistics when the caller has none
+  std::vector<uint8_t> host;
+  auto need_host = [&]() { return !(stats && stats->has_minmax); };
+  if (encoding == TDBG_ENC_FOR) {
+    // [base i128 LE][width u8][3 pad][codes: n_rows x width LE] (forBlockOf)
+    if (body_bytes < 20) return fail(TDBG_ERR_INVALID, "FOR header truncated");
+    const int64_t base = (int64_t)rd_le(body, 8);
+    const uint64_t base_hi = rd_le(body + 8, 8);
+    if (base_hi != (base < 0 ? ~0ull : 0ull)) return fail(TDBG_ERR_INVALID, "FOR base beyond 64 bits");
+    const uint32_t width = body[16];
+    if (width != 1 && width != 2 && width != 4 && width != 8) return fail(TDBG_ERR_INVALID, "FOR width %u", width);
+    if (body_bytes < 20 + n_rows * width) return fail(TDBG_ERR_INVALID, "FOR codes truncated");
+    const uint8_t* codes = body + 20;
+    tdbg_block_stats st{};
+    if (!need_host()) st = *stats;
+    else {
+      host.resize(n_rows * elem);
+      for (uint64_t r = 0; r < n_rows; r++) {
+        const uint64_t v = (uint64_t)base + rd_le(codes + r * width, width);
+        memcpy(host.data() + r * elem, &v, elem);  // little-endian: the low `elem` bytes, wrapped
+      }
+      host_stats(type, host.data(), valid, n_rows, &st);
+    }
+    return stage_block(ctx, key, type, valid, n_rows, st, [&](uint8_t* dst, uint32_t el) -> int32_t {
+      uint8_t* d = nullptr;
+      int32_t rc = stage_scratch(ctx, n_rows * width, &d);
+      if (rc) return rc;
+      HIP_TRY(hipMemcpy(d, codes, n_rows * width, hipMemcpyHostToDevice), "stage FOR codes");
+      HIP_TRY(launch_for_expand(d, width, base, el, n_rows, dst, nullptr), "FOR expand");
+      HIP_TRY(hipStreamSynchronize(nullptr), "FOR expand");
+      return TDBG_OK;
+    });
+  }
+  // RLE: [n_runs u32][value_width u8][3 pad][values: n_runs x width][lengths: n_runs x u32] (rleBlockOf)
+  if (body_bytes < 8) return fail(TDBG_ERR_INVALID, "RLE header truncated");
+  const uint32_t n_runs = (uint32_t)rd_le(body, 4);
+  const uint32_t width = body[4];
+  if (width != elem) return fail(TDBG_ERR_INVALID, "RLE value width %u for a %u-byte column", width, elem);
+  if (body_bytes < 8 + (uint64_t)n_runs * (width + 4)) return fail(TDBG_ERR_INVALID, "RLE runs truncated");
+  const uint8_t* vals = body + 8;
+  const uint8_t* lens = vals + (uint64_t)n_runs * width;
+  // run starts, clamped to the block (fillRuns): runs past the last row drop
+  std::vector<uint32_t> starts;
+  uint64_t pos = 0;
+  uint32_t used = 0;
+  for (; used < n_runs && pos < n_rows; used++) {
+    starts.push_back((uint32_t)pos);
+    pos += std::min<uint64_t>(rd_le(lens + (uint64_t)used * 4, 4), n_rows - pos);
+  }
+  const uint32_t covered = (uint32_t)pos;
+  tdbg_block_stats st{};
+  if (!need_host()) st = *stats;
+  else {
+    host.assign(n_rows * elem, 0);
+    for (uint32_t r = 0; r < used; r++) {
+      const uint64_t end = r + 1 < used ? starts[r + 1] : covered;
+      for (uint64_t i = starts[r]; i < end; i++) memcpy(host.data() + i * elem, vals + (uint64_t)r * width, elem);
+    }
+    host_stats(type, host.data(), valid, n_rows, &st);
+  }
+  return stage_block(ctx, key, type, valid, n_rows, st, [&](uint8_t* dst, uint32_t el) -> int32_t {
+    const uint64_t vals_b = align_up((uint64_t)used * width, 256);
+    uint8_t* d = nullptr;
+    int32_t rc = stage_scratch(ctx, vals_b + (uint64_t)used * 4 + 256, &d);
+    if (rc) return rc;
+    if (used) {
+      HIP_TRY(hipMemcpy(d, vals, (uint64_t)used * width, hipMemcpyHostToDevice), "stage RLE values");
+      HIP_TRY(hipMemcpy(d + vals_b, starts.data(), (uint64_t)used * 4, hipMemcpyHostToDevice), "stage RLE runs");
+    }
+    HIP_TRY(launch_rle_expand(d, reinterpret_cast<const uint32_t*>(d + vals_b), used, covered, el, n_rows, dst, nullptr),
+            "RLE expand");
+    HIP_TRY(hipStreamSynchronize(nullptr), "RLE expand");
+    return TDBG_OK;
+  });
 }
 
 int32_t tdbg_column_evict(tdbg_context* ctx, const tdbg_column_key* key) {

@@ -170,6 +170,9 @@ struct tdbg_context {
   std::mutex tune_mu;
   std::map<uint32_t, tdbg::rt::PlanHint> tuned;
   std::shared_ptr<tdbg::rt::PinnedPool> pinned = std::make_shared<tdbg::rt::PinnedPool>();
+  // device staging of encoded block bodies (tdbg_column_stage_encoded)
+  void* d_stage = nullptr;
+  uint64_t d_stage_bytes = 0;
 };
 
 struct tdbg_result {

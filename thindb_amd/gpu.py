@@ -30,6 +30,15 @@ class Status(enum.IntEnum):
     NOT_STAGED = -6
 
 
+class Encoding(enum.IntEnum):
+    """Block value encodings (src/storage/format.zig:166-172)."""
+    RAW = 0
+    FOR = 1
+    DICT = 2
+    RLE = 3
+    FSST = 4
+
+
 class Type(enum.IntEnum):
     """Physical column types (src/types.zig subset)."""
     BOOLEAN = 1
@@ -186,6 +195,8 @@ EXPORTS = {
     "tdbg_context_create": (C.c_int32, [C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "tdbg_context_destroy": (None, [C.c_void_p]),
     "tdbg_context_bytes_in_use": (C.c_uint64, [C.c_void_p]),
+    "tdbg_column_stage_encoded": (C.c_int32, [C.c_void_p, C.POINTER(ColumnKey), C.c_uint32, C.c_uint32, C.c_void_p,
+                                              C.c_uint64, C.c_uint64, C.c_uint32, C.POINTER(BlockStats)]),
     "tdbg_column_stage": (C.c_int32, [C.c_void_p, C.POINTER(ColumnKey), C.c_uint32, C.c_void_p, C.c_void_p,
                                       C.c_uint64, C.POINTER(BlockStats)]),
     "tdbg_column_evict": (C.c_int32, [C.c_void_p, C.POINTER(ColumnKey)]),
@@ -278,6 +289,22 @@ class Context:
                 st.min_i, st.max_i = int(stats[0]), int(stats[1])
         check(self._lib.tdbg_column_stage(self._h, C.byref(key), int(type_), values.ctypes.data_as(C.c_void_p),
                                           vptr, len(values), C.byref(st) if st is not None else None))
+
+    def stage_encoded(self, table_uid: int, segment_id: int, rg_idx: int, column_idx: int, type_: Type,
+                      encoding: Encoding, payload: bytes, n_rows: int, has_nulls: bool = False, stats=None) -> None:
+        """Stages one block as thinDB's block cache holds it: the validity
+        bitmap first when `has_nulls`, then the raw/FOR/RLE body; the
+        library expands it in HBM."""
+        key = ColumnKey(table_uid, segment_id, rg_idx, column_idx)
+        buf = np.frombuffer(bytes(payload), dtype=np.uint8)
+        st = None
+        if stats is not None:
+            st = BlockStats()
+            st.has_minmax = 1
+            st.min_i, st.max_i = int(stats[0]), int(stats[1])
+        check(self._lib.tdbg_column_stage_encoded(self._h, C.byref(key), int(type_), int(encoding),
+                                                  buf.ctypes.data_as(C.c_void_p), len(buf), n_rows,
+                                                  1 if has_nulls else 0, C.byref(st) if st is not None else None))
 
     def evict(self, table_uid: int, segment_id: int, rg_idx: int, column_idx: int) -> None:
         key = ColumnKey(table_uid, segment_id, rg_idx, column_idx)
