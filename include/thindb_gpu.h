@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define TDBG_ABI_VERSION 1u
+#define TDBG_ABI_VERSION 2u
 
 enum tdbg_status {
   TDBG_OK = 0,
@@ -169,9 +169,51 @@ typedef struct tdbg_order_spec {
   uint32_t desc;
 } tdbg_order_spec;
 
+/* HAVING (`Request.having_filter`, src/exec/v2_shape_group_topn.zig:77;
+ * evaluator havingPasses/havingOutputNum :2461-2530): the predicate tree over
+ * the output columns in postfix order, applied to every group after
+ * aggregation and before ORDER BY / OFFSET / LIMIT.
+ *
+ *   CMP          pushes `output[target] op literal`; values and literals
+ *                are numbers as valueAsNum (:2391-2407) makes them: an
+ *                integer (`lit_lo`/`lit_hi`, i128 little-endian halves) or,
+ *                with `lit_float`, a double; a comparison with a double on
+ *                either side is made in f64 under types.floatOrder
+ *                (numOrder :2409-2422).  An output is the emitted value
+ *                (a BIGINT SUM wrapped, AVG divided by 10^scale, a FLOAT
+ *                MIN/MAX before narrowing to f32); a NULL output makes the
+ *                leaf false.
+ *   IS_NULL / IS_NOT_NULL  push whether `output[target]` is NULL (or not).
+ *   NOT          replaces the top with its negation (two-valued: NOT of a
+ *                leaf over a NULL is true, as in the reference).
+ *   AND / OR     pop `arity` entries and push their conjunction /
+ *                disjunction (arity 0: true / false).
+ * The array must leave exactly one entry; at most TDBG_MAX_HAVING nodes. */
+#define TDBG_MAX_HAVING 32u
+enum tdbg_having_kind {
+  TDBG_HAVING_CMP = 0,
+  TDBG_HAVING_IS_NULL = 1,
+  TDBG_HAVING_IS_NOT_NULL = 2,
+  TDBG_HAVING_NOT = 3,
+  TDBG_HAVING_AND = 4,
+  TDBG_HAVING_OR = 5,
+};
+typedef struct tdbg_having_node {
+  uint32_t kind;      /* enum tdbg_having_kind */
+  uint32_t target;    /* CMP / IS_[NOT_]NULL: output column index */
+  uint32_t op;        /* CMP: enum tdbg_pred_op */
+  uint32_t arity;     /* AND / OR: children popped */
+  uint32_t lit_float; /* CMP: 1 = the literal is `lit_f64` */
+  uint32_t pad;
+  int64_t lit_lo;     /* CMP integer literal, low 64 bits */
+  int64_t lit_hi;     /* its high 64 bits (sign extension for an i64) */
+  double lit_f64;
+} tdbg_having_node;
+
 /* A grouped query: the part of v2_shape_group_topn.zig:63-101 (`Request`)
  * the GPU operator consumes.  `limit == 0` means no limit, as in
- * src/exec/v2_lowcard_group.zig:2072-2081.  The WHERE is the AND of `preds`. */
+ * src/exec/v2_lowcard_group.zig:2072-2081.  The WHERE is the AND of `preds`;
+ * `having` (n_having nodes, may be NULL when 0) filters the groups. */
 typedef struct tdbg_plan_desc {
   uint64_t table_uid;
   uint32_t n_keys;
@@ -185,6 +227,9 @@ typedef struct tdbg_plan_desc {
   uint64_t limit;
   uint64_t offset;
   uint64_t est_groups; /* planner's NDV-product bound; 0 = unknown */
+  uint32_t n_having;
+  uint32_t pad;
+  const tdbg_having_node* having;
 } tdbg_plan_desc;
 
 /* One result column, library-owned host memory valid until

@@ -27,6 +27,12 @@
  *   - ORDER BY: SortCtx.lessThan/compareBySpec, :2198-2271 (ties by gid,
  *     BIGINT SUM by its wrapped value, AVG by the unscaled mean, floats by
  *     types.floatOrder, src/types.zig:579-587);
+ *   - HAVING: havingPasses/havingOutputNum/numOrder,
+ *     src/exec/v2_shape_group_topn.zig:2385-2535 (groups filtered before
+ *     ORDER BY; a NULL operand fails its leaf; NOT is two-valued; integers
+ *     compare as i128, anything with a double in f64 under floatOrder; the
+ *     emitted value of each output: integerAggValue :1448-1452, avgValue
+ *     :1438-1445, aggIsNull :1411-1423);
  *   - OFFSET/LIMIT: emitSorted :2069-2087 (limit 0 = all);
  *   - emit: emitGroup/appendInt/appendFloat :2089-2311 and the output types
  *     of src/exec/aggregate.zig:3700-3735 (COUNT -> BIGINT, SUM int -> BIGINT
@@ -545,6 +551,86 @@ static int gid_cmp(const void* pa, const void* pb) {
   return a < b ? -1 : (a > b);
 }
 
+/* ---- HAVING (havingPasses / havingOutputNum) --------------------------------- */
+typedef struct {
+  i128 i;
+  double f;
+  int is_f;
+  int null;
+} hnum;
+
+/* An output column's value for a group, as emitted (havingOutputNum). */
+static hnum output_num(const cplan* p, const wstate* m, uint32_t target, uint32_t gid) {
+  hnum v = {0, 0.0, 0, 0};
+  if (target < p->n_parts) {
+    v.i = unpack_part(m->keys[gid], &p->parts[target]);
+    return v;
+  }
+  uint32_t a = target - p->n_parts;
+  const agg_plan* ag = &p->aggs[a];
+  size_t o = (size_t)gid * p->n_aggs + a;
+  uint64_t ns = m->ns[o];
+  if (ag->op == OP_COUNT_STAR) { v.i = (i128)m->counts[gid]; return v; }
+  if (ag->op == OP_COUNT_COL) { v.i = (i128)ns; return v; }
+  if (ag->nullable && ns == 0) { v.null = 1; return v; } /* aggIsNull */
+  if (ag->op == OP_AVG) {
+    v.is_f = 1;
+    if (ag->is_float) v.f = ns ? m->fslots[o] / (double)ns : 0.0;
+    else v.f = ns ? (double)m->slots[o] / (double)ns / pow(10.0, (double)ag->scale) : 0.0;
+    return v;
+  }
+  if (ag->is_float) { v.is_f = 1; v.f = m->fslots[o]; return v; }
+  v.i = ag->out_type == TDBG_BIGINT ? (i128)(int64_t)(uint64_t)m->slots[o] : m->slots[o]; /* integerAggValue */
+  return v;
+}
+
+static int num_order(hnum a, hnum b) { /* numOrder */
+  if (a.is_f || b.is_f) return float_order(a.is_f ? a.f : (double)a.i, b.is_f ? b.f : (double)b.i);
+  return a.i < b.i ? -1 : (a.i > b.i);
+}
+
+/* Evaluates the subtree whose root is node `at` of the postfix array; its
+ * first node's index goes to *first. */
+static int having_eval(const cplan* p, const wstate* m, uint32_t gid, int32_t at, int32_t* first) {
+  const tdbg_having_node* h = &p->d->having[at];
+  switch (h->kind) {
+    case TDBG_HAVING_CMP: {
+      *first = at;
+      hnum lhs = output_num(p, m, h->target, gid);
+      if (lhs.null) return 0;
+      hnum rhs = {0, 0.0, 0, 0};
+      if (h->lit_float) { rhs.is_f = 1; rhs.f = h->lit_f64; }
+      else rhs.i = (i128)(((u128)(uint64_t)h->lit_hi << 64) | (uint64_t)h->lit_lo);
+      int o = num_order(lhs, rhs);
+      switch (h->op) {
+        case TDBG_EQ: return o == 0;
+        case TDBG_NEQ: return o != 0;
+        case TDBG_LT: return o < 0;
+        case TDBG_LTE: return o <= 0;
+        case TDBG_GT: return o > 0;
+        default: return o >= 0;
+      }
+    }
+    case TDBG_HAVING_IS_NULL: *first = at; return output_num(p, m, h->target, gid).null;
+    case TDBG_HAVING_IS_NOT_NULL: *first = at; return !output_num(p, m, h->target, gid).null;
+    case TDBG_HAVING_NOT: return !having_eval(p, m, gid, at - 1, first);
+    default: { /* AND / OR: children end at at-1, each preceding the next */
+      int is_and = h->kind == TDBG_HAVING_AND;
+      int acc = is_and;
+      int32_t end = at - 1;
+      *first = at;
+      for (uint32_t c = 0; c < h->arity; c++) {
+        int32_t f = end;
+        int r = having_eval(p, m, gid, end, &f);
+        acc = is_and ? (acc && r) : (acc || r);
+        *first = f;
+        end = f - 1;
+      }
+      return acc;
+    }
+  }
+}
+
 /* ---- emit (emitGroup / appendInt / appendFloat) ------------------------------ */
 static void put_int(void* data, uint32_t type, uint64_t row, i128 v) {
   switch (type) {
@@ -560,10 +646,13 @@ static void put_int(void* data, uint32_t type, uint64_t row, i128 v) {
 
 static int emit(const cplan* p, const wstate* m, or_result** out) {
   const tdbg_plan_desc* d = p->d;
-  uint64_t ng = m->n_groups;
-  uint32_t* order = malloc((ng + 1) * sizeof(uint32_t));
+  uint64_t ng = 0;
+  uint32_t* order = malloc((m->n_groups + 1) * sizeof(uint32_t));
   if (!order) return -1;
-  for (uint64_t i = 0; i < ng; i++) order[i] = (uint32_t)i;
+  for (uint64_t i = 0; i < m->n_groups; i++) {
+    int32_t first = 0;
+    if (d->n_having == 0 || having_eval(p, m, (uint32_t)i, (int32_t)d->n_having - 1, &first)) order[ng++] = (uint32_t)i;
+  }
   if (d->n_order > 0) {
     sort_ctx ctx = {p, m};
     g_sort = &ctx;
@@ -735,6 +824,29 @@ static int compile(cplan* p, const tdbg_plan_desc* d, const or_table* t) {
   }
   for (uint32_t i = 0; i < d->n_order; i++)
     if (d->order[i].target >= d->n_keys + d->n_aggs) { set_err("order target"); return TDBG_ERR_INVALID; }
+  /* HAVING: a postfix array that forms exactly one tree */
+  if (d->n_having && !d->having) { set_err("having array"); return TDBG_ERR_INVALID; }
+  if (d->n_having > TDBG_MAX_HAVING) { set_err("having size"); return TDBG_ERR_UNSUPPORTED; }
+  int64_t depth = 0;
+  for (uint32_t i = 0; i < d->n_having; i++) {
+    const tdbg_having_node* h = &d->having[i];
+    if (h->kind <= TDBG_HAVING_IS_NOT_NULL) {
+      if (h->target >= d->n_keys + d->n_aggs || (h->kind == TDBG_HAVING_CMP && h->op > TDBG_GTE)) {
+        set_err("having leaf");
+        return TDBG_ERR_INVALID;
+      }
+      depth++;
+    } else if (h->kind == TDBG_HAVING_NOT) {
+      if (depth < 1) { set_err("having NOT"); return TDBG_ERR_INVALID; }
+    } else if (h->kind == TDBG_HAVING_AND || h->kind == TDBG_HAVING_OR) {
+      if ((int64_t)h->arity > depth) { set_err("having arity"); return TDBG_ERR_INVALID; }
+      depth -= (int64_t)h->arity - 1;
+    } else {
+      set_err("having kind");
+      return TDBG_ERR_INVALID;
+    }
+  }
+  if (d->n_having && depth != 1) { set_err("having shape"); return TDBG_ERR_INVALID; }
   /* autoScanBatch (exec.zig:77-85) */
   uint64_t raw = row_bytes ? ((1u << 20) / 8) / row_bytes : 32768;
   if (raw < 2048) raw = 2048;

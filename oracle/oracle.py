@@ -56,11 +56,21 @@ class _Order(C.Structure):
     _fields_ = [("target", C.c_uint32), ("desc", C.c_uint32)]
 
 
+class _Having(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("target", C.c_uint32), ("op", C.c_uint32), ("arity", C.c_uint32),
+                ("lit_float", C.c_uint32), ("pad", C.c_uint32), ("lit_lo", C.c_uint64), ("lit_hi", C.c_uint64),
+                ("lit_f64", C.c_double)]
+
+
 class _Desc(C.Structure):
     _fields_ = [("table_uid", C.c_uint64), ("n_keys", C.c_uint32), ("n_aggs", C.c_uint32),
                 ("keys", C.POINTER(_Key)), ("aggs", C.POINTER(_Agg)), ("n_preds", C.c_uint32),
                 ("n_order", C.c_uint32), ("preds", C.POINTER(_Pred)), ("order", C.POINTER(_Order)),
-                ("limit", C.c_uint64), ("offset", C.c_uint64), ("est_groups", C.c_uint64)]
+                ("limit", C.c_uint64), ("offset", C.c_uint64), ("est_groups", C.c_uint64),
+                ("n_having", C.c_uint32), ("pad", C.c_uint32), ("having", C.POINTER(_Having))]
+
+
+H_CMP, H_IS_NULL, H_IS_NOT_NULL, H_NOT, H_AND, H_OR = range(6)
 
 
 class _View(C.Structure):
@@ -159,6 +169,71 @@ def _flatten(expr) -> list:
     raise Unsupported("only AND of leaves")
 
 
+def _having_literal(val, out_type: int, out_scale: int):
+    """A HAVING literal as a number in the output column's domain: scaled to
+    the mantissa of a DECIMAL output; an integral value is an integer, any
+    other a double.  Returns (is_float, int_value, float_value)."""
+    if isinstance(val, bool):
+        val = int(val)
+    if isinstance(val, float) and out_type not in (T_DECIMAL64, T_DECIMAL128):
+        return 1, 0, val
+    x = Decimal(repr(val)) if isinstance(val, float) else Decimal(val)
+    if out_type in (T_DECIMAL64, T_DECIMAL128):
+        x = x * (Decimal(10) ** out_scale)
+    if x.is_finite() and x == x.to_integral_value():
+        return 0, int(x), 0.0
+    return 1, 0, float(x)
+
+
+def _having_nodes(expr, names: list, types: list, scales: list) -> list:
+    """The predicate tree in postfix order (children first)."""
+    kind = type(expr).__name__
+    if kind == "Predicate":
+        t = names.index(expr.col.lower())
+        fl, iv, fv = _having_literal(expr.val, types[t], scales[t])
+        h = _Having(H_CMP, t, OPS[expr.op], 0, fl)
+        u = iv & ((1 << 128) - 1)
+        h.lit_lo, h.lit_hi, h.lit_f64 = u & ((1 << 64) - 1), u >> 64, fv
+        return [h]
+    if kind in ("IsNull", "IsNotNull"):
+        return [_Having(H_IS_NULL if kind == "IsNull" else H_IS_NOT_NULL, names.index(expr.col.lower()))]
+    if kind == "Not":
+        return _having_nodes(expr.child, names, types, scales) + [_Having(H_NOT)]
+    if kind in ("And", "Or"):
+        out = []
+        for ch in expr.children:
+            out.extend(_having_nodes(ch, names, types, scales))
+        return out + [_Having(H_AND if kind == "And" else H_OR, 0, 0, len(expr.children))]
+    raise Unsupported(f"HAVING form {kind}")
+
+
+def _out_types(schema, idx, request) -> tuple:
+    """Output column types and DECIMAL scales (keys, then aggregates)."""
+    types, scales = [], []
+    for k in request.group_cols:
+        c = schema[idx[k.lower()]]
+        types.append(int(c.type))
+        scales.append(c.scale)
+    for a in request.aggs:
+        f = a.func.lower()
+        if f == "count":
+            types.append(T_BIGINT)
+            scales.append(0)
+            continue
+        c = schema[idx[a.col.lower()]]
+        ct = int(c.type)
+        if f == "avg" or (f == "sum" and ct in FLOATS):
+            types.append(T_DOUBLE)
+            scales.append(0)
+        elif f == "sum":
+            types.append(T_DECIMAL128 if ct == T_DECIMAL64 else T_BIGINT)
+            scales.append(c.scale if ct == T_DECIMAL64 else 0)
+        else:
+            types.append(ct)
+            scales.append(c.scale)
+    return types, scales
+
+
 def run(table, request, dop: int = 1) -> OracleResult:
     """Runs `request` over `table` with the CPU restatement."""
     L = lib()
@@ -213,8 +288,14 @@ def run(table, request, dop: int = 1) -> OracleResult:
     for i, s in enumerate(request.order_specs):
         order[i].target = lower.index(s.col.lower())
         order[i].desc = int(bool(s.desc))
+    hl = []
+    if getattr(request, "having_filter", None) is not None:
+        types, scales = _out_types(schema, idx, request)
+        hl = _having_nodes(request.having_filter, lower, types, scales)
+    having = (_Having * max(1, len(hl)))(*hl)
     desc = _Desc(table.uid, len(request.group_cols), len(request.aggs), keys, aggs, len(leaves),
-                 len(request.order_specs), preds, order, request.limit, request.offset, request.est_groups)
+                 len(request.order_specs), preds, order, request.limit, request.offset, request.est_groups,
+                 len(hl), 0, having)
     h = C.c_void_p()
     rc = L.oracle_group_by(C.byref(desc), C.byref(t), dop, C.byref(h))
     if rc == -2:

@@ -383,8 +383,9 @@ def test_declines(session):
     assert try_build_gpu_group(session, t, Request(["b"], [AggSpec("count", None, "c")])) is None
     assert try_build_gpu_group(session, t, Request(["d"], [AggSpec("count", None, "c")])) is None
     assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("sum", "t", "x")])) is None
+    # HAVING over a column that is not an output
     assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("count", None, "c")],
-                                                   having_filter=Predicate("c", "gt", 1))) is None
+                                                   having_filter=Predicate("i", "gt", 1))) is None
 
 
 def test_plan_reuse_same_result(session):

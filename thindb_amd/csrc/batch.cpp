@@ -212,6 +212,8 @@ int32_t build(tdbg_batch* b) {
     q.rx = P.rx;
     q.op = P.op;
     q.fp = P.fp;
+    q.n_having = (uint32_t)pl->having.size();
+    q.having = static_cast<const HavingNode*>(pl->having_nodes.p);
     for (size_t w = 0; w < P.tmpl.size(); w++) q.tmpl[w] = P.tmpl[w];
     q.init_records = P.radix ? 0u : 1u;
     if (P.radix) q.rx.tile_rows = std::min<uint32_t>(q.rx.tile_rows, kBatchThreads * kVec);

@@ -232,6 +232,23 @@ struct FinalParams {
   uint32_t g_rec, g_count_off, g_key_off, g_mode;
 };
 
+// HAVING node (the C-ABI's tdbg_having_node, postfix order): CMP pushes
+// `output[target] op literal` (NULL output -> false), IS_NULL/IS_NOT_NULL
+// push the output's nullness, NOT negates the top, AND/OR fold `arity`
+// entries (src/exec/v2_shape_group_topn.zig:2502-2535).
+enum HavingKind : uint32_t { H_CMP = 0, H_IS_NULL, H_IS_NOT_NULL, H_NOT, H_AND, H_OR };
+constexpr uint32_t kMaxHaving = 32;
+struct HavingNode {
+  uint32_t kind;
+  uint32_t target;
+  uint32_t op;
+  uint32_t arity;
+  uint32_t lit_float;
+  uint32_t pad;
+  int64_t lit_lo, lit_hi;
+  double lit_f64;
+};
+
 // ---- batched execution (one persistent launch over many plans) ------------
 
 // A work item of the persistent kernel: tiles [a0, a1) of plan q for a scan
@@ -256,7 +273,8 @@ struct BatchQuery {
   uint32_t blk0, blk1;           // compaction blocks [blk0, blk1) over this plan's records
   uint32_t n_hist, n_scatter;    // radix partition items of this plan
   uint32_t init_records;         // records start from tmpl (else the leaf pass writes them all)
-  uint32_t pad;
+  uint32_t n_having;             // HAVING nodes at `having` (0 = none)
+  const HavingNode* having;
   uint32_t* sync;                // [0] histogram items done, [1] bucket starts published,
                                  // [2] scatter items done
   uint32_t* item_hist;           // radix: [n_hist][2^bits1] per-item digit counts, then offsets
@@ -298,6 +316,12 @@ size_t sort_candidates_scratch_bytes(uint32_t m, uint32_t n_comps);
 hipError_t launch_sort_candidates(const OrderParams& p, const uint32_t* gids, uint64_t n, const uint32_t* idx,
                                   uint32_t m, uint64_t need, void* scratch, uint32_t* out_sel, hipStream_t s);
 size_t sort_scratch_bytes(uint64_t n, uint32_t n_comps);
+// HAVING over the compacted positions gids_in[0, *n_in): the survivors, in
+// order, to gids_out and their count to *n_kept.  `max_n` bounds *n_in;
+// block_counts holds ceil(max_n / 4096) + 1 words.
+hipError_t launch_having(const FinalParams& p, const HavingNode* nodes, uint32_t n_nodes, const uint32_t* gids_in,
+                         const uint64_t* n_in, uint64_t max_n, uint32_t* block_counts, uint32_t* gids_out,
+                         uint64_t* n_kept, hipStream_t s);
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
                            hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);

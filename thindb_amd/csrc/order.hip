@@ -103,6 +103,75 @@ hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_
   return hipGetLastError();
 }
 
+// ---- HAVING: survivors of the compacted positions, in order ----------------------
+// (the reference filters the grouped rows before its final sort,
+// src/exec/v2_shape_group_topn.zig:672 applyHaving, :2537 havingEmitPass)
+
+__global__ void __launch_bounds__(kCompactThreads) having_count_kernel(const FinalParams P,
+                                                                       const HavingNode* __restrict__ H,
+                                                                       uint32_t n_nodes,
+                                                                       const uint32_t* __restrict__ gids,
+                                                                       const uint64_t* __restrict__ n_in,
+                                                                       uint64_t max_n, uint32_t* block_counts) {
+  __shared__ uint32_t red[kCompactThreads / 64];
+  const uint64_t n = *n_in < max_n ? *n_in : max_n;
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + (uint64_t)threadIdx.x * kCompactPer;
+  uint32_t c = 0;
+  if ((uint64_t)blockIdx.x * kCompactChunk < n)
+    for (uint32_t j = 0; j < kCompactPer; j++)
+      if (base + j < n && having_pass(P, H, n_nodes, gids[base + j])) c++;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < kCompactThreads / 64; w++) t += red[w];
+    block_counts[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kCompactThreads) having_write_kernel(const FinalParams P,
+                                                                       const HavingNode* __restrict__ H,
+                                                                       uint32_t n_nodes,
+                                                                       const uint32_t* __restrict__ gids,
+                                                                       const uint64_t* __restrict__ n_in,
+                                                                       uint64_t max_n,
+                                                                       const uint32_t* __restrict__ block_offs,
+                                                                       uint32_t* __restrict__ out) {
+  __shared__ uint32_t pre[kCompactThreads];
+  const uint64_t n = *n_in < max_n ? *n_in : max_n;
+  if ((uint64_t)blockIdx.x * kCompactChunk >= n) return;  // uniform per workgroup
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + (uint64_t)threadIdx.x * kCompactPer;
+  uint32_t bits = 0, c = 0;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if (base + j < n && having_pass(P, H, n_nodes, gids[base + j])) { bits |= 1u << j; c++; }
+  pre[threadIdx.x] = c;
+  __syncthreads();
+  for (uint32_t o = 1; o < kCompactThreads; o <<= 1) {
+    uint32_t v = threadIdx.x >= o ? pre[threadIdx.x - o] : 0;
+    __syncthreads();
+    pre[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t w = block_offs[blockIdx.x] + pre[threadIdx.x] - c;
+  for (uint32_t j = 0; j < kCompactPer; j++)
+    if ((bits >> j) & 1u) out[w++] = gids[base + j];
+}
+
+hipError_t launch_having(const FinalParams& p, const HavingNode* nodes, uint32_t n_nodes, const uint32_t* gids_in,
+                         const uint64_t* n_in, uint64_t max_n, uint32_t* block_counts, uint32_t* gids_out,
+                         uint64_t* n_kept, hipStream_t s) {
+  const uint64_t nb = (max_n + kCompactChunk - 1) / kCompactChunk;
+  const uint32_t blocks = (uint32_t)(nb ? nb : 1);
+  if (n_nodes == 0 || n_nodes > kMaxHaving || !nodes) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(having_count_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, p, nodes, n_nodes, gids_in, n_in,
+                     max_n, block_counts);
+  hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, block_counts, blocks, n_kept);
+  hipLaunchKernelGGL(having_write_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, p, nodes, n_nodes, gids_in, n_in,
+                     max_n, (const uint32_t*)block_counts, gids_out);
+  return hipGetLastError();
+}
+
 // ---- order words ---------------------------------------------------------------
 
 __global__ void make_comps_kernel(const OrderParams P, const uint32_t* __restrict__ gids, uint64_t n,
@@ -1152,19 +1221,24 @@ hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n
 
 // ---- batched compaction and finalize ---------------------------------------------
 // Blocks of kBatchChunk (= kCompactChunk) records or output rows of one plan
-// each; a plan's compacted positions are contiguous in block order.
+// each; a plan's compacted positions are contiguous in block order.  A plan
+// with HAVING keeps only its passing groups.
+
+__device__ __forceinline__ bool batch_keep(const BatchQuery& q, uint64_t i) {
+  return occupied(q.op, i) && (q.n_having == 0 || having_pass(q.fp, q.having, q.n_having, (uint32_t)i));
+}
 
 __global__ void __launch_bounds__(kCompactThreads) batch_compact_count_kernel(const BatchQuery* __restrict__ Q,
                                                                               const BatchBlock* __restrict__ blocks,
                                                                               uint32_t* block_counts) {
   __shared__ uint32_t red[kCompactThreads / 64];
   const BatchBlock B = blocks[blockIdx.x];
-  const OrderParams& P = Q[B.q].op;
-  const uint64_t n_rec = Q[B.q].sp.g_cap;
+  const BatchQuery& q = Q[B.q];
+  const uint64_t n_rec = q.sp.g_cap;
   const uint64_t base = B.r0 + (uint64_t)threadIdx.x * kCompactPer;
   uint32_t c = 0;
   for (uint32_t j = 0; j < kCompactPer; j++)
-    if (base + j < n_rec && occupied(P, base + j)) c++;
+    if (base + j < n_rec && batch_keep(q, base + j)) c++;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -1181,12 +1255,12 @@ __global__ void __launch_bounds__(kCompactThreads) batch_compact_write_kernel(co
                                                                               uint32_t* gids) {
   __shared__ uint32_t pre[kCompactThreads];
   const BatchBlock B = blocks[blockIdx.x];
-  const OrderParams& P = Q[B.q].op;
-  const uint64_t n_rec = Q[B.q].sp.g_cap;
+  const BatchQuery& q = Q[B.q];
+  const uint64_t n_rec = q.sp.g_cap;
   const uint64_t base = B.r0 + (uint64_t)threadIdx.x * kCompactPer;
   uint32_t bits = 0, c = 0;
   for (uint32_t j = 0; j < kCompactPer; j++)
-    if (base + j < n_rec && occupied(P, base + j)) { bits |= 1u << j; c++; }
+    if (base + j < n_rec && batch_keep(q, base + j)) { bits |= 1u << j; c++; }
   pre[threadIdx.x] = c;
   __syncthreads();
   for (uint32_t o = 1; o < kCompactThreads; o <<= 1) {

@@ -105,4 +105,108 @@ __device__ __forceinline__ uint64_t comp_word(const OrderParams& P, const CompDe
   return c.desc ? ~w : w;
 }
 
+// ---- HAVING ----------------------------------------------------------------------
+
+// A HAVING operand: an integer (i128) or a double, or NULL (valueAsNum /
+// havingOutputNum, src/exec/v2_shape_group_topn.zig:2391-2500).
+struct HNum {
+  __int128 i;
+  double f;
+  bool is_f;
+  bool null;
+};
+
+// Output column o of a group as the emitted value: a BIGINT SUM is its
+// wrapped 64-bit value (integerAggValue :1448-1452), AVG = f64(sum)/f64(n)
+// /10^scale (avgValue :1438-1445), a float MIN/MAX the f64 state before any
+// narrowing to FLOAT (rowStateFloat :1465-1468); SUM/AVG/MIN/MAX over no
+// non-null input are NULL (aggIsNull :1411-1423).
+__device__ __forceinline__ HNum having_value(const FinalParams& P, const OutDesc& o, const uint8_t* rec,
+                                             uint32_t gid) {
+  HNum v{0, 0.0, false, false};
+  const uint64_t count = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off);
+  switch (o.kind) {
+    case O_KEY: v.i = part_value(P.parts[o.idx], group_key(rec, gid, P.g_mode, P.g_key_off)); return v;
+    case O_COUNT: v.i = (__int128)count; return v;
+    case O_NS: v.i = (__int128)*reinterpret_cast<const uint64_t*>(rec + o.off); return v;
+    default: break;
+  }
+  const uint64_t ns = o.ns_off != ~0u ? *reinterpret_cast<const uint64_t*>(rec + o.ns_off) : count;
+  if (ns == 0) {
+    v.null = true;
+    return v;
+  }
+  switch (o.kind) {
+    case O_SUM_I64:
+    case O_MINMAX_I: v.i = *reinterpret_cast<const int64_t*>(rec + o.off); break;
+    case O_SUM_I128: v.i = load_i128(rec + o.off); break;
+    case O_SUM_F64: v.is_f = true; v.f = *reinterpret_cast<const double*>(rec + o.off); break;
+    case O_AVG_I: v.is_f = true; v.f = i128_to_f64(load_i128(rec + o.off)) / (double)ns / o.scale_div; break;
+    case O_AVG_F: v.is_f = true; v.f = *reinterpret_cast<const double*>(rec + o.off) / (double)ns; break;
+    default: v.is_f = true; v.f = unord_f64(*reinterpret_cast<const int64_t*>(rec + o.off)); break;  // O_MINMAX_F
+  }
+  return v;
+}
+
+// numOrder (:2409-2422): -1 / 0 / 1; in f64 under types.floatOrder when
+// either side is a double.
+__device__ __forceinline__ int having_order(const HNum& a, const HavingNode& h) {
+  const __int128 b = (__int128)(((unsigned __int128)(uint64_t)h.lit_hi << 64) | (uint64_t)h.lit_lo);
+  if (a.is_f || h.lit_float) {
+    const uint64_t x = ord_f64(a.is_f ? a.f : i128_to_f64(a.i));
+    const uint64_t y = ord_f64(h.lit_float ? h.lit_f64 : i128_to_f64(b));
+    return x < y ? -1 : (x > y ? 1 : 0);
+  }
+  return a.i < b ? -1 : (a.i > b ? 1 : 0);
+}
+
+// havingPasses (:2502-2535) over the postfix nodes; the stack is one bit per
+// pending entry (at most kMaxHaving).
+__device__ __forceinline__ bool having_pass(const FinalParams& P, const HavingNode* __restrict__ H, uint32_t n,
+                                           uint32_t gid) {
+  const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
+  uint64_t st = 0;
+  for (uint32_t k = 0; k < n; k++) {
+    const HavingNode h = H[k];
+    bool r;
+    switch (h.kind) {
+      case H_CMP: {
+        const HNum a = having_value(P, P.outs[h.target], rec, gid);
+        if (a.null) {
+          r = false;
+          break;
+        }
+        const int o = having_order(a, h);
+        switch (h.op) {
+          case 0: r = o == 0; break;   // eq
+          case 1: r = o != 0; break;   // neq
+          case 2: r = o < 0; break;    // lt
+          case 3: r = o <= 0; break;   // lte
+          case 4: r = o > 0; break;    // gt
+          default: r = o >= 0; break;  // gte
+        }
+        break;
+      }
+      case H_IS_NULL:
+      case H_IS_NOT_NULL: {
+        const bool is_null = having_value(P, P.outs[h.target], rec, gid).null;
+        r = h.kind == H_IS_NULL ? is_null : !is_null;
+        break;
+      }
+      case H_NOT:
+        st ^= 1ull;
+        continue;
+      default: {  // H_AND / H_OR over the top `arity` entries
+        const uint64_t m = h.arity >= 64 ? ~0ull : ((1ull << h.arity) - 1);
+        const uint64_t top = st & m;
+        st = h.arity >= 64 ? 0 : (st >> h.arity);
+        r = h.kind == H_AND ? top == m : top != 0;
+        break;
+      }
+    }
+    st = (st << 1) | (r ? 1ull : 0ull);
+  }
+  return (st & 1ull) != 0;
+}
+
 }  // namespace tdbg

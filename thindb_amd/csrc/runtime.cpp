@@ -263,6 +263,41 @@ int32_t validate(tdbg_plan* pl) {
   if (pl->order.size() > 6) return fail(TDBG_ERR_UNSUPPORTED, "more than 6 ORDER BY items");
   for (auto& o : pl->order)
     if (o.target >= pl->keys.size() + pl->aggs.size()) return fail(TDBG_ERR_INVALID, "ORDER BY target");
+  // HAVING: a well-formed postfix tree over the output columns (every output
+  // here is a key or a COUNT/SUM/AVG/MIN/MAX, all of which the reference's
+  // havingLeafColSupported accepts, v2_shape_group_topn.zig:2430-2441)
+  static_assert(sizeof(HavingNode) == sizeof(tdbg_having_node) && kMaxHaving == TDBG_MAX_HAVING,
+                "HavingNode mirrors tdbg_having_node");
+  if (pl->having.size() > kMaxHaving) return fail(TDBG_ERR_UNSUPPORTED, "more than %u HAVING nodes", kMaxHaving);
+  {
+    const size_t n_out = pl->keys.size() + pl->aggs.size();
+    uint64_t depth = 0;
+    for (size_t i = 0; i < pl->having.size(); i++) {
+      const tdbg_having_node& h = pl->having[i];
+      switch (h.kind) {
+        case TDBG_HAVING_CMP:
+          if (h.op > TDBG_GTE) return fail(TDBG_ERR_INVALID, "HAVING node %zu: op %u", i, h.op);
+          if (h.lit_float > 1) return fail(TDBG_ERR_INVALID, "HAVING node %zu: lit_float %u", i, h.lit_float);
+          [[fallthrough]];
+        case TDBG_HAVING_IS_NULL:
+        case TDBG_HAVING_IS_NOT_NULL:
+          if (h.target >= n_out) return fail(TDBG_ERR_INVALID, "HAVING node %zu: target %u", i, h.target);
+          depth++;
+          break;
+        case TDBG_HAVING_NOT:
+          if (depth < 1) return fail(TDBG_ERR_INVALID, "HAVING node %zu: NOT of nothing", i);
+          break;
+        case TDBG_HAVING_AND:
+        case TDBG_HAVING_OR:
+          if (h.arity > depth) return fail(TDBG_ERR_INVALID, "HAVING node %zu: arity %u over %llu entries", i, h.arity,
+                                           (unsigned long long)depth);
+          depth = depth - h.arity + 1;
+          break;
+        default: return fail(TDBG_ERR_INVALID, "HAVING node %zu: kind %u", i, h.kind);
+      }
+    }
+    if (!pl->having.empty() && depth != 1) return fail(TDBG_ERR_INVALID, "HAVING leaves %llu entries", (unsigned long long)depth);
+  }
   // output types: src/exec/aggregate.zig:3700-3735
   for (auto& a : pl->aggs) {
     AggOut o{};
@@ -1141,7 +1176,8 @@ int32_t tdbg_column_evict(tdbg_context* ctx, const tdbg_column_key* key) {
 int32_t tdbg_plan_create(tdbg_context* ctx, const tdbg_plan_desc* d, tdbg_plan** out) {
   if (!ctx || !d || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
-  if ((d->n_keys && !d->keys) || (d->n_aggs && !d->aggs) || (d->n_preds && !d->preds) || (d->n_order && !d->order))
+  if ((d->n_keys && !d->keys) || (d->n_aggs && !d->aggs) || (d->n_preds && !d->preds) || (d->n_order && !d->order) ||
+      (d->n_having && !d->having))
     return fail(TDBG_ERR_INVALID, "null array in plan");
   std::unique_ptr<tdbg_plan> pl(new tdbg_plan());
   pl->ctx = ctx;
@@ -1150,14 +1186,29 @@ int32_t tdbg_plan_create(tdbg_context* ctx, const tdbg_plan_desc* d, tdbg_plan**
   pl->aggs.assign(d->aggs, d->aggs + d->n_aggs);
   pl->preds.assign(d->preds, d->preds + d->n_preds);
   pl->order.assign(d->order, d->order + d->n_order);
+  if (d->n_having) pl->having.assign(d->having, d->having + d->n_having);
   pl->limit = d->limit;
   pl->offset = d->offset;
   pl->est_groups = d->est_groups;
   int32_t rc = validate(pl.get());
   if (rc) return rc;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
-  HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
-  for (auto& e : pl->ev) HIP_TRY(hipEventCreate(&e), "event");
+  // device resources; a failure part way releases what was made
+  auto setup = [&]() -> int32_t {
+    HIP_TRY(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "stream");
+    for (auto& e : pl->ev) HIP_TRY(hipEventCreate(&e), "event");
+    if (!pl->having.empty()) {
+      const uint64_t hb = pl->having.size() * sizeof(tdbg_having_node);
+      int32_t r = ensure(pl.get(), pl->having_nodes, hb);
+      if (r) return r;
+      HIP_TRY(hipMemcpy(pl->having_nodes.p, pl->having.data(), hb, hipMemcpyHostToDevice), "HAVING nodes");
+    }
+    return TDBG_OK;
+  };
+  if ((rc = setup())) {
+    tdbg_plan_destroy(pl.release());
+    return rc;
+  }
   *out = pl.release();
   return TDBG_OK;
 }
@@ -1167,7 +1218,8 @@ void tdbg_plan_destroy(tdbg_plan* pl) {
   (void)hipSetDevice(pl->ctx->device);
   if (pl->stream) (void)hipStreamSynchronize(pl->stream);
   for (auto* b : {&pl->state, &pl->tiles, &pl->err, &pl->gids, &pl->counts, &pl->sel, &pl->scratch, &pl->out, &pl->tmpl, &pl->nout,
-                 &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf, &pl->rx_words})
+                 &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf, &pl->rx_words, &pl->having_nodes,
+                 &pl->having_gids})
     release(pl, *b);
   for (auto& e : pl->ev) if (e) (void)hipEventDestroy(e);
   if (pl->h_in) (void)hipHostFree(pl->h_in);
@@ -1341,7 +1393,7 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     // without another pass over the groups
     R.counts_out = R.count_hist = nullptr;
     if (!R.dense_out && P.op.n_comps > 0 && P.op.comps[0].src == C_COUNT && need_lim <= kSortCandidatesMax &&
-        rows > 65536) {
+        rows > 65536 && pl->having.empty()) {
       if ((rc = ensure(pl, pl->rx_words, rows * 4 + 256))) return rc;
       R.counts_out = static_cast<uint32_t*>(pl->rx_words.p);
       R.count_hist = reinterpret_cast<uint32_t*>(R.bucket_fill + nb1 + lg_words);
@@ -1378,11 +1430,24 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     HIP_TRY(launch_compact(P.op, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->gids.p),
                            static_cast<uint64_t*>(pl->nout.p), s), "compact");
   }
-  uint64_t n_groups = 0;
+  // HAVING: the passing groups' positions, in order, to having_gids and their
+  // number to nout[4]
+  uint32_t* gids = static_cast<uint32_t*>(pl->gids.p);
+  uint64_t* nout = static_cast<uint64_t*>(pl->nout.p);
+  const bool having = !pl->having.empty();
+  if (having) {
+    if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
+    if ((rc = ensure(pl, pl->having_gids, n_rec * 4 + 256))) return rc;
+    HIP_TRY(launch_having(P.fp, static_cast<const HavingNode*>(pl->having_nodes.p), (uint32_t)pl->having.size(), gids,
+                          nout, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->having_gids.p),
+                          nout + 4, s), "having");
+  }
+  uint64_t n_groups = 0, n_kept = 0;
   uint32_t err_bits = 0;
   uint32_t count_hist[kRxCountBins];
   const bool count_topn = P.radix && P.rx.counts_out;
   HIP_TRY(hipMemcpyAsync(&n_groups, pl->nout.p, 8, hipMemcpyDeviceToHost, s), "group count");
+  if (having) HIP_TRY(hipMemcpyAsync(&n_kept, nout + 4, 8, hipMemcpyDeviceToHost, s), "HAVING count");
   HIP_TRY(hipMemcpyAsync(&err_bits, sp.err, 4, hipMemcpyDeviceToHost, s), "error flags");
   if (count_topn)
     HIP_TRY(hipMemcpyAsync(count_hist, P.rx.count_hist, sizeof count_hist, hipMemcpyDeviceToHost, s), "count histogram");
@@ -1393,11 +1458,18 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   if (P.radix && !P.rx.dense_out && n_groups > n_rec)
     return fail(TDBG_ERR_DEVICE, "radix pass emitted %llu groups for %llu records", (unsigned long long)n_groups,
                 (unsigned long long)n_rec);
+  if (having) {
+    if (n_kept > n_groups)
+      return fail(TDBG_ERR_DEVICE, "HAVING kept %llu of %llu groups", (unsigned long long)n_kept,
+                  (unsigned long long)n_groups);
+    n_groups = n_kept;
+    gids = static_cast<uint32_t*>(pl->having_gids.p);
+  }
   // ORDER BY / OFFSET / LIMIT
   const uint64_t start = std::min<uint64_t>(pl->offset, n_groups);
   const uint64_t end = std::min<uint64_t>(need_lim, n_groups);
   const uint64_t n_out = end - start;
-  const uint32_t* sel = static_cast<const uint32_t*>(pl->gids.p) + start;
+  const uint32_t* sel = gids + start;
   if (P.op.n_comps > 0 && n_groups > 0 && end > 0) {
     if ((rc = ensure(pl, pl->sel, end * 4 + 256))) return rc;
     // COUNT(*) first: the cut is the count at which the histogram, walked
@@ -1423,14 +1495,14 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
         uint32_t* idx = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(pl->scratch.p) + cb);
         HIP_TRY(launch_radix_count_take(P.rx, cut, desc, idx, m, reinterpret_cast<uint32_t*>(pl->nout.p) + 2, s),
                 "count cut");
-        HIP_TRY(launch_sort_candidates(P.op, static_cast<const uint32_t*>(pl->gids.p), n_groups, idx, m, end,
-                                       pl->scratch.p, static_cast<uint32_t*>(pl->sel.p), s), "order");
+        HIP_TRY(launch_sort_candidates(P.op, gids, n_groups, idx, m, end, pl->scratch.p,
+                                       static_cast<uint32_t*>(pl->sel.p), s), "order");
         done = true;
       }
     }
     if (!done) {
       if ((rc = ensure(pl, pl->scratch, sort_scratch_bytes(n_groups, P.op.n_comps)))) return rc;
-      HIP_TRY(launch_sort(P.op, static_cast<uint32_t*>(pl->gids.p), n_groups, end, pl->scratch.p, pl->scratch.bytes,
+      HIP_TRY(launch_sort(P.op, gids, n_groups, end, pl->scratch.p, pl->scratch.bytes,
                           static_cast<uint32_t*>(pl->sel.p), s), "order");
     }
     sel = static_cast<const uint32_t*>(pl->sel.p) + start;

@@ -190,6 +190,7 @@ struct tdbg_plan {
   std::vector<tdbg_agg_spec> aggs;
   std::vector<tdbg_predicate> preds;
   std::vector<tdbg_order_spec> order;
+  std::vector<tdbg_having_node> having;  // postfix HAVING nodes, empty = none
   uint64_t limit = 0, offset = 0, est_groups = 0;
   std::vector<tdbg::rt::AggOut> outs;
   std::mutex mu;
@@ -201,7 +202,8 @@ struct tdbg_plan {
   struct Buf {
     void* p = nullptr;
     uint64_t bytes = 0;
-  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words;
+  } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words,
+      having_nodes, having_gids;
   // pinned host staging: inputs (tiles, template) and the result the
   // columns of a returned tdbg_result view
   uint8_t* h_in = nullptr;

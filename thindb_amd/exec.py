@@ -78,7 +78,19 @@ class Not:
     child: "PredicateExpr"
 
 
-PredicateExpr = Union[Predicate, And, Or, Not]
+@dataclass(frozen=True)
+class IsNull:
+    """``col IS NULL`` (PredicateExpr.is_null)."""
+    col: str
+
+
+@dataclass(frozen=True)
+class IsNotNull:
+    """``col IS NOT NULL`` (PredicateExpr.is_not_null)."""
+    col: str
+
+
+PredicateExpr = Union[Predicate, And, Or, Not, IsNull, IsNotNull]
 
 
 @dataclass(frozen=True)
@@ -244,6 +256,63 @@ def _coerce(col: Column, op: str, val) -> Tuple[PredOp, Union[int, float]]:
     raise Declined("fractional equality literal on an integer column")
 
 
+_DECIMALS = (Type.DECIMAL64, Type.DECIMAL128)
+
+
+def _having_number(val, col: Column) -> Tuple[bool, Union[int, float]]:
+    """A HAVING literal in the output column's domain (valueAsNum,
+    src/exec/v2_shape_group_topn.zig:2391-2407, after coercion): a DECIMAL
+    output compares mantissas, so the literal is scaled by its scale; an
+    integral result is an integer, anything else a double."""
+    if isinstance(val, bool):
+        val = int(val)
+    if col.type not in _DECIMALS:
+        if isinstance(val, float):
+            return True, val
+        if isinstance(val, int):
+            return False, val
+    x = Decimal(repr(val)) if isinstance(val, float) else Decimal(val)
+    if col.type in _DECIMALS:
+        x = x.scaleb(col.scale)
+    if not x.is_finite():
+        raise Declined("non-finite HAVING literal")
+    if x == x.to_integral_value():
+        return False, int(x)
+    return True, float(x)
+
+
+def _having_postfix(expr, schema: List[Column], out: list) -> None:
+    """Appends the HAVING tree in postfix order as gpu.Plan nodes; declines
+    the forms havingExprSupported (v2_shape_group_topn.zig:2443-2459)
+    rejects."""
+    names = [c.name.lower() for c in schema]
+
+    def target(name: str) -> int:
+        try:
+            return names.index(name.lower())
+        except ValueError:
+            raise Declined(f"HAVING on {name}, which is not an output column") from None
+
+    if isinstance(expr, Predicate):
+        if expr.op not in _OPS:
+            raise Declined(f"HAVING op {expr.op}")
+        t = target(expr.col)
+        is_f, v = _having_number(expr.val, schema[t])
+        out.append((gpu.HavingKind.CMP, t, _OPS[expr.op], float(v) if is_f else int(v)))
+    elif isinstance(expr, (IsNull, IsNotNull)):
+        kind = gpu.HavingKind.IS_NULL if isinstance(expr, IsNull) else gpu.HavingKind.IS_NOT_NULL
+        out.append((kind, target(expr.col)))
+    elif isinstance(expr, Not):
+        _having_postfix(expr.child, schema, out)
+        out.append((gpu.HavingKind.NOT,))
+    elif isinstance(expr, (And, Or)):
+        for ch in expr.children:
+            _having_postfix(ch, schema, out)
+        out.append((gpu.HavingKind.AND if isinstance(expr, And) else gpu.HavingKind.OR, len(expr.children)))
+    else:
+        raise Declined(f"HAVING form {type(expr).__name__}")
+
+
 def output_schema(table: Table, request: Request) -> List[Column]:
     """Keys then aggregates (v2_lowcard_group.zig:1674-1684; output types of
     src/exec/aggregate.zig:3700-3735)."""
@@ -351,8 +420,6 @@ class GpuSession:
 
 def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu.Plan, List[str]]:
     """Translates a Request into a tdbg plan; raises Declined for uncovered shapes."""
-    if request.having_filter is not None:
-        raise Declined("HAVING stays on the CPU path")
     if request.derived:
         raise Declined("derived columns stay on the CPU path")
     if not request.group_cols or not request.aggs:
@@ -398,9 +465,14 @@ def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu
         if idx is None:
             raise Declined(f"ORDER BY {s.col} is not an output column")
         order.append((idx, s.desc))
+    having: list = []
+    if request.having_filter is not None:
+        _having_postfix(request.having_filter, schema, having)
+        if len(having) > gpu.MAX_HAVING:
+            raise Declined(f"HAVING of {len(having)} nodes")
     try:
         plan = gpu.Plan(session.ctx, table.uid, keys, aggs, preds, order, request.limit, request.offset,
-                        request.est_groups)
+                        request.est_groups, having)
     except gpu.Unsupported as e:
         raise Declined(str(e)) from e
     return plan, cols

@@ -16,7 +16,7 @@ import numpy as np
 
 _LIB_PATH = Path(os.environ.get("THINDB_GPU_LIB", Path(__file__).resolve().parent / "libthindb_gpu.so"))
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class Status(enum.IntEnum):
@@ -85,6 +85,19 @@ class PredOp(enum.IntEnum):
     LTE = 3
     GT = 4
     GTE = 5
+
+
+MAX_HAVING = 32  # TDBG_MAX_HAVING
+
+
+class HavingKind(enum.IntEnum):
+    """enum tdbg_having_kind: postfix HAVING nodes."""
+    CMP = 0
+    IS_NULL = 1
+    IS_NOT_NULL = 2
+    NOT = 3
+    AND = 4
+    OR = 5
 
 
 # ---- errors (status -> the reference's error set) ---------------------------
@@ -158,12 +171,19 @@ class OrderSpecC(C.Structure):
     _fields_ = [("target", C.c_uint32), ("desc", C.c_uint32)]
 
 
+class HavingNodeC(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("target", C.c_uint32), ("op", C.c_uint32), ("arity", C.c_uint32),
+                ("lit_float", C.c_uint32), ("pad", C.c_uint32), ("lit_lo", C.c_int64), ("lit_hi", C.c_int64),
+                ("lit_f64", C.c_double)]
+
+
 class PlanDesc(C.Structure):
     _fields_ = [("table_uid", C.c_uint64), ("n_keys", C.c_uint32), ("n_aggs", C.c_uint32),
                 ("keys", C.POINTER(KeyPart)), ("aggs", C.POINTER(AggSpecC)),
                 ("n_preds", C.c_uint32), ("n_order", C.c_uint32),
                 ("preds", C.POINTER(PredicateC)), ("order", C.POINTER(OrderSpecC)),
-                ("limit", C.c_uint64), ("offset", C.c_uint64), ("est_groups", C.c_uint64)]
+                ("limit", C.c_uint64), ("offset", C.c_uint64), ("est_groups", C.c_uint64),
+                ("n_having", C.c_uint32), ("pad", C.c_uint32), ("having", C.POINTER(HavingNodeC))]
 
 
 class ColumnView(C.Structure):
@@ -364,14 +384,42 @@ def _column_from_view(v: ColumnView) -> ResultColumn:
     return ResultColumn(t, int(v.scale), bool(v.nullable), vals, valid)
 
 
+def having_node(t: tuple) -> HavingNodeC:
+    """One tdbg_having_node from (kind, ...) as Plan documents it."""
+    kind = HavingKind(t[0])
+    h = HavingNodeC()
+    h.kind = int(kind)
+    if kind == HavingKind.CMP:
+        _, target, op, lit = t
+        h.target = int(target)
+        h.op = int(op)
+        if isinstance(lit, float):
+            h.lit_float = 1
+            h.lit_f64 = lit
+        else:
+            v = int(lit)
+            if not -(1 << 127) <= v < (1 << 127):
+                raise Unsupported(f"HAVING literal {v} exceeds 128 bits")
+            u = v & ((1 << 128) - 1)
+            h.lit_lo = C.c_int64(u & ((1 << 64) - 1)).value
+            h.lit_hi = C.c_int64(u >> 64).value
+    elif kind in (HavingKind.IS_NULL, HavingKind.IS_NOT_NULL):
+        h.target = int(t[1])
+    elif kind in (HavingKind.AND, HavingKind.OR):
+        h.arity = int(t[1])
+    return h
+
+
 class Plan:
     """A grouped query prepared on one context (tdbg_plan)."""
 
     def __init__(self, ctx: Context, table_uid: int, keys: Sequence[tuple], aggs: Sequence[tuple],
                  preds: Sequence[tuple] = (), order: Sequence[tuple] = (), limit: int = 0, offset: int = 0,
-                 est_groups: int = 0):
+                 est_groups: int = 0, having: Sequence[tuple] = ()):
         """keys: (column_idx, Type); aggs: (AggOp, column_idx|-1, Type, scale, nullable);
-        preds: (column_idx, Type, PredOp, literal, nullable); order: (target, desc)."""
+        preds: (column_idx, Type, PredOp, literal, nullable); order: (target, desc);
+        having: postfix nodes (HavingKind.CMP, target, PredOp, int|float literal),
+        (IS_NULL|IS_NOT_NULL, target), (NOT,), (AND|OR, arity)."""
         lib = load()
         self._lib = lib
         self.ctx = ctx
@@ -388,8 +436,9 @@ class Plan:
             pl.append(p)
         self._preds = (PredicateC * max(1, len(pl)))(*pl)
         self._order = (OrderSpecC * max(1, len(order)))(*[OrderSpecC(t, int(bool(d))) for t, d in order])
+        self._having = (HavingNodeC * max(1, len(having)))(*[having_node(h) for h in having])
         desc = PlanDesc(table_uid, len(keys), len(aggs), self._keys, self._aggs, len(pl), len(order), self._preds,
-                        self._order, limit, offset, est_groups)
+                        self._order, limit, offset, est_groups, len(having), 0, self._having)
         h = C.c_void_p()
         check(lib.tdbg_plan_create(ctx.handle, C.byref(desc), C.byref(h)))
         self._h = h
