@@ -28,6 +28,7 @@ through the single-query operator.  A step is one pass over all 512 queries,
 results delivered to host memory; `value` is queries per second.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload filtered|plain|highcard|batched]
+       [--having]
 """
 from __future__ import annotations
 
@@ -61,19 +62,30 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_request(workload: str):
+# --having: the HAVING each workload adds (groups filtered before ORDER BY / LIMIT)
+HAVING = {
+    "filtered": "HAVING AVG(price) > 500",
+    "plain": "HAVING COUNT(*) > 3900 AND MAX(v) > 0",
+    "highcard": "HAVING COUNT(*) >= 2 AND SUM(v) > 0",
+}
+
+
+def make_request(workload: str, having: bool = False):
     from thindb_amd import gen
     from thindb_amd.exec import And, AggSpec, Predicate, Request, SortSpec
     if workload == "filtered":
         return Request(["region"], [AggSpec("count", None, "c"), AggSpec("sum", "amount", "s"),
                                     AggSpec("avg", "price", "a")], [SortSpec("s", True)], 100,
-                       where_filter=And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI))))
+                       where_filter=And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI))),
+                       having_filter=Predicate("a", "gt", 500) if having else None)
     if workload == "plain":
         return Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"), AggSpec("min", "v", "mn"),
-                               AggSpec("max", "v", "mx")])
+                               AggSpec("max", "v", "mx")],
+                       having_filter=And((Predicate("c", "gt", 3900), Predicate("mx", "gt", 0))) if having else None)
     return Request(["user_id", "bucket"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"),
                                            AggSpec("max", "w", "m")], [SortSpec("c", True)], 10,
-                   est_groups=150_000_000)
+                   est_groups=150_000_000,
+                   having_filter=And((Predicate("c", "gte", 2), Predicate("s", "gt", 0))) if having else None)
 
 
 def sync():
@@ -397,6 +409,8 @@ def main() -> int:
                     help="override the row count (batched: cap per table; testing only)")
     ap.add_argument("--tables", type=int, default=512, help="batched: number of tables/queries")
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="serving: seconds of arrivals per rate")
+    ap.add_argument("--having", action="store_true",
+                    help="filtered/plain/highcard: add the workload's HAVING (bench.HAVING)")
     ap.add_argument("--cpu-sample-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -434,7 +448,9 @@ def main() -> int:
     t0 = time.perf_counter()
     table = gen.table(args.workload, rows)
     log(f"generated in {time.perf_counter() - t0:.1f}s")
-    req = make_request(args.workload)
+    req = make_request(args.workload, args.having)
+    if args.having:
+        desc = f"{desc} {HAVING[args.workload]}"
 
     sess = GpuSession(0)
     q = try_build_gpu_group(sess, table, req)
@@ -489,7 +505,7 @@ def main() -> int:
     radix = int(st.global_mode) == 3
     pmc = ROOT / "profiles" / (f"radix_{args.workload}_200M_pmc.json" if radix
                                else f"scan_jit_{args.workload}_200M_pmc.json")
-    if pmc.exists() and rows == WORKLOADS[args.workload][0] and (st.specialized or radix):
+    if pmc.exists() and rows == WORKLOADS[args.workload][0] and (st.specialized or radix) and not args.having:
         traffic = json.loads(pmc.read_text())["traffic_bytes"]
 
     out = {

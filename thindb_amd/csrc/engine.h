@@ -318,10 +318,11 @@ hipError_t launch_sort_candidates(const OrderParams& p, const uint32_t* gids, ui
 size_t sort_scratch_bytes(uint64_t n, uint32_t n_comps);
 // HAVING over the compacted positions gids_in[0, *n_in): the survivors, in
 // order, to gids_out and their count to *n_kept.  `max_n` bounds *n_in;
-// block_counts holds ceil(max_n / 4096) + 1 words.
+// `scratch` holds having_scratch_bytes(max_n) bytes (pass masks, counts).
+size_t having_scratch_bytes(uint64_t max_n);
 hipError_t launch_having(const FinalParams& p, const HavingNode* nodes, uint32_t n_nodes, const uint32_t* gids_in,
-                         const uint64_t* n_in, uint64_t max_n, uint32_t* block_counts, uint32_t* gids_out,
-                         uint64_t* n_kept, hipStream_t s);
+                         const uint64_t* n_in, uint64_t max_n, void* scratch, uint32_t* gids_out, uint64_t* n_kept,
+                         hipStream_t s);
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
                            hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);

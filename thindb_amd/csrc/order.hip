@@ -106,22 +106,35 @@ hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_
 // ---- HAVING: survivors of the compacted positions, in order ----------------------
 // (the reference filters the grouped rows before its final sort,
 // src/exec/v2_shape_group_topn.zig:672 applyHaving, :2537 havingEmitPass)
+//
+// A workgroup owns kCompactChunk consecutive positions, visited in
+// kCompactPer rounds of 256 lanes so that position, gid and record loads are
+// coalesced.  Pass 1 evaluates the predicate once per group and keeps one
+// 64-bit pass mask per (round, wavefront); pass 2 writes the survivors' gids
+// at their ordered positions from the masks alone.
 
-__global__ void __launch_bounds__(kCompactThreads) having_count_kernel(const FinalParams P,
-                                                                       const HavingNode* __restrict__ H,
-                                                                       uint32_t n_nodes,
-                                                                       const uint32_t* __restrict__ gids,
-                                                                       const uint64_t* __restrict__ n_in,
-                                                                       uint64_t max_n, uint32_t* block_counts) {
+__global__ void __launch_bounds__(kCompactThreads) having_mask_kernel(const FinalParams P,
+                                                                      const HavingNode* __restrict__ H,
+                                                                      uint32_t n_nodes,
+                                                                      const uint32_t* __restrict__ gids,
+                                                                      const uint64_t* __restrict__ n_in,
+                                                                      uint64_t max_n, uint64_t* __restrict__ masks,
+                                                                      uint32_t* __restrict__ block_counts) {
   __shared__ uint32_t red[kCompactThreads / 64];
   const uint64_t n = *n_in < max_n ? *n_in : max_n;
-  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + (uint64_t)threadIdx.x * kCompactPer;
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk;
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t c = 0;
-  if ((uint64_t)blockIdx.x * kCompactChunk < n)
-    for (uint32_t j = 0; j < kCompactPer; j++)
-      if (base + j < n && having_pass(P, H, n_nodes, gids[base + j])) c++;
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  if (base < n) {  // uniform per workgroup
+    for (uint32_t r = 0; r < kCompactPer; r++) {
+      const uint64_t i = base + r * kCompactThreads + threadIdx.x;
+      const bool ok = i < n && having_pass(P, H, n_nodes, gids[i]);
+      const uint64_t m = __ballot(ok);
+      if (lane == 0) masks[(base >> 6) + r * (kCompactThreads / 64) + wave] = m;
+      c += (uint32_t)__popcll(m);
+    }
+  }
+  if (lane == 0) red[wave] = c;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t = 0;
@@ -130,45 +143,60 @@ __global__ void __launch_bounds__(kCompactThreads) having_count_kernel(const Fin
   }
 }
 
-__global__ void __launch_bounds__(kCompactThreads) having_write_kernel(const FinalParams P,
-                                                                       const HavingNode* __restrict__ H,
-                                                                       uint32_t n_nodes,
-                                                                       const uint32_t* __restrict__ gids,
+__global__ void __launch_bounds__(kCompactThreads) having_write_kernel(const uint32_t* __restrict__ gids,
                                                                        const uint64_t* __restrict__ n_in,
                                                                        uint64_t max_n,
+                                                                       const uint64_t* __restrict__ masks,
                                                                        const uint32_t* __restrict__ block_offs,
                                                                        uint32_t* __restrict__ out) {
-  __shared__ uint32_t pre[kCompactThreads];
+  constexpr uint32_t kMasks = kCompactChunk / 64;  // 64 masks per workgroup
+  __shared__ uint64_t mk[kMasks];
+  __shared__ uint32_t pre[kMasks];
   const uint64_t n = *n_in < max_n ? *n_in : max_n;
-  if ((uint64_t)blockIdx.x * kCompactChunk >= n) return;  // uniform per workgroup
-  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk + (uint64_t)threadIdx.x * kCompactPer;
-  uint32_t bits = 0, c = 0;
-  for (uint32_t j = 0; j < kCompactPer; j++)
-    if (base + j < n && having_pass(P, H, n_nodes, gids[base + j])) { bits |= 1u << j; c++; }
-  pre[threadIdx.x] = c;
-  __syncthreads();
-  for (uint32_t o = 1; o < kCompactThreads; o <<= 1) {
-    uint32_t v = threadIdx.x >= o ? pre[threadIdx.x - o] : 0;
-    __syncthreads();
-    pre[threadIdx.x] += v;
-    __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kCompactChunk;
+  if (base >= n) return;  // uniform per workgroup
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x < kMasks) {  // wavefront 0: exclusive scan of the masks' counts
+    const uint64_t m = masks[(base >> 6) + threadIdx.x];
+    const uint32_t c = (uint32_t)__popcll(m);
+    uint32_t x = c;
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(x, o);
+      if (lane >= o) x += v;
+    }
+    mk[threadIdx.x] = m;
+    pre[threadIdx.x] = x - c;
   }
-  uint32_t w = block_offs[blockIdx.x] + pre[threadIdx.x] - c;
-  for (uint32_t j = 0; j < kCompactPer; j++)
-    if ((bits >> j) & 1u) out[w++] = gids[base + j];
+  __syncthreads();
+  const uint32_t off = block_offs[blockIdx.x];
+  const uint64_t below = (1ull << lane) - 1;
+  for (uint32_t r = 0; r < kCompactPer; r++) {
+    const uint32_t k = r * (kCompactThreads / 64) + wave;
+    const uint64_t m = mk[k];
+    if ((m >> lane) & 1ull)
+      out[off + pre[k] + (uint32_t)__popcll(m & below)] = gids[base + r * kCompactThreads + threadIdx.x];
+  }
+}
+
+size_t having_scratch_bytes(uint64_t max_n) {
+  const uint64_t blocks = (max_n + kCompactChunk - 1) / kCompactChunk + 1;
+  return blocks * (kCompactChunk / 64) * 8 + blocks * 4 + 256;
 }
 
 hipError_t launch_having(const FinalParams& p, const HavingNode* nodes, uint32_t n_nodes, const uint32_t* gids_in,
-                         const uint64_t* n_in, uint64_t max_n, uint32_t* block_counts, uint32_t* gids_out,
-                         uint64_t* n_kept, hipStream_t s) {
+                         const uint64_t* n_in, uint64_t max_n, void* scratch, uint32_t* gids_out, uint64_t* n_kept,
+                         hipStream_t s) {
+  static_assert(kCompactChunk / 64 == 64, "one wavefront scans a workgroup's masks");
   const uint64_t nb = (max_n + kCompactChunk - 1) / kCompactChunk;
   const uint32_t blocks = (uint32_t)(nb ? nb : 1);
   if (n_nodes == 0 || n_nodes > kMaxHaving || !nodes) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(having_count_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, p, nodes, n_nodes, gids_in, n_in,
-                     max_n, block_counts);
-  hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, block_counts, blocks, n_kept);
-  hipLaunchKernelGGL(having_write_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, p, nodes, n_nodes, gids_in, n_in,
-                     max_n, (const uint32_t*)block_counts, gids_out);
+  uint64_t* masks = static_cast<uint64_t*>(scratch);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(masks + (uint64_t)blocks * (kCompactChunk / 64));
+  hipLaunchKernelGGL(having_mask_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, p, nodes, n_nodes, gids_in, n_in,
+                     max_n, masks, counts);
+  hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, counts, blocks, n_kept);
+  hipLaunchKernelGGL(having_write_kernel, dim3(blocks), dim3(kCompactThreads), 0, s, gids_in, n_in, max_n,
+                     (const uint64_t*)masks, (const uint32_t*)counts, gids_out);
   return hipGetLastError();
 }
 

@@ -1436,11 +1436,12 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   uint64_t* nout = static_cast<uint64_t*>(pl->nout.p);
   const bool having = !pl->having.empty();
   if (having) {
-    if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
-    if ((rc = ensure(pl, pl->having_gids, n_rec * 4 + 256))) return rc;
+    // survivors' gids, then the pass masks and per-workgroup counts
+    const uint64_t out_b = align_up(n_rec * 4 + 256, 256);
+    if ((rc = ensure(pl, pl->having_gids, out_b + having_scratch_bytes(n_rec)))) return rc;
+    uint8_t* hb = static_cast<uint8_t*>(pl->having_gids.p);
     HIP_TRY(launch_having(P.fp, static_cast<const HavingNode*>(pl->having_nodes.p), (uint32_t)pl->having.size(), gids,
-                          nout, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->having_gids.p),
-                          nout + 4, s), "having");
+                          nout, n_rec, hb + out_b, reinterpret_cast<uint32_t*>(hb), nout + 4, s), "having");
   }
   uint64_t n_groups = 0, n_kept = 0;
   uint32_t err_bits = 0;
