@@ -70,9 +70,17 @@ HAVING = {
 }
 
 
-def make_request(workload: str, having: bool = False):
+# --distinct: GROUP BY bucket, COUNT(DISTINCT user_id) over the high-cardinality table
+DISTINCT_DESC = "GROUP BY bucket COUNT(*), COUNT(DISTINCT user_id) top-10 by the distinct count, 200M rows"
+DISTINCT_BYTES_PER_ROW = 9  # bucket 1 + user_id 8
+
+
+def make_request(workload: str, having: bool = False, distinct: bool = False):
     from thindb_amd import gen
     from thindb_amd.exec import And, AggSpec, Predicate, Request, SortSpec
+    if distinct:
+        return Request(["bucket"], [AggSpec("count", None, "c"), AggSpec("count_distinct", "user_id", "u")],
+                       [SortSpec("u", True)], 10)
     if workload == "filtered":
         return Request(["region"], [AggSpec("count", None, "c"), AggSpec("sum", "amount", "s"),
                                     AggSpec("avg", "price", "a")], [SortSpec("s", True)], 100,
@@ -411,6 +419,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="serving: seconds of arrivals per rate")
     ap.add_argument("--having", action="store_true",
                     help="filtered/plain/highcard: add the workload's HAVING (bench.HAVING)")
+    ap.add_argument("--distinct", action="store_true",
+                    help="highcard: GROUP BY bucket, COUNT(DISTINCT user_id) instead of the headline query")
     ap.add_argument("--cpu-sample-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -448,9 +458,14 @@ def main() -> int:
     t0 = time.perf_counter()
     table = gen.table(args.workload, rows)
     log(f"generated in {time.perf_counter() - t0:.1f}s")
-    req = make_request(args.workload, args.having)
+    if args.distinct and args.workload != "highcard":
+        log("--distinct runs over the highcard workload")
+        return 2
+    req = make_request(args.workload, args.having, args.distinct)
     if args.having:
         desc = f"{desc} {HAVING[args.workload]}"
+    if args.distinct:
+        desc, bytes_per_row = DISTINCT_DESC, DISTINCT_BYTES_PER_ROW
 
     sess = GpuSession(0)
     q = try_build_gpu_group(sess, table, req)
@@ -482,9 +497,11 @@ def main() -> int:
     st = q.plan.stats()
     n_rows_scanned = int(st.rows_scanned)
     alg_bytes = n_rows_scanned * bytes_per_row
-    if args.workload == "highcard":
+    if args.workload == "highcard" and not args.distinct:
         # plus the compulsory group state: key 9 + count 8 + sum 8 + max 8 bytes per group
         alg_bytes += int(st.n_groups) * HIGHCARD_GROUP_BYTES
+    if args.distinct:  # the distinct pass runs after the scan: price the whole device time
+        scan_ms = total_ms
     avg_scan_s = float(np.mean(scan_ms)) / 1e3
     achieved = alg_bytes / avg_scan_s / 1e9
     log(f"{ms_per_step:.3f} ms/step wall, scan kernel {np.mean(scan_ms):.3f} ms avg "
@@ -505,7 +522,8 @@ def main() -> int:
     radix = int(st.global_mode) == 3
     pmc = ROOT / "profiles" / (f"radix_{args.workload}_200M_pmc.json" if radix
                                else f"scan_jit_{args.workload}_200M_pmc.json")
-    if pmc.exists() and rows == WORKLOADS[args.workload][0] and (st.specialized or radix) and not args.having:
+    if pmc.exists() and rows == WORKLOADS[args.workload][0] and (st.specialized or radix) and not args.having \
+            and not args.distinct:
         traffic = json.loads(pmc.read_text())["traffic_bytes"]
 
     out = {
@@ -529,7 +547,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "traffic_source": str(pmc.relative_to(ROOT)) if traffic else None,
-                     "kernel": ("radix pipeline rx_hist+rx_scatter1+rx_hist2+rx_scatter2+rx_agg" if radix
+                     "kernel": ("device total: scan + distinct_insert_kernel + gather" if args.distinct
+                                else "radix pipeline rx_hist+rx_scatter1+rx_hist2+rx_scatter2+rx_agg" if radix
                                 else "tdbg_scan_jit" if st.specialized else "scan_agg_kernel"),
                      "kernel_ms": float(np.mean(scan_ms)),
                      "alg_bytes_per_launch": alg_bytes},

@@ -81,14 +81,18 @@ enum tdbg_type {
 };
 
 /* src/exec/aggregate.zig:88-92 (count, sum, min, max, avg); COUNT(*) is
- * count with no column (aggregate.zig:244-246). */
+ * count with no column (aggregate.zig:244-246).  COUNT(DISTINCT col)
+ * (aggregate.zig:130 count_distinct; planned by v2_lowcard_group.zig:363-387,
+ * folded by :1348-1418) counts each group's distinct non-NULL values of an
+ * integer-family column: BIGINT, never NULL (0 when every value is NULL). */
 enum tdbg_agg_op {
   TDBG_AGG_COUNT_STAR = 0,
   TDBG_AGG_COUNT = 1,
   TDBG_AGG_SUM = 2,
   TDBG_AGG_AVG = 3,
   TDBG_AGG_MIN = 4,
-  TDBG_AGG_MAX = 5
+  TDBG_AGG_MAX = 5,
+  TDBG_AGG_COUNT_DISTINCT = 6
 };
 
 /* src/exec/predicate.zig:33 `PredicateOp = enum { eq, neq, lt, lte, gt, gte }`. */
@@ -321,8 +325,8 @@ int32_t tdbg_cancel(tdbg_plan* plan);
  * staged row group of its table - as one persistent device launch (captured
  * once as a HIP graph and replayed), so small and uneven queries share the
  * GPU instead of paying a launch-and-drain each.  Plans with ORDER BY,
- * LIMIT or OFFSET are declined (TDBG_ERR_UNSUPPORTED): run them with
- * tdbg_execute.  Every result is the one tdbg_execute returns for the plan.
+ * LIMIT, OFFSET or COUNT(DISTINCT) are declined (TDBG_ERR_UNSUPPORTED): run
+ * them with tdbg_execute.  Every result is the one tdbg_execute returns for the plan.
  * A batch of 64 plans or more runs as 2 waves (THINDB_GPU_BATCH_WAVES sets
  * the number; at least 32 plans each) so that the result copy of each wave
  * overlaps the kernels of the next. */

@@ -27,6 +27,9 @@
  *   - ORDER BY: SortCtx.lessThan/compareBySpec, :2198-2271 (ties by gid,
  *     BIGINT SUM by its wrapped value, AVG by the unscaled mean, floats by
  *     types.floatOrder, src/types.zig:579-587);
+ *   - COUNT(DISTINCT): the (group, value) composites of rows with a non-NULL
+ *     integer value, deduplicated and counted per group (foldDistinctValues /
+ *     compositeOf, v2_lowcard_group.zig:1359-1418, :1497-1499);
  *   - HAVING: havingPasses/havingOutputNum/numOrder,
  *     src/exec/v2_shape_group_topn.zig:2385-2535 (groups filtered before
  *     ORDER BY; a NULL operand fails its leaf; NOT is two-valued; integers
@@ -123,7 +126,7 @@ typedef struct {
   uint32_t offset, width;
 } key_part;
 
-enum { OP_COUNT_STAR, OP_COUNT_COL, OP_SUM, OP_AVG, OP_MIN, OP_MAX };
+enum { OP_COUNT_STAR, OP_COUNT_COL, OP_SUM, OP_AVG, OP_MIN, OP_MAX, OP_COUNT_DISTINCT };
 
 typedef struct {
   int op;
@@ -303,6 +306,25 @@ static inline int pred_pass(const tdbg_predicate* pr, const or_column* c, uint64
   }
 }
 
+/* The packed key of row r: each part's low `width` bits at its offset. */
+static u128 pack_key(const cplan* p, uint64_t r) {
+  u128 key = 0;
+  for (uint32_t k = 0; k < p->n_parts; k++) {
+    const key_part* kp = &p->parts[k];
+    uint64_t raw = (uint64_t)load_i(&p->t->cols[kp->col], r);
+    uint64_t m = kp->width >= 64 ? ~0ull : ((1ull << kp->width) - 1);
+    key |= (u128)(raw & m) << kp->offset;
+  }
+  return key;
+}
+
+static int row_passes(const cplan* p, uint64_t r) {
+  const tdbg_plan_desc* d = p->d;
+  for (uint32_t k = 0; k < d->n_preds; k++)
+    if (!pred_pass(&d->preds[k], &p->t->cols[d->preds[k].column_idx], r)) return 0;
+  return 1;
+}
+
 /* ---- one sub-batch: foldBatch (v2_lowcard_group.zig:977-1050) --------------- */
 static void fold_batch(wstate* w, const cplan* p, uint64_t lo, uint64_t hi) {
   const or_table* t = p->t;
@@ -315,16 +337,7 @@ static void fold_batch(wstate* w, const cplan* p, uint64_t lo, uint64_t hi) {
   }
   if (n == 0) return;
   /* packKeysForPart / packIntPart */
-  for (uint64_t i = 0; i < n; i++) {
-    u128 key = 0;
-    for (uint32_t k = 0; k < p->n_parts; k++) {
-      const key_part* kp = &p->parts[k];
-      uint64_t raw = (uint64_t)load_i(&t->cols[kp->col], w->sel[i]);
-      uint64_t m = kp->width >= 64 ? ~0ull : ((1ull << kp->width) - 1);
-      key |= (u128)(raw & m) << kp->offset;
-    }
-    w->bkeys[i] = key;
-  }
+  for (uint64_t i = 0; i < n; i++) w->bkeys[i] = pack_key(p, w->sel[i]);
   /* runEnds */
   uint32_t runs = 0;
   for (uint64_t i = 1; i < n; i++)
@@ -355,7 +368,7 @@ static void fold_batch(wstate* w, const cplan* p, uint64_t lo, uint64_t hi) {
   /* fold kernels, one aggregate at a time over the runs */
   for (uint32_t a = 0; a < p->n_aggs; a++) {
     const agg_plan* ag = &p->aggs[a];
-    if (ag->op == OP_COUNT_STAR) continue;
+    if (ag->op == OP_COUNT_STAR || ag->op == OP_COUNT_DISTINCT) continue; /* distinct: after the merge */
     const or_column* c = &t->cols[ag->col];
     uint32_t lo_i = 0;
     for (uint32_t ri = 0; ri < runs; ri++) {
@@ -523,7 +536,7 @@ static int compare_spec(const tdbg_order_spec* sp, uint32_t a, uint32_t b) {
   size_t oa = (size_t)a * p->n_aggs + i, ob = (size_t)b * p->n_aggs + i;
   switch (ag->op) {
     case OP_COUNT_STAR: return m->counts[a] < m->counts[b] ? -1 : (m->counts[a] > m->counts[b]);
-    case OP_COUNT_COL: return m->ns[oa] < m->ns[ob] ? -1 : (m->ns[oa] > m->ns[ob]);
+    case OP_COUNT_COL: case OP_COUNT_DISTINCT: return m->ns[oa] < m->ns[ob] ? -1 : (m->ns[oa] > m->ns[ob]);
     case OP_SUM: case OP_MIN: case OP_MAX:
       if (ag->is_float) return float_order(m->fslots[oa], m->fslots[ob]);
       if (ag->out_type == TDBG_BIGINT) {
@@ -571,7 +584,7 @@ static hnum output_num(const cplan* p, const wstate* m, uint32_t target, uint32_
   size_t o = (size_t)gid * p->n_aggs + a;
   uint64_t ns = m->ns[o];
   if (ag->op == OP_COUNT_STAR) { v.i = (i128)m->counts[gid]; return v; }
-  if (ag->op == OP_COUNT_COL) { v.i = (i128)ns; return v; }
+  if (ag->op == OP_COUNT_COL || ag->op == OP_COUNT_DISTINCT) { v.i = (i128)ns; return v; }
   if (ag->nullable && ns == 0) { v.null = 1; return v; } /* aggIsNull */
   if (ag->op == OP_AVG) {
     v.is_f = 1;
@@ -680,7 +693,7 @@ static int emit(const cplan* p, const wstate* m, or_result** out) {
     tdbg_column_view* c = &r->cols[p->n_parts + a];
     c->type = ag->out_type;
     c->scale = (ag->out_type == TDBG_DECIMAL128 || ag->out_type == TDBG_DECIMAL64) ? ag->scale : 0;
-    c->nullable = !(ag->op == OP_COUNT_STAR || ag->op == OP_COUNT_COL);
+    c->nullable = !(ag->op == OP_COUNT_STAR || ag->op == OP_COUNT_COL || ag->op == OP_COUNT_DISTINCT);
     c->n_rows = nr;
     void* data = calloc(nr + 1, type_bytes(c->type));
     uint8_t* valid = calloc(nr / 8 + 1, 1);
@@ -695,7 +708,7 @@ static int emit(const cplan* p, const wstate* m, or_result** out) {
       int is_null = 0;
       switch (ag->op) {
         case OP_COUNT_STAR: ((int64_t*)data)[i] = (int64_t)m->counts[gid]; break;
-        case OP_COUNT_COL: ((int64_t*)data)[i] = (int64_t)ns; break;
+        case OP_COUNT_COL: case OP_COUNT_DISTINCT: ((int64_t*)data)[i] = (int64_t)ns; break;
         case OP_SUM: case OP_MIN: case OP_MAX:
           if (ns == 0) { is_null = 1; break; }
           if (ag->is_float) {
@@ -724,6 +737,55 @@ static int emit(const cplan* p, const wstate* m, or_result** out) {
   }
   free(order);
   *out = r;
+  return 0;
+}
+
+/* ---- COUNT(DISTINCT): the (group, value) composites of the filtered rows
+ * with a non-NULL value, deduplicated, counted per group (the distinct sets
+ * of foldDistinctValues/compositeOf, v2_lowcard_group.zig:1359-1418,
+ * :1497-1499, counted after the merge :1538-1552) ------------------------------ */
+typedef struct {
+  uint32_t gid;
+  int64_t v;
+} dpair;
+
+static int dpair_cmp(const void* pa, const void* pb) {
+  const dpair* a = pa;
+  const dpair* b = pb;
+  if (a->gid != b->gid) return a->gid < b->gid ? -1 : 1;
+  return a->v < b->v ? -1 : (a->v > b->v);
+}
+
+static uint32_t ws_find(const wstate* w, const cplan* p, u128 key) {
+  uint64_t mask = w->cap - 1;
+  for (uint64_t h = hash_key(p, key) & mask; w->tgids[h] != EMPTY_GID; h = (h + 1) & mask)
+    if (w->tkeys[h] == key) return w->tgids[h];
+  return EMPTY_GID;
+}
+
+static int distinct_counts(const cplan* p, wstate* m) {
+  const or_table* t = p->t;
+  for (uint32_t a = 0; a < p->n_aggs; a++) {
+    const agg_plan* ag = &p->aggs[a];
+    if (ag->op != OP_COUNT_DISTINCT) continue;
+    const or_column* c = &t->cols[ag->col];
+    dpair* pairs = malloc((t->n_rows + 1) * sizeof(dpair));
+    if (!pairs) return -1;
+    uint64_t n = 0;
+    for (uint64_t r = 0; r < t->n_rows; r++) {
+      if (!is_valid(c, r) || !row_passes(p, r)) continue;
+      uint32_t gid = ws_find(m, p, pack_key(p, r));
+      if (gid == EMPTY_GID) { free(pairs); return -1; } /* every passing row has its group */
+      pairs[n].gid = gid;
+      pairs[n].v = load_i(c, r);
+      n++;
+    }
+    qsort(pairs, n, sizeof(dpair), dpair_cmp);
+    for (uint64_t i = 0; i < n; i++)
+      if (i == 0 || pairs[i].gid != pairs[i - 1].gid || pairs[i].v != pairs[i - 1].v)
+        m->ns[(size_t)pairs[i].gid * p->n_aggs + a]++;
+    free(pairs);
+  }
   return 0;
 }
 
@@ -801,6 +863,12 @@ static int compile(cplan* p, const tdbg_plan_desc* d, const or_table* t) {
     ag->nullable = t->cols[s->column_idx].validity != NULL;
     row_bytes += type_bytes(s->input_type);
     if (s->op == TDBG_AGG_COUNT) { ag->op = OP_COUNT_COL; ag->out_type = TDBG_BIGINT; continue; }
+    if (s->op == TDBG_AGG_COUNT_DISTINCT) { /* integer values only (distinctValueBits of an int) */
+      if (!key_type_bits(s->input_type)) { set_err("COUNT(DISTINCT) input type"); return TDBG_ERR_UNSUPPORTED; }
+      ag->op = OP_COUNT_DISTINCT;
+      ag->out_type = TDBG_BIGINT;
+      continue;
+    }
     int fl = is_float_type(s->input_type);
     if (!fl && !key_type_bits(s->input_type)) { set_err("agg input type"); return TDBG_ERR_UNSUPPORTED; }
     if ((s->input_type == TDBG_DATE || s->input_type == TDBG_DATETIME) &&
@@ -917,7 +985,7 @@ int32_t oracle_group_by(const tdbg_plan_desc* d, const or_table* t, uint32_t dop
   }
   if (!fail) {
     /* COUNT(*) reads `counts`; the worker slots only track ns for inputs */
-    if (emit(&p, &merged, out)) fail = 1;
+    if (distinct_counts(&p, &merged) || emit(&p, &merged, out)) fail = 1;
   }
   if (have_merged) ws_free(&merged);
   for (uint32_t i = 0; ws && i < dop; i++) ws_free(&ws[i]);

@@ -27,7 +27,7 @@ FLOATS = (T_FLOAT, T_DOUBLE)
 DTYPE = {T_BOOLEAN: np.uint8, T_TINYINT: np.int8, T_SMALLINT: np.int16, T_INT: np.int32, T_BIGINT: np.int64,
          T_DATE: np.int32, T_DATETIME: np.int64, T_DECIMAL64: np.int64, T_FLOAT: np.float32, T_DOUBLE: np.float64}
 OPS = {"eq": 0, "neq": 1, "lt": 2, "lte": 3, "gt": 4, "gte": 5}
-AGG = {"count*": 0, "count": 1, "sum": 2, "avg": 3, "min": 4, "max": 5}
+AGG = {"count*": 0, "count": 1, "sum": 2, "avg": 3, "min": 4, "max": 5, "count_distinct": 6}
 
 
 class _Col(C.Structure):
@@ -216,7 +216,7 @@ def _out_types(schema, idx, request) -> tuple:
         scales.append(c.scale)
     for a in request.aggs:
         f = a.func.lower()
-        if f == "count":
+        if f in ("count", "count_distinct"):
             types.append(T_BIGINT)
             scales.append(0)
             continue

@@ -150,6 +150,9 @@ def brute_force(table: Table, request: Request) -> Dict[tuple, list]:
             if f == "count":
                 vals.append(rows if a.col is None else len(xs))
                 continue
+            if f == "count_distinct":
+                vals.append(len(set(xs)))
+                continue
             if not xs:
                 vals.append(None)
                 continue

@@ -1,0 +1,77 @@
+"""COUNT(DISTINCT col) on the GPU, through the C-ABI, against the oracle:
+the LDS direct table, two keys with filters, ORDER BY / LIMIT / HAVING over
+the count, radix partitioning with compacted and dense output, the
+high-cardinality workload, and the shapes that decline."""
+import numpy as np
+import pytest
+
+from helpers import brute_force, rows_of
+from test_count_distinct import AGGS, _table
+from test_gpu_parity import check_vs_oracle, run_gpu
+from thindb_amd import gen, gpu
+from thindb_amd.exec import (AggSpec, And, Column, GpuBatchRun, Predicate, Request, SortSpec, Table,
+                             try_build_gpu_group)
+from thindb_amd.gpu import Type
+
+pytestmark = pytest.mark.gpu
+
+
+def test_count_distinct_vs_oracle(session, oracle_lib):
+    t = _table()
+    where = And((Predicate("i", "gt", -2 ** 30), Predicate("d", "gte", -500.0)))
+    for req in (Request(["k1"], AGGS),
+                Request(["k1", "k2"], AGGS, where_filter=where),
+                Request(["k2"], AGGS, [SortSpec("ds", True)], where_filter=where),
+                Request(["k1", "k2"], AGGS, [SortSpec("dm", True), SortSpec("k1"), SortSpec("k2")], 20, 3),
+                Request(["k1"], AGGS, having_filter=Predicate("ds", "lt", 60)),
+                Request(["k1"], AGGS, where_filter=Predicate("i", "gt", 2 ** 31))):
+        got, want, _ = check_vs_oracle(session, oracle_lib, t, req)
+    # brute force on the plain grouping, and the all-NULL groups
+    b, _ = run_gpu(session, t, Request(["k1"], AGGS))
+    bf = brute_force(t, Request(["k1"], AGGS))
+    for r in rows_of(b):
+        assert list(r[1:]) == bf[(r[0],)]
+    assert all(r[3] == 0 for r in rows_of(b) if r[0] < -45)
+
+
+@pytest.mark.parametrize("est", [0, 100_000_000])
+def test_count_distinct_radix(session, oracle_lib, monkeypatch, est):
+    """Radix partitioning (compacted leaves, then dense output) with the
+    distinct count of a column that is not a key."""
+    t = gen.table("highcard", 1_200_000, seed=21)
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    aggs = [AggSpec("count", None, "c"), AggSpec("count_distinct", "v", "dv"), AggSpec("sum", "v", "s")]
+    got, want, st = check_vs_oracle(session, oracle_lib, t, Request(["user_id", "bucket"], aggs, est_groups=est),
+                                    dop=8)
+    assert st.global_mode == 3
+    check_vs_oracle(session, oracle_lib, t, Request(["user_id", "bucket"], aggs, [SortSpec("dv", True)], 10,
+                                                    est_groups=est, having_filter=Predicate("c", "gte", 2)), dop=8)
+
+
+def test_count_distinct_highcard_values(session, oracle_lib):
+    """Few groups, many distinct values: GROUP BY bucket,
+    COUNT(DISTINCT user_id) (the distinct set holds ~1.2M composites)."""
+    t = gen.table("highcard", 1_500_000, seed=22)
+    req = Request(["bucket"], [AggSpec("count", None, "c"), AggSpec("count_distinct", "user_id", "u")],
+                  [SortSpec("u", True)], 10)
+    got, want, _ = check_vs_oracle(session, oracle_lib, t, req, dop=8)
+    assert len(got) == 10 and all(r[2] <= r[1] for r in got)
+
+
+def test_count_distinct_declines(session):
+    """A float input declines at planning; a sparse key domain beyond 2^31
+    relative keys declines at execution; a batch refuses the plan."""
+    rng = np.random.default_rng(1)
+    t = _table()
+    assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("count_distinct", "d", "x")])) is None
+    n = 20_000
+    keys = rng.integers(-2 ** 40, 2 ** 40, 500)
+    wide = Table([Column("k", Type.BIGINT), Column("v", Type.INT)],
+                 {"k": keys[rng.integers(0, 500, n)], "v": rng.integers(0, 9, n).astype(np.int32)})
+    q = try_build_gpu_group(session, wide, Request(["k"], [AggSpec("count_distinct", "v", "d")]))
+    assert q is not None
+    with pytest.raises(gpu.Unsupported):
+        q.next()
+    q.deinit()
+    with pytest.raises(Exception):
+        GpuBatchRun(session, [(t, Request(["k1"], AGGS))])

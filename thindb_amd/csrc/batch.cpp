@@ -663,8 +663,8 @@ int32_t tdbg_batch_create(tdbg_context* ctx, tdbg_plan* const* plans, uint32_t n
   for (uint32_t i = 0; i < n_plans; i++) {
     const tdbg_plan* pl = plans[i];
     if (!pl || pl->ctx != ctx) return fail(TDBG_ERR_INVALID, "plan %u is null or of another context", i);
-    if (!pl->order.empty() || pl->limit || pl->offset)
-      return fail(TDBG_ERR_UNSUPPORTED, "plan %u has ORDER BY / LIMIT / OFFSET: run it alone", i);
+    if (!plan_batchable(pl))
+      return fail(TDBG_ERR_UNSUPPORTED, "plan %u has ORDER BY / LIMIT / OFFSET or COUNT(DISTINCT): run it alone", i);
   }
   int32_t rc = make_batch(ctx, plans, n_plans, nullptr, out);
   if (rc == TDBG_OK && (rc = split_waves(*out))) {

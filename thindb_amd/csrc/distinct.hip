@@ -1,0 +1,105 @@
+// distinct.hip — COUNT(DISTINCT x) per group.
+//
+// The reference folds each row's (packed group key, value bits) composite
+// into per-worker distinct sets and counts each group's members after the
+// merge (foldDistinctKernel/foldDistinctValues/compositeOf,
+// src/exec/v2_lowcard_group.zig:1348-1418, :1497-1499; partition merge
+// :1538-1552).  Here one pass over the plan's tiles (the same strips,
+// predicates and relative keys as the scan) inserts the composite
+// key * span + (x - base) into an open-addressing set in HBM with one
+// compare-and-swap; the insertion that finds the slot empty adds one to the
+// group's counter, so every (group, value) pair is counted exactly once.
+// NULL values are never inserted (a NULL is not a counted value).
+#include "order_device.h"
+#include "radix_device.h"
+
+namespace tdbg {
+
+// LDS_COUNTS: a domain of at most kDistinctLdsKeys groups counts in LDS per
+// workgroup and adds its totals once at the end (few groups with many
+// distinct values would otherwise serialise on a handful of counters).
+template <bool LDS_COUNTS>
+__global__ void __launch_bounds__(256) distinct_insert_kernel(const ScanParams P, const DistinctParams D,
+                                                              uint32_t n_keys) {
+  __shared__ uint32_t lcnt[LDS_COUNTS ? kDistinctLdsKeys : 1];
+  if (LDS_COUNTS) {
+    for (uint32_t k = threadIdx.x; k < n_keys; k += blockDim.x) lcnt[k] = 0;
+    __syncthreads();
+  }
+  for (uint32_t t = blockIdx.x; t < P.n_tiles; t += gridDim.x) {
+    const Tile T = P.tiles[t];
+    for (uint32_t s = threadIdx.x * kVec; s < T.n; s += blockDim.x * kVec) {
+      const uint32_t cnt = T.n - s < (uint32_t)kVec ? T.n - s : (uint32_t)kVec;
+      const uint64_t r = T.row0 + s;
+      uint64_t key[kVec];
+      uint32_t mask = rx_strip_keys(P, r, cnt, key, true) & valid4(D.x, r);
+      if (!mask) continue;
+      int64_t v[kVec];
+      decode_i(load_raw(D.x, r), D.x.kind, v);
+#pragma unroll
+      for (int j = 0; j < kVec; j++) {
+        if (!((mask >> j) & 1u)) continue;
+        const uint64_t rel = (uint64_t)v[j] - (uint64_t)D.base;
+        if (rel >= D.span) {  // outside the zone-map statistics
+          atomicOr(P.err, ERR_KEY_RANGE);
+          continue;
+        }
+        const unsigned long long c = key[j] * D.span + rel;
+        // at least twice as many slots as rows scanned: an empty slot ends
+        // every probe
+        uint64_t h = mix64(c) & D.set_mask;
+        for (;;) {
+          const unsigned long long prev = atomicCAS(&D.set[h], ~0ull, c);
+          if (prev == ~0ull) {
+            if (LDS_COUNTS) atomicAdd(&lcnt[key[j]], 1u);
+            else atomicAdd(&D.counts[key[j]], 1u);
+            break;
+          }
+          if (prev == c) break;
+          h = (h + 1) & D.set_mask;
+        }
+      }
+    }
+  }
+  if (LDS_COUNTS) {
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n_keys; k += blockDim.x)
+      if (lcnt[k]) atomicAdd(&D.counts[k], lcnt[k]);
+  }
+}
+
+// Each compacted group's count from its relative key (the dense index, or
+// the key word of a keyed record).
+__global__ void __launch_bounds__(256) distinct_gather_kernel(const FinalParams F, const DistinctParams D,
+                                                              const uint32_t* __restrict__ gids,
+                                                              const uint64_t* __restrict__ n_groups, uint64_t max_n) {
+  const uint64_t n = *n_groups < max_n ? *n_groups : max_n;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t gid = gids[i];
+    uint8_t* rec = const_cast<uint8_t*>(F.g_state) + (uint64_t)gid * F.g_rec;
+    const uint64_t key = group_key(rec, gid, F.g_mode, F.g_key_off);
+    *reinterpret_cast<uint64_t*>(rec + D.dslot_off) = D.counts[key];
+  }
+}
+
+hipError_t launch_distinct_insert(const ScanParams& sp, const DistinctParams& D, uint64_t n_keys, uint32_t grid,
+                                  hipStream_t s) {
+  if (sp.n_tiles == 0) return hipSuccess;
+  if (grid == 0 || !D.set || !D.counts || D.span == 0 || n_keys == 0) return hipErrorInvalidValue;
+  if (n_keys <= kDistinctLdsKeys)
+    hipLaunchKernelGGL(distinct_insert_kernel<true>, dim3(grid), dim3(256), 0, s, sp, D, (uint32_t)n_keys);
+  else
+    hipLaunchKernelGGL(distinct_insert_kernel<false>, dim3(grid), dim3(256), 0, s, sp, D, 0u);
+  return hipGetLastError();
+}
+
+hipError_t launch_distinct_gather(const FinalParams& fp, const DistinctParams& D, const uint32_t* gids,
+                                  const uint64_t* n_groups, uint64_t max_n, hipStream_t s) {
+  if (max_n == 0) return hipSuccess;
+  const uint64_t g = (max_n + 255) / 256;
+  hipLaunchKernelGGL(distinct_gather_kernel, dim3((uint32_t)(g < 8192 ? g : 8192)), dim3(256), 0, s, fp, D, gids,
+                     n_groups, max_n);
+  return hipGetLastError();
+}
+
+}  // namespace tdbg

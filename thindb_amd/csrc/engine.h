@@ -249,6 +249,21 @@ struct HavingNode {
   double lit_f64;
 };
 
+// COUNT(DISTINCT x) of one plan: every filtered row with a non-NULL x adds
+// its (group, x) composite, key * span + (x - base), to an open-addressing
+// set; a composite's first insertion counts one for its group (`counts`,
+// indexed by the group's relative key).
+struct DistinctParams {
+  ColDesc x;                // the input column
+  int64_t base;             // its zone-map minimum
+  uint64_t span;            // max - min + 1
+  unsigned long long* set;  // 2^set_bits slots, empty = ~0
+  uint64_t set_mask;
+  uint32_t* counts;         // one per relative key of the plan's domain
+  uint32_t dslot_off;       // record offset of the group's distinct count
+  uint32_t pad;
+};
+
 // ---- batched execution (one persistent launch over many plans) ------------
 
 // A work item of the persistent kernel: tiles [a0, a1) of plan q for a scan
@@ -323,6 +338,14 @@ size_t having_scratch_bytes(uint64_t max_n);
 hipError_t launch_having(const FinalParams& p, const HavingNode* nodes, uint32_t n_nodes, const uint32_t* gids_in,
                          const uint64_t* n_in, uint64_t max_n, void* scratch, uint32_t* gids_out, uint64_t* n_kept,
                          hipStream_t s);
+// COUNT(DISTINCT): inserts the scan's (group, x) composites (the plan's
+// tiles and predicates), then writes each compacted group's count into its
+// record at D.dslot_off.  The set must hold at least twice the rows scanned.
+constexpr uint32_t kDistinctLdsKeys = 4096;  // domains up to this many groups count in LDS
+hipError_t launch_distinct_insert(const ScanParams& sp, const DistinctParams& D, uint64_t n_keys, uint32_t grid,
+                                  hipStream_t s);
+hipError_t launch_distinct_gather(const FinalParams& fp, const DistinctParams& D, const uint32_t* gids,
+                                  const uint64_t* n_groups, uint64_t max_n, hipStream_t s);
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
                            hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);

@@ -245,11 +245,13 @@ int32_t validate(tdbg_plan* pl) {
   };
   for (auto& k : pl->keys) use(k.column_idx);
   for (auto& a : pl->aggs) {
-    if (a.op > TDBG_AGG_MAX) return fail(TDBG_ERR_INVALID, "aggregate op %u", a.op);
+    if (a.op > TDBG_AGG_COUNT_DISTINCT) return fail(TDBG_ERR_INVALID, "aggregate op %u", a.op);
     if (a.op == TDBG_AGG_COUNT_STAR) continue;
     if (a.column_idx < 0) return fail(TDBG_ERR_INVALID, "aggregate without a column");
     if (!is_int_type(a.input_type) && !is_float_type(a.input_type))
       return fail(TDBG_ERR_UNSUPPORTED, "aggregate input type %u", a.input_type);
+    if (a.op == TDBG_AGG_COUNT_DISTINCT && !is_int_type(a.input_type))
+      return fail(TDBG_ERR_UNSUPPORTED, "COUNT(DISTINCT) over a float column");
     if ((a.input_type == TDBG_DATE || a.input_type == TDBG_DATETIME) && (a.op == TDBG_AGG_SUM || a.op == TDBG_AGG_AVG))
       return fail(TDBG_ERR_UNSUPPORTED, "temporal SUM/AVG");
     use((uint32_t)a.column_idx);
@@ -308,7 +310,7 @@ int32_t validate(tdbg_plan* pl) {
     o.is_float = is_float_type(a.input_type);
     o.nullable = a.nullable != 0;
     switch (a.op) {
-      case TDBG_AGG_COUNT_STAR: case TDBG_AGG_COUNT: o.out_type = TDBG_BIGINT; break;
+      case TDBG_AGG_COUNT_STAR: case TDBG_AGG_COUNT: case TDBG_AGG_COUNT_DISTINCT: o.out_type = TDBG_BIGINT; break;
       case TDBG_AGG_SUM:
         o.out_type = o.is_float ? TDBG_DOUBLE : (a.input_type == TDBG_DECIMAL64 ? TDBG_DECIMAL128 : TDBG_BIGINT);
         break;
@@ -511,6 +513,12 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     if (it->second.type != a.input_type) return fail(TDBG_ERR_INVALID, "column %d type mismatch", a.column_idx);
     const bool nullable = it->second.d_valid != nullptr;
     o.nullable = nullable;
+    if (a.op == TDBG_AGG_COUNT_DISTINCT) {  // a count slot of its own, filled after compaction
+      o.nullable = false;
+      o.dslot = (int)P.distinct.size();
+      P.distinct.push_back({(uint32_t)a.column_idx, 0u});
+      continue;
+    }
     if (a.op == TDBG_AGG_COUNT && !nullable) continue;  // = COUNT(*)
     uint32_t kind;
     switch (a.op) {
@@ -557,6 +565,11 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     if (d.kind == A_SUM_I) { tmpl_put(off, 0); off += 8; }
     d.g_ns_off = ~0u;
     if (d.nullable) { d.g_ns_off = off; tmpl_put(off, 0); off += 8; }
+  }
+  for (auto& d : P.distinct) {
+    d.dslot_off = off;
+    tmpl_put(off, 0);
+    off += 8;
   }
   // an integer SUM state read only as a wrapped BIGINT (no AVG, no DECIMAL
   // SUM over it) needs just its low word: single wrapping HBM adds
@@ -715,6 +728,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
             sp.aggs[k].g_off += 8;
             if (sp.aggs[k].g_ns_off != ~0u) sp.aggs[k].g_ns_off += 8;
           }
+          for (auto& dd : P.distinct) dd.dslot_off += 8;
           sp.g_rec += 8;
           P.tmpl.insert(P.tmpl.begin(), kEmptyKey);
         }
@@ -790,6 +804,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     };
     if (o.target < nk) { add(C_KEY, o.target, 0, ~0u); continue; }
     const AggOut& ao = pl->outs[o.target - nk];
+    if (ao.dslot >= 0) { add(C_U64_AT, 0, P.distinct[ao.dslot].dslot_off, ~0u); continue; }
     if (ao.kagg < 0) { add(C_COUNT, 0, 0, ~0u); continue; }
     const AggDesc& d = sp.aggs[ao.kagg];
     switch (ao.op) {
@@ -829,6 +844,11 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     switch (ao.op) {
       case TDBG_AGG_COUNT_STAR: o.kind = O_COUNT; o.valid_off = ~0ull; break;
       case TDBG_AGG_COUNT: o.kind = d ? O_NS : O_COUNT; o.valid_off = ~0ull; break;
+      case TDBG_AGG_COUNT_DISTINCT:
+        o.kind = O_NS;
+        o.off = P.distinct[ao.dslot].dslot_off;
+        o.valid_off = ~0ull;
+        break;
       case TDBG_AGG_SUM:
         o.kind = ao.is_float ? O_SUM_F64 : (ao.out_type == TDBG_DECIMAL128 ? O_SUM_I128 : O_SUM_I64);
         break;
@@ -1219,7 +1239,7 @@ void tdbg_plan_destroy(tdbg_plan* pl) {
   if (pl->stream) (void)hipStreamSynchronize(pl->stream);
   for (auto* b : {&pl->state, &pl->tiles, &pl->err, &pl->gids, &pl->counts, &pl->sel, &pl->scratch, &pl->out, &pl->tmpl, &pl->nout,
                  &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf, &pl->rx_words, &pl->having_nodes,
-                 &pl->having_gids})
+                 &pl->having_gids, &pl->dist_set, &pl->dist_counts})
     release(pl, *b);
   for (auto& e : pl->ev) if (e) (void)hipEventDestroy(e);
   if (pl->h_in) (void)hipHostFree(pl->h_in);
@@ -1260,7 +1280,7 @@ int32_t tdbg::rt::make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host
       const AggOut& o = pl->outs[c - nk];
       type = o.out_type;
       if (o.out_type == TDBG_DECIMAL128 || o.out_type == TDBG_DECIMAL64) scale = o.scale;
-      nullable = o.op != TDBG_AGG_COUNT_STAR && o.op != TDBG_AGG_COUNT;
+      nullable = o.op != TDBG_AGG_COUNT_STAR && o.op != TDBG_AGG_COUNT && o.op != TDBG_AGG_COUNT_DISTINCT;
     }
     const uint64_t bytes = n_out * (uint64_t)type_bytes(type);
     if (!keep) {
@@ -1286,6 +1306,49 @@ int32_t tdbg::rt::make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host
     rc.n_rows = n_out;
   }
   *out = r;
+  return TDBG_OK;
+}
+
+// COUNT(DISTINCT dd.col) of the plan's groups (distinct.hip): a set of
+// (group, value) composites with at least twice as many slots as rows
+// scanned, a counter per relative key, then each compacted group's count.
+int32_t tdbg::rt::run_distinct(tdbg_plan* pl, Table* t, const ScanSet& S, const Prepared::Distinct& dd,
+                               const uint32_t* gids, const uint64_t* n_groups, uint64_t max_groups) {
+  Prepared& P = pl->prep;
+  const ScanParams& sp = P.sp;
+  hipStream_t s = pl->stream;
+  auto it = t->cols.find(dd.col);
+  if (it == t->cols.end()) return fail(TDBG_ERR_NOT_STAGED, "column %u not staged", dd.col);
+  const ColumnArena& a = it->second;
+  unsigned __int128 dom = 1;
+  for (uint32_t i = 0; i < sp.n_parts; i++) dom *= sp.parts[i].span;
+  if (dom > kDistinctDomainMax)
+    return fail(TDBG_ERR_UNSUPPORTED, "COUNT(DISTINCT) over a key domain of more than 2^31 groups");
+  DistinctParams D{};
+  D.x.data = a.d_data;
+  D.x.valid = a.d_valid;
+  D.x.kind = kind_of(a.type);
+  D.base = a.has_stats ? a.min_i : 0;
+  D.span = a.has_stats ? (uint64_t)a.max_i - (uint64_t)a.min_i + 1 : 1;
+  if (D.span == 0 || dom * D.span >= ((unsigned __int128)1 << 63))
+    return fail(TDBG_ERR_UNSUPPORTED, "COUNT(DISTINCT) composites wider than 63 bits");
+  D.dslot_off = dd.dslot_off;
+  const uint64_t pairs = std::min<uint64_t>(std::max<uint64_t>(S.rows, 1), (uint64_t)(dom * D.span));
+  uint64_t slots = 1024;
+  while (slots < 2 * pairs) slots <<= 1;
+  int32_t rc;
+  if ((rc = ensure(pl, pl->dist_set, slots * 8))) return rc;
+  if ((rc = ensure(pl, pl->dist_counts, (uint64_t)dom * 4 + 256))) return rc;
+  D.set = static_cast<unsigned long long*>(pl->dist_set.p);
+  D.set_mask = slots - 1;
+  D.counts = static_cast<uint32_t*>(pl->dist_counts.p);
+  HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(D.set), slots * 2, ~0u, s), "distinct set");
+  HIP_TRY(launch_fill_u32(D.counts, (uint64_t)dom, 0, s), "distinct counts");
+  // LDS counters (small domains) are flushed once per workgroup: fewer, longer-lived workgroups
+  const uint32_t per_cu = (uint64_t)dom <= kDistinctLdsKeys ? 4u : 8u;
+  const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(sp.n_tiles, (uint32_t)std::max(pl->ctx->n_cu, 8) * per_cu));
+  HIP_TRY(launch_distinct_insert(sp, D, (uint64_t)dom, grid, s), "distinct insert");
+  HIP_TRY(launch_distinct_gather(P.fp, D, gids, n_groups, max_groups, s), "distinct counts to groups");
   return TDBG_OK;
 }
 
@@ -1430,10 +1493,15 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     HIP_TRY(launch_compact(P.op, n_rec, static_cast<uint32_t*>(pl->counts.p), static_cast<uint32_t*>(pl->gids.p),
                            static_cast<uint64_t*>(pl->nout.p), s), "compact");
   }
-  // HAVING: the passing groups' positions, in order, to having_gids and their
-  // number to nout[4]
   uint32_t* gids = static_cast<uint32_t*>(pl->gids.p);
   uint64_t* nout = static_cast<uint64_t*>(pl->nout.p);
+  // COUNT(DISTINCT): one insertion pass per input column, then each group's
+  // count into its record
+  for (const Prepared::Distinct& dd : P.distinct) {
+    if ((rc = run_distinct(pl, t, S, dd, gids, nout, n_rec))) return rc;
+  }
+  // HAVING: the passing groups' positions, in order, to having_gids and their
+  // number to nout[4]
   const bool having = !pl->having.empty();
   if (having) {
     // survivors' gids, then the pass masks and per-workgroup counts

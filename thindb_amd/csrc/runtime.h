@@ -90,6 +90,7 @@ inline uint64_t rg_hash(uint64_t seg, uint32_t rg) { return seg * 0x9e3779b97f4a
 struct AggOut {
   uint32_t op;          // tdbg_agg_op
   int kagg;             // index into kernel aggs, -1 when served by the row count
+  int dslot = -1;       // COUNT(DISTINCT): index into Prepared::distinct
   uint32_t in_type, out_type, scale;
   bool is_float;
   bool nullable;
@@ -121,6 +122,13 @@ struct Prepared {
   bool radix = false;          // high cardinality: radix partitioning (radix.hip)
   RadixParams rx{};
   uint32_t agg_grid = 0;
+  // COUNT(DISTINCT) inputs: the table column and the record offset of each
+  // group's count (distinct.hip fills it after compaction)
+  struct Distinct {
+    uint32_t col;
+    uint32_t dslot_off;
+  };
+  std::vector<Distinct> distinct;
 };
 
 struct MergedPred {
@@ -203,7 +211,7 @@ struct tdbg_plan {
     void* p = nullptr;
     uint64_t bytes = 0;
   } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words,
-      having_nodes, having_gids;
+      having_nodes, having_gids, dist_set, dist_counts;
   // pinned host staging: inputs (tiles, template) and the result the
   // columns of a returned tdbg_result view
   uint8_t* h_in = nullptr;
@@ -225,6 +233,15 @@ struct tdbg_plan {
 namespace tdbg {
 namespace rt {
 
+// Plans a batch (and the serving loop's batched lane) can run: no ORDER BY,
+// LIMIT, OFFSET or COUNT(DISTINCT).
+inline bool plan_batchable(const tdbg_plan* pl) {
+  if (!pl->order.empty() || pl->limit || pl->offset) return false;
+  for (auto& a : pl->aggs)
+    if (a.op == TDBG_AGG_COUNT_DISTINCT) return false;
+  return true;
+}
+
 int32_t dev_alloc(tdbg_context* ctx, void** p, uint64_t bytes);
 void dev_free(tdbg_context* ctx, void* p, uint64_t bytes);
 int32_t ensure(tdbg_plan* pl, tdbg_plan::Buf& b, uint64_t bytes);
@@ -239,6 +256,12 @@ int32_t collect_scan(tdbg_plan* pl, Table* t, const tdbg_rg_ref* rgs, uint32_t n
 
 // Specialises the plan to the table's statistics and the scanned row count.
 int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& s, const PlanHint& hint);
+
+// COUNT(DISTINCT) over the compacted groups gids[0, *n_groups) (device count,
+// at most max_groups), on the plan's stream.
+constexpr uint64_t kDistinctDomainMax = 1ull << 31;
+int32_t run_distinct(tdbg_plan* pl, Table* t, const ScanSet& S, const Prepared::Distinct& dd, const uint32_t* gids,
+                     const uint64_t* n_groups, uint64_t max_groups);
 
 // A result of n_out rows whose column c sits at host + data_off[c] (validity
 // bytes at host + valid_off[c], or ~0 for none): copied out of `host`, or,

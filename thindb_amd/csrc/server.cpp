@@ -213,7 +213,7 @@ int32_t tdbg_server_submit(tdbg_server* s, tdbg_plan* plan, tdbg_ticket** out) {
   t->t_submit = now_ns();
   uint64_t rows = 0;
   t->est = estimate(s->ctx, plan, &rows);
-  t->large = rows >= s->cfg.large_rows || !plan->order.empty() || plan->limit || plan->offset;
+  t->large = rows >= s->cfg.large_rows || !plan_batchable(plan);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stopping) {

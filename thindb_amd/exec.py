@@ -48,7 +48,7 @@ class Column:
 
 @dataclass(frozen=True)
 class AggSpec:
-    """src/exec/aggregate.zig:237-268; func in count/sum/avg/min/max/udf."""
+    """src/exec/aggregate.zig:237-268; func in count/count_distinct/sum/avg/min/max/udf."""
     func: str
     col: Optional[str]
     as_: str
@@ -212,7 +212,7 @@ class Batch:
 
 # ---- request -> plan ------------------------------------------------------------------
 
-_AGG_FUNCS = {"count", "sum", "avg", "min", "max"}
+_AGG_FUNCS = {"count", "count_distinct", "sum", "avg", "min", "max"}
 _OPS = {"eq": PredOp.EQ, "neq": PredOp.NEQ, "lt": PredOp.LT, "lte": PredOp.LTE, "gt": PredOp.GT, "gte": PredOp.GTE}
 _INT_TYPES = {Type.BOOLEAN, Type.TINYINT, Type.SMALLINT, Type.INT, Type.BIGINT, Type.DATE, Type.DATETIME,
               Type.DECIMAL64}
@@ -320,7 +320,7 @@ def output_schema(table: Table, request: Request) -> List[Column]:
            for k in request.group_cols]
     for a in request.aggs:
         f = a.func.lower()
-        if f == "count":
+        if f in ("count", "count_distinct"):
             out.append(Column(a.as_, Type.BIGINT, False))
             continue
         c = table.column(a.col)
@@ -447,7 +447,10 @@ def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu
         c = table.column(a.col)
         if c.type not in _INT_TYPES and c.type not in _FLOAT_TYPES:
             raise Declined(f"aggregate input {a.col}")
-        op = {"count": AggOp.COUNT, "sum": AggOp.SUM, "avg": AggOp.AVG, "min": AggOp.MIN, "max": AggOp.MAX}[f]
+        if f == "count_distinct" and c.type not in _INT_TYPES:
+            raise Declined(f"COUNT(DISTINCT) over the float column {a.col}")
+        op = {"count": AggOp.COUNT, "count_distinct": AggOp.COUNT_DISTINCT, "sum": AggOp.SUM, "avg": AggOp.AVG,
+              "min": AggOp.MIN, "max": AggOp.MAX}[f]
         aggs.append((op, table.column_index(a.col), c.type, c.scale, c.nullable))
         cols.append(c.name)
     preds = []
@@ -490,8 +493,8 @@ def try_build_gpu_group(session: GpuSession, table: Table, request: Request) -> 
 class GpuBatchRun:
     """Many grouped requests over (possibly different) tables run together in
     one persistent device launch (gpu.Batch); each result equals what the
-    request returns alone.  ORDER BY / LIMIT / OFFSET requests are declined:
-    run those through try_build_gpu_group."""
+    request returns alone.  ORDER BY / LIMIT / OFFSET and COUNT(DISTINCT)
+    requests are declined: run those through try_build_gpu_group."""
 
     def __init__(self, session: GpuSession, pairs: Sequence[Tuple[Table, Request]]):
         self.session = session
@@ -501,6 +504,8 @@ class GpuBatchRun:
             for t, r in pairs:
                 if r.order_specs or r.limit or r.offset:
                     raise Declined("ORDER BY / LIMIT / OFFSET requests run alone")
+                if any(a.func.lower() == "count_distinct" for a in r.aggs):
+                    raise Declined("COUNT(DISTINCT) requests run alone")
                 plan, cols = build_plan(session, t, r)
                 self.items.append((t, r, plan, output_schema(t, r)))
                 plans.append(plan)

@@ -75,6 +75,7 @@ class AggOp(enum.IntEnum):
     AVG = 3
     MIN = 4
     MAX = 5
+    COUNT_DISTINCT = 6
 
 
 class PredOp(enum.IntEnum):
