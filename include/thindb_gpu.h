@@ -135,9 +135,11 @@ typedef struct tdbg_rg_ref {
   uint32_t pad;
 } tdbg_rg_ref;
 
-/* One GROUP BY key part.  Nullable key columns are not covered (the caller
- * declines), as in src/exec/v2_shape_group_topn.zig:2051-2094's packed path
- * restricted to non-null parts. */
+/* One GROUP BY key part (packKeysForPart, src/exec/v2_lowcard_group.zig:886-966;
+ * src/exec/v2_shape_group_topn.zig:2051-2094).  A key column staged with a
+ * validity bitmap is a nullable part: its NULL rows form their own group
+ * (apart from the value 0), emitted with a NULL key, ordered first ascending
+ * and last descending (compareBySpec, v2_lowcard_group.zig:2219-2224). */
 typedef struct tdbg_key_part {
   uint32_t column_idx;
   uint32_t type; /* enum tdbg_type, integer family only */

@@ -124,6 +124,8 @@ typedef struct {
   uint32_t col;
   uint32_t type;
   uint32_t offset, width;
+  int nullable;       /* the key column has a validity bitmap */
+  uint32_t valid_bit; /* set in the packed key for a non-NULL value */
 } key_part;
 
 enum { OP_COUNT_STAR, OP_COUNT_COL, OP_SUM, OP_AVG, OP_MIN, OP_MAX, OP_COUNT_DISTINCT };
@@ -311,9 +313,12 @@ static u128 pack_key(const cplan* p, uint64_t r) {
   u128 key = 0;
   for (uint32_t k = 0; k < p->n_parts; k++) {
     const key_part* kp = &p->parts[k];
-    uint64_t raw = (uint64_t)load_i(&p->t->cols[kp->col], r);
+    const or_column* c = &p->t->cols[kp->col];
+    if (kp->nullable && !is_valid(c, r)) continue; /* a NULL packs value 0, its validity bit clear */
+    uint64_t raw = (uint64_t)load_i(c, r);
     uint64_t m = kp->width >= 64 ? ~0ull : ((1ull << kp->width) - 1);
     key |= (u128)(raw & m) << kp->offset;
+    if (kp->nullable) key |= (u128)1 << kp->valid_bit;
   }
   return key;
 }
@@ -516,6 +521,8 @@ static int float_order(double a, double b) { /* types.floatOrder: NaN largest */
   return a < b ? -1 : (a > b ? 1 : 0);
 }
 
+static int part_is_null(u128 key, const key_part* kp) { return kp->nullable && !((key >> kp->valid_bit) & 1); }
+
 static int64_t unpack_part(u128 key, const key_part* kp) {
   uint64_t raw = (uint64_t)(key >> kp->offset);
   if (kp->width >= 64) return (int64_t)raw;
@@ -527,8 +534,13 @@ static int compare_spec(const tdbg_order_spec* sp, uint32_t a, uint32_t b) {
   const cplan* p = g_sort->p;
   const wstate* m = g_sort->m;
   if (sp->target < p->n_parts) {
-    int64_t x = unpack_part(m->keys[a], &p->parts[sp->target]);
-    int64_t y = unpack_part(m->keys[b], &p->parts[sp->target]);
+    const key_part* kp = &p->parts[sp->target];
+    /* NULL orders first (the caller flips it last for DESC) */
+    int an = part_is_null(m->keys[a], kp), bn = part_is_null(m->keys[b], kp);
+    if (an != bn) return an ? -1 : 1;
+    if (an) return 0;
+    int64_t x = unpack_part(m->keys[a], kp);
+    int64_t y = unpack_part(m->keys[b], kp);
     return x < y ? -1 : (x > y ? 1 : 0);
   }
   uint32_t i = sp->target - p->n_parts;
@@ -576,7 +588,8 @@ typedef struct {
 static hnum output_num(const cplan* p, const wstate* m, uint32_t target, uint32_t gid) {
   hnum v = {0, 0.0, 0, 0};
   if (target < p->n_parts) {
-    v.i = unpack_part(m->keys[gid], &p->parts[target]);
+    if (part_is_null(m->keys[gid], &p->parts[target])) v.null = 1; /* keyPartIsNull */
+    else v.i = unpack_part(m->keys[gid], &p->parts[target]);
     return v;
   }
   uint32_t a = target - p->n_parts;
@@ -684,9 +697,18 @@ static int emit(const cplan* p, const wstate* m, or_result** out) {
     tdbg_column_view* c = &r->cols[k];
     c->type = p->parts[k].type;
     c->n_rows = nr;
+    c->nullable = p->parts[k].nullable;
     void* data = calloc(nr + 1, type_bytes(c->type));
-    for (uint64_t i = 0; i < nr; i++) put_int(data, c->type, i, unpack_part(m->keys[order[start + i]], &p->parts[k]));
+    uint8_t* valid = calloc(nr / 8 + 1, 1);
+    int any_null = 0;
+    for (uint64_t i = 0; i < nr; i++) {
+      u128 key = m->keys[order[start + i]];
+      if (part_is_null(key, &p->parts[k])) { any_null = 1; continue; }
+      put_int(data, c->type, i, unpack_part(key, &p->parts[k]));
+      valid[i >> 3] |= (uint8_t)(1u << (i & 7));
+    }
     c->data = data;
+    if (any_null) c->validity = valid; else free(valid);
   }
   for (uint32_t a = 0; a < p->n_aggs; a++) {
     const agg_plan* ag = &p->aggs[a];
@@ -842,10 +864,13 @@ static int compile(cplan* p, const tdbg_plan_desc* d, const or_table* t) {
     if (t->cols[kp->column_idx].type != kp->type) { set_err("key type"); return TDBG_ERR_INVALID; }
     int w = key_type_bits(kp->type);
     if (!w) { set_err("key type not integer"); return TDBG_ERR_UNSUPPORTED; }
-    if (t->cols[kp->column_idx].validity) { set_err("nullable key"); return TDBG_ERR_UNSUPPORTED; }
-    p->parts[k] = (key_part){kp->column_idx, kp->type, bits, (uint32_t)w};
+    p->parts[k] = (key_part){kp->column_idx, kp->type, bits, (uint32_t)w, t->cols[kp->column_idx].validity != NULL, 0};
     bits += (uint32_t)w;
   }
+  /* a nullable part's validity bit sits above every value bit (packValidity,
+   * v2_lowcard_group.zig:954-966; tryBuild :245-272) */
+  for (uint32_t k = 0; k < d->n_keys; k++)
+    if (p->parts[k].nullable) p->parts[k].valid_bit = bits++;
   if (bits > 128) { set_err("key wider than 128 bits"); return TDBG_ERR_UNSUPPORTED; }
   p->n_parts = d->n_keys;
   p->key_bits = bits;
@@ -921,7 +946,7 @@ static int compile(cplan* p, const tdbg_plan_desc* d, const or_table* t) {
   if (raw > 32768) raw = 32768;
   p->batch_rows = (uint32_t)(raw & ~63ull);
   /* directRangeOf (:509-530) over the key column's min/max */
-  if (d->n_keys == 1 && t->n_rows > 0) {
+  if (d->n_keys == 1 && t->n_rows > 0 && !p->parts[0].nullable) { /* a nullable part has no direct range (:510) */
     const or_column* c = &t->cols[d->keys[0].column_idx];
     int64_t mn = load_i(c, 0), mx = mn;
     for (uint64_t r = 1; r < t->n_rows; r++) {

@@ -131,7 +131,8 @@ def brute_force(table: Table, request: Request) -> Dict[tuple, list]:
     for r in range(table.n_rows):
         if not _row_passes(table, leaves, r):
             continue
-        key = tuple(int(table.columns[table.column(k).name][r]) for k in request.group_cols)
+        key = tuple(None if (table.column(k).name in table.validity and not table.validity[table.column(k).name][r])
+                    else int(table.columns[table.column(k).name][r]) for k in request.group_cols)
         st = groups.setdefault(key, [[0, []] for _ in request.aggs])
         for a, s in zip(request.aggs, st):
             s[0] += 1  # rows
@@ -236,9 +237,12 @@ def assert_same_rows(got: List[tuple], want: List[tuple], float_cols: Sequence[i
     def eq_row(a, b):
         return all(_close(x, y, rel) if i in float_cols else x == y for i, (x, y) in enumerate(zip(a, b)))
 
+    def key_of(r):  # NULL keys sort first
+        return tuple((x is not None, x if x is not None else 0) for x in r[:n_keys])
+
     if order_cols is None:
-        ga = sorted(got, key=lambda r: r[:n_keys])
-        wa = sorted(want, key=lambda r: r[:n_keys])
+        ga = sorted(got, key=key_of)
+        wa = sorted(want, key=key_of)
         for a, b in zip(ga, wa):
             assert eq_row(a, b), f"{a} != {b}"
         return

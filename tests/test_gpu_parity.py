@@ -380,7 +380,6 @@ def test_empty_and_ragged(session, oracle_lib):
 def test_declines(session):
     rng = np.random.default_rng(0)
     t = random_table(rng, 1000)
-    assert try_build_gpu_group(session, t, Request(["b"], [AggSpec("count", None, "c")])) is None
     assert try_build_gpu_group(session, t, Request(["d"], [AggSpec("count", None, "c")])) is None
     assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("sum", "t", "x")])) is None
     # HAVING over a column that is not an output

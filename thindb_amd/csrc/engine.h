@@ -54,9 +54,12 @@ struct PredDesc {
 
 // Key part: relative value rel = v - base, 0 <= rel < span, folded into a
 // mixed-radix group index key = sum(rel_i * stride_i).
+// A key part's relative value rel = v - base in [0, span); the key is the
+// mixed-radix sum of rel * stride.  A nullable part keeps rel 0 for NULL and
+// its base one below the smallest value, so NULL sorts first.
 struct PartDesc {
   uint32_t col;
-  uint32_t pad;
+  uint32_t nullable;
   int64_t base;
   uint64_t span;
   uint64_t stride;

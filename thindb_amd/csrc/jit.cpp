@@ -87,7 +87,7 @@ std::string jit_signature(const ScanParams& p) {
   s << "p";
   for (uint32_t i = 0; i < p.n_preds; i++) s << p.preds[i].kind << ":" << p.preds[i].col << ",";
   s << "k";
-  for (uint32_t i = 0; i < p.n_parts; i++) s << p.parts[i].col << ",";
+  for (uint32_t i = 0; i < p.n_parts; i++) s << p.parts[i].col << (p.parts[i].nullable ? "n" : "") << ",";
   s << "a";
   for (uint32_t i = 0; i < p.n_aggs; i++)
     s << p.aggs[i].kind << ":" << p.aggs[i].col << (p.aggs[i].nullable ? "n" : "") << (p.aggs[i].split ? "s" : "")
@@ -131,7 +131,10 @@ std::string jit_source(const ScanParams& p, bool prefetch) {
   for (uint32_t i = 0; i < p.n_parts; i++) {
     const PartDesc& d = p.parts[i];
     s << "    { int64_t v[kVec]; dec_i<" << kind_name(p.cols[d.col].kind) << ">(x" << d.col << ", v); key_part<"
-      << (i == 0 ? "true" : "false") << ">(P, P.parts[" << i << "], v, key, mask); }\n";
+      << (i == 0 ? "true" : "false") << ">(P, P.parts[" << i << "], v, key, mask, ";
+    if (d.nullable) s << "valid4(P.cols[" << d.col << "], r)";
+    else s << "0xfu";
+    s << "); }\n";
   }
   s << "    if (!mask) return;\n"
        "    int slot[kVec];\n"

@@ -1178,9 +1178,16 @@ __device__ __forceinline__ void finalize_row(const FinalParams& P, uint32_t gid,
       const OutDesc& o = P.outs[c];
       uint8_t valid = 1;
       switch (o.kind) {
-        case O_KEY:
-          put(out, o.data_off, o.bytes, i, (uint64_t)part_value(P.parts[o.idx], group_key(rec, gid, P.g_mode, P.g_key_off)));
+        case O_KEY: {
+          const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+          if (part_null(P.parts[o.idx], k)) {
+            valid = 0;
+            put(out, o.data_off, o.bytes, i, 0);
+          } else {
+            put(out, o.data_off, o.bytes, i, (uint64_t)part_value(P.parts[o.idx], k));
+          }
           break;
+        }
         case O_COUNT: put(out, o.data_off, 8, i, count); break;
         case O_NS: put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
         default: {

@@ -59,9 +59,15 @@ __device__ __forceinline__ uint64_t group_key(const uint8_t* rec, uint32_t gid, 
   return g_mode == G_DENSE ? (uint64_t)gid : *reinterpret_cast<const uint64_t*>(rec + key_off);
 }
 
+__device__ __forceinline__ uint64_t part_rel(const PartDesc& pt, uint64_t key) { return (key / pt.stride) % pt.span; }
+
 __device__ __forceinline__ int64_t part_value(const PartDesc& pt, uint64_t key) {
-  const uint64_t rel = (key / pt.stride) % pt.span;
-  return (int64_t)((uint64_t)pt.base + rel);
+  return (int64_t)((uint64_t)pt.base + part_rel(pt, key));
+}
+
+// A nullable part's NULL is relative value 0.
+__device__ __forceinline__ bool part_null(const PartDesc& pt, uint64_t key) {
+  return pt.nullable && part_rel(pt, key) == 0;
 }
 
 __device__ __forceinline__ uint64_t rec_ns(const uint8_t* rec, uint32_t ns_off, uint32_t count_off) {
@@ -73,7 +79,14 @@ __device__ __forceinline__ uint64_t comp_word(const OrderParams& P, const CompDe
   const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
   uint64_t w = 0;
   switch (c.src) {
-    case C_KEY: w = ord_i64(part_value(P.parts[c.idx], group_key(rec, gid, P.g_mode, P.g_key_off))); break;
+    case C_KEY: {
+      // a nullable part orders by its relative value: NULL (0) first
+      // ascending, last descending (compareBySpec, v2_lowcard_group.zig:2219-2224)
+      const PartDesc& pt = P.parts[c.idx];
+      const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+      w = pt.nullable ? part_rel(pt, k) : ord_i64(part_value(pt, k));
+      break;
+    }
     case C_COUNT: w = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off); break;
     case C_U64_AT: w = *reinterpret_cast<const uint64_t*>(rec + c.off); break;
     case C_I64_AT:
@@ -126,7 +139,12 @@ __device__ __forceinline__ HNum having_value(const FinalParams& P, const OutDesc
   HNum v{0, 0.0, false, false};
   const uint64_t count = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off);
   switch (o.kind) {
-    case O_KEY: v.i = part_value(P.parts[o.idx], group_key(rec, gid, P.g_mode, P.g_key_off)); return v;
+    case O_KEY: {
+      const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+      if (part_null(P.parts[o.idx], k)) v.null = true;  // keyPartIsNull (:1475-1479)
+      else v.i = part_value(P.parts[o.idx], k);
+      return v;
+    }
     case O_COUNT: v.i = (__int128)count; return v;
     case O_NS: v.i = (__int128)*reinterpret_cast<const uint64_t*>(rec + o.off); return v;
     default: break;

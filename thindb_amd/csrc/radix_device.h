@@ -26,8 +26,9 @@ __device__ __forceinline__ uint32_t rx_strip_keys(const ScanParams& P, uint64_t 
     const PartDesc& pt = P.parts[i];
     int64_t v[kVec];
     decode_i(load_raw(P.cols[pt.col], r), P.cols[pt.col].kind, v);
-    if (i == 0) key_part<true>(P, pt, v, key, mask);
-    else key_part<false>(P, pt, v, key, mask);
+    const uint32_t pv = pt.nullable ? valid4(P.cols[pt.col], r) : 0xfu;
+    if (i == 0) key_part<true>(P, pt, v, key, mask, pv);
+    else key_part<false>(P, pt, v, key, mask, pv);
   }
   return mask;
 }

@@ -488,12 +488,19 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     if (it == t->cols.end()) return fail(TDBG_ERR_NOT_STAGED, "key column %u not staged", pl->keys[i].column_idx);
     const ColumnArena& a = it->second;
     if (a.type != pl->keys[i].type) return fail(TDBG_ERR_INVALID, "key column %u type mismatch", pl->keys[i].column_idx);
-    if (a.d_valid) return fail(TDBG_ERR_UNSUPPORTED, "nullable group key");
     PartDesc d{};
     d.col = col_slot(pl->keys[i].column_idx);
     d.base = a.has_stats ? a.min_i : 0;
     d.span = a.has_stats ? (uint64_t)a.max_i - (uint64_t)a.min_i + 1 : 1;
     if (d.span == 0) return fail(TDBG_ERR_UNSUPPORTED, "key domain of 2^64 values");
+    if (a.d_valid) {
+      // NULL takes relative value 0, the values 1.. (packValidity: the NULL
+      // group stays apart from every value, v2_lowcard_group.zig:44-49)
+      d.nullable = 1;
+      d.base = (int64_t)((uint64_t)d.base - 1);
+      d.span = a.has_stats ? d.span + 1 : 1;
+      if (d.span == 0) return fail(TDBG_ERR_UNSUPPORTED, "key domain of 2^64 values");
+    }
     d.stride = (uint64_t)domain;
     domain *= d.span;
     if (domain >= ((unsigned __int128)1 << 62)) return fail(TDBG_ERR_UNSUPPORTED, "key domain wider than 62 bits");
@@ -830,7 +837,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
   for (uint32_t i = 0; i < sp.n_parts; i++) {
     OutDesc o{};
     o.kind = O_KEY; o.idx = i; o.bytes = (uint32_t)type_bytes(pl->keys[i].type);
-    o.valid_off = ~0ull; o.ns_off = ~0u; o.scale_div = 1.0;
+    o.valid_off = sp.parts[i].nullable ? 0 : ~0ull; o.ns_off = ~0u; o.scale_div = 1.0;
     fp.outs[fp.n_outs++] = o;
   }
   for (auto& ao : pl->outs) {
@@ -1276,6 +1283,7 @@ int32_t tdbg::rt::make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host
     uint32_t type, scale = 0, nullable = 0;
     if (c < nk) {
       type = pl->keys[c].type;
+      nullable = pl->prep.sp.parts[c].nullable;
     } else {
       const AggOut& o = pl->outs[c - nk];
       type = o.out_type;

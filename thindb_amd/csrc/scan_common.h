@@ -417,12 +417,14 @@ __device__ __forceinline__ void lds_flush(const ScanParams& P, uint8_t* lds, uin
 
 // Folds key part `pt` (values `v`) into the strip's relative keys; a value
 // outside the staged statistics raises ERR_KEY_RANGE and drops the row.
+// `valid`: the part column's validity bits of the strip (0xf for a
+// non-nullable part); a NULL value takes relative value 0.
 template <bool FIRST>
 __device__ __forceinline__ void key_part(const ScanParams& P, const PartDesc& pt, const int64_t v[kVec],
-                                         uint64_t key[kVec], uint32_t& mask) {
+                                         uint64_t key[kVec], uint32_t& mask, uint32_t valid) {
 #pragma unroll
   for (int j = 0; j < kVec; j++) {
-    const uint64_t rel = (uint64_t)v[j] - (uint64_t)pt.base;
+    const uint64_t rel = ((valid >> j) & 1u) ? (uint64_t)v[j] - (uint64_t)pt.base : 0ull;
     if (rel >= pt.span && ((mask >> j) & 1u)) {
       atomicOr(P.err, ERR_KEY_RANGE);
       mask &= ~(1u << j);

@@ -63,8 +63,9 @@ struct GenericStrip {
       const PartDesc& pt = P.parts[i];
       int64_t v[kVec];
       decode_i(fetch<FAST>(P, raw, pt.col, r), P.cols[pt.col].kind, v);
-      if (i == 0) key_part<true>(P, pt, v, key, mask);
-      else key_part<false>(P, pt, v, key, mask);
+      const uint32_t pv = pt.nullable ? valid4(P.cols[pt.col], r) : 0xfu;
+      if (i == 0) key_part<true>(P, pt, v, key, mask, pv);
+      else key_part<false>(P, pt, v, key, mask, pv);
     }
     if (!mask) return;
     int slot[kVec];

@@ -316,7 +316,7 @@ def _having_postfix(expr, schema: List[Column], out: list) -> None:
 def output_schema(table: Table, request: Request) -> List[Column]:
     """Keys then aggregates (v2_lowcard_group.zig:1674-1684; output types of
     src/exec/aggregate.zig:3700-3735)."""
-    out = [Column(table.column(k).name, table.column(k).type, False, table.column(k).scale)
+    out = [Column(table.column(k).name, table.column(k).type, table.column(k).nullable, table.column(k).scale)
            for k in request.group_cols]
     for a in request.aggs:
         f = a.func.lower()
@@ -430,8 +430,6 @@ def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu
         c = table.column(k)
         if c.type not in _INT_TYPES:
             raise Declined(f"group key {k} is not integer-typed")
-        if c.nullable:
-            raise Declined(f"group key {k} is nullable")
         keys.append((table.column_index(k), c.type))
         cols.append(c.name)
     aggs = []
