@@ -54,6 +54,8 @@ PlanHint batch_hint(PlanHint h) {
   h.max_bits1 = 10;
   h.single_level = true;
   h.dense_out = true;
+  const char* env = getenv("THINDB_GPU_BATCH_WIDE_LEAVES");
+  h.wide_leaves = !(env && env[0] == '0');
   return h;
 }
 

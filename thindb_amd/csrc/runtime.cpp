@@ -677,7 +677,9 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     while (fit_bits < L && fit_bits < lb_max && ((uint64_t)sb << (fit_bits + 1)) <= leaf_budget) fit_bits++;
     // at least 2^10 leaves where the domain allows (several per workgroup of
     // the leaf pass), each of at least 64 keys
-    const uint32_t leaf_bits = std::min<uint32_t>(fit_bits, std::max<uint32_t>(6, L > 10 ? L - 10 : 0));
+    // (a batch runs many plans at once and takes the widest leaf instead)
+    const uint32_t leaf_bits =
+        hint.wide_leaves ? fit_bits : std::min<uint32_t>(fit_bits, std::max<uint32_t>(6, L > 10 ? L - 10 : 0));
     RadixParams& R = P.rx;
     R = RadixParams{};
     R.row.flags_off = ~0u;

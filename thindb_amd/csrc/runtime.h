@@ -107,6 +107,7 @@ struct PlanHint {
   uint32_t max_bits1 = 8;       // S_RADIX: widest level-1 digit
   bool single_level = false;    // S_RADIX: one scatter level only
   bool dense_out = false;       // S_RADIX: leaves write a dense record array
+  bool wide_leaves = false;     // S_RADIX: the widest leaf that fits (parallelism comes from elsewhere)
 };
 
 struct Prepared {
