@@ -46,6 +46,9 @@ def test_count_distinct_radix(session, oracle_lib, monkeypatch, est):
     assert st.global_mode == 3
     check_vs_oracle(session, oracle_lib, t, Request(["user_id", "bucket"], aggs, [SortSpec("dv", True)], 10,
                                                     est_groups=est, having_filter=Predicate("c", "gte", 2)), dop=8)
+    # HAVING over the distinct count itself (filled after compaction)
+    check_vs_oracle(session, oracle_lib, t, Request(["user_id", "bucket"], aggs, est_groups=est,
+                                                    having_filter=Predicate("dv", "gte", 2)), dop=8)
 
 
 def test_count_distinct_highcard_values(session, oracle_lib):

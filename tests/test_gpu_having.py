@@ -61,6 +61,22 @@ def test_having_radix(session, oracle_lib, monkeypatch, est):
                                                     est_groups=est, having_filter=h), dop=8)
 
 
+def test_having_in_radix_leaves(session, oracle_lib, monkeypatch):
+    """Every HAVING tree evaluated by the radix leaf pass on its LDS table
+    (compacted leaf output: a small est_groups), with and without ORDER BY."""
+    t = _table()
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    for having in _cases(t):
+        print(f"[having/leaf] {having}", flush=True)
+        _, _, st = check_vs_oracle(session, oracle_lib, t, Request(["k1", "k2"], ALL_AGGS, est_groups=10,
+                                                                   having_filter=having))
+        assert st.global_mode == 3
+        # COUNT ties broken by the keys, so the OFFSET cuts a total order
+        check_vs_oracle(session, oracle_lib, t, Request(["k2", "k1"], ALL_AGGS,
+                                                        [SortSpec("c", True), SortSpec("k2"), SortSpec("k1")], 7, 1,
+                                                        est_groups=10, having_filter=having))
+
+
 def test_having_filtered_workload(session, oracle_lib):
     """The filtered scan workload (WHERE on ts) with HAVING on the DECIMAL
     SUM and the AVG, ordered by the SUM."""

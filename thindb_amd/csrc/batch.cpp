@@ -212,6 +212,7 @@ int32_t build(tdbg_batch* b) {
     memset(&q, 0, sizeof q);
     q.sp = P.sp;
     q.rx = P.rx;
+    q.rx.having = nullptr;  // batched leaves write dense records; HAVING joins the compaction
     q.op = P.op;
     q.fp = P.fp;
     q.n_having = (uint32_t)pl->having.size();

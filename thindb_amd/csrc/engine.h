@@ -144,6 +144,8 @@ struct RowLayout {
   uint32_t val_bytes[kMaxAggs];
 };
 
+struct HavingLeaf;
+
 struct RadixParams {
   RowLayout row;
   uint32_t shift1, bits1;      // level-1 digit = key >> shift1 (bits1 wide)
@@ -165,6 +167,7 @@ struct RadixParams {
   uint64_t* leaf_start;        // [n_leaves + 1] leaf row ranges in the final array
   uint32_t* leaf_groups;       // [n_leaves] groups each leaf wrote (compacted records from its first row),
                                // then scratch for their positions: [1 + n_leaves / 256 + 1] words
+  const HavingLeaf* having;    // compacted leaves: only groups passing this HAVING are written (null: all)
 };
 
 // ---- ordering / emission -------------------------------------------------
@@ -250,6 +253,22 @@ struct HavingNode {
   uint32_t pad;
   int64_t lit_lo, lit_hi;
   double lit_f64;
+};
+
+// HAVING evaluated by the radix leaf pass on its LDS table before records
+// are written: where each output column's value lives there.
+struct HavingSrc {
+  uint32_t kind;     // OutKind of the output column
+  uint32_t agg;      // ScanParams aggregate holding its state (aggregate outputs)
+  uint32_t part;     // key part (O_KEY)
+  uint32_t pad;
+  double scale_div;  // 10^scale for AVG of DECIMAL
+};
+struct HavingLeaf {
+  uint32_t n;
+  uint32_t pad;
+  HavingNode nodes[kMaxHaving];
+  HavingSrc src[kMaxParts + kMaxAggs];
 };
 
 // COUNT(DISTINCT x) of one plan: every filtered row with a non-NULL x adds

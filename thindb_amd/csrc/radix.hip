@@ -232,6 +232,9 @@ __global__ void __launch_bounds__(kRxThreads) rx_scatter2_kernel(const RadixPara
 
 constexpr uint32_t kRxAggTmp = 512;  // LDS words for compaction counts: 9 + 2^leaf_bits / 64
 
+// HAVING: compacted leaves write only the groups passing R.having (a
+// separate instantiation, so the plain leaf pass keeps its registers).
+template <bool HAVING>
 __global__ void __launch_bounds__(512) rx_agg_kernel(const ScanParams P, const RadixParams R) {
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t tmp[kRxAggTmp];
@@ -241,7 +244,8 @@ __global__ void __launch_bounds__(512) rx_agg_kernel(const ScanParams P, const R
     __syncthreads();
   }
   for (uint32_t leaf = blockIdx.x; leaf < R.n_leaves; leaf += gridDim.x) {
-    if (R.dense_out) rx_agg_leaf<true>(P, R, leaf, lds, tmp);
+    if (HAVING) rx_agg_leaf<false, true>(P, R, leaf, lds, tmp, chist);
+    else if (R.dense_out) rx_agg_leaf<true>(P, R, leaf, lds, tmp);
     else rx_agg_leaf<false>(P, R, leaf, lds, tmp, chist);
   }
   if (R.counts_out) {
@@ -371,7 +375,10 @@ hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid
       e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_scatter2_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e == hipSuccess)  // rx_agg also holds ~2 KiB of static LDS
-      e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_agg_kernel),
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_agg_kernel<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 4096);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_agg_kernel<true>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 4096);
     return e;
   }();
@@ -400,7 +407,10 @@ hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid
     }
     case RX_AGG:
       if (r.counts_out && (r.dense_out || !r.count_hist)) return hipErrorInvalidValue;
-      hipLaunchKernelGGL(rx_agg_kernel, dim3(grid), dim3(512), p.lds_bytes, s, p, r);
+      if (r.having && !r.dense_out)
+        hipLaunchKernelGGL(rx_agg_kernel<true>, dim3(grid), dim3(512), p.lds_bytes, s, p, r);
+      else
+        hipLaunchKernelGGL(rx_agg_kernel<false>, dim3(grid), dim3(512), p.lds_bytes, s, p, r);
       break;
     default: return hipErrorInvalidValue;
   }

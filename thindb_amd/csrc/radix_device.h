@@ -3,6 +3,7 @@
 // kernel (batch.hip), which runs them on work items of many plans.
 #pragma once
 
+#include "order_device.h"
 #include "scan_common.h"
 
 namespace tdbg {
@@ -419,6 +420,103 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
   }
 }
 
+// Output column c of LDS slot s (relative key `key`) as havingOutputNum
+// reads it (src/exec/v2_shape_group_topn.zig:2461-2500): the same values
+// rx_write_record would store and having_value would read back.
+__device__ __forceinline__ HNum rx_having_value(const ScanParams& P, const HavingLeaf& L, uint32_t c, uint8_t* lds,
+                                                uint32_t s, uint64_t key) {
+  const HavingSrc& h = L.src[c];
+  HNum v{0, 0.0, false, false};
+  const uint32_t count = lds_at<uint32_t>(lds, P.lds_count_off)[s];
+  switch (h.kind) {
+    case O_KEY: {
+      const PartDesc& pt = P.parts[h.part];
+      if (part_null(pt, key)) v.null = true;
+      else v.i = part_value(pt, key);
+      return v;
+    }
+    case O_COUNT: v.i = count; return v;
+    case O_NS: v.i = lds_at<uint32_t>(lds, P.aggs[h.agg].lds_off)[s]; return v;
+    default: break;
+  }
+  const AggDesc& d = P.aggs[h.agg];
+  const uint64_t ns = d.nullable ? lds_at<uint32_t>(lds, d.lds_ns_off)[s] : count;
+  if (ns == 0) {
+    v.null = true;
+    return v;
+  }
+  const uint64_t w = lds_at<uint64_t>(lds, d.lds_off)[s];
+  switch (h.kind) {
+    case O_SUM_I64:
+    case O_SUM_I128:
+    case O_AVG_I: {
+      __int128 x;
+      if (d.split) {
+        const int64_t hs = (int64_t)lds_at<uint64_t>(lds, d.lds_off)[P.lds_cap + s];
+        x = (__int128)hs * ((__int128)1 << 32) + (__int128)w;
+      } else {
+        x = (__int128)(int64_t)w;
+      }
+      if (h.kind == O_SUM_I64) v.i = (int64_t)(uint64_t)x;  // the wrapped BIGINT
+      else if (h.kind == O_SUM_I128) v.i = x;
+      else { v.is_f = true; v.f = i128_to_f64(x) / (double)ns / h.scale_div; }
+      break;
+    }
+    case O_SUM_F64: v.is_f = true; v.f = __longlong_as_double((long long)w); break;
+    case O_AVG_F: v.is_f = true; v.f = __longlong_as_double((long long)w) / (double)ns; break;
+    case O_MINMAX_I: v.i = (int64_t)w; break;
+    default: v.is_f = true; v.f = unord_f64((int64_t)w); break;  // O_MINMAX_F
+  }
+  return v;
+}
+
+// havingPasses (:2502-2535) over LDS slot s.
+__device__ __forceinline__ bool rx_having_pass(const ScanParams& P, const HavingLeaf& L, uint8_t* lds, uint32_t s,
+                                               uint64_t key) {
+  uint64_t st = 0;
+  for (uint32_t k = 0; k < L.n; k++) {
+    const HavingNode& h = L.nodes[k];
+    bool r;
+    switch (h.kind) {
+      case H_CMP: {
+        const HNum a = rx_having_value(P, L, h.target, lds, s, key);
+        if (a.null) {
+          r = false;
+          break;
+        }
+        const int o = having_order(a, h);
+        switch (h.op) {
+          case 0: r = o == 0; break;
+          case 1: r = o != 0; break;
+          case 2: r = o < 0; break;
+          case 3: r = o <= 0; break;
+          case 4: r = o > 0; break;
+          default: r = o >= 0; break;
+        }
+        break;
+      }
+      case H_IS_NULL:
+      case H_IS_NOT_NULL: {
+        const bool nul = rx_having_value(P, L, h.target, lds, s, key).null;
+        r = h.kind == H_IS_NULL ? nul : !nul;
+        break;
+      }
+      case H_NOT:
+        st ^= 1ull;
+        continue;
+      default: {  // H_AND / H_OR
+        const uint64_t m = h.arity >= 64 ? ~0ull : ((1ull << h.arity) - 1);
+        const uint64_t top = st & m;
+        st = h.arity >= 64 ? 0 : (st >> h.arity);
+        r = h.kind == H_AND ? top == m : top != 0;
+        break;
+      }
+    }
+    st = (st << 1) | (r ? 1ull : 0ull);
+  }
+  return (st & 1ull) != 0;
+}
+
 // Aggregates leaf `leaf` in an LDS direct table and writes its groups: with
 // DENSE_OUT every key of the leaf's range below g_cap at record index key
 // (empty keys get a zero count), else the occupied slots in slot order to
@@ -429,8 +527,9 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
 //
 // With R.counts_out (compacted only) each written group's row count also
 // goes to R.counts_out[record] and into chist (kRxCountBins LDS counters of
-// min(count, kRxCountBins - 1)).
-template <bool DENSE_OUT>
+// min(count, kRxCountBins - 1)).  With R.having (compacted only) a group is
+// written, counted and histogrammed only when it passes that HAVING.
+template <bool DENSE_OUT, bool HAVING = false>
 __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint8_t* lds,
                                             uint32_t* tmp, uint32_t* chist = nullptr) {
   const uint32_t cap = P.lds_cap, stride = R.row.stride;
@@ -478,9 +577,16 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
   const uint32_t rounds = (cap + blockDim.x - 1) / blockDim.x;
   uint32_t* scan_tmp = tmp;
   uint32_t* wcnt = tmp + nw + 1;  // rounds * nw entries
+  // with HAVING, a group is written only when it passes (evaluated on the
+  // LDS table, in both rounds below)
+  const HavingLeaf* H = HAVING ? R.having : nullptr;
+  uint64_t keep = 0;  // each round's outcome (rounds beyond 64 evaluate again below)
   for (uint32_t k = 0; k < rounds; k++) {
     const uint32_t slot = k * blockDim.x + threadIdx.x;
-    const uint64_t m = __ballot(slot < cap && count[slot] != 0);
+    bool occ = slot < cap && count[slot] != 0;
+    if (H && occ) occ = rx_having_pass(P, *H, lds, slot, leaf_key0 + slot);
+    if (k < 64 && occ) keep |= 1ull << k;
+    const uint64_t m = __ballot(occ);
     if ((threadIdx.x & 63) == 0) wcnt[k * nw + wave] = (uint32_t)__popcll(m);
   }
   __syncthreads();
@@ -497,7 +603,13 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t k = 0; k < rounds; k++) {
     const uint32_t slot = k * blockDim.x + threadIdx.x;
-    const bool occ = slot < cap && count[slot] != 0;
+    bool occ;
+    if (k < 64) {
+      occ = (keep >> k) & 1ull;
+    } else {
+      occ = slot < cap && count[slot] != 0;
+      if (H && occ) occ = rx_having_pass(P, *H, lds, slot, leaf_key0 + slot);
+    }
     const uint64_t m = __ballot(occ);
     const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
     if (occ) rx_write_record(P, lds, slot, leaf_key0 + slot, P.g_state + at * P.g_rec);

@@ -875,6 +875,29 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     }
     fp.outs[fp.n_outs++] = o;
   }
+  // HAVING in the leaf pass: every output's value must be in the leaf's LDS
+  // table (a COUNT(DISTINCT) count is not: it is filled after compaction)
+  if (!pl->having.empty() && P.radix && !P.rx.dense_out) {
+    bool ok = true;
+    HavingLeaf& L = P.hleaf;
+    L.n = (uint32_t)pl->having.size();
+    for (size_t k = 0; k < pl->having.size(); k++) memcpy(&L.nodes[k], &pl->having[k], sizeof(HavingNode));
+    for (uint32_t c = 0; c < fp.n_outs && ok; c++) {
+      const OutDesc& o = fp.outs[c];
+      HavingSrc& hs = L.src[c];
+      hs = HavingSrc{};
+      hs.kind = o.kind;
+      hs.part = o.idx;
+      hs.scale_div = o.scale_div;
+      if (o.kind == O_KEY || o.kind == O_COUNT) continue;
+      if (c >= sp.n_parts && pl->outs[c - sp.n_parts].dslot >= 0) { ok = false; break; }
+      hs.agg = ~0u;
+      for (uint32_t k = 0; k < sp.n_aggs; k++)
+        if (sp.aggs[k].g_off == o.off) hs.agg = k;
+      if (hs.agg == ~0u) ok = false;
+    }
+    P.leaf_having = ok;
+  }
   P.version = t->version;
   P.rows_hint = n_rows_scan;
   return TDBG_OK;
@@ -1427,11 +1450,18 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   // buffer, so the copies are asynchronous (the previous execution's copies
   // completed at its final synchronisation)
   const uint64_t tiles_b = tiles.size() * sizeof(Tile), tmpl_b = P.tmpl.size() * 8;
-  if ((rc = ensure_pinned(pl->h_in, pl->h_in_cap, align_up(tiles_b, 256) + tmpl_b))) return rc;
+  const uint64_t hl_at = align_up(align_up(tiles_b, 256) + tmpl_b, 256);
+  const uint64_t hl_b = P.leaf_having ? sizeof(HavingLeaf) : 0;
+  if ((rc = ensure_pinned(pl->h_in, pl->h_in_cap, hl_at + hl_b))) return rc;
   memcpy(pl->h_in, tiles.data(), tiles_b);
   memcpy(pl->h_in + align_up(tiles_b, 256), P.tmpl.data(), tmpl_b);
   HIP_TRY(hipMemcpyAsync(pl->tiles.p, pl->h_in, tiles_b, hipMemcpyHostToDevice, s), "tiles");
   HIP_TRY(hipMemcpyAsync(pl->tmpl.p, pl->h_in + align_up(tiles_b, 256), tmpl_b, hipMemcpyHostToDevice, s), "template");
+  if (hl_b) {
+    if ((rc = ensure(pl, pl->having_leaf, hl_b))) return rc;
+    memcpy(pl->h_in + hl_at, &P.hleaf, hl_b);
+    HIP_TRY(hipMemcpyAsync(pl->having_leaf.p, pl->h_in + hl_at, hl_b, hipMemcpyHostToDevice, s), "HAVING");
+  }
   bool specialized = false;
   const uint64_t n_rec = sp.g_cap;
   if ((rc = ensure(pl, pl->gids, n_rec * 4 + 256))) return rc;
@@ -1464,9 +1494,10 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     // native COUNT top-N, v2_shape_group_topn.zig:846-870): the leaves also
     // write each group's count and a histogram of them, which fixes the cut
     // without another pass over the groups
+    R.having = P.leaf_having ? static_cast<const HavingLeaf*>(pl->having_leaf.p) : nullptr;
     R.counts_out = R.count_hist = nullptr;
     if (!R.dense_out && P.op.n_comps > 0 && P.op.comps[0].src == C_COUNT && need_lim <= kSortCandidatesMax &&
-        rows > 65536 && pl->having.empty()) {
+        rows > 65536 && (pl->having.empty() || P.leaf_having)) {
       if ((rc = ensure(pl, pl->rx_words, rows * 4 + 256))) return rc;
       R.counts_out = static_cast<uint32_t*>(pl->rx_words.p);
       R.count_hist = reinterpret_cast<uint32_t*>(R.bucket_fill + nb1 + lg_words);
@@ -1512,7 +1543,7 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   }
   // HAVING: the passing groups' positions, in order, to having_gids and their
   // number to nout[4]
-  const bool having = !pl->having.empty();
+  const bool having = !pl->having.empty() && !P.leaf_having;  // else the leaf pass applied it
   if (having) {
     // survivors' gids, then the pass masks and per-workgroup counts
     const uint64_t out_b = align_up(n_rec * 4 + 256, 256);

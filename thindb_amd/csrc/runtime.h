@@ -130,6 +130,10 @@ struct Prepared {
     uint32_t dslot_off;
   };
   std::vector<Distinct> distinct;
+  // HAVING evaluated in the radix leaf pass (compacted leaves, every target
+  // held in the leaf's LDS table): only passing groups are written
+  bool leaf_having = false;
+  HavingLeaf hleaf{};
 };
 
 struct MergedPred {
@@ -212,7 +216,7 @@ struct tdbg_plan {
     void* p = nullptr;
     uint64_t bytes = 0;
   } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words,
-      having_nodes, having_gids, dist_set, dist_counts;
+      having_nodes, having_gids, dist_set, dist_counts, having_leaf;
   // pinned host staging: inputs (tiles, template) and the result the
   // columns of a returned tdbg_result view
   uint8_t* h_in = nullptr;
