@@ -704,16 +704,17 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     R.tile_rows = std::min<uint32_t>(4096, (budget - 24 * 1024) / R.row.stride / 256 * 256);
     R.tile_rows2 = std::min<uint32_t>(8192, (budget - 24 * 1024) / R.row.stride / 256 * 256);
     const uint32_t rem = L - leaf_bits;
-    // one level up to 2^10 buckets, else 2^max_bits1 x the rest
+    // one level up to 2^10 buckets, else 2^max_bits1 x the rest (the first
+    // level widened up to 2^10 when the second would exceed 2^10)
     const uint32_t max_bits1 = std::min<uint32_t>(hint.max_bits1 ? hint.max_bits1 : 8u, 10u);
-    const uint32_t max_rem = hint.single_level ? 10u : max_bits1 + 10u;
+    const uint32_t max_rem = hint.single_level ? 10u : 20u;
     // automatic: at least one row per four keys of the domain on average, so
     // leaves are well filled (sparser domains stay with the HBM hash table)
     const bool fits = fit_bits >= 8 && dom > 65536 && n_rows_scan >= (1u << 20) && dom / 4 <= n_rows_scan;
     if ((forced || fits) && rem <= max_rem && R.tile_rows >= 256) {
       P.radix = true;
       R.leaf_bits = leaf_bits;
-      R.bits1 = rem <= 10 ? rem : std::min<uint32_t>(max_bits1, rem);
+      R.bits1 = rem <= 10 ? rem : std::max<uint32_t>(std::min<uint32_t>(max_bits1, rem), rem - 10);
       const char* b1_env = getenv("THINDB_GPU_RADIX_BITS1");  // tests: force a second level
       if (b1_env && !hint.single_level) R.bits1 = std::min<uint32_t>(R.bits1, (uint32_t)atoi(b1_env));
       R.bits2 = rem - R.bits1;
