@@ -244,6 +244,18 @@ def test_radix_two_levels_at_width(session, oracle_lib, monkeypatch):
     assert st.global_mode == 3 and st.n_groups == len(want)
 
 
+def test_radix_wide_first_level(session, oracle_lib, monkeypatch):
+    """Leaves of 2^9 keys over the ~3.2e8-key high-cardinality domain need
+    2^20 buckets: a first level of 2^10 and a second of 2^10."""
+    t = gen.table("highcard", 1_500_000, seed=13)
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    monkeypatch.setenv("THINDB_GPU_RADIX_LEAF_BITS", "9")
+    aggs = [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"), AggSpec("max", "w", "m")]
+    got, want, st = check_vs_oracle(session, oracle_lib, t, Request(["user_id", "bucket"], aggs), dop=8)
+    assert st.global_mode == 3 and st.n_groups == len(want)
+    check_vs_oracle(session, oracle_lib, t, Request(["user_id", "bucket"], aggs, [SortSpec("c", True)], 10), dop=8)
+
+
 @pytest.mark.parametrize("leaf_bits,bits1", [("3", ""), ("1", "5"), ("32", "")])
 def test_radix_partition_passes(session, oracle_lib, monkeypatch, leaf_bits, bits1):
     """The partition passes forced onto a small typed table (a 700-key
