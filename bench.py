@@ -75,17 +75,19 @@ DISTINCT_DESC = "GROUP BY bucket COUNT(*), COUNT(DISTINCT user_id) top-10 by the
 DISTINCT_BYTES_PER_ROW = 9  # bucket 1 + user_id 8
 
 
-def make_request(workload: str, having: bool = False, distinct: bool = False):
+def make_request(workload: str, having: bool = False, distinct: bool = False, where_tree: bool = False):
     from thindb_amd import gen
-    from thindb_amd.exec import And, AggSpec, Predicate, Request, SortSpec
+    from thindb_amd.exec import And, AggSpec, Not, Or, Predicate, Request, SortSpec
     if distinct:
         return Request(["bucket"], [AggSpec("count", None, "c"), AggSpec("count_distinct", "user_id", "u")],
                        [SortSpec("u", True)], 10)
     if workload == "filtered":
+        where = And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI)))
+        if where_tree:  # the same rows as a tree the scan evaluates per row (no fused range)
+            where = Not(Or((Predicate("ts", "lt", gen.FILTER_LO), Predicate("ts", "gte", gen.FILTER_HI))))
         return Request(["region"], [AggSpec("count", None, "c"), AggSpec("sum", "amount", "s"),
                                     AggSpec("avg", "price", "a")], [SortSpec("s", True)], 100,
-                       where_filter=And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI))),
-                       having_filter=Predicate("a", "gt", 500) if having else None)
+                       where_filter=where, having_filter=Predicate("a", "gt", 500) if having else None)
     if workload == "plain":
         return Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"), AggSpec("min", "v", "mn"),
                                AggSpec("max", "v", "mx")],
@@ -421,6 +423,8 @@ def main() -> int:
                     help="filtered/plain/highcard: add the workload's HAVING (bench.HAVING)")
     ap.add_argument("--distinct", action="store_true",
                     help="highcard: GROUP BY bucket, COUNT(DISTINCT user_id) instead of the headline query")
+    ap.add_argument("--where-tree", action="store_true",
+                    help="filtered: the WHERE as NOT (ts < lo OR ts >= hi), evaluated as a tree")
     ap.add_argument("--cpu-sample-rows", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -461,7 +465,9 @@ def main() -> int:
     if args.distinct and args.workload != "highcard":
         log("--distinct runs over the highcard workload")
         return 2
-    req = make_request(args.workload, args.having, args.distinct)
+    req = make_request(args.workload, args.having, args.distinct, args.where_tree)
+    if args.where_tree:
+        desc = f"{desc}, WHERE NOT (ts < lo OR ts >= hi)"
     if args.having:
         desc = f"{desc} {HAVING[args.workload]}"
     if args.distinct:
@@ -523,7 +529,7 @@ def main() -> int:
     pmc = ROOT / "profiles" / (f"radix_{args.workload}_200M_pmc.json" if radix
                                else f"scan_jit_{args.workload}_200M_pmc.json")
     if pmc.exists() and rows == WORKLOADS[args.workload][0] and (st.specialized or radix) and not args.having \
-            and not args.distinct:
+            and not args.distinct and not args.where_tree:
         traffic = json.loads(pmc.read_text())["traffic_bytes"]
 
     out = {

@@ -50,7 +50,7 @@
 extern "C" {
 #endif
 
-#define TDBG_ABI_VERSION 2u
+#define TDBG_ABI_VERSION 3u
 
 enum tdbg_status {
   TDBG_OK = 0,
@@ -216,10 +216,32 @@ typedef struct tdbg_having_node {
   double lit_f64;
 } tdbg_having_node;
 
+/* A WHERE beyond a conjunction of `col op literal` leaves
+ * (`PredicateExpr` .leaf / .is_null / .is_not_null / .not / .and / .or,
+ * src/exec/predicate.zig:55-160), in postfix order, evaluated per row as
+ * evaluatePredicate does (:1727-1790): a comparison with a NULL value does
+ * not match, IS [NOT] NULL reads the validity, NOT inverts its child's
+ * matches, AND / OR combine `arity` children (arity 0: every row / none).
+ * Kinds are enum tdbg_having_kind; a comparison's literal is coerced to the
+ * column's type as in tdbg_predicate.  The plan's filter is the AND of
+ * `preds` and this tree.  At most TDBG_MAX_WHERE nodes. */
+#define TDBG_MAX_WHERE 12u
+typedef struct tdbg_filter_node {
+  uint32_t kind;       /* enum tdbg_having_kind */
+  uint32_t column_idx; /* CMP / IS_[NOT_]NULL */
+  uint32_t type;       /* CMP: enum tdbg_type of the column */
+  uint32_t op;         /* CMP: enum tdbg_pred_op */
+  uint32_t arity;      /* AND / OR: children popped */
+  uint32_t pad;
+  int64_t i64;         /* CMP literal, integer family */
+  double f64;          /* CMP literal, FLOAT / DOUBLE */
+} tdbg_filter_node;
+
 /* A grouped query: the part of v2_shape_group_topn.zig:63-101 (`Request`)
  * the GPU operator consumes.  `limit == 0` means no limit, as in
- * src/exec/v2_lowcard_group.zig:2072-2081.  The WHERE is the AND of `preds`;
- * `having` (n_having nodes, may be NULL when 0) filters the groups. */
+ * src/exec/v2_lowcard_group.zig:2072-2081.  The WHERE is the AND of `preds`
+ * and the `where` tree; `having` (n_having nodes, may be NULL when 0) filters
+ * the groups. */
 typedef struct tdbg_plan_desc {
   uint64_t table_uid;
   uint32_t n_keys;
@@ -234,8 +256,9 @@ typedef struct tdbg_plan_desc {
   uint64_t offset;
   uint64_t est_groups; /* planner's NDV-product bound; 0 = unknown */
   uint32_t n_having;
-  uint32_t pad;
+  uint32_t n_where;
   const tdbg_having_node* having;
+  const tdbg_filter_node* where; /* n_where nodes (may be NULL when 0) */
 } tdbg_plan_desc;
 
 /* One result column, library-owned host memory valid until

@@ -27,6 +27,8 @@
  *   - ORDER BY: SortCtx.lessThan/compareBySpec, :2198-2271 (ties by gid,
  *     BIGINT SUM by its wrapped value, AVG by the unscaled mean, floats by
  *     types.floatOrder, src/types.zig:579-587);
+ *   - WHERE trees (AND / OR / NOT / IS [NOT] NULL over `col op literal`
+ *     leaves): evaluatePredicate, predicate.zig:1727-1790;
  *   - COUNT(DISTINCT): the (group, value) composites of rows with a non-NULL
  *     integer value, deduplicated and counted per group (foldDistinctValues /
  *     compositeOf, v2_lowcard_group.zig:1359-1418, :1497-1499);
@@ -323,11 +325,45 @@ static u128 pack_key(const cplan* p, uint64_t r) {
   return key;
 }
 
+/* The WHERE tree's subtree rooted at postfix node `at` over row r
+ * (evaluatePredicate, predicate.zig:1727-1790: a NULL never matches a
+ * comparison, NOT inverts, AND/OR over `arity` children); the subtree's
+ * first node index goes to *first. */
+static int where_eval(const cplan* p, uint64_t r, int32_t at, int32_t* first) {
+  const tdbg_filter_node* f = &p->d->where[at];
+  const or_column* c = &p->t->cols[f->column_idx];
+  switch (f->kind) {
+    case TDBG_HAVING_CMP: {
+      *first = at;
+      tdbg_predicate leaf = {f->column_idx, f->type, f->op, c->validity != NULL, f->i64, f->f64};
+      return pred_pass(&leaf, c, r);
+    }
+    case TDBG_HAVING_IS_NULL: *first = at; return !is_valid(c, r);
+    case TDBG_HAVING_IS_NOT_NULL: *first = at; return is_valid(c, r);
+    case TDBG_HAVING_NOT: return !where_eval(p, r, at - 1, first);
+    default: {
+      int is_and = f->kind == TDBG_HAVING_AND;
+      int acc = is_and;
+      int32_t end = at - 1;
+      *first = at;
+      for (uint32_t k = 0; k < f->arity; k++) {
+        int32_t fi = end;
+        int v = where_eval(p, r, end, &fi);
+        acc = is_and ? (acc && v) : (acc || v);
+        *first = fi;
+        end = fi - 1;
+      }
+      return acc;
+    }
+  }
+}
+
 static int row_passes(const cplan* p, uint64_t r) {
   const tdbg_plan_desc* d = p->d;
   for (uint32_t k = 0; k < d->n_preds; k++)
     if (!pred_pass(&d->preds[k], &p->t->cols[d->preds[k].column_idx], r)) return 0;
-  return 1;
+  int32_t first = 0;
+  return d->n_where == 0 || where_eval(p, r, (int32_t)d->n_where - 1, &first);
 }
 
 /* ---- one sub-batch: foldBatch (v2_lowcard_group.zig:977-1050) --------------- */
@@ -335,11 +371,8 @@ static void fold_batch(wstate* w, const cplan* p, uint64_t lo, uint64_t hi) {
   const or_table* t = p->t;
   const tdbg_plan_desc* d = p->d;
   uint64_t n = 0;
-  for (uint64_t r = lo; r < hi; r++) {
-    int ok = 1;
-    for (uint32_t k = 0; k < d->n_preds && ok; k++) ok = pred_pass(&d->preds[k], &t->cols[d->preds[k].column_idx], r);
-    if (ok) w->sel[n++] = r;
-  }
+  for (uint64_t r = lo; r < hi; r++)
+    if (row_passes(p, r)) w->sel[n++] = r;
   if (n == 0) return;
   /* packKeysForPart / packIntPart */
   for (uint64_t i = 0; i < n; i++) w->bkeys[i] = pack_key(p, w->sel[i]);
@@ -917,6 +950,28 @@ static int compile(cplan* p, const tdbg_plan_desc* d, const or_table* t) {
   }
   for (uint32_t i = 0; i < d->n_order; i++)
     if (d->order[i].target >= d->n_keys + d->n_aggs) { set_err("order target"); return TDBG_ERR_INVALID; }
+  /* WHERE tree: a postfix array that forms exactly one tree */
+  if (d->n_where && !d->where) { set_err("where array"); return TDBG_ERR_INVALID; }
+  int64_t wdepth = 0;
+  for (uint32_t i = 0; i < d->n_where; i++) {
+    const tdbg_filter_node* f = &d->where[i];
+    if (f->kind <= TDBG_HAVING_IS_NOT_NULL) {
+      if (f->column_idx >= t->n_cols || (f->kind == TDBG_HAVING_CMP && f->op > TDBG_GTE)) {
+        set_err("where leaf");
+        return TDBG_ERR_INVALID;
+      }
+      wdepth++;
+    } else if (f->kind == TDBG_HAVING_NOT) {
+      if (wdepth < 1) { set_err("where NOT"); return TDBG_ERR_INVALID; }
+    } else if (f->kind == TDBG_HAVING_AND || f->kind == TDBG_HAVING_OR) {
+      if ((int64_t)f->arity > wdepth) { set_err("where arity"); return TDBG_ERR_INVALID; }
+      wdepth -= (int64_t)f->arity - 1;
+    } else {
+      set_err("where kind");
+      return TDBG_ERR_INVALID;
+    }
+  }
+  if (d->n_where && wdepth != 1) { set_err("where shape"); return TDBG_ERR_INVALID; }
   /* HAVING: a postfix array that forms exactly one tree */
   if (d->n_having && !d->having) { set_err("having array"); return TDBG_ERR_INVALID; }
   if (d->n_having > TDBG_MAX_HAVING) { set_err("having size"); return TDBG_ERR_UNSUPPORTED; }

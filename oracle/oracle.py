@@ -62,12 +62,18 @@ class _Having(C.Structure):
                 ("lit_f64", C.c_double)]
 
 
+class _Filter(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("column_idx", C.c_uint32), ("type", C.c_uint32), ("op", C.c_uint32),
+                ("arity", C.c_uint32), ("pad", C.c_uint32), ("i64", C.c_int64), ("f64", C.c_double)]
+
+
 class _Desc(C.Structure):
     _fields_ = [("table_uid", C.c_uint64), ("n_keys", C.c_uint32), ("n_aggs", C.c_uint32),
                 ("keys", C.POINTER(_Key)), ("aggs", C.POINTER(_Agg)), ("n_preds", C.c_uint32),
                 ("n_order", C.c_uint32), ("preds", C.POINTER(_Pred)), ("order", C.POINTER(_Order)),
                 ("limit", C.c_uint64), ("offset", C.c_uint64), ("est_groups", C.c_uint64),
-                ("n_having", C.c_uint32), ("pad", C.c_uint32), ("having", C.POINTER(_Having))]
+                ("n_having", C.c_uint32), ("n_where", C.c_uint32), ("having", C.POINTER(_Having)),
+                ("where", C.POINTER(_Filter))]
 
 
 H_CMP, H_IS_NULL, H_IS_NOT_NULL, H_NOT, H_AND, H_OR = range(6)
@@ -156,17 +162,35 @@ def _literal(col_type: int, scale: int, op: str, val):
 
 
 def _flatten(expr) -> list:
+    """The top-level AND's members (nested ANDs flattened)."""
     if expr is None:
         return []
-    name = type(expr).__name__
-    if name == "Predicate":
-        return [expr]
-    if name == "And":
+    if type(expr).__name__ == "And":
         out = []
         for ch in expr.children:
             out.extend(_flatten(ch))
         return out
-    raise Unsupported("only AND of leaves")
+    return [expr]
+
+
+def _where_nodes(expr, schema, idx) -> list:
+    """A WHERE subtree in postfix order (children first)."""
+    kind = type(expr).__name__
+    if kind == "Predicate":
+        i = idx[expr.col.lower()]
+        c = schema[i]
+        op, iv, fv = _literal(int(c.type), c.scale, expr.op, expr.val)
+        return [_Filter(H_CMP, i, int(c.type), op, 0, 0, iv, fv)]
+    if kind in ("IsNull", "IsNotNull"):
+        return [_Filter(H_IS_NULL if kind == "IsNull" else H_IS_NOT_NULL, idx[expr.col.lower()])]
+    if kind == "Not":
+        return _where_nodes(expr.child, schema, idx) + [_Filter(H_NOT)]
+    if kind in ("And", "Or"):
+        out = []
+        for ch in expr.children:
+            out.extend(_where_nodes(ch, schema, idx))
+        return out + [_Filter(H_AND if kind == "And" else H_OR, 0, 0, 0, len(expr.children))]
+    raise Unsupported(f"WHERE form {kind}")
 
 
 def _having_literal(val, out_type: int, out_scale: int):
@@ -272,7 +296,15 @@ def run(table, request, dop: int = 1) -> OracleResult:
             aggs[i].input_scale = c.scale
             aggs[i].nullable = int(c.nullable)
         out_names.append(a.as_)
-    leaves = _flatten(request.where_filter)
+    members = _flatten(request.where_filter)
+    leaves = [m for m in members if type(m).__name__ == "Predicate"]
+    rest = [m for m in members if type(m).__name__ != "Predicate"]
+    wl = []
+    for m in rest:
+        wl.extend(_where_nodes(m, schema, idx))
+    if len(rest) > 1:
+        wl.append(_Filter(H_AND, 0, 0, 0, len(rest)))
+    where = (_Filter * max(1, len(wl)))(*wl)
     preds = (_Pred * max(1, len(leaves)))()
     for i, p in enumerate(leaves):
         c = schema[idx[p.col.lower()]]
@@ -295,7 +327,7 @@ def run(table, request, dop: int = 1) -> OracleResult:
     having = (_Having * max(1, len(hl)))(*hl)
     desc = _Desc(table.uid, len(request.group_cols), len(request.aggs), keys, aggs, len(leaves),
                  len(request.order_specs), preds, order, request.limit, request.offset, request.est_groups,
-                 len(hl), 0, having)
+                 len(hl), len(wl), having, where)
     h = C.c_void_p()
     rc = L.oracle_group_by(C.byref(desc), C.byref(t), dop, C.byref(h))
     if rc == -2:

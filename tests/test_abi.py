@@ -28,13 +28,14 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 def test_abi_version_handshake():
     lib = gpu.load()
-    assert lib.tdbg_abi_version() == gpu.ABI_VERSION == 2
+    assert lib.tdbg_abi_version() == gpu.ABI_VERSION == 3
 
 
 STRUCTS = {  # header type -> ctypes binding
     "tdbg_column_key": gpu.ColumnKey, "tdbg_block_stats": gpu.BlockStats, "tdbg_rg_ref": gpu.RgRef,
     "tdbg_key_part": gpu.KeyPart, "tdbg_agg_spec": gpu.AggSpecC, "tdbg_predicate": gpu.PredicateC,
-    "tdbg_order_spec": gpu.OrderSpecC, "tdbg_having_node": gpu.HavingNodeC, "tdbg_plan_desc": gpu.PlanDesc, "tdbg_column_view": gpu.ColumnView,
+    "tdbg_order_spec": gpu.OrderSpecC, "tdbg_having_node": gpu.HavingNodeC, "tdbg_filter_node": gpu.FilterNodeC,
+    "tdbg_plan_desc": gpu.PlanDesc, "tdbg_column_view": gpu.ColumnView,
     "tdbg_exec_stats": gpu.ExecStats, "tdbg_batch_stats": gpu.BatchStats, "tdbg_batch_route": gpu.BatchRoute,
     "tdbg_server_config": gpu.ServerConfig, "tdbg_ticket_report": gpu.TicketReport,
     "tdbg_server_counters": gpu.ServerCounters,

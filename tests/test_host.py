@@ -36,13 +36,21 @@ def test_literal_coercion_matches_integer_semantics():
                 assert want == got, (op, lit, v)
 
 
-def test_only_conjunctions_fuse():
+def test_where_conjuncts_and_tree():
+    """Top-level AND leaves become fused predicate terms; everything else one
+    postfix tree AND-ed with them."""
     p, q = Predicate("a", "gt", 1), Predicate("b", "lt", 2)
-    assert ex._leaves(And((p, And((q,))))) == [p, q]
+    assert ex._conjuncts(And((p, And((q,))))) == [p, q]
+    assert ex._conjuncts(Or((p, q))) == [Or((p, q))]
+    t = Table([Column("a", Type.INT), Column("b", Type.DOUBLE, True)], {"a": [1], "b": [1.0]}, {"b": [True]})
+    out, cols = [], []
+    ex._where_postfix(Or((Not(p), And((q, ex.IsNull("b"))))), t, out, cols)
+    K = ex.gpu.HavingKind
+    assert out == [(K.CMP, 0, Type.INT, PredOp.GT, 1), (K.NOT,), (K.CMP, 1, Type.DOUBLE, PredOp.LT, 2.0),
+                   (K.IS_NULL, 1), (K.AND, 2), (K.OR, 2)]
+    assert cols == ["a", "b", "b"]
     with pytest.raises(ex.Declined):
-        ex._leaves(Or((p, q)))
-    with pytest.raises(ex.Declined):
-        ex._leaves(Not(p))
+        ex._where_postfix("text", t, [], [])
 
 
 def test_output_schema_types():

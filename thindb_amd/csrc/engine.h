@@ -53,10 +53,9 @@ struct PredDesc {
 };
 
 // Key part: relative value rel = v - base, 0 <= rel < span, folded into a
-// mixed-radix group index key = sum(rel_i * stride_i).
-// A key part's relative value rel = v - base in [0, span); the key is the
-// mixed-radix sum of rel * stride.  A nullable part keeps rel 0 for NULL and
-// its base one below the smallest value, so NULL sorts first.
+// mixed-radix group index key = sum(rel_i * stride_i).  A nullable part keeps
+// rel 0 for NULL and its base one below the smallest value, so NULL sorts
+// first.
 struct PartDesc {
   uint32_t col;
   uint32_t nullable;
@@ -101,9 +100,22 @@ struct Tile {
 // Error bits raised by kernels (host maps them to status codes).
 enum ErrBit : uint32_t { ERR_KEY_RANGE = 1u, ERR_HASH_FULL = 2u };
 
+// One node of a WHERE tree in postfix order (kinds of HavingKind: H_CMP,
+// H_IS_NULL, H_IS_NOT_NULL, H_NOT, H_AND, H_OR).  A comparison is one
+// predicate term over column pd.col; IS [NOT] NULL reads pd.col's validity.
+constexpr int kMaxFilter = 12;
+struct FilterNode {
+  uint32_t kind;
+  uint32_t arity;  // H_AND / H_OR
+  PredDesc pd;
+};
+
 struct ScanParams {
   ColDesc cols[kMaxCols];
   PredDesc preds[kMaxPreds];
+  FilterNode filt[kMaxFilter];  // AND-ed with preds: the WHERE beyond a conjunction of leaves
+  uint32_t n_filt;
+  uint32_t pad_filt;
   PartDesc parts[kMaxParts];
   AggDesc aggs[kMaxAggs];
   uint32_t n_cols, n_preds, n_parts, n_aggs;

@@ -88,6 +88,9 @@ std::string jit_signature(const ScanParams& p) {
   for (uint32_t i = 0; i < p.n_preds; i++) s << p.preds[i].kind << ":" << p.preds[i].col << ",";
   s << "k";
   for (uint32_t i = 0; i < p.n_parts; i++) s << p.parts[i].col << (p.parts[i].nullable ? "n" : "") << ",";
+  s << "f";
+  for (uint32_t i = 0; i < p.n_filt; i++)
+    s << p.filt[i].kind << ":" << p.filt[i].arity << ":" << p.filt[i].pd.kind << ":" << p.filt[i].pd.col << ",";
   s << "a";
   for (uint32_t i = 0; i < p.n_aggs; i++)
     s << p.aggs[i].kind << ":" << p.aggs[i].col << (p.aggs[i].nullable ? "n" : "") << (p.aggs[i].split ? "s" : "")
@@ -126,7 +129,42 @@ std::string jit_source(const ScanParams& p, bool prefetch) {
     if (p.cols[d.col].valid) s << " & valid4(P.cols[" << d.col << "], r)";
     s << ";\n";
   }
-  if (p.n_preds) s << "    if (!mask) return;\n";
+  if (p.n_filt) {
+    // the WHERE tree, its postfix order unrolled into named strip masks
+    std::vector<std::string> stk;
+    for (uint32_t i = 0; i < p.n_filt; i++) {
+      const FilterNode& f = p.filt[i];
+      const std::string v = "f" + std::to_string(i);
+      const bool valid = f.kind <= H_IS_NOT_NULL && p.cols[f.pd.col].valid;
+      const std::string vcol = "valid4(P.cols[" + std::to_string(f.pd.col) + "], r)";
+      std::string e;
+      switch (f.kind) {
+        case H_CMP:
+          e = std::string("pred_k<") + pred_name(f.pd.kind) + ", " + kind_name(p.cols[f.pd.col].kind) + ">(P.filt[" +
+              std::to_string(i) + "].pd, x" + std::to_string(f.pd.col) + ")" + (valid ? " & " + vcol : "");
+          break;
+        case H_IS_NULL: e = valid ? "(~" + vcol + " & 0xfu)" : "0u"; break;
+        case H_IS_NOT_NULL: e = valid ? vcol : "0xfu"; break;
+        case H_NOT:
+          e = "(~" + stk.back() + " & 0xfu)";
+          stk.pop_back();
+          break;
+        default: {
+          const char* op = f.kind == H_AND ? " & " : " | ";
+          e = f.kind == H_AND ? "0xfu" : "0u";
+          for (uint32_t a = 0; a < f.arity; a++) {
+            e += op + stk.back();
+            stk.pop_back();
+          }
+          break;
+        }
+      }
+      s << "    const uint32_t " << v << " = " << e << ";\n";
+      stk.push_back(v);
+    }
+    s << "    mask &= " << stk.back() << ";\n";
+  }
+  if (p.n_preds || p.n_filt) s << "    if (!mask) return;\n";
   s << "    uint64_t key[kVec];\n";
   for (uint32_t i = 0; i < p.n_parts; i++) {
     const PartDesc& d = p.parts[i];

@@ -22,6 +22,7 @@ __device__ __forceinline__ uint32_t rx_strip_keys(const ScanParams& P, uint64_t 
       const ColDesc& c = P.cols[pd.col];
       mask &= pred_rt(pd, load_raw(c, r), c.kind) & valid4(c, r);
     }
+    if (P.n_filt) mask &= filter_mask(P, r, [&](uint32_t col) { return load_raw(P.cols[col], r); });
   }
   for (uint32_t i = 0; i < P.n_parts; i++) {
     const PartDesc& pt = P.parts[i];

@@ -261,10 +261,43 @@ int32_t validate(tdbg_plan* pl) {
     if (!is_int_type(p.type) && !is_float_type(p.type)) return fail(TDBG_ERR_UNSUPPORTED, "predicate type");
     use(p.column_idx);
   }
-  if (cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than %d distinct columns", kMaxCols);
   if (pl->order.size() > 6) return fail(TDBG_ERR_UNSUPPORTED, "more than 6 ORDER BY items");
   for (auto& o : pl->order)
     if (o.target >= pl->keys.size() + pl->aggs.size()) return fail(TDBG_ERR_INVALID, "ORDER BY target");
+  // WHERE tree: a well-formed postfix tree of comparisons over integer or
+  // float columns and NULL tests
+  if (pl->where.size() > TDBG_MAX_WHERE) return fail(TDBG_ERR_UNSUPPORTED, "more than %u WHERE nodes", TDBG_MAX_WHERE);
+  {
+    uint64_t depth = 0;
+    for (size_t i = 0; i < pl->where.size(); i++) {
+      const tdbg_filter_node& f = pl->where[i];
+      switch (f.kind) {
+        case TDBG_HAVING_CMP:
+          if (f.op > TDBG_GTE) return fail(TDBG_ERR_INVALID, "WHERE node %zu: op %u", i, f.op);
+          if (!is_int_type(f.type) && !is_float_type(f.type)) return fail(TDBG_ERR_UNSUPPORTED, "WHERE node %zu: type", i);
+          use(f.column_idx);
+          depth++;
+          break;
+        case TDBG_HAVING_IS_NULL:
+        case TDBG_HAVING_IS_NOT_NULL:
+          use(f.column_idx);
+          depth++;
+          break;
+        case TDBG_HAVING_NOT:
+          if (depth < 1) return fail(TDBG_ERR_INVALID, "WHERE node %zu: NOT of nothing", i);
+          break;
+        case TDBG_HAVING_AND:
+        case TDBG_HAVING_OR:
+          if (f.arity > depth) return fail(TDBG_ERR_INVALID, "WHERE node %zu: arity %u over %llu entries", i, f.arity,
+                                           (unsigned long long)depth);
+          depth = depth - f.arity + 1;
+          break;
+        default: return fail(TDBG_ERR_INVALID, "WHERE node %zu: kind %u", i, f.kind);
+      }
+    }
+    if (!pl->where.empty() && depth != 1) return fail(TDBG_ERR_INVALID, "WHERE leaves %llu entries", (unsigned long long)depth);
+  }
+  if (cols.size() > (size_t)kMaxCols) return fail(TDBG_ERR_UNSUPPORTED, "more than %d distinct columns", kMaxCols);
   // HAVING: a well-formed postfix tree over the output columns (every output
   // here is a key or a COUNT/SUM/AVG/MIN/MAX, all of which the reference's
   // havingLeafColSupported accepts, v2_shape_group_topn.zig:2430-2441)
@@ -395,6 +428,8 @@ int32_t collect_scan(tdbg_plan* pl, Table* t, const tdbg_rg_ref* rgs, uint32_t n
   for (auto& k : pl->keys) need_cols.push_back(k.column_idx);
   for (auto& a : pl->aggs) if (a.op != TDBG_AGG_COUNT_STAR) need_cols.push_back((uint32_t)a.column_idx);
   for (auto& p : pl->preds) need_cols.push_back(p.column_idx);
+  for (auto& f : pl->where)
+    if (f.kind <= TDBG_HAVING_IS_NOT_NULL) need_cols.push_back(f.column_idx);
   std::vector<const RowGroupEntry*> scan;
   auto consider = [&](const RowGroupEntry& rg) -> int32_t {
     for (uint32_t c : need_cols) {
@@ -481,6 +516,39 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     sp.preds[sp.n_preds++] = d;
   }
   if (sp.n_preds > (uint32_t)kMaxPreds) return fail(TDBG_ERR_UNSUPPORTED, "too many predicate terms");
+  // the WHERE tree: each comparison as one range or NEQ term of its own
+  sp.n_filt = 0;
+  for (const tdbg_filter_node& f : pl->where) {
+    FilterNode& n = sp.filt[sp.n_filt++];
+    n = FilterNode{};
+    n.kind = f.kind;
+    n.arity = f.arity;
+    if (f.kind > TDBG_HAVING_IS_NOT_NULL) continue;
+    auto it = t->cols.find(f.column_idx);
+    if (it == t->cols.end()) return fail(TDBG_ERR_NOT_STAGED, "WHERE column %u not staged", f.column_idx);
+    n.pd.col = col_slot(f.column_idx);
+    if (f.kind != TDBG_HAVING_CMP) continue;
+    if (it->second.type != f.type) return fail(TDBG_ERR_INVALID, "WHERE column %u type mismatch", f.column_idx);
+    tdbg_predicate leaf{f.column_idx, f.type, f.op, 0u, f.i64, f.f64};
+    std::vector<MergedPred> one;
+    std::vector<tdbg_predicate> neq;
+    const bool sat = merge_preds({leaf}, one, neq);
+    if (!neq.empty()) {
+      n.pd.kind = is_float_type(f.type) ? P_NEQ_F : P_NEQ_I;
+      n.pd.lo_i = f.i64;
+      n.pd.lo_f = f.f64;
+    } else {
+      const MergedPred& m = one[0];
+      n.pd.kind = m.is_float ? P_RANGE_F : P_RANGE_I;
+      n.pd.lo_i = m.lo_i; n.pd.hi_i = m.hi_i;
+      n.pd.lo_f = m.lo_f; n.pd.hi_f = m.hi_f;
+      n.pd.lo_incl = m.lo_incl; n.pd.hi_incl = m.hi_incl;
+      if (!sat) {  // a comparison no value meets
+        n.pd.lo_i = 1; n.pd.hi_i = 0;
+        n.pd.lo_f = 1.0; n.pd.hi_f = 0.0;
+      }
+    }
+  }
   // key parts: relative values in a mixed-radix index
   unsigned __int128 domain = 1;
   for (size_t i = 0; i < pl->keys.size(); i++) {
@@ -1230,7 +1298,7 @@ int32_t tdbg_plan_create(tdbg_context* ctx, const tdbg_plan_desc* d, tdbg_plan**
   if (!ctx || !d || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
   if ((d->n_keys && !d->keys) || (d->n_aggs && !d->aggs) || (d->n_preds && !d->preds) || (d->n_order && !d->order) ||
-      (d->n_having && !d->having))
+      (d->n_having && !d->having) || (d->n_where && !d->where))
     return fail(TDBG_ERR_INVALID, "null array in plan");
   std::unique_ptr<tdbg_plan> pl(new tdbg_plan());
   pl->ctx = ctx;
@@ -1240,6 +1308,7 @@ int32_t tdbg_plan_create(tdbg_context* ctx, const tdbg_plan_desc* d, tdbg_plan**
   pl->preds.assign(d->preds, d->preds + d->n_preds);
   pl->order.assign(d->order, d->order + d->n_order);
   if (d->n_having) pl->having.assign(d->having, d->having + d->n_having);
+  if (d->n_where) pl->where.assign(d->where, d->where + d->n_where);
   pl->limit = d->limit;
   pl->offset = d->offset;
   pl->est_groups = d->est_groups;

@@ -204,6 +204,7 @@ struct tdbg_plan {
   std::vector<tdbg_predicate> preds;
   std::vector<tdbg_order_spec> order;
   std::vector<tdbg_having_node> having;  // postfix HAVING nodes, empty = none
+  std::vector<tdbg_filter_node> where;   // postfix WHERE tree AND-ed with preds, empty = none
   uint64_t limit = 0, offset = 0, est_groups = 0;
   std::vector<tdbg::rt::AggOut> outs;
   std::mutex mu;

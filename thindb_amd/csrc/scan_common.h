@@ -186,6 +186,42 @@ __device__ __forceinline__ uint32_t pred_rt(const PredDesc& p, const Raw& x, uin
   }
 }
 
+// The WHERE tree over one strip (evaluatePredicate,
+// src/exec/predicate.zig:1727-1790): a comparison with a NULL value does not
+// match, IS [NOT] NULL read the validity, NOT inverts its child, AND / OR
+// combine (arity 0: all / none).  `fetch(col)` gives column col's strip.
+// The stack holds one 4-bit strip mask per pending node (at most kMaxFilter).
+template <typename Fetch>
+__device__ __forceinline__ uint32_t filter_mask(const ScanParams& P, uint64_t r, Fetch fetch) {
+  uint64_t st = 0;
+  for (uint32_t k = 0; k < P.n_filt; k++) {
+    const FilterNode& f = P.filt[k];
+    uint32_t m;
+    switch (f.kind) {
+      case H_CMP: {
+        const ColDesc& c = P.cols[f.pd.col];
+        m = pred_rt(f.pd, fetch(f.pd.col), c.kind) & valid4(c, r);
+        break;
+      }
+      case H_IS_NULL: m = ~valid4(P.cols[f.pd.col], r) & 0xfu; break;
+      case H_IS_NOT_NULL: m = valid4(P.cols[f.pd.col], r); break;
+      case H_NOT: st ^= 0xfull; continue;
+      default: {  // H_AND / H_OR
+        const bool all = f.kind == H_AND;
+        m = all ? 0xfu : 0u;
+        for (uint32_t a = 0; a < f.arity; a++) {
+          const uint32_t x = (uint32_t)(st & 0xfull);
+          st >>= 4;
+          m = all ? (m & x) : (m | x);
+        }
+        break;
+      }
+    }
+    st = (st << 4) | m;
+  }
+  return (uint32_t)(st & 0xfull);
+}
+
 // ---- HBM records --------------------------------------------------------------------
 
 __device__ __forceinline__ void g_add_u64(uint8_t* p, uint64_t v) {

@@ -57,6 +57,7 @@ struct GenericStrip {
       const ColDesc& c = P.cols[pd.col];
       mask &= pred_rt(pd, fetch<FAST>(P, raw, pd.col, r), c.kind) & valid4(c, r);
     }
+    if (P.n_filt) mask &= filter_mask(P, r, [&](uint32_t col) { return fetch<FAST>(P, raw, col, r); });
     if (!mask) return;
     uint64_t key[kVec] = {0, 0, 0, 0};
     for (uint32_t i = 0; i < P.n_parts; i++) {

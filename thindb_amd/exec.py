@@ -223,17 +223,43 @@ class Declined(Exception):
     """The GPU operator does not cover this shape."""
 
 
-def _leaves(expr: Optional[PredicateExpr]) -> List[Predicate]:
+def _conjuncts(expr: Optional[PredicateExpr]) -> list:
+    """The top-level AND's children (nested ANDs flattened)."""
     if expr is None:
         return []
-    if isinstance(expr, Predicate):
-        return [expr]
     if isinstance(expr, And):
-        out: List[Predicate] = []
+        out: list = []
         for ch in expr.children:
-            out.extend(_leaves(ch))
+            out.extend(_conjuncts(ch))
         return out
-    raise Declined("only AND-ed col-op-literal leaves fuse into the GPU scan")
+    return [expr]
+
+
+def _where_postfix(expr, table: "Table", out: list, cols: List[str]) -> None:
+    """A WHERE subtree in postfix order as gpu.Plan `where` nodes
+    (PredicateExpr leaf / is_null / is_not_null / not / and / or,
+    src/exec/predicate.zig:55-160); other forms decline."""
+    if isinstance(expr, Predicate):
+        c = table.column(expr.col)
+        if c.type not in _INT_TYPES and c.type not in _FLOAT_TYPES:
+            raise Declined(f"predicate on {expr.col}")
+        op, lit = _coerce(c, expr.op, expr.val)
+        out.append((gpu.HavingKind.CMP, table.column_index(expr.col), c.type, op, lit))
+        cols.append(c.name)
+    elif isinstance(expr, (IsNull, IsNotNull)):
+        c = table.column(expr.col)
+        kind = gpu.HavingKind.IS_NULL if isinstance(expr, IsNull) else gpu.HavingKind.IS_NOT_NULL
+        out.append((kind, table.column_index(expr.col)))
+        cols.append(c.name)
+    elif isinstance(expr, Not):
+        _where_postfix(expr.child, table, out, cols)
+        out.append((gpu.HavingKind.NOT,))
+    elif isinstance(expr, (And, Or)):
+        for ch in expr.children:
+            _where_postfix(ch, table, out, cols)
+        out.append((gpu.HavingKind.AND if isinstance(expr, And) else gpu.HavingKind.OR, len(expr.children)))
+    else:
+        raise Declined(f"WHERE form {type(expr).__name__}")
 
 
 def _coerce(col: Column, op: str, val) -> Tuple[PredOp, Union[int, float]]:
@@ -451,14 +477,27 @@ def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu
               "min": AggOp.MIN, "max": AggOp.MAX}[f]
         aggs.append((op, table.column_index(a.col), c.type, c.scale, c.nullable))
         cols.append(c.name)
+    # the WHERE's top-level `col op literal` conjuncts prune by zone maps and
+    # fuse into the scan; the rest is one tree AND-ed with them
     preds = []
-    for p in _leaves(request.where_filter):
+    where: list = []
+    rest = []
+    for p in _conjuncts(request.where_filter):
+        if not isinstance(p, Predicate):
+            rest.append(p)
+            continue
         c = table.column(p.col)
         if c.type not in _INT_TYPES and c.type not in _FLOAT_TYPES:
             raise Declined(f"predicate on {p.col}")
         op, lit = _coerce(c, p.op, p.val)
         preds.append((table.column_index(p.col), c.type, op, lit, c.nullable))
         cols.append(c.name)
+    for r in rest:
+        _where_postfix(r, table, where, cols)
+    if len(rest) > 1:
+        where.append((gpu.HavingKind.AND, len(rest)))
+    if len(where) > gpu.MAX_WHERE:
+        raise Declined(f"WHERE tree of {len(where)} nodes")
     schema = output_schema(table, request)
     order = []
     for s in request.order_specs:
@@ -473,7 +512,7 @@ def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu
             raise Declined(f"HAVING of {len(having)} nodes")
     try:
         plan = gpu.Plan(session.ctx, table.uid, keys, aggs, preds, order, request.limit, request.offset,
-                        request.est_groups, having)
+                        request.est_groups, having, where)
     except gpu.Unsupported as e:
         raise Declined(str(e)) from e
     return plan, cols

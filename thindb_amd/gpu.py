@@ -16,7 +16,7 @@ import numpy as np
 
 _LIB_PATH = Path(os.environ.get("THINDB_GPU_LIB", Path(__file__).resolve().parent / "libthindb_gpu.so"))
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class Status(enum.IntEnum):
@@ -89,6 +89,7 @@ class PredOp(enum.IntEnum):
 
 
 MAX_HAVING = 32  # TDBG_MAX_HAVING
+MAX_WHERE = 12   # TDBG_MAX_WHERE
 
 
 class HavingKind(enum.IntEnum):
@@ -178,13 +179,19 @@ class HavingNodeC(C.Structure):
                 ("lit_f64", C.c_double)]
 
 
+class FilterNodeC(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("column_idx", C.c_uint32), ("type", C.c_uint32), ("op", C.c_uint32),
+                ("arity", C.c_uint32), ("pad", C.c_uint32), ("i64", C.c_int64), ("f64", C.c_double)]
+
+
 class PlanDesc(C.Structure):
     _fields_ = [("table_uid", C.c_uint64), ("n_keys", C.c_uint32), ("n_aggs", C.c_uint32),
                 ("keys", C.POINTER(KeyPart)), ("aggs", C.POINTER(AggSpecC)),
                 ("n_preds", C.c_uint32), ("n_order", C.c_uint32),
                 ("preds", C.POINTER(PredicateC)), ("order", C.POINTER(OrderSpecC)),
                 ("limit", C.c_uint64), ("offset", C.c_uint64), ("est_groups", C.c_uint64),
-                ("n_having", C.c_uint32), ("pad", C.c_uint32), ("having", C.POINTER(HavingNodeC))]
+                ("n_having", C.c_uint32), ("n_where", C.c_uint32), ("having", C.POINTER(HavingNodeC)),
+                ("where", C.POINTER(FilterNodeC))]
 
 
 class ColumnView(C.Structure):
@@ -411,16 +418,37 @@ def having_node(t: tuple) -> HavingNodeC:
     return h
 
 
+def filter_node(t: tuple) -> FilterNodeC:
+    """One tdbg_filter_node from (kind, ...) as Plan documents it."""
+    kind = HavingKind(t[0])
+    f = FilterNodeC()
+    f.kind = int(kind)
+    if kind == HavingKind.CMP:
+        _, col, typ, op, lit = t
+        f.column_idx, f.type, f.op = int(col), int(typ), int(op)
+        if Type(typ) in (Type.FLOAT, Type.DOUBLE):
+            f.f64 = float(lit)
+        else:
+            f.i64 = int(lit)
+    elif kind in (HavingKind.IS_NULL, HavingKind.IS_NOT_NULL):
+        f.column_idx = int(t[1])
+    elif kind in (HavingKind.AND, HavingKind.OR):
+        f.arity = int(t[1])
+    return f
+
+
 class Plan:
     """A grouped query prepared on one context (tdbg_plan)."""
 
     def __init__(self, ctx: Context, table_uid: int, keys: Sequence[tuple], aggs: Sequence[tuple],
                  preds: Sequence[tuple] = (), order: Sequence[tuple] = (), limit: int = 0, offset: int = 0,
-                 est_groups: int = 0, having: Sequence[tuple] = ()):
+                 est_groups: int = 0, having: Sequence[tuple] = (), where: Sequence[tuple] = ()):
         """keys: (column_idx, Type); aggs: (AggOp, column_idx|-1, Type, scale, nullable);
         preds: (column_idx, Type, PredOp, literal, nullable); order: (target, desc);
         having: postfix nodes (HavingKind.CMP, target, PredOp, int|float literal),
-        (IS_NULL|IS_NOT_NULL, target), (NOT,), (AND|OR, arity)."""
+        (IS_NULL|IS_NOT_NULL, target), (NOT,), (AND|OR, arity);
+        where: postfix WHERE tree AND-ed with preds, (HavingKind.CMP, column_idx, Type, PredOp,
+        literal), (IS_NULL|IS_NOT_NULL, column_idx), (NOT,), (AND|OR, arity)."""
         lib = load()
         self._lib = lib
         self.ctx = ctx
@@ -438,8 +466,9 @@ class Plan:
         self._preds = (PredicateC * max(1, len(pl)))(*pl)
         self._order = (OrderSpecC * max(1, len(order)))(*[OrderSpecC(t, int(bool(d))) for t, d in order])
         self._having = (HavingNodeC * max(1, len(having)))(*[having_node(h) for h in having])
+        self._where = (FilterNodeC * max(1, len(where)))(*[filter_node(w) for w in where])
         desc = PlanDesc(table_uid, len(keys), len(aggs), self._keys, self._aggs, len(pl), len(order), self._preds,
-                        self._order, limit, offset, est_groups, len(having), 0, self._having)
+                        self._order, limit, offset, est_groups, len(having), len(where), self._having, self._where)
         h = C.c_void_p()
         check(lib.tdbg_plan_create(ctx.handle, C.byref(desc), C.byref(h)))
         self._h = h
