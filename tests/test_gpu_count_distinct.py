@@ -61,20 +61,25 @@ def test_count_distinct_highcard_values(session, oracle_lib):
     assert len(got) == 10 and all(r[2] <= r[1] for r in got)
 
 
-def test_count_distinct_declines(session):
-    """A float input declines at planning; a sparse key domain beyond 2^31
-    relative keys declines at execution; a batch refuses the plan."""
+def test_count_distinct_sparse_domain(session, oracle_lib):
+    """A 2^41-key sparse domain (the HBM hash table): composites use the
+    record index and counts go straight into the records."""
     rng = np.random.default_rng(1)
+    n = 200_000
+    keys = rng.integers(-2 ** 40, 2 ** 40, 3000)
+    wide = Table([Column("k", Type.BIGINT), Column("v", Type.INT, True), Column("w", Type.BIGINT)],
+                 {"k": keys[rng.integers(0, 3000, n)], "v": rng.integers(0, 90, n).astype(np.int32),
+                  "w": rng.integers(-2 ** 40, 2 ** 40, n)}, {"v": rng.random(n) > 0.3})
+    aggs = [AggSpec("count", None, "c"), AggSpec("count_distinct", "v", "dv"), AggSpec("count_distinct", "w", "dw")]
+    got, want, st = check_vs_oracle(session, oracle_lib, wide, Request(["k"], aggs))
+    assert st.global_mode == 2 and len(got) == 3000
+    check_vs_oracle(session, oracle_lib, wide, Request(["k"], aggs, [SortSpec("dv", True), SortSpec("k")], 20,
+                                                       having_filter=Predicate("dw", "gt", 60)))
+
+
+def test_count_distinct_declines(session):
+    """A float input declines at planning; a batch refuses the plan."""
     t = _table()
     assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("count_distinct", "d", "x")])) is None
-    n = 20_000
-    keys = rng.integers(-2 ** 40, 2 ** 40, 500)
-    wide = Table([Column("k", Type.BIGINT), Column("v", Type.INT)],
-                 {"k": keys[rng.integers(0, 500, n)], "v": rng.integers(0, 9, n).astype(np.int32)})
-    q = try_build_gpu_group(session, wide, Request(["k"], [AggSpec("count_distinct", "v", "d")]))
-    assert q is not None
-    with pytest.raises(gpu.Unsupported):
-        q.next()
-    q.deinit()
     with pytest.raises(Exception):
         GpuBatchRun(session, [(t, Request(["k1"], AGGS))])

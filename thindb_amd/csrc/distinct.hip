@@ -18,7 +18,10 @@ namespace tdbg {
 // LDS_COUNTS: a domain of at most kDistinctLdsKeys groups counts in LDS per
 // workgroup and adds its totals once at the end (few groups with many
 // distinct values would otherwise serialise on a handful of counters).
-template <bool LDS_COUNTS>
+// TO_RECORDS: the plan's records are addressable by key (dense index or HBM
+// hash table): the composite takes the record index instead of the relative
+// key (so sparse key domains fit 63 bits) and counts go to the record.
+template <bool LDS_COUNTS, bool TO_RECORDS>
 __global__ void __launch_bounds__(256) distinct_insert_kernel(const ScanParams P, const DistinctParams D,
                                                               uint32_t n_keys) {
   __shared__ uint32_t lcnt[LDS_COUNTS ? kDistinctLdsKeys : 1];
@@ -44,7 +47,14 @@ __global__ void __launch_bounds__(256) distinct_insert_kernel(const ScanParams P
           atomicOr(P.err, ERR_KEY_RANGE);
           continue;
         }
-        const unsigned long long c = key[j] * D.span + rel;
+        uint8_t* rec = nullptr;
+        uint64_t group = key[j];
+        if (TO_RECORDS) {  // the scan made the group's record: this finds it
+          rec = g_record(P, key[j]);
+          if (!rec) continue;
+          group = (uint64_t)(rec - P.g_state) / P.g_rec;
+        }
+        const unsigned long long c = group * D.span + rel;
         // at least twice as many slots as rows scanned: an empty slot ends
         // every probe
         uint64_t h = mix64(c) & D.set_mask;
@@ -52,6 +62,7 @@ __global__ void __launch_bounds__(256) distinct_insert_kernel(const ScanParams P
           const unsigned long long prev = atomicCAS(&D.set[h], ~0ull, c);
           if (prev == ~0ull) {
             if (LDS_COUNTS) atomicAdd(&lcnt[key[j]], 1u);
+            else if (TO_RECORDS) g_add_u64(rec + D.dslot_off, 1);
             else atomicAdd(&D.counts[key[j]], 1u);
             break;
           }
@@ -63,13 +74,17 @@ __global__ void __launch_bounds__(256) distinct_insert_kernel(const ScanParams P
   }
   if (LDS_COUNTS) {
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < n_keys; k += blockDim.x)
-      if (lcnt[k]) atomicAdd(&D.counts[k], lcnt[k]);
+    for (uint32_t k = threadIdx.x; k < n_keys; k += blockDim.x) {
+      if (!lcnt[k]) continue;
+      if (TO_RECORDS) g_add_u64(P.g_state + (uint64_t)k * P.g_rec + D.dslot_off, lcnt[k]);  // dense: index = key
+      else atomicAdd(&D.counts[k], lcnt[k]);
+    }
   }
 }
 
 // Each compacted group's count from its relative key (the dense index, or
-// the key word of a keyed record).
+// the key word of a keyed record); radix plans only (their records are not
+// addressable by key while the set is built).
 __global__ void __launch_bounds__(256) distinct_gather_kernel(const FinalParams F, const DistinctParams D,
                                                               const uint32_t* __restrict__ gids,
                                                               const uint64_t* __restrict__ n_groups, uint64_t max_n) {
@@ -85,11 +100,16 @@ __global__ void __launch_bounds__(256) distinct_gather_kernel(const FinalParams 
 hipError_t launch_distinct_insert(const ScanParams& sp, const DistinctParams& D, uint64_t n_keys, uint32_t grid,
                                   hipStream_t s) {
   if (sp.n_tiles == 0) return hipSuccess;
-  if (grid == 0 || !D.set || !D.counts || D.span == 0 || n_keys == 0) return hipErrorInvalidValue;
-  if (n_keys <= kDistinctLdsKeys)
-    hipLaunchKernelGGL(distinct_insert_kernel<true>, dim3(grid), dim3(256), 0, s, sp, D, (uint32_t)n_keys);
-  else
-    hipLaunchKernelGGL(distinct_insert_kernel<false>, dim3(grid), dim3(256), 0, s, sp, D, 0u);
+  if (grid == 0 || !D.set || (!D.to_records && !D.counts) || D.span == 0 || n_keys == 0) return hipErrorInvalidValue;
+  // LDS counters flushed to records need the dense layout (record index = key)
+  const bool lds = n_keys <= kDistinctLdsKeys && (!D.to_records || sp.g_mode == G_DENSE);
+  if (D.to_records) {
+    if (lds) hipLaunchKernelGGL((distinct_insert_kernel<true, true>), dim3(grid), dim3(256), 0, s, sp, D, (uint32_t)n_keys);
+    else hipLaunchKernelGGL((distinct_insert_kernel<false, true>), dim3(grid), dim3(256), 0, s, sp, D, 0u);
+  } else {
+    if (lds) hipLaunchKernelGGL((distinct_insert_kernel<true, false>), dim3(grid), dim3(256), 0, s, sp, D, (uint32_t)n_keys);
+    else hipLaunchKernelGGL((distinct_insert_kernel<false, false>), dim3(grid), dim3(256), 0, s, sp, D, 0u);
+  }
   return hipGetLastError();
 }
 

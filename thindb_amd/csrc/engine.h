@@ -293,9 +293,10 @@ struct DistinctParams {
   uint64_t span;            // max - min + 1
   unsigned long long* set;  // 2^set_bits slots, empty = ~0
   uint64_t set_mask;
-  uint32_t* counts;         // one per relative key of the plan's domain
+  uint32_t* counts;         // one per relative key of the plan's domain (not to_records)
   uint32_t dslot_off;       // record offset of the group's distinct count
-  uint32_t pad;
+  uint32_t to_records;      // the plan's HBM records are addressable by key (not radix):
+                            // composites use the record index, counts go straight to the records
 };
 
 // ---- batched execution (one persistent launch over many plans) ------------
@@ -377,7 +378,7 @@ hipError_t launch_having(const FinalParams& p, const HavingNode* nodes, uint32_t
 // record at D.dslot_off.  The set must hold at least twice the rows scanned.
 constexpr uint32_t kDistinctLdsKeys = 4096;  // domains up to this many groups count in LDS
 hipError_t launch_distinct_insert(const ScanParams& sp, const DistinctParams& D, uint64_t n_keys, uint32_t grid,
-                                  hipStream_t s);
+                                  hipStream_t s);  // n_keys: the relative key domain
 hipError_t launch_distinct_gather(const FinalParams& fp, const DistinctParams& D, const uint32_t* gids,
                                   const uint64_t* n_groups, uint64_t max_n, hipStream_t s);
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,

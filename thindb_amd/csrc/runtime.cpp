@@ -1425,33 +1425,43 @@ int32_t tdbg::rt::run_distinct(tdbg_plan* pl, Table* t, const ScanSet& S, const 
   const ColumnArena& a = it->second;
   unsigned __int128 dom = 1;
   for (uint32_t i = 0; i < sp.n_parts; i++) dom *= sp.parts[i].span;
-  if (dom > kDistinctDomainMax)
+  // a scan's records are addressable by key (dense index or the HBM hash
+  // table); radix leaves write them where their rows started, so radix
+  // plans count per relative key and gather afterwards
+  const bool to_records = !P.radix;
+  if (!to_records && dom > kDistinctDomainMax)
     return fail(TDBG_ERR_UNSUPPORTED, "COUNT(DISTINCT) over a key domain of more than 2^31 groups");
   DistinctParams D{};
+  D.to_records = to_records ? 1u : 0u;
   D.x.data = a.d_data;
   D.x.valid = a.d_valid;
   D.x.kind = kind_of(a.type);
   D.base = a.has_stats ? a.min_i : 0;
   D.span = a.has_stats ? (uint64_t)a.max_i - (uint64_t)a.min_i + 1 : 1;
-  if (D.span == 0 || dom * D.span >= ((unsigned __int128)1 << 63))
+  const unsigned __int128 groups = to_records ? (unsigned __int128)sp.g_cap : dom;  // composite's group index
+  if (D.span == 0 || groups * D.span >= ((unsigned __int128)1 << 63))
     return fail(TDBG_ERR_UNSUPPORTED, "COUNT(DISTINCT) composites wider than 63 bits");
   D.dslot_off = dd.dslot_off;
-  const uint64_t pairs = std::min<uint64_t>(std::max<uint64_t>(S.rows, 1), (uint64_t)(dom * D.span));
+  const uint64_t pairs = std::min<uint64_t>(std::max<uint64_t>(S.rows, 1), (uint64_t)(groups * D.span));
   uint64_t slots = 1024;
   while (slots < 2 * pairs) slots <<= 1;
   int32_t rc;
   if ((rc = ensure(pl, pl->dist_set, slots * 8))) return rc;
-  if ((rc = ensure(pl, pl->dist_counts, (uint64_t)dom * 4 + 256))) return rc;
   D.set = static_cast<unsigned long long*>(pl->dist_set.p);
   D.set_mask = slots - 1;
-  D.counts = static_cast<uint32_t*>(pl->dist_counts.p);
   HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(D.set), slots * 2, ~0u, s), "distinct set");
-  HIP_TRY(launch_fill_u32(D.counts, (uint64_t)dom, 0, s), "distinct counts");
+  if (!to_records) {
+    if ((rc = ensure(pl, pl->dist_counts, (uint64_t)dom * 4 + 256))) return rc;
+    D.counts = static_cast<uint32_t*>(pl->dist_counts.p);
+    HIP_TRY(launch_fill_u32(D.counts, (uint64_t)dom, 0, s), "distinct counts");
+  }
   // LDS counters (small domains) are flushed once per workgroup: fewer, longer-lived workgroups
   const uint32_t per_cu = (uint64_t)dom <= kDistinctLdsKeys ? 4u : 8u;
   const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(sp.n_tiles, (uint32_t)std::max(pl->ctx->n_cu, 8) * per_cu));
-  HIP_TRY(launch_distinct_insert(sp, D, (uint64_t)dom, grid, s), "distinct insert");
-  HIP_TRY(launch_distinct_gather(P.fp, D, gids, n_groups, max_groups, s), "distinct counts to groups");
+  // (the LDS counters need the domain below 2^32; larger domains never take them)
+  HIP_TRY(launch_distinct_insert(sp, D, (uint64_t)std::min<unsigned __int128>(dom, ~0u), grid, s), "distinct insert");
+  if (!to_records)
+    HIP_TRY(launch_distinct_gather(P.fp, D, gids, n_groups, max_groups, s), "distinct counts to groups");
   return TDBG_OK;
 }
 
