@@ -278,11 +278,65 @@ __device__ __forceinline__ void sort_block(const uint64_t* comps, uint32_t first
   for (uint32_t i = threadIdx.x; i < n_out; i += blockDim.x) pos_out[i] = sp[i];
 }
 
+// The same for n <= blockDim.x positions, one per thread in registers: the
+// compare-exchange steps within a wavefront (partner distance < 64) go
+// through shuffles, only the wider ones through LDS, so a 1024-position sort
+// waits at 20 barriers instead of 55.
+__device__ __forceinline__ void sort_block_reg(const uint64_t* comps, uint32_t first, uint32_t n_comps, uint64_t n_all,
+                                              const uint32_t* pos_in, uint32_t base, uint32_t n, uint32_t* pos_out,
+                                              uint32_t n_out, uint32_t* sp, uint64_t* sw) {
+  uint32_t m = 1;
+  while (m < n) m <<= 1;
+  const uint32_t i = threadIdx.x;
+  uint32_t p = 0xffffffffu;  // none: after every position
+  uint64_t w = ~0ull;
+  if (i < n) {
+    p = pos_in ? pos_in[i] : base + i;
+    w = first < n_comps ? comps[(uint64_t)first * n_all + p] : 0ull;
+  }
+  for (uint32_t k = 2; k <= m; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      uint32_t q;
+      uint64_t v;
+      if (j >= 64) {  // uniform
+        __syncthreads();
+        sp[i] = p;
+        sw[i] = w;
+        __syncthreads();
+        q = sp[i ^ j];
+        v = sw[i ^ j];
+      } else {
+        q = __shfl_xor(p, (int)j);
+        const uint32_t vh = __shfl_xor((uint32_t)(w >> 32), (int)j), vl = __shfl_xor((uint32_t)w, (int)j);
+        v = ((uint64_t)vh << 32) | vl;
+      }
+      // the lower index of a pair keeps the smaller element in an ascending
+      // run (i & k == 0), the larger in a descending one
+      const bool keep_min = ((i & j) == 0) == ((i & k) == 0);
+      bool take;
+      if (q == p) take = false;  // both none
+      else if (q == 0xffffffffu) take = !keep_min;
+      else if (p == 0xffffffffu) take = keep_min;
+      else take = keep_min ? pos_less_from(comps, first, n_comps, n_all, q, v, p, w)
+                           : pos_less_from(comps, first, n_comps, n_all, p, w, q, v);
+      if (take) {
+        p = q;
+        w = v;
+      }
+    }
+  }
+  __syncthreads();
+  sp[i] = p;
+  __syncthreads();
+  for (uint32_t t = threadIdx.x; t < n_out; t += blockDim.x) pos_out[t] = sp[t];
+}
+
 __global__ void __launch_bounds__(1024) small_sort_kernel(const uint64_t* comps, uint32_t n_comps, uint64_t n_all,
                                                           const uint32_t* pos_in, uint32_t n, uint32_t* pos_out) {
   __shared__ uint32_t sp[kSmallSort];
   __shared__ uint64_t sw[kSmallSort];
-  sort_block(comps, 0, n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
+  if (n <= blockDim.x) sort_block_reg(comps, 0, n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
+  else sort_block(comps, 0, n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
 }
 
 // The first word whose min and max (mm[2c], mm[2c + 1]) differ, or n_comps.
@@ -302,8 +356,12 @@ __global__ void __launch_bounds__(1024) chunk_sort_kernel(const uint64_t* comps,
   __shared__ uint64_t sw[kSmallSort];
   const uint32_t base = blockIdx.x * chunk;
   const uint32_t len = n - base < chunk ? n - base : chunk;
-  sort_block(comps, first_varying(mm, n_comps), n_comps, n_all, nullptr, base, len,
-             out + (uint64_t)blockIdx.x * keep, keep < len ? keep : len, sp, sw);
+  if (len <= blockDim.x)
+    sort_block_reg(comps, first_varying(mm, n_comps), n_comps, n_all, nullptr, base, len,
+                   out + (uint64_t)blockIdx.x * keep, keep < len ? keep : len, sp, sw);
+  else
+    sort_block(comps, first_varying(mm, n_comps), n_comps, n_all, nullptr, base, len,
+               out + (uint64_t)blockIdx.x * keep, keep < len ? keep : len, sp, sw);
 }
 
 // The candidates pos_in[0..n) sorted the same way.
@@ -312,7 +370,8 @@ __global__ void __launch_bounds__(1024) cand_sort_kernel(const uint64_t* comps, 
                                                          uint32_t* pos_out) {
   __shared__ uint32_t sp[kSmallSort];
   __shared__ uint64_t sw[kSmallSort];
-  sort_block(comps, first_varying(mm, n_comps), n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
+  if (n <= blockDim.x) sort_block_reg(comps, first_varying(mm, n_comps), n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
+  else sort_block(comps, first_varying(mm, n_comps), n_comps, n_all, pos_in, 0, n, pos_out, n, sp, sw);
 }
 
 // ---- per-workgroup top-k (k <= kTopMaxK) -----------------------------------------
