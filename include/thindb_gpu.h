@@ -84,7 +84,9 @@ enum tdbg_type {
  * count with no column (aggregate.zig:244-246).  COUNT(DISTINCT col)
  * (aggregate.zig:130 count_distinct; planned by v2_lowcard_group.zig:363-387,
  * folded by :1348-1418) counts each group's distinct non-NULL values of an
- * integer-family column: BIGINT, never NULL (0 when every value is NULL). */
+ * integer-family or float column (a float by its canonical f64 bits: one
+ * +0.0, one NaN; types.zig:593-600): BIGINT, never NULL (0 when every value
+ * is NULL). */
 enum tdbg_agg_op {
   TDBG_AGG_COUNT_STAR = 0,
   TDBG_AGG_COUNT = 1,

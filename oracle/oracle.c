@@ -832,7 +832,16 @@ static int distinct_counts(const cplan* p, wstate* m) {
       uint32_t gid = ws_find(m, p, pack_key(p, r));
       if (gid == EMPTY_GID) { free(pairs); return -1; } /* every passing row has its group */
       pairs[n].gid = gid;
-      pairs[n].v = load_i(c, r);
+      if (ag->is_float) { /* canonicalFloatBits of the f64 (types.zig:593-600, silo_group_core.zig:8052): one +0, one NaN */
+        double x = load_f(c, r);
+        uint64_t bits;
+        if (isnan(x)) bits = 0x7ff8000000000000ull;
+        else if (x == 0.0) bits = 0;
+        else memcpy(&bits, &x, 8);
+        pairs[n].v = (int64_t)bits;
+      } else {
+        pairs[n].v = load_i(c, r);
+      }
       n++;
     }
     qsort(pairs, n, sizeof(dpair), dpair_cmp);
@@ -921,9 +930,13 @@ static int compile(cplan* p, const tdbg_plan_desc* d, const or_table* t) {
     ag->nullable = t->cols[s->column_idx].validity != NULL;
     row_bytes += type_bytes(s->input_type);
     if (s->op == TDBG_AGG_COUNT) { ag->op = OP_COUNT_COL; ag->out_type = TDBG_BIGINT; continue; }
-    if (s->op == TDBG_AGG_COUNT_DISTINCT) { /* integer values only (distinctValueBits of an int) */
-      if (!key_type_bits(s->input_type)) { set_err("COUNT(DISTINCT) input type"); return TDBG_ERR_UNSUPPORTED; }
+    if (s->op == TDBG_AGG_COUNT_DISTINCT) { /* distinctValueBits: an int's bits, a float's canonical f64 bits */
+      if (!key_type_bits(s->input_type) && !is_float_type(s->input_type)) {
+        set_err("COUNT(DISTINCT) input type");
+        return TDBG_ERR_UNSUPPORTED;
+      }
       ag->op = OP_COUNT_DISTINCT;
+      ag->is_float = is_float_type(s->input_type);
       ag->out_type = TDBG_BIGINT;
       continue;
     }

@@ -151,8 +151,8 @@ def brute_force(table: Table, request: Request) -> Dict[tuple, list]:
             if f == "count":
                 vals.append(rows if a.col is None else len(xs))
                 continue
-            if f == "count_distinct":
-                vals.append(len(set(xs)))
+            if f == "count_distinct":  # floats: one +0.0 and one NaN (types.canonicalFloat)
+                vals.append(len({"nan" if x != x else (0.0 if x == 0 else x) for x in xs}))
                 continue
             if not xs:
                 vals.append(None)

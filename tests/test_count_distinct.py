@@ -78,6 +78,56 @@ def test_oracle_count_distinct_invariants(oracle_lib):
     assert sorted(got) == [(i, 1) for i in range(17)]
 
 
+def float_domain(rng):
+    """+0.0, -0.0, four NaNs (quiet and signalling payloads, both signs) and
+    45 ordinary values: 47 distinct values once canonicalised."""
+    nans = np.array([0x7FF8000000000000, 0x7FF8000000000123, 0xFFF8000000000000, 0x7FF0000000000001],
+                    dtype=np.uint64).view(np.float64)
+    return np.concatenate([[1.5, -2.25, 0.0, -0.0, 1e300, -1e-300, 3.0], nans, rng.uniform(-100, 100, 40)])
+
+
+def float_table(seed=6, n=40_000):
+    """k1 INT keys; d DOUBLE (nullable) and f FLOAT over small domains that
+    hold +0.0 and -0.0 and NaNs of several payloads and signs; w BIGINT
+    (nullable) over values spread across the whole int64 range (its span
+    does not fit a 63-bit composite); g BIGINT, 500 keys sparse in +-2^40."""
+    rng = np.random.default_rng(seed)
+    dvals = float_domain(rng)
+    fnans = np.array([0x7FC00000, 0x7FC00123, 0xFFC00000, 0x7F800001], dtype=np.uint32).view(np.float32)
+    fvals = np.concatenate([np.array([1.5, -2.25, 0.0, -0.0, 3.0e38, 1e-40], dtype=np.float32), fnans,
+                            rng.normal(0, 10, 30).astype(np.float32)])
+    wvals = np.concatenate([[-2 ** 63, 2 ** 63 - 1, 0, -1], rng.integers(-2 ** 62, 2 ** 62, 200, dtype=np.int64)])
+    gvals = rng.integers(-2 ** 40, 2 ** 40, 500, dtype=np.int64)
+    cols = {"k1": rng.integers(-20, 20, n).astype(np.int32), "d": dvals[rng.integers(0, len(dvals), n)],
+            "f": fvals[rng.integers(0, len(fvals), n)], "w": wvals[rng.integers(0, len(wvals), n)].astype(np.int64),
+            "g": gvals[rng.integers(0, len(gvals), n)]}
+    schema = [Column("k1", Type.INT), Column("d", Type.DOUBLE, True), Column("f", Type.FLOAT),
+              Column("w", Type.BIGINT, True), Column("g", Type.BIGINT)]
+    return Table(schema, cols, {"d": rng.random(n) > 0.15, "w": rng.random(n) > 0.1}, rg_rows=997)
+
+
+FLOAT_AGGS = [AggSpec("count", None, "c"), AggSpec("count_distinct", "d", "dd"), AggSpec("count_distinct", "f", "df"),
+              AggSpec("count_distinct", "w", "dw")]
+
+
+def test_oracle_count_distinct_floats_and_wide_values(oracle_lib):
+    """Float values count one +0.0 and one NaN (every NaN payload and sign is
+    one value), a FLOAT widens to f64; BIGINT values across the whole range
+    count exactly."""
+    t = float_table()
+    for keys, wf in ((["k1"], None), (["k1"], Predicate("w", "gt", 0)), (["g"], None), (["k1", "g"], None)):
+        req = Request(keys, FLOAT_AGGS, where_filter=wf)
+        want = brute_force(t, req)
+        got = rows_of(oracle_lib.run(t, req, dop=3))
+        assert len(got) == len(want)
+        for r in got:
+            assert list(r[len(keys):]) == want[tuple(r[:len(keys)])], (keys, r)
+    # the closed form over one group: 47 distinct values in the domain
+    d = float_domain(np.random.default_rng(0))
+    one = Table([Column("k", Type.INT), Column("d", Type.DOUBLE)], {"k": np.zeros(len(d), np.int32), "d": d})
+    assert rows_of(oracle_lib.run(one, Request(["k"], [AggSpec("count_distinct", "d", "x")]), dop=1)) == [(0, 47)]
+
+
 def test_count_distinct_schema_and_declines():
     from thindb_amd.exec import Declined, _having_postfix  # noqa: F401
     t = _table()

@@ -1,16 +1,16 @@
 """COUNT(DISTINCT col) on the GPU, through the C-ABI, against the oracle:
 the LDS direct table, two keys with filters, ORDER BY / LIMIT / HAVING over
 the count, radix partitioning with compacted and dense output, the
-high-cardinality workload, and the shapes that decline."""
+high-cardinality workload, float and full-range BIGINT values, and the
+batch that declines."""
 import numpy as np
 import pytest
 
 from helpers import brute_force, rows_of
-from test_count_distinct import AGGS, _table
+from test_count_distinct import AGGS, FLOAT_AGGS, _table, float_table
 from test_gpu_parity import check_vs_oracle, run_gpu
 from thindb_amd import gen, gpu
-from thindb_amd.exec import (AggSpec, And, Column, GpuBatchRun, Predicate, Request, SortSpec, Table,
-                             try_build_gpu_group)
+from thindb_amd.exec import AggSpec, And, Column, GpuBatchRun, Predicate, Request, SortSpec, Table
 from thindb_amd.gpu import Type
 
 pytestmark = pytest.mark.gpu
@@ -77,9 +77,32 @@ def test_count_distinct_sparse_domain(session, oracle_lib):
                                                        having_filter=Predicate("dw", "gt", 60)))
 
 
+def test_count_distinct_floats_and_wide_values(session, oracle_lib, monkeypatch):
+    """The wide form: DOUBLE and FLOAT values (one +0.0, one NaN) and BIGINT
+    values across the whole int64 range, over the LDS table, the HBM hash
+    table (a sparse second key) and radix partitioning."""
+    t = float_table()
+    for req in (Request(["k1"], FLOAT_AGGS),
+                Request(["k1"], FLOAT_AGGS, [SortSpec("dd", True), SortSpec("k1")], 7, 2,
+                        where_filter=Predicate("w", "gt", 0)),
+                Request(["g"], FLOAT_AGGS, having_filter=Predicate("df", "gte", 2))):
+        got, want, st = check_vs_oracle(session, oracle_lib, t, req)
+    assert st.global_mode == 2
+    b, _ = run_gpu(session, t, Request(["k1"], FLOAT_AGGS))
+    bf = brute_force(t, Request(["k1"], FLOAT_AGGS))
+    for r in rows_of(b):
+        assert list(r[1:]) == bf[(r[0],)]
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    h = gen.table("highcard", 600_000, seed=23)
+    aggs = [AggSpec("count", None, "c"), AggSpec("count_distinct", "w", "dw")]
+    for est in (0, 100_000_000):
+        got, want, st = check_vs_oracle(session, oracle_lib, h, Request(["user_id", "bucket"], aggs, est_groups=est),
+                                        dop=8)
+        assert st.global_mode == 3
+
+
 def test_count_distinct_declines(session):
-    """A float input declines at planning; a batch refuses the plan."""
+    """A batch refuses a COUNT(DISTINCT) plan."""
     t = _table()
-    assert try_build_gpu_group(session, t, Request(["k1"], [AggSpec("count_distinct", "d", "x")])) is None
     with pytest.raises(Exception):
         GpuBatchRun(session, [(t, Request(["k1"], AGGS))])

@@ -122,8 +122,6 @@ def test_declines_like_the_reference(oracle_lib):
         oracle_lib.run(t, Request(["k1"], [AggSpec("sum", "t", "x")]), 1)  # temporal SUM
     with pytest.raises(oracle_lib.Unsupported):
         oracle_lib.run(t, Request(["d"], [AggSpec("count", None, "c")]), 1)  # float key
-    with pytest.raises(oracle_lib.Unsupported):
-        oracle_lib.run(t, Request(["k1"], [AggSpec("count_distinct", "d", "x")]), 1)  # float COUNT(DISTINCT)
 
 
 def test_planted_mistakes_are_caught(oracle_lib):

@@ -297,6 +297,12 @@ struct DistinctParams {
   uint32_t dslot_off;       // record offset of the group's distinct count
   uint32_t to_records;      // the plan's HBM records are addressable by key (not radix):
                             // composites use the record index, counts go straight to the records
+  // WIDE: a composite that does not fit 63 bits (a float input, or a value span
+  // too wide): each slot holds the row index whose entry (group, value bits)
+  // it stands for; entries[2r], entries[2r + 1] belong to row r
+  uint32_t wide;
+  uint32_t is_float;        // WIDE: the value's canonical f64 bits (types.canonicalFloatBits)
+  uint64_t* entries;
 };
 
 // ---- batched execution (one persistent launch over many plans) ------------

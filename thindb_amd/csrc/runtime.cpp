@@ -250,8 +250,6 @@ int32_t validate(tdbg_plan* pl) {
     if (a.column_idx < 0) return fail(TDBG_ERR_INVALID, "aggregate without a column");
     if (!is_int_type(a.input_type) && !is_float_type(a.input_type))
       return fail(TDBG_ERR_UNSUPPORTED, "aggregate input type %u", a.input_type);
-    if (a.op == TDBG_AGG_COUNT_DISTINCT && !is_int_type(a.input_type))
-      return fail(TDBG_ERR_UNSUPPORTED, "COUNT(DISTINCT) over a float column");
     if ((a.input_type == TDBG_DATE || a.input_type == TDBG_DATETIME) && (a.op == TDBG_AGG_SUM || a.op == TDBG_AGG_AVG))
       return fail(TDBG_ERR_UNSUPPORTED, "temporal SUM/AVG");
     use((uint32_t)a.column_idx);
@@ -1341,7 +1339,7 @@ void tdbg_plan_destroy(tdbg_plan* pl) {
   if (pl->stream) (void)hipStreamSynchronize(pl->stream);
   for (auto* b : {&pl->state, &pl->tiles, &pl->err, &pl->gids, &pl->counts, &pl->sel, &pl->scratch, &pl->out, &pl->tmpl, &pl->nout,
                  &pl->rx_misc, &pl->rx_rows1, &pl->rx_rows2, &pl->rx_leaf, &pl->rx_words, &pl->having_nodes,
-                 &pl->having_gids, &pl->dist_set, &pl->dist_counts})
+                 &pl->having_gids, &pl->dist_set, &pl->dist_counts, &pl->dist_entries, &pl->having_leaf})
     release(pl, *b);
   for (auto& e : pl->ev) if (e) (void)hipEventDestroy(e);
   if (pl->h_in) (void)hipHostFree(pl->h_in);
@@ -1436,17 +1434,24 @@ int32_t tdbg::rt::run_distinct(tdbg_plan* pl, Table* t, const ScanSet& S, const 
   D.x.data = a.d_data;
   D.x.valid = a.d_valid;
   D.x.kind = kind_of(a.type);
+  D.is_float = is_float_type(a.type) ? 1u : 0u;
   D.base = a.has_stats ? a.min_i : 0;
   D.span = a.has_stats ? (uint64_t)a.max_i - (uint64_t)a.min_i + 1 : 1;
   const unsigned __int128 groups = to_records ? (unsigned __int128)sp.g_cap : dom;  // composite's group index
-  if (D.span == 0 || groups * D.span >= ((unsigned __int128)1 << 63))
-    return fail(TDBG_ERR_UNSUPPORTED, "COUNT(DISTINCT) composites wider than 63 bits");
+  // a composite key * span + rel when it fits 63 bits, else (a float, a
+  // wide span) entries of (group, value bits) behind the slots
+  D.wide = (D.is_float || D.span == 0 || groups * D.span >= ((unsigned __int128)1 << 63)) ? 1u : 0u;
   D.dslot_off = dd.dslot_off;
-  const uint64_t pairs = std::min<uint64_t>(std::max<uint64_t>(S.rows, 1), (uint64_t)(groups * D.span));
+  const uint64_t pairs = D.wide ? std::max<uint64_t>(S.rows, 1)
+                                : std::min<uint64_t>(std::max<uint64_t>(S.rows, 1), (uint64_t)(groups * D.span));
   uint64_t slots = 1024;
   while (slots < 2 * pairs) slots <<= 1;
   int32_t rc;
   if ((rc = ensure(pl, pl->dist_set, slots * 8))) return rc;
+  if (D.wide) {  // one entry per arena row a tile may hold
+    if ((rc = ensure(pl, pl->dist_entries, std::max<uint64_t>(t->next_row, 1) * 16 + 256))) return rc;
+    D.entries = static_cast<uint64_t*>(pl->dist_entries.p);
+  }
   D.set = static_cast<unsigned long long*>(pl->dist_set.p);
   D.set_mask = slots - 1;
   HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(D.set), slots * 2, ~0u, s), "distinct set");

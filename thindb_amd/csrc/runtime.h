@@ -217,7 +217,7 @@ struct tdbg_plan {
     void* p = nullptr;
     uint64_t bytes = 0;
   } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words,
-      having_nodes, having_gids, dist_set, dist_counts, having_leaf;
+      having_nodes, having_gids, dist_set, dist_counts, having_leaf, dist_entries;
   // pinned host staging: inputs (tiles, template) and the result the
   // columns of a returned tdbg_result view
   uint8_t* h_in = nullptr;

@@ -471,8 +471,6 @@ def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu
         c = table.column(a.col)
         if c.type not in _INT_TYPES and c.type not in _FLOAT_TYPES:
             raise Declined(f"aggregate input {a.col}")
-        if f == "count_distinct" and c.type not in _INT_TYPES:
-            raise Declined(f"COUNT(DISTINCT) over the float column {a.col}")
         op = {"count": AggOp.COUNT, "count_distinct": AggOp.COUNT_DISTINCT, "sum": AggOp.SUM, "avg": AggOp.AVG,
               "min": AggOp.MIN, "max": AggOp.MAX}[f]
         aggs.append((op, table.column_index(a.col), c.type, c.scale, c.nullable))
