@@ -329,6 +329,57 @@ int32_t tdbg_column_stage_encoded(tdbg_context* ctx, const tdbg_column_key* key,
                                   const void* payload, uint64_t payload_bytes, uint64_t n_rows, uint32_t has_nulls,
                                   const tdbg_block_stats* stats);
 int32_t tdbg_column_evict(tdbg_context* ctx, const tdbg_column_key* key);
+
+/* ---- segment files (src/storage/segment_reader.zig) --------------------- */
+/* A thinDB segment file (.dat, format version 17) read from a host image of
+ * the whole file (a read or an mmap; it must outlive the tdbg_segment).
+ * tdbg_segment_open parses the 32-byte header, the trailer and the footer as
+ * readSegment does (segment_reader.zig:1350-1455; string-table directory
+ * :1459-1512) and fails TDBG_ERR_INVALID on anything malformed
+ * (format.Error: BadMagic, CorruptFooter, ...) and TDBG_ERR_UNSUPPORTED on
+ * another format version.  `n_columns` is the table schema's column count,
+ * which the footer layout depends on. */
+typedef struct tdbg_segment tdbg_segment;
+typedef struct tdbg_segment_info {
+  uint64_t segment_id;
+  uint64_t row_count;
+  uint64_t schema_fingerprint;
+  uint32_t n_row_groups;
+  uint32_t n_columns;
+} tdbg_segment_info;
+/* One column block's header (format.ColumnBlockHeader, format.zig:203-231)
+ * and its footer Stats (format.zig:233-270: i128 min/max/sum as lo/hi
+ * halves, null_count). */
+typedef struct tdbg_segment_block {
+  uint32_t row_count;
+  uint32_t payload_bytes; /* uncompressed_size */
+  uint32_t stored_bytes;  /* compressed_size */
+  uint8_t compression;    /* 0 none, 1 zstd, 2 lz4 */
+  uint8_t has_nulls;
+  uint8_t int_streams;
+  uint8_t encoding;       /* TDBG_ENC_*, or 5 = segment dictionary (strings) */
+  uint64_t null_count;
+  uint64_t min_lo, min_hi, max_lo, max_hi, sum_lo, sum_hi;
+} tdbg_segment_block;
+int32_t tdbg_segment_open(const void* bytes, uint64_t n_bytes, uint32_t n_columns, tdbg_segment** out);
+void tdbg_segment_close(tdbg_segment* seg);
+int32_t tdbg_segment_get_info(const tdbg_segment* seg, tdbg_segment_info* out);
+int32_t tdbg_segment_block_info(const tdbg_segment* seg, uint32_t row_group_idx, uint32_t column_idx,
+                                tdbg_segment_block* out);
+/* The block's payload as thinDB's block cache holds it: decompressed (zstd or
+ * LZ4 block, through the system's libzstd.so.1 / liblz4.so.1) and, for an
+ * int_streams block, its integer arrays restored (readBlockRaw and
+ * restoreIntStreams, segment_reader.zig:1620-1716; int_streams.zig).
+ * `capacity` must hold payload_bytes.  Host only: no device is touched. */
+int32_t tdbg_segment_read_block(const tdbg_segment* seg, uint32_t row_group_idx, uint32_t column_idx, void* out,
+                                uint64_t capacity);
+/* Reads one block as tdbg_segment_read_block and stages it with
+ * tdbg_column_stage_encoded under {table_uid, the segment's id, row group,
+ * column} (decodeColumnMaybeCached, segment_reader.zig:115, filling the HBM
+ * column cache); an integer-family block with non-NULL rows carries its
+ * footer zone map.  String blocks are TDBG_ERR_UNSUPPORTED. */
+int32_t tdbg_segment_stage(tdbg_context* ctx, const tdbg_segment* seg, uint64_t table_uid, uint32_t row_group_idx,
+                           uint32_t column_idx, uint32_t type);
 /* Pre-sizes a table's HBM arenas for `rows` rows (a hint; arenas grow). */
 int32_t tdbg_table_reserve(tdbg_context* ctx, uint64_t table_uid, uint64_t rows);
 int32_t tdbg_table_drop(tdbg_context* ctx, uint64_t table_uid);

@@ -436,6 +436,27 @@ class GpuSession:
                 copied += vals.nbytes
         return copied
 
+    def stage_segment(self, table: Table, segment: "gpu.Segment", columns: Optional[Sequence[str]] = None) -> int:
+        """Stages the blocks of `columns` (default: every column) of one
+        segment file of `table` straight from the file image: each block is
+        decompressed and restored on the host and expanded in HBM.  Returns
+        the number of blocks staged."""
+        info = segment.info()
+        names = list(columns) if columns is not None else [c.name for c in table.schema]
+        if not any(k[0] == table.uid for k in self._staged):
+            self.ctx.reserve(table.uid, table.n_rows + len(table.row_groups) * 256)
+        n = 0
+        for rg in range(info.n_row_groups):
+            for name in names:
+                ci = table.column_index(name)
+                key = (table.uid, info.segment_id, rg, ci)
+                if key in self._staged:
+                    continue
+                self.ctx.stage_segment(segment, table.uid, rg, ci, table.column(name).type)
+                self._staged.add(key)
+                n += 1
+        return n
+
     def evict_table(self, table: Table) -> None:
         self.ctx.drop_table(table.uid)
         self._staged = {k for k in self._staged if k[0] != table.uid}

@@ -146,6 +146,35 @@ class ColumnKey(C.Structure):
                 ("row_group_idx", C.c_uint32), ("column_idx", C.c_uint32)]
 
 
+class SegmentInfo(C.Structure):
+    _fields_ = [("segment_id", C.c_uint64), ("row_count", C.c_uint64), ("schema_fingerprint", C.c_uint64),
+                ("n_row_groups", C.c_uint32), ("n_columns", C.c_uint32)]
+
+
+class SegmentBlock(C.Structure):
+    _fields_ = [("row_count", C.c_uint32), ("payload_bytes", C.c_uint32), ("stored_bytes", C.c_uint32),
+                ("compression", C.c_uint8), ("has_nulls", C.c_uint8), ("int_streams", C.c_uint8),
+                ("encoding", C.c_uint8), ("null_count", C.c_uint64), ("min_lo", C.c_uint64), ("min_hi", C.c_uint64),
+                ("max_lo", C.c_uint64), ("max_hi", C.c_uint64), ("sum_lo", C.c_uint64), ("sum_hi", C.c_uint64)]
+
+    @staticmethod
+    def _i128(lo: int, hi: int) -> int:
+        v = (hi << 64) | lo
+        return v - (1 << 128) if v >> 127 else v
+
+    @property
+    def min(self) -> int:
+        return self._i128(self.min_lo, self.min_hi)
+
+    @property
+    def max(self) -> int:
+        return self._i128(self.max_lo, self.max_hi)
+
+    @property
+    def sum(self) -> int:
+        return self._i128(self.sum_lo, self.sum_hi)
+
+
 class BlockStats(C.Structure):
     _fields_ = [("min_i", C.c_int64), ("max_i", C.c_int64), ("min_f", C.c_double), ("max_f", C.c_double),
                 ("has_minmax", C.c_uint32), ("pad", C.c_uint32)]
@@ -228,6 +257,12 @@ EXPORTS = {
     "tdbg_column_stage": (C.c_int32, [C.c_void_p, C.POINTER(ColumnKey), C.c_uint32, C.c_void_p, C.c_void_p,
                                       C.c_uint64, C.POINTER(BlockStats)]),
     "tdbg_column_evict": (C.c_int32, [C.c_void_p, C.POINTER(ColumnKey)]),
+    "tdbg_segment_open": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tdbg_segment_close": (None, [C.c_void_p]),
+    "tdbg_segment_get_info": (C.c_int32, [C.c_void_p, C.POINTER(SegmentInfo)]),
+    "tdbg_segment_block_info": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(SegmentBlock)]),
+    "tdbg_segment_read_block": (C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint64]),
+    "tdbg_segment_stage": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
     "tdbg_table_reserve": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "tdbg_table_drop": (C.c_int32, [C.c_void_p, C.c_uint64]),
     "tdbg_plan_create": (C.c_int32, [C.c_void_p, C.POINTER(PlanDesc), C.POINTER(C.c_void_p)]),
@@ -334,6 +369,11 @@ class Context:
                                                   buf.ctypes.data_as(C.c_void_p), len(buf), n_rows,
                                                   1 if has_nulls else 0, C.byref(st) if st is not None else None))
 
+    def stage_segment(self, segment: "Segment", table_uid: int, rg_idx: int, column_idx: int, type_: Type) -> None:
+        """Reads one block of a segment file and stages it (cache.Key =
+        {table_uid, the segment's id, rg_idx, column_idx})."""
+        check(self._lib.tdbg_segment_stage(self._h, segment.handle, table_uid, rg_idx, column_idx, int(type_)))
+
     def evict(self, table_uid: int, segment_id: int, rg_idx: int, column_idx: int) -> None:
         key = ColumnKey(table_uid, segment_id, rg_idx, column_idx)
         check(self._lib.tdbg_column_evict(self._h, C.byref(key)))
@@ -435,6 +475,51 @@ def filter_node(t: tuple) -> FilterNodeC:
     elif kind in (HavingKind.AND, HavingKind.OR):
         f.arity = int(t[1])
     return f
+
+
+class Segment:
+    """A thinDB segment file (.dat) parsed from a host image of the file
+    (bytes, or an np.memmap of it); the image is kept alive with it."""
+
+    def __init__(self, image, n_columns: int):
+        lib = load()
+        self._img = np.frombuffer(image, dtype=np.uint8) if not isinstance(image, np.ndarray) else image
+        h = C.c_void_p()
+        check(lib.tdbg_segment_open(self._img.ctypes.data_as(C.c_void_p), self._img.nbytes, n_columns, C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def info(self) -> SegmentInfo:
+        out = SegmentInfo()
+        check(self._lib.tdbg_segment_get_info(self._h, C.byref(out)))
+        return out
+
+    def block(self, rg_idx: int, column_idx: int) -> SegmentBlock:
+        out = SegmentBlock()
+        check(self._lib.tdbg_segment_block_info(self._h, rg_idx, column_idx, C.byref(out)))
+        return out
+
+    def read_block(self, rg_idx: int, column_idx: int) -> bytes:
+        """The block's payload as thinDB's block cache holds it."""
+        n = self.block(rg_idx, column_idx).payload_bytes
+        buf = np.empty(max(n, 1), dtype=np.uint8)
+        check(self._lib.tdbg_segment_read_block(self._h, rg_idx, column_idx, buf.ctypes.data_as(C.c_void_p), n))
+        return buf[:n].tobytes()
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.tdbg_segment_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Plan:
