@@ -163,3 +163,100 @@ def keyed_table(seed: int, n: int, card: int) -> Table:
     value = np.empty(n, np.int64)
     lib().tdbg_gen_keyed(seed, card, 0, n, _ptr(key, C.c_int32), _ptr(value, C.c_int64), _threads())
     return Table(BATCHED_SCHEMA, {"key": key, "value": value})
+
+
+# ---- segment files -------------------------------------------------------------------------
+# A table written as thinDB segment files (format version 17), for measuring
+# ingestion: the header, row groups of column blocks and the footer as the
+# reference's writer lays them out (src/storage/segment_writer.zig:90-99,
+# :359-390, :642-680, block header :1196-1250).  Integer columns take the
+# frame-of-reference body when it is narrower than the native width (base =
+# min, codes = value - min; tryEncodeFor, :1254-1290), else raw; floats are
+# raw; a block keeps its zstd frame only when that is smaller (appendBlock).
+
+def _zstd():
+    z = C.CDLL("libzstd.so.1")
+    z.ZSTD_compressBound.restype = C.c_size_t
+    z.ZSTD_compressBound.argtypes = [C.c_size_t]
+    z.ZSTD_compress.restype = C.c_size_t
+    z.ZSTD_compress.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int]
+    z.ZSTD_isError.restype = C.c_uint
+    z.ZSTD_isError.argtypes = [C.c_size_t]
+    return z
+
+
+def _float_order_key(f: float) -> int:
+    """encodeFloatOrder (src/storage/format.zig:301-309)."""
+    bits = int(np.array([0.0 if f == 0 else f], dtype=np.float64).view(np.uint64)[0])
+    return (~bits) & (2 ** 64 - 1) if bits >> 63 else bits | (1 << 63)
+
+
+def _segment_block(col: Column, v: np.ndarray, zstd, level: int):
+    """(block bytes, (min, max, sum, null_count)) of one non-nullable block."""
+    if col.type in (Type.FLOAT, Type.DOUBLE):
+        enc, payload = 0, np.ascontiguousarray(v).tobytes()
+        live = v[~np.isnan(v)]
+        mn = _float_order_key(float(live.min())) if len(live) else 0
+        mx = _float_order_key(float(live.max())) if len(live) else 0
+        sm = int(np.array([v.astype(np.float64).sum()]).view(np.int64)[0])
+    else:
+        lo, hi = int(v.min()), int(v.max())
+        span = hi - lo
+        width = 1 if span < 1 << 8 else 2 if span < 1 << 16 else 4 if span < 1 << 32 else 8
+        mn, mx, sm = lo, hi, int(v.astype(np.int64).sum(dtype=np.int64))
+        if width < v.dtype.itemsize:
+            codes = (v.astype(np.int64) - lo).astype(np.dtype(f"<u{width}"))
+            enc = 1
+            payload = lo.to_bytes(16, "little", signed=True) + bytes([width, 0, 0, 0]) + codes.tobytes()
+        else:
+            enc, payload = 0, np.ascontiguousarray(v).tobytes()
+    comp, stored = 0, payload
+    if zstd is not None:
+        cap = zstd.ZSTD_compressBound(len(payload))
+        out = C.create_string_buffer(cap)
+        n = zstd.ZSTD_compress(out, cap, payload, len(payload), level)
+        if not zstd.ZSTD_isError(n) and n < len(payload):
+            comp, stored = 1, out.raw[:n]
+    head = int(comp).to_bytes(1, "little") + b"\x00" + bytes([enc, 0]) + len(payload).to_bytes(4, "little") + \
+        len(stored).to_bytes(4, "little")
+    return head + stored, (mn, mx, sm, 0)
+
+
+def segment_images(t: Table, compression: str = "zstd", level: int = 3, threads: int = 0) -> Dict[int, bytes]:
+    """Every segment of `t` (no NULLs) as a file image, by segment id."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    zstd = _zstd() if compression == "zstd" else None
+    by_seg: Dict[int, list] = {}
+    for rg in t.row_groups:
+        by_seg.setdefault(rg.segment_id, []).append(rg)
+
+    def row_group(rg):
+        return [_segment_block(c, t.columns[c.name][rg.start:rg.start + rg.n_rows], zstd, level) for c in t.schema]
+
+    out = {}
+    with ThreadPoolExecutor(threads or _threads()) as ex:
+        for seg_id, rgs in by_seg.items():
+            blocks = list(ex.map(row_group, rgs))
+            n = sum(rg.n_rows for rg in rgs)
+            img = bytearray(b"tDBS" + (17).to_bytes(2, "little") + bytes(2) + bytes(8) +
+                            int(seg_id).to_bytes(8, "little") + n.to_bytes(8, "little"))
+            footer = bytearray(len(rgs).to_bytes(4, "little"))
+            for rg, cols in zip(rgs, blocks):
+                off = len(img)
+                img += rg.n_rows.to_bytes(4, "little") + bytes(4)
+                offs = []
+                for b, _ in cols:
+                    offs.append((len(img), len(b)))
+                    img += b
+                footer += off.to_bytes(8, "little") + (len(img) - off).to_bytes(4, "little") + \
+                    rg.n_rows.to_bytes(4, "little")
+                for _, (mn, mx, sm, nulls) in cols:
+                    footer += b"".join(x.to_bytes(16, "little", signed=True) for x in (mn, mx, sm))
+                    footer += nulls.to_bytes(8, "little")
+                footer += b"".join(o.to_bytes(8, "little") for o, _ in offs)
+                footer += b"".join(l.to_bytes(4, "little") for _, l in offs)
+            footer += bytes(4)  # no string tables
+            img += footer + (len(footer) + 8).to_bytes(4, "little") + b"tDBS"
+            out[seg_id] = bytes(img)
+    return out
