@@ -192,3 +192,36 @@ def test_block_bytes_header():
     comp, flags, enc, res, usize, csize = struct.unpack_from("<BBBBII", b)
     assert (comp, flags, enc, res) == (2, 3, 1, 0) and csize == len(b) - 12
     assert usize == len(payload_of(Type.INT, v, np.ones(300, bool), True, "for")[0])
+
+
+def test_segment_survives_corruption():
+    """Random byte flips and truncations of a valid file: opening and
+    reading either succeed or fail with an error status; every offset the
+    reader follows is checked first (a crash would end the test process)."""
+    rng = np.random.default_rng(5)
+    v = rng.integers(-50, 50, 2000).astype(np.int64)
+    base = write_segment([SegmentColumn(Type.BIGINT, v, rng.random(2000) > 0.1, True,
+                                        [BlockSpec("for", "zstd", "delta", True), BlockSpec("rle", "lz4", "frame"),
+                                         BlockSpec("raw", "none", "plain")]),
+                          SegmentColumn(Type.INT, v.astype(np.int32), specs=[BlockSpec("raw", "zstd", "frame")])],
+                         9, 512)
+    outcomes = {"ok": 0, "error": 0}
+    for trial in range(400):
+        img = bytearray(base)
+        if trial % 5 == 4:
+            img = img[:int(rng.integers(0, len(img)))]
+        else:
+            for _ in range(int(rng.integers(1, 8))):
+                img[int(rng.integers(0, len(img)))] = int(rng.integers(0, 256))
+        try:
+            seg = gpu.Segment(bytes(img), 2)
+            info = seg.info()
+            for rg in range(info.n_row_groups):
+                for ci in range(2):
+                    seg.read_block(rg, ci)
+            seg.close()
+            outcomes["ok"] += 1
+        except gpu.GpuError as e:
+            assert e.status in (gpu.Status.INVALID, gpu.Status.UNSUPPORTED), e
+            outcomes["error"] += 1
+    assert outcomes["error"] > 0 and outcomes["ok"] > 0
