@@ -71,15 +71,16 @@ HAVING = {
 
 
 # --distinct: GROUP BY bucket, COUNT(DISTINCT user_id) over the high-cardinality table
-DISTINCT_DESC = "GROUP BY bucket COUNT(*), COUNT(DISTINCT user_id) top-10 by the distinct count, 200M rows"
+DISTINCT_DESC = "GROUP BY bucket COUNT(*), COUNT(DISTINCT {col}) top-10 by the distinct count, 200M rows"
 DISTINCT_BYTES_PER_ROW = 9  # bucket 1 + user_id 8
 
 
-def make_request(workload: str, having: bool = False, distinct: bool = False, where_tree: bool = False):
+def make_request(workload: str, having: bool = False, distinct: bool = False, where_tree: bool = False,
+                 distinct_col: str = "user_id"):
     from thindb_amd import gen
     from thindb_amd.exec import And, AggSpec, Not, Or, Predicate, Request, SortSpec
     if distinct:
-        return Request(["bucket"], [AggSpec("count", None, "c"), AggSpec("count_distinct", "user_id", "u")],
+        return Request(["bucket"], [AggSpec("count", None, "c"), AggSpec("count_distinct", distinct_col, "u")],
                        [SortSpec("u", True)], 10)
     if workload == "filtered":
         where = And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI)))
@@ -458,6 +459,8 @@ def main() -> int:
                     help="filtered/plain/highcard: add the workload's HAVING (bench.HAVING)")
     ap.add_argument("--distinct", action="store_true",
                     help="highcard: GROUP BY bucket, COUNT(DISTINCT user_id) instead of the headline query")
+    ap.add_argument("--distinct-col", default="user_id", choices=["user_id", "w"],
+                    help="--distinct over user_id (BIGINT) or w (DOUBLE: the wide, row-entry form)")
     ap.add_argument("--where-tree", action="store_true",
                     help="filtered: the WHERE as NOT (ts < lo OR ts >= hi), evaluated as a tree")
     ap.add_argument("--from-segments", action="store_true",
@@ -502,13 +505,13 @@ def main() -> int:
     if args.distinct and args.workload != "highcard":
         log("--distinct runs over the highcard workload")
         return 2
-    req = make_request(args.workload, args.having, args.distinct, args.where_tree)
+    req = make_request(args.workload, args.having, args.distinct, args.where_tree, args.distinct_col)
     if args.where_tree:
         desc = f"{desc}, WHERE NOT (ts < lo OR ts >= hi)"
     if args.having:
         desc = f"{desc} {HAVING[args.workload]}"
     if args.distinct:
-        desc, bytes_per_row = DISTINCT_DESC, DISTINCT_BYTES_PER_ROW
+        desc, bytes_per_row = DISTINCT_DESC.format(col=args.distinct_col), DISTINCT_BYTES_PER_ROW
 
     sess = GpuSession(0)
     q = try_build_gpu_group(sess, table, req)
