@@ -225,3 +225,32 @@ def test_segment_survives_corruption():
             assert e.status in (gpu.Status.INVALID, gpu.Status.UNSUPPORTED), e
             outcomes["error"] += 1
     assert outcomes["error"] > 0 and outcomes["ok"] > 0
+
+
+def test_segment_parser_under_asan(tmp_path):
+    """The parser built for the host with AddressSanitizer and UBSan
+    (tests/native/segment_fuzz.cpp; the device staging call stubbed) reads
+    4,000 corrupted and truncated copies of a segment file without a
+    sanitizer report."""
+    import shutil
+    import subprocess
+    from pathlib import Path
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = Path(__file__).resolve().parents[1]
+    rng = np.random.default_rng(5)
+    v = rng.integers(-50, 50, 2000).astype(np.int64)
+    img = write_segment([SegmentColumn(Type.BIGINT, v, rng.random(2000) > 0.1, True,
+                                       [BlockSpec("for", "zstd", "delta", True), BlockSpec("rle", "lz4", "frame"),
+                                        BlockSpec("raw", "none", "plain")]),
+                         SegmentColumn(Type.INT, v.astype(np.int32), specs=[BlockSpec("raw", "zstd", "frame")])],
+                        9, 512)
+    (tmp_path / "seg.dat").write_bytes(img)
+    exe = tmp_path / "fuzz"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    f"-I{root / 'thindb_amd' / 'csrc'}", str(root / "thindb_amd" / "csrc" / "segment.cpp"),
+                    str(root / "tests" / "native" / "segment_fuzz.cpp"), "-o", str(exe), "-ldl"], check=True)
+    out = subprocess.run([str(exe), str(tmp_path / "seg.dat"), "4000"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert "ERROR: AddressSanitizer" not in out.stderr and "runtime error" not in out.stderr
