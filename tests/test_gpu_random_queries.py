@@ -81,3 +81,54 @@ def test_random_queries_vs_oracle(session, oracle_lib, seed):
         check_vs_oracle(session, oracle_lib, t, req)
         ran += 1
     assert declined <= 3 and ran >= 27, (declined, ran)
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_random_batch_vs_oracle(session, oracle_lib, seed):
+    """24 random queries (no LIMIT / OFFSET / COUNT(DISTINCT), which a batch
+    declines) in one batched launch, each against the oracle."""
+    from dataclasses import replace
+
+    from helpers import assert_same_rows, rows_of
+    from test_gpu_parity import REL, float_cols_of
+    from thindb_amd.exec import GpuBatchRun
+    rng = np.random.default_rng(200 + seed)
+    t = random_table(rng, int(rng.integers(5_000, 30_000)), rg_rows=int(rng.choice([512, 1000, 4096])), nulls=True)
+    reqs = []
+    while len(reqs) < 24:
+        r = _request(rng, t)
+        aggs = [a for a in r.aggs if a.func != "count_distinct"] or [AggSpec("count", None, "a0")]
+        names = [a.as_ for a in aggs]
+        having = r.having_filter if r.having_filter is not None and r.having_filter.col in names else None
+        reqs.append(replace(r, aggs=aggs, order_specs=(), limit=0, offset=0, having_filter=having))
+    print(f"[random batch] seed {seed}: {len(reqs)} queries", flush=True)
+    run = GpuBatchRun(session, [(t, r) for r in reqs])
+    try:
+        for r, b in zip(reqs, run.execute()):
+            want = rows_of(oracle_lib.run(t, r, 4))
+            assert_same_rows(rows_of(b), want, float_cols_of(t, r), REL, len(r.group_cols))
+    finally:
+        run.close()
+
+
+def test_random_radix_vs_oracle(session, oracle_lib, monkeypatch):
+    """Random queries over high-cardinality keys through radix partitioning
+    (compacted and dense leaf output)."""
+    from dataclasses import replace
+    monkeypatch.setenv("THINDB_GPU_RADIX", "1")
+    rng = np.random.default_rng(300)
+    t = random_table(rng, 40_000, rg_rows=4096, nulls=True)
+    ran = 0
+    for case in range(16):
+        keys = [["i"], ["k1", "i"], ["t"], ["i", "k2"], ["s", "i"]][case % 5]
+        r = replace(_request(rng, t), group_cols=keys, est_groups=int(rng.choice([0, 100_000_000])))
+        r = replace(r, order_specs=tuple(s for s in r.order_specs if s.col not in ("k1", "k2", "s", "i", "t"))
+                    + tuple(SortSpec(k, False) for k in keys) if r.order_specs else ())
+        q = try_build_gpu_group(session, t, r)
+        if q is None:
+            continue
+        q.deinit()
+        print(f"[random radix] case {case}: {r}", flush=True)
+        check_vs_oracle(session, oracle_lib, t, r)
+        ran += 1
+    assert ran >= 14
