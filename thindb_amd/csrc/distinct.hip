@@ -86,6 +86,12 @@ __global__ void __launch_bounds__(256) distinct_insert_kernel(const ScanParams P
           group = (uint64_t)(rec - P.g_state) / P.g_rec;
         }
         const unsigned long long c = group * D.span + rel;
+        if (D.bitmap) {  // a dense composite range: one bit per pair, set once
+          const uint32_t bit = 1u << (c & 31u);
+          if (!(atomicOr(reinterpret_cast<uint32_t*>(D.set) + (c >> 5), bit) & bit))
+            distinct_count<LDS_COUNTS, TO_RECORDS>(D, lcnt, key[j], rec);
+          continue;
+        }
         // at least twice as many slots as rows scanned: an empty slot ends
         // every probe
         uint64_t h = mix64(c) & D.set_mask;

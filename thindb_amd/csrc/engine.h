@@ -302,6 +302,7 @@ struct DistinctParams {
   // it stands for; entries[2r], entries[2r + 1] belong to row r
   uint32_t wide;
   uint32_t is_float;        // WIDE: the value's canonical f64 bits (types.canonicalFloatBits)
+  uint32_t bitmap;          // composites index a bit array (`set` as 32-bit words), not hash slots
   uint64_t* entries;
 };
 

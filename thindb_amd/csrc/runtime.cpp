@@ -1446,15 +1446,22 @@ int32_t tdbg::rt::run_distinct(tdbg_plan* pl, Table* t, const ScanSet& S, const 
                                 : std::min<uint64_t>(std::max<uint64_t>(S.rows, 1), (uint64_t)(groups * D.span));
   uint64_t slots = 1024;
   while (slots < 2 * pairs) slots <<= 1;
+  // a composite range small enough for a bit array (up to 2^33 bits, and
+  // no larger than the hash set) takes one bit per pair: no probing, and a
+  // footprint the device's last-level cache can hold
+  const unsigned __int128 bits = groups * D.span;
+  D.bitmap = (!D.wide && bits <= ((unsigned __int128)1 << 33) && (uint64_t)(bits / 8) <= slots * 8) ? 1u : 0u;
+  const uint64_t words = D.bitmap ? (uint64_t)((bits + 31) / 32) : 0;
   int32_t rc;
-  if ((rc = ensure(pl, pl->dist_set, slots * 8))) return rc;
+  if ((rc = ensure(pl, pl->dist_set, D.bitmap ? words * 4 + 256 : slots * 8))) return rc;
   if (D.wide) {  // one entry per arena row a tile may hold
     if ((rc = ensure(pl, pl->dist_entries, std::max<uint64_t>(t->next_row, 1) * 16 + 256))) return rc;
     D.entries = static_cast<uint64_t*>(pl->dist_entries.p);
   }
   D.set = static_cast<unsigned long long*>(pl->dist_set.p);
   D.set_mask = slots - 1;
-  HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(D.set), slots * 2, ~0u, s), "distinct set");
+  if (D.bitmap) HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(D.set), words, 0u, s), "distinct bit array");
+  else HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(D.set), slots * 2, ~0u, s), "distinct set");
   if (!to_records) {
     if ((rc = ensure(pl, pl->dist_counts, (uint64_t)dom * 4 + 256))) return rc;
     D.counts = static_cast<uint32_t*>(pl->dist_counts.p);
