@@ -9,7 +9,9 @@
 // key * span + (x - base) into an open-addressing set in HBM with one
 // compare-and-swap; the insertion that finds the slot empty adds one to the
 // group's counter, so every (group, value) pair is counted exactly once.
-// NULL values are never inserted (a NULL is not a counted value).  Float
+// NULL values are never inserted (a NULL is not a counted value).  A dense
+// composite range (at most 2^33 pairs) sets one bit per pair in a bit array
+// instead, counting the pair whose bit was clear.  Float
 // values, and integer spans whose composite would not fit 63 bits, take the
 // wide form: the set holds row indices of staged (group, value bits) entries.
 #include "order_device.h"
