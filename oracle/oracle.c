@@ -24,9 +24,16 @@
  *     MIN/MAX);
  *   - the worker merge: mergeScalars, :1938-1999 (i128 add, f64 add, min/max
  *     with the `had` rule);
- *   - ORDER BY: SortCtx.lessThan/compareBySpec, :2198-2271 (ties by gid,
- *     BIGINT SUM by its wrapped value, AVG by the unscaled mean, floats by
- *     types.floatOrder, src/types.zig:579-587);
+ *   - ORDER BY: the silo's result order, rowOrderBefore/compareOrderKey/
+ *     compareAggregateValue, src/exec/v2_shape_group_topn.zig:1327-1395
+ *     (NULL keys and all-NULL aggregates first ascending, BIGINT SUM by its
+ *     wrapped value, AVG by the emitted f64(sum)/n/10^scale, floats by
+ *     types.floatOrder, src/types.zig:579-587; ties broken by the packed
+ *     group key, so the order is total).  The low-cardinality core sorts by
+ *     the same items (SortCtx.lessThan/compareBySpec, v2_lowcard_group.zig:
+ *     2198-2271, which never sees a NULL aggregate: it declines nullable
+ *     inputs, :319-330) but breaks ties by gid, an order its parallel merge
+ *     leaves unspecified; the packed key is one order it admits;
  *   - WHERE trees (AND / OR / NOT / IS [NOT] NULL over `col op literal`
  *     leaves): evaluatePredicate, predicate.zig:1727-1790;
  *   - COUNT(DISTINCT): the (group, value) composites of rows with a non-NULL
@@ -563,12 +570,34 @@ static int64_t unpack_part(u128 key, const key_part* kp) {
   return ((int64_t)(raw << sh)) >> sh;
 }
 
+/* One ORDER BY item, ascending: compareOrderKey / compareAggregateValue
+ * (src/exec/v2_shape_group_topn.zig:1337-1395).  A NULL key part, and an
+ * all-NULL SUM/AVG/MIN/MAX (aggIsNull :1411-1423: a nullable input with no
+ * non-null value), orders first; the caller flips the result for DESC, so
+ * it is last there.  Values compare as emitted: a BIGINT SUM by its wrapped
+ * value (integerAggValue :1446-1452), a DECIMAL SUM as i128, AVG as
+ * f64(sum)/f64(n)/10^scale (avgValue :1436-1444; a FLOAT/DOUBLE input sum/n),
+ * floats under types.floatOrder (src/types.zig:579-587: -0 = +0, NaN last,
+ * as the rank prefix OrderPrefix.ofFloat :1219-1226 orders it). */
+/* Planted mistakes (tests only: each must make a comparator test fail, so
+ * the tests that pin this comparator are shown to catch it).  0 = none. */
+enum {
+  PLANT_NONE = 0,
+  PLANT_DECIMAL_LOW_WORD = 1, /* a DECIMAL128 SUM compared by its low 64 bits */
+  PLANT_AVG_TRUNC = 2,        /* an integer AVG compared by sum / n in integer division */
+  PLANT_NULL_AS_ZERO = 3,     /* an all-NULL aggregate compared as a zero value */
+  PLANT_TIES_BY_GID = 4,      /* ties broken by first appearance instead of the packed key */
+  PLANT_NAN_FIRST = 5,        /* NaN ordered below every number */
+  PLANT_TIES_SIGNED = 6       /* ties broken by the key parts' signed values, not their bits */
+};
+static int g_plant = PLANT_NONE;
+void oracle_plant(int which) { g_plant = which; }
+
 static int compare_spec(const tdbg_order_spec* sp, uint32_t a, uint32_t b) {
   const cplan* p = g_sort->p;
   const wstate* m = g_sort->m;
   if (sp->target < p->n_parts) {
     const key_part* kp = &p->parts[sp->target];
-    /* NULL orders first (the caller flips it last for DESC) */
     int an = part_is_null(m->keys[a], kp), bn = part_is_null(m->keys[b], kp);
     if (an != bn) return an ? -1 : 1;
     if (an) return 0;
@@ -579,25 +608,71 @@ static int compare_spec(const tdbg_order_spec* sp, uint32_t a, uint32_t b) {
   uint32_t i = sp->target - p->n_parts;
   const agg_plan* ag = &p->aggs[i];
   size_t oa = (size_t)a * p->n_aggs + i, ob = (size_t)b * p->n_aggs + i;
+  if ((ag->op == OP_SUM || ag->op == OP_AVG || ag->op == OP_MIN || ag->op == OP_MAX) && g_plant != PLANT_NULL_AS_ZERO) {
+    int an = ag->nullable && m->ns[oa] == 0, bn = ag->nullable && m->ns[ob] == 0;
+    if (an || bn) return an && bn ? 0 : (an ? -1 : 1);
+  }
   switch (ag->op) {
     case OP_COUNT_STAR: return m->counts[a] < m->counts[b] ? -1 : (m->counts[a] > m->counts[b]);
     case OP_COUNT_COL: case OP_COUNT_DISTINCT: return m->ns[oa] < m->ns[ob] ? -1 : (m->ns[oa] > m->ns[ob]);
     case OP_SUM: case OP_MIN: case OP_MAX:
-      if (ag->is_float) return float_order(m->fslots[oa], m->fslots[ob]);
+      if (ag->is_float) {
+        if (g_plant == PLANT_NAN_FIRST && (isnan(m->fslots[oa]) || isnan(m->fslots[ob])))
+          return isnan(m->fslots[oa]) && isnan(m->fslots[ob]) ? 0 : (isnan(m->fslots[oa]) ? -1 : 1);
+        return float_order(m->fslots[oa], m->fslots[ob]);
+      }
+      if (g_plant == PLANT_DECIMAL_LOW_WORD && ag->out_type == TDBG_DECIMAL128) {
+        uint64_t x = (uint64_t)m->slots[oa], y = (uint64_t)m->slots[ob];
+        return x < y ? -1 : (x > y);
+      }
       if (ag->out_type == TDBG_BIGINT) {
         int64_t x = (int64_t)(uint64_t)m->slots[oa], y = (int64_t)(uint64_t)m->slots[ob];
         return x < y ? -1 : (x > y);
       }
       return m->slots[oa] < m->slots[ob] ? -1 : (m->slots[oa] > m->slots[ob]);
-    default: { /* AVG: unscaled mean, 0 for an empty group */
+    default: { /* AVG, as emitted */
       uint64_t an = m->ns[oa], bn = m->ns[ob];
-      double af = an == 0 ? 0 : (ag->is_float ? m->fslots[oa] / (double)an : (double)m->slots[oa] / (double)an);
-      double bf = bn == 0 ? 0 : (ag->is_float ? m->fslots[ob] / (double)bn : (double)m->slots[ob] / (double)bn);
+      if (g_plant == PLANT_AVG_TRUNC && !ag->is_float && an && bn) {
+        i128 x = m->slots[oa] / (i128)an, y = m->slots[ob] / (i128)bn;
+        return x < y ? -1 : (x > y);
+      }
+      double sc = pow(10.0, (double)ag->scale);
+      double af = an == 0 ? 0 : (ag->is_float ? m->fslots[oa] / (double)an : (double)m->slots[oa] / (double)an / sc);
+      double bf = bn == 0 ? 0 : (ag->is_float ? m->fslots[ob] / (double)bn : (double)m->slots[ob] / (double)bn / sc);
       return float_order(af, bf);
     }
   }
 }
 
+/* The silo's packed group key compared as a u128 (the last word of
+ * rowOrderBefore, v2_shape_group_topn.zig:1327-1335).  Its layout
+ * (validateShape :2051-2094, genericKeyFromViews silo_group_core.zig:
+ * 5265-5283): part i at the bit offset of the widths before it, the value's
+ * two's-complement bits masked to the type's width (intTypeBits :2698-2706),
+ * a nullable part with a validity bit above them (0 = NULL, payload 0).  So
+ * the comparison runs over the parts from the last (most significant) to
+ * the first, each unsigned over (valid, bits).  Keys wider than 128 bits
+ * are hashed by the silo, whose tie order is then unspecified. */
+static int silo_key_cmp(const cplan* p, u128 ka, u128 kb) {
+  for (int k = (int)p->n_parts - 1; k >= 0; k--) {
+    const key_part* kp = &p->parts[k];
+    int an = part_is_null(ka, kp), bn = part_is_null(kb, kp);
+    if (an != bn) return an ? -1 : 1;
+    if (an) continue;
+    if (g_plant == PLANT_TIES_SIGNED) {
+      int64_t x = unpack_part(ka, kp), y = unpack_part(kb, kp);
+      if (x != y) return x < y ? -1 : 1;
+      continue;
+    }
+    uint64_t mask = kp->width >= 64 ? ~0ull : ((1ull << kp->width) - 1);
+    uint64_t x = (uint64_t)unpack_part(ka, kp) & mask, y = (uint64_t)unpack_part(kb, kp) & mask;
+    if (x != y) return x < y ? -1 : 1;
+  }
+  return 0;
+}
+
+/* rowOrderBefore: the ORDER BY items, then the packed group key ascending,
+ * so no two groups tie. */
 static int gid_cmp(const void* pa, const void* pb) {
   uint32_t a = *(const uint32_t*)pa, b = *(const uint32_t*)pb;
   const tdbg_plan_desc* d = g_sort->p->d;
@@ -606,6 +681,8 @@ static int gid_cmp(const void* pa, const void* pb) {
     if (o < 0) return d->order[s].desc ? 1 : -1;
     if (o > 0) return d->order[s].desc ? -1 : 1;
   }
+  int k = g_plant == PLANT_TIES_BY_GID ? 0 : silo_key_cmp(g_sort->p, g_sort->m->keys[a], g_sort->m->keys[b]);
+  if (k) return k;
   return a < b ? -1 : (a > b);
 }
 

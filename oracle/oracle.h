@@ -53,6 +53,9 @@ int32_t oracle_result_compensated(const or_result* r, uint32_t i, double* out);
 uint64_t oracle_result_num_groups(const or_result* r);
 void oracle_result_free(or_result* r);
 const char* oracle_last_error(void);
+/* Tests only: plants one known mistake in the ORDER BY comparator (0 = none),
+ * so a test can show that its checks catch that mistake. */
+void oracle_plant(int which);
 
 #ifdef __cplusplus
 }

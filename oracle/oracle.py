@@ -118,6 +118,8 @@ def lib():
         L.oracle_result_compensated.restype = C.c_int32
         L.oracle_result_free.argtypes = [C.c_void_p]
         L.oracle_last_error.restype = C.c_char_p
+        L.oracle_plant.argtypes = [C.c_int]
+        L.oracle_plant.restype = None
         _lib = L
     return _lib
 
@@ -365,3 +367,20 @@ def run(table, request, dop: int = 1) -> OracleResult:
         return OracleResult(out, int(L.oracle_result_num_groups(h)))
     finally:
         L.oracle_result_free(h)
+
+
+class planted:
+    """Context manager: the oracle's ORDER BY comparator with one known
+    mistake planted (tests only; see oracle_plant in oracle.c)."""
+    DECIMAL_LOW_WORD, AVG_TRUNC, NULL_AS_ZERO, TIES_BY_GID, NAN_FIRST, TIES_SIGNED = range(1, 7)
+
+    def __init__(self, which: int):
+        self.which = which
+
+    def __enter__(self):
+        lib().oracle_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_plant(0)
+        return False

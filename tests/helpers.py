@@ -2,6 +2,7 @@
 comparison under the reference's tie rule."""
 from __future__ import annotations
 
+import dataclasses
 import json
 import math
 from pathlib import Path
@@ -92,6 +93,95 @@ def heavy_table() -> Table:
                  rg_rows=g["row_group_size"])
 
 
+def ranked_table() -> Table:
+    """The ranked-emit test's table (ranked_pages.json): 140,009 groups."""
+    g = golden("ranked_pages")
+    n_groups = g["group_count"]
+    k = np.arange(n_groups, dtype=np.int64)
+    reps = 1 + k % 3
+    kk = np.repeat(k, reps)
+    j = np.arange(len(kk)) - np.repeat(np.cumsum(reps) - reps, reps)
+    schema = [Column("id", Type.BIGINT), Column("g", Type.BIGINT), Column("v", Type.INT)]
+    return Table(schema, {"id": np.arange(len(kk), dtype=np.int64), "g": kk * 1_000_003,
+                          "v": ((kk * 31 + j * 7) % 17).astype(np.int32)}, rg_rows=g["row_group_size"])
+
+
+def nullable_agg_table() -> Table:
+    """The nullable-aggregate test's table (nullable_aggs.json)."""
+    g = golden("nullable_aggs")
+    G = g["group_count"]
+    i = np.arange(6 * G, dtype=np.int64)
+    grp = (i // 2) % G
+    z = grp == 0
+    schema = [Column("id", Type.BIGINT), Column("g", Type.INT, True), Column("v", Type.INT, True),
+              Column("b", Type.BIGINT, True), Column("d", Type.DOUBLE, True)]
+    cols = {"id": i, "g": grp.astype(np.int32), "v": np.where(z | (i % 7 == 0), 0, (i * 37) % 1000 - 500).astype(np.int32),
+            "b": np.where(z | (i % 5 == 0), 0, i * 3_000_000_019), "d": (((i * 13) % 512) - 256.0) * 0.25}
+    valid = {"g": np.ones(len(i), bool), "v": ~(z | (i % 7 == 0)), "b": ~(z | (i % 5 == 0)), "d": ~(z | (i % 3 == 0))}
+    return Table(schema, cols, valid, rg_rows=g["row_group_size"])
+
+
+def order_table(seed: int = 0, n: int = 4000, wide: bool = False) -> Table:
+    """A table built to separate ORDER BY comparators: keys spanning both
+    signs (the packed key's bit order differs from the numeric order), a
+    nullable key, DECIMAL sums beyond 64 bits, groups whose SMALLINT input is
+    all NULL, integer AVGs sharing integer parts, DOUBLE sums holding NaN and
+    signed zeros (values are multiples of 1/4, so every summation order gives
+    the same sum), and COUNT(*) ties.  `wide` spreads k1 over a 6e8-value
+    span (a sparse key domain: the engine's HBM hash table)."""
+    rng = np.random.default_rng(seed)
+    k1 = rng.integers(-6, 7, n).astype(np.int32)
+    k1_key = k1 * 50_000_001 if wide else k1
+    k2 = rng.integers(-2, 3, n).astype(np.int8)
+    d = rng.integers(-256, 257, n) * 0.25
+    d = np.where(k1 == -5, np.nan, d)
+    d = np.where(k1 == 0, np.where(rng.random(n) < 0.5, -0.0, 0.0), d)
+    e = rng.integers(-8, 9, n) * 0.5
+    e = np.where(e == 0, np.where(rng.random(n) < 0.5, -0.0, 0.0), e)
+    schema = [Column("k1", Type.INT), Column("k2", Type.TINYINT, True), Column("m", Type.DECIMAL64, False, 2),
+              Column("s", Type.SMALLINT, True), Column("i", Type.INT), Column("d", Type.DOUBLE),
+              Column("e", Type.DOUBLE, True)]
+    cols = {"k1": k1_key.astype(np.int32), "k2": k2, "m": rng.integers(-9 * 10 ** 18, 9 * 10 ** 18, n, dtype=np.int64),
+            "s": rng.integers(-300, 300, n).astype(np.int16), "i": rng.integers(-5, 6, n).astype(np.int32),
+            "d": d, "e": e}
+    valid = {"k2": rng.random(n) > 0.15, "s": (k1 != 3) & (rng.random(n) > 0.3), "e": (k1 != 2) | (k2 != 1)}
+    return Table(schema, cols, valid, rg_rows=256)
+
+
+def nan_zero_table(n: int = 6000, seed: int = 0) -> Table:
+    """A DOUBLE column whose groups hold NaNs (both signs, several payloads),
+    +0.0 and -0.0: group 0 only NaNs, 1 NaNs and numbers, 2 only signed
+    zeros, 3 zeros and positives, 4 zeros and negatives, 5 plain numbers."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, 6, n).astype(np.int32)
+    nans = np.array([0x7ff8000000000000, 0xfff8000000000000, 0x7ff0000000000001, 0xfff4000000000abc],
+                    dtype=np.uint64).view(np.float64)
+    pick = rng.integers(0, 4, n)
+    zero = np.where(rng.random(n) < 0.5, -0.0, 0.0)
+    num = rng.integers(-1000, 1001, n) * 0.5
+    x = np.select([k == 0, (k == 1) & (rng.random(n) < 0.3), k == 2, k == 3, k == 4],
+                  [nans[pick], nans[pick], zero, np.where(rng.random(n) < 0.3, zero, np.abs(num) + 1),
+                   np.where(rng.random(n) < 0.3, zero, -np.abs(num) - 1)], num)
+    return Table([Column("k", Type.INT), Column("x", Type.DOUBLE)], {"k": k, "x": x}, rg_rows=512)
+
+
+def reachable_extremes(xs: Sequence[float], want_max: bool) -> set:
+    """Bit patterns a MIN (or MAX) of DOUBLE values can take under the
+    reference's fold for SOME row order: the first value seeds the state and
+    a later one replaces it when IEEE `<` (`>`) holds
+    (foldExtremeFloat, src/exec/v2_lowcard_group.zig:1318-1347; mergeScalars
+    :1981-1992).  A NaN first stays (nothing compares below it); otherwise
+    NaNs never enter, and among zeros the first one seen stays."""
+    import struct
+    bits = lambda v: struct.unpack("<Q", struct.pack("<d", v))[0]
+    out = {bits(v) for v in xs if v != v}
+    nums = [v for v in xs if v == v]
+    if nums:
+        m = max(nums) if want_max else min(nums)
+        out |= {bits(v) for v in nums if v == m}
+    return out
+
+
 # ---- brute force (pure Python, exact) -------------------------------------------------
 
 def _row_passes(table: Table, leaves: Sequence[Predicate], r: int) -> bool:
@@ -180,6 +270,74 @@ def brute_force(table: Table, request: Request) -> Dict[tuple, list]:
     return out
 
 
+_KEY_BITS = {Type.BOOLEAN: 8, Type.TINYINT: 8, Type.SMALLINT: 16, Type.INT: 32, Type.DATE: 32}
+
+
+def _float_cmp(x: float, y: float) -> int:
+    """types.floatOrder: -0 = +0, NaN after every number."""
+    xn, yn = x != x, y != y
+    if xn or yn:
+        return 0 if xn and yn else (1 if xn else -1)
+    return (x > y) - (x < y)
+
+
+def brute_order(table: Table, request: Request) -> List[tuple]:
+    """The reference's result rows, from the brute force's groups sorted by a
+    comparator written from the silo's definition (rowOrderBefore /
+    compareOrderKey / compareAggregateValue, src/exec/v2_shape_group_topn.zig:
+    1327-1395): each ORDER BY item with NULL first ascending and last
+    descending, floats under types.floatOrder, then the packed group key
+    ascending (its fields from the last key part to the first, each as
+    (validity, two's-complement bits of the type's width)); then OFFSET and
+    LIMIT.  No HAVING."""
+    import functools
+    assert request.having_filter is None
+    groups = brute_force(table, request)
+    nk = len(request.group_cols)
+    names = [k.lower() for k in request.group_cols] + [a.as_.lower() for a in request.aggs]
+    key_bits = [_KEY_BITS.get(table.column(k).type, 64) for k in request.group_cols]
+    rows = [tuple(k) + tuple(v) for k, v in groups.items()]
+
+    def item(i, desc, a, b):
+        x, y = a[i], b[i]
+        if x is None or y is None:
+            c = 0 if x is None and y is None else (-1 if x is None else 1)
+        elif isinstance(x, float) or isinstance(y, float):
+            c = _float_cmp(float(x), float(y))
+        else:
+            c = (x > y) - (x < y)
+        return -c if desc else c
+
+    def cmp(a, b):
+        for sp in request.order_specs:
+            c = item(names.index(sp.col.lower()), sp.desc, a, b)
+            if c:
+                return c
+        for p in range(nk - 1, -1, -1):
+            fa = (0, 0) if a[p] is None else (1, a[p] & ((1 << key_bits[p]) - 1))
+            fb = (0, 0) if b[p] is None else (1, b[p] & ((1 << key_bits[p]) - 1))
+            if fa != fb:
+                return -1 if fa < fb else 1
+        return 0
+
+    rows.sort(key=functools.cmp_to_key(cmp))
+    start = min(request.offset, len(rows))
+    end = len(rows) if not request.limit else min(len(rows), start + request.limit)
+    return rows[start:end]
+
+
+def same_values(a: tuple, b: tuple) -> bool:
+    """Row equality with NaN equal to NaN (and -0.0 equal to 0.0)."""
+    if len(a) != len(b):
+        return False
+    for x, y in zip(a, b):
+        if isinstance(x, float) and isinstance(y, float) and x != x and y != y:
+            continue
+        if x != y:
+            return False
+    return True
+
+
 # ---- result comparison --------------------------------------------------------------------
 
 def columns_of(result) -> List[tuple]:
@@ -225,13 +383,30 @@ def _close(a, b, rel: float) -> bool:
 
 
 def assert_same_rows(got: List[tuple], want: List[tuple], float_cols: Sequence[int] = (), rel: float = 0.0,
-                     n_keys: int = 1, order_cols: Optional[Sequence[int]] = None) -> None:
-    """Compares two results under the reference's tie rule: with ORDER BY
-    (`order_cols` = the output columns ranked on), the ranked values agree
-    position by position, every returned row is a true group row, and the
-    rows strictly above the last ranked value agree as a set.  Without
-    ORDER BY the rows agree as a set.  Columns in `float_cols` compare
-    within relative tolerance `rel`; everything else bit-exactly."""
+                     n_keys: int = 1, order_cols: Optional[Sequence[int]] = None,
+                     full: Optional[List[tuple]] = None, limited: bool = False) -> None:
+    """Compares a result with the oracle's.
+
+    `want` is the oracle's result for the same request; `full` (required
+    with ORDER BY or LIMIT/OFFSET) the oracle's rows of every group that
+    passes HAVING, unordered and unsliced.  Columns in `float_cols` compare
+    within relative tolerance `rel` (a DOUBLE SUM/AVG, whose summation order
+    is free), everything else bit-exactly.
+
+    * Every returned row equals its group's row of `full`, keys and every
+      aggregate, rows tied at a LIMIT cut included (the reference's own
+      parity rule: a returned row is a true group row).
+    * With ORDER BY (`order_cols` = the output columns ranked on): the ranked
+      values agree position by position.  The order is total - ties break
+      by the packed group key, as the reference's silo breaks them
+      (rowOrderBefore, src/exec/v2_shape_group_topn.zig:1327-1335) - so when
+      no ranked column is a float the rows agree position by position,
+      keys included.  When one is, two groups whose float values differ by
+      less than the tolerance may trade places, so the rows strictly above
+      the last ranked value agree as a set instead.
+    * Without ORDER BY and without LIMIT/OFFSET the rows agree as a set; with
+      `limited` (LIMIT/OFFSET and no ORDER BY: any groups may be returned)
+      only the count and the per-row check above apply."""
     assert len(got) == len(want), f"{len(got)} rows vs {len(want)}"
 
     def eq_row(a, b):
@@ -240,26 +415,83 @@ def assert_same_rows(got: List[tuple], want: List[tuple], float_cols: Sequence[i
     def key_of(r):  # NULL keys sort first
         return tuple((x is not None, x if x is not None else 0) for x in r[:n_keys])
 
+    if order_cols is not None or limited:
+        assert full is not None, "ORDER BY / LIMIT results are checked against the oracle's full group table"
+    if full is not None:
+        by_key = {tuple(r[:n_keys]): r for r in full}
+        assert len(by_key) == len(full), "oracle groups are not unique by key"
+        for r in got:
+            k = tuple(r[:n_keys])
+            assert k in by_key, f"returned group {k} does not exist"
+            assert eq_row(r, by_key[k]), f"row {r} != its group {by_key[k]}"
+        assert len({tuple(r[:n_keys]) for r in got}) == len(got), "a group is returned twice"
     if order_cols is None:
+        if limited:
+            return
         ga = sorted(got, key=key_of)
         wa = sorted(want, key=key_of)
         for a, b in zip(ga, wa):
             assert eq_row(a, b), f"{a} != {b}"
         return
+
     def same_rank(a, b):
         return all(_close(a[i], b[i], rel) if i in float_cols else a[i] == b[i] for i in order_cols)
 
-    for a, b in zip(got, want):
-        assert same_rank(a, b), f"ranked value {a} vs {b}"
+    for i, (a, b) in enumerate(zip(got, want)):
+        assert same_rank(a, b), f"position {i}: ranked value {a} vs {b}"
+    if not any(i in float_cols for i in order_cols):
+        for i, (a, b) in enumerate(zip(got, want)):
+            assert eq_row(a, b), f"position {i}: {a} != {b}"
+        return
     if not got:
         return
     last = want[-1]
     above_g = [r for r in got if not same_rank(r, last)]
     above_w = [r for r in want if not same_rank(r, last)]
     assert len(above_g) == len(above_w), (len(above_g), len(above_w))
-    by_key = {r[:n_keys]: r for r in above_w}
+    by_key_w = {r[:n_keys]: r for r in above_w}
     for r in above_g:
-        assert r[:n_keys] in by_key and eq_row(r, by_key[r[:n_keys]]), f"row {r} not in expected"
+        assert r[:n_keys] in by_key_w and eq_row(r, by_key_w[r[:n_keys]]), f"row {r} not in expected"
+
+
+def float_cols_of(table: Table, req: Request) -> List[int]:
+    """Output columns compared within tolerance: DOUBLE SUM and every AVG."""
+    nk = len(req.group_cols)
+    out = []
+    for j, a in enumerate(req.aggs):
+        if a.col is None:
+            continue
+        c = table.column(a.col)
+        if a.func.lower() == "avg" or (a.func.lower() == "sum" and c.type in (Type.FLOAT, Type.DOUBLE)):
+            out.append(nk + j)
+    return out
+
+
+def order_cols_of(schema_names: Sequence[str], req: Request) -> Optional[List[int]]:
+    if not req.order_specs:
+        return None
+    names = [n.lower() for n in schema_names]
+    return [names.index(s.col.lower()) for s in req.order_specs]
+
+
+def unsliced(req: Request) -> Request:
+    """The same request with no ORDER BY, LIMIT or OFFSET: every group."""
+    return dataclasses.replace(req, order_specs=(), limit=0, offset=0)
+
+
+def check_with_oracle(got: List[tuple], oracle_lib, table: Table, req: Request, dop: int = 2,
+                      rel: float = 1e-12, want: Optional[List[tuple]] = None) -> List[tuple]:
+    """`got` (rows of a GPU result for `req` over `table`) against the
+    oracle under assert_same_rows; returns the oracle's rows."""
+    if want is None:
+        want = rows_of(oracle_lib.run(table, req, dop))
+    names = [k for k in req.group_cols] + [a.as_ for a in req.aggs]
+    order_cols = order_cols_of(names, req)
+    limited = bool(req.limit or req.offset)
+    full = rows_of(oracle_lib.run(table, unsliced(req), dop)) if (order_cols is not None or limited) else None
+    assert_same_rows(got, want, float_cols_of(table, req), rel, len(req.group_cols), order_cols, full,
+                     limited=limited and order_cols is None)
+    return want
 
 
 def assert_rows_in_groups(got: List[tuple], groups: Dict[tuple, list], n_keys: int, float_cols: Sequence[int] = (),

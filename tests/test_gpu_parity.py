@@ -2,7 +2,8 @@
 
 Integer, decimal, COUNT, MIN and MAX results are compared bit-exactly; DOUBLE
 SUM/AVG within 1e-12 relative error (the worst error is printed).  ORDER BY
-results follow the reference's tie rule (helpers.assert_same_rows).
+results are checked row by row against the oracle's full group table and
+position by position in order (helpers.assert_same_rows).
 """
 import math
 import sys
@@ -10,8 +11,8 @@ import sys
 import numpy as np
 import pytest
 
-from helpers import (ALL_AGGS, assert_rows_in_groups, assert_same_rows, brute_force, decimal_table, float_table,
-                     golden, heavy_table, random_table, rows_of, wrap_table)
+from helpers import (ALL_AGGS, assert_rows_in_groups, assert_same_rows, brute_force, check_with_oracle, decimal_table,
+                     float_cols_of, float_table, golden, heavy_table, random_table, rows_of, wrap_table)
 from thindb_amd import gen
 from thindb_amd.exec import (And, AggSpec, Batch, Column, GpuSession, Predicate, Request, SortSpec, Table,
                              try_build_gpu_group)
@@ -37,18 +38,6 @@ def run_gpu(session, table, req):
     return b, stats
 
 
-def float_cols_of(table, req):
-    nk = len(req.group_cols)
-    out = []
-    for j, a in enumerate(req.aggs):
-        if a.col is None:
-            continue
-        c = table.column(a.col)
-        if a.func.lower() == "avg" or (a.func.lower() == "sum" and c.type in (Type.FLOAT, Type.DOUBLE)):
-            out.append(nk + j)
-    return out
-
-
 def worst_rel(got, want, cols, nk):
     w = 0.0
     by = {r[:nk]: r for r in want}
@@ -64,17 +53,14 @@ def worst_rel(got, want, cols, nk):
 
 
 def check_vs_oracle(session, oracle_lib, table, req, dop=4):
+    """The GPU result against the oracle's: every returned row against the
+    oracle's full group table, the order position by position
+    (helpers.assert_same_rows)."""
     b, st = run_gpu(session, table, req)
     got = rows_of(b)
-    want_res = oracle_lib.run(table, req, dop)
-    want = rows_of(want_res)
+    want = check_with_oracle(got, oracle_lib, table, req, dop, REL)
     fc = float_cols_of(table, req)
     nk = len(req.group_cols)
-    order_cols = None
-    if req.order_specs:
-        names = [c.name.lower() for c in b.schema]
-        order_cols = [names.index(s.col.lower()) for s in req.order_specs]
-    assert_same_rows(got, want, fc, REL, nk, order_cols)
     if fc and got:
         print(f"  worst DOUBLE rel error vs oracle: {worst_rel(got, want, fc, nk):.3e}", flush=True)
     return got, want, st
@@ -86,7 +72,9 @@ def test_wrap_sum_known_answers(session):
     g = golden("wrap_sum")
     t = wrap_table()
     for q in g["queries"]:
-        req = Request(q["keys"], [AggSpec("sum", "v", "s")], [SortSpec(c, d) for c, d in q["order"]], q["limit"])
+        having = Predicate(*q["having"]) if "having" in q else None
+        req = Request(q["keys"], [AggSpec("sum", "v", "s")], [SortSpec(c, d) for c, d in q["order"]], q["limit"],
+                      having_filter=having)
         b, _ = run_gpu(session, t, req)
         assert [list(r) for r in rows_of(b)] == q["expect"], q["sql"]
     b, _ = run_gpu(session, t, Request(["hk"], [AggSpec("sum", "v", "s")]))
@@ -474,3 +462,77 @@ def test_filtered_full_size_checksums(session):
     assert sum(r[2] for r in rows) == want
     print(f"  200M filtered: scan {st.scan_ms:.3f} ms, {st.bytes_scanned / st.scan_ms / 1e6:.1f} GB/s", flush=True)
     session.evict_table(t)
+
+
+# ---- ORDER BY pinned by the reference's pages and the brute force ---------------------------
+
+def test_ranked_emit_pages(session):
+    """The reference's ranked-emit pages (v2_group_topn_test.zig:1408-1460):
+    140,009 groups in a sparse BIGINT key domain (the HBM hash table), ties
+    in MIN(v) broken by the group key, HAVING COUNT(*) > 1, OFFSET 39,990."""
+    from helpers import ranked_table
+    from test_order_pinning import RANKED_CASES, ranked_request
+    t = ranked_table()
+    for case in RANKED_CASES:
+        b, st = run_gpu(session, t, ranked_request(case))
+        assert [[r[0], r[1]] for r in rows_of(b)] == case["page"], case["sql"]
+
+
+@pytest.mark.parametrize("kind", ["int", "float"])
+def test_nullable_aggregate_pages(session, kind):
+    """The reference's nullable-aggregate pages (v2_group_topn_test.zig:
+    1283-1400): NULL-only group 0, nullable INT/BIGINT/DOUBLE inputs."""
+    import hashlib
+    import json
+    from helpers import nullable_agg_table
+    from test_order_pinning import nullable_request
+    g = golden("nullable_aggs")
+    t = nullable_agg_table()
+    b, _ = run_gpu(session, t, nullable_request(kind, 25, 7))
+    assert [list(r) for r in rows_of(b)] == g[kind]["page_25_7"]
+    b, _ = run_gpu(session, t, nullable_request(kind))
+    every = [list(r) for r in rows_of(b)]
+    assert len(every) == g["group_count"]
+    assert hashlib.sha256(json.dumps(every).encode()).hexdigest() == g[kind]["digest_all"]
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_order_against_brute_force(session, oracle_lib, case, monkeypatch):
+    """Every comparator case of test_order_pinning on the GPU, exactly equal to
+    the brute force (DOUBLE sums are exact here: multiples of 1/4): through
+    the dense table and, with k1 spread over a sparse domain, the HBM hash
+    table (whose slot order is not the key order, so the packed-key words
+    decide ties)."""
+    from helpers import brute_order, order_table, same_values
+    from test_order_pinning import ORDER_CASES, order_request
+    req = order_request(ORDER_CASES[case])
+    for wide in (False, True):
+        t = order_table(seed=case, wide=wide)
+        want = brute_order(t, req)
+        b, st = run_gpu(session, t, req)
+        assert st.global_mode == (2 if wide and "k1" in req.group_cols else 1), st.global_mode
+        got = rows_of(b)
+        assert len(got) == len(want)
+        for i, (x, y) in enumerate(zip(got, want)):
+            assert same_values(x, y), f"wide {wide} position {i}: {x} != {y}"
+
+
+def test_min_max_double_nan_and_signed_zero(session):
+    """MIN/MAX of DOUBLE over NaNs (both signs, several payloads) and signed
+    zeros: the reference's result depends on row order (the first value
+    seeds, IEEE `<`/`>` compare), so the GPU's - the extreme under a total
+    order of the bits: -0 below +0, a NaN with the sign bit below every
+    number and one without above - must be a result the reference produces
+    for some row order."""
+    import struct
+    from helpers import nan_zero_table, reachable_extremes
+    t = nan_zero_table()
+    req = Request(["k"], [AggSpec("min", "x", "mn"), AggSpec("max", "x", "mx")], [SortSpec("k")])
+    b, _ = run_gpu(session, t, req)
+    rows = rows_of(b)
+    assert [r[0] for r in rows] == list(range(6))
+    bits = lambda v: struct.unpack("<Q", struct.pack("<d", v))[0]
+    for k, mn, mx in rows:
+        xs = [float(v) for v in t.columns["x"][t.columns["k"] == k]]
+        assert bits(mn) in reachable_extremes(xs, False), (k, hex(bits(mn)))
+        assert bits(mx) in reachable_extremes(xs, True), (k, hex(bits(mx)))

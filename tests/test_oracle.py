@@ -20,7 +20,9 @@ def test_wrap_sum_known_answers(oracle_lib):
     g = golden("wrap_sum")
     t = wrap_table()
     for q in g["queries"]:
-        req = Request(q["keys"], [AggSpec("sum", "v", "s")], [SortSpec(c, d) for c, d in q["order"]], q["limit"])
+        having = Predicate(*q["having"]) if "having" in q else None
+        req = Request(q["keys"], [AggSpec("sum", "v", "s")], [SortSpec(c, d) for c, d in q["order"]], q["limit"],
+                      having_filter=having)
         for dop in (1, 4):
             got = rows_of(oracle_lib.run(t, req, dop))
             assert [list(r) for r in got] == q["expect"], (q["sql"], dop)

@@ -1,10 +1,11 @@
 """The device request loop (tdbg_server_*), through the C-ABI: every request's
-result is the same alone and in a concurrent mix that the server batches,
-overlaps on its lanes and admits by estimated device bytes."""
+result equals the oracle's (bit-exact integers, DOUBLE within 1e-12
+relative) and is the same alone and in a concurrent mix that the server
+batches, overlaps on its lanes and admits by estimated device bytes."""
 import numpy as np
 import pytest
 
-from helpers import assert_same_rows, rows_of
+from helpers import assert_same_rows, check_with_oracle, float_cols_of, order_cols_of, rows_of
 from thindb_amd import gen, gpu
 from thindb_amd.exec import And, AggSpec, Batch, Predicate, Request, SortSpec, build_plan, output_schema
 
@@ -44,21 +45,29 @@ def alone(items, plans):
     return out
 
 
-def float_cols(table, req):
-    from test_gpu_parity import float_cols_of
-    return float_cols_of(table, req)
-
-
 def compare(items, got, want):
+    """Served results against the same requests run alone: identical rows."""
     for (table, req), g, w in zip(items, got, want):
-        names = [c.name.lower() for c in output_schema(table, req)]
-        order = [names.index(s.col.lower()) for s in req.order_specs] or None
-        assert_same_rows(g, w, float_cols(table, req), REL, len(req.group_cols), order)
+        names = [c.name for c in output_schema(table, req)]
+        assert_same_rows(g, w, float_cols_of(table, req), REL, len(req.group_cols), order_cols_of(names, req),
+                         full=w if req.order_specs else None)
 
 
-def test_same_alone_and_in_the_mix(session):
+def oracle_rows(oracle_lib, items):
+    return [rows_of(oracle_lib.run(table, req, 8)) for table, req in items]
+
+
+def compare_oracle(oracle_lib, items, got, want_oracle):
+    """Served results against the oracle (the CPU path) for each request."""
+    for (table, req), g, w in zip(items, got, want_oracle):
+        check_with_oracle(g, oracle_lib, table, req, 8, REL, want=w)
+
+
+def test_same_alone_and_in_the_mix(session, oracle_lib):
     items, plans = mix(session)
     want = alone(items, plans)
+    want_oracle = oracle_rows(oracle_lib, items)
+    compare_oracle(oracle_lib, items, want, want_oracle)
     print(f"[gpu] serving mix of {len(items)} requests, 3 rounds", flush=True)
     srv = gpu.Server(session.ctx, lanes=3, max_batch=16)
     try:
@@ -76,6 +85,7 @@ def test_same_alone_and_in_the_mix(session):
                 sizes.append(rep.batch_size)
                 t.close()
             compare(items, got, want)
+            compare_oracle(oracle_lib, items, got, want_oracle)
             print(f"  round {rnd}: batch sizes {sorted(set(sizes))}", flush=True)
         c = srv.counters()
         assert c.completed == c.submitted == 3 * len(items)
@@ -86,9 +96,10 @@ def test_same_alone_and_in_the_mix(session):
             p.close()
 
 
-def test_admission_by_device_bytes(session):
+def test_admission_by_device_bytes(session, oracle_lib):
     items, plans = mix(session)
     want = alone(items, plans)
+    want_oracle = oracle_rows(oracle_lib, items)
     # room for about one medium request at a time: later arrivals wait for memory
     est = []
     srv = gpu.Server(session.ctx, lanes=3, max_batch=8, admit_bytes=64 << 20)
@@ -101,6 +112,7 @@ def test_admission_by_device_bytes(session):
             got.append(rows_of(Batch(output_schema(table, req), cols, len(cols[0].values))))
             t.close()
         compare(items, got, want)
+        compare_oracle(oracle_lib, items, got, want_oracle)
         c = srv.counters()
         print(f"[gpu] admission: peak {c.peak_admitted >> 20} MiB of 64, waits {c.admission_waits}, "
               f"largest estimate {max(est) >> 20} MiB", flush=True)

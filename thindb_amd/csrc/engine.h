@@ -194,8 +194,11 @@ enum CompSrc : uint32_t {
   C_I128_LO,      // low word (unsigned)
   C_F64_AT,       // f64 at record offset
   C_FORD_AT,      // float MIN/MAX stored through the order map
-  C_AVG_I,        // i128 sum / n  (unscaled mean)
-  C_AVG_F         // f64 sum / n
+  C_AVG_I,        // f64(i128 sum) / n / 10^aux  (the emitted AVG)
+  C_AVG_F,        // f64 sum / n
+  C_AGG_NULL,     // 0 for a group with no non-null input (the aggregate is NULL), else 1
+  C_PKEY_VALID,   // key part `idx` of a nullable part: 0 NULL, 1 valid
+  C_PKEY          // key part `idx`: the value's two's-complement bits masked to aux bits (0 for NULL)
 };
 struct CompDesc {
   uint32_t src;
@@ -203,10 +206,10 @@ struct CompDesc {
   uint32_t off;      // record offset of the value
   uint32_t ns_off;   // record offset of the non-null count, or ~0 (use count)
   uint32_t desc;
-  uint32_t pad;
+  uint32_t aux;      // C_AVG_I: decimal scale; C_PKEY: the key type's width in bits
 };
 
-constexpr int kMaxComps = 12;
+constexpr int kMaxComps = 24;
 struct OrderParams {
   CompDesc comps[kMaxComps];
   uint32_t n_comps;

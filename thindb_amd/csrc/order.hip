@@ -7,12 +7,14 @@
 // src/exec/v2_shape_group_topn.zig:1095-1180, :1399-1470) and the bounded
 // exec.TopN (src/exec/topn.zig:1-60).
 //
-// Order semantics: every ORDER BY item becomes one or two 64-bit words whose
-// unsigned order is the reference's order for that item (desc = bitwise
-// NOT); the final tie-break is the group's position in slot order.  The
-// reference breaks ties by gid (first-seen order), which its parallel scan
-// leaves unspecified; slot order is one admissible choice and is
-// deterministic here.
+// Order semantics: every ORDER BY item becomes one to three 64-bit words
+// whose unsigned order is the reference's order for that item (desc =
+// bitwise NOT), followed by the silo's tie-break, the packed group key
+// ascending (rowOrderBefore, src/exec/v2_shape_group_topn.zig:1327-1335): as
+// words of their own, or - where the slot order already is that order (a
+// dense table over key parts whose values share one sign) - the group's
+// position in slot order.  The order is total, so the page at a LIMIT cut is
+// the reference silo's page.
 #include <cstdlib>
 #include <vector>
 
@@ -701,7 +703,7 @@ struct SortScratch {
   uint32_t* hist;      // 256 * tiles
   uint32_t* sums;      // hist chunks
   uint32_t* sums2;     // chunk sums of sums (one scan level more)
-  uint64_t* mm;        // 2 words
+  uint64_t* mm;        // min and max per order word, then a u32 count at word 2 * kMaxComps
   uint64_t* total;     // 1 word
 };
 
@@ -729,7 +731,7 @@ static SortScratch carve(void* base, uint64_t n, uint32_t n_comps) {
   s.hist = reinterpret_cast<uint32_t*>(p); p += al256(hist * 4);
   s.sums = reinterpret_cast<uint32_t*>(p); p += al256(chunks * 4);
   s.sums2 = reinterpret_cast<uint32_t*>(p); p += al256(((chunks + kScanChunk - 1) / kScanChunk + 1) * 4);
-  s.mm = reinterpret_cast<uint64_t*>(p); p += 256;
+  s.mm = reinterpret_cast<uint64_t*>(p); p += 512;  // 2 words per order word, then a count
   s.total = reinterpret_cast<uint64_t*>(p);
   return s;
 }

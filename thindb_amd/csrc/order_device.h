@@ -2,10 +2,15 @@
 // by the sorter and result writer (order.hip) and the radix leaf pass
 // (radix.hip), which emits the first word of each group it writes.
 //
-// Follows the reference's SortCtx.lessThan/compareBySpec
-// (src/exec/v2_lowcard_group.zig:2069-2311) and compareAggregateValue
-// (src/exec/v2_shape_group_topn.zig:1361): every ORDER BY item becomes one or
-// two words whose unsigned order is the item's order (desc = bitwise NOT).
+// Follows the silo's result order, rowOrderBefore/compareOrderKey/
+// compareAggregateValue (src/exec/v2_shape_group_topn.zig:1327-1395); the
+// low-cardinality core's SortCtx.lessThan/compareBySpec
+// (src/exec/v2_lowcard_group.zig:2198-2271) orders by the same items (a
+// DECIMAL AVG by its unscaled mean, the same order up to rounding ties) and
+// breaks ties by gid, which its parallel merge leaves unspecified.  Each
+// ORDER BY item becomes one to three words whose unsigned order is the item's
+// order (desc = bitwise NOT), and the packed group key's fields (C_PKEY)
+// break ties.
 #pragma once
 
 #include "engine.h"
@@ -94,8 +99,8 @@ __device__ __forceinline__ uint64_t comp_word(const OrderParams& P, const CompDe
     case C_I128_LO:
     case C_F64_AT:
     case C_FORD_AT: {
-      // A group with no non-null input compares as a zero slot
-      // (compareBySpec reads the untouched slot).
+      // A group with no non-null input is NULL: a C_AGG_NULL word before
+      // this one orders it, so its value word is a placeholder 0.
       const bool empty = rec_ns(rec, c.ns_off, P.g_count_off) == 0;
       if (c.src == C_I64_AT) w = ord_i64(empty ? 0 : *reinterpret_cast<const int64_t*>(rec + c.off));
       else if (c.src == C_I128_HI) w = ord_i64(empty ? 0 : *reinterpret_cast<const int64_t*>(rec + c.off + 8));
@@ -104,14 +109,31 @@ __device__ __forceinline__ uint64_t comp_word(const OrderParams& P, const CompDe
       else w = ord_f64(empty ? 0.0 : unord_f64(*reinterpret_cast<const int64_t*>(rec + c.off)));
       break;
     }
-    case C_AVG_I: {
+    case C_AVG_I: {  // the emitted value (avgValue, v2_shape_group_topn.zig:1436-1444)
       const uint64_t n = rec_ns(rec, c.ns_off, P.g_count_off);
-      w = ord_f64(n == 0 ? 0.0 : i128_to_f64(load_i128(rec + c.off)) / (double)n);
+      double p10 = 1.0;
+      for (uint32_t s = 0; s < c.aux; s++) p10 *= 10.0;
+      w = ord_f64(n == 0 ? 0.0 : i128_to_f64(load_i128(rec + c.off)) / (double)n / p10);
       break;
     }
-    default: {  // C_AVG_F
+    case C_AVG_F: {
       const uint64_t n = rec_ns(rec, c.ns_off, P.g_count_off);
       w = ord_f64(n == 0 ? 0.0 : *reinterpret_cast<const double*>(rec + c.off) / (double)n);
+      break;
+    }
+    case C_AGG_NULL:  // an all-NULL aggregate orders first (compareAggregateValue :1361-1370)
+      w = rec_ns(rec, c.ns_off, P.g_count_off) == 0 ? 0ull : 1ull;
+      break;
+    case C_PKEY_VALID: {
+      const PartDesc& pt = P.parts[c.idx];
+      w = part_null(pt, group_key(rec, gid, P.g_mode, P.g_key_off)) ? 0ull : 1ull;
+      break;
+    }
+    default: {  // C_PKEY: the part's field of the silo's packed key, below its validity bit
+      const PartDesc& pt = P.parts[c.idx];
+      const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+      const uint64_t mask = c.aux >= 64 ? ~0ull : ((1ull << c.aux) - 1);
+      w = part_null(pt, k) ? 0ull : ((uint64_t)part_value(pt, k) & mask);
       break;
     }
   }

@@ -871,36 +871,71 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
   op.g_count_off = fp.g_count_off = sp.g_count_off;
   op.g_key_off = fp.g_key_off = sp.g_key_off;
   op.g_mode = fp.g_mode = sp.g_mode;
+  auto add_comp = [&](uint32_t src, uint32_t idx, uint32_t at, uint32_t ns_off, bool desc, uint32_t aux) -> bool {
+    if (op.n_comps >= (uint32_t)kMaxComps) return false;
+    CompDesc c{};
+    c.src = src; c.idx = idx; c.off = at; c.ns_off = ns_off; c.desc = desc ? 1 : 0; c.aux = aux;
+    op.comps[op.n_comps++] = c;
+    return true;
+  };
+  bool comps_fit = true;
   for (auto& o : pl->order) {
     const uint32_t nk = (uint32_t)pl->keys.size();
-    auto add = [&](uint32_t src, uint32_t idx, uint32_t at, uint32_t ns_off) {
-      CompDesc c{};
-      c.src = src; c.idx = idx; c.off = at; c.ns_off = ns_off; c.desc = o.desc ? 1 : 0;
-      op.comps[op.n_comps++] = c;
+    auto add = [&](uint32_t src, uint32_t at, uint32_t ns_off, uint32_t aux = 0) {
+      comps_fit = add_comp(src, src == C_KEY ? o.target : 0, at, ns_off, o.desc != 0, aux) && comps_fit;
     };
-    if (o.target < nk) { add(C_KEY, o.target, 0, ~0u); continue; }
+    if (o.target < nk) { add(C_KEY, 0, ~0u); continue; }
     const AggOut& ao = pl->outs[o.target - nk];
-    if (ao.dslot >= 0) { add(C_U64_AT, 0, P.distinct[ao.dslot].dslot_off, ~0u); continue; }
-    if (ao.kagg < 0) { add(C_COUNT, 0, 0, ~0u); continue; }
+    if (ao.dslot >= 0) { add(C_U64_AT, P.distinct[ao.dslot].dslot_off, ~0u); continue; }
+    if (ao.kagg < 0) { add(C_COUNT, 0, ~0u); continue; }
     const AggDesc& d = sp.aggs[ao.kagg];
+    // an all-NULL SUM/AVG/MIN/MAX orders first ascending, last descending
+    // (compareAggregateValue, v2_shape_group_topn.zig:1361-1370)
+    if (ao.op != TDBG_AGG_COUNT && d.g_ns_off != ~0u) add(C_AGG_NULL, 0, d.g_ns_off);
     switch (ao.op) {
-      case TDBG_AGG_COUNT: add(C_U64_AT, 0, d.g_off, ~0u); break;
+      case TDBG_AGG_COUNT: add(C_U64_AT, d.g_off, ~0u); break;
       case TDBG_AGG_SUM:
-        if (ao.is_float) add(C_F64_AT, 0, d.g_off, d.g_ns_off);
-        else if (ao.out_type == TDBG_DECIMAL128) { add(C_I128_HI, 0, d.g_off, d.g_ns_off); add(C_I128_LO, 0, d.g_off, d.g_ns_off); }
-        else add(C_I64_AT, 0, d.g_off, d.g_ns_off);  // BIGINT SUM orders by its wrapped value
+        if (ao.is_float) add(C_F64_AT, d.g_off, d.g_ns_off);
+        else if (ao.out_type == TDBG_DECIMAL128) { add(C_I128_HI, d.g_off, d.g_ns_off); add(C_I128_LO, d.g_off, d.g_ns_off); }
+        else add(C_I64_AT, d.g_off, d.g_ns_off);  // BIGINT SUM orders by its wrapped value
         break;
-      case TDBG_AGG_AVG: add(ao.is_float ? C_AVG_F : C_AVG_I, 0, d.g_off, d.g_ns_off); break;
-      default: add(ao.is_float ? C_FORD_AT : C_I64_AT, 0, d.g_off, d.g_ns_off); break;
+      case TDBG_AGG_AVG:
+        if (ao.is_float) add(C_AVG_F, d.g_off, d.g_ns_off);
+        else add(C_AVG_I, d.g_off, d.g_ns_off, ao.scale);
+        break;
+      default: add(ao.is_float ? C_FORD_AT : C_I64_AT, d.g_off, d.g_ns_off); break;
     }
   }
-  if (P.radix && !P.rx.dense_out && op.n_comps < (uint32_t)kMaxComps) {
-    // leaves emit groups in completion order: the relative key as the last
-    // word gives ties (and an unordered result) the dense table's key order
-    CompDesc c{};
-    c.src = C_U64_AT; c.off = sp.g_key_off; c.ns_off = ~0u; c.desc = 0;
-    op.comps[op.n_comps++] = c;
+  // Ties break by the silo's packed group key, ascending (rowOrderBefore,
+  // v2_shape_group_topn.zig:1327-1335): its fields compare from the last key
+  // part to the first, each as (validity, two's-complement bits).  Over key
+  // parts whose values share one sign that is the order of the relative key
+  // (the mixed-radix index, last part most significant, NULL = 0): a dense
+  // table's slot order, which the sorts' final position tie-break keeps, or
+  // the relative key stored in keyed records as one word.  Otherwise every
+  // part's field is a word of its own.
+  bool one_sign = true;
+  for (uint32_t i = 0; i < sp.n_parts; i++) {
+    const PartDesc& d = sp.parts[i];
+    const __int128 lo = (__int128)d.base + (d.nullable ? 1 : 0);
+    const __int128 hi = (__int128)d.base + (__int128)d.span - 1;
+    if (lo <= hi && lo < 0 && hi >= 0) one_sign = false;
   }
+  const bool keyed = sp.g_mode != G_DENSE;  // compacted radix leaves and the HBM hash table
+  if (!op.n_comps && !(P.radix && !P.rx.dense_out)) {
+    // no ORDER BY: nothing is sorted
+  } else if (one_sign && keyed) {
+    comps_fit = add_comp(C_U64_AT, 0, sp.g_key_off, ~0u, false, 0) && comps_fit;
+  } else if (!one_sign) {
+    for (int i = (int)sp.n_parts - 1; i >= 0; i--) {
+      const uint32_t ty = pl->keys[i].type;
+      const uint32_t bits = ty == TDBG_BOOLEAN || ty == TDBG_TINYINT ? 8u : ty == TDBG_SMALLINT ? 16u
+                          : (ty == TDBG_INT || ty == TDBG_DATE) ? 32u : 64u;
+      if (sp.parts[i].nullable) comps_fit = add_comp(C_PKEY_VALID, (uint32_t)i, 0, ~0u, false, 0) && comps_fit;
+      comps_fit = add_comp(C_PKEY, (uint32_t)i, 0, ~0u, false, bits) && comps_fit;
+    }
+  }
+  if (!comps_fit) return fail(TDBG_ERR_UNSUPPORTED, "ORDER BY needs more than %d order words", kMaxComps);
   // finalize writers; output buffer offsets are assigned per execution
   fp.n_outs = 0;
   for (uint32_t i = 0; i < sp.n_parts; i++) {
