@@ -497,6 +497,31 @@ int32_t tdbg_server_stats(const tdbg_server* server, tdbg_server_counters* out);
 /* Runs what is queued, then stops the lanes. */
 void tdbg_server_stop(tdbg_server* server);
 
+/* ---- allocation-free execution ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * or pinned memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Runs `plan` as tdbg_execute does and
+ * returns the same rows, but calls no hipMalloc, hipFree, hipHostMalloc or
+ * other allocating routine: every device buffer its kernels use comes from
+ * one scratch array of TDBG_NOALLOC_SCRATCH_BYTES compiled into the library,
+ * which each GPU holds until the process ends (its address is resolved by
+ * tdbg_context_create).  The result is written into the caller's host buffer
+ * `out` of `out_capacity` bytes (per output column: rows x the type's width,
+ * then one byte per row for a nullable column, each 16-byte aligned; that
+ * is at most rows x (width + 1) + 16 bytes per column); `cols` (n_cols = the
+ * plan's output columns) receive views into `out` (validity as a bitmap,
+ * NULL when no row is NULL) and *n_rows the row count.  Limits: an execution
+ * whose buffers exceed the scratch array fails with TDBG_ERR_UNSUPPORTED (run
+ * it with tdbg_execute); `out` too small fails with TDBG_ERR_INVALID.  The
+ * array keeps the last call's contents; every kernel initialises what it
+ * reads.  Calls on one GPU share its array, so they are serialised by the
+ * library (a second call waits for the first).  The plan-specialised scan
+ * kernel is used when an earlier tdbg_execute of the same plan shape
+ * compiled it (compiling allocates), else the generic one. */
+#define TDBG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int32_t tdbg_execute_noalloc(tdbg_plan* plan, const tdbg_rg_ref* rgs, uint32_t n_rgs, void* out,
+                             uint64_t out_capacity, tdbg_column_view* cols, uint32_t n_cols, uint64_t* n_rows);
+
 uint64_t tdbg_result_num_rows(const tdbg_result* r);
 uint32_t tdbg_result_num_columns(const tdbg_result* r);
 int32_t tdbg_result_column(const tdbg_result* r, uint32_t i, tdbg_column_view* out);

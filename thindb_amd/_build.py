@@ -22,7 +22,7 @@ GPU_LIB = PKG / "libthindb_gpu.so"
 GEN_LIB = PKG / "libthindb_gen.so"
 
 HIP_SOURCES = ["scan_agg.hip", "order.hip", "radix.hip", "batch.hip", "decode.hip", "distinct.hip", "runtime.cpp", "batch.cpp",
-               "server.cpp", "jit.cpp", "segment.cpp"]
+               "server.cpp", "jit.cpp", "segment.cpp", "noalloc.cpp"]
 # Device headers the run-time compiler needs, embedded as string literals.
 JIT_HEADERS = [("kJitEngineH", "engine.h"), ("kJitScanCommonH", "scan_common.h")]
 ARCH = os.environ.get("THINDB_GPU_ARCH", "gfx950")

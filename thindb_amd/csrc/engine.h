@@ -362,7 +362,10 @@ hipError_t launch_fill_records(uint8_t* state, uint64_t n_rec, uint32_t rec_byte
                                hipStream_t s);
 hipError_t launch_scan_agg(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s);
 // Plan-specialised kernel (jit.cpp); hipErrorNotSupported = use launch_scan_agg.
-hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s, bool* used);
+// `cached_only`: launch only a kernel an earlier call compiled (compiling
+// and loading a module allocates), else hipErrorNotSupported.
+hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s, bool* used,
+                           bool cached_only = false);
 const char* jit_last_log();
 hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_counts, uint32_t* gids,
                           uint64_t* n_out, hipStream_t s);

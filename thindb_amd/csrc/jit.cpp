@@ -242,7 +242,8 @@ static bool compile(const std::string& src, std::vector<char>& code, std::string
 
 // Launches the plan-specialised kernel; returns hipErrorNotSupported when
 // run-time compilation is unavailable so the caller takes the generic one.
-hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s, bool* used) {
+hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s, bool* used,
+                           bool cached_only) {
   *used = false;
   if (!jit_enabled() || p.n_cols > 8) return hipErrorNotSupported;
   int dev = 0;
@@ -252,6 +253,10 @@ hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, h
   hipFunction_t fn = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_mu);
+    if (cached_only) {
+      auto it = g_cache.find(key);
+      if (it == g_cache.end() || !it->second.fn) return hipErrorNotSupported;
+    }
     JitKernel& k = g_cache[key];
     if (k.failed) return hipErrorNotSupported;
     if (!k.fn) {

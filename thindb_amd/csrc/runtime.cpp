@@ -100,6 +100,18 @@ void dev_free(tdbg_context* ctx, void* p, uint64_t bytes) {
 
 int32_t ensure(tdbg_plan* pl, tdbg_plan::Buf& b, uint64_t bytes) {
   if (b.bytes >= bytes && b.p) return TDBG_OK;
+  if (pl->arena_on) {  // tdbg_execute_noalloc: carve from the scratch region, never allocate
+    tdbg_plan::Arena& A = pl->arena;
+    const uint64_t at = (A.used + 255) & ~255ull;
+    const uint64_t want = std::max<uint64_t>(bytes, 256);
+    if (at > A.cap || want > A.cap - at)
+      return fail(TDBG_ERR_UNSUPPORTED, "execution needs more than the %llu-byte no-alloc scratch array",
+                  (unsigned long long)A.cap);
+    b.p = A.base + at;
+    b.bytes = want;
+    A.used = at + want;
+    return TDBG_OK;
+  }
   dev_free(pl->ctx, b.p, b.bytes);
   b.p = nullptr;
   b.bytes = 0;
@@ -1035,6 +1047,8 @@ int32_t tdbg_context_create(int32_t device, uint64_t hbm_budget_bytes, tdbg_cont
   if (e != hipSuccess || n == 0) return fail(TDBG_ERR_DEVICE, "no HIP device: %s", hipGetErrorString(e));
   if (device < 0 || device >= n) return fail(TDBG_ERR_INVALID, "device %d of %d", device, n);
   HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  // the device's copy of the no-alloc scratch array, resolved once here
+  if (int32_t rc = noalloc_setup(device)) return rc;
   auto* ctx = new tdbg_context();
   ctx->device = device;
   hipDeviceGetAttribute(&ctx->n_cu, hipDeviceAttributeMultiprocessorCount, device);
@@ -1512,14 +1526,102 @@ int32_t tdbg::rt::run_distinct(tdbg_plan* pl, Table* t, const ScanSet& S, const 
   return TDBG_OK;
 }
 
+namespace {
+
+// tdbg_execute_noalloc: for one execution, the plan's device buffers are
+// set aside and ensure() carves fresh ones from the scratch region; the
+// plan's own buffers come back afterwards, untouched.
+struct ArenaScope {
+  tdbg_plan* pl;
+  tdbg_plan::Buf saved[20];
+  static constexpr tdbg_plan::Buf tdbg_plan::*kBufs[20] = {
+      &tdbg_plan::state, &tdbg_plan::tiles, &tdbg_plan::err, &tdbg_plan::gids, &tdbg_plan::counts,
+      &tdbg_plan::sel, &tdbg_plan::scratch, &tdbg_plan::out, &tdbg_plan::tmpl, &tdbg_plan::nout,
+      &tdbg_plan::rx_misc, &tdbg_plan::rx_rows1, &tdbg_plan::rx_rows2, &tdbg_plan::rx_leaf, &tdbg_plan::rx_words,
+      &tdbg_plan::having_gids, &tdbg_plan::dist_set, &tdbg_plan::dist_counts, &tdbg_plan::having_leaf,
+      &tdbg_plan::dist_entries};  // every buffer but having_nodes, which the plan uploaded at creation
+  ArenaScope(tdbg_plan* p, uint8_t* base, uint64_t cap) : pl(p) {
+    for (int i = 0; i < 20; i++) {
+      saved[i] = pl->*kBufs[i];
+      pl->*kBufs[i] = tdbg_plan::Buf{};
+    }
+    pl->arena = tdbg_plan::Arena{base, cap, 0};
+    pl->arena_on = true;
+  }
+  ~ArenaScope() {
+    for (int i = 0; i < 20; i++) pl->*kBufs[i] = saved[i];
+    pl->arena_on = false;
+    pl->arena = tdbg_plan::Arena{};
+  }
+};
+
+int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
+                 const NoAllocOut* na);
+
+}  // namespace
+
 int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out) {
   if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
+  return run_plan(pl, rgs, n_rgs, views, out, nullptr);
+}
+
+int32_t tdbg::rt::execute_plan_noalloc(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, const NoAllocOut& na) {
+  if (!pl || !na.cols || !na.n_rows || (!na.out && na.out_capacity)) return fail(TDBG_ERR_INVALID, "null argument");
+  *na.n_rows = 0;
+  if (na.n_cols != pl->keys.size() + pl->outs.size())
+    return fail(TDBG_ERR_INVALID, "%u column views for a plan of %zu output columns", na.n_cols,
+                pl->keys.size() + pl->outs.size());
+  return run_plan(pl, rgs, n_rgs, false, nullptr, &na);
+}
+
+namespace {
+
+// The result's column views into host memory holding, for column c, n_out
+// values at data_off[c] and (valid_off[c] != ~0) one validity byte per row,
+// which is rewritten in place as the views' bitmap (bit r of byte r / 8).
+void views_in_place(tdbg_plan* pl, uint64_t n_out, uint8_t* host, const std::vector<uint64_t>& data_off,
+                    const std::vector<uint64_t>& valid_off, tdbg_column_view* cols) {
+  const size_t nk = pl->keys.size();
+  const size_t nc = nk + pl->outs.size();
+  for (size_t c = 0; c < nc; c++) {
+    tdbg_column_view& v = cols[c];
+    v = tdbg_column_view{};
+    if (c < nk) {
+      v.type = pl->keys[c].type;
+      v.nullable = pl->prep.sp.parts[c].nullable;
+    } else {
+      const AggOut& o = pl->outs[c - nk];
+      v.type = o.out_type;
+      if (o.out_type == TDBG_DECIMAL128 || o.out_type == TDBG_DECIMAL64) v.scale = o.scale;
+      v.nullable = o.op != TDBG_AGG_COUNT_STAR && o.op != TDBG_AGG_COUNT && o.op != TDBG_AGG_COUNT_DISTINCT;
+    }
+    v.n_rows = n_out;
+    v.data = n_out ? host + data_off[c] : nullptr;
+    if (valid_off[c] == ~0ull || !n_out) continue;
+    uint8_t* vb = host + valid_off[c];
+    bool any_null = false;
+    for (uint64_t j = 0; j < (n_out + 7) / 8; j++) {  // byte j is written after bytes 8j.. are read
+      uint8_t m = 0;
+      for (uint64_t i = 8 * j; i < std::min<uint64_t>(n_out, 8 * j + 8); i++) {
+        if (vb[i]) m |= (uint8_t)(1u << (i & 7));
+        else any_null = true;
+      }
+      vb[j] = m;
+    }
+    v.validity = any_null ? vb : nullptr;
+  }
+}
+
+int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
+                 const NoAllocOut* na) {
   tdbg_context* ctx = pl->ctx;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   // lock order everywhere: the context's table lock, then the plan's
   std::shared_lock<std::shared_mutex> lk(ctx->mu);
   std::lock_guard<std::mutex> plk(pl->mu);
+  std::unique_ptr<ArenaScope> arena;
+  if (na) arena.reset(new ArenaScope(pl, na->scratch, na->scratch_bytes));
   pl->cancel.store(false);
   pl->last = tdbg_exec_stats{};
   auto tit = ctx->tables.find(pl->table_uid);
@@ -1555,6 +1657,10 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   const uint64_t need_lim = pl->limit == 0 ? ~0ull : pl->offset + pl->limit;
   if (tiles.empty()) {
     pl->last.rows_scanned = 0;
+    if (na) {
+      views_in_place(pl, 0, nullptr, data_off, valid_off, na->cols);
+      return TDBG_OK;
+    }
     return make_result(pl, 0, nullptr, data_off, valid_off, nullptr, out);
   }
   hipStream_t s = pl->stream;
@@ -1579,15 +1685,25 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   const uint64_t tiles_b = tiles.size() * sizeof(Tile), tmpl_b = P.tmpl.size() * 8;
   const uint64_t hl_at = align_up(align_up(tiles_b, 256) + tmpl_b, 256);
   const uint64_t hl_b = P.leaf_having ? sizeof(HavingLeaf) : 0;
-  if ((rc = ensure_pinned(pl->h_in, pl->h_in_cap, hl_at + hl_b))) return rc;
-  memcpy(pl->h_in, tiles.data(), tiles_b);
-  memcpy(pl->h_in + align_up(tiles_b, 256), P.tmpl.data(), tmpl_b);
-  HIP_TRY(hipMemcpyAsync(pl->tiles.p, pl->h_in, tiles_b, hipMemcpyHostToDevice, s), "tiles");
-  HIP_TRY(hipMemcpyAsync(pl->tmpl.p, pl->h_in + align_up(tiles_b, 256), tmpl_b, hipMemcpyHostToDevice, s), "template");
+  // (no-alloc: pageable host memory that outlives the copies, which end at
+  // the final synchronisation)
+  std::vector<uint8_t> h_pageable;
+  uint8_t* h_in = nullptr;
+  if (na) {
+    h_pageable.resize(hl_at + hl_b);
+    h_in = h_pageable.data();
+  } else {
+    if ((rc = ensure_pinned(pl->h_in, pl->h_in_cap, hl_at + hl_b))) return rc;
+    h_in = pl->h_in;
+  }
+  memcpy(h_in, tiles.data(), tiles_b);
+  memcpy(h_in + align_up(tiles_b, 256), P.tmpl.data(), tmpl_b);
+  HIP_TRY(hipMemcpyAsync(pl->tiles.p, h_in, tiles_b, hipMemcpyHostToDevice, s), "tiles");
+  HIP_TRY(hipMemcpyAsync(pl->tmpl.p, h_in + align_up(tiles_b, 256), tmpl_b, hipMemcpyHostToDevice, s), "template");
   if (hl_b) {
     if ((rc = ensure(pl, pl->having_leaf, hl_b))) return rc;
-    memcpy(pl->h_in + hl_at, &P.hleaf, hl_b);
-    HIP_TRY(hipMemcpyAsync(pl->having_leaf.p, pl->h_in + hl_at, hl_b, hipMemcpyHostToDevice, s), "HAVING");
+    memcpy(h_in + hl_at, &P.hleaf, hl_b);
+    HIP_TRY(hipMemcpyAsync(pl->having_leaf.p, h_in + hl_at, hl_b, hipMemcpyHostToDevice, s), "HAVING");
   }
   bool specialized = false;
   const uint64_t n_rec = sp.g_cap;
@@ -1648,7 +1764,7 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
     HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
     HIP_TRY(launch_fill_u32(sp.err, 16, 0, s), "init flags");
     HIP_TRY(hipEventRecord(pl->ev[1], s), "event");
-    if (launch_scan_jit(sp, P.grid, P.block, s, &specialized) != hipSuccess || !specialized) {
+    if (launch_scan_jit(sp, P.grid, P.block, s, &specialized, na != nullptr) != hipSuccess || !specialized) {
       (void)hipGetLastError();
       HIP_TRY(launch_scan_agg(sp, P.grid, P.block, s), "scan_agg launch");
     }
@@ -1763,14 +1879,24 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   if ((rc = ensure(pl, pl->out, ob + 256))) return rc;
   HIP_TRY(launch_finalize(P.fp, sel, n_out, static_cast<uint8_t*>(pl->out.p), s), "finalize");
   HIP_TRY(hipEventRecord(pl->ev[3], s), "event");
-  // the result lands in pinned memory its columns then view in place; a
-  // buffer a live earlier result still views is left to it
-  if (!pl->h_out.p || pl->h_out.p.use_count() > 1 || pl->h_out.cap < ob + 16) {
-    pl->h_out = PinnedBuf{};  // back to the pool unless a result still views it
-    if ((rc = pinned_acquire(ctx->pinned, ob + 16, pl->h_out))) return rc;
+  if (na) {  // straight into the caller's host buffer
+    if (ob > na->out_capacity) {
+      (void)hipStreamSynchronize(s);
+      return fail(TDBG_ERR_INVALID, "result of %llu rows needs %llu output bytes, %llu given",
+                  (unsigned long long)n_out, (unsigned long long)ob, (unsigned long long)na->out_capacity);
+    }
+    if (n_out) HIP_TRY(hipMemcpyAsync(na->out, pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
+    if ((rc = sync_check(pl, "finalize"))) return rc;
+  } else {
+    // the result lands in pinned memory its columns then view in place; a
+    // buffer a live earlier result still views is left to it
+    if (!pl->h_out.p || pl->h_out.p.use_count() > 1 || pl->h_out.cap < ob + 16) {
+      pl->h_out = PinnedBuf{};  // back to the pool unless a result still views it
+      if ((rc = pinned_acquire(ctx->pinned, ob + 16, pl->h_out))) return rc;
+    }
+    if (n_out) HIP_TRY(hipMemcpyAsync(pl->h_out.p.get(), pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
+    if ((rc = sync_check(pl, "finalize"))) return rc;
   }
-  if (n_out) HIP_TRY(hipMemcpyAsync(pl->h_out.p.get(), pl->out.p, ob, hipMemcpyDeviceToHost, s), "result copy");
-  if ((rc = sync_check(pl, "finalize"))) return rc;
   float ms_total = 0, ms_scan = 0;
   hipEventElapsedTime(&ms_total, pl->ev[0], pl->ev[3]);
   hipEventElapsedTime(&ms_scan, pl->ev[1], pl->ev[2]);
@@ -1783,9 +1909,16 @@ int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n
   pl->last.global_mode = P.radix ? 3u : sp.g_mode;
   pl->last.specialized = specialized ? 1u : 0u;
   pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
+  if (na) {
+    views_in_place(pl, n_out, static_cast<uint8_t*>(na->out), data_off, valid_off, na->cols);
+    *na->n_rows = n_out;
+    return TDBG_OK;
+  }
   return make_result(pl, n_out, pl->h_out.p.get(), data_off, valid_off,
                      views ? std::shared_ptr<void>(pl->h_out.p, pl->h_out.p.get()) : nullptr, out);
 }
+
+}  // namespace
 
 extern "C" {
 

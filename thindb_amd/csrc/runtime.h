@@ -218,6 +218,13 @@ struct tdbg_plan {
     uint64_t bytes = 0;
   } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words,
       having_nodes, having_gids, dist_set, dist_counts, having_leaf, dist_entries;
+  // tdbg_execute_noalloc: while set (under `mu`), ensure() carves every
+  // buffer from this device scratch region instead of allocating
+  struct Arena {
+    uint8_t* base = nullptr;
+    uint64_t cap = 0, used = 0;
+  } arena;
+  bool arena_on = false;
   // pinned host staging: inputs (tiles, template) and the result the
   // columns of a returned tdbg_result view
   uint8_t* h_in = nullptr;
@@ -279,6 +286,24 @@ int32_t make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host, const st
 // plan's pinned result buffer (`views`) or are copied out of it, which leaves
 // the buffer free for the plan's next execution (the serving loop).
 int32_t execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out);
+
+// tdbg_execute_noalloc: the same execution with every device buffer carved
+// from `scratch` (device memory of `scratch_bytes`), the plan's staging
+// through pageable host memory, the plan-specialised kernel only when
+// already compiled, and the result written into the caller's host buffer.
+struct NoAllocOut {
+  uint8_t* scratch;
+  uint64_t scratch_bytes;
+  void* out;
+  uint64_t out_capacity;
+  tdbg_column_view* cols;
+  uint32_t n_cols;
+  uint64_t* n_rows;
+};
+int32_t execute_plan_noalloc(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, const NoAllocOut& na);
+// Resolves `device`'s copy of the no-alloc scratch array (noalloc.cpp) with
+// that device current; idempotent.  Called by tdbg_context_create.
+int32_t noalloc_setup(int device);
 
 // A reusable batch for the serving loop (batch.cpp): eager launches, device
 // and pinned buffers kept and grown across runs of different plan sets.

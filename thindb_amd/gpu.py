@@ -268,6 +268,8 @@ EXPORTS = {
     "tdbg_plan_create": (C.c_int32, [C.c_void_p, C.POINTER(PlanDesc), C.POINTER(C.c_void_p)]),
     "tdbg_plan_destroy": (None, [C.c_void_p]),
     "tdbg_execute": (C.c_int32, [C.c_void_p, C.POINTER(RgRef), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "tdbg_execute_noalloc": (C.c_int32, [C.c_void_p, C.POINTER(RgRef), C.c_uint32, C.c_void_p, C.c_uint64,
+                                         C.POINTER(ColumnView), C.c_uint32, C.POINTER(C.c_uint64)]),
     "tdbg_plan_last_stats": (C.c_int32, [C.c_void_p, C.POINTER(ExecStats)]),
     "tdbg_cancel": (C.c_int32, [C.c_void_p]),
     "tdbg_batch_create": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -557,6 +559,7 @@ class Plan:
         h = C.c_void_p()
         check(lib.tdbg_plan_create(ctx.handle, C.byref(desc), C.byref(h)))
         self._h = h
+        self.n_outputs = len(keys) + len(aggs)
 
     @staticmethod
     def rg_array(rgs: Sequence[tuple]):
@@ -579,6 +582,20 @@ class Plan:
                 and not isinstance(rgs[0], int) else self.rg_array(rgs)
             check(self._lib.tdbg_execute(self._h, arr, n, C.byref(out)))
         return _result_columns(self._lib, out)
+
+    def execute_noalloc(self, out_capacity: int, rgs=None) -> list:
+        """tdbg_execute_noalloc: the same result as :meth:`execute`, with no
+        allocation in the library (its scratch array and `out_capacity`
+        bytes of host memory given here)."""
+        buf = (C.c_uint8 * max(1, out_capacity))()
+        views = (ColumnView * self.n_outputs)()
+        n = C.c_uint64()
+        arr, cnt = (None, 0) if rgs is None else self.rg_array(rgs)
+        check(self._lib.tdbg_execute_noalloc(self._h, arr, cnt, buf, out_capacity, views, self.n_outputs,
+                                             C.byref(n)))
+        cols = [_column_from_view(views[i]) for i in range(self.n_outputs)]
+        assert all(len(c.values) == n.value for c in cols)
+        return cols
 
     @property
     def handle(self) -> C.c_void_p:
