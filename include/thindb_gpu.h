@@ -392,8 +392,12 @@ void tdbg_plan_destroy(tdbg_plan* plan);
  * of the table, in staging order) and returns one result batch. */
 int32_t tdbg_execute(tdbg_plan* plan, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out);
 int32_t tdbg_plan_last_stats(const tdbg_plan* plan, tdbg_exec_stats* out);
-/* Asks a running or queued execution of `plan` to stop; it then returns
- * TDBG_ERR_CANCELLED.  Cleared when the next execution starts. */
+/* Asks every execution of `plan` already asked for - running, waiting for the
+ * plan, or queued in a server - to stop; each returns TDBG_ERR_CANCELLED (a
+ * served request cancelled while queued never reaches the GPU).  Executions
+ * asked for after this call are not affected.  Cooperative, as
+ * memory.checkCancelled (src/exec/exec.zig:913-921): device work already
+ * launched finishes first. */
 int32_t tdbg_cancel(tdbg_plan* plan);
 
 /* ---- batched execution --------------------------------------------------

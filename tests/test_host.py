@@ -109,3 +109,17 @@ def test_generator_distributions(gen_lib):
     assert p["k"].min() == 0 and p["k"].max() == 1023 and abs(p["v"]).max() <= 10 ** 6
     rows, card, filt = g.batched_shape(7, 3)
     assert 10 ** 4 <= rows <= 2 * 10 ** 7 and 4 <= card <= 10 ** 6
+
+
+def test_declines_where_over_computed_columns():
+    """A request whose WHERE has conjuncts over computed columns
+    (residual_filter / computed_where, v2_shape_group_topn.zig:71-76) is
+    declined before any device work, so its filter is never dropped."""
+    import pytest
+    from thindb_amd.exec import AggSpec, Column, Declined, Predicate, Request, Table, build_plan
+    from thindb_amd.gpu import Type
+    t = Table([Column("k", Type.INT), Column("v", Type.BIGINT)],
+              {"k": np.arange(10, dtype=np.int32), "v": np.arange(10, dtype=np.int64)})
+    for extra in ({"residual_filter": Predicate("v2", "gt", 0)}, {"computed_where": object()}):
+        with pytest.raises(Declined, match="computed columns"):
+            build_plan(None, t, Request(["k"], [AggSpec("sum", "v", "s")], **extra))

@@ -1398,7 +1398,7 @@ void tdbg_plan_destroy(tdbg_plan* pl) {
 
 int32_t tdbg_cancel(tdbg_plan* pl) {
   if (!pl) return fail(TDBG_ERR_INVALID, "null plan");
-  pl->cancel.store(true);
+  pl->cancel_gen.fetch_add(1);
   return TDBG_OK;
 }
 
@@ -1556,14 +1556,15 @@ struct ArenaScope {
 };
 
 int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
-                 const NoAllocOut* na);
+                 const NoAllocOut* na, uint64_t gen);
 
 }  // namespace
 
-int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out) {
+int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
+                               uint64_t cancel_gen) {
   if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
-  return run_plan(pl, rgs, n_rgs, views, out, nullptr);
+  return run_plan(pl, rgs, n_rgs, views, out, nullptr, cancel_gen);
 }
 
 int32_t tdbg::rt::execute_plan_noalloc(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, const NoAllocOut& na) {
@@ -1572,7 +1573,7 @@ int32_t tdbg::rt::execute_plan_noalloc(tdbg_plan* pl, const tdbg_rg_ref* rgs, ui
   if (na.n_cols != pl->keys.size() + pl->outs.size())
     return fail(TDBG_ERR_INVALID, "%u column views for a plan of %zu output columns", na.n_cols,
                 pl->keys.size() + pl->outs.size());
-  return run_plan(pl, rgs, n_rgs, false, nullptr, &na);
+  return run_plan(pl, rgs, n_rgs, false, nullptr, &na, pl->cancel_gen.load());
 }
 
 namespace {
@@ -1614,7 +1615,7 @@ void views_in_place(tdbg_plan* pl, uint64_t n_out, uint8_t* host, const std::vec
 }
 
 int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
-                 const NoAllocOut* na) {
+                 const NoAllocOut* na, uint64_t gen) {
   tdbg_context* ctx = pl->ctx;
   HIP_TRY(hipSetDevice(ctx->device), "hipSetDevice");
   // lock order everywhere: the context's table lock, then the plan's
@@ -1622,7 +1623,7 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   std::lock_guard<std::mutex> plk(pl->mu);
   std::unique_ptr<ArenaScope> arena;
   if (na) arena.reset(new ArenaScope(pl, na->scratch, na->scratch_bytes));
-  pl->cancel.store(false);
+  if (cancelled(pl, gen)) return fail(TDBG_ERR_CANCELLED, "cancelled before it started");
   pl->last = tdbg_exec_stats{};
   auto tit = ctx->tables.find(pl->table_uid);
   if (tit == ctx->tables.end()) return fail(TDBG_ERR_NOT_STAGED, "table %llu has no staged columns", (unsigned long long)pl->table_uid);
@@ -1770,7 +1771,7 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     }
     HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
   }
-  if (pl->cancel.load()) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
+  if (cancelled(pl, gen)) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
   // compaction (compacted radix leaves listed their groups' positions above)
   if (!P.radix || P.rx.dense_out) {
     if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;
@@ -1807,7 +1808,7 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   if ((rc = sync_check(pl, "scan"))) return rc;
   if (err_bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
   if (err_bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
-  if (pl->cancel.load()) return fail(TDBG_ERR_CANCELLED, "cancelled");
+  if (cancelled(pl, gen)) return fail(TDBG_ERR_CANCELLED, "cancelled");
   if (P.radix && !P.rx.dense_out && n_groups > n_rec)
     return fail(TDBG_ERR_DEVICE, "radix pass emitted %llu groups for %llu records", (unsigned long long)n_groups,
                 (unsigned long long)n_rec);
@@ -1923,7 +1924,8 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
 extern "C" {
 
 int32_t tdbg_execute(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, tdbg_result** out) {
-  return execute_plan(pl, rgs, n_rgs, true, out);
+  // a tdbg_cancel from here on (while this call waits for the plan or runs) stops it
+  return execute_plan(pl, rgs, n_rgs, true, out, pl ? pl->cancel_gen.load() : 0);
 }
 
 uint64_t tdbg_result_num_rows(const tdbg_result* r) { return r ? r->n_rows : 0; }

@@ -208,7 +208,10 @@ struct tdbg_plan {
   uint64_t limit = 0, offset = 0, est_groups = 0;
   std::vector<tdbg::rt::AggOut> outs;
   std::mutex mu;
-  std::atomic<bool> cancel{false};
+  // tdbg_cancel bumps it; an execution captures it when it is asked for
+  // (tdbg_execute's entry, a server ticket's submission) and stops when it
+  // has changed since, so a cancel reaches queued and waiting executions
+  std::atomic<uint64_t> cancel_gen{0};
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   tdbg::rt::Prepared prep;
@@ -285,7 +288,9 @@ int32_t make_result(tdbg_plan* pl, uint64_t n_out, const uint8_t* host, const st
 // One execution of a plan (tdbg_execute): the result's columns view the
 // plan's pinned result buffer (`views`) or are copied out of it, which leaves
 // the buffer free for the plan's next execution (the serving loop).
-int32_t execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out);
+int32_t execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
+                     uint64_t cancel_gen);
+inline bool cancelled(const tdbg_plan* pl, uint64_t gen) { return pl->cancel_gen.load() != gen; }
 
 // tdbg_execute_noalloc: the same execution with every device buffer carved
 // from `scratch` (device memory of `scratch_bytes`), the plan's staging

@@ -30,6 +30,7 @@ int64_t now_ns() {
 
 struct tdbg_ticket {
   tdbg_plan* plan = nullptr;
+  uint64_t cancel_gen = 0;  // the plan's cancel generation at submission
   uint64_t est = 0;
   bool large = false;
   std::mutex m;
@@ -149,27 +150,43 @@ void lane_main(tdbg_server* s, uint32_t lane) {
     }
     const int64_t t0 = now_ns();
     for (tdbg_ticket* t : unit) t->t_start = t0;
-    if (unit.size() == 1) {
+    // requests cancelled while queued end here, before any device work
+    const size_t n_unit = unit.size();
+    std::vector<tdbg_ticket*> live;
+    for (tdbg_ticket* t : unit) {
+      if (cancelled(t->plan, t->cancel_gen)) {
+        fail(TDBG_ERR_CANCELLED, "cancelled while queued");
+        finish(t, TDBG_ERR_CANCELLED, nullptr, 1, lane);
+      } else {
+        live.push_back(t);
+      }
+    }
+    if (live.size() == 1) {
       tdbg_result* r = nullptr;
-      const int32_t rc = execute_plan(unit[0]->plan, nullptr, 0, true, &r);
-      finish(unit[0], rc, rc == TDBG_OK ? r : nullptr, 1, lane);
-    } else {
+      const int32_t rc = execute_plan(live[0]->plan, nullptr, 0, true, &r, live[0]->cancel_gen);
+      finish(live[0], rc, rc == TDBG_OK ? r : nullptr, 1, lane);
+    } else if (!live.empty()) {
       plans.clear();
-      for (tdbg_ticket* t : unit) plans.push_back(t->plan);
+      for (tdbg_ticket* t : live) plans.push_back(t->plan);
       int32_t rc = batch_run_plans(s->batches[lane], plans.data(), (uint32_t)plans.size());
-      for (size_t i = 0; i < unit.size(); i++) {
+      for (size_t i = 0; i < live.size(); i++) {
         tdbg_result* r = nullptr;
         int32_t ri = rc;
         if (rc == TDBG_OK) ri = batch_take_result(s->batches[lane], (uint32_t)i, &r);
-        finish(unit[i], ri, ri == TDBG_OK ? r : nullptr, (uint32_t)unit.size(), lane);
+        if (ri == TDBG_OK && cancelled(live[i]->plan, live[i]->cancel_gen)) {  // cancelled while it ran
+          tdbg_result_free(r);
+          r = nullptr;
+          ri = fail(TDBG_ERR_CANCELLED, "cancelled");
+        }
+        finish(live[i], ri, ri == TDBG_OK ? r : nullptr, (uint32_t)live.size(), lane);
       }
     }
     {
       std::lock_guard<std::mutex> lk(s->mu);
       s->admitted -= est;
-      s->stats.completed += unit.size();
+      s->stats.completed += n_unit;
       s->stats.units++;
-      if (unit.size() > 1) s->stats.batched += unit.size();
+      if (live.size() > 1) s->stats.batched += live.size();
     }
     s->cv.notify_all();
   }
@@ -210,6 +227,7 @@ int32_t tdbg_server_submit(tdbg_server* s, tdbg_plan* plan, tdbg_ticket** out) {
   if (plan->ctx != s->ctx) return fail(TDBG_ERR_INVALID, "plan of another context");
   auto* t = new tdbg_ticket();
   t->plan = plan;
+  t->cancel_gen = plan->cancel_gen.load();  // a tdbg_cancel from here on reaches this request
   t->t_submit = now_ns();
   uint64_t rows = 0;
   t->est = estimate(s->ctx, plan, &rows);

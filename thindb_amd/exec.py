@@ -113,6 +113,10 @@ class Request:
     dop: int = 1
     udf_registry: Optional[object] = None
     est_groups: int = 0
+    # the WHERE's conjuncts over derived / computed columns (:71-76): the GPU
+    # operator declines a request that carries either
+    residual_filter: Optional[PredicateExpr] = None
+    computed_where: Optional[object] = None
 
 
 # ---- tables -------------------------------------------------------------------------
@@ -469,6 +473,8 @@ def build_plan(session: GpuSession, table: Table, request: Request) -> Tuple[gpu
     """Translates a Request into a tdbg plan; raises Declined for uncovered shapes."""
     if request.derived:
         raise Declined("derived columns stay on the CPU path")
+    if request.residual_filter is not None or request.computed_where is not None:
+        raise Declined("a WHERE over computed columns stays on the CPU path")
     if not request.group_cols or not request.aggs:
         raise Declined("needs group keys and aggregates")
     cols: List[str] = []
