@@ -536,3 +536,38 @@ def test_min_max_double_nan_and_signed_zero(session):
         xs = [float(v) for v in t.columns["x"][t.columns["k"] == k]]
         assert bits(mn) in reachable_extremes(xs, False), (k, hex(bits(mn)))
         assert bits(mx) in reachable_extremes(xs, True), (k, hex(bits(mx)))
+
+
+def test_fast_finish_matches_general_path(session, oracle_lib, monkeypatch):
+    """The one-workgroup finish of dense tables (order, slice and typed rows in
+    one launch) against the general compaction + sort path and the oracle:
+    ORDER BY over every word kind with OFFSET, ties broken by the packed key,
+    more ties at the cut than it sorts (it hands over to the general path),
+    unordered LIMIT/OFFSET pages, and no ORDER BY at all."""
+    rng = np.random.default_rng(77)
+    n = 200_000
+    base = random_table(rng, n, rg_rows=8192, nulls=True)
+    cols = dict(base.columns)
+    cols["k1"] = rng.integers(-3000, 3000, n).astype(np.int32)  # 6,000 groups spanning both signs
+    t = Table(base.schema, cols, base.validity, rg_rows=8192)
+    # 5,000 groups of one row and 5,000 of two: a cut at COUNT 1 ties 5,000 groups
+    k = np.concatenate([np.arange(5000), np.arange(5000, 10000).repeat(2)]).astype(np.int32)
+    ties = Table([Column("k", Type.INT), Column("v", Type.BIGINT)], {"k": k, "v": np.arange(len(k), dtype=np.int64)},
+                 rg_rows=4096)
+    cases = [(t, Request(["k1"], ALL_AGGS, [SortSpec("sb", True)], 100)),
+             (t, Request(["k1"], ALL_AGGS, [SortSpec("c"), SortSpec("ad", True)], 50, 7)),
+             (t, Request(["k1", "k2"], ALL_AGGS, [SortSpec("mins")], 1000, 24)),
+             (t, Request(["k1"], ALL_AGGS, [SortSpec("sm")], 10)),
+             (t, Request(["k1"], ALL_AGGS, [SortSpec("c", True)], 300)),   # ties decided by the packed key
+             (ties, Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s")], [SortSpec("c")], 10)),
+             (ties, Request(["k"], [AggSpec("count", None, "c")], [SortSpec("c", True)], 3000, 1000)),
+             (t, Request(["k1"], ALL_AGGS[:5], limit=40, offset=9)),
+             (t, Request(["k2"], ALL_AGGS))]
+    for table, req in cases:
+        monkeypatch.setenv("THINDB_GPU_FAST_FINISH", "1")
+        got, _, st = check_vs_oracle(session, oracle_lib, table, req)
+        monkeypatch.setenv("THINDB_GPU_FAST_FINISH", "0")
+        b, _ = run_gpu(session, table, req)
+        general = rows_of(b)
+        nk = len(req.group_cols)
+        assert [r[:nk] for r in got] == [r[:nk] for r in general]  # the same groups in the same order

@@ -359,7 +359,7 @@ enum BatchErr : uint32_t { ERR_BATCH_WAIT = 4u };  // a dependency wait gave up 
 #if !defined(__HIPCC_RTC__)
 // ---- kernel launchers (defined in the .hip files) -------------------------
 hipError_t launch_fill_records(uint8_t* state, uint64_t n_rec, uint32_t rec_bytes, const uint64_t* tmpl,
-                               hipStream_t s);
+                               hipStream_t s, uint32_t* flags = nullptr, uint32_t n_flags = 0);
 hipError_t launch_scan_agg(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s);
 // Plan-specialised kernel (jit.cpp); hipErrorNotSupported = use launch_scan_agg.
 // `cached_only`: launch only a kernel an earlier call compiled (compiling
@@ -397,6 +397,29 @@ hipError_t launch_distinct_gather(const FinalParams& fp, const DistinctParams& D
 hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n_out, uint8_t* out,
                            hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+// One workgroup finishes a dense group table of at most kFastRec records:
+// occupancy, ORDER BY (radix select of the first varying order word, then a
+// bitonic sort of the candidates by every word and the slot) or slot order
+// without one, OFFSET/LIMIT, and the typed result columns - in one launch,
+// with the group count and the scan's error flags in a header, so the host
+// synchronises once per query.  `fallback` in the header: more than
+// kFastCand groups tie at the cut (the host runs the general path over the
+// same records).
+struct FastHeader {
+  uint64_t n_groups;  // occupied records
+  uint64_t n_out;     // result rows written
+  uint32_t err;       // the scan's error flags
+  uint32_t fallback;
+  uint64_t pad[5];
+};
+constexpr uint32_t kFastRec = 65536;   // records one workgroup finishes
+constexpr uint32_t kFastNeed = 1024;   // OFFSET + LIMIT it serves
+constexpr uint32_t kFastCand = 2048;   // candidates it sorts
+constexpr uint32_t kFastWords = 4;     // order words from the first varying one on
+constexpr uint32_t kFastUnordered = 4096;  // records of an unordered, unlimited result
+hipError_t launch_dense_topn(const OrderParams& op, const FinalParams& fp, uint32_t n_rec, uint32_t offset,
+                             uint32_t need, const uint32_t* err, uint8_t* out, FastHeader* hdr, hipStream_t s);
 hipError_t launch_iota(uint32_t* p, uint64_t n, hipStream_t s);
 // Radix partitioning phases (radix.hip): RX_HIST .. RX_AGG in order.
 enum RadixPhase : int { RX_HIST = 0, RX_OFFSETS, RX_SCATTER1, RX_SCATTER2, RX_AGG };

@@ -1315,6 +1315,233 @@ hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n
   return hipGetLastError();
 }
 
+// ---- one workgroup: dense table -> ordered, sliced, typed result --------------------
+
+constexpr uint32_t kFastThreads = 1024;
+constexpr uint32_t kFastBins = 4096;  // 12-bit digits of the radix select
+
+// Block-wide sum of one u32 per thread (result to every thread).
+__device__ __forceinline__ uint32_t fast_block_sum(uint32_t v, uint32_t* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (uint32_t w = 0; w < kFastThreads / 64; w++) t += red[w];
+  return t;
+}
+
+// The groups of [0, n_rec) at result positions [offset, need) in slot order
+// (no ORDER BY): rounds of kFastThreads slots, positions by wavefront ballots.
+__device__ void fast_slot_order(const OrderParams& P, const FinalParams& F, uint32_t n_rec, uint32_t offset,
+                                uint32_t need, uint8_t* out, uint32_t* red, uint64_t* n_groups, uint64_t* n_out) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t base = 0;  // position of this round's first occupied slot
+  for (uint32_t r0 = 0; r0 < n_rec; r0 += kFastThreads) {
+    const uint32_t g = r0 + threadIdx.x;
+    const bool occ = g < n_rec && occupied(P, g);
+    const uint64_t m = __ballot(occ);
+    __syncthreads();
+    if (lane == 0) red[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t w = 0; w < kFastThreads / 64; w++) {
+      if (w < wave) before += red[w];
+      total += red[w];
+    }
+    const uint32_t pos = base + before + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    if (occ && pos >= offset && pos < need) finalize_row(F, g, pos - offset, out);
+    base += total;
+  }
+  *n_groups = base;
+  *n_out = base > offset ? (need < base ? need : base) - offset : 0;
+}
+
+__global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderParams P, const FinalParams F,
+                                                                  uint32_t n_rec, uint32_t offset, uint32_t need,
+                                                                  const uint32_t* err, uint8_t* out,
+                                                                  FastHeader* hdr) {
+  __shared__ uint32_t hist[kFastBins];
+  __shared__ uint64_t cw[kFastCand * kFastWords];  // candidates' words from the first varying one on
+  __shared__ uint32_t cg[kFastCand];               // their slots
+  __shared__ uint32_t red[kFastThreads / 64];
+  __shared__ unsigned long long mm_min, mm_max;
+  __shared__ uint32_t s_cut, s_n;
+  uint64_t n_groups = 0, n_out = 0;
+  uint32_t fallback = 0;
+  // first varying order word (words equal for every group decide nothing)
+  uint32_t first = P.n_comps;
+  for (uint32_t c = 0; c < P.n_comps; c++) {
+    if (threadIdx.x == 0) { mm_min = ~0ull; mm_max = 0ull; }
+    __syncthreads();
+    uint64_t lo = ~0ull, hi = 0;
+    for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) {
+      if (!occupied(P, g)) continue;
+      const uint64_t w = comp_word(P, P.comps[c], g);
+      lo = w < lo ? w : lo;
+      hi = w > hi ? w : hi;
+    }
+    atomicMin(&mm_min, (unsigned long long)lo);
+    atomicMax(&mm_max, (unsigned long long)hi);
+    __syncthreads();
+    const bool varies = mm_min < mm_max;
+    if (varies) { first = c; break; }
+    __syncthreads();
+  }
+  if (first == P.n_comps || P.n_comps - first > kFastWords) {
+    // every group ties on every word (or no ORDER BY): slot order is the order
+    if (P.n_comps - first > kFastWords && first < P.n_comps) {
+      fallback = 1;
+    } else {
+      fast_slot_order(P, F, n_rec, offset, need, out, red, &n_groups, &n_out);
+    }
+  } else {
+    // radix select on word `first`: the region (w & mask) == val holds the
+    // need-th smallest; everything below the region is in
+    const OrderParams& Q = P;
+    const uint64_t lo0 = mm_min, hi0 = mm_max;
+    uint64_t mask = 0, val = 0;
+    int top = 63 - __clzll(lo0 ^ hi0);  // highest varying bit
+    if (top < 63) {
+      mask = ~((2ull << top) - 1);
+      val = lo0 & mask;
+    }
+    uint32_t in = 0;  // groups below the region
+    uint32_t occ_total = 0;
+    {
+      uint32_t c = 0;
+      for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) c += occupied(P, g) ? 1u : 0u;
+      occ_total = fast_block_sum(c, red);
+    }
+    n_groups = occ_total;
+    uint32_t maybe = occ_total;
+    const uint32_t want = need < occ_total ? need : occ_total;
+    while (top >= 0 && in + maybe > kFastCand) {
+      const int sh = top >= 11 ? top - 11 : 0;
+      const uint32_t bits = (uint32_t)(top - sh + 1);
+      for (uint32_t b = threadIdx.x; b < kFastBins; b += kFastThreads) hist[b] = 0;
+      __syncthreads();
+      for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) {
+        if (!occupied(P, g)) continue;
+        const uint64_t w = comp_word(Q, Q.comps[first], g);
+        if ((w & mask) != val) continue;
+        atomicAdd(&hist[(uint32_t)(w >> sh) & ((1u << bits) - 1)], 1u);
+      }
+      __syncthreads();
+      // the bin where the count from the smallest end reaches `want`: each
+      // thread sums its run of kFastBins / kFastThreads bins, an exclusive
+      // scan of those sums places every run, and the run holding the
+      // crossing names the bin
+      {
+        constexpr uint32_t per = kFastBins / kFastThreads;
+        const uint32_t b0 = threadIdx.x * per;
+        uint32_t mine = 0;
+        for (uint32_t q = 0; q < per; q++) mine += hist[b0 + q];
+        const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        uint32_t incl = mine;  // inclusive scan within the wavefront
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t t = __shfl_up(incl, o);
+          if ((int)lane >= o) incl += t;
+        }
+        if (lane == 63) red[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0;
+        for (uint32_t w = 0; w < wave; w++) before += red[w];
+        uint32_t cum = in + before + incl - mine;  // groups below this thread's run
+        if (cum < want && cum + mine >= want) {
+          uint32_t q = 0;
+          for (; q < per; q++) {
+            if (cum + hist[b0 + q] >= want) break;
+            cum += hist[b0 + q];
+          }
+          s_cut = b0 + q;
+          s_n = cum;  // groups below the cut bin
+        }
+        __syncthreads();
+      }
+      const uint32_t b = s_cut;
+      maybe = hist[b];
+      in = s_n;
+      mask |= (((bits >= 64 ? ~0ull : ((1ull << bits) - 1))) << sh);
+      val |= (uint64_t)b << sh;
+      top = sh - 1;
+      __syncthreads();
+    }
+    if (in + maybe > kFastCand) {
+      fallback = 1;  // more than kFastCand groups tie at the cut
+    } else {
+      // candidates: every group whose word is at most the region's top
+      const uint64_t region_max = val | ~mask;
+      const uint32_t nw = P.n_comps - first;
+      if (threadIdx.x == 0) s_n = 0;
+      __syncthreads();
+      for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) {
+        if (!occupied(P, g)) continue;
+        const uint64_t w = comp_word(Q, Q.comps[first], g);
+        if (w > region_max) continue;
+        const uint32_t at = atomicAdd(&s_n, 1u);
+        cg[at] = g;
+        cw[at * kFastWords] = w;
+        for (uint32_t c = 1; c < nw; c++) cw[at * kFastWords + c] = comp_word(Q, Q.comps[first + c], g);
+      }
+      __syncthreads();
+      const uint32_t n = s_n;
+      uint32_t m = 1;
+      while (m < n) m <<= 1;
+      for (uint32_t i = n + threadIdx.x; i < m; i += kFastThreads) cg[i] = 0xffffffffu;  // after every group
+      __syncthreads();
+      // bitonic sort by (words, slot)
+      for (uint32_t k = 2; k <= m; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+          for (uint32_t i = threadIdx.x; i < m; i += kFastThreads) {
+            const uint32_t l = i ^ j;
+            if (l <= i) continue;
+            const uint32_t ga = cg[i], gb = cg[l];
+            bool l_less;  // element l orders before element i
+            if (gb == 0xffffffffu) l_less = false;
+            else if (ga == 0xffffffffu) l_less = true;
+            else {
+              l_less = gb < ga;
+              for (uint32_t c = 0; c < nw; c++) {
+                const uint64_t xa = cw[i * kFastWords + c], xb = cw[l * kFastWords + c];
+                if (xa != xb) { l_less = xb < xa; break; }
+              }
+            }
+            const bool up = (i & k) == 0;
+            if (l_less == up) {
+              cg[i] = gb;
+              cg[l] = ga;
+              for (uint32_t c = 0; c < nw; c++) {
+                const uint64_t t = cw[i * kFastWords + c];
+                cw[i * kFastWords + c] = cw[l * kFastWords + c];
+                cw[l * kFastWords + c] = t;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      const uint32_t end = want < n ? want : n;
+      for (uint32_t i = offset + threadIdx.x; i < end; i += kFastThreads) finalize_row(F, cg[i], i - offset, out);
+      n_out = end > offset ? end - offset : 0;
+    }
+  }
+  if (threadIdx.x == 0) {
+    hdr->n_groups = n_groups;
+    hdr->n_out = fallback ? 0 : n_out;
+    hdr->err = err[0];
+    hdr->fallback = fallback;
+  }
+}
+
+hipError_t launch_dense_topn(const OrderParams& op, const FinalParams& fp, uint32_t n_rec, uint32_t offset,
+                             uint32_t need, const uint32_t* err, uint8_t* out, FastHeader* hdr, hipStream_t s) {
+  if (n_rec == 0 || n_rec > kFastRec) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dense_topn_kernel, dim3(1), dim3(kFastThreads), 0, s, op, fp, n_rec, offset, need, err, out,
+                     hdr);
+  return hipGetLastError();
+}
+
 // ---- batched compaction and finalize ---------------------------------------------
 // Blocks of kBatchChunk (= kCompactChunk) records or output rows of one plan
 // each; a plan's compacted positions are contiguous in block order.  A plan

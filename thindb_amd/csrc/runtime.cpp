@@ -494,6 +494,7 @@ int32_t collect_scan(tdbg_plan* pl, Table* t, const tdbg_rg_ref* rgs, uint32_t n
 }
 
 int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint) {
+  pl->prep_stamp++;  // what an execution prepared and uploaded is stale from here on
   const uint64_t n_rows_scan = S.rows;
   const uint32_t n_tiles = (uint32_t)S.tiles.size();
   const std::vector<MergedPred>& ranges = S.ranges;
@@ -1648,7 +1649,20 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   const ScanSet& S = C.S;
   const std::vector<Tile>& tiles = S.tiles;
   const uint64_t rows = S.rows;
-  if ((rc = prepare(pl, t, S, PlanHint{}))) return rc;
+  // The plan specialised to this scan is reused while the scan set is (and
+  // nothing else - a batch's hints - prepared the plan since, and the
+  // planner's switches read the same).
+  std::string env_now;
+  for (const char* v : {"THINDB_GPU_RADIX", "THINDB_GPU_RADIX_LEAF_BITS", "THINDB_GPU_RADIX_BITS1"}) {
+    const char* e = getenv(v);
+    env_now += e ? e : "-";
+    env_now += '|';
+  }
+  if (!(hit && pl->exec_prep_stamp == pl->prep_stamp && pl->exec_prep_env == env_now)) {
+    if ((rc = prepare(pl, t, S, PlanHint{}))) return rc;
+    pl->exec_prep_stamp = pl->prep_stamp;
+    pl->exec_prep_env = env_now;
+  }
   Prepared& P = pl->prep;
   ScanParams& sp = P.sp;
   // offsets of the output buffer
@@ -1697,14 +1711,26 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     if ((rc = ensure_pinned(pl->h_in, pl->h_in_cap, hl_at + hl_b))) return rc;
     h_in = pl->h_in;
   }
-  memcpy(h_in, tiles.data(), tiles_b);
-  memcpy(h_in + align_up(tiles_b, 256), P.tmpl.data(), tmpl_b);
-  HIP_TRY(hipMemcpyAsync(pl->tiles.p, h_in, tiles_b, hipMemcpyHostToDevice, s), "tiles");
-  HIP_TRY(hipMemcpyAsync(pl->tmpl.p, h_in + align_up(tiles_b, 256), tmpl_b, hipMemcpyHostToDevice, s), "template");
-  if (hl_b) {
-    if ((rc = ensure(pl, pl->having_leaf, hl_b))) return rc;
-    memcpy(h_in + hl_at, &P.hleaf, hl_b);
-    HIP_TRY(hipMemcpyAsync(pl->having_leaf.p, h_in + hl_at, hl_b, hipMemcpyHostToDevice, s), "HAVING");
+  if (hl_b && (rc = ensure(pl, pl->having_leaf, hl_b))) return rc;
+  // the device copies stay valid while the preparation and the buffers do
+  // (a no-alloc execution carves fresh buffers: it always uploads)
+  const bool uploaded = !na && pl->up_stamp == pl->prep_stamp && pl->up_tiles == pl->tiles.p &&
+                        pl->up_tmpl == pl->tmpl.p && pl->up_hleaf == (hl_b ? pl->having_leaf.p : nullptr);
+  if (!uploaded) {
+    memcpy(h_in, tiles.data(), tiles_b);
+    memcpy(h_in + align_up(tiles_b, 256), P.tmpl.data(), tmpl_b);
+    HIP_TRY(hipMemcpyAsync(pl->tiles.p, h_in, tiles_b, hipMemcpyHostToDevice, s), "tiles");
+    HIP_TRY(hipMemcpyAsync(pl->tmpl.p, h_in + align_up(tiles_b, 256), tmpl_b, hipMemcpyHostToDevice, s), "template");
+    if (hl_b) {
+      memcpy(h_in + hl_at, &P.hleaf, hl_b);
+      HIP_TRY(hipMemcpyAsync(pl->having_leaf.p, h_in + hl_at, hl_b, hipMemcpyHostToDevice, s), "HAVING");
+    }
+    if (!na) {
+      pl->up_stamp = pl->prep_stamp;
+      pl->up_tiles = pl->tiles.p;
+      pl->up_tmpl = pl->tmpl.p;
+      pl->up_hleaf = hl_b ? pl->having_leaf.p : nullptr;
+    }
   }
   bool specialized = false;
   const uint64_t n_rec = sp.g_cap;
@@ -1762,8 +1788,8 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
               "group positions");
   } else {
     HIP_TRY(hipEventRecord(pl->ev[0], s), "event");
-    HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s), "init records");
-    HIP_TRY(launch_fill_u32(sp.err, 16, 0, s), "init flags");
+    HIP_TRY(launch_fill_records(sp.g_state, sp.g_cap, sp.g_rec, static_cast<const uint64_t*>(pl->tmpl.p), s, sp.err,
+                                16), "init records and flags");
     HIP_TRY(hipEventRecord(pl->ev[1], s), "event");
     if (launch_scan_jit(sp, P.grid, P.block, s, &specialized, na != nullptr) != hipSuccess || !specialized) {
       (void)hipGetLastError();
@@ -1772,6 +1798,69 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     HIP_TRY(hipEventRecord(pl->ev[2], s), "event");
   }
   if (cancelled(pl, gen)) { (void)hipStreamSynchronize(s); return fail(TDBG_ERR_CANCELLED, "cancelled"); }
+  // A dense table of at most kFastRec records without HAVING or
+  // COUNT(DISTINCT), ordered with a small OFFSET + LIMIT (or small and
+  // unordered): one workgroup orders, slices and writes the result, and one
+  // copy brings it back with the group count and the error flags, so the
+  // query synchronises once.  THINDB_GPU_FAST_FINISH=0 keeps the general path.
+  {
+    const char* ff_env = getenv("THINDB_GPU_FAST_FINISH");
+    const uint64_t rows_cap = std::min<uint64_t>(need_lim, n_rec);
+    const bool fast = !na && !(ff_env && ff_env[0] == '0') && !P.radix && sp.g_mode == G_DENSE &&
+                      P.distinct.empty() && pl->having.empty() && n_rec <= kFastRec && pl->offset < rows_cap &&
+                      (P.op.n_comps ? need_lim <= kFastNeed : (need_lim <= kFastNeed || n_rec <= kFastUnordered));
+    if (fast) {
+      const uint64_t cap_out = rows_cap - pl->offset;
+      uint64_t ob = 0;
+      for (size_t c = 0; c < nc; c++) {
+        OutDesc& o = P.fp.outs[c];
+        ob = align_up(ob, 16);
+        data_off[c] = o.data_off = ob;
+        ob += cap_out * o.bytes;
+        if (o.valid_off != ~0ull) {
+          ob = align_up(ob, 16);
+          valid_off[c] = o.valid_off = ob;
+          ob += cap_out;
+        }
+      }
+      const uint64_t head = 256;  // FastHeader, then the columns
+      if ((rc = ensure(pl, pl->out, head + ob + 256))) return rc;
+      uint8_t* d_out = static_cast<uint8_t*>(pl->out.p);
+      HIP_TRY(launch_dense_topn(P.op, P.fp, (uint32_t)n_rec, (uint32_t)pl->offset, (uint32_t)rows_cap, sp.err,
+                                d_out + head, reinterpret_cast<FastHeader*>(d_out), s), "dense top-N");
+      HIP_TRY(hipEventRecord(pl->ev[3], s), "event");
+      if (!pl->h_out.p || pl->h_out.p.use_count() > 1 || pl->h_out.cap < head + ob + 16) {
+        pl->h_out = PinnedBuf{};
+        if ((rc = pinned_acquire(ctx->pinned, head + ob + 16, pl->h_out))) return rc;
+      }
+      uint8_t* h = pl->h_out.p.get();
+      HIP_TRY(hipMemcpyAsync(h, d_out, head + ob, hipMemcpyDeviceToHost, s), "result copy");
+      if ((rc = sync_check(pl, "scan"))) return rc;
+      const FastHeader hd = *reinterpret_cast<const FastHeader*>(h);
+      if (hd.err & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
+      if (hd.err & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
+      if (cancelled(pl, gen)) return fail(TDBG_ERR_CANCELLED, "cancelled");
+      if (!hd.fallback) {
+        if (hd.n_out > cap_out) return fail(TDBG_ERR_DEVICE, "dense top-N wrote %llu rows of %llu",
+                                            (unsigned long long)hd.n_out, (unsigned long long)cap_out);
+        float ms_total = 0, ms_scan = 0;
+        hipEventElapsedTime(&ms_total, pl->ev[0], pl->ev[3]);
+        hipEventElapsedTime(&ms_scan, pl->ev[1], pl->ev[2]);
+        pl->last.total_ms = ms_total;
+        pl->last.scan_ms = ms_scan;
+        pl->last.rows_scanned = rows;
+        pl->last.bytes_scanned = scanned_bytes;
+        pl->last.n_groups = hd.n_groups;
+        pl->last.lds_mode = sp.lds_mode;
+        pl->last.global_mode = sp.g_mode;
+        pl->last.specialized = specialized ? 1u : 0u;
+        pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
+        return make_result(pl, hd.n_out, h + head, data_off, valid_off,
+                           views ? std::shared_ptr<void>(pl->h_out.p, h) : nullptr, out);
+      }
+      // more than kFastCand groups tie at the cut: the general path below
+    }
+  }
   // compaction (compacted radix leaves listed their groups' positions above)
   if (!P.radix || P.rx.dense_out) {
     if ((rc = ensure(pl, pl->counts, ((n_rec + 4095) / 4096 + 1) * 4))) return rc;

@@ -221,6 +221,13 @@ struct tdbg_plan {
     uint64_t bytes = 0;
   } state, tiles, err, gids, counts, sel, scratch, out, tmpl, nout, rx_misc, rx_rows1, rx_rows2, rx_leaf, rx_words,
       having_nodes, having_gids, dist_set, dist_counts, having_leaf, dist_entries;
+  // prepare() bumps prep_stamp; an execution reuses the preparation it made
+  // (exec_prep_*) while the stamp, its scan set and the planner's switches
+  // stay the same, and skips re-uploading tiles and the record template
+  // while the buffers it uploaded them to (up_*) are the current ones
+  uint64_t prep_stamp = 0, exec_prep_stamp = ~0ull, up_stamp = ~0ull;
+  std::string exec_prep_env;
+  void *up_tiles = nullptr, *up_tmpl = nullptr, *up_hleaf = nullptr;
   // tdbg_execute_noalloc: while set (under `mu`), ensure() carves every
   // buffer from this device scratch region instead of allocating
   struct Arena {

@@ -6,6 +6,8 @@
 // plan the runtime accepts.  Plans run faster through the kernels jit.cpp
 // specialises per plan signature; this kernel is the path when run-time
 // compilation is unavailable or disabled (THINDB_GPU_JIT=0).
+#include <algorithm>
+
 #include "strip_generic.h"
 
 namespace tdbg {
@@ -54,20 +56,20 @@ hipError_t launch_scan_agg(const ScanParams& p, uint32_t grid, uint32_t block, h
 // Initialise n_rec records from one template record (a fill kernel rather
 // than hipMemsetAsync so the sequence stays graph-capturable).
 __global__ void fill_records_kernel(uint64_t* state, uint64_t n_words, uint32_t rec_words,
-                                    const uint64_t* __restrict__ tmpl) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_words;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    state[i] = tmpl[i % rec_words];
+                                    const uint64_t* __restrict__ tmpl, uint32_t* flags, uint32_t n_flags) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n_flags) flags[t] = 0;
+  for (uint64_t i = t; i < n_words; i += (uint64_t)gridDim.x * blockDim.x) state[i] = tmpl[i % rec_words];
 }
 
 hipError_t launch_fill_records(uint8_t* state, uint64_t n_rec, uint32_t rec_bytes, const uint64_t* tmpl,
-                               hipStream_t s) {
+                               hipStream_t s, uint32_t* flags, uint32_t n_flags) {
   const uint64_t words = n_rec * (rec_bytes / 8);
-  uint64_t blocks = (words + 255) / 256;
+  uint64_t blocks = (std::max<uint64_t>(words, n_flags) + 255) / 256;
   if (blocks > 65536) blocks = 65536;
   if (blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(fill_records_kernel, dim3((uint32_t)blocks), dim3(256), 0, s,
-                     reinterpret_cast<uint64_t*>(state), words, rec_bytes / 8, tmpl);
+                     reinterpret_cast<uint64_t*>(state), words, rec_bytes / 8, tmpl, flags, n_flags);
   return hipGetLastError();
 }
 
