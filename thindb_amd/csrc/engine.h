@@ -413,10 +413,10 @@ struct FastHeader {
   uint32_t fallback;
   uint64_t pad[5];
 };
-constexpr uint32_t kFastRec = 65536;   // records one workgroup finishes
+constexpr uint32_t kFastRec = 12288;   // records one workgroup finishes (12 per thread)
 constexpr uint32_t kFastNeed = 1024;   // OFFSET + LIMIT it serves
-constexpr uint32_t kFastCand = 2048;   // candidates it sorts
-constexpr uint32_t kFastWords = 4;     // order words from the first varying one on
+constexpr uint32_t kFastCand = 1024;   // candidates it sorts
+constexpr uint32_t kFastWords = 4;     // order words it handles
 constexpr uint32_t kFastUnordered = 4096;  // records of an unordered, unlimited result
 hipError_t launch_dense_topn(const OrderParams& op, const FinalParams& fp, uint32_t n_rec, uint32_t offset,
                              uint32_t need, const uint32_t* err, uint8_t* out, FastHeader* hdr, hipStream_t s);

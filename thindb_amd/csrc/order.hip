@@ -1316,8 +1316,15 @@ hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n
 }
 
 // ---- one workgroup: dense table -> ordered, sliced, typed result --------------------
+//
+// Thread t owns slots t, t + kFastThreads, ... (at most kFastJ of them).  The
+// records are read twice at most per order word (occupancy, then each word
+// until one varies), a few loads in flight per thread; the first varying
+// word of every slot then stays in LDS, where the radix select and the
+// candidate gather read it.
 
 constexpr uint32_t kFastThreads = 1024;
+constexpr uint32_t kFastJ = kFastRec / kFastThreads;
 constexpr uint32_t kFastBins = 4096;  // 12-bit digits of the radix select
 
 // Block-wide sum of one u32 per thread (result to every thread).
@@ -1331,113 +1338,125 @@ __device__ __forceinline__ uint32_t fast_block_sum(uint32_t v, uint32_t* red) {
   return t;
 }
 
-// The groups of [0, n_rec) at result positions [offset, need) in slot order
-// (no ORDER BY): rounds of kFastThreads slots, positions by wavefront ballots.
-__device__ void fast_slot_order(const OrderParams& P, const FinalParams& F, uint32_t n_rec, uint32_t offset,
-                                uint32_t need, uint8_t* out, uint32_t* red, uint64_t* n_groups, uint64_t* n_out) {
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t base = 0;  // position of this round's first occupied slot
-  for (uint32_t r0 = 0; r0 < n_rec; r0 += kFastThreads) {
-    const uint32_t g = r0 + threadIdx.x;
-    const bool occ = g < n_rec && occupied(P, g);
-    const uint64_t m = __ballot(occ);
-    __syncthreads();
-    if (lane == 0) red[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t before = 0, total = 0;
-    for (uint32_t w = 0; w < kFastThreads / 64; w++) {
-      if (w < wave) before += red[w];
-      total += red[w];
-    }
-    const uint32_t pos = base + before + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-    if (occ && pos >= offset && pos < need) finalize_row(F, g, pos - offset, out);
-    base += total;
-  }
-  *n_groups = base;
-  *n_out = base > offset ? (need < base ? need : base) - offset : 0;
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int o) {
+  return ((uint64_t)__shfl_xor((uint32_t)(v >> 32), o) << 32) | __shfl_xor((uint32_t)v, o);
 }
 
 __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderParams P, const FinalParams F,
                                                                   uint32_t n_rec, uint32_t offset, uint32_t need,
                                                                   const uint32_t* err, uint8_t* out,
                                                                   FastHeader* hdr) {
-  __shared__ uint32_t hist[kFastBins];
-  __shared__ uint64_t cw[kFastCand * kFastWords];  // candidates' words from the first varying one on
-  __shared__ uint32_t cg[kFastCand];               // their slots
+  __shared__ uint64_t lw[kFastRec];  // each slot's first varying word
+  __shared__ union {
+    uint32_t hist[kFastBins];                  // the radix select's digit counts
+    uint64_t cw[kFastCand * kFastWords];       // then the candidates' words from the first varying one on
+  } u;
+  __shared__ uint32_t cg[kFastCand];           // the candidates' slots
   __shared__ uint32_t red[kFastThreads / 64];
-  __shared__ unsigned long long mm_min, mm_max;
+  __shared__ unsigned long long mm[2];
   __shared__ uint32_t s_cut, s_n;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   uint64_t n_groups = 0, n_out = 0;
   uint32_t fallback = 0;
-  // first varying order word (words equal for every group decide nothing)
-  uint32_t first = P.n_comps;
-  for (uint32_t c = 0; c < P.n_comps; c++) {
-    if (threadIdx.x == 0) { mm_min = ~0ull; mm_max = 0ull; }
+  const uint32_t nc = P.n_comps;  // <= kFastWords (checked by the host)
+  uint32_t occm = 0;
+#pragma unroll 4
+  for (uint32_t j = 0; j < kFastJ; j++) {
+    const uint32_t g = tid + j * kFastThreads;
+    if (g < n_rec && *reinterpret_cast<const uint64_t*>(P.g_state + (uint64_t)g * P.g_rec + P.g_count_off) != 0)
+      occm |= 1u << j;
+  }
+  const uint32_t occ_total = fast_block_sum((uint32_t)__popc(occm), red);
+  n_groups = occ_total;
+  // the first word whose min and max differ; its values stay in lw
+  uint32_t first = nc;
+  for (uint32_t c = 0; c < nc; c++) {
+    if (tid == 0) { mm[0] = ~0ull; mm[1] = 0ull; }
     __syncthreads();
+    const CompDesc cd = P.comps[c];
     uint64_t lo = ~0ull, hi = 0;
-    for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) {
-      if (!occupied(P, g)) continue;
-      const uint64_t w = comp_word(P, P.comps[c], g);
+#pragma unroll 4
+    for (uint32_t j = 0; j < kFastJ; j++) {
+      if (!((occm >> j) & 1u)) continue;
+      const uint32_t g = tid + j * kFastThreads;
+      const uint64_t w = comp_word(P, cd, g);
+      lw[g] = w;
       lo = w < lo ? w : lo;
       hi = w > hi ? w : hi;
     }
-    atomicMin(&mm_min, (unsigned long long)lo);
-    atomicMax(&mm_max, (unsigned long long)hi);
+    for (int o = 32; o > 0; o >>= 1) {  // one LDS atomic per wavefront (atomics on one address serialise)
+      const uint64_t l2 = shfl_xor_u64(lo, o), h2 = shfl_xor_u64(hi, o);
+      lo = l2 < lo ? l2 : lo;
+      hi = h2 > hi ? h2 : hi;
+    }
+    if (lane == 0) {
+      atomicMin(&mm[0], (unsigned long long)lo);
+      atomicMax(&mm[1], (unsigned long long)hi);
+    }
     __syncthreads();
-    const bool varies = mm_min < mm_max;
+    const bool varies = mm[0] < mm[1];
     if (varies) { first = c; break; }
-    __syncthreads();
   }
-  if (first == P.n_comps || P.n_comps - first > kFastWords) {
-    // every group ties on every word (or no ORDER BY): slot order is the order
-    if (P.n_comps - first > kFastWords && first < P.n_comps) {
-      fallback = 1;
-    } else {
-      fast_slot_order(P, F, n_rec, offset, need, out, red, &n_groups, &n_out);
+  if (first == nc) {
+    // every group ties on every word (or no ORDER BY): slot order is the
+    // order; positions by wavefront ballots, round j covering slots
+    // [j * kFastThreads, (j + 1) * kFastThreads)
+    uint32_t base = 0;
+    for (uint32_t j = 0; j * kFastThreads < n_rec; j++) {
+      const bool occ = (occm >> j) & 1u;
+      const uint64_t m = __ballot(occ);
+      __syncthreads();
+      if (lane == 0) red[wave] = (uint32_t)__popcll(m);
+      __syncthreads();
+      uint32_t before = 0, total = 0;
+      for (uint32_t w = 0; w < kFastThreads / 64; w++) {
+        if (w < wave) before += red[w];
+        total += red[w];
+      }
+      const uint32_t pos = base + before + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+      if (occ && pos >= offset && pos < need) finalize_row(F, tid + j * kFastThreads, pos - offset, out);
+      base += total;
     }
+    n_out = base > offset ? (need < base ? need : base) - offset : 0;
   } else {
-    // radix select on word `first`: the region (w & mask) == val holds the
-    // need-th smallest; everything below the region is in
-    const OrderParams& Q = P;
-    const uint64_t lo0 = mm_min, hi0 = mm_max;
-    uint64_t mask = 0, val = 0;
+    // radix select: the region (w & mask) == val holds the want-th smallest
+    // word; every group below the region is in
+    const uint64_t lo0 = mm[0], hi0 = mm[1];
     int top = 63 - __clzll(lo0 ^ hi0);  // highest varying bit
-    if (top < 63) {
-      mask = ~((2ull << top) - 1);
-      val = lo0 & mask;
-    }
-    uint32_t in = 0;  // groups below the region
-    uint32_t occ_total = 0;
-    {
-      uint32_t c = 0;
-      for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) c += occupied(P, g) ? 1u : 0u;
-      occ_total = fast_block_sum(c, red);
-    }
-    n_groups = occ_total;
-    uint32_t maybe = occ_total;
+    uint64_t mask = top < 63 ? ~((2ull << top) - 1) : 0ull;
+    uint64_t val = lo0 & mask;
     const uint32_t want = need < occ_total ? need : occ_total;
+    uint32_t in = 0, maybe = occ_total;
     while (top >= 0 && in + maybe > kFastCand) {
       const int sh = top >= 11 ? top - 11 : 0;
       const uint32_t bits = (uint32_t)(top - sh + 1);
-      for (uint32_t b = threadIdx.x; b < kFastBins; b += kFastThreads) hist[b] = 0;
+      for (uint32_t b = tid; b < kFastBins; b += kFastThreads) u.hist[b] = 0;
       __syncthreads();
-      for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) {
-        if (!occupied(P, g)) continue;
-        const uint64_t w = comp_word(Q, Q.comps[first], g);
-        if ((w & mask) != val) continue;
-        atomicAdd(&hist[(uint32_t)(w >> sh) & ((1u << bits) - 1)], 1u);
+      // words cluster (most groups share the few top digits of a skewed
+      // distribution): lanes with the digit of the wavefront's first active
+      // lane add once through that lane, the others one by one
+      for (uint32_t j = 0; j < kFastJ; j++) {
+        const uint32_t g = tid + j * kFastThreads;
+        const uint64_t w = (occm >> j) & 1u ? lw[g] : 0;
+        const bool act = ((occm >> j) & 1u) && (w & mask) == val;
+        const uint64_t am = __ballot(act);
+        if (!am) continue;
+        const uint32_t d = (uint32_t)(w >> sh) & ((1u << bits) - 1);
+        const int lead = __ffsll((long long)am) - 1;
+        const uint32_t d0 = __shfl(d, lead);
+        const uint64_t same = __ballot(act && d == d0);
+        if ((int)lane == lead) atomicAdd(&u.hist[d0], (uint32_t)__popcll(same));
+        else if (act && d != d0) atomicAdd(&u.hist[d], 1u);
       }
       __syncthreads();
       // the bin where the count from the smallest end reaches `want`: each
-      // thread sums its run of kFastBins / kFastThreads bins, an exclusive
-      // scan of those sums places every run, and the run holding the
-      // crossing names the bin
+      // thread sums its run of kFastBins / kFastThreads bins, a scan of the
+      // runs places them, and the run holding the crossing names the bin
       {
         constexpr uint32_t per = kFastBins / kFastThreads;
-        const uint32_t b0 = threadIdx.x * per;
+        const uint32_t b0 = tid * per;
         uint32_t mine = 0;
-        for (uint32_t q = 0; q < per; q++) mine += hist[b0 + q];
-        const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (uint32_t q = 0; q < per; q++) mine += u.hist[b0 + q];
         uint32_t incl = mine;  // inclusive scan within the wavefront
         for (int o = 1; o < 64; o <<= 1) {
           const uint32_t t = __shfl_up(incl, o);
@@ -1451,8 +1470,8 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
         if (cum < want && cum + mine >= want) {
           uint32_t q = 0;
           for (; q < per; q++) {
-            if (cum + hist[b0 + q] >= want) break;
-            cum += hist[b0 + q];
+            if (cum + u.hist[b0 + q] >= want) break;
+            cum += u.hist[b0 + q];
           }
           s_cut = b0 + q;
           s_n = cum;  // groups below the cut bin
@@ -1460,9 +1479,9 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
         __syncthreads();
       }
       const uint32_t b = s_cut;
-      maybe = hist[b];
+      maybe = u.hist[b];
       in = s_n;
-      mask |= (((bits >= 64 ? ~0ull : ((1ull << bits) - 1))) << sh);
+      mask |= ((1ull << bits) - 1) << sh;
       val |= (uint64_t)b << sh;
       top = sh - 1;
       __syncthreads();
@@ -1472,28 +1491,35 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
     } else {
       // candidates: every group whose word is at most the region's top
       const uint64_t region_max = val | ~mask;
-      const uint32_t nw = P.n_comps - first;
-      if (threadIdx.x == 0) s_n = 0;
+      const uint32_t nw = nc - first;
+      if (tid == 0) s_n = 0;
       __syncthreads();
-      for (uint32_t g = threadIdx.x; g < n_rec; g += kFastThreads) {
-        if (!occupied(P, g)) continue;
-        const uint64_t w = comp_word(Q, Q.comps[first], g);
-        if (w > region_max) continue;
-        const uint32_t at = atomicAdd(&s_n, 1u);
+      for (uint32_t j = 0; j < kFastJ; j++) {
+        const uint32_t g = tid + j * kFastThreads;
+        const uint64_t w = (occm >> j) & 1u ? lw[g] : 0;
+        const bool take = ((occm >> j) & 1u) && w <= region_max;
+        const uint64_t m = __ballot(take);  // one LDS atomic per wavefront and slot round
+        if (!m) continue;
+        const int lead = __ffsll((long long)m) - 1;
+        uint32_t at0 = 0;
+        if ((int)lane == lead) at0 = atomicAdd(&s_n, (uint32_t)__popcll(m));
+        at0 = __shfl(at0, lead);
+        if (!take) continue;
+        const uint32_t at = at0 + (uint32_t)__popcll(m & ((1ull << lane) - 1));
         cg[at] = g;
-        cw[at * kFastWords] = w;
-        for (uint32_t c = 1; c < nw; c++) cw[at * kFastWords + c] = comp_word(Q, Q.comps[first + c], g);
+        u.cw[at * kFastWords] = w;
+        for (uint32_t c = 1; c < nw; c++) u.cw[at * kFastWords + c] = comp_word(P, P.comps[first + c], g);
       }
       __syncthreads();
       const uint32_t n = s_n;
       uint32_t m = 1;
       while (m < n) m <<= 1;
-      for (uint32_t i = n + threadIdx.x; i < m; i += kFastThreads) cg[i] = 0xffffffffu;  // after every group
+      for (uint32_t i = n + tid; i < m; i += kFastThreads) cg[i] = 0xffffffffu;  // after every group
       __syncthreads();
       // bitonic sort by (words, slot)
       for (uint32_t k = 2; k <= m; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-          for (uint32_t i = threadIdx.x; i < m; i += kFastThreads) {
+          for (uint32_t i = tid; i < m; i += kFastThreads) {
             const uint32_t l = i ^ j;
             if (l <= i) continue;
             const uint32_t ga = cg[i], gb = cg[l];
@@ -1503,7 +1529,7 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
             else {
               l_less = gb < ga;
               for (uint32_t c = 0; c < nw; c++) {
-                const uint64_t xa = cw[i * kFastWords + c], xb = cw[l * kFastWords + c];
+                const uint64_t xa = u.cw[i * kFastWords + c], xb = u.cw[l * kFastWords + c];
                 if (xa != xb) { l_less = xb < xa; break; }
               }
             }
@@ -1512,9 +1538,9 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
               cg[i] = gb;
               cg[l] = ga;
               for (uint32_t c = 0; c < nw; c++) {
-                const uint64_t t = cw[i * kFastWords + c];
-                cw[i * kFastWords + c] = cw[l * kFastWords + c];
-                cw[l * kFastWords + c] = t;
+                const uint64_t t = u.cw[i * kFastWords + c];
+                u.cw[i * kFastWords + c] = u.cw[l * kFastWords + c];
+                u.cw[l * kFastWords + c] = t;
               }
             }
           }
@@ -1522,11 +1548,11 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
         }
       }
       const uint32_t end = want < n ? want : n;
-      for (uint32_t i = offset + threadIdx.x; i < end; i += kFastThreads) finalize_row(F, cg[i], i - offset, out);
+      for (uint32_t i = offset + tid; i < end; i += kFastThreads) finalize_row(F, cg[i], i - offset, out);
       n_out = end > offset ? end - offset : 0;
     }
   }
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     hdr->n_groups = n_groups;
     hdr->n_out = fallback ? 0 : n_out;
     hdr->err = err[0];

@@ -1808,6 +1808,7 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     const uint64_t rows_cap = std::min<uint64_t>(need_lim, n_rec);
     const bool fast = !na && !(ff_env && ff_env[0] == '0') && !P.radix && sp.g_mode == G_DENSE &&
                       P.distinct.empty() && pl->having.empty() && n_rec <= kFastRec && pl->offset < rows_cap &&
+                      P.op.n_comps <= kFastWords &&
                       (P.op.n_comps ? need_lim <= kFastNeed : (need_lim <= kFastNeed || n_rec <= kFastUnordered));
     if (fast) {
       const uint64_t cap_out = rows_cap - pl->offset;
