@@ -1,7 +1,15 @@
 #!/usr/bin/env python3
 """Benchmark of the GPU GROUP BY path on one MI355X.
 
-Default workload (the filtered scan -> filter -> GROUP BY -> top-N query on
+With no --workload, one run measures every configuration and prints ONE
+JSON line: the headline (the filtered query below) at the top level, and
+`configs` holding a record per other configuration - plain (4M rows),
+highcard (200M rows, ~149M groups), batched (512 uneven queries in one
+launch) and serving (Poisson arrivals) - each with its own value, roofline
+and CPU baseline, plus `measured_peak`: the device's read and copy bandwidth
+measured in the same run (the roofline's second denominator).
+
+Headline workload (the filtered scan -> filter -> GROUP BY -> top-N query on
 200M rows in 1M-row segments / 65,536-row row groups):
   SELECT region, COUNT(*), SUM(amount), AVG(price) FROM t
   WHERE ts >= T0+10d AND ts < T0+17d
@@ -15,9 +23,10 @@ roofline block prices the dominant kernel (the fused scan/filter/aggregate)
 at 28 algorithmic bytes per row (4+8+8+8) against 8 TB/s, with its duration
 from HIP events on the plan's stream.  For `highcard` it prices the radix
 partition/aggregate kernels together (events around the sequence) at 21
-bytes per row plus 33 bytes of group state per group.  The CPU baseline times the oracle
-(the C restatement of thinDB's low-cardinality grouped aggregate) on a
-bounded sample of the same table on the host cores.
+bytes per row plus 33 bytes of group state per group.  CPU baselines time
+the oracle (the C restatement of thinDB's grouped aggregate) on a bounded
+sample of the same table, on every host core this job has (at most 16, the
+box's share per GPU) and on one core.
 
 Workload `batched`: 512 GROUP BY queries over 512 tables of uneven size
 (rows log-uniform in [1e4, 2e7], key cardinality log-uniform in [4, 1e6],
@@ -27,8 +36,8 @@ persistent launch, replayed as a captured HIP graph, strategies autotuned per
 through the single-query operator.  A step is one pass over all 512 queries,
 results delivered to host memory; `value` is queries per second.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload filtered|plain|highcard|batched]
-       [--having]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+       [--workload all|filtered|plain|highcard|batched|serving] [--having]
 """
 from __future__ import annotations
 
@@ -143,29 +152,76 @@ def stage_from_segments(sess, table, cols, threads: int = 8) -> dict:
             "codec": "zstd level 3, FOR or raw bodies"}
 
 
-def cpu_baseline(table, request, sample_rows: int, threads: int) -> dict:
-    """The oracle (thinDB-algorithm CPU restatement) on the first
-    `sample_rows` rows of the same table, `threads` workers."""
+def host_threads() -> int:
+    """Host cores this job may use: its CPU affinity, at most 16 (the box's share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _time_oracle(table, request, n: int, threads: int, reps: int = 3):
     from oracle import oracle
     from thindb_amd.exec import Table
-    n = min(sample_rows, table.n_rows)
-    sample = Table(table.schema, {k: v[:n] for k, v in table.columns.items()}, rg_rows=table.rg_rows,
-                   segment_rows=table.segment_rows, uid=table.uid)
+    sample = table if n >= table.n_rows else Table(
+        table.schema, {k: v[:n] for k, v in table.columns.items()},
+        {k: v[:n] for k, v in table.validity.items()}, rg_rows=table.rg_rows, segment_rows=table.segment_rows,
+        uid=table.uid)
     oracle.run(sample, request, threads)  # warm
     times = []
-    for _ in range(3):
+    for _ in range(reps):
         t0 = time.perf_counter()
         oracle.run(sample, request, threads)
         times.append(time.perf_counter() - t0)
-    best = min(times)
-    med = sorted(times)[1]
+    return min(n, table.n_rows), sorted(times)[len(times) // 2], min(times)
+
+
+def cpu_baseline(table, request, sample_rows: int, threads: int, dop1_rows: int, kind_note: str) -> dict:
+    """The oracle (thinDB-algorithm CPU restatement) on the first
+    `sample_rows` rows of the same table with `threads` workers, and on the
+    first `dop1_rows` rows with one."""
+    n, med, best = _time_oracle(table, request, sample_rows, threads)
+    n1, med1, best1 = _time_oracle(table, request, dop1_rows, 1)
     return {"value": n / med, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": f"first {n} rows of the same table, median of 3 runs after 1 warm-up "
-                      f"(best {n / best:.4g} rows/s); oracle = C restatement of thinDB's "
-                      f"low-cardinality grouped aggregate, not thinDB itself"}
+                      f"(best {n / best:.4g} rows/s); {kind_note}",
+            "dop1": {"value": n1 / med1, "unit": "rows/s", "cores": 1,
+                     "sample": f"first {n1} rows, median of 3 runs after 1 warm-up"}}
 
 
-def run_batched(args) -> int:
+ORACLE_NOTE = ("oracle = C restatement of thinDB's grouped aggregate (per-worker tables merged in worker order, "
+               "the low-cardinality core's algorithm), not thinDB itself")
+
+
+def measured_peak() -> dict:
+    """The device's read and copy bandwidth in this run: torch's int64 sum
+    over 5.6 GB (read) and copy_ of 2.8 GB (read + write), HIP events,
+    median of 10 after 3 warm-ups."""
+    import torch
+    n = 5_600_000_000 // 8
+    x = torch.ones(n, dtype=torch.int64, device="cuda")
+    y = torch.empty(n // 2, dtype=torch.int64, device="cuda")
+    out = {}
+    for name, fn, nbytes in (("read_gbps", lambda: x.sum(), n * 8), ("copy_gbps", lambda: y.copy_(x[: n // 2]), n * 8)):
+        for _ in range(3):
+            fn()
+        ms = []
+        for _ in range(10):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        out[name] = nbytes / (float(np.median(ms)) / 1e3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    out["method"] = "torch int64 sum over 5.6 GB (read); torch copy_ of 2.8 GB (bytes read + written)"
+    return out
+
+
+def run_batched(args, sess=None, steps: int = 0, warmup: int = 0) -> dict:
     """The batched workload: one batch of all queries vs the same plans run
     one after another; both deliver every result to host memory."""
     import ctypes as C
@@ -174,12 +230,16 @@ def run_batched(args) -> int:
     from thindb_amd import gen, gpu
     from thindb_amd.exec import GpuBatchRun, GpuSession
 
+    steps = steps or args.steps
+    warmup = warmup or args.warmup
+    own = sess is None
     n_tables = args.tables
     t0 = time.perf_counter()
     pairs = [(gen.batched_table(BATCH_SEED, t, args.rows), gen.batched_request(BATCH_SEED, t)) for t in range(n_tables)]
     rows = sum(t.n_rows for t, _ in pairs)
     log(f"generated {n_tables} tables, {rows} rows in {time.perf_counter() - t0:.1f}s")
-    sess = GpuSession(0)
+    if own:
+        sess = GpuSession(0)
     t0 = time.perf_counter()
     run = GpuBatchRun(sess, pairs)
     stage_s = time.perf_counter() - t0
@@ -193,30 +253,30 @@ def run_batched(args) -> int:
             gpu.check(lib.tdbg_execute(p.handle, None, 0, C.byref(out)))
             lib.tdbg_result_free(out)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         sequential()
     sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         sequential()
     sync()
-    seq_s = (time.perf_counter() - t0) / args.steps
+    seq_s = (time.perf_counter() - t0) / steps
     log(f"sequential: {seq_s * 1e3:.2f} ms per {n_tables} queries ({n_tables / seq_s:.1f} queries/s)")
 
     t0 = time.perf_counter()
     run.autotune(2)
     tune_s = time.perf_counter() - t0
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         run.batch.execute()
     routes = [run.batch.route(i) for i in range(n_tables)]
     log(f"autotuned in {tune_s:.1f}s; strategies " +
         str({gpu.Batch.STRATEGIES[k]: sum(r.strategy == k for r in routes) for k in gpu.Batch.STRATEGIES}))
     sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         run.batch.execute()
     sync()
-    batch_s = (time.perf_counter() - t0) / args.steps
+    batch_s = (time.perf_counter() - t0) / steps
     st = run.batch.stats()
     log(f"batched: {batch_s * 1e3:.2f} ms per {n_tables} queries ({n_tables / batch_s:.1f} queries/s), "
         f"device {st.total_ms:.2f} ms, {st.n_items} items, graph {st.graph}, results {st.result_bytes / 1e6:.1f} MB")
@@ -256,24 +316,34 @@ def run_batched(args) -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_threads()
         oracle.run(pairs[0][0], pairs[0][1], threads)
         t0 = time.perf_counter()
         for t, r in pairs:
             oracle.run(t, r, threads)
         cpu_s = time.perf_counter() - t0
+        # one core: every 16th query (the same uneven mix, a sixteenth of it)
+        sub = pairs[::16]
+        t0 = time.perf_counter()
+        for t, r in sub:
+            oracle.run(t, r, 1)
+        cpu1_s = time.perf_counter() - t0
         cpu = {"value": n_tables / cpu_s, "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"all {n_tables} queries once, after a warm-up query; oracle = C restatement of "
-                         f"thinDB's low-cardinality grouped aggregate, not thinDB itself"}
-        log(f"cpu baseline {cpu['value']:.2f} queries/s")
+               "sample": f"all {n_tables} queries once, after a warm-up query; {ORACLE_NOTE}",
+               "dop1": {"value": len(sub) / cpu1_s, "unit": "queries/s", "cores": 1,
+                        "sample": f"every 16th query ({len(sub)} of {n_tables}), once"}}
+        log(f"cpu baseline {cpu['value']:.2f} queries/s on {threads} cores, {cpu['dop1']['value']:.2f} on one")
+    pmc = ROOT / "profiles" / "batch_kernel_512_pmc.json"
+    traffic = json.loads(pmc.read_text())["traffic_bytes"] if pmc.exists() and n_tables == 512 and not args.rows \
+        else None
 
     out = {
         "metric": "rows/s & HBM GB/s for filtered GROUP BY; batched QPS + p50/p99 latency",
         "value": n_tables / batch_s,
         "unit": "queries/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": batch_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -289,20 +359,38 @@ def run_batched(args) -> int:
         "autotune_s": tune_s,
         "kernel_by_strategy": breakdown,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None, "kernel": "batch_kernel",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_source": str(pmc.relative_to(ROOT)) if traffic else None, "kernel": "batch_kernel",
                      "kernel_ms": kernel_ms, "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
     run.close()
-    sess.close()
-    print(json.dumps(out), flush=True)
-    return 0
+    if own:
+        sess.close()
+    return out
 
 
 SERVE_RATES = (100, 250, 500, 1000, 2500, 5000)
 
 
-def run_serving(args) -> int:
+def request_bytes(table, req) -> int:
+    """Native-width bytes of every column a request reads, over the whole table."""
+    from thindb_amd.gpu import NUMPY_DTYPE
+    names = {c for c in req.group_cols}
+    names |= {a.col for a in req.aggs if a.col}
+    stack = [req.where_filter] if req.where_filter is not None else []
+    while stack:
+        e = stack.pop()
+        if hasattr(e, "children"):
+            stack.extend(e.children)
+        elif hasattr(e, "child"):
+            stack.append(e.child)
+        elif hasattr(e, "col"):
+            names.add(e.col)
+    return sum(np.dtype(NUMPY_DTYPE[table.column(n).type]).itemsize for n in names) * table.n_rows
+
+
+def run_serving(args, sess=None, big=None) -> dict:
     """The serving loop under Poisson arrivals: 70% small (batched-workload
     tables of 1e4-1e6 rows), 25% medium (1e7 rows, 1e5 groups), 5% large (the
     200M-row filtered query).  The served form (tdbg_server: continuous
@@ -315,7 +403,9 @@ def run_serving(args) -> int:
     from thindb_amd.exec import And, AggSpec, GpuSession, Predicate, Request, SortSpec, build_plan
 
     rng = np.random.default_rng(41)
-    sess = GpuSession(0)
+    own = sess is None
+    if own:
+        sess = GpuSession(0)
     t0 = time.perf_counter()
     classes = {"small": [], "medium": [], "large": []}
     for t in range(48):
@@ -323,15 +413,18 @@ def run_serving(args) -> int:
         classes["small"].append((gen.batched_table(41, t, n), gen.batched_request(41, t)))
     for t in range(6):
         classes["medium"].append((gen.keyed_table(100 + t, 10_000_000, 100_000), gen.batched_request(41, t)))
-    big = gen.table("filtered", args.rows or 200_000_000)
+    if big is None:
+        big = gen.table("filtered", args.rows or 200_000_000)
     classes["large"].append((big, make_request("filtered")))
     plans = {k: [] for k in classes}
     staged = 0
+    plan_bytes = {}
     for k, pairs in classes.items():
         for table, req in pairs:
             plan, cols = build_plan(sess, table, req)
             staged += sess.stage_table(table, cols)
             plans[k].append(plan)
+            plan_bytes[id(plan)] = request_bytes(table, req)
     log(f"resident: {staged / 1e9:.2f} GB in {sum(len(v) for v in plans.values())} tables, "
         f"{time.perf_counter() - t0:.1f}s")
 
@@ -375,6 +468,7 @@ def run_serving(args) -> int:
             while time.perf_counter() - start < at:
                 pass
             tickets.append(server.submit(p))
+        served_bytes = sum(plan_bytes[id(p)] for p in reqs)
         lat, first, last = [], None, None
         units = {}
         for t in tickets:
@@ -393,6 +487,7 @@ def run_serving(args) -> int:
             n, tot = by.get(k, (0, 0.0))
             by[k] = (n + 1, tot + ms)
         return {"offered_qps": rate, "throughput_qps": len(reqs) / ((last - first) / 1e9),
+                "served_gbps": served_bytes / ((last - first) / 1e9) / 1e9,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                 "mean_ms": float(lat.mean()),
                 "launches": {k: {"count": n, "mean_ms": tot / n} for k, (n, tot) in sorted(by.items())}}
@@ -416,6 +511,27 @@ def run_serving(args) -> int:
             f"units {c.units}, batched {c.batched}, admission waits {c.admission_waits}; launches {sat['launches']}")
         srv.stop()
     served, plain = results["served_saturated"], results["plain_saturated"]
+    # the CPU path over the same mix: the oracle's mean time per request of
+    # each class (a sample of the class's requests; the large query on a
+    # 40M-row prefix, scaled by rows), weighted 70/25/5, closed loop
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = host_threads()
+        per = {}
+        for k, pairs in classes.items():
+            ts = []
+            for table, req in pairs[:8]:
+                n = min(table.n_rows, 40_000_000)
+                got_n, med, _ = _time_oracle(table, req, n, threads, reps=1)
+                ts.append(med * table.n_rows / got_n)
+            per[k] = float(np.mean(ts))
+        mix_s = 0.70 * per["small"] + 0.25 * per["medium"] + 0.05 * per["large"]
+        cpu = {"value": 1.0 / mix_s, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "mean oracle time per request of each class (up to 8 requests per class; the large "
+                         "query timed on its first 40M rows and scaled by rows), weighted 70/25/5; " + ORACLE_NOTE,
+               "per_class_ms": {k: v * 1e3 for k, v in per.items()}}
+        log(f"cpu baseline {cpu['value']:.1f} queries/s on {threads} cores")
     out = {
         "metric": "rows/s & HBM GB/s for filtered GROUP BY; batched QPS + p50/p99 latency",
         "value": served["throughput_qps"],
@@ -434,15 +550,135 @@ def run_serving(args) -> int:
         "single_query_ms": single,
         "sweep": results,
         "speedup_vs_plain_saturated": served["throughput_qps"] / plain["throughput_qps"],
-        "roofline": None,
-        "cpu_baseline": None,
+        # the served work at saturation: the algorithmic column bytes of every
+        # completed request per second of wall clock (GPU busy or not)
+        "roofline": {"bound": "hbm", "achieved": served["served_gbps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": served["served_gbps"] / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel": "every launch of the served mix (wall clock at saturation)"},
+        "cpu_baseline": cpu,
     }
     for ps in plans.values():
         for p in ps:
             p.close()
-    sess.close()
-    print(json.dumps(out), flush=True)
-    return 0
+    if own:
+        sess.close()
+    return out
+
+
+def run_single(args, workload: str, sess, steps: int, warmup: int, table=None):
+    """One query of `workload` (filtered / plain / highcard) over its
+    HBM-resident table; returns (record, table)."""
+    from thindb_amd import gen
+    from thindb_amd.exec import try_build_gpu_group
+
+    rows, bytes_per_row, desc = WORKLOADS[workload]
+    if args.rows and workload == args.workload:
+        rows = args.rows
+    if table is None:
+        log(f"generating {rows} rows ({workload})")
+        t0 = time.perf_counter()
+        table = gen.table(workload, rows)
+        log(f"generated in {time.perf_counter() - t0:.1f}s")
+    opts = args.workload == workload  # --having / --distinct / --where-tree apply to the chosen workload
+    having, distinct, where_tree = opts and args.having, opts and args.distinct, opts and args.where_tree
+    req = make_request(workload, having, distinct, where_tree, args.distinct_col)
+    if where_tree:
+        desc = f"{desc}, WHERE NOT (ts < lo OR ts >= hi)"
+    if having:
+        desc = f"{desc} {HAVING[workload]}"
+    if distinct:
+        desc, bytes_per_row = DISTINCT_DESC.format(col=args.distinct_col), DISTINCT_BYTES_PER_ROW
+    q = try_build_gpu_group(sess, table, req)
+    if q is None:
+        raise RuntimeError(f"GPU operator declined the {workload} workload")
+    cols = q._columns
+    ingest = None
+    if opts and args.from_segments:
+        ingest = stage_from_segments(sess, table, cols)
+    t0 = time.perf_counter()
+    staged = sess.stage_table(table, cols)
+    stage_s = time.perf_counter() - t0
+    if staged:
+        log(f"staged {staged / 1e9:.2f} GB in {stage_s:.2f}s ({staged / stage_s / 1e9:.1f} GB/s incl. PCIe)")
+
+    rgs = q.plan.rg_array([(rg.segment_id, rg.idx) for rg in table.row_groups])
+    for _ in range(warmup):
+        q.plan.execute(rgs)
+    scan_ms, total_ms = [], []
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        q.plan.execute(rgs)
+        st = q.plan.stats()
+        scan_ms.append(st.scan_ms)
+        total_ms.append(st.total_ms)
+    sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / steps
+    st = q.plan.stats()
+    n_rows_scanned = int(st.rows_scanned)
+    alg_bytes = n_rows_scanned * bytes_per_row
+    if workload == "highcard" and not distinct:
+        # plus the compulsory group state: key 9 + count 8 + sum 8 + max 8 bytes per group
+        alg_bytes += int(st.n_groups) * HIGHCARD_GROUP_BYTES
+    if distinct:  # the distinct pass runs after the scan: price the whole device time
+        scan_ms = total_ms
+    avg_scan_s = float(np.mean(scan_ms)) / 1e3
+    achieved = alg_bytes / avg_scan_s / 1e9
+    log(f"{workload}: {ms_per_step:.3f} ms/step wall, scan kernel {np.mean(scan_ms):.3f} ms avg "
+        f"(min {np.min(scan_ms):.3f}), device total {np.mean(total_ms):.3f} ms, groups {st.n_groups}, "
+        f"lds_mode {st.lds_mode} global_mode {st.global_mode}")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = host_threads()
+        big = workload in ("filtered", "highcard")
+        sample = args.cpu_sample_rows or min(rows, 40_000_000 if workload == "filtered" else 10_000_000)
+        cpu = cpu_baseline(table, req, sample, threads, min(sample, 4_000_000 if big else rows),
+                           ORACLE_NOTE + ("; the silo's 2-level radix is not restated: this is the hash-table "
+                                          "fold over the same groups" if workload == "highcard" else ""))
+        log(f"cpu baseline {cpu['value']:.4g} rows/s on {threads} cores, {cpu['dop1']['value']:.4g} on one")
+
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # FETCH_SIZE/WRITE_SIZE counter passes (scaled by a 1 GiB copy kernel).
+    traffic = None
+    radix = int(st.global_mode) == 3
+    pmc = ROOT / "profiles" / (f"radix_{workload}_200M_pmc.json" if radix else f"scan_jit_{workload}_200M_pmc.json")
+    if pmc.exists() and rows == WORKLOADS[workload][0] and (st.specialized or radix) and not having \
+            and not distinct and not where_tree:
+        traffic = json.loads(pmc.read_text())["traffic_bytes"]
+
+    out = {
+        "metric": "rows/s & HBM GB/s for filtered GROUP BY; batched QPS + p50/p99 latency",
+        "value": n_rows_scanned / (ms_per_step / 1e3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded counter-based generator)",
+        "config": {"workload": desc, "rows": n_rows_scanned, "row_group_rows": table.rg_rows,
+                   "segment_rows": table.segment_rows, "groups": int(st.n_groups)},
+        "hbm_gbps": alg_bytes / (ms_per_step / 1e3) / 1e9,
+        "device_ms_per_step": float(np.mean(total_ms)),
+        "stage_gbps_pcie": staged / stage_s / 1e9 if staged else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                     "traffic_source": str(pmc.relative_to(ROOT)) if traffic else None,
+                     "kernel": ("device total: scan + distinct_insert_kernel + gather" if distinct
+                                else "radix pipeline rx_hist+rx_scatter1+rx_hist2+rx_scatter2+rx_agg" if radix
+                                else "tdbg_scan_jit" if st.specialized else "scan_agg_kernel"),
+                     "kernel_ms": float(np.mean(scan_ms)),
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+        "ingest": ingest,
+    }
+    q.deinit()
+    return out, table
 
 
 def main() -> int:
@@ -450,7 +686,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="filtered", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS))
     ap.add_argument("--rows", type=int, default=0,
                     help="override the row count (batched: cap per table; testing only)")
     ap.add_argument("--tables", type=int, default=512, help="batched: number of tables/queries")
@@ -475,14 +711,11 @@ def main() -> int:
         print(f"bench.py measures one GPU: this engine has no multi-GPU path (asked for {args.gpus}); "
               "see the sharding design in DESIGN.md", file=sys.stderr)
         return 2
+    if args.distinct and args.workload != "highcard":
+        log("--distinct runs over the highcard workload")
+        return 2
 
-    from thindb_amd import gen
-    from thindb_amd.exec import GpuSession, try_build_gpu_group
-
-    if args.workload == "batched":
-        return run_batched(args)
-    if args.workload == "serving":
-        return run_serving(args)
+    from thindb_amd.exec import GpuSession
 
     if args.copy_calibration:
         import torch
@@ -495,117 +728,37 @@ def main() -> int:
         log(f"copy calibration: 3 copies of {n} bytes")
         del src, dst
 
-    rows, bytes_per_row, desc = WORKLOADS[args.workload]
-    if args.rows:
-        rows = args.rows
-    log(f"generating {rows} rows ({args.workload})")
-    t0 = time.perf_counter()
-    table = gen.table(args.workload, rows)
-    log(f"generated in {time.perf_counter() - t0:.1f}s")
-    if args.distinct and args.workload != "highcard":
-        log("--distinct runs over the highcard workload")
-        return 2
-    req = make_request(args.workload, args.having, args.distinct, args.where_tree, args.distinct_col)
-    if args.where_tree:
-        desc = f"{desc}, WHERE NOT (ts < lo OR ts >= hi)"
-    if args.having:
-        desc = f"{desc} {HAVING[args.workload]}"
-    if args.distinct:
-        desc, bytes_per_row = DISTINCT_DESC.format(col=args.distinct_col), DISTINCT_BYTES_PER_ROW
-
-    sess = GpuSession(0)
-    q = try_build_gpu_group(sess, table, req)
-    if q is None:
-        log("GPU operator declined the workload")
-        return 1
-    cols = q._columns
-    ingest = None
-    if args.from_segments:
-        ingest = stage_from_segments(sess, table, cols)
-    t0 = time.perf_counter()
-    staged = sess.stage_table(table, cols)
-    stage_s = time.perf_counter() - t0
-    log(f"staged {staged / 1e9:.2f} GB in {stage_s:.2f}s ({staged / stage_s / 1e9:.1f} GB/s incl. PCIe)")
-
-    rgs = q.plan.rg_array([(rg.segment_id, rg.idx) for rg in table.row_groups])
-    for i in range(args.warmup):
-        q.plan.execute(rgs)
-    log(f"warmup {args.warmup} done")
-    scan_ms = []
-    total_ms = []
-    sync()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        res = q.plan.execute(rgs)
-        st = q.plan.stats()
-        scan_ms.append(st.scan_ms)
-        total_ms.append(st.total_ms)
-    sync()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    st = q.plan.stats()
-    n_rows_scanned = int(st.rows_scanned)
-    alg_bytes = n_rows_scanned * bytes_per_row
-    if args.workload == "highcard" and not args.distinct:
-        # plus the compulsory group state: key 9 + count 8 + sum 8 + max 8 bytes per group
-        alg_bytes += int(st.n_groups) * HIGHCARD_GROUP_BYTES
-    if args.distinct:  # the distinct pass runs after the scan: price the whole device time
-        scan_ms = total_ms
-    avg_scan_s = float(np.mean(scan_ms)) / 1e3
-    achieved = alg_bytes / avg_scan_s / 1e9
-    log(f"{ms_per_step:.3f} ms/step wall, scan kernel {np.mean(scan_ms):.3f} ms avg "
-        f"(min {np.min(scan_ms):.3f}), device total {np.mean(total_ms):.3f} ms, groups {st.n_groups}, "
-        f"lds_mode {st.lds_mode} global_mode {st.global_mode}")
-
-    cpu = None
-    if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        sample = args.cpu_sample_rows or min(rows, 40_000_000 if args.workload != "highcard" else 10_000_000)
-        log(f"cpu baseline: oracle on {sample} rows, {threads} threads")
-        cpu = cpu_baseline(table, req, sample, threads)
-        log(f"cpu baseline {cpu['value']:.4g} rows/s")
-
-    # HBM bytes per launch of the dominant kernel, from the committed
-    # FETCH_SIZE/WRITE_SIZE counter passes (scaled by a 1 GiB copy kernel).
-    traffic = None
-    radix = int(st.global_mode) == 3
-    pmc = ROOT / "profiles" / (f"radix_{args.workload}_200M_pmc.json" if radix
-                               else f"scan_jit_{args.workload}_200M_pmc.json")
-    if pmc.exists() and rows == WORKLOADS[args.workload][0] and (st.specialized or radix) and not args.having \
-            and not args.distinct and not args.where_tree:
-        traffic = json.loads(pmc.read_text())["traffic_bytes"]
-
-    out = {
-        "metric": "rows/s & HBM GB/s for filtered GROUP BY; batched QPS + p50/p99 latency",
-        "value": n_rows_scanned / (ms_per_step / 1e3),
-        "unit": "rows/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int64",
-        "data": "synthetic (seeded counter-based generator)",
-        "config": {"workload": desc, "rows": n_rows_scanned, "row_group_rows": table.rg_rows,
-                   "segment_rows": table.segment_rows, "groups": int(st.n_groups)},
-        "hbm_gbps": alg_bytes / (ms_per_step / 1e3) / 1e9,
-        "device_ms_per_step": float(np.mean(total_ms)),
-        "stage_gbps_pcie": staged / stage_s / 1e9 if staged else None,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "traffic_source": str(pmc.relative_to(ROOT)) if traffic else None,
-                     "kernel": ("device total: scan + distinct_insert_kernel + gather" if args.distinct
-                                else "radix pipeline rx_hist+rx_scatter1+rx_hist2+rx_scatter2+rx_agg" if radix
-                                else "tdbg_scan_jit" if st.specialized else "scan_agg_kernel"),
-                     "kernel_ms": float(np.mean(scan_ms)),
-                     "alg_bytes_per_launch": alg_bytes},
-        "cpu_baseline": cpu,
-        "ingest": ingest,
-    }
-    q.deinit()
-    sess.close()
+    if args.workload == "batched":
+        out = run_batched(args)
+    elif args.workload == "serving":
+        out = run_serving(args)
+    elif args.workload != "all":
+        sess = GpuSession(0)
+        out, _ = run_single(args, args.workload, sess, args.steps, args.warmup)
+        sess.close()
+    else:
+        # every configuration in one record: the headline at the top level
+        t_start = time.perf_counter()
+        sess = GpuSession(0)
+        out, filtered = run_single(args, "filtered", sess, args.steps, args.warmup)
+        peak = measured_peak()  # after the headline, so nothing of torch's runs before it
+        log(f"measured peak: read {peak['read_gbps']:.0f} GB/s, copy {peak['copy_gbps']:.0f} GB/s")
+        out["roofline"]["frac_of_measured_read"] = out["roofline"]["achieved"] / peak["read_gbps"]
+        configs = {}
+        configs["plain"], _ = run_single(args, "plain", sess, max(args.steps, 20), max(args.warmup, 3))
+        hc, hc_table = run_single(args, "highcard", sess, min(args.steps, 10), min(args.warmup, 2))
+        configs["highcard"] = hc
+        sess.evict_table(hc_table)
+        del hc_table
+        configs["batched"] = run_batched(args, sess, steps=min(args.steps, 5), warmup=min(args.warmup, 2))
+        configs["serving"] = run_serving(args, sess, big=filtered)
+        for c in configs.values():
+            if c.get("roofline"):
+                c["roofline"]["frac_of_measured_read"] = c["roofline"]["achieved"] / peak["read_gbps"]
+        out["configs"] = configs
+        out["measured_peak"] = peak
+        out["bench_seconds"] = time.perf_counter() - t_start
+        sess.close()
     print(json.dumps(out), flush=True)
     return 0
 

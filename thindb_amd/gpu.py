@@ -420,9 +420,12 @@ def _column_from_view(v: ColumnView) -> ResultColumn:
     if t == Type.DECIMAL128:
         raw = np.ctypeslib.as_array(C.cast(v.data, C.POINTER(C.c_uint64)), shape=(2 * n,)).copy() if n else \
             np.zeros(0, np.uint64)
-        lo = raw[0::2].astype(object)
-        hi = raw[1::2].view(np.int64).astype(object)
-        vals = np.array([int(h) * (1 << 64) + int(l) for l, h in zip(lo, hi)], dtype=object)
+        lo = raw[0::2]
+        hi = raw[1::2].view(np.int64)
+        if np.array_equal(hi, lo.view(np.int64) >> 63):  # every value fits 64 bits
+            vals = lo.view(np.int64).astype(object)
+        else:
+            vals = hi.astype(object) * (1 << 64) + lo.astype(object)
     elif n:
         ct = np.ctypeslib.as_ctypes_type(NUMPY_DTYPE[t])
         vals = np.ctypeslib.as_array(C.cast(v.data, C.POINTER(ct)), shape=(n,)).copy()
