@@ -285,7 +285,7 @@ typedef struct tdbg_exec_stats {
   uint64_t bytes_scanned; /* native-width bytes of every referenced column */
   uint64_t n_groups;
   uint32_t lds_mode;    /* 0 none, 1 direct, 2 hash */
-  uint32_t global_mode; /* 1 dense, 2 hash, 3 radix-partitioned */
+  uint32_t global_mode; /* 1 dense, 2 hash, 3 radix-partitioned by key range, 4 hash-partitioned (wide/sparse keys) */
   uint32_t specialized; /* 1: plan-specialised kernel, 0: generic kernel */
   uint32_t lds_parts;   /* workgroups sharing one tile range (direct mode) */
 } tdbg_exec_stats;

@@ -148,6 +148,40 @@ def order_table(seed: int = 0, n: int = 4000, wide: bool = False) -> Table:
     return Table(schema, cols, valid, rg_rows=256)
 
 
+def wide_key_table(n: int = 20_000, groups: int = 5_000, seed: int = 0, nullable: bool = False,
+                   rg_rows: int = 4096) -> Table:
+    """GROUP BY keys no relative packing narrows: `a` and `b` are BIGINTs
+    drawn from the whole 64-bit range (both signs, the extremes included),
+    so (a, b) packs into 128 bits; `c` is an INT of both signs, NULL in about
+    one row in nine when `nullable` (a, b, c then pack into 64 + 64 + 33
+    bits, beyond 128: with a, c alone into 97).  Rows pick among `groups`
+    (a, b, c) triples with a skew, so groups hold from one row to many;
+    aggregate inputs: v INT, w BIGINT (full range), d DOUBLE, x SMALLINT
+    (nullable)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.iinfo(np.int64).min, np.iinfo(np.int64).max
+    ga = rng.integers(lo, hi, groups, dtype=np.int64, endpoint=True)
+    gb = rng.integers(lo, hi, groups, dtype=np.int64, endpoint=True)
+    ga[:4] = [lo, hi, -1, 0]
+    gb[:4] = [hi, lo, 0, -1]
+    gb[4:8] = gb[0]                  # pairs sharing b: ties broken by a
+    ga[8:12] = ga[1]                 # pairs sharing a: ties broken by b
+    gc = rng.integers(-2 ** 31, 2 ** 31, groups, dtype=np.int64).astype(np.int32)
+    pick = np.minimum((rng.pareto(1.2, n) * groups / 50).astype(np.int64), groups - 1)
+    pick[:groups] = np.arange(groups)  # every group holds a row
+    rng.shuffle(pick)
+    schema = [Column("a", Type.BIGINT), Column("b", Type.BIGINT), Column("c", Type.INT, nullable),
+              Column("v", Type.INT), Column("w", Type.BIGINT), Column("d", Type.DOUBLE),
+              Column("x", Type.SMALLINT, True)]
+    cols = {"a": ga[pick], "b": gb[pick], "c": gc[pick], "v": rng.integers(-10 ** 6, 10 ** 6, n).astype(np.int32),
+            "w": rng.integers(lo, hi, n, dtype=np.int64, endpoint=True), "d": rng.integers(-4000, 4000, n) * 0.25,
+            "x": rng.integers(-500, 500, n).astype(np.int16)}
+    valid = {"x": rng.random(n) > 0.2}
+    if nullable:
+        valid["c"] = rng.random(groups)[pick] > 0.11
+    return Table(schema, cols, valid, rg_rows=rg_rows)
+
+
 def nan_zero_table(n: int = 6000, seed: int = 0) -> Table:
     """A DOUBLE column whose groups hold NaNs (both signs, several payloads),
     +0.0 and -0.0: group 0 only NaNs, 1 NaNs and numbers, 2 only signed

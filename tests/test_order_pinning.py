@@ -19,7 +19,8 @@ import json
 
 import pytest
 
-from helpers import (brute_order, golden, nullable_agg_table, order_table, ranked_table, rows_of, same_values)
+from helpers import (brute_order, golden, nullable_agg_table, order_table, ranked_table, rows_of, same_values,
+                     wide_key_table)
 from thindb_amd.exec import AggSpec, Predicate, Request, SortSpec
 
 def ranked_request(case):
@@ -133,6 +134,40 @@ def test_planted_comparator_mistakes_fail(oracle_lib, plant, case):
     req = order_request(ORDER_CASES[case])
     check_order(oracle_lib, t, req)  # the true comparator passes
     with oracle_lib.planted(getattr(oracle_lib.planted, plant)):
+        with pytest.raises(AssertionError):
+            check_order(oracle_lib, t, req)
+
+
+# ---- 128-bit keys ------------------------------------------------------------------------
+
+WIDE_AGGS = [AggSpec("count", None, "c"), AggSpec("sum", "v", "sv"), AggSpec("min", "w", "mnw"),
+             AggSpec("max", "w", "mxw"), AggSpec("avg", "d", "ad"), AggSpec("sum", "x", "sx"),
+             AggSpec("sum", "w", "sw")]
+
+WIDE_CASES = [
+    (["a", "b"], [SortSpec("c", True)], 60, 0),       # COUNT ties over keys of both signs and the extremes
+    (["a", "b"], [SortSpec("b", True), SortSpec("a")], 40, 5),  # key items over the full range
+    (["b", "a"], [SortSpec("sx")], 50, 0),            # NULL SUMs first, ties by the packed key
+    (["a", "c"], [SortSpec("sw", True)], 30, 0),      # wrapped BIGINT SUM, nullable INT key
+    (["c", "a"], [SortSpec("c"), SortSpec("mxw", True)], 45, 2),
+]
+
+
+@pytest.mark.parametrize("case", range(len(WIDE_CASES)))
+def test_wide_keys_against_brute_force(oracle_lib, case):
+    """Keys that pack into up to 128 bits (two full-range BIGINTs; a BIGINT
+    and a nullable INT) under the brute-force comparator."""
+    keys, order, limit, offset = WIDE_CASES[case]
+    t = wide_key_table(n=5000, groups=1200, seed=case, nullable="c" in keys)
+    check_order(oracle_lib, t, Request(keys, WIDE_AGGS, order, limit, offset))
+
+
+def test_planted_tie_mistake_fails_on_wide_keys(oracle_lib):
+    t = wide_key_table(n=5000, groups=1200, seed=0)
+    keys, order, limit, _ = WIDE_CASES[0]
+    req = Request(keys, WIDE_AGGS, order, limit)
+    check_order(oracle_lib, t, req)
+    with oracle_lib.planted(oracle_lib.planted.TIES_SIGNED):
         with pytest.raises(AssertionError):
             check_order(oracle_lib, t, req)
 

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) distinct_gather_kernel(const FinalParams 
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t gid = gids[i];
     uint8_t* rec = const_cast<uint8_t*>(F.g_state) + (uint64_t)gid * F.g_rec;
-    const uint64_t key = group_key(rec, gid, F.g_mode, F.g_key_off);
+    const uint64_t key = group_key(rec, gid, F.g_mode, F.g_key_off).lo;  // never G_WIDE
     *reinterpret_cast<uint64_t*>(rec + D.dslot_off) = D.counts[key];
   }
 }

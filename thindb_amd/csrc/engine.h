@@ -55,13 +55,25 @@ struct PredDesc {
 // Key part: relative value rel = v - base, 0 <= rel < span, folded into a
 // mixed-radix group index key = sum(rel_i * stride_i).  A nullable part keeps
 // rel 0 for NULL and its base one below the smallest value, so NULL sorts
-// first.
+// first.  A packed 128-bit key (G_WIDE) holds rel as a bit field instead:
+// bits [bit_off, bit_off + bit_width) of the key, parts from the first at the
+// low end, so the last part is the most significant as in the mixed radix;
+// there span 0 stands for 2^64 (a full-range BIGINT).
 struct PartDesc {
   uint32_t col;
   uint32_t nullable;
   int64_t base;
   uint64_t span;
   uint64_t stride;
+  uint32_t bit_off;
+  uint32_t bit_width;  // 0: mixed radix (G_WIDE: a constant part, rel 0)
+};
+
+// A group key as stored in records: the relative key in `lo` (G_DENSE: the
+// record index; G_HASH: the mixed-radix key), or the packed 128-bit key
+// (G_WIDE).
+struct GKey {
+  uint64_t lo, hi;
 };
 
 // Aggregate kernels.  Integer SUM/AVG accumulate exactly: in LDS either one
@@ -89,7 +101,10 @@ struct AggDesc {
 };
 
 enum LdsMode : uint32_t { LDS_NONE = 0, LDS_DIRECT = 1, LDS_HASH = 2 };
-enum GlobalMode : uint32_t { G_DENSE = 1, G_HASH = 2 };
+// G_DENSE: record index = relative key; G_HASH: records carry the u64
+// relative key at g_key_off; G_WIDE: records carry a packed 128-bit key (two
+// u64 words, low first) at g_key_off.
+enum GlobalMode : uint32_t { G_DENSE = 1, G_HASH = 2, G_WIDE = 3 };
 
 struct Tile {
   uint64_t row0;
@@ -146,12 +161,16 @@ struct ScanParams {
 
 // ---- radix partitioning (high cardinality) ----------------------------------
 
-// A surviving row as the partition passes move it: the relative key (u32),
-// a validity word when an aggregate input is nullable, then each kernel
-// aggregate's input value at its column's native width.
+// A surviving row as the partition passes move it: the relative key (u32;
+// hash partitions: the key's 32-bit hash, then the key itself as key_words
+// u64 words at key_off), a validity word when an aggregate input is
+// nullable, then each kernel aggregate's input value at its column's native
+// width.
 struct RowLayout {
   uint32_t stride;             // bytes per row (multiple of 8)
   uint32_t flags_off;          // ~0u: no validity word
+  uint32_t key_off;            // hash partitions: byte offset of the key words (multiple of 8)
+  uint32_t key_words;          // hash partitions: 1 (G_HASH relative key) or 2 (G_WIDE packed key)
   uint32_t val_off[kMaxAggs];  // byte offset of aggregate a's value
   uint32_t val_bytes[kMaxAggs];
 };
@@ -180,6 +199,13 @@ struct RadixParams {
   uint32_t* leaf_groups;       // [n_leaves] groups each leaf wrote (compacted records from its first row),
                                // then scratch for their positions: [1 + n_leaves / 256 + 1] words
   const HavingLeaf* having;    // compacted leaves: only groups passing this HAVING are written (null: all)
+  // Hash partitions (wide or sparse keys): digits are taken from the top of
+  // a 32-bit hash of the key (leaf_bits = the hash bits below both digits),
+  // and each leaf folds its rows into an LDS hash table of ScanParams.lds_cap
+  // slots: u32 tags at tag_off (0 = free), the keys (row.key_words u64 words
+  // per slot) at ScanParams.lds_keys_off, the aggregate arrays as usual.
+  uint32_t hashed;
+  uint32_t tag_off;
 };
 
 // ---- ordering / emission -------------------------------------------------

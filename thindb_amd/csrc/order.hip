@@ -1240,7 +1240,7 @@ __device__ __forceinline__ void finalize_row(const FinalParams& P, uint32_t gid,
       uint8_t valid = 1;
       switch (o.kind) {
         case O_KEY: {
-          const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+          const GKey k = group_key(rec, gid, P.g_mode, P.g_key_off);
           if (part_null(P.parts[o.idx], k)) {
             valid = 0;
             put(out, o.data_off, o.bytes, i, 0);

@@ -60,18 +60,29 @@ __device__ __forceinline__ __int128 load_i128(const uint8_t* p) {
   return (__int128)(((unsigned __int128)(uint64_t)hi << 64) | lo);
 }
 
-__device__ __forceinline__ uint64_t group_key(const uint8_t* rec, uint32_t gid, uint32_t g_mode, uint32_t key_off) {
-  return g_mode == G_DENSE ? (uint64_t)gid : *reinterpret_cast<const uint64_t*>(rec + key_off);
+__device__ __forceinline__ GKey group_key(const uint8_t* rec, uint32_t gid, uint32_t g_mode, uint32_t key_off) {
+  if (g_mode == G_DENSE) return GKey{(uint64_t)gid, 0};
+  const uint64_t* k = reinterpret_cast<const uint64_t*>(rec + key_off);
+  return GKey{k[0], g_mode == G_WIDE ? k[1] : 0ull};
 }
 
-__device__ __forceinline__ uint64_t part_rel(const PartDesc& pt, uint64_t key) { return (key / pt.stride) % pt.span; }
+// The part's relative value: its bit field of a packed key, else its digit
+// of the mixed-radix key.
+__device__ __forceinline__ uint64_t part_rel(const PartDesc& pt, GKey key) {
+  if (pt.bit_width) {
+    const unsigned __int128 k = ((unsigned __int128)key.hi << 64) | key.lo;
+    const uint64_t f = (uint64_t)(k >> pt.bit_off);
+    return pt.bit_width >= 64 ? f : f & ((1ull << pt.bit_width) - 1);
+  }
+  return (key.lo / pt.stride) % pt.span;
+}
 
-__device__ __forceinline__ int64_t part_value(const PartDesc& pt, uint64_t key) {
+__device__ __forceinline__ int64_t part_value(const PartDesc& pt, GKey key) {
   return (int64_t)((uint64_t)pt.base + part_rel(pt, key));
 }
 
 // A nullable part's NULL is relative value 0.
-__device__ __forceinline__ bool part_null(const PartDesc& pt, uint64_t key) {
+__device__ __forceinline__ bool part_null(const PartDesc& pt, GKey key) {
   return pt.nullable && part_rel(pt, key) == 0;
 }
 
@@ -88,7 +99,7 @@ __device__ __forceinline__ uint64_t comp_word(const OrderParams& P, const CompDe
       // a nullable part orders by its relative value: NULL (0) first
       // ascending, last descending (compareBySpec, v2_lowcard_group.zig:2219-2224)
       const PartDesc& pt = P.parts[c.idx];
-      const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+      const GKey k = group_key(rec, gid, P.g_mode, P.g_key_off);
       w = pt.nullable ? part_rel(pt, k) : ord_i64(part_value(pt, k));
       break;
     }
@@ -131,7 +142,7 @@ __device__ __forceinline__ uint64_t comp_word(const OrderParams& P, const CompDe
     }
     default: {  // C_PKEY: the part's field of the silo's packed key, below its validity bit
       const PartDesc& pt = P.parts[c.idx];
-      const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+      const GKey k = group_key(rec, gid, P.g_mode, P.g_key_off);
       const uint64_t mask = c.aux >= 64 ? ~0ull : ((1ull << c.aux) - 1);
       w = part_null(pt, k) ? 0ull : ((uint64_t)part_value(pt, k) & mask);
       break;
@@ -162,7 +173,7 @@ __device__ __forceinline__ HNum having_value(const FinalParams& P, const OutDesc
   const uint64_t count = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off);
   switch (o.kind) {
     case O_KEY: {
-      const uint64_t k = group_key(rec, gid, P.g_mode, P.g_key_off);
+      const GKey k = group_key(rec, gid, P.g_mode, P.g_key_off);
       if (part_null(P.parts[o.idx], k)) v.null = true;  // keyPartIsNull (:1475-1479)
       else v.i = part_value(P.parts[o.idx], k);
       return v;

@@ -26,12 +26,13 @@
 
 namespace tdbg {
 
+template <bool HASHED>
 __global__ void __launch_bounds__(kRxThreads) rx_hist_kernel(const ScanParams P, const RadixParams R,
                                                              unsigned long long* hist) {
   __shared__ uint32_t h[1u << kRxMaxBits1];
   uint32_t t0, t1;
   rx_tile_range(P, t0, t1);
-  rx_hist_range(P, R, t0, t1, h, hist);
+  rx_hist_range<HASHED>(P, R, t0, t1, h, hist);
 }
 
 __global__ void __launch_bounds__(256) rx_offsets_kernel(const RadixParams R, const unsigned long long* hist) {
@@ -39,11 +40,12 @@ __global__ void __launch_bounds__(256) rx_offsets_kernel(const RadixParams R, co
   rx_offsets(R, hist, tmp);
 }
 
+template <bool HASHED>
 __global__ void __launch_bounds__(kRxThreads) rx_scatter1_kernel(const ScanParams P, const RadixParams R) {
   extern __shared__ __align__(16) uint8_t lds[];
   uint32_t t0, t1;
   rx_tile_range(P, t0, t1);
-  rx_scatter_range(P, R, t0, t1, lds);
+  rx_scatter_range<HASHED>(P, R, t0, t1, lds);
 }
 
 // Level 2 runs on chunks2 workgroups per level-1 bucket; chunk c of bucket b
@@ -230,7 +232,7 @@ __global__ void __launch_bounds__(kRxThreads) rx_scatter2_kernel(const RadixPara
   }
 }
 
-constexpr uint32_t kRxAggTmp = 512;  // LDS words for compaction counts: 9 + 2^leaf_bits / 64
+constexpr uint32_t kRxAggTmp = 512;  // LDS words for compaction counts: 9 + table slots / 64
 
 // HAVING: compacted leaves write only the groups passing R.having (a
 // separate instantiation, so the plain leaf pass keeps its registers).
@@ -248,6 +250,25 @@ __global__ void __launch_bounds__(512) rx_agg_kernel(const ScanParams P, const R
     else if (R.dense_out) rx_agg_leaf<true>(P, R, leaf, lds, tmp);
     else rx_agg_leaf<false>(P, R, leaf, lds, tmp, chist);
   }
+  if (R.counts_out) {
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x)
+      if (chist[b]) atomicAdd(&R.count_hist[b], chist[b]);
+  }
+}
+
+// Hash partitions: the same persistent leaf loop over LDS hash tables.
+template <bool HAVING>
+__global__ void __launch_bounds__(512) rx_agg_hashed_kernel(const ScanParams P, const RadixParams R) {
+  extern __shared__ __align__(16) uint8_t lds[];
+  __shared__ uint32_t tmp[kRxAggTmp];
+  __shared__ uint32_t chist[kRxCountBins];
+  if (R.counts_out) {  // uniform
+    for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x) chist[b] = 0;
+    __syncthreads();
+  }
+  for (uint32_t leaf = blockIdx.x; leaf < R.n_leaves; leaf += gridDim.x)
+    rx_agg_leaf_hashed<HAVING>(P, R, leaf, lds, tmp, chist);
   if (R.counts_out) {
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x)
@@ -369,8 +390,17 @@ hipError_t launch_radix_gids(const RadixParams& r, uint32_t* gids, uint64_t* n_o
 hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid, unsigned long long* hist,
                         hipStream_t s, int phase) {
   static const hipError_t attr = [] {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_scatter1_kernel),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_scatter1_kernel<false>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_scatter1_kernel<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_agg_hashed_kernel<false>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 4096);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_agg_hashed_kernel<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 4096);
     if (e == hipSuccess)
       e = hipFuncSetAttribute(reinterpret_cast<const void*>(rx_scatter2_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -383,20 +413,32 @@ hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid
     return e;
   }();
   if (attr != hipSuccess) return attr;
+  // hash partitions: digits from a 32-bit hash, any table size (compacted,
+  // keyed records); key-range partitions: a leaf's 2^leaf_bits keys fill the table
+  const bool table_ok = r.hashed ? (r.bits1 + r.bits2 + r.leaf_bits == 32 && !r.dense_out && p.lds_cap >= 64 &&
+                                    p.g_mode != G_DENSE && (r.row.key_words == 1 || r.row.key_words == 2) &&
+                                    (r.row.key_off & 7u) == 0 && r.row.key_off + 8 * r.row.key_words <= r.row.stride &&
+                                    (p.g_mode == G_WIDE) == (r.row.key_words == 2))
+                                 : p.lds_cap == (1u << r.leaf_bits);
   if (r.bits1 > kRxMaxBits1 || r.bits2 > 10 || (r.row.stride & 7u) || r.tile_rows == 0 || r.tile_rows > kRxTile ||
-      (r.tile_rows & (kVec - 1)) || rx_scatter_lds_bytes(r) > 160 * 1024 || grid == 0 ||
-      p.lds_cap != (1u << r.leaf_bits) || r.n_leaves != (1u << (r.bits1 + r.bits2)) ||
-      (1u << r.leaf_bits) / 64 + 512 / 64 + 1 > kRxAggTmp || (!r.dense_out && !r.leaf_groups) ||
+      (r.tile_rows & (kVec - 1)) || rx_scatter_lds_bytes(r) > 160 * 1024 || grid == 0 || !table_ok ||
+      r.n_leaves != (1u << (r.bits1 + r.bits2)) || 512 / 64 + 1 + (p.lds_cap + 511) / 512 * (512 / 64) > kRxAggTmp ||
+      (!r.dense_out && !r.leaf_groups) ||
       (r.bits2 && (r.chunks2 == 0 || r.chunks2 > 64 || !r.hist2 || !r.leaf_start || r.tile_rows2 == 0 ||
                    r.tile_rows2 > kRxThreads * kRx2Per || rx_scatter2_lds_bytes(r) > 160 * 1024)) ||
       p.lds_bytes > 160 * 1024 - 4096)
     return hipErrorInvalidConfiguration;
   switch (phase) {
-    case RX_HIST: hipLaunchKernelGGL(rx_hist_kernel, dim3(grid), dim3(kRxThreads), 0, s, p, r, hist); break;
+    case RX_HIST:
+      if (r.hashed) hipLaunchKernelGGL(rx_hist_kernel<true>, dim3(grid), dim3(kRxThreads), 0, s, p, r, hist);
+      else hipLaunchKernelGGL(rx_hist_kernel<false>, dim3(grid), dim3(kRxThreads), 0, s, p, r, hist);
+      break;
     case RX_OFFSETS: hipLaunchKernelGGL(rx_offsets_kernel, dim3(1), dim3(256), 0, s, r, hist); break;
     case RX_SCATTER1:
-      hipLaunchKernelGGL(rx_scatter1_kernel, dim3(grid), dim3(kRxThreads), rx_scatter_lds_bytes(r), s, p,
-                         r);
+      if (r.hashed)
+        hipLaunchKernelGGL(rx_scatter1_kernel<true>, dim3(grid), dim3(kRxThreads), rx_scatter_lds_bytes(r), s, p, r);
+      else
+        hipLaunchKernelGGL(rx_scatter1_kernel<false>, dim3(grid), dim3(kRxThreads), rx_scatter_lds_bytes(r), s, p, r);
       break;
     case RX_SCATTER2: {
       const dim3 g((1u << r.bits1) * r.chunks2);
@@ -407,7 +449,11 @@ hipError_t launch_radix(const ScanParams& p, const RadixParams& r, uint32_t grid
     }
     case RX_AGG:
       if (r.counts_out && (r.dense_out || !r.count_hist)) return hipErrorInvalidValue;
-      if (r.having && !r.dense_out)
+      if (r.hashed && r.having)
+        hipLaunchKernelGGL(rx_agg_hashed_kernel<true>, dim3(grid), dim3(512), p.lds_bytes, s, p, r);
+      else if (r.hashed)
+        hipLaunchKernelGGL(rx_agg_hashed_kernel<false>, dim3(grid), dim3(512), p.lds_bytes, s, p, r);
+      else if (r.having && !r.dense_out)
         hipLaunchKernelGGL(rx_agg_kernel<true>, dim3(grid), dim3(512), p.lds_bytes, s, p, r);
       else
         hipLaunchKernelGGL(rx_agg_kernel<false>, dim3(grid), dim3(512), p.lds_bytes, s, p, r);

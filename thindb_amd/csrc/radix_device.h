@@ -14,7 +14,7 @@ constexpr uint32_t kRxTile = kRxThreads * kVec;  // most rows staged per tile (R
 
 // Filter and key of one strip (kinds read at run time).
 __device__ __forceinline__ uint32_t rx_strip_keys(const ScanParams& P, uint64_t r, uint32_t cnt, uint64_t key[kVec],
-                                                  bool with_preds) {
+                                                  bool with_preds, bool with_parts = true) {
   uint32_t mask = (1u << cnt) - 1u;
   if (with_preds) {
     for (uint32_t i = 0; i < P.n_preds; i++) {
@@ -24,6 +24,7 @@ __device__ __forceinline__ uint32_t rx_strip_keys(const ScanParams& P, uint64_t 
     }
     if (P.n_filt) mask &= filter_mask(P, r, [&](uint32_t col) { return load_raw(P.cols[col], r); });
   }
+  if (!with_parts) return mask;
   for (uint32_t i = 0; i < P.n_parts; i++) {
     const PartDesc& pt = P.parts[i];
     int64_t v[kVec];
@@ -32,6 +33,64 @@ __device__ __forceinline__ uint32_t rx_strip_keys(const ScanParams& P, uint64_t 
     if (i == 0) key_part<true>(P, pt, v, key, mask, pv);
     else key_part<false>(P, pt, v, key, mask, pv);
   }
+  return mask;
+}
+
+// The packed 128-bit key (G_WIDE) of a strip's live rows: each part's
+// relative value (NULL = 0) as its bit field, the first part lowest.  A
+// value outside the part's zone-map range raises ERR_KEY_RANGE and drops the
+// row, as key_part does.
+__device__ __forceinline__ uint32_t rx_strip_wide(const ScanParams& P, uint64_t r, uint32_t mask, uint64_t lo[kVec],
+                                                  uint64_t hi[kVec]) {
+#pragma unroll
+  for (int j = 0; j < kVec; j++) lo[j] = hi[j] = 0;
+  for (uint32_t i = 0; i < P.n_parts; i++) {
+    const PartDesc& pt = P.parts[i];
+    int64_t v[kVec];
+    decode_i(load_raw(P.cols[pt.col], r), P.cols[pt.col].kind, v);
+    const uint32_t pv = pt.nullable ? valid4(P.cols[pt.col], r) : 0xfu;
+    const uint32_t o = pt.bit_off, w = pt.bit_width;
+#pragma unroll
+    for (int j = 0; j < kVec; j++) {
+      const uint64_t rel = ((pv >> j) & 1u) ? (uint64_t)v[j] - (uint64_t)pt.base : 0ull;
+      if (pt.span && rel >= pt.span && ((mask >> j) & 1u)) {  // span 0: all 2^64 values
+        atomicOr(P.err, ERR_KEY_RANGE);
+        mask &= ~(1u << j);
+      }
+      if (w == 0) continue;
+      if (o >= 64) {
+        hi[j] |= rel << (o - 64);
+      } else {
+        lo[j] |= rel << o;
+        if (o && o + w > 64) hi[j] |= rel >> (64 - o);
+      }
+    }
+  }
+  return mask;
+}
+
+// 64-bit hash of a group key (lo, hi): hash partitions take their digits
+// from its top 32 bits, leaf tables their first slot and tag from the rest.
+__device__ __forceinline__ uint64_t rx_key_hash(uint64_t lo, uint64_t hi) {
+  return mix64(lo ^ mix64(hi + 0x9e3779b97f4a7c15ull));
+}
+
+// Filter, group key (lo, and hi for a packed key) and the value the digits
+// are taken from: the relative key itself, or with HASHED the top 32 bits
+// of the key's hash.
+template <bool HASHED>
+__device__ __forceinline__ uint32_t rx_strip_route(const ScanParams& P, uint64_t r, uint32_t cnt, bool with_preds,
+                                                   uint64_t key[kVec], uint64_t khi[kVec], uint64_t route[kVec]) {
+  uint32_t mask;
+  if (HASHED && P.g_mode == G_WIDE) {
+    mask = rx_strip_wide(P, r, rx_strip_keys(P, r, cnt, key, with_preds, false), key, khi);
+  } else {
+    mask = rx_strip_keys(P, r, cnt, key, with_preds);
+#pragma unroll
+    for (int j = 0; j < kVec; j++) khi[j] = 0;
+  }
+#pragma unroll
+  for (int j = 0; j < kVec; j++) route[j] = HASHED ? rx_key_hash(key[j], khi[j]) >> 32 : key[j];
   return mask;
 }
 
@@ -277,10 +336,11 @@ __device__ __forceinline__ void rx_fold_row(const ScanParams& P, const RadixPara
 
 // Writes slot s of the LDS table as a group record (key word, count, each
 // aggregate's HBM state) at `rec`.
-__device__ __forceinline__ void rx_write_record(const ScanParams& P, uint8_t* lds, uint32_t s, uint64_t key,
+__device__ __forceinline__ void rx_write_record(const ScanParams& P, uint8_t* lds, uint32_t s, GKey key,
                                                 uint8_t* rec) {
   const uint32_t cap = P.lds_cap;
-  if (P.g_mode == G_HASH) *reinterpret_cast<uint64_t*>(rec + P.g_key_off) = key;  // dense: key = index
+  if (P.g_mode != G_DENSE) *reinterpret_cast<uint64_t*>(rec + P.g_key_off) = key.lo;  // dense: key = index
+  if (P.g_mode == G_WIDE) *reinterpret_cast<uint64_t*>(rec + P.g_key_off + 8) = key.hi;
   *reinterpret_cast<uint64_t*>(rec + P.g_count_off) = lds_at<uint32_t>(lds, P.lds_count_off)[s];
   for (uint32_t a = 0; a < P.n_aggs; a++) {
     const AggDesc& d = P.aggs[a];
@@ -310,6 +370,7 @@ __device__ __forceinline__ void rx_write_record(const ScanParams& P, uint8_t* ld
 // Level-1 digit histogram of tiles [t0, t1) (every row, before filtering:
 // the counts bound each bucket's size) through `h` (2^bits1 LDS counters)
 // into `hist`.
+template <bool HASHED = false>
 __device__ __forceinline__ void rx_hist_range(const ScanParams& P, const RadixParams& R, uint32_t t0, uint32_t t1,
                                               uint32_t* h, unsigned long long* hist) {
   const uint32_t nb = 1u << R.bits1;
@@ -319,11 +380,11 @@ __device__ __forceinline__ void rx_hist_range(const ScanParams& P, const RadixPa
     const Tile T = P.tiles[t];
     for (uint32_t s = threadIdx.x * kVec; s < T.n; s += blockDim.x * kVec) {
       const uint32_t cnt = T.n - s < (uint32_t)kVec ? T.n - s : (uint32_t)kVec;
-      uint64_t key[kVec];
-      const uint32_t mask = rx_strip_keys(P, T.row0 + s, cnt, key, false);
+      uint64_t key[kVec], khi[kVec], route[kVec];
+      const uint32_t mask = rx_strip_route<HASHED>(P, T.row0 + s, cnt, false, key, khi, route);
 #pragma unroll
       for (int j = 0; j < kVec; j++)
-        if ((mask >> j) & 1u) atomicAdd(&h[(uint32_t)(key[j] >> R.shift1)], 1u);
+        if ((mask >> j) & 1u) atomicAdd(&h[(uint32_t)(route[j] >> R.shift1)], 1u);
     }
   }
   __syncthreads();
@@ -340,6 +401,7 @@ __device__ __forceinline__ void rx_hist_range(const ScanParams& P, const RadixPa
 // With `item_off` (the item's row offset per digit, from rx_item_offsets)
 // the runs go to consecutive places from there, tracked in LDS, and no
 // device atomic is needed.
+template <bool HASHED = false>
 __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const RadixParams& R, uint32_t t0, uint32_t t1,
                                                  uint8_t* lds, const uint32_t* item_off = nullptr) {
   uint32_t* cnt = reinterpret_cast<uint32_t*>(lds + kRxCntOff);
@@ -359,14 +421,14 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
       const uint32_t s = s0 + threadIdx.x * kVec;
       const uint64_t r = T.row0 + s;
       uint32_t mask = 0, digit[kVec], rank[kVec];
-      uint64_t key[kVec];
+      uint64_t key[kVec], khi[kVec], route[kVec];
       if (threadIdx.x * kVec < R.tile_rows && s < T.n) {
         const uint32_t c = T.n - s < (uint32_t)kVec ? T.n - s : (uint32_t)kVec;
-        mask = rx_strip_keys(P, r, c, key, true);
+        mask = rx_strip_route<HASHED>(P, r, c, true, key, khi, route);
 #pragma unroll
         for (int j = 0; j < kVec; j++)
           if ((mask >> j) & 1u) {
-            digit[j] = (uint32_t)(key[j] >> R.shift1);
+            digit[j] = (uint32_t)(route[j] >> R.shift1);
             rank[j] = atomicAdd(&cnt[digit[j]], 1u);
           }
       }
@@ -390,7 +452,14 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
 #pragma unroll
         for (int j = 0; j < kVec; j++) {
           row[j] = stage + (size_t)(((mask >> j) & 1u) ? off[digit[j]] + rank[j] : 0) * stride;
-          if ((mask >> j) & 1u) *reinterpret_cast<uint32_t*>(row[j]) = (uint32_t)key[j];
+          if ((mask >> j) & 1u) {
+            *reinterpret_cast<uint32_t*>(row[j]) = (uint32_t)route[j];
+            if (HASHED) {  // the key itself after the hash
+              uint64_t* kp = reinterpret_cast<uint64_t*>(row[j] + R.row.key_off);
+              kp[0] = key[j];
+              if (R.row.key_words == 2) kp[1] = khi[j];
+            }
+          }
         }
         for (uint32_t a = 0; a < P.n_aggs; a++) {
           const ColDesc& col = P.cols[P.aggs[a].col];
@@ -425,7 +494,7 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
 // reads it (src/exec/v2_shape_group_topn.zig:2461-2500): the same values
 // rx_write_record would store and having_value would read back.
 __device__ __forceinline__ HNum rx_having_value(const ScanParams& P, const HavingLeaf& L, uint32_t c, uint8_t* lds,
-                                                uint32_t s, uint64_t key) {
+                                                uint32_t s, GKey key) {
   const HavingSrc& h = L.src[c];
   HNum v{0, 0.0, false, false};
   const uint32_t count = lds_at<uint32_t>(lds, P.lds_count_off)[s];
@@ -473,7 +542,7 @@ __device__ __forceinline__ HNum rx_having_value(const ScanParams& P, const Havin
 
 // havingPasses (:2502-2535) over LDS slot s.
 __device__ __forceinline__ bool rx_having_pass(const ScanParams& P, const HavingLeaf& L, uint8_t* lds, uint32_t s,
-                                               uint64_t key) {
+                                               GKey key) {
   uint64_t st = 0;
   for (uint32_t k = 0; k < L.n; k++) {
     const HavingNode& h = L.nodes[k];
@@ -518,74 +587,31 @@ __device__ __forceinline__ bool rx_having_pass(const ScanParams& P, const Having
   return (st & 1ull) != 0;
 }
 
-// Aggregates leaf `leaf` in an LDS direct table and writes its groups: with
-// DENSE_OUT every key of the leaf's range below g_cap at record index key
-// (empty keys get a zero count), else the occupied slots in slot order to
-// consecutive records from the leaf's first row index (a leaf holds at least
-// as many rows as groups, so leaves never overlap), and the number of
-// groups to R.leaf_groups[leaf].  tmp: blockDim.x / 64 + 1 + cap / 64 LDS
-// words (unused with DENSE_OUT).
+// The occupied slots of a leaf's LDS table (count != 0; with HAVING those
+// passing it) as group records from row index s on, in slot order, and
+// their number to R.leaf_groups[leaf]: slot k * blockDim.x + t is handled by
+// thread t in round k, so each record store is coalesced; per (round,
+// wavefront) counts are scanned in that order.  key_of(slot) gives a slot's
+// group key.  tmp: blockDim.x / 64 + 1 + cap / 64 LDS words.
 //
-// With R.counts_out (compacted only) each written group's row count also
-// goes to R.counts_out[record] and into chist (kRxCountBins LDS counters of
-// min(count, kRxCountBins - 1)).  With R.having (compacted only) a group is
-// written, counted and histogrammed only when it passes that HAVING.
-template <bool DENSE_OUT, bool HAVING = false>
-__device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint8_t* lds,
-                                            uint32_t* tmp, uint32_t* chist = nullptr) {
-  const uint32_t cap = P.lds_cap, stride = R.row.stride;
-  const uint64_t mask = (uint64_t)cap - 1u;
-  uint64_t s, e;
-  if (R.bits2) {
-    s = R.leaf_start[2 * (uint64_t)leaf];
-    e = R.leaf_start[2 * (uint64_t)leaf + 1];
-  } else {
-    s = R.bucket_start[leaf];
-    e = R.bucket_fill[leaf];
-  }
-  const uint64_t leaf_key0 = (uint64_t)leaf << R.leaf_bits;
-  if (s == e) {  // uniform
-    if (DENSE_OUT) {
-      for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x)
-        *reinterpret_cast<uint64_t*>(P.g_state + (leaf_key0 + slot) * P.g_rec + P.g_count_off) = 0;
-    } else if (threadIdx.x == 0) {
-      R.leaf_groups[leaf] = 0;
-    }
-    return;
-  }
-  lds_init(P, lds);
-  __syncthreads();
-  for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
-    const uint8_t* row = R.rows2 + i * stride;
-    const uint32_t key = *reinterpret_cast<const uint32_t*>(row);
-    rx_fold_row(P, R, lds, row, (uint32_t)(key & mask));
-  }
-  __syncthreads();
+// With R.counts_out each written group's row count also goes to
+// R.counts_out[record] and into chist (kRxCountBins LDS counters of
+// min(count, kRxCountBins - 1)).
+template <bool HAVING, typename KeyOf>
+__device__ __forceinline__ void rx_emit_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint64_t s,
+                                             uint8_t* lds, uint32_t* tmp, uint32_t* chist, KeyOf key_of) {
+  const uint32_t cap = P.lds_cap;
   const uint32_t* count = lds_at<uint32_t>(lds, P.lds_count_off);
-  if (DENSE_OUT) {
-    for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x) {
-      uint8_t* rec = P.g_state + (leaf_key0 + slot) * P.g_rec;
-      if (count[slot]) rx_write_record(P, lds, slot, leaf_key0 + slot, rec);
-      else *reinterpret_cast<uint64_t*>(rec + P.g_count_off) = 0;
-    }
-    __syncthreads();
-    return;
-  }
-  // occupied slots in slot order: slot k * blockDim.x + t is handled by
-  // thread t in round k, so each record store is coalesced; per (round,
-  // wavefront) counts are scanned in that order
   const uint32_t nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
   const uint32_t rounds = (cap + blockDim.x - 1) / blockDim.x;
   uint32_t* scan_tmp = tmp;
   uint32_t* wcnt = tmp + nw + 1;  // rounds * nw entries
-  // with HAVING, a group is written only when it passes (evaluated on the
-  // LDS table, in both rounds below)
   const HavingLeaf* H = HAVING ? R.having : nullptr;
   uint64_t keep = 0;  // each round's outcome (rounds beyond 64 evaluate again below)
   for (uint32_t k = 0; k < rounds; k++) {
     const uint32_t slot = k * blockDim.x + threadIdx.x;
     bool occ = slot < cap && count[slot] != 0;
-    if (H && occ) occ = rx_having_pass(P, *H, lds, slot, leaf_key0 + slot);
+    if (H && occ) occ = rx_having_pass(P, *H, lds, slot, key_of(slot));
     if (k < 64 && occ) keep |= 1ull << k;
     const uint64_t m = __ballot(occ);
     if ((threadIdx.x & 63) == 0) wcnt[k * nw + wave] = (uint32_t)__popcll(m);
@@ -609,11 +635,11 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
       occ = (keep >> k) & 1ull;
     } else {
       occ = slot < cap && count[slot] != 0;
-      if (H && occ) occ = rx_having_pass(P, *H, lds, slot, leaf_key0 + slot);
+      if (H && occ) occ = rx_having_pass(P, *H, lds, slot, key_of(slot));
     }
     const uint64_t m = __ballot(occ);
     const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
-    if (occ) rx_write_record(P, lds, slot, leaf_key0 + slot, P.g_state + at * P.g_rec);
+    if (occ) rx_write_record(P, lds, slot, key_of(slot), P.g_state + at * P.g_rec);
     if (R.counts_out) {  // uniform
       const uint32_t c = occ ? count[slot] : 0u;
       if (occ) R.counts_out[at] = c;
@@ -628,6 +654,152 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
   }
   if (threadIdx.x == 0) R.leaf_groups[leaf] = total;
   __syncthreads();
+}
+
+// A leaf's row range in the final partitioned array.
+__device__ __forceinline__ void rx_leaf_range(const RadixParams& R, uint32_t leaf, uint64_t& s, uint64_t& e) {
+  if (R.bits2) {
+    s = R.leaf_start[2 * (uint64_t)leaf];
+    e = R.leaf_start[2 * (uint64_t)leaf + 1];
+  } else {
+    s = R.bucket_start[leaf];
+    e = R.bucket_fill[leaf];
+  }
+}
+
+// Aggregates leaf `leaf` in an LDS direct table and writes its groups: with
+// DENSE_OUT every key of the leaf's range below g_cap at record index key
+// (empty keys get a zero count), else compacted (rx_emit_leaf; a leaf holds
+// at least as many rows as groups, so leaves never overlap).  With HAVING
+// (compacted only) a group is written, counted and histogrammed only when
+// it passes R.having.  tmp: blockDim.x / 64 + 1 + cap / 64 LDS words
+// (unused with DENSE_OUT).
+template <bool DENSE_OUT, bool HAVING = false>
+__device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint8_t* lds,
+                                            uint32_t* tmp, uint32_t* chist = nullptr) {
+  const uint32_t cap = P.lds_cap, stride = R.row.stride;
+  const uint64_t mask = (uint64_t)cap - 1u;
+  uint64_t s, e;
+  rx_leaf_range(R, leaf, s, e);
+  const uint64_t leaf_key0 = (uint64_t)leaf << R.leaf_bits;
+  if (s == e) {  // uniform
+    if (DENSE_OUT) {
+      for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x)
+        *reinterpret_cast<uint64_t*>(P.g_state + (leaf_key0 + slot) * P.g_rec + P.g_count_off) = 0;
+    } else if (threadIdx.x == 0) {
+      R.leaf_groups[leaf] = 0;
+    }
+    return;
+  }
+  lds_init(P, lds);
+  __syncthreads();
+  for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+    const uint8_t* row = R.rows2 + i * stride;
+    const uint32_t key = *reinterpret_cast<const uint32_t*>(row);
+    rx_fold_row(P, R, lds, row, (uint32_t)(key & mask));
+  }
+  __syncthreads();
+  if (DENSE_OUT) {
+    const uint32_t* count = lds_at<uint32_t>(lds, P.lds_count_off);
+    for (uint32_t slot = threadIdx.x; slot < cap && leaf_key0 + slot < P.g_cap; slot += blockDim.x) {
+      uint8_t* rec = P.g_state + (leaf_key0 + slot) * P.g_rec;
+      if (count[slot]) rx_write_record(P, lds, slot, GKey{leaf_key0 + slot, 0}, rec);
+      else *reinterpret_cast<uint64_t*>(rec + P.g_count_off) = 0;
+    }
+    __syncthreads();
+    return;
+  }
+  rx_emit_leaf<HAVING>(P, R, leaf, s, lds, tmp, chist, [&](uint32_t slot) { return GKey{leaf_key0 + slot, 0}; });
+}
+
+// Hash partitions: leaf `leaf`'s rows fold into an LDS hash table (u32 tags,
+// 0 = free; the key words of each claimed slot), then its groups go out as
+// rx_emit_leaf writes them.  Rows are taken blockDim.x at a time, and each
+// batch settles in rounds between barriers, so no thread ever waits on
+// another: a row either claims a free slot by CAS on its tag (and writes the
+// key), or stops at a slot holding its tag, whose key is compared after the
+// barrier (the claimer's key is then in place); a mismatch probes on from
+// the next slot in the next round.  A table with no free slot left for a row
+// raises ERR_HASH_FULL (the host then partitions into more leaves).
+template <bool HAVING>
+__device__ __forceinline__ void rx_agg_leaf_hashed(const ScanParams& P, const RadixParams& R, uint32_t leaf,
+                                                   uint8_t* lds, uint32_t* tmp, uint32_t* chist) {
+  const uint32_t cap = P.lds_cap, stride = R.row.stride, kw = R.row.key_words;
+  uint64_t s, e;
+  rx_leaf_range(R, leaf, s, e);
+  if (s == e) {  // uniform
+    if (threadIdx.x == 0) R.leaf_groups[leaf] = 0;
+    return;
+  }
+  uint32_t* tag = lds_at<uint32_t>(lds, R.tag_off);
+  uint64_t* keys = lds_at<uint64_t>(lds, P.lds_keys_off);
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) tag[i] = 0;
+  lds_init(P, lds);
+  __syncthreads();
+  for (uint64_t b = s; b < e; b += blockDim.x) {  // uniform
+    const uint64_t i = b + threadIdx.x;
+    bool pending = i < e;
+    const uint8_t* row = R.rows2 + (pending ? i : s) * stride;
+    uint64_t klo = 0, khi = 0;
+    uint32_t slot = 0, t = 0, probes = 0;
+    if (pending) {
+      const uint64_t* kp = reinterpret_cast<const uint64_t*>(row + R.row.key_off);
+      klo = kp[0];
+      khi = kw == 2 ? kp[1] : 0ull;
+      const uint64_t h = rx_key_hash(klo, khi);
+      slot = __umulhi((uint32_t)h, cap);
+      t = (uint32_t)(h >> 16) | 1u;
+    }
+    while (__syncthreads_or(pending)) {
+      int cand = -1;
+      if (pending) {
+        for (;;) {
+          uint32_t cur = __hip_atomic_load(tag + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (cur == 0) {
+            uint32_t expect = 0;
+            if (__hip_atomic_compare_exchange_strong(tag + slot, &expect, t, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) {
+              keys[(uint64_t)slot * kw] = klo;
+              if (kw == 2) keys[(uint64_t)slot * kw + 1] = khi;
+              rx_fold_row(P, R, lds, row, slot);
+              pending = false;
+              break;
+            }
+            cur = expect;
+          }
+          if (cur == t) {  // same tag: compare keys after the barrier
+            cand = (int)slot;
+            break;
+          }
+          slot = slot + 1 == cap ? 0 : slot + 1;
+          if (++probes >= cap) {
+            atomicOr(P.err, ERR_HASH_FULL);
+            pending = false;
+            break;
+          }
+        }
+      }
+      __syncthreads();
+      if (cand >= 0) {
+        const uint64_t* kk = keys + (uint64_t)cand * kw;
+        if (kk[0] == klo && (kw == 1 || kk[1] == khi)) {
+          rx_fold_row(P, R, lds, row, (uint32_t)cand);
+          pending = false;
+        } else {
+          slot = (uint32_t)cand + 1 == cap ? 0 : (uint32_t)cand + 1;
+          if (++probes >= cap) {
+            atomicOr(P.err, ERR_HASH_FULL);
+            pending = false;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  rx_emit_leaf<HAVING>(P, R, leaf, s, lds, tmp, chist, [&](uint32_t slot) {
+    const uint64_t* kk = keys + (uint64_t)slot * kw;
+    return GKey{kk[0], kw == 2 ? kk[1] : 0ull};
+  });
 }
 
 }  // namespace tdbg

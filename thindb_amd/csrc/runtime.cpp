@@ -560,8 +560,12 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
       }
     }
   }
-  // key parts: relative values in a mixed-radix index
+  // key parts: relative values in a mixed-radix index, or bit fields of a
+  // packed key of up to 128 bits when the domain reaches 2^64 (the silo packs
+  // keys into at most 128 bits, v2_shape_group_topn.zig:2051-2094)
   unsigned __int128 domain = 1;
+  const unsigned __int128 kDomainCap = (unsigned __int128)1 << 100;  // saturates: only compared with 2^64
+  uint32_t key_bits = 0, part_bits[kMaxParts] = {};
   for (size_t i = 0; i < pl->keys.size(); i++) {
     auto it = t->cols.find(pl->keys[i].column_idx);
     if (it == t->cols.end()) return fail(TDBG_ERR_NOT_STAGED, "key column %u not staged", pl->keys[i].column_idx);
@@ -570,22 +574,39 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     PartDesc d{};
     d.col = col_slot(pl->keys[i].column_idx);
     d.base = a.has_stats ? a.min_i : 0;
-    d.span = a.has_stats ? (uint64_t)a.max_i - (uint64_t)a.min_i + 1 : 1;
-    if (d.span == 0) return fail(TDBG_ERR_UNSUPPORTED, "key domain of 2^64 values");
+    unsigned __int128 span = a.has_stats ? (unsigned __int128)((uint64_t)a.max_i - (uint64_t)a.min_i) + 1 : 1;
     if (a.d_valid) {
       // NULL takes relative value 0, the values 1.. (packValidity: the NULL
       // group stays apart from every value, v2_lowcard_group.zig:44-49)
       d.nullable = 1;
       d.base = (int64_t)((uint64_t)d.base - 1);
-      d.span = a.has_stats ? d.span + 1 : 1;
-      if (d.span == 0) return fail(TDBG_ERR_UNSUPPORTED, "key domain of 2^64 values");
+      span = a.has_stats ? span + 1 : 1;
     }
-    d.stride = (uint64_t)domain;
-    domain *= d.span;
-    if (domain >= ((unsigned __int128)1 << 62)) return fail(TDBG_ERR_UNSUPPORTED, "key domain wider than 62 bits");
+    if (span > ((unsigned __int128)1 << 64))
+      return fail(TDBG_ERR_UNSUPPORTED, "nullable key part over all 2^64 values (65-bit field)");
+    d.span = (uint64_t)span;  // 0: all 2^64 values
+    uint32_t w = 0;
+    while (w < 64 && ((unsigned __int128)1 << w) < span) w++;
+    part_bits[i] = w;
+    key_bits += w;
+    d.stride = domain < ((unsigned __int128)1 << 64) ? (uint64_t)domain : 0;
+    domain = domain > kDomainCap / span ? kDomainCap : domain * span;
     sp.parts[sp.n_parts++] = d;
   }
-  const uint64_t dom = (uint64_t)domain;
+  if (key_bits > 128) return fail(TDBG_ERR_UNSUPPORTED, "group key packs into %u bits (more than 128)", key_bits);
+  // a domain of 2^62 or more has only the hash-partitioned path; of 2^64 or
+  // more it needs the packed 128-bit key (G_WIDE)
+  const bool big_domain = domain >= ((unsigned __int128)1 << 62);
+  const bool packed_key = domain >= ((unsigned __int128)1 << 64);
+  if (packed_key) {
+    for (uint32_t i = 0, off = 0; i < sp.n_parts; i++) {
+      sp.parts[i].bit_off = off;
+      sp.parts[i].bit_width = part_bits[i];
+      sp.parts[i].stride = 1;
+      off += part_bits[i];
+    }
+  }
+  const uint64_t dom = domain > (unsigned __int128)~0ull ? ~0ull : (uint64_t)domain;
   // aggregates (kernel states shared by aggs of the same class and column)
   struct KAgg { uint32_t kind; uint32_t col; bool nullable; };
   std::vector<KAgg> kaggs;
@@ -742,6 +763,38 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
   // key's high bits until a leaf's key range fits an LDS direct table.
   // THINDB_GPU_RADIX=0 never takes this path, =1 takes it whenever the
   // domain allows (tests use it to cover the partition passes on small tables).
+  // The partitioned row (RowLayout) and the scatter tiles: the u32 relative
+  // key or hash, the validity word, the key words of hash partitions, then
+  // the aggregate inputs at native width (aggregates over one column share it).
+  auto lay_out_rows = [&](RadixParams& R, uint32_t key_words) {
+    R.row.flags_off = ~0u;
+    uint32_t ro = 4;
+    for (uint32_t k = 0; k < sp.n_aggs; k++)
+      if (sp.cols[sp.aggs[k].col].valid) R.row.flags_off = 0;
+    if (R.row.flags_off == 0) { R.row.flags_off = ro; ro += 4; }
+    if (key_words) {
+      ro = (uint32_t)align_up(ro, 8);
+      R.row.key_off = ro;
+      R.row.key_words = key_words;
+      ro += 8 * key_words;
+    }
+    for (uint32_t k = 0; k < sp.n_aggs; k++) {
+      const AggDesc& d = sp.aggs[k];
+      const uint32_t b = d.kind == A_COUNT_COL ? 0u : (uint32_t)t->cols[kaggs[k].col].elem;
+      R.row.val_bytes[k] = b;
+      int same = -1;
+      for (uint32_t j = 0; j < k; j++)
+        if (b && sp.aggs[j].col == d.col && R.row.val_bytes[j] == b) same = (int)j;
+      if (same >= 0) { R.row.val_off[k] = R.row.val_off[same]; continue; }
+      ro = (uint32_t)align_up(ro, std::max<uint32_t>(b, 1));
+      R.row.val_off[k] = ro;
+      ro += b;
+    }
+    R.row.stride = (uint32_t)align_up(ro, 8);
+    // staged scatter tile: 24 KiB of digit tables + tile_rows rows in LDS
+    R.tile_rows = std::min<uint32_t>(4096, (budget - 24 * 1024) / R.row.stride / 256 * 256);
+    R.tile_rows2 = std::min<uint32_t>(8192, (budget - 24 * 1024) / R.row.stride / 256 * 256);
+  };
   const char* rx_env = getenv("THINDB_GPU_RADIX");
   const bool forced = strat == S_RADIX || (strat == S_AUTO && rx_env && rx_env[0] == '1');
   const bool rx_off = (strat != S_AUTO && strat != S_RADIX) || (strat == S_AUTO && rx_env && rx_env[0] == '0');
@@ -761,27 +814,7 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
         hint.wide_leaves ? fit_bits : std::min<uint32_t>(fit_bits, std::max<uint32_t>(6, L > 10 ? L - 10 : 0));
     RadixParams& R = P.rx;
     R = RadixParams{};
-    R.row.flags_off = ~0u;
-    uint32_t ro = 4;  // u32 relative key first
-    for (uint32_t k = 0; k < sp.n_aggs; k++)
-      if (sp.cols[sp.aggs[k].col].valid) R.row.flags_off = 0;
-    if (R.row.flags_off == 0) { R.row.flags_off = ro; ro += 4; }
-    for (uint32_t k = 0; k < sp.n_aggs; k++) {
-      const AggDesc& d = sp.aggs[k];
-      const uint32_t b = d.kind == A_COUNT_COL ? 0u : (uint32_t)t->cols[kaggs[k].col].elem;
-      R.row.val_bytes[k] = b;
-      int same = -1;  // aggregates over one column share its value
-      for (uint32_t j = 0; j < k; j++)
-        if (b && sp.aggs[j].col == d.col && R.row.val_bytes[j] == b) same = (int)j;
-      if (same >= 0) { R.row.val_off[k] = R.row.val_off[same]; continue; }
-      ro = (uint32_t)align_up(ro, std::max<uint32_t>(b, 1));
-      R.row.val_off[k] = ro;
-      ro += b;
-    }
-    R.row.stride = (uint32_t)align_up(ro, 8);
-    // staged scatter tile: 24 KiB of digit tables + tile_rows rows in LDS
-    R.tile_rows = std::min<uint32_t>(4096, (budget - 24 * 1024) / R.row.stride / 256 * 256);
-    R.tile_rows2 = std::min<uint32_t>(8192, (budget - 24 * 1024) / R.row.stride / 256 * 256);
+    lay_out_rows(R, 0);
     const uint32_t rem = L - leaf_bits;
     // one level up to 2^10 buckets, else 2^max_bits1 x the rest (the first
     // level widened up to 2^10 when the second would exceed 2^10)
@@ -827,9 +860,71 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
       }
     }
   }
+  // Wide or sparse keys: hash partitions (the silo routes rows by hash bits,
+  // silo_group_core.zig:5226-5235), leaves folding into LDS hash tables.  A
+  // domain of 2^62 or more has no other path; a sparse domain takes it over
+  // per-row HBM atomics once the table is large.  THINDB_GPU_RADIX_HASH=1
+  // takes it whenever the plan allows, =0 never.
+  const char* rh_env = getenv("THINDB_GPU_RADIX_HASH");
+  const bool h_forced = rh_env && rh_env[0] == '1';
+  const bool h_off = rh_env && rh_env[0] == '0';
+  if (!P.radix && !h_off && (strat == S_AUTO || strat == S_RADIX) && P.distinct.empty() &&
+      (big_domain || h_forced || (sp.lds_mode == LDS_NONE && !dense && !rx_off && n_rows_scan >= (1u << 20)))) {
+    RadixParams& R = P.rx;
+    R = RadixParams{};
+    const uint32_t kw = packed_key ? 2u : 1u;
+    lay_out_rows(R, kw);
+    // a slot: tag, key words, then the aggregate states; two leaf
+    // workgroups per CU
+    const uint32_t sb = slot_bytes_for(n_rows_scan) + 4 + 8 * kw;
+    const uint32_t leaf_budget = std::min<uint32_t>(budget, (uint32_t)pl->ctx->max_lds / 2) - 4096;
+    const uint32_t pad = 8 * (2 * sp.n_aggs + 5) + 16;
+    const uint32_t cap_max = std::min<uint32_t>((leaf_budget - pad) / sb / 64 * 64, 8192);
+    // leaves for the estimated groups at most ~45% of a table on average
+    // (the fullest of them then stays well inside it); more after a table
+    // overflowed
+    const uint64_t G = std::max<uint64_t>(1, pl->est_groups ? std::min<uint64_t>(pl->est_groups, dom)
+                                                            : std::min<uint64_t>(n_rows_scan, dom));
+    uint32_t rem = 8;
+    while (rem < 20 && (G >> rem) > (uint64_t)cap_max * 45 / 100) rem++;
+    rem = std::min<uint32_t>(20, rem + pl->rx_hash_boost);
+    const uint64_t per_leaf = (G + (1ull << rem) - 1) >> rem;
+    if (cap_max >= 256 && R.tile_rows >= 256) {
+      P.radix = true;
+      const uint32_t cap = (uint32_t)std::min<uint64_t>(cap_max, std::max<uint64_t>(256, align_up(2 * per_leaf + 128, 64)));
+      const uint32_t max_bits1 = std::min<uint32_t>(hint.max_bits1 ? hint.max_bits1 : 8u, 10u);
+      R.hashed = 1;
+      R.bits1 = rem <= 10 ? rem : std::max<uint32_t>(std::min<uint32_t>(max_bits1, rem), rem - 10);
+      R.bits2 = rem - R.bits1;
+      R.leaf_bits = 32 - rem;  // hash bits below the digits
+      R.shift1 = 32 - R.bits1;
+      R.shift2 = 32 - rem;
+      R.n_leaves = 1u << rem;
+      sp.lds_mode = LDS_HASH;
+      lds_cap = cap;
+      parts = 1;
+      // keyed records, compacted leaf by leaf from each leaf's first row
+      const uint32_t had = sp.g_mode == G_DENSE ? 0u : 8u;  // key bytes the layout holds
+      const uint32_t shift = 8 * kw - had;
+      sp.g_key_off = 0;
+      sp.g_count_off += shift;
+      for (uint32_t k = 0; k < sp.n_aggs; k++) {
+        sp.aggs[k].g_off += shift;
+        if (sp.aggs[k].g_ns_off != ~0u) sp.aggs[k].g_ns_off += shift;
+      }
+      for (auto& dd : P.distinct) dd.dslot_off += shift;
+      sp.g_rec += shift;
+      P.tmpl.insert(P.tmpl.begin(), shift / 8, kEmptyKey);
+      sp.g_mode = packed_key ? G_WIDE : G_HASH;
+      sp.g_cap = std::max<uint64_t>(1, n_rows_scan);
+    }
+  }
+  if (big_domain && !P.radix)
+    return fail(TDBG_ERR_UNSUPPORTED, "a key domain of 2^62 or more values runs only on hash partitions%s",
+                P.distinct.empty() ? "" : " (not with COUNT(DISTINCT))");
   sp.lds_cap = lds_cap;
   sp.lds_parts = parts;
-  sp.lds_key64 = key_bytes == 8;
+  sp.lds_key64 = key_bytes == 8 || P.rx.hashed;
   uint32_t lo = 0;
   auto lds_take = [&](uint32_t bytes) {
     lo = (uint32_t)align_up(lo, 8);
@@ -838,7 +933,8 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     return at;
   };
   if (sp.lds_mode != LDS_NONE) {
-    if (sp.lds_mode == LDS_HASH) sp.lds_keys_off = lds_take(lds_cap * key_bytes);
+    if (sp.lds_mode == LDS_HASH) sp.lds_keys_off = lds_take(lds_cap * (P.rx.hashed ? 8 * P.rx.row.key_words : key_bytes));
+    if (P.rx.hashed) P.rx.tag_off = lds_take(lds_cap * 4);
     for (uint32_t k = 0; k < sp.n_aggs; k++) {
       AggDesc& d = sp.aggs[k];
       if (d.kind == A_COUNT_COL) continue;
@@ -931,13 +1027,14 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
   for (uint32_t i = 0; i < sp.n_parts; i++) {
     const PartDesc& d = sp.parts[i];
     const __int128 lo = (__int128)d.base + (d.nullable ? 1 : 0);
-    const __int128 hi = (__int128)d.base + (__int128)d.span - 1;
+    const __int128 hi = (__int128)d.base + (d.span ? (__int128)d.span : ((__int128)1 << 64)) - 1;
     if (lo <= hi && lo < 0 && hi >= 0) one_sign = false;
   }
   const bool keyed = sp.g_mode != G_DENSE;  // compacted radix leaves and the HBM hash table
   if (!op.n_comps && !(P.radix && !P.rx.dense_out)) {
     // no ORDER BY: nothing is sorted
   } else if (one_sign && keyed) {
+    if (sp.g_mode == G_WIDE) comps_fit = add_comp(C_U64_AT, 0, sp.g_key_off + 8, ~0u, false, 0) && comps_fit;
     comps_fit = add_comp(C_U64_AT, 0, sp.g_key_off, ~0u, false, 0) && comps_fit;
   } else if (!one_sign) {
     for (int i = (int)sp.n_parts - 1; i >= 0; i--) {
@@ -1559,13 +1656,19 @@ struct ArenaScope {
 int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
                  const NoAllocOut* na, uint64_t gen);
 
+// run_plan's request to run again after re-planning (never returned to a
+// caller: the loop ends because each retry doubles the leaves, up to 2^20)
+constexpr int32_t kRetryMoreLeaves = -1000;
+
 }  // namespace
 
 int32_t tdbg::rt::execute_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool views, tdbg_result** out,
                                uint64_t cancel_gen) {
   if (!pl || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
-  return run_plan(pl, rgs, n_rgs, views, out, nullptr, cancel_gen);
+  int32_t rc;
+  while ((rc = run_plan(pl, rgs, n_rgs, views, out, nullptr, cancel_gen)) == kRetryMoreLeaves) {}
+  return rc;
 }
 
 int32_t tdbg::rt::execute_plan_noalloc(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, const NoAllocOut& na) {
@@ -1574,7 +1677,10 @@ int32_t tdbg::rt::execute_plan_noalloc(tdbg_plan* pl, const tdbg_rg_ref* rgs, ui
   if (na.n_cols != pl->keys.size() + pl->outs.size())
     return fail(TDBG_ERR_INVALID, "%u column views for a plan of %zu output columns", na.n_cols,
                 pl->keys.size() + pl->outs.size());
-  return run_plan(pl, rgs, n_rgs, false, nullptr, &na, pl->cancel_gen.load());
+  const uint64_t gen = pl->cancel_gen.load();
+  int32_t rc;
+  while ((rc = run_plan(pl, rgs, n_rgs, false, nullptr, &na, gen)) == kRetryMoreLeaves) {}
+  return rc;
 }
 
 namespace {
@@ -1653,7 +1759,7 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   // nothing else - a batch's hints - prepared the plan since, and the
   // planner's switches read the same).
   std::string env_now;
-  for (const char* v : {"THINDB_GPU_RADIX", "THINDB_GPU_RADIX_LEAF_BITS", "THINDB_GPU_RADIX_BITS1"}) {
+  for (const char* v : {"THINDB_GPU_RADIX", "THINDB_GPU_RADIX_LEAF_BITS", "THINDB_GPU_RADIX_BITS1", "THINDB_GPU_RADIX_HASH"}) {
     const char* e = getenv(v);
     env_now += e ? e : "-";
     env_now += '|';
@@ -1897,6 +2003,12 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     HIP_TRY(hipMemcpyAsync(count_hist, P.rx.count_hist, sizeof count_hist, hipMemcpyDeviceToHost, s), "count histogram");
   if ((rc = sync_check(pl, "scan"))) return rc;
   if (err_bits & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
+  if ((err_bits & ERR_HASH_FULL) && P.radix && P.rx.hashed && P.rx.n_leaves < (1u << 20)) {
+    // a leaf held more groups than its table: partition into twice the leaves
+    pl->rx_hash_boost++;
+    pl->exec_prep_stamp = ~0ull;
+    return kRetryMoreLeaves;
+  }
   if (err_bits & ERR_HASH_FULL) return fail(TDBG_ERR_OOM, "group table full: est_groups too small");
   if (cancelled(pl, gen)) return fail(TDBG_ERR_CANCELLED, "cancelled");
   if (P.radix && !P.rx.dense_out && n_groups > n_rec)
@@ -1997,7 +2109,7 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   pl->last.bytes_scanned = scanned_bytes;
   pl->last.n_groups = n_groups;
   pl->last.lds_mode = sp.lds_mode;
-  pl->last.global_mode = P.radix ? 3u : sp.g_mode;
+  pl->last.global_mode = P.radix ? (P.rx.hashed ? 4u : 3u) : sp.g_mode;
   pl->last.specialized = specialized ? 1u : 0u;
   pl->last.lds_parts = sp.lds_mode == LDS_DIRECT ? sp.lds_parts : 1u;
   if (na) {

@@ -227,6 +227,9 @@ struct tdbg_plan {
   // while the buffers it uploaded them to (up_*) are the current ones
   uint64_t prep_stamp = 0, exec_prep_stamp = ~0ull, up_stamp = ~0ull;
   std::string exec_prep_env;
+  // hash partitions: extra leaf bits after a leaf table overflowed (kept for
+  // later executions of the plan)
+  uint32_t rx_hash_boost = 0;
   void *up_tiles = nullptr, *up_tmpl = nullptr, *up_hleaf = nullptr;
   // tdbg_execute_noalloc: while set (under `mu`), ensure() carves every
   // buffer from this device scratch region instead of allocating

@@ -173,6 +173,9 @@ void lane_main(tdbg_server* s, uint32_t lane) {
         tdbg_result* r = nullptr;
         int32_t ri = rc;
         if (rc == TDBG_OK) ri = batch_take_result(s->batches[lane], (uint32_t)i, &r);
+        // a plan a batch cannot hold (a key domain only hash partitions
+        // cover): each request runs alone
+        if (rc == TDBG_ERR_UNSUPPORTED) ri = execute_plan(live[i]->plan, nullptr, 0, true, &r, live[i]->cancel_gen);
         if (ri == TDBG_OK && cancelled(live[i]->plan, live[i]->cancel_gen)) {  // cancelled while it ran
           tdbg_result_free(r);
           r = nullptr;
