@@ -103,3 +103,16 @@ def test_leaf_overflow_retries_with_more_leaves(session, oracle_lib, big_wide):
     req = Request(["a", "b"], AGGS[:4], [SortSpec("c", True)], 20, est_groups=1200)
     got, want, st = check_vs_oracle(session, oracle_lib, t, req, dop=8)
     assert st.global_mode == 4 and st.n_groups == 1_200_000
+
+
+@pytest.mark.parametrize("env", [{"THINDB_GPU_RADIX_HASH_TAG_BITS": "2"}, {"THINDB_GPU_RADIX_HASH_EXACT": "1"}])
+def test_hashed_fold_fallbacks(session, oracle_lib, monkeypatch, env):
+    """The leaf fold's exact form: reached through a tag collision (tags
+    narrowed to two bits, so rows meet other keys' tags on their probe paths
+    and their leaves are folded again), and run alone."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    t = wide_key_table(n=600_000, groups=150_000, seed=5)
+    _, want, st = check_vs_oracle(session, oracle_lib, t, Request(["a", "b"], AGGS), dop=8)
+    assert st.global_mode == 4 and st.n_groups == len(want) == 150_000
+    check_vs_oracle(session, oracle_lib, t, Request(["a", "b"], AGGS, [SortSpec("c", True)], 30), dop=8)

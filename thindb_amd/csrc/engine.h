@@ -203,9 +203,13 @@ struct RadixParams {
   // a 32-bit hash of the key (leaf_bits = the hash bits below both digits),
   // and each leaf folds its rows into an LDS hash table of ScanParams.lds_cap
   // slots: u32 tags at tag_off (0 = free), the keys (row.key_words u64 words
-  // per slot) at ScanParams.lds_keys_off, the aggregate arrays as usual.
+  // per slot) at ScanParams.lds_keys_off, the list of claimed slots at
+  // list_off, the aggregate arrays as usual.
   uint32_t hashed;
   uint32_t tag_off;
+  uint32_t list_off;           // u32 [lds_cap]: the slots a leaf claimed, in claim order
+  uint32_t tag_mask;           // tag bits kept (all but in tests, which narrow them to force tag collisions)
+  uint32_t hash_exact;         // leaves fold only in the exact (round-based) form
 };
 
 // ---- ordering / emission -------------------------------------------------

@@ -263,12 +263,14 @@ __global__ void __launch_bounds__(512) rx_agg_hashed_kernel(const ScanParams P, 
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t tmp[kRxAggTmp];
   __shared__ uint32_t chist[kRxCountBins];
+  __shared__ uint32_t n_claimed;
   if (R.counts_out) {  // uniform
     for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x) chist[b] = 0;
     __syncthreads();
   }
+  rx_hashed_clear(P, R, lds, &n_claimed);  // each leaf leaves the table as clean as it found it
   for (uint32_t leaf = blockIdx.x; leaf < R.n_leaves; leaf += gridDim.x)
-    rx_agg_leaf_hashed<HAVING>(P, R, leaf, lds, tmp, chist);
+    rx_agg_leaf_hashed<HAVING>(P, R, leaf, lds, tmp, chist, &n_claimed);
   if (R.counts_out) {
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x)

@@ -587,30 +587,45 @@ __device__ __forceinline__ bool rx_having_pass(const ScanParams& P, const Having
   return (st & 1ull) != 0;
 }
 
+// A written group's row count to R.counts_out[at] and its bin of chist
+// (kRxCountBins LDS counters of min(count, kRxCountBins - 1)); every lane of
+// the wavefront calls it (occ: this lane wrote a group).
+__device__ __forceinline__ void rx_count_out(const RadixParams& R, uint64_t at, uint32_t c, bool occ,
+                                             uint32_t* chist) {
+  const uint32_t lane = threadIdx.x & 63;
+  if (occ) R.counts_out[at] = c;
+  const uint32_t b = c < kRxCountBins ? c : kRxCountBins - 1;
+  // the common small counts: one LDS atomic per wavefront and value
+  for (uint32_t v = 1; v <= 3; v++) {
+    const uint64_t mv = __ballot(occ && b == v);
+    if (lane == 0 && mv) atomicAdd(&chist[v], (uint32_t)__popcll(mv));
+  }
+  if (occ && b > 3) atomicAdd(&chist[b], 1u);
+}
+
 // The occupied slots of a leaf's LDS table (count != 0; with HAVING those
-// passing it) as group records from row index s on, in slot order, and
-// their number to R.leaf_groups[leaf]: slot k * blockDim.x + t is handled by
-// thread t in round k, so each record store is coalesced; per (round,
+// passing it) as group records from row index s on, and their number to
+// R.leaf_groups[leaf].  Positions p < n_pos name slot slot_of(p) (the table
+// in slot order, or a list of slots); position k * blockDim.x + t is handled
+// by thread t in round k, so each record store is coalesced; per (round,
 // wavefront) counts are scanned in that order.  key_of(slot) gives a slot's
-// group key.  tmp: blockDim.x / 64 + 1 + cap / 64 LDS words.
-//
-// With R.counts_out each written group's row count also goes to
-// R.counts_out[record] and into chist (kRxCountBins LDS counters of
-// min(count, kRxCountBins - 1)).
-template <bool HAVING, typename KeyOf>
+// group key.  tmp: blockDim.x / 64 + 1 + n_pos / 64 LDS words.  With
+// R.counts_out each written group's row count goes out too (rx_count_out).
+template <bool HAVING, typename SlotOf, typename KeyOf>
 __device__ __forceinline__ void rx_emit_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint64_t s,
-                                             uint8_t* lds, uint32_t* tmp, uint32_t* chist, KeyOf key_of) {
-  const uint32_t cap = P.lds_cap;
+                                             uint32_t n_pos, uint8_t* lds, uint32_t* tmp, uint32_t* chist,
+                                             SlotOf slot_of, KeyOf key_of) {
   const uint32_t* count = lds_at<uint32_t>(lds, P.lds_count_off);
   const uint32_t nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
-  const uint32_t rounds = (cap + blockDim.x - 1) / blockDim.x;
+  const uint32_t rounds = (n_pos + blockDim.x - 1) / blockDim.x;
   uint32_t* scan_tmp = tmp;
   uint32_t* wcnt = tmp + nw + 1;  // rounds * nw entries
   const HavingLeaf* H = HAVING ? R.having : nullptr;
   uint64_t keep = 0;  // each round's outcome (rounds beyond 64 evaluate again below)
   for (uint32_t k = 0; k < rounds; k++) {
-    const uint32_t slot = k * blockDim.x + threadIdx.x;
-    bool occ = slot < cap && count[slot] != 0;
+    const uint32_t p = k * blockDim.x + threadIdx.x;
+    const uint32_t slot = p < n_pos ? slot_of(p) : 0u;
+    bool occ = p < n_pos && count[slot] != 0;
     if (H && occ) occ = rx_having_pass(P, *H, lds, slot, key_of(slot));
     if (k < 64 && occ) keep |= 1ull << k;
     const uint64_t m = __ballot(occ);
@@ -629,28 +644,19 @@ __device__ __forceinline__ void rx_emit_leaf(const ScanParams& P, const RadixPar
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63;
   for (uint32_t k = 0; k < rounds; k++) {
-    const uint32_t slot = k * blockDim.x + threadIdx.x;
+    const uint32_t p = k * blockDim.x + threadIdx.x;
+    const uint32_t slot = p < n_pos ? slot_of(p) : 0u;
     bool occ;
     if (k < 64) {
       occ = (keep >> k) & 1ull;
     } else {
-      occ = slot < cap && count[slot] != 0;
+      occ = p < n_pos && count[slot] != 0;
       if (H && occ) occ = rx_having_pass(P, *H, lds, slot, key_of(slot));
     }
     const uint64_t m = __ballot(occ);
     const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
     if (occ) rx_write_record(P, lds, slot, key_of(slot), P.g_state + at * P.g_rec);
-    if (R.counts_out) {  // uniform
-      const uint32_t c = occ ? count[slot] : 0u;
-      if (occ) R.counts_out[at] = c;
-      const uint32_t b = c < kRxCountBins ? c : kRxCountBins - 1;
-      // the common small counts: one LDS atomic per wavefront and value
-      for (uint32_t v = 1; v <= 3; v++) {
-        const uint64_t mv = __ballot(occ && b == v);
-        if (lane == 0 && mv) atomicAdd(&chist[v], (uint32_t)__popcll(mv));
-      }
-      if (occ && b > 3) atomicAdd(&chist[b], 1u);
-    }
+    if (R.counts_out) rx_count_out(R, at, occ ? count[slot] : 0u, occ, chist);  // uniform
   }
   if (threadIdx.x == 0) R.leaf_groups[leaf] = total;
   __syncthreads();
@@ -709,33 +715,39 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
     __syncthreads();
     return;
   }
-  rx_emit_leaf<HAVING>(P, R, leaf, s, lds, tmp, chist, [&](uint32_t slot) { return GKey{leaf_key0 + slot, 0}; });
+  rx_emit_leaf<HAVING>(P, R, leaf, s, cap, lds, tmp, chist, [](uint32_t p) { return p; },
+                       [&](uint32_t slot) { return GKey{leaf_key0 + slot, 0}; });
 }
 
-// Hash partitions: leaf `leaf`'s rows fold into an LDS hash table (u32 tags,
-// 0 = free; the key words of each claimed slot), then its groups go out as
-// rx_emit_leaf writes them.  Rows are taken blockDim.x at a time, and each
-// batch settles in rounds between barriers, so no thread ever waits on
-// another: a row either claims a free slot by CAS on its tag (and writes the
-// key), or stops at a slot holding its tag, whose key is compared after the
-// barrier (the claimer's key is then in place); a mismatch probes on from
-// the next slot in the next round.  A table with no free slot left for a row
-// raises ERR_HASH_FULL (the host then partitions into more leaves).
-template <bool HAVING>
-__device__ __forceinline__ void rx_agg_leaf_hashed(const ScanParams& P, const RadixParams& R, uint32_t leaf,
-                                                   uint8_t* lds, uint32_t* tmp, uint32_t* chist) {
+// Records a claimed slot in the leaf's list of claimed slots (list, *n
+// entries): one LDS atomic per wavefront.  Every lane of the wavefront calls
+// it (claimed: this lane claimed `slot`).
+__device__ __forceinline__ void rx_list_claim(uint32_t* list, uint32_t* n, bool claimed, uint32_t slot) {
+  const uint64_t m = __ballot(claimed);
+  if (m == 0) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(n, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (claimed) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = slot;
+}
+
+// Hash partitions, exact form: rows [s, e) of a leaf fold into the LDS hash
+// table (u32 tags, 0 = free; each claimed slot's key words, and the list of
+// claimed slots).  Rows are taken blockDim.x at a time, and each batch
+// settles in rounds between barriers, so no thread ever waits on another: a
+// row either claims a free slot by CAS on its tag (and writes the key), or
+// stops at a slot holding its tag, whose key is compared after the barrier
+// (the claimer's key is then in place); a mismatch probes on from the next
+// slot in the next round.  A table with no free slot left for a row raises
+// ERR_HASH_FULL (the host then partitions into more leaves).
+__device__ __forceinline__ void rx_hashed_fold_exact(const ScanParams& P, const RadixParams& R, uint64_t s,
+                                                     uint64_t e, uint8_t* lds, uint32_t* n_claimed) {
   const uint32_t cap = P.lds_cap, stride = R.row.stride, kw = R.row.key_words;
-  uint64_t s, e;
-  rx_leaf_range(R, leaf, s, e);
-  if (s == e) {  // uniform
-    if (threadIdx.x == 0) R.leaf_groups[leaf] = 0;
-    return;
-  }
   uint32_t* tag = lds_at<uint32_t>(lds, R.tag_off);
+  uint32_t* list = lds_at<uint32_t>(lds, R.list_off);
   uint64_t* keys = lds_at<uint64_t>(lds, P.lds_keys_off);
-  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) tag[i] = 0;
-  lds_init(P, lds);
-  __syncthreads();
   for (uint64_t b = s; b < e; b += blockDim.x) {  // uniform
     const uint64_t i = b + threadIdx.x;
     bool pending = i < e;
@@ -748,10 +760,11 @@ __device__ __forceinline__ void rx_agg_leaf_hashed(const ScanParams& P, const Ra
       khi = kw == 2 ? kp[1] : 0ull;
       const uint64_t h = rx_key_hash(klo, khi);
       slot = __umulhi((uint32_t)h, cap);
-      t = (uint32_t)(h >> 16) | 1u;
+      t = ((uint32_t)(h >> 16) & R.tag_mask) | 1u;
     }
     while (__syncthreads_or(pending)) {
       int cand = -1;
+      bool claimed = false;
       if (pending) {
         for (;;) {
           uint32_t cur = __hip_atomic_load(tag + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -763,6 +776,7 @@ __device__ __forceinline__ void rx_agg_leaf_hashed(const ScanParams& P, const Ra
               if (kw == 2) keys[(uint64_t)slot * kw + 1] = khi;
               rx_fold_row(P, R, lds, row, slot);
               pending = false;
+              claimed = true;
               break;
             }
             cur = expect;
@@ -779,6 +793,7 @@ __device__ __forceinline__ void rx_agg_leaf_hashed(const ScanParams& P, const Ra
           }
         }
       }
+      rx_list_claim(list, n_claimed, claimed, slot);
       __syncthreads();
       if (cand >= 0) {
         const uint64_t* kk = keys + (uint64_t)cand * kw;
@@ -795,11 +810,151 @@ __device__ __forceinline__ void rx_agg_leaf_hashed(const ScanParams& P, const Ra
       }
     }
   }
+}
+
+// Clears the whole hash table: tags free, aggregate states initial, no
+// claimed slots.
+__device__ __forceinline__ void rx_hashed_clear(const ScanParams& P, const RadixParams& R, uint8_t* lds,
+                                                uint32_t* n_claimed) {
+  uint32_t* tag = lds_at<uint32_t>(lds, R.tag_off);
+  for (uint32_t i = threadIdx.x; i < P.lds_cap; i += blockDim.x) tag[i] = 0;
+  lds_init(P, lds);
+  if (threadIdx.x == 0) *n_claimed = 0;
   __syncthreads();
-  rx_emit_leaf<HAVING>(P, R, leaf, s, lds, tmp, chist, [&](uint32_t slot) {
-    const uint64_t* kk = keys + (uint64_t)slot * kw;
+}
+
+// Hash partitions: leaf `leaf`'s rows fold into an LDS hash table, its groups
+// go out as records from the leaf's first row on, and the slots it claimed
+// are cleared again (the table starts clean: rx_hashed_clear once per
+// workgroup), so a leaf costs in proportion to its rows and groups, not to
+// the table's size.
+//
+// The fold is optimistic: each thread loads the keys of up to kRxHashPer
+// rows at once (one memory round trip for most leaves), and each row folds
+// straight into the slot it claims or the first slot holding its tag.
+// After a barrier every row that followed a tag checks the slot's key; a
+// mismatch (two keys of one 31-bit tag on one probe path: rare) voids the
+// whole leaf, which is cleared and folded again in the exact form.  With
+// R.hash_exact only the exact form runs.
+//
+// Without HAVING each claimed slot is a group: list position p goes to
+// record s + p.  With HAVING, rx_emit_leaf compacts the passing ones.
+constexpr uint32_t kRxHashPer = 4;
+
+template <bool HAVING>
+__device__ __forceinline__ void rx_agg_leaf_hashed(const ScanParams& P, const RadixParams& R, uint32_t leaf,
+                                                   uint8_t* lds, uint32_t* tmp, uint32_t* chist,
+                                                   uint32_t* n_claimed) {
+  const uint32_t cap = P.lds_cap, stride = R.row.stride, kw = R.row.key_words;
+  uint64_t s, e;
+  rx_leaf_range(R, leaf, s, e);
+  if (s == e) {  // uniform
+    if (threadIdx.x == 0) R.leaf_groups[leaf] = 0;
+    return;
+  }
+  uint32_t* tag = lds_at<uint32_t>(lds, R.tag_off);
+  uint32_t* list = lds_at<uint32_t>(lds, R.list_off);
+  uint64_t* keys = lds_at<uint64_t>(lds, P.lds_keys_off);
+  bool exact = R.hash_exact != 0;
+  for (uint64_t b = s; b < e && !exact; b += (uint64_t)blockDim.x * kRxHashPer) {  // uniform
+    uint64_t klo[kRxHashPer], khi[kRxHashPer];
+    uint32_t slot[kRxHashPer];
+    uint32_t live = 0, follow = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRxHashPer; k++) {
+      const uint64_t i = b + threadIdx.x + (uint64_t)k * blockDim.x;
+      klo[k] = khi[k] = 0;
+      if (i < e) {
+        const uint64_t* kp = reinterpret_cast<const uint64_t*>(R.rows2 + i * stride + R.row.key_off);
+        live |= 1u << k;
+        klo[k] = kp[0];
+        if (kw == 2) khi[k] = kp[1];
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kRxHashPer; k++) {
+      bool claimed = false, placed = false;
+      uint32_t sl = 0;
+      if ((live >> k) & 1u) {
+        const uint64_t h = rx_key_hash(klo[k], khi[k]);
+        const uint32_t t = ((uint32_t)(h >> 16) & R.tag_mask) | 1u;
+        uint32_t probes = 0;
+        sl = __umulhi((uint32_t)h, cap);
+        for (;;) {
+          uint32_t cur = __hip_atomic_load(tag + sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (cur == 0) {
+            uint32_t expect = 0;
+            if (__hip_atomic_compare_exchange_strong(tag + sl, &expect, t, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) {
+              keys[(uint64_t)sl * kw] = klo[k];
+              if (kw == 2) keys[(uint64_t)sl * kw + 1] = khi[k];
+              claimed = placed = true;
+              break;
+            }
+            cur = expect;
+          }
+          if (cur == t) {  // its key is compared after the barrier
+            follow |= 1u << k;
+            placed = true;
+            break;
+          }
+          sl = sl + 1 == cap ? 0 : sl + 1;
+          if (++probes >= cap) break;
+        }
+        if (placed) {
+          const uint64_t i = b + threadIdx.x + (uint64_t)k * blockDim.x;
+          rx_fold_row(P, R, lds, R.rows2 + i * stride, sl);
+        } else {
+          atomicOr(P.err, ERR_HASH_FULL);
+        }
+      }
+      slot[k] = sl;
+      rx_list_claim(list, n_claimed, claimed, sl);
+    }
+    __syncthreads();
+    bool bad = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kRxHashPer; k++) {
+      if (!((follow >> k) & 1u)) continue;
+      const uint64_t* kk = keys + (uint64_t)slot[k] * kw;
+      if (kk[0] != klo[k] || (kw == 2 && kk[1] != khi[k])) bad = true;
+    }
+    if (__syncthreads_or(bad)) {  // a row went to another key's group: fold the leaf again, exactly
+      rx_hashed_clear(P, R, lds, n_claimed);
+      exact = true;
+    }
+  }
+  if (exact) {
+    rx_hashed_fold_exact(P, R, s, e, lds, n_claimed);
+    __syncthreads();
+  }
+  const uint32_t n = *n_claimed;
+  auto key_of = [&](uint32_t sl) {
+    const uint64_t* kk = keys + (uint64_t)sl * kw;
     return GKey{kk[0], kw == 2 ? kk[1] : 0ull};
-  });
+  };
+  if (HAVING) {
+    rx_emit_leaf<true>(P, R, leaf, s, n, lds, tmp, chist, [&](uint32_t p) { return list[p]; }, key_of);
+  } else {
+    const uint32_t* count = lds_at<uint32_t>(lds, P.lds_count_off);
+    for (uint32_t p0 = 0; p0 < n; p0 += blockDim.x) {  // uniform
+      const uint32_t p = p0 + threadIdx.x;
+      const bool occ = p < n;
+      const uint32_t sl = occ ? list[p] : 0u;
+      if (occ) rx_write_record(P, lds, sl, key_of(sl), P.g_state + (s + p) * P.g_rec);
+      if (R.counts_out) rx_count_out(R, s + p, occ ? count[sl] : 0u, occ, chist);
+    }
+    if (threadIdx.x == 0) R.leaf_groups[leaf] = n;
+    __syncthreads();
+  }
+  // the claimed slots back to their initial state
+  for (uint32_t p = threadIdx.x; p < n; p += blockDim.x) {
+    const uint32_t sl = list[p];
+    tag[sl] = 0;
+    lds_init_slot(P, lds, sl);
+  }
+  if (threadIdx.x == 0) *n_claimed = 0;
+  __syncthreads();
 }
 
 }  // namespace tdbg

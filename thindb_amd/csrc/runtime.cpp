@@ -876,8 +876,10 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     lay_out_rows(R, kw);
     // a slot: tag, key words, then the aggregate states; two leaf
     // workgroups per CU
-    const uint32_t sb = slot_bytes_for(n_rows_scan) + 4 + 8 * kw;
-    const uint32_t leaf_budget = std::min<uint32_t>(budget, (uint32_t)pl->ctx->max_lds / 2) - 4096;
+    const uint32_t sb = slot_bytes_for(n_rows_scan) + 8 + 8 * kw;  // + tag, list entry, key words
+    const char* wg_env = getenv("THINDB_GPU_RADIX_HASH_WGS");  // leaf workgroups per CU (tuning)
+    const uint32_t wgs = std::max<uint32_t>(1, std::min<uint32_t>(4, wg_env ? (uint32_t)atoi(wg_env) : 2u));
+    const uint32_t leaf_budget = std::min<uint32_t>(budget, (uint32_t)pl->ctx->max_lds / wgs) - 4096;
     const uint32_t pad = 8 * (2 * sp.n_aggs + 5) + 16;
     const uint32_t cap_max = std::min<uint32_t>((leaf_budget - pad) / sb / 64 * 64, 8192);
     // leaves for the estimated groups at most ~45% of a table on average
@@ -885,8 +887,10 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     // overflowed
     const uint64_t G = std::max<uint64_t>(1, pl->est_groups ? std::min<uint64_t>(pl->est_groups, dom)
                                                             : std::min<uint64_t>(n_rows_scan, dom));
+    const char* fill_env = getenv("THINDB_GPU_RADIX_HASH_FILL");  // percent of a table (tuning)
+    const uint64_t fill = fill_env ? std::max(10, std::min(90, atoi(fill_env))) : 45;
     uint32_t rem = 8;
-    while (rem < 20 && (G >> rem) > (uint64_t)cap_max * 45 / 100) rem++;
+    while (rem < 20 && (G >> rem) > (uint64_t)cap_max * fill / 100) rem++;
     rem = std::min<uint32_t>(20, rem + pl->rx_hash_boost);
     const uint64_t per_leaf = (G + (1ull << rem) - 1) >> rem;
     if (cap_max >= 256 && R.tile_rows >= 256) {
@@ -894,6 +898,13 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
       const uint32_t cap = (uint32_t)std::min<uint64_t>(cap_max, std::max<uint64_t>(256, align_up(2 * per_leaf + 128, 64)));
       const uint32_t max_bits1 = std::min<uint32_t>(hint.max_bits1 ? hint.max_bits1 : 8u, 10u);
       R.hashed = 1;
+      // tests narrow the tags to force collisions (THINDB_GPU_RADIX_HASH_TAG_BITS)
+      // or run the exact fold alone (THINDB_GPU_RADIX_HASH_EXACT=1)
+      const char* tb_env = getenv("THINDB_GPU_RADIX_HASH_TAG_BITS");
+      const uint32_t tb = tb_env ? std::min<uint32_t>(32, std::max<uint32_t>(1, (uint32_t)atoi(tb_env))) : 32u;
+      R.tag_mask = tb >= 32 ? ~0u : (1u << tb) - 1u;
+      const char* ex_env = getenv("THINDB_GPU_RADIX_HASH_EXACT");
+      R.hash_exact = ex_env && ex_env[0] == '1' ? 1u : 0u;
       R.bits1 = rem <= 10 ? rem : std::max<uint32_t>(std::min<uint32_t>(max_bits1, rem), rem - 10);
       R.bits2 = rem - R.bits1;
       R.leaf_bits = 32 - rem;  // hash bits below the digits
@@ -934,7 +945,10 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
   };
   if (sp.lds_mode != LDS_NONE) {
     if (sp.lds_mode == LDS_HASH) sp.lds_keys_off = lds_take(lds_cap * (P.rx.hashed ? 8 * P.rx.row.key_words : key_bytes));
-    if (P.rx.hashed) P.rx.tag_off = lds_take(lds_cap * 4);
+    if (P.rx.hashed) {
+      P.rx.tag_off = lds_take(lds_cap * 4);
+      P.rx.list_off = lds_take(lds_cap * 4);
+    }
     for (uint32_t k = 0; k < sp.n_aggs; k++) {
       AggDesc& d = sp.aggs[k];
       if (d.kind == A_COUNT_COL) continue;
@@ -1759,7 +1773,9 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   // nothing else - a batch's hints - prepared the plan since, and the
   // planner's switches read the same).
   std::string env_now;
-  for (const char* v : {"THINDB_GPU_RADIX", "THINDB_GPU_RADIX_LEAF_BITS", "THINDB_GPU_RADIX_BITS1", "THINDB_GPU_RADIX_HASH"}) {
+  for (const char* v : {"THINDB_GPU_RADIX", "THINDB_GPU_RADIX_LEAF_BITS", "THINDB_GPU_RADIX_BITS1", "THINDB_GPU_RADIX_HASH",
+                        "THINDB_GPU_RADIX_HASH_TAG_BITS", "THINDB_GPU_RADIX_HASH_EXACT", "THINDB_GPU_RADIX_HASH_WGS",
+                        "THINDB_GPU_RADIX_HASH_FILL"}) {
     const char* e = getenv(v);
     env_now += e ? e : "-";
     env_now += '|';

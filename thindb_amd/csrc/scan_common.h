@@ -404,6 +404,25 @@ __device__ __forceinline__ void lds_init(const ScanParams& P, uint8_t* lds) {
   }
 }
 
+// One slot of the LDS table back to its initial state (as lds_init leaves it).
+__device__ __forceinline__ void lds_init_slot(const ScanParams& P, uint8_t* lds, uint32_t slot) {
+  const uint32_t cap = P.lds_cap;
+  lds_at<uint32_t>(lds, P.lds_count_off)[slot] = 0;
+  for (uint32_t a = 0; a < P.n_aggs; a++) {
+    const AggDesc& d = P.aggs[a];
+    if (d.kind == A_COUNT_COL) {
+      lds_at<uint32_t>(lds, d.lds_off)[slot] = 0;
+      continue;
+    }
+    uint64_t init = 0;
+    if (d.kind == A_MIN_I || d.kind == A_MIN_F) init = 0x7fffffffffffffffull;
+    if (d.kind == A_MAX_I || d.kind == A_MAX_F) init = 0x8000000000000000ull;
+    lds_at<uint64_t>(lds, d.lds_off)[slot] = init;
+    if (d.kind == A_SUM_I && d.split) lds_at<uint64_t>(lds, d.lds_off)[cap + slot] = init;
+    if (d.nullable) lds_at<uint32_t>(lds, d.lds_ns_off)[slot] = 0;
+  }
+}
+
 // Every occupied LDS slot goes to its HBM record once.
 __device__ __forceinline__ void lds_flush(const ScanParams& P, uint8_t* lds, uint64_t part_lo) {
   const uint32_t cap = P.lds_cap;
