@@ -4,7 +4,8 @@
 With no --workload, one run measures every configuration and prints ONE
 JSON line: the headline (the filtered query below) at the top level, and
 `configs` holding a record per other configuration - plain (4M rows),
-highcard (200M rows, ~149M groups), batched (512 uneven queries in one
+highcard (200M rows, ~149M groups), wide (100M rows grouped by two
+full-range BIGINT keys, ~43M groups), batched (512 uneven queries in one
 launch) and serving (Poisson arrivals) - each with its own value, roofline
 and CPU baseline, plus `measured_peak`: the device's read and copy bandwidth
 measured in the same run (the roofline's second denominator).
@@ -23,7 +24,9 @@ roofline block prices the dominant kernel (the fused scan/filter/aggregate)
 at 28 algorithmic bytes per row (4+8+8+8) against 8 TB/s, with its duration
 from HIP events on the plan's stream.  For `highcard` it prices the radix
 partition/aggregate kernels together (events around the sequence) at 21
-bytes per row plus 33 bytes of group state per group.  CPU baselines time
+bytes per row plus 33 bytes of group state per group; for `wide` the hash
+partition kernels at 20 bytes per row plus 32 per group (128-bit key, COUNT,
+SUM).  CPU baselines time
 the oracle (the C restatement of thinDB's grouped aggregate) on a bounded
 sample of the same table, on every host core this job has (at most 16, the
 box's share per GPU) and on one core.
@@ -37,7 +40,7 @@ through the single-query operator.  A step is one pass over all 512 queries,
 results delivered to host memory; `value` is queries per second.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
-       [--workload all|filtered|plain|highcard|batched|serving] [--having]
+       [--workload all|filtered|plain|highcard|wide|batched|serving] [--having]
 """
 from __future__ import annotations
 
@@ -60,11 +63,13 @@ WORKLOADS = {
     "filtered": (200_000_000, 28, "filtered GROUP BY region top-100 by SUM(amount), 200M rows"),
     "plain": (4_000_000, 16, "GROUP BY k COUNT/SUM/MIN/MAX, 4M rows"),
     "highcard": (200_000_000, 21, "GROUP BY (user_id, bucket) COUNT/SUM/MAX top-10 by COUNT, 200M rows"),
+    "wide": (100_000_000, 20, "GROUP BY (a, b) over two full-range BIGINT keys COUNT/SUM top-10 by COUNT, 100M rows"),
     "batched": (0, 12, "512 GROUP BY key COUNT/SUM/MIN/MAX queries over 512 uneven tables, one batch"),
     "serving": (0, 0, "serving loop under Poisson arrivals"),
 }
 BATCH_SEED = 7
 HIGHCARD_GROUP_BYTES = 33  # key 9 + COUNT 8 + SUM 8 + MAX 8 per group
+WIDE_GROUP_BYTES = 32      # key 16 + COUNT 8 + SUM 8 per group
 
 
 def log(msg: str) -> None:
@@ -102,6 +107,8 @@ def make_request(workload: str, having: bool = False, distinct: bool = False, wh
         return Request(["k"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"), AggSpec("min", "v", "mn"),
                                AggSpec("max", "v", "mx")],
                        having_filter=And((Predicate("c", "gt", 3900), Predicate("mx", "gt", 0))) if having else None)
+    if workload == "wide":
+        return Request(["a", "b"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s")], [SortSpec("c", True)], 10)
     return Request(["user_id", "bucket"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"),
                                            AggSpec("max", "w", "m")], [SortSpec("c", True)], 10,
                    est_groups=150_000_000,
@@ -621,6 +628,8 @@ def run_single(args, workload: str, sess, steps: int, warmup: int, table=None):
     if workload == "highcard" and not distinct:
         # plus the compulsory group state: key 9 + count 8 + sum 8 + max 8 bytes per group
         alg_bytes += int(st.n_groups) * HIGHCARD_GROUP_BYTES
+    if workload == "wide":
+        alg_bytes += int(st.n_groups) * WIDE_GROUP_BYTES
     if distinct:  # the distinct pass runs after the scan: price the whole device time
         scan_ms = total_ms
     avg_scan_s = float(np.mean(scan_ms)) / 1e3
@@ -632,18 +641,20 @@ def run_single(args, workload: str, sess, steps: int, warmup: int, table=None):
     cpu = None
     if not args.no_cpu:
         threads = host_threads()
-        big = workload in ("filtered", "highcard")
-        sample = args.cpu_sample_rows or min(rows, 40_000_000 if workload == "filtered" else 10_000_000)
-        cpu = cpu_baseline(table, req, sample, threads, min(sample, 4_000_000 if big else rows),
+        big = workload in ("filtered", "highcard", "wide")
+        sample = args.cpu_sample_rows or min(rows, {"filtered": 40_000_000, "wide": 4_000_000}.get(workload, 10_000_000))
+        cpu = cpu_baseline(table, req, sample, threads,
+                           min(sample, 2_000_000 if workload == "wide" else 4_000_000 if big else rows),
                            ORACLE_NOTE + ("; the silo's 2-level radix is not restated: this is the hash-table "
-                                          "fold over the same groups" if workload == "highcard" else ""))
+                                          "fold over the same groups" if workload in ("highcard", "wide") else ""))
         log(f"cpu baseline {cpu['value']:.4g} rows/s on {threads} cores, {cpu['dop1']['value']:.4g} on one")
 
     # HBM bytes per launch of the dominant kernel, from the committed
     # FETCH_SIZE/WRITE_SIZE counter passes (scaled by a 1 GiB copy kernel).
     traffic = None
-    radix = int(st.global_mode) == 3
-    pmc = ROOT / "profiles" / (f"radix_{workload}_200M_pmc.json" if radix else f"scan_jit_{workload}_200M_pmc.json")
+    radix = int(st.global_mode) in (3, 4)
+    size = f"{WORKLOADS[workload][0] // 1_000_000}M"
+    pmc = ROOT / "profiles" / (f"radix_{workload}_{size}_pmc.json" if radix else f"scan_jit_{workload}_{size}_pmc.json")
     if pmc.exists() and rows == WORKLOADS[workload][0] and (st.specialized or radix) and not having \
             and not distinct and not where_tree:
         traffic = json.loads(pmc.read_text())["traffic_bytes"]
@@ -670,6 +681,8 @@ def run_single(args, workload: str, sess, steps: int, warmup: int, table=None):
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
                      "traffic_source": str(pmc.relative_to(ROOT)) if traffic else None,
                      "kernel": ("device total: scan + distinct_insert_kernel + gather" if distinct
+                                else "hash-partition pipeline rx_hist+rx_scatter1+rx_hist2+rx_scatter2+rx_agg_hashed"
+                                if int(st.global_mode) == 4
                                 else "radix pipeline rx_hist+rx_scatter1+rx_hist2+rx_scatter2+rx_agg" if radix
                                 else "tdbg_scan_jit" if st.specialized else "scan_agg_kernel"),
                      "kernel_ms": float(np.mean(scan_ms)),
@@ -750,6 +763,10 @@ def main() -> int:
         configs["highcard"] = hc
         sess.evict_table(hc_table)
         del hc_table
+        wd, wd_table = run_single(args, "wide", sess, min(args.steps, 10), min(args.warmup, 2))
+        configs["wide"] = wd
+        sess.evict_table(wd_table)
+        del wd_table
         configs["batched"] = run_batched(args, sess, steps=min(args.steps, 5), warmup=min(args.warmup, 2))
         configs["serving"] = run_serving(args, sess, big=filtered)
         for c in configs.values():

@@ -162,10 +162,10 @@ struct ScanParams {
 // ---- radix partitioning (high cardinality) ----------------------------------
 
 // A surviving row as the partition passes move it: the relative key (u32;
-// hash partitions: the key's 32-bit hash, then the key itself as key_words
-// u64 words at key_off), a validity word when an aggregate input is
-// nullable, then each kernel aggregate's input value at its column's native
-// width.
+// hash partitions: the key's 32-bit hash), a validity word when an aggregate
+// input is nullable, each kernel aggregate's input value at its column's
+// native width, and with hash partitions the key itself last, as key_words
+// u64 words at key_off.
 struct RowLayout {
   uint32_t stride;             // bytes per row (multiple of 8)
   uint32_t flags_off;          // ~0u: no validity word

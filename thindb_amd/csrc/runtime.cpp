@@ -764,20 +764,16 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
   // THINDB_GPU_RADIX=0 never takes this path, =1 takes it whenever the
   // domain allows (tests use it to cover the partition passes on small tables).
   // The partitioned row (RowLayout) and the scatter tiles: the u32 relative
-  // key or hash, the validity word, the key words of hash partitions, then
-  // the aggregate inputs at native width (aggregates over one column share it).
+  // key or hash, the validity word, the aggregate inputs at native width
+  // (aggregates over one column share it), then the key words of hash
+  // partitions (last, 8-byte aligned, so narrow inputs fill the gap after
+  // the hash: a hash, an INT input and a 128-bit key make 24 bytes).
   auto lay_out_rows = [&](RadixParams& R, uint32_t key_words) {
     R.row.flags_off = ~0u;
     uint32_t ro = 4;
     for (uint32_t k = 0; k < sp.n_aggs; k++)
       if (sp.cols[sp.aggs[k].col].valid) R.row.flags_off = 0;
     if (R.row.flags_off == 0) { R.row.flags_off = ro; ro += 4; }
-    if (key_words) {
-      ro = (uint32_t)align_up(ro, 8);
-      R.row.key_off = ro;
-      R.row.key_words = key_words;
-      ro += 8 * key_words;
-    }
     for (uint32_t k = 0; k < sp.n_aggs; k++) {
       const AggDesc& d = sp.aggs[k];
       const uint32_t b = d.kind == A_COUNT_COL ? 0u : (uint32_t)t->cols[kaggs[k].col].elem;
@@ -789,6 +785,12 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
       ro = (uint32_t)align_up(ro, std::max<uint32_t>(b, 1));
       R.row.val_off[k] = ro;
       ro += b;
+    }
+    if (key_words) {
+      ro = (uint32_t)align_up(ro, 8);
+      R.row.key_off = ro;
+      R.row.key_words = key_words;
+      ro += 8 * key_words;
     }
     R.row.stride = (uint32_t)align_up(ro, 8);
     // staged scatter tile: 24 KiB of digit tables + tile_rows rows in LDS

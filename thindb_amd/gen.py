@@ -99,6 +99,23 @@ HIGHCARD_SCHEMA = [Column("user_id", Type.BIGINT), Column("bucket", Type.TINYINT
                    Column("w", Type.DOUBLE)]
 
 
+def wide(n: int, seed: int = 4) -> Dict[str, np.ndarray]:
+    """Two full-range BIGINT keys (a, b): n // 2 random pairs over all 2^64
+    values of each, every row picking one uniformly (about 43% of n distinct
+    pairs); v INT uniform in [-1000, 1000).  The shape of GROUP BY WatchID,
+    ClientIP: no relative packing narrows the key below 128 bits."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.iinfo(np.int64).min, np.iinfo(np.int64).max
+    pairs = max(1, n // 2)
+    ga = rng.integers(lo, hi, pairs, dtype=np.int64, endpoint=True)
+    gb = rng.integers(lo, hi, pairs, dtype=np.int64, endpoint=True)
+    pick = rng.integers(0, pairs, n)
+    return {"a": ga[pick], "b": gb[pick], "v": rng.integers(-1000, 1000, n).astype(np.int32)}
+
+
+WIDE_SCHEMA = [Column("a", Type.BIGINT), Column("b", Type.BIGINT), Column("v", Type.INT)]
+
+
 def batched_shape(seed: int, t: int) -> Tuple[int, int, bool]:
     """(rows, key cardinality, filtered) of batched table t."""
     rows = C.c_uint64()
@@ -124,13 +141,15 @@ def zipf_pmf(n: int, s: float, k: int) -> float:
 
 
 def table(kind: str, n: int, seed: int = None, **kw) -> Table:
-    """A host Table of one workload kind: plain, filtered or highcard."""
+    """A host Table of one workload kind: plain, filtered, highcard or wide."""
     if kind == "plain":
         return Table(PLAIN_SCHEMA, plain(n, seed if seed is not None else 1), **kw)
     if kind == "filtered":
         return Table(FILTERED_SCHEMA, filtered(n, seed if seed is not None else 2), **kw)
     if kind == "highcard":
         return Table(HIGHCARD_SCHEMA, highcard(n, seed if seed is not None else 3), **kw)
+    if kind == "wide":
+        return Table(WIDE_SCHEMA, wide(n, seed if seed is not None else 4), **kw)
     raise ValueError(kind)
 
 
