@@ -367,6 +367,7 @@ int32_t build(tdbg_batch* b) {
       q.item_hist = at<uint32_t>(b->d_misc, misc_o[i] + align_up(((2ull << q.rx.bits1) + 2) * 8, 256));
       q.rx.rows1 = at<uint8_t>(b->d_rows, rows_o[i]);
       q.rx.rows2 = q.rx.rows1;
+      q.rx.digits1 = nullptr;  // one level in a batch
       q.rx.bucket_start = at<uint64_t>(b->d_misc, misc_o[i]);
       q.rx.bucket_fill = q.rx.bucket_start + nb + 1;
       q.rx.leaf_start = nullptr;

@@ -192,6 +192,8 @@ struct RadixParams {
   uint32_t* counts_out;        // compacted leaves: each group's row count by record index, or null
   uint32_t* count_hist;        // [kRxCountBins] groups by min(row count, kRxCountBins - 1) (zeroed)
   uint8_t* rows1;              // level-1 partitioned rows
+  uint32_t* digits1;           // two levels: each level-1 row's u32 route value (its first word), packed, so
+                               // the level-2 count reads 4 bytes per row instead of the row (null: one level)
   uint8_t* rows2;              // level-2 partitioned rows (== rows1 when one level)
   uint64_t* bucket_start;      // [2^bits1 + 1] level-1 bucket starts (from the histogram)
   uint64_t* bucket_fill;       // [2^bits1] level-1 cursors: rows written per bucket

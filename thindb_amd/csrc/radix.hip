@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(kRxThreads) rx_hist2_kernel(const RadixParams 
   for (uint32_t d = threadIdx.x; d < nb2; d += blockDim.x) cnt[d] = 0;
   __syncthreads();
   for (uint32_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
-    const uint32_t key = *reinterpret_cast<const uint32_t*>(R.rows1 + (lo + i) * stride);
+    const uint32_t key = R.digits1 ? R.digits1[lo + i] : *reinterpret_cast<const uint32_t*>(R.rows1 + (lo + i) * stride);
     atomicAdd(&cnt[(key >> R.shift2) & m2], 1u);
   }
   __syncthreads();

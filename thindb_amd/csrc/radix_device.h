@@ -482,8 +482,11 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
       __syncthreads();
       for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
         const uint8_t* row = stage + (size_t)i * stride;
-        const uint32_t d = *reinterpret_cast<const uint32_t*>(row) >> R.shift1;
-        rx_copy_row(R.rows1 + (base[d] + (i - off[d])) * stride, row, stride);
+        const uint32_t route = *reinterpret_cast<const uint32_t*>(row);
+        const uint32_t d = route >> R.shift1;
+        const uint64_t at = base[d] + (i - off[d]);
+        rx_copy_row(R.rows1 + at * stride, row, stride);
+        if (R.digits1) R.digits1[at] = route;
       }
       __syncthreads();
     }

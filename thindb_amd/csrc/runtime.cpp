@@ -1865,7 +1865,7 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     RadixParams& R = P.rx;
     const uint64_t nb1 = 1ull << R.bits1;
     const uint64_t row_bytes = rows * R.row.stride + 256;
-    if ((rc = ensure(pl, pl->rx_rows1, row_bytes))) return rc;
+    if ((rc = ensure(pl, pl->rx_rows1, row_bytes + (R.bits2 ? rows * 4 + 256 : 0)))) return rc;
     if (R.bits2 && (rc = ensure(pl, pl->rx_rows2, row_bytes))) return rc;
     // leaf_groups: n_leaves counts, a total, then one sum per 256 leaves;
     // then the groups' count histogram
@@ -1881,6 +1881,8 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     R.bucket_fill = R.bucket_start + nb1 + 1;
     R.leaf_groups = reinterpret_cast<uint32_t*>(R.bucket_fill + nb1);
     R.rows1 = static_cast<uint8_t*>(pl->rx_rows1.p);
+    // two levels: the level-1 routes after the rows (the level-2 count reads them alone)
+    R.digits1 = R.bits2 ? reinterpret_cast<uint32_t*>(R.rows1 + align_up(row_bytes, 256)) : nullptr;
     R.rows2 = R.bits2 ? static_cast<uint8_t*>(pl->rx_rows2.p) : R.rows1;
     R.leaf_start = R.bits2 ? static_cast<uint64_t*>(pl->rx_leaf.p) : nullptr;
     R.hist2 = R.bits2 ? reinterpret_cast<uint32_t*>(R.leaf_start + 2ull * R.n_leaves + 32) : nullptr;
