@@ -95,8 +95,8 @@ struct AggDesc {
   uint32_t lds_ns_off;// byte offset of its LDS non-null counter array
   uint32_t g_off;     // byte offset of its state in the HBM record
   uint32_t g_ns_off;  // byte offset of its non-null count in the record
-  uint32_t lo_only;   // A_SUM_I: only the state's low 64 bits are read (a wrapped BIGINT SUM), so HBM
-                      // updates are single wrapping 64-bit adds (the high word stays 0)
+  uint32_t lo_only;   // A_SUM_I: only the state's low 64 bits are read (a wrapped BIGINT SUM), so the
+                      // record holds an 8-byte state and HBM updates are single wrapping 64-bit adds
   uint32_t pad;
 };
 

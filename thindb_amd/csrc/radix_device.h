@@ -359,7 +359,7 @@ __device__ __forceinline__ void rx_write_record(const ScanParams& P, uint8_t* ld
         v = (__int128)(int64_t)w;
       }
       *reinterpret_cast<uint64_t*>(g) = (uint64_t)v;
-      *reinterpret_cast<uint64_t*>(g + 8) = (uint64_t)(v >> 64);
+      if (!d.lo_only) *reinterpret_cast<uint64_t*>(g + 8) = (uint64_t)(v >> 64);
     } else {
       *reinterpret_cast<uint64_t*>(g) = w;  // f64 bits, or (ordered) i64
     }

@@ -667,9 +667,18 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     uint64_t init = 0;
     if (d.kind == A_MIN_I || d.kind == A_MIN_F) init = 0x7fffffffffffffffull;
     if (d.kind == A_MAX_I || d.kind == A_MAX_F) init = 0x8000000000000000ull;
+    // an integer SUM state read only as a wrapped BIGINT (no AVG, no DECIMAL
+    // SUM over it) needs just its low word: single wrapping HBM adds, and an
+    // 8-byte state instead of the i128
+    if (d.kind == A_SUM_I) {
+      bool lo = true;
+      for (auto& o : pl->outs)
+        if (o.kagg == (int)k && (o.op != TDBG_AGG_SUM || o.out_type == TDBG_DECIMAL128)) lo = false;
+      d.lo_only = lo ? 1u : 0u;
+    }
     tmpl_put(off, init);
     off += 8;
-    if (d.kind == A_SUM_I) { tmpl_put(off, 0); off += 8; }
+    if (d.kind == A_SUM_I && !d.lo_only) { tmpl_put(off, 0); off += 8; }
     d.g_ns_off = ~0u;
     if (d.nullable) { d.g_ns_off = off; tmpl_put(off, 0); off += 8; }
   }
@@ -677,15 +686,6 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     d.dslot_off = off;
     tmpl_put(off, 0);
     off += 8;
-  }
-  // an integer SUM state read only as a wrapped BIGINT (no AVG, no DECIMAL
-  // SUM over it) needs just its low word: single wrapping HBM adds
-  for (size_t k = 0; k < kaggs.size(); k++) {
-    if (sp.aggs[k].kind != A_SUM_I) continue;
-    bool lo = true;
-    for (auto& o : pl->outs)
-      if (o.kagg == (int)k && (o.op != TDBG_AGG_SUM || o.out_type == TDBG_DECIMAL128)) lo = false;
-    sp.aggs[k].lo_only = lo ? 1u : 0u;
   }
   sp.g_rec = std::max<uint32_t>(8, off);
   tmpl.resize(sp.g_rec / 8, 0);
