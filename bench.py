@@ -426,12 +426,14 @@ def run_serving(args, sess=None, big=None) -> dict:
     plans = {k: [] for k in classes}
     staged = 0
     plan_bytes = {}
+    plan_class = {}
     for k, pairs in classes.items():
         for table, req in pairs:
             plan, cols = build_plan(sess, table, req)
             staged += sess.stage_table(table, cols)
             plans[k].append(plan)
             plan_bytes[id(plan)] = request_bytes(table, req)
+            plan_class[id(plan)] = k
     log(f"resident: {staged / 1e9:.2f} GB in {sum(len(v) for v in plans.values())} tables, "
         f"{time.perf_counter() - t0:.1f}s")
 
@@ -475,6 +477,7 @@ def run_serving(args, sess=None, big=None) -> dict:
             while time.perf_counter() - start < at:
                 pass
             tickets.append(server.submit(p))
+        cls = [plan_class[id(p)] for p in reqs]
         served_bytes = sum(plan_bytes[id(p)] for p in reqs)
         lat, first, last = [], None, None
         units = {}
@@ -487,6 +490,14 @@ def run_serving(args, sess=None, big=None) -> dict:
             last = rep.done_ns if last is None else max(last, rep.done_ns)
             t.close()
         lat = np.array(lat)
+        # per request class: p50/p99 and the ratio of p99 to the class's single-query latency
+        per_class = {}
+        for k in ("small", "medium", "large"):
+            lk = np.array([x for x, c in zip(lat, cls) if c == k])
+            if len(lk):
+                per_class[k] = {"n": int(len(lk)), "p50_ms": float(np.percentile(lk, 50)),
+                                "p99_ms": float(np.percentile(lk, 99)),
+                                "p99_over_single": float(np.percentile(lk, 99)) / single[k]}
         # launches by size: how many, mean duration (ms)
         by = {}
         for size, ms in units.values():
@@ -496,7 +507,7 @@ def run_serving(args, sess=None, big=None) -> dict:
         return {"offered_qps": rate, "throughput_qps": len(reqs) / ((last - first) / 1e9),
                 "served_gbps": served_bytes / ((last - first) / 1e9) / 1e9,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "mean_ms": float(lat.mean()),
+                "mean_ms": float(lat.mean()), "per_class": per_class,
                 "launches": {k: {"count": n, "mean_ms": tot / n} for k, (n, tot) in sorted(by.items())}}
 
     results = {"served": [], "plain": []}
@@ -509,7 +520,8 @@ def run_serving(args, sess=None, big=None) -> dict:
             res["requests"] = n
             results[name].append(res)
             log(f"{name} {rate}/s: {res['throughput_qps']:.0f} q/s, p50 {res['p50_ms']:.2f} ms, "
-                f"p99 {res['p99_ms']:.2f} ms")
+                f"p99 {res['p99_ms']:.2f} ms; p99 / single by class: "
+                + ", ".join(f"{k} {v['p99_over_single']:.1f}" for k, v in res["per_class"].items()))
         sat = drive(srv, draw(3000, 99), 0)
         sat["requests"] = 3000
         results[name + "_saturated"] = sat

@@ -457,16 +457,18 @@ void tdbg_batch_destroy(tdbg_batch* batch);
  * keep one unit of work on the GPU (own stream), so units overlap.  A unit is
  * either one large request (many rows, or ORDER BY/LIMIT/OFFSET) run alone,
  * or the small requests queued at that moment (up to max_batch, FIFO) run as
- * one batched persistent launch: continuous batching.  Requests are admitted
- * in arrival order while the estimated device bytes of the running units
- * (group tables, partition scratch, results) stay within admit_bytes. */
+ * one batched persistent launch: continuous batching.  With two lanes or
+ * more, lane 0 runs small requests only, so large ones never hold them back.
+ * Requests are admitted in arrival order while the estimated device bytes of
+ * the running units (group tables, partition scratch, results) stay within
+ * admit_bytes. */
 typedef struct tdbg_server tdbg_server;
 typedef struct tdbg_ticket tdbg_ticket;
 typedef struct tdbg_server_config {
   uint32_t lanes;        /* units on the device at once (0 = 3) */
   uint32_t max_batch;    /* requests per batched launch (0 = 64) */
   uint64_t admit_bytes;  /* estimated device bytes of running units (0 = a quarter of the budget) */
-  uint64_t large_rows;   /* requests scanning this many rows run alone (0 = 16M) */
+  uint64_t large_rows;   /* requests scanning this many rows run alone (0 = 4M) */
 } tdbg_server_config;
 typedef struct tdbg_ticket_report {
   int64_t submit_ns;     /* steady-clock times of submission, dispatch and completion */
@@ -491,6 +493,10 @@ int32_t tdbg_server_start(tdbg_context* ctx, const tdbg_server_config* cfg, tdbg
 /* Queues one execution of `plan` over every staged row group of its table.
  * The plan must outlive the ticket's completion. */
 int32_t tdbg_server_submit(tdbg_server* server, tdbg_plan* plan, tdbg_ticket** out);
+/* The same over the row groups `rgs` names (a query's snapshot, as the
+ * operator passes them to tdbg_execute; n_rgs 0 scans nothing), copied. */
+int32_t tdbg_server_submit_rgs(tdbg_server* server, tdbg_plan* plan, const tdbg_rg_ref* rgs, uint32_t n_rgs,
+                               tdbg_ticket** out);
 /* TDBG_PENDING while queued or running, else the execution's status. */
 int32_t tdbg_ticket_poll(tdbg_ticket* ticket);
 /* Blocks until done; on TDBG_OK hands over the result (caller frees it). */

@@ -122,3 +122,59 @@ def test_admission_by_device_bytes(session, oracle_lib):
         srv.stop()
         for p in plans:
             p.close()
+
+
+def test_requests_over_named_row_groups(session, oracle_lib):
+    """tdbg_server_submit_rgs: each request scans the row groups it names (a
+    query's snapshot), batched or alone, and equals the plan executed over
+    the same row groups and the oracle over those rows; an empty list scans
+    nothing."""
+    from thindb_amd.exec import Table
+    items, plans = [], []
+    for t in range(12):  # small tables of 8K-row row groups: batched, over a subset of them
+        base = gen.batched_table(23, t, 60_000)
+        table = Table(base.schema, base.columns, base.validity, rg_rows=8192)
+        items.append((table, gen.batched_request(23, t)))
+    big = gen.table("filtered", 3_000_000, seed=10)
+    where = And((Predicate("ts", "gte", gen.FILTER_LO), Predicate("ts", "lt", gen.FILTER_HI)))
+    items.append((big, Request(["region"], [AggSpec("count", None, "c"), AggSpec("sum", "amount", "s")],
+                               [SortSpec("s", True)], 50, where_filter=where)))
+    for table, req in items:
+        plan, cols = build_plan(session, table, req)
+        session.stage_table(table, cols)
+        plans.append(plan)
+    picked = []
+    for (table, req), p in zip(items, plans):
+        rgs = table.row_groups
+        keep = [rg for i, rg in enumerate(rgs) if i % 3 != 1] or rgs[:1]
+        picked.append(keep)
+    want = []
+    for (table, req), p, keep in zip(items, plans, picked):
+        cols = p.execute([(rg.segment_id, rg.idx) for rg in keep])
+        want.append(rows_of(Batch(output_schema(table, req), cols, len(cols[0].values) if cols else 0)))
+    srv = gpu.Server(session.ctx, lanes=3, max_batch=16)
+    try:
+        tickets = [srv.submit(p, [(rg.segment_id, rg.idx) for rg in keep]) for p, keep in zip(plans, picked)]
+        got = []
+        sizes = []
+        for (table, req), t in zip(items, tickets):
+            cols = t.wait()
+            got.append(rows_of(Batch(output_schema(table, req), cols, len(cols[0].values) if cols else 0)))
+            sizes.append(t.report().batch_size)
+            t.close()
+        compare(items, got, want)
+        assert max(sizes) > 1, "requests over named row groups share launches"
+        # the oracle over exactly the picked rows
+        for (table, req), g, keep in zip(items, got, picked):
+            idx = np.concatenate([np.arange(rg.start, rg.start + rg.n_rows) for rg in keep])
+            sub = Table(table.schema, {k: v[idx] for k, v in table.columns.items()},
+                        {k: v[idx] for k, v in table.validity.items()}, rg_rows=table.rg_rows)
+            check_with_oracle(g, oracle_lib, sub, req, 8, REL)
+        t = srv.submit(plans[0], [])
+        cols = t.wait()
+        t.close()
+        assert all(len(c.values) == 0 for c in cols)
+    finally:
+        srv.stop()
+        for p in plans:
+            p.close()

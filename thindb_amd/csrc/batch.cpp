@@ -64,6 +64,9 @@ PlanHint batch_hint(PlanHint h) {
 struct tdbg_batch {
   tdbg_context* ctx = nullptr;
   std::vector<tdbg_plan*> plans;
+  // per plan, the row groups it scans (empty: every staged row group; a
+  // request of the serving loop names its snapshot's)
+  std::vector<std::vector<tdbg_rg_ref>> plan_rgs;
   std::vector<PlanHint> forced;  // autotuner trial: this hint for every plan
   bool use_graph = true;
   hipStream_t stream = nullptr;
@@ -179,7 +182,9 @@ int32_t build(tdbg_batch* b) {
     Table* t = tit->second.get();
     b->versions.emplace_back(pl->table_uid, t->version);
     ScanSet& S = sets[i];
-    int32_t rc = collect_scan(pl, t, nullptr, 0, S);
+    const bool named = i < b->plan_rgs.size() && !b->plan_rgs[i].empty();
+    int32_t rc = collect_scan(pl, t, named ? b->plan_rgs[i].data() : nullptr,
+                              named ? (uint32_t)b->plan_rgs[i].size() : 0u, S);
     if (rc) return rc;
     PlanHint h;
     // THINDB_GPU_BATCH_STRATEGY=direct|global|hash|radix forces one strategy
@@ -637,9 +642,14 @@ int32_t batch_open(tdbg_context* ctx, tdbg_batch** out) {
   return rc;
 }
 
-int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n) {
+int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n,
+                        const std::vector<tdbg_rg_ref>* const* rgs) {
   std::lock_guard<std::mutex> lk(b->mu);
   b->plans.assign(plans, plans + n);
+  b->plan_rgs.assign(n, {});
+  if (rgs)
+    for (uint32_t i = 0; i < n; i++)
+      if (rgs[i]) b->plan_rgs[i] = *rgs[i];
   release_built(b);
   return run(b);
 }

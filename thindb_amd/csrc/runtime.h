@@ -323,7 +323,9 @@ int32_t noalloc_setup(int device);
 // A reusable batch for the serving loop (batch.cpp): eager launches, device
 // and pinned buffers kept and grown across runs of different plan sets.
 int32_t batch_open(tdbg_context* ctx, tdbg_batch** out);
-int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n);
+// rgs (optional): per plan, the row groups it scans (null or empty: all staged)
+int32_t batch_run_plans(tdbg_batch* b, tdbg_plan* const* plans, uint32_t n,
+                        const std::vector<tdbg_rg_ref>* const* rgs = nullptr);
 int32_t batch_take_result(tdbg_batch* b, uint32_t i, tdbg_result** out);  // views pooled pinned memory
 void batch_close(tdbg_batch* b);
 

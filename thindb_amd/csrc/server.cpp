@@ -6,13 +6,23 @@
 // Lane loop: take the queue head; if it is large (many rows, or ORDER BY /
 // LIMIT / OFFSET) run it alone through the single-query operator, else take
 // every small request queued behind it (up to max_batch, stopping at a large
-// one) and run them as one batch.  A unit is admitted only while the
-// estimated bytes of the running units stay within admit_bytes (a unit always
-// runs when nothing else does, so an oversized request cannot starve).
+// one) and run them as one batch.  With two lanes or more, lane 0 takes small
+// requests only (the first ones queued, skipping large ones), so a run of
+// large requests on the other lanes never holds small ones back.  A unit is
+// admitted only while the estimated bytes of the running units stay within
+// admit_bytes (a unit always runs when nothing else does, so an oversized
+// request cannot starve).
+//
+// Hand-offs spin briefly before they sleep: an idle lane polls the queue and
+// a waiting caller polls its ticket for a few tens of microseconds, about
+// what a futex wake-up costs, so an unloaded server adds little to a
+// request's latency.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <thread>
+
+#include <immintrin.h>
 
 #include "runtime.h"
 
@@ -26,16 +36,32 @@ int64_t now_ns() {
       .count();
 }
 
+constexpr int64_t kSpinNs = 40000;  // polling before a hand-off sleeps
+
+// Polls `ready` for up to kSpinNs; true when it held.
+template <typename F>
+bool spin_until(F ready) {
+  const int64_t end = now_ns() + kSpinNs;
+  do {
+    if (ready()) return true;
+    for (int i = 0; i < 32; i++) _mm_pause();
+  } while (now_ns() < end);
+  return ready();
+}
+
 }  // namespace
 
 struct tdbg_ticket {
   tdbg_plan* plan = nullptr;
+  bool all_rgs = true;            // scans every staged row group, else the ones in `rgs`
+  std::vector<tdbg_rg_ref> rgs;
   uint64_t cancel_gen = 0;  // the plan's cancel generation at submission
   uint64_t est = 0;
   bool large = false;
   std::mutex m;
   std::condition_variable cv;
   bool done = false;
+  std::atomic<bool> done_flag{false};  // `done`, readable without the lock (spin-waits)
   int32_t rc = TDBG_PENDING;
   std::string err;
   tdbg_result* result = nullptr;
@@ -50,6 +76,7 @@ struct tdbg_server {
   std::mutex mu;
   std::condition_variable cv;
   std::deque<tdbg_ticket*> queue;
+  std::atomic<uint32_t> queued{0};  // queue.size(), readable without the lock (spin-waits)
   uint64_t admitted = 0;
   bool stopping = false;
   std::vector<std::thread> lanes;
@@ -76,9 +103,17 @@ void finish(tdbg_ticket* t, int32_t rc, tdbg_result* r, uint32_t batch_size, uin
     t->lane = lane;
     t->t_done = now_ns();
     t->done = true;
+    t->done_flag.store(true, std::memory_order_release);
   }
   t->cv.notify_all();
   unref(t);
+}
+
+// Runs one request alone through the single-query operator.
+int32_t run_alone(tdbg_ticket* t, tdbg_result** r) {
+  static const tdbg_rg_ref none{};  // an empty list scans nothing
+  const tdbg_rg_ref* rgs = t->all_rgs ? nullptr : (t->rgs.empty() ? &none : t->rgs.data());
+  return execute_plan(t->plan, rgs, (uint32_t)t->rgs.size(), true, r, t->cancel_gen);
 }
 
 // Estimated device bytes of one execution: the group table, radix partition
@@ -114,13 +149,21 @@ void lane_main(tdbg_server* s, uint32_t lane) {
   (void)hipSetDevice(s->ctx->device);
   std::vector<tdbg_ticket*> unit;
   std::vector<tdbg_plan*> plans;
+  std::vector<const std::vector<tdbg_rg_ref>*> rgs;
+  const bool small_only = lane == 0 && s->cfg.lanes >= 2;
   for (;;) {
     uint64_t est = 0;
+    spin_until([&] { return s->queued.load(std::memory_order_relaxed) != 0; });
     {
       std::unique_lock<std::mutex> lk(s->mu);
       for (;;) {
-        if (!s->queue.empty()) {
-          tdbg_ticket* head = s->queue.front();
+        // the first request this lane may take: the head, or for the
+        // small-request lane the first small one
+        auto first = s->queue.begin();
+        if (small_only)
+          while (first != s->queue.end() && (*first)->large) ++first;
+        if (first != s->queue.end()) {
+          tdbg_ticket* head = *first;
           if (s->admitted != 0 && s->admitted + head->est > s->cfg.admit_bytes) {
             s->stats.admission_waits++;
             s->cv.wait(lk);  // a running unit frees memory
@@ -130,16 +173,22 @@ void lane_main(tdbg_server* s, uint32_t lane) {
           if (head->large) {
             unit.push_back(head);
             est = head->est;
-            s->queue.pop_front();
+            s->queue.erase(first);
           } else {
-            while (!s->queue.empty() && unit.size() < s->cfg.max_batch) {
-              tdbg_ticket* t = s->queue.front();
-              if (t->large || (!unit.empty() && s->admitted + est + t->est > s->cfg.admit_bytes)) break;
+            auto it = first;
+            while (it != s->queue.end() && unit.size() < s->cfg.max_batch) {
+              tdbg_ticket* t = *it;
+              if (t->large) {
+                if (small_only) { ++it; continue; }
+                break;
+              }
+              if (!unit.empty() && s->admitted + est + t->est > s->cfg.admit_bytes) break;
               unit.push_back(t);
               est += t->est;
-              s->queue.pop_front();
+              it = s->queue.erase(it);
             }
           }
+          s->queued.store((uint32_t)s->queue.size(), std::memory_order_relaxed);
           s->admitted += est;
           s->stats.peak_admitted = std::max(s->stats.peak_admitted, s->admitted);
           break;
@@ -163,19 +212,23 @@ void lane_main(tdbg_server* s, uint32_t lane) {
     }
     if (live.size() == 1) {
       tdbg_result* r = nullptr;
-      const int32_t rc = execute_plan(live[0]->plan, nullptr, 0, true, &r, live[0]->cancel_gen);
+      const int32_t rc = run_alone(live[0], &r);
       finish(live[0], rc, rc == TDBG_OK ? r : nullptr, 1, lane);
     } else if (!live.empty()) {
       plans.clear();
-      for (tdbg_ticket* t : live) plans.push_back(t->plan);
-      int32_t rc = batch_run_plans(s->batches[lane], plans.data(), (uint32_t)plans.size());
+      rgs.clear();
+      for (tdbg_ticket* t : live) {
+        plans.push_back(t->plan);
+        rgs.push_back(&t->rgs);
+      }
+      int32_t rc = batch_run_plans(s->batches[lane], plans.data(), (uint32_t)plans.size(), rgs.data());
       for (size_t i = 0; i < live.size(); i++) {
         tdbg_result* r = nullptr;
         int32_t ri = rc;
         if (rc == TDBG_OK) ri = batch_take_result(s->batches[lane], (uint32_t)i, &r);
         // a plan a batch cannot hold (a key domain only hash partitions
         // cover): each request runs alone
-        if (rc == TDBG_ERR_UNSUPPORTED) ri = execute_plan(live[i]->plan, nullptr, 0, true, &r, live[i]->cancel_gen);
+        if (rc == TDBG_ERR_UNSUPPORTED) ri = run_alone(live[i], &r);
         if (ri == TDBG_OK && cancelled(live[i]->plan, live[i]->cancel_gen)) {  // cancelled while it ran
           tdbg_result_free(r);
           r = nullptr;
@@ -208,7 +261,7 @@ int32_t tdbg_server_start(tdbg_context* ctx, const tdbg_server_config* cfg, tdbg
   if (!s->cfg.lanes) s->cfg.lanes = 3;
   if (!s->cfg.max_batch) s->cfg.max_batch = 64;
   if (!s->cfg.admit_bytes) s->cfg.admit_bytes = ctx->budget / 4;
-  if (!s->cfg.large_rows) s->cfg.large_rows = 16u << 20;
+  if (!s->cfg.large_rows) s->cfg.large_rows = 4u << 20;
   if (s->cfg.lanes > 16) return fail(TDBG_ERR_INVALID, "%u lanes (at most 16)", s->cfg.lanes);
   for (uint32_t i = 0; i < s->cfg.lanes; i++) {
     tdbg_batch* b = nullptr;
@@ -225,16 +278,24 @@ int32_t tdbg_server_start(tdbg_context* ctx, const tdbg_server_config* cfg, tdbg
 }
 
 int32_t tdbg_server_submit(tdbg_server* s, tdbg_plan* plan, tdbg_ticket** out) {
-  if (!s || !plan || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  return tdbg_server_submit_rgs(s, plan, nullptr, 0, out);
+}
+
+int32_t tdbg_server_submit_rgs(tdbg_server* s, tdbg_plan* plan, const tdbg_rg_ref* rgs, uint32_t n_rgs,
+                               tdbg_ticket** out) {
+  if (!s || !plan || !out || (n_rgs && !rgs)) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
   if (plan->ctx != s->ctx) return fail(TDBG_ERR_INVALID, "plan of another context");
   auto* t = new tdbg_ticket();
   t->plan = plan;
+  t->all_rgs = rgs == nullptr;
+  if (rgs) t->rgs.assign(rgs, rgs + n_rgs);
   t->cancel_gen = plan->cancel_gen.load();  // a tdbg_cancel from here on reaches this request
   t->t_submit = now_ns();
   uint64_t rows = 0;
   t->est = estimate(s->ctx, plan, &rows);
-  t->large = rows >= s->cfg.large_rows || !plan_batchable(plan);
+  // (an empty row-group list runs alone: a batch reads an empty list as every row group)
+  t->large = rows >= s->cfg.large_rows || !plan_batchable(plan) || (!t->all_rgs && t->rgs.empty());
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stopping) {
@@ -242,9 +303,12 @@ int32_t tdbg_server_submit(tdbg_server* s, tdbg_plan* plan, tdbg_ticket** out) {
       return fail(TDBG_ERR_INVALID, "server is stopping");
     }
     s->queue.push_back(t);
+    s->queued.store((uint32_t)s->queue.size(), std::memory_order_relaxed);
     s->stats.submitted++;
   }
-  s->cv.notify_one();
+  // every lane: one woken alone could be the small-request lane, which
+  // leaves a large request to the others
+  s->cv.notify_all();
   *out = t;
   return TDBG_OK;
 }
@@ -260,6 +324,7 @@ int32_t tdbg_ticket_poll(tdbg_ticket* t) {
 int32_t tdbg_ticket_wait(tdbg_ticket* t, tdbg_result** out) {
   if (!t || !out) return fail(TDBG_ERR_INVALID, "null argument");
   *out = nullptr;
+  spin_until([&] { return t->done_flag.load(std::memory_order_acquire); });
   std::unique_lock<std::mutex> lk(t->m);
   t->cv.wait(lk, [&] { return t->done; });
   if (t->rc != TDBG_OK) return fail(t->rc, "%s", t->err.c_str());

@@ -707,6 +707,7 @@ class ServerCounters(C.Structure):
 EXPORTS.update({
     "tdbg_server_start": (C.c_int32, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     "tdbg_server_submit": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "tdbg_server_submit_rgs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "tdbg_ticket_poll": (C.c_int32, [C.c_void_p]),
     "tdbg_ticket_wait": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "tdbg_ticket_info": (C.c_int32, [C.c_void_p, C.POINTER(TicketReport)]),
@@ -765,9 +766,17 @@ class Server:
         check(lib.tdbg_server_start(ctx.handle, C.byref(cfg), C.byref(h)))
         self._h = h
 
-    def submit(self, plan: Plan) -> Ticket:
+    def submit(self, plan: Plan, rgs=None) -> Ticket:
+        """Queues one execution of `plan` over `rgs` (None = every staged row
+        group, else a list of (segment_id, row_group_idx) or a prepared
+        :meth:`Plan.rg_array`)."""
         t = C.c_void_p()
-        check(self._lib.tdbg_server_submit(self._h, plan.handle, C.byref(t)))
+        if rgs is None:
+            check(self._lib.tdbg_server_submit(self._h, plan.handle, C.byref(t)))
+        else:
+            arr, n = rgs if isinstance(rgs, tuple) and len(rgs) == 2 and isinstance(rgs[1], int) \
+                and not isinstance(rgs[0], int) else Plan.rg_array(rgs)
+            check(self._lib.tdbg_server_submit_rgs(self._h, plan.handle, C.cast(arr, C.c_void_p), n, C.byref(t)))
         return Ticket(self._lib, t)
 
     def counters(self) -> ServerCounters:
