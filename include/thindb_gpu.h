@@ -501,6 +501,11 @@ int32_t tdbg_server_submit_rgs(tdbg_server* server, tdbg_plan* plan, const tdbg_
 int32_t tdbg_ticket_poll(tdbg_ticket* ticket);
 /* Blocks until done; on TDBG_OK hands over the result (caller frees it). */
 int32_t tdbg_ticket_wait(tdbg_ticket* ticket, tdbg_result** out);
+/* The same, giving up after about timeout_us microseconds with TDBG_PENDING
+ * (the ticket stays valid): a connection thread waits in slices and checks
+ * its KILL flag between them (memory.checkCancelled, src/memory.zig:60),
+ * forwarding a KILL with tdbg_cancel. */
+int32_t tdbg_ticket_wait_for(tdbg_ticket* ticket, uint64_t timeout_us, tdbg_result** out);
 int32_t tdbg_ticket_info(const tdbg_ticket* ticket, tdbg_ticket_report* out);
 void tdbg_ticket_free(tdbg_ticket* ticket);
 int32_t tdbg_server_stats(const tdbg_server* server, tdbg_server_counters* out);

@@ -130,3 +130,52 @@ def test_cancel_from_another_thread(session, oracle_lib):
         assert rows(t, req, plan.execute()) == first
     finally:
         plan.close()
+
+
+def test_wait_in_slices_then_cancel(session, oracle_lib):
+    """tdbg_ticket_wait_for: a connection thread waits in slices (PENDING while
+    its request is queued behind a long one), checks for a KILL between
+    slices and forwards it with tdbg_cancel; the request then ends cancelled
+    or whole, never partial."""
+    big = gen.table("highcard", 20_000_000, seed=6)
+    big_req = Request(["user_id", "bucket"], [AggSpec("count", None, "c")], [SortSpec("c", True)], 10)
+    t, r = gen.batched_table(32, 3, 200_000), gen.batched_request(32, 3)
+    plans = []
+    for tt, rr in ((big, big_req), (t, r)):
+        p, cols = build_plan(session, tt, rr)
+        session.stage_table(tt, cols)
+        plans.append(p)
+    srv = gpu.Server(session.ctx, lanes=1, max_batch=1)
+    try:
+        want = rows(t, r, plans[1].execute())
+        check_with_oracle(want, oracle_lib, t, r, 8)
+        for cancel in (False, True):
+            tk_big = srv.submit(plans[0])
+            tk = srv.submit(plans[1])
+            slices, got = 0, None
+            while True:
+                try:
+                    cols = tk.wait_for(200)
+                except gpu.QueryCancelled:
+                    got = "cancelled"
+                    break
+                if cols is not None:
+                    got = rows(t, r, cols)
+                    break
+                slices += 1
+                if cancel and slices == 1:
+                    plans[1].cancel()  # the KILL seen between slices
+            print(f"[errors] wait_for: {slices} pending slices, cancel={cancel} -> "
+                  f"{got if got == 'cancelled' else 'whole result'}", flush=True)
+            assert slices >= 1, "queued behind the long request"
+            if not cancel:
+                assert got == want
+            else:
+                assert got == "cancelled" or got == want
+            tk_big.wait()
+            tk_big.close()
+            tk.close()
+    finally:
+        srv.stop()
+        for p in plans:
+            p.close()

@@ -38,10 +38,10 @@ int64_t now_ns() {
 
 constexpr int64_t kSpinNs = 40000;  // polling before a hand-off sleeps
 
-// Polls `ready` for up to kSpinNs; true when it held.
+// Polls `ready` for up to `ns` (kSpinNs by default); true when it held.
 template <typename F>
-bool spin_until(F ready) {
-  const int64_t end = now_ns() + kSpinNs;
+bool spin_until(F ready, int64_t ns = kSpinNs) {
+  const int64_t end = now_ns() + ns;
   do {
     if (ready()) return true;
     for (int i = 0; i < 32; i++) _mm_pause();
@@ -327,6 +327,20 @@ int32_t tdbg_ticket_wait(tdbg_ticket* t, tdbg_result** out) {
   spin_until([&] { return t->done_flag.load(std::memory_order_acquire); });
   std::unique_lock<std::mutex> lk(t->m);
   t->cv.wait(lk, [&] { return t->done; });
+  if (t->rc != TDBG_OK) return fail(t->rc, "%s", t->err.c_str());
+  *out = t->result;
+  t->result = nullptr;
+  return TDBG_OK;
+}
+
+int32_t tdbg_ticket_wait_for(tdbg_ticket* t, uint64_t timeout_us, tdbg_result** out) {
+  if (!t || !out) return fail(TDBG_ERR_INVALID, "null argument");
+  *out = nullptr;
+  const int64_t budget = (int64_t)std::min<uint64_t>(timeout_us, 1ull << 40) * 1000;
+  spin_until([&] { return t->done_flag.load(std::memory_order_acquire); }, std::min<int64_t>(budget, kSpinNs));
+  std::unique_lock<std::mutex> lk(t->m);
+  if (!t->cv.wait_for(lk, std::chrono::nanoseconds(std::max<int64_t>(0, budget - kSpinNs)), [&] { return t->done; }))
+    return TDBG_PENDING;
   if (t->rc != TDBG_OK) return fail(t->rc, "%s", t->err.c_str());
   *out = t->result;
   t->result = nullptr;

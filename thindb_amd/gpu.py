@@ -710,6 +710,7 @@ EXPORTS.update({
     "tdbg_server_submit_rgs": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "tdbg_ticket_poll": (C.c_int32, [C.c_void_p]),
     "tdbg_ticket_wait": (C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "tdbg_ticket_wait_for": (C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "tdbg_ticket_info": (C.c_int32, [C.c_void_p, C.POINTER(TicketReport)]),
     "tdbg_ticket_free": (None, [C.c_void_p]),
     "tdbg_server_stats": (C.c_int32, [C.c_void_p, C.POINTER(ServerCounters)]),
@@ -735,6 +736,16 @@ class Ticket:
         """Blocks until the request has run; returns its result columns."""
         out = C.c_void_p()
         check(self._lib.tdbg_ticket_wait(self._h, C.byref(out)))
+        return _result_columns(self._lib, out)
+
+    def wait_for(self, timeout_us: int):
+        """Waits up to about `timeout_us`; the result columns, or None while
+        the request is still queued or running."""
+        out = C.c_void_p()
+        rc = self._lib.tdbg_ticket_wait_for(self._h, timeout_us, C.byref(out))
+        if rc == Status.PENDING:
+            return None
+        check(rc)
         return _result_columns(self._lib, out)
 
     def report(self) -> TicketReport:

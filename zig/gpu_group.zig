@@ -33,6 +33,7 @@ const exec = @import("exec.zig");
 const aggregate = @import("aggregate.zig");
 const Scan = @import("scan.zig").Scan;
 const v2_pipeline = @import("v2_pipeline.zig");
+const gpu_serving = @import("gpu_serving.zig");
 const PredicateExpr = exec.PredicateExpr;
 
 const c = @cImport(@cInclude("thindb_gpu.h"));
@@ -353,8 +354,15 @@ pub const GpuGroup = struct {
         }
         try exec.memory.checkCancelled(self.allocator);
         var res: ?*c.tdbg_result = null;
-        const rc = c.tdbg_execute(self.plan, rgs.items.ptr, @intCast(rgs.items.len), &res);
-        if (rc != c.TDBG_OK) return mapStatus(rc);
+        // through the process-wide device loop (gpu_serving.zig: continuous
+        // batching, admission), else directly
+        if (gpu_serving.enabled()) {
+            res = try gpu_serving.execute(self.allocator, g_ctx.?, self.plan, rgs.items, mapStatus);
+        }
+        if (res == null) {
+            const rc = c.tdbg_execute(self.plan, rgs.items.ptr, @intCast(rgs.items.len), &res);
+            if (rc != c.TDBG_OK) return mapStatus(rc);
+        }
         defer c.tdbg_result_free(res);
         try self.copyResult(res.?);
         self.emitted = true;
