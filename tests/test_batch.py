@@ -147,3 +147,29 @@ def test_waves_same_results(session, oracle_lib, monkeypatch):
     finally:
         run.close()
     check(oracle_lib, pairs[::8], res[::8])
+
+
+@pytest.mark.slow
+def test_batched_full_size_properties(session):
+    """The batched config at full size: the 512 uneven tables of the bench
+    (1.29G rows) in one batch; per plan, sum of COUNT(*) = the rows passing
+    its filter, sum of SUM(value) = their exact total, the group count = the
+    distinct keys among them, MIN/MAX = the extremes."""
+    seed = 7
+    pairs = [(gen.batched_table(seed, t), gen.batched_request(seed, t)) for t in range(512)]
+    res, routes, st = run_batch(session, pairs)
+    print(f"[gpu] full-size batch: {st.rows} rows, {st.n_items} items, strategies "
+          f"{sorted(set(r.strategy for r in routes))}", flush=True)
+    for i, ((t, r), b) in enumerate(zip(pairs, res)):
+        if i % 64 == 0:
+            print(f"  checking plan {i}", flush=True)
+        key, val = t.columns["key"], t.columns["value"]
+        if r.where_filter is not None:
+            m = (val >= gen.BATCH_FILTER_LO) & (val < gen.BATCH_FILTER_HI)
+            key, val = key[m], val[m]
+        c, s_, mn, mx = (b.values[i].values for i in range(1, 5))
+        assert b.row_count == np.unique(key).size
+        assert int(c.sum()) == key.size
+        assert int(s_.sum()) == int(val.sum())
+        if key.size:
+            assert int(mn.min()) == int(val.min()) and int(mx.max()) == int(val.max())

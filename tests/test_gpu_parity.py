@@ -464,6 +464,41 @@ def test_filtered_full_size_checksums(session):
     session.evict_table(t)
 
 
+@pytest.mark.slow
+def test_highcard_full_size_properties(session):
+    """The high-cardinality config at full size (200M rows, ~149M groups),
+    through radix partitioning: sum of COUNT(*) = rows, sum of SUM(v) = the
+    exact total, the group count = the distinct (user_id, bucket) pairs, and
+    each of the COUNT(*) top-10 recounted from the rows."""
+    n = 200_000_000
+    t = gen.table("highcard", n, seed=3)
+    uid, bucket, v = t.columns["user_id"], t.columns["bucket"], t.columns["v"]
+    pair = uid * 16 + bucket.astype(np.int64)  # user_id < 2e7, bucket < 16
+    _, sizes = np.unique(pair, return_counts=True)
+    n_pairs, best = int(sizes.size), int(sizes.max())
+    req = Request(["user_id", "bucket"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s")],
+                  est_groups=150_000_000)
+    b, st = run_gpu(session, t, req)
+    assert st.global_mode == 3
+    c, sm = b.values[2].values, b.values[3].values
+    print(f"  200M highcard: {b.row_count} groups, {n_pairs} distinct pairs", flush=True)
+    assert b.row_count == n_pairs == int(st.n_groups)
+    assert int(c.sum()) == n
+    assert int(sm.sum()) == int(v.astype(np.int64).sum())
+    top_req = Request(["user_id", "bucket"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"),
+                                              AggSpec("max", "w", "m")], [SortSpec("c", True)], 10,
+                      est_groups=150_000_000)
+    b, st = run_gpu(session, t, top_req)
+    top = rows_of(b)
+    assert top[0][2] == best  # the largest group
+    for u, k, cnt, s_, m in top:
+        rows = (uid == u) & (bucket == k)
+        assert int(rows.sum()) == cnt and int(v[rows].astype(np.int64).sum()) == s_
+        assert float(t.columns["w"][rows].max()) == m
+    assert int((sizes > top[-1][2]).sum()) <= 10  # no group off the page is larger than its last row
+    session.evict_table(t)
+
+
 # ---- ORDER BY pinned by the reference's pages and the brute force ---------------------------
 
 def test_ranked_emit_pages(session):
