@@ -676,6 +676,8 @@ __device__ __forceinline__ void rx_leaf_range(const RadixParams& R, uint32_t lea
   }
 }
 
+constexpr uint32_t kRxLeafPer = 4;  // rows per thread whose keys a leaf pass loads at once
+
 // Aggregates leaf `leaf` in an LDS direct table and writes its groups: with
 // DENSE_OUT every key of the leaf's range below g_cap at record index key
 // (empty keys get a zero count), else compacted (rx_emit_leaf; a leaf holds
@@ -702,10 +704,21 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
   }
   lds_init(P, lds);
   __syncthreads();
-  for (uint64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
-    const uint8_t* row = R.rows2 + i * stride;
-    const uint32_t key = *reinterpret_cast<const uint32_t*>(row);
-    rx_fold_row(P, R, lds, row, (uint32_t)(key & mask));
+  // the keys of up to kRxLeafPer rows per thread are loaded first, so the
+  // leaf's rows arrive in one memory round trip (their values, in the same
+  // cache lines, are then cache hits)
+  for (uint64_t b = s; b < e; b += (uint64_t)blockDim.x * kRxLeafPer) {  // uniform
+    uint32_t key[kRxLeafPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kRxLeafPer; k++) {
+      const uint64_t i = b + threadIdx.x + (uint64_t)k * blockDim.x;
+      key[k] = i < e ? *reinterpret_cast<const uint32_t*>(R.rows2 + i * stride) : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kRxLeafPer; k++) {
+      const uint64_t i = b + threadIdx.x + (uint64_t)k * blockDim.x;
+      if (i < e) rx_fold_row(P, R, lds, R.rows2 + i * stride, (uint32_t)(key[k] & mask));
+    }
   }
   __syncthreads();
   if (DENSE_OUT) {

@@ -414,26 +414,30 @@ class ResultColumn:
         return bool((self.validity[row >> 3] >> (row & 7)) & 1)
 
 
+def _copy_out(addr, n: int, dtype) -> np.ndarray:
+    """n values of `dtype` copied out of host memory at `addr` (one memmove)."""
+    a = np.empty(n, dtype)
+    if n:
+        C.memmove(a.ctypes.data, addr, a.nbytes)
+    return a
+
+
 def _column_from_view(v: ColumnView) -> ResultColumn:
     t = Type(v.type)
     n = int(v.n_rows)
     if t == Type.DECIMAL128:
-        raw = np.ctypeslib.as_array(C.cast(v.data, C.POINTER(C.c_uint64)), shape=(2 * n,)).copy() if n else \
-            np.zeros(0, np.uint64)
+        raw = _copy_out(v.data, 2 * n, np.uint64)
         lo = raw[0::2]
         hi = raw[1::2].view(np.int64)
         if np.array_equal(hi, lo.view(np.int64) >> 63):  # every value fits 64 bits
             vals = lo.view(np.int64).astype(object)
         else:
             vals = hi.astype(object) * (1 << 64) + lo.astype(object)
-    elif n:
-        ct = np.ctypeslib.as_ctypes_type(NUMPY_DTYPE[t])
-        vals = np.ctypeslib.as_array(C.cast(v.data, C.POINTER(ct)), shape=(n,)).copy()
     else:
-        vals = np.zeros(0, NUMPY_DTYPE[t])
+        vals = _copy_out(v.data, n, NUMPY_DTYPE[t])
     valid = None
     if v.validity:
-        valid = np.ctypeslib.as_array(C.cast(v.validity, C.POINTER(C.c_uint8)), shape=((n + 7) // 8,)).copy()
+        valid = _copy_out(v.validity, (n + 7) // 8, np.uint8)
     return ResultColumn(t, int(v.scale), bool(v.nullable), vals, valid)
 
 
