@@ -168,28 +168,30 @@ def host_threads() -> int:
     return max(1, min(16, n))
 
 
-def _time_oracle(table, request, n: int, threads: int, reps: int = 3):
+def _time_oracle(table, request, n: int, threads: int, reps: int = 3, radix: bool = False):
     from oracle import oracle
     from thindb_amd.exec import Table
     sample = table if n >= table.n_rows else Table(
         table.schema, {k: v[:n] for k, v in table.columns.items()},
         {k: v[:n] for k, v in table.validity.items()}, rg_rows=table.rg_rows, segment_rows=table.segment_rows,
         uid=table.uid)
-    oracle.run(sample, request, threads)  # warm
+    oracle.run(sample, request, threads, radix=radix)  # warm
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
-        oracle.run(sample, request, threads)
+        oracle.run(sample, request, threads, radix=radix)
         times.append(time.perf_counter() - t0)
     return min(n, table.n_rows), sorted(times)[len(times) // 2], min(times)
 
 
-def cpu_baseline(table, request, sample_rows: int, threads: int, dop1_rows: int, kind_note: str) -> dict:
+def cpu_baseline(table, request, sample_rows: int, threads: int, dop1_rows: int, kind_note: str,
+                 radix: bool = False) -> dict:
     """The oracle (thinDB-algorithm CPU restatement) on the first
     `sample_rows` rows of the same table with `threads` workers, and on the
-    first `dop1_rows` rows with one."""
-    n, med, best = _time_oracle(table, request, sample_rows, threads)
-    n1, med1, best1 = _time_oracle(table, request, dop1_rows, 1)
+    first `dop1_rows` rows with one (radix: the silo's high-cardinality
+    arrangement of it)."""
+    n, med, best = _time_oracle(table, request, sample_rows, threads, radix=radix)
+    n1, med1, best1 = _time_oracle(table, request, dop1_rows, 1, radix=radix)
     return {"value": n / med, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": f"first {n} rows of the same table, median of 3 runs after 1 warm-up "
                       f"(best {n / best:.4g} rows/s); {kind_note}",
@@ -654,11 +656,15 @@ def run_single(args, workload: str, sess, steps: int, warmup: int, table=None):
     if not args.no_cpu:
         threads = host_threads()
         big = workload in ("filtered", "highcard", "wide")
-        sample = args.cpu_sample_rows or min(rows, {"filtered": 40_000_000, "wide": 4_000_000}.get(workload, 10_000_000))
+        hc = workload in ("highcard", "wide") and not distinct
+        sample = args.cpu_sample_rows or min(rows, {"filtered": 40_000_000, "wide": 20_000_000,
+                                                    "highcard": 20_000_000}.get(workload, 10_000_000))
         cpu = cpu_baseline(table, req, sample, threads,
-                           min(sample, 2_000_000 if workload == "wide" else 4_000_000 if big else rows),
-                           ORACLE_NOTE + ("; the silo's 2-level radix is not restated: this is the hash-table "
-                                          "fold over the same groups" if workload in ("highcard", "wide") else ""))
+                           min(sample, 4_000_000 if big else rows),
+                           (ORACLE_NOTE + "; for this high-cardinality query in the silo's arrangement "
+                            "(oracle_group_by_radix: rows routed by key hash into 256 buckets, each split into "
+                            "parts of a few thousand rows folded into tables of their own, silo_group_core.zig)")
+                           if hc else ORACLE_NOTE, radix=hc)
         log(f"cpu baseline {cpu['value']:.4g} rows/s on {threads} cores, {cpu['dop1']['value']:.4g} on one")
 
     # HBM bytes per launch of the dominant kernel, from the committed

@@ -10,6 +10,11 @@
  *     (declines: >8 keys, >16 aggs, non-integer keys, temporal SUM/AVG,
  *     decimal128 inputs) and the silo's <=128-bit key packing,
  *     src/exec/v2_shape_group_topn.zig:2051-2094;
+ *   - the silo's high-cardinality arrangement, for the CPU baseline only
+ *     (oracle_group_by_radix, end of file): routeHashKey bucket routing, the
+ *     staged bucket rows and the deferred parts each folded into a table of
+ *     its own, src/exec/silo_group_core.zig:5218-5235, :9228-9400,
+ *     :10799-10960 - the same result as oracle_group_by, tested equal;
  *   - key packing: packIntPart, v2_lowcard_group.zig:886-948
  *     (`(bits(v) & mask(width)) << offset`, widths from keyTypeBits :184-192);
  *   - the worker model: tiles of TILE_RGS=16 row groups claimed from a shared
@@ -162,6 +167,7 @@ typedef struct {
   int direct;
   uint64_t direct_base, direct_mask;
   uint32_t direct_span;
+  uint32_t dop;  /* workers (emit's top-k selection splits its groups over them) */
 } cplan;
 
 /* ---- per-worker state ----------------------------------------------------- */
@@ -376,7 +382,6 @@ static int row_passes(const cplan* p, uint64_t r) {
 /* ---- one sub-batch: foldBatch (v2_lowcard_group.zig:977-1050) --------------- */
 static void fold_batch(wstate* w, const cplan* p, uint64_t lo, uint64_t hi) {
   const or_table* t = p->t;
-  const tdbg_plan_desc* d = p->d;
   uint64_t n = 0;
   for (uint64_t r = lo; r < hi; r++)
     if (row_passes(p, r)) w->sel[n++] = r;
@@ -780,23 +785,94 @@ static void put_int(void* data, uint32_t type, uint64_t row, i128 v) {
   }
 }
 
-static int emit(const cplan* p, const wstate* m, or_result** out) {
+/* The best k gids of v[0..n) under gid_cmp, kept in place at v[0..kept)
+ * (a heap whose top is the worst kept). */
+typedef struct {
+  uint32_t* v;
+  uint64_t n, k, kept;
+  const sort_ctx* ctx;  /* the comparator's context (thread-local: set in each worker) */
+} topk_arg;
+
+static void* topk_main(void* arg) {
+  topk_arg* a = arg;
+  g_sort = a->ctx;
+  uint32_t* h = a->v;
+  uint64_t n = 0;
+  for (uint64_t i = 0; i < a->n; i++) {
+    uint32_t g = a->v[i];
+    if (n < a->k) {
+      uint64_t c = n++;
+      h[c] = g;
+      while (c > 0 && gid_cmp(&h[(c - 1) / 2], &h[c]) < 0) {
+        uint32_t x = h[c]; h[c] = h[(c - 1) / 2]; h[(c - 1) / 2] = x;
+        c = (c - 1) / 2;
+      }
+    } else if (gid_cmp(&g, &h[0]) < 0) {
+      h[0] = g;
+      uint64_t c = 0;
+      for (;;) {
+        uint64_t l = 2 * c + 1, r = l + 1, big = c;
+        if (l < n && gid_cmp(&h[l], &h[big]) > 0) big = l;
+        if (r < n && gid_cmp(&h[r], &h[big]) > 0) big = r;
+        if (big == c) break;
+        uint32_t x = h[c]; h[c] = h[big]; h[big] = x;
+        c = big;
+      }
+    }
+  }
+  a->kept = n;
+  return NULL;
+}
+
+/* The groups `live` (n_live gids; null: every gid below m->n_groups) of m,
+ * HAVING-filtered, ordered, sliced and written as result columns. */
+static int emit_live(const cplan* p, const wstate* m, const uint32_t* live, uint64_t n_live, or_result** out) {
   const tdbg_plan_desc* d = p->d;
-  uint64_t ng = 0;
-  uint32_t* order = malloc((m->n_groups + 1) * sizeof(uint32_t));
+  uint64_t ng = 0, ng_sorted = ~0ull;
+  if (!live) n_live = m->n_groups;
+  uint32_t* order = malloc((n_live + 1) * sizeof(uint32_t));
   if (!order) return -1;
-  for (uint64_t i = 0; i < m->n_groups; i++) {
+  for (uint64_t j = 0; j < n_live; j++) {
+    const uint32_t i = live ? live[j] : (uint32_t)j;
     int32_t first = 0;
-    if (d->n_having == 0 || having_eval(p, m, (uint32_t)i, (int32_t)d->n_having - 1, &first)) order[ng++] = (uint32_t)i;
+    if (d->n_having == 0 || having_eval(p, m, i, (int32_t)d->n_having - 1, &first)) order[ng++] = i;
   }
   if (d->n_order > 0) {
     sort_ctx ctx = {p, m};
     g_sort = &ctx;
-    qsort(order, ng, sizeof(uint32_t), gid_cmp);
+    const uint64_t k = d->limit ? d->offset + d->limit : 0;
+    if (k && k < ng / 4) {
+      /* only the first offset + limit groups of the order matter (the
+       * reference's TopSet/RankedSet, v2_shape_group_topn.zig:846-870): each
+       * of the workers keeps the best k of its share in a heap whose top is
+       * the worst kept, and the kept ones are sorted; the comparator is a
+       * total order, so they hold the full sort's first k */
+      uint32_t nt = p->dop > 1 && ng > (1u << 20) ? p->dop : 1;
+      if (nt > 64) nt = 64;
+      topk_arg ta[64];
+      pthread_t th[64];
+      for (uint32_t i = 0; i < nt; i++) {
+        ta[i] = (topk_arg){order + ng * i / nt, ng * (i + 1) / nt - ng * i / nt, k, 0, &ctx};
+        if (nt > 1) pthread_create(&th[i], NULL, topk_main, &ta[i]);
+        else topk_main(&ta[i]);
+      }
+      uint64_t n = 0;
+      for (uint32_t i = 0; i < nt; i++) {
+        if (nt > 1) pthread_join(th[i], NULL);
+        memmove(order + n, ta[i].v, ta[i].kept * sizeof(uint32_t));  /* shares are in order: no overlap */
+        n += ta[i].kept;
+      }
+      qsort(order, n, sizeof(uint32_t), gid_cmp);
+      if (n > k) n = k;
+      ng_sorted = n;
+    } else {
+      qsort(order, ng, sizeof(uint32_t), gid_cmp);
+    }
     g_sort = NULL;
   }
   uint64_t start = d->offset < ng ? d->offset : ng;
   uint64_t end = d->limit == 0 ? ng : (start + d->limit < ng ? start + d->limit : ng);
+  if (end > ng_sorted) end = ng_sorted;  /* (a top-k selection kept exactly these) */
   uint64_t nr = end - start;
   or_result* r = calloc(1, sizeof *r);
   if (!r) { free(order); return -1; }
@@ -871,6 +947,8 @@ static int emit(const cplan* p, const wstate* m, or_result** out) {
   *out = r;
   return 0;
 }
+
+static int emit(const cplan* p, const wstate* m, or_result** out) { return emit_live(p, m, NULL, 0, out); }
 
 /* ---- COUNT(DISTINCT): the (group, value) composites of the filtered rows
  * with a non-NULL value, deduplicated, counted per group (the distinct sets
@@ -1118,6 +1196,7 @@ int32_t oracle_group_by(const tdbg_plan_desc* d, const or_table* t, uint32_t dop
   int rc = compile(&p, d, t);
   if (rc) return rc;
   if (dop == 0) dop = 1;
+  p.dop = dop;
   if (t->rg_rows == 0) { set_err("rg_rows"); return TDBG_ERR_INVALID; }
   uint64_t n_rgs = (t->n_rows + t->rg_rows - 1) / t->rg_rows;
   uint64_t n_tiles = (n_rgs + TILE_RGS - 1) / TILE_RGS;
@@ -1160,6 +1239,300 @@ int32_t oracle_group_by(const tdbg_plan_desc* d, const or_table* t, uint32_t dop
   if (have_merged) ws_free(&merged);
   for (uint32_t i = 0; ws && i < dop; i++) ws_free(&ws[i]);
   free(ws); free(args); free(th);
+  if (fail) { set_err("out of memory"); return TDBG_ERR_OOM; }
+  return 0;
+}
+
+/* ---- the silo's high-cardinality route / bucket fold, restated for the CPU
+ * baseline (silo_group_core.zig): rows are routed by routeHashKey of their
+ * packed key (:5218-5224) into 256 buckets (bucketIndexHash, power of two:
+ * the hash's low bits, :5232-5235), staged per worker with their aggregate
+ * inputs (the stage/scatter of :9228-9400), then each bucket is re-split by
+ * the next hash bits into parts of a few thousand rows (the deferred radix
+ * parts, :10799-10830) and each part folded into a table of its own
+ * (foldDeferredBucket, :10868-10960).  Buckets hold disjoint keys, so their
+ * groups are concatenated, not merged.  The result is the same as
+ * oracle_group_by's (same folds, same emit); only the work is arranged
+ * differently, for timing.  COUNT(DISTINCT) declines here. -------------------- */
+
+#define RX_BUCKETS 256u
+#define RX_PART_ROWS 4096u
+
+typedef struct {
+  u128 key;
+  uint64_t valid; /* bit a: aggregate a's input is non-NULL */
+  uint64_t v[];   /* per aggregate: the input (i64, or f64 bits) */
+} rx_row;
+
+typedef struct {
+  uint8_t* data;
+  uint64_t n, cap;
+} rx_vec;
+
+static inline uint64_t route_hash(u128 key) {
+  uint64_t lo = (uint64_t)key, hi = (uint64_t)(key >> 64);
+  uint64_t x = lo ^ (hi * 0x9e3779b97f4a7c15ull);
+  x ^= x >> 32;
+  return x;
+}
+
+typedef struct {
+  const cplan* p;
+  _Atomic uint64_t* cursor;
+  uint64_t n_tiles, stride;
+  rx_vec* b; /* [RX_BUCKETS] this worker's staged rows per bucket */
+  uint64_t* sel;
+  int err;
+} rx_stage_arg;
+
+static void* rx_stage_main(void* arg) {
+  rx_stage_arg* a = arg;
+  const cplan* p = a->p;
+  const or_table* t = p->t;
+  const uint64_t tile_rows = (uint64_t)t->rg_rows * TILE_RGS;
+  for (;;) {
+    uint64_t tile = atomic_fetch_add(a->cursor, 1);
+    if (tile >= a->n_tiles) break;
+    uint64_t lo = tile * tile_rows, hi = lo + tile_rows < t->n_rows ? lo + tile_rows : t->n_rows;
+    for (uint64_t r = lo; r < hi; r++) {
+      if (!row_passes(p, r)) continue;
+      const u128 key = pack_key(p, r);
+      rx_vec* v = &a->b[route_hash(key) & (RX_BUCKETS - 1)];
+      if (v->n == v->cap) {
+        uint64_t nc = v->cap ? v->cap * 2 : 1024;
+        uint8_t* nd = realloc(v->data, nc * a->stride);
+        if (!nd) { a->err = 1; return NULL; }
+        v->data = nd;
+        v->cap = nc;
+      }
+      rx_row* row = (rx_row*)(v->data + v->n++ * a->stride);
+      row->key = key;
+      row->valid = 0;
+      for (uint32_t g = 0; g < p->n_aggs; g++) {
+        const agg_plan* ag = &p->aggs[g];
+        row->v[g] = 0;
+        if (ag->col < 0) continue;
+        const or_column* c = &t->cols[ag->col];
+        if (!is_valid(c, r)) continue;
+        row->valid |= 1ull << g;
+        if (ag->is_float) { double x = load_f(c, r); memcpy(&row->v[g], &x, 8); }
+        else row->v[g] = (uint64_t)load_i(c, r);
+      }
+    }
+  }
+  return NULL;
+}
+
+/* One row folded into group `gid` of `w` (the fold kernels of fold_batch,
+ * one row at a time: MIN/MAX seeded by the group's first value). */
+static void rx_fold_row(wstate* w, const cplan* p, uint32_t gid, const rx_row* row) {
+  w->counts[gid]++;
+  for (uint32_t a = 0; a < p->n_aggs; a++) {
+    const agg_plan* ag = &p->aggs[a];
+    if (ag->op == OP_COUNT_STAR || !((row->valid >> a) & 1)) continue;
+    size_t o = (size_t)gid * p->n_aggs + a;
+    const int first = w->ns[o] == 0;
+    w->ns[o]++;
+    if (ag->op == OP_COUNT_COL) continue;
+    if (ag->is_float) {
+      double x;
+      memcpy(&x, &row->v[a], 8);
+      if (ag->op == OP_SUM || ag->op == OP_AVG) {
+        w->fslots[o] += x;
+        neumaier_add(&w->csum[o], &w->ccomp[o], x);
+      } else if (first || (ag->op == OP_MIN ? x < w->fslots[o] : x > w->fslots[o])) {
+        w->fslots[o] = x;
+      }
+    } else {
+      const i128 x = (int64_t)row->v[a];
+      if (ag->op == OP_SUM || ag->op == OP_AVG) w->slots[o] += x;
+      else if (first || (ag->op == OP_MIN ? x < w->slots[o] : x > w->slots[o])) w->slots[o] = x;
+    }
+  }
+}
+
+typedef struct {
+  const cplan* p;
+  rx_stage_arg* st; /* the stage workers (their bucket vectors) */
+  uint32_t n_st;
+  uint64_t stride;
+  _Atomic uint32_t* cursor;
+  wstate* all;             /* the query's group arrays, one slot per staged row */
+  const uint64_t* base;    /* [RX_BUCKETS] bucket b's groups start at gid base[b] */
+  uint64_t* n_groups;      /* [RX_BUCKETS] groups bucket b found */
+  int err;
+} rx_fold_arg;
+
+static void* rx_fold_main(void* arg) {
+  rx_fold_arg* a = arg;
+  const cplan* p = a->p;
+  const uint64_t stride = a->stride;
+  uint8_t* tmp = NULL;
+  uint64_t tmp_cap = 0;
+  for (;;) {
+    uint32_t b = atomic_fetch_add(a->cursor, 1);
+    if (b >= RX_BUCKETS) break;
+    uint64_t n = 0;
+    for (uint32_t w = 0; w < a->n_st; w++) n += a->st[w].b[b].n;
+    /* the bucket's groups go to gids base[b].. of the shared arrays (a bucket
+     * has at most as many groups as rows): a view of them */
+    wstate gv;
+    memset(&gv, 0, sizeof gv);
+    wstate* g = &gv;
+    const uint64_t o0 = a->base[b], na = p->n_aggs;
+    g->keys = a->all->keys + o0;
+    g->counts = a->all->counts + o0;
+    g->slots = a->all->slots + o0 * na;
+    g->fslots = a->all->fslots + o0 * na;
+    g->csum = a->all->csum + o0 * na;
+    g->ccomp = a->all->ccomp + o0 * na;
+    g->ns = a->all->ns + o0 * na;
+    g->gcap = n;
+    a->n_groups[b] = 0;
+    if (n == 0) continue;
+    /* the deferred parts: the bucket's rows split by the next hash bits */
+    uint32_t parts = 1;
+    while ((uint64_t)parts * RX_PART_ROWS < n && parts < 4096) parts *= 2;
+    if (tmp_cap < n) {
+      free(tmp);
+      tmp = malloc(n * stride);
+      tmp_cap = tmp ? n : 0;
+      if (!tmp) { a->err = 1; break; }
+    }
+    uint64_t* off = calloc(parts + 1, sizeof(uint64_t));
+    if (!off) { a->err = 1; break; }
+    for (uint32_t w = 0; w < a->n_st; w++) {
+      const rx_vec* v = &a->st[w].b[b];
+      for (uint64_t i = 0; i < v->n; i++)
+        off[((route_hash(((const rx_row*)(v->data + i * stride))->key) >> 8) & (parts - 1)) + 1]++;
+    }
+    for (uint32_t q = 0; q < parts; q++) off[q + 1] += off[q];
+    for (uint32_t w = 0; w < a->n_st; w++) {  /* worker order, row order within a worker */
+      const rx_vec* v = &a->st[w].b[b];
+      for (uint64_t i = 0; i < v->n; i++) {
+        const rx_row* row = (const rx_row*)(v->data + i * stride);
+        const uint32_t q = (route_hash(row->key) >> 8) & (parts - 1);
+        memcpy(tmp + off[q]++ * stride, row, stride);
+      }
+    }
+    /* off[q] now ends part q; fold each part into a fresh table */
+    uint64_t s = 0;
+    for (uint32_t q = 0; q < parts && !a->err; q++) {
+      const uint64_t e = off[q];
+      uint64_t cap = 16;
+      while (cap < 2 * (e - s)) cap *= 2;
+      u128* tk = malloc(cap * sizeof(u128));
+      uint32_t* tg = malloc(cap * sizeof(uint32_t));
+      if (!tk || !tg) { free(tk); free(tg); a->err = 1; break; }
+      for (uint64_t i = 0; i < cap; i++) tg[i] = EMPTY_GID;
+      for (uint64_t i = s; i < e; i++) {
+        const rx_row* row = (const rx_row*)(tmp + i * stride);
+        uint64_t h = hash_key(p, row->key) & (cap - 1);
+        while (tg[h] != EMPTY_GID && tk[h] != row->key) h = (h + 1) & (cap - 1);
+        if (tg[h] == EMPTY_GID) {
+          const uint32_t gid = (uint32_t)g->n_groups++;  /* < n: every new group has a row */
+          g->keys[gid] = row->key;
+          g->counts[gid] = 0;
+          for (uint32_t x = 0; x < p->n_aggs; x++) {
+            size_t o = (size_t)gid * p->n_aggs + x;
+            g->slots[o] = 0; g->fslots[o] = 0; g->csum[o] = 0; g->ccomp[o] = 0; g->ns[o] = 0;
+          }
+          tk[h] = row->key;
+          tg[h] = gid;
+        }
+        rx_fold_row(g, p, tg[h], row);
+      }
+      free(tk);
+      free(tg);
+      s = e;
+    }
+    free(off);
+    a->n_groups[b] = g->n_groups;
+  }
+  free(tmp);
+  return NULL;
+}
+
+int32_t oracle_group_by_radix(const tdbg_plan_desc* d, const or_table* t, uint32_t dop, or_result** out) {
+  cplan p;
+  int rc = compile(&p, d, t);
+  if (rc) return rc;
+  for (uint32_t a = 0; a < p.n_aggs; a++)
+    if (p.aggs[a].op == OP_COUNT_DISTINCT) { set_err("COUNT(DISTINCT) on the radix baseline"); return TDBG_ERR_UNSUPPORTED; }
+  if (p.n_aggs > 60) { set_err("agg count"); return TDBG_ERR_UNSUPPORTED; }
+  if (dop == 0) dop = 1;
+  p.dop = dop;
+  if (t->rg_rows == 0) { set_err("rg_rows"); return TDBG_ERR_INVALID; }
+  const uint64_t stride = sizeof(rx_row) + 8ull * p.n_aggs;
+  const uint64_t n_rgs = (t->n_rows + t->rg_rows - 1) / t->rg_rows;
+  const uint64_t n_tiles = (n_rgs + TILE_RGS - 1) / TILE_RGS;
+  int fail = 0;
+  /* stage: workers route their tiles' rows into per-worker buckets */
+  rx_stage_arg* st = calloc(dop, sizeof *st);
+  pthread_t* th = calloc(dop, sizeof *th);
+  _Atomic uint64_t tcur = 0;
+  if (!st || !th) fail = 1;
+  for (uint32_t w = 0; w < dop && !fail; w++) {
+    st[w] = (rx_stage_arg){&p, &tcur, n_tiles, stride, calloc(RX_BUCKETS, sizeof(rx_vec)), NULL, 0};
+    if (!st[w].b) fail = 1;
+  }
+  if (!fail) {
+    for (uint32_t w = 0; w < dop; w++) pthread_create(&th[w], NULL, rx_stage_main, &st[w]);
+    for (uint32_t w = 0; w < dop; w++) pthread_join(th[w], NULL);
+    for (uint32_t w = 0; w < dop; w++) if (st[w].err) fail = 1;
+  }
+  /* fold: workers take whole buckets, writing their groups into one set of
+   * arrays with a slot per staged row (bucket b from base[b]) */
+  uint64_t base[RX_BUCKETS + 1], ngr[RX_BUCKETS];
+  base[0] = 0;
+  for (uint32_t b = 0; b < RX_BUCKETS; b++) {
+    uint64_t n = 0;
+    for (uint32_t w = 0; !fail && w < dop; w++) n += st[w].b[b].n;
+    base[b + 1] = base[b] + n;
+  }
+  wstate m;
+  memset(&m, 0, sizeof m);
+  const uint64_t slots = base[RX_BUCKETS] + 1, na = p.n_aggs ? p.n_aggs : 1;
+  if (!fail) {
+    m.keys = malloc(slots * sizeof(u128));
+    m.counts = malloc(slots * sizeof(uint64_t));
+    m.slots = malloc(slots * na * sizeof(i128));
+    m.fslots = malloc(slots * na * sizeof(double));
+    m.csum = malloc(slots * na * sizeof(double));
+    m.ccomp = malloc(slots * na * sizeof(double));
+    m.ns = malloc(slots * na * sizeof(uint64_t));
+    if (!m.keys || !m.counts || !m.slots || !m.fslots || !m.csum || !m.ccomp || !m.ns) fail = 1;
+  }
+  rx_fold_arg* fa = fail ? NULL : calloc(dop, sizeof *fa);
+  _Atomic uint32_t bcur = 0;
+  if (!fail && !fa) fail = 1;
+  if (!fail) {
+    for (uint32_t w = 0; w < dop; w++) fa[w] = (rx_fold_arg){&p, st, dop, stride, &bcur, &m, base, ngr, 0};
+    for (uint32_t w = 0; w < dop; w++) pthread_create(&th[w], NULL, rx_fold_main, &fa[w]);
+    for (uint32_t w = 0; w < dop; w++) pthread_join(th[w], NULL);
+    for (uint32_t w = 0; w < dop; w++) if (fa[w].err) fail = 1;
+  }
+  /* the buckets' groups (disjoint keys) as one list of gids, then emitted */
+  if (!fail) {
+    uint64_t total = 0;
+    for (uint32_t b = 0; b < RX_BUCKETS; b++) total += ngr[b];
+    uint32_t* live = malloc((total + 1) * sizeof(uint32_t));
+    if (!live || base[RX_BUCKETS] >= EMPTY_GID) fail = 1;
+    uint64_t at = 0;
+    for (uint32_t b = 0; b < RX_BUCKETS && !fail; b++)
+      for (uint64_t i = 0; i < ngr[b]; i++) live[at++] = (uint32_t)(base[b] + i);
+    m.n_groups = slots;
+    if (!fail && emit_live(&p, &m, live, total, out)) fail = 1;
+    free(live);
+  }
+  free(m.keys); free(m.counts); free(m.slots); free(m.fslots); free(m.csum); free(m.ccomp); free(m.ns);
+  free(fa);
+  for (uint32_t w = 0; st && w < dop; w++) {
+    for (uint32_t b = 0; st[w].b && b < RX_BUCKETS; b++) free(st[w].b[b].data);
+    free(st[w].b);
+  }
+  free(st);
+  free(th);
   if (fail) { set_err("out of memory"); return TDBG_ERR_OOM; }
   return 0;
 }

@@ -43,6 +43,11 @@ typedef struct or_result or_result;
  * deterministic single-worker order.  Returns 0, TDBG_ERR_UNSUPPORTED for
  * shapes the reference's low-card/silo cores decline, or TDBG_ERR_INVALID. */
 int32_t oracle_group_by(const tdbg_plan_desc* plan, const or_table* t, uint32_t dop, or_result** out);
+/* The same result through the silo's high-cardinality arrangement (rows
+ * routed by key hash into buckets, buckets split into parts, each part
+ * folded into its own table): the CPU baseline for high-cardinality
+ * queries.  COUNT(DISTINCT) returns TDBG_ERR_UNSUPPORTED. */
+int32_t oracle_group_by_radix(const tdbg_plan_desc* plan, const or_table* t, uint32_t dop, or_result** out);
 uint64_t oracle_result_num_rows(const or_result* r);
 uint32_t oracle_result_num_columns(const or_result* r);
 int32_t oracle_result_column(const or_result* r, uint32_t i, tdbg_column_view* out);

@@ -106,6 +106,8 @@ def lib():
         L = C.CDLL(str(LIB))
         L.oracle_group_by.argtypes = [C.POINTER(_Desc), C.POINTER(_Table), C.c_uint32, C.POINTER(C.c_void_p)]
         L.oracle_group_by.restype = C.c_int32
+        L.oracle_group_by_radix.argtypes = [C.POINTER(_Desc), C.POINTER(_Table), C.c_uint32, C.POINTER(C.c_void_p)]
+        L.oracle_group_by_radix.restype = C.c_int32
         L.oracle_result_num_rows.argtypes = [C.c_void_p]
         L.oracle_result_num_rows.restype = C.c_uint64
         L.oracle_result_num_groups.argtypes = [C.c_void_p]
@@ -260,8 +262,10 @@ def _out_types(schema, idx, request) -> tuple:
     return types, scales
 
 
-def run(table, request, dop: int = 1) -> OracleResult:
-    """Runs `request` over `table` with the CPU restatement."""
+def run(table, request, dop: int = 1, radix: bool = False) -> OracleResult:
+    """Runs `request` over `table` with the CPU restatement (radix: through
+    the silo's high-cardinality arrangement, oracle_group_by_radix: the same
+    result, the CPU baseline for high-cardinality queries)."""
     L = lib()
     schema = table.schema
     idx = {c.name.lower(): i for i, c in enumerate(schema)}
@@ -331,7 +335,7 @@ def run(table, request, dop: int = 1) -> OracleResult:
                  len(request.order_specs), preds, order, request.limit, request.offset, request.est_groups,
                  len(hl), len(wl), having, where)
     h = C.c_void_p()
-    rc = L.oracle_group_by(C.byref(desc), C.byref(t), dop, C.byref(h))
+    rc = (L.oracle_group_by_radix if radix else L.oracle_group_by)(C.byref(desc), C.byref(t), dop, C.byref(h))
     if rc == -2:
         raise Unsupported(L.oracle_last_error().decode())
     if rc:

@@ -142,3 +142,41 @@ def test_planted_mistakes_are_caught(oracle_lib):
     for bad in (bad_drop, bad_sign, bad_carry):
         with pytest.raises(AssertionError):
             assert_rows_in_groups(bad, want, 1)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_radix_arrangement_equals_the_oracle(oracle_lib, seed):
+    """oracle_group_by_radix (the silo's route / bucket / part arrangement, the
+    CPU baseline for high-cardinality queries) returns the oracle's rows:
+    random typed tables with NULLs and filters, every aggregate kind, one to
+    two keys, ORDER BY with OFFSET/LIMIT (the top-k selection) and HAVING;
+    and a high-cardinality table whose buckets split into parts."""
+    from helpers import float_cols_of, order_cols_of, unsliced
+    rng = np.random.default_rng(300 + seed)
+    t = random_table(rng, 30_000 + 5_000 * seed, rg_rows=1_000 + 137 * seed)
+    aggs = [a for a in ALL_AGGS if a.func != "count_distinct"]
+    reqs = [Request(["k1"], aggs), Request(["k1", "k2"], aggs, [SortSpec(aggs[1].as_, True)], 25, 3),
+            Request(["k2", "k1"], aggs[:5], where_filter=Predicate("i", "gt", 0)),
+            Request(["k1", "k2"], aggs[:4], [SortSpec("k1", True)], 7,
+                    having_filter=Predicate(aggs[0].as_, "gte", 2))]
+    for req in reqs:
+        want = rows_of(oracle_lib.run(t, req, 4))
+        got = rows_of(oracle_lib.run(t, req, 4, radix=True))
+        names = [k for k in req.group_cols] + [a.as_ for a in req.aggs]
+        full = rows_of(oracle_lib.run(t, unsliced(req), 4)) if req.limit or req.offset else None
+        from helpers import assert_same_rows
+        assert_same_rows(got, want, float_cols_of(t, req), 1e-12, len(req.group_cols), order_cols_of(names, req),
+                         full)
+    # many groups: every bucket splits into parts
+    n = 400_000
+    from thindb_amd.exec import Column, Table
+    from thindb_amd.gpu import Type
+    hc = Table([Column("u", Type.BIGINT), Column("b", Type.TINYINT), Column("v", Type.INT)],
+               {"u": rng.integers(0, 150_000, n), "b": rng.integers(0, 4, n).astype(np.int8),
+                "v": rng.integers(-1000, 1000, n).astype(np.int32)})
+    req = Request(["u", "b"], [AggSpec("count", None, "c"), AggSpec("sum", "v", "s"), AggSpec("max", "v", "m")],
+                  [SortSpec("c", True)], 10)
+    assert rows_of(oracle_lib.run(hc, req, 8, radix=True)) == rows_of(oracle_lib.run(hc, req, 8))
+    r = Request(["u", "b"], req.aggs)
+    a = sorted(rows_of(oracle_lib.run(hc, r, 8, radix=True)))
+    assert a == sorted(rows_of(oracle_lib.run(hc, r, 1)))
