@@ -203,10 +203,29 @@ ORACLE_NOTE = ("oracle = C restatement of thinDB's grouped aggregate (per-worker
                "the low-cardinality core's algorithm), not thinDB itself")
 
 
+_PEAK_CODE = r"""
+import json, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import bench
+print(json.dumps(bench._measured_peak_here()), flush=True)
+"""
+
+
 def measured_peak() -> dict:
-    """The device's read and copy bandwidth in this run: torch's int64 sum
-    over 5.6 GB (read) and copy_ of 2.8 GB (read + write), HIP events,
-    median of 10 after 3 warm-ups."""
+    """The device's read and copy bandwidth in this run, measured by torch in
+    a child process: torch bundles its own HIP runtime, which cannot share the
+    device with the one libthindb_gpu.so already initialised in this process."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", _PEAK_CODE, str(ROOT)], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"bandwidth probe failed: {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def _measured_peak_here() -> dict:
+    """torch's int64 sum over 5.6 GB (read) and copy_ of 2.8 GB (read +
+    write), HIP events, median of 10 after 3 warm-ups."""
     import torch
     n = 5_600_000_000 // 8
     x = torch.ones(n, dtype=torch.int64, device="cuda")
