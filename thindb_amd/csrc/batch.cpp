@@ -1,8 +1,9 @@
-// batch.cpp — host side of batched execution (batch.hip): many plans become
-// one queue of work items over one device layout, launched as a short fixed
-// kernel sequence (record init -> persistent scan/aggregate -> compaction ->
-// finalize) that is captured once as a HIP graph and replayed; results come
-// back in one device-to-host copy into pinned memory.
+// batch.hip's host side: many plans become one set of work items over one
+// device layout, launched as a short fixed sequence (counters, record init,
+// the persistent kernel) that is captured once as a HIP graph and replayed.
+// The persistent kernel writes every plan's result rows straight into pinned
+// host memory as the plan finishes; only the header (group counts, error
+// flags) is copied afterwards.
 //
 // The autotuner times the candidate strategies (LDS direct table, HBM
 // atomics, LDS hash table, radix partitions at two leaf sizes) on the batch's
@@ -25,7 +26,7 @@ constexpr uint32_t kTuneSample = 64;           // plans per bucket the autotuner
 constexpr uint64_t kTuneRows = 128ull << 20;   // rows a tuning trial runs (the sample repeated)
 constexpr uint32_t kTuneCopies = 16;           // repeats of the sample, at most
 constexpr uint32_t kRowsPerSlot = 256;         // scan rows per LDS slot flushed, at least, per item
-constexpr uint32_t kWaves = 2;                 // waves of a large batch (result copies overlap later waves)
+constexpr uint32_t kWaves = 1;                 // waves of a large batch (THINDB_GPU_BATCH_WAVES: more)
 constexpr uint32_t kWavePlansMin = 32;         // plans per wave, at least
 
 uint32_t ilog2(uint64_t x) { return x ? 63u - (uint32_t)__builtin_clzll(x) : 0u; }
@@ -79,10 +80,12 @@ struct tdbg_batch {
   std::vector<tdbg_batch_route> info;
   std::vector<std::vector<uint64_t>> data_off, valid_off;
   uint64_t rows = 0, bytes = 0;
-  uint32_t n_items = 0, n_cblocks = 0, n_fblocks = 0, grid = 0, lds_bytes = 0;
-  uint64_t zero_words = 0, out_bytes = 0, err_off = 0, total_off = 0;
-  DevBuf d_q, d_items, d_tiles, d_rec, d_rows, d_misc, d_zero, d_cblocks, d_fblocks, d_counts, d_gids, d_out;
+  uint32_t n_items = 0, n_static = 0, n_cblocks = 0, grid = 0, lds_bytes = 0, emit_cap = 0;
+  uint64_t zero_words = 0, out_bytes = 0, err_off = 0, head_bytes = 0, result_bytes = 0;
+  uint64_t sched_off = 0, slots_off[kBatchQueues] = {0, 0, 0};
+  DevBuf d_q, d_items, d_tiles, d_rec, d_rows, d_misc, d_zero, d_cblocks, d_head, d_run;
   PinnedBuf h_out;                 // results keep the buffer they view
+  uint8_t* h_run = nullptr;        // pinned: the output base the kernel writes through
   uint8_t* h_desc = nullptr;       // pinned staging of the descriptors
   uint64_t h_desc_cap = 0;
   hipGraph_t graph = nullptr;
@@ -125,7 +128,7 @@ void release_built(tdbg_batch* b) {
 void release_buffers(tdbg_batch* b) {
   release_built(b);
   for (DevBuf* d : {&b->d_q, &b->d_items, &b->d_tiles, &b->d_rec, &b->d_rows, &b->d_misc, &b->d_zero, &b->d_cblocks,
-                    &b->d_fblocks, &b->d_counts, &b->d_gids, &b->d_out}) {
+                    &b->d_head, &b->d_run}) {
     dev_free(b->ctx, d->p, d->bytes);
     *d = DevBuf{};
   }
@@ -141,21 +144,14 @@ T* at(const DevBuf& d, uint64_t off) {
 int32_t enqueue(tdbg_batch* b, bool timed) {
   hipStream_t s = b->stream;
   HIP_TRY(launch_fill_u32(static_cast<uint32_t*>(b->d_zero.p), b->zero_words, 0, s), "batch counters");
-  HIP_TRY(launch_fill_u32(at<uint32_t>(b->d_out, b->err_off), 16, 0, s), "batch error flags");
+  HIP_TRY(launch_fill_u32(at<uint32_t>(b->d_head, b->err_off), 16, 0, s), "batch error flags");
   HIP_TRY(launch_batch_init(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchBlock*>(b->d_cblocks.p),
                             b->n_cblocks, s), "batch record init");
   if (timed) HIP_TRY(hipEventRecord(b->ev[1], s), "event");
-  HIP_TRY(launch_batch_scan(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchItem*>(b->d_items.p),
-                            b->n_items, static_cast<uint32_t*>(b->d_zero.p), at<uint32_t>(b->d_out, b->err_off),
-                            kSpinLimit, b->grid, b->lds_bytes, s), "batch scan");
+  HIP_TRY(launch_batch(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchItem*>(b->d_items.p),
+                       static_cast<const BatchArgs*>(b->d_run.p), b->n_items, b->grid, b->lds_bytes, s),
+          "batch kernel");
   if (timed) HIP_TRY(hipEventRecord(b->ev[2], s), "event");
-  uint32_t* counts = static_cast<uint32_t*>(b->d_counts.p);
-  uint64_t* total = at<uint64_t>(b->d_counts, b->total_off);
-  HIP_TRY(launch_batch_compact(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchBlock*>(b->d_cblocks.p),
-                               b->n_cblocks, counts, total, static_cast<uint32_t*>(b->d_gids.p), s), "batch compaction");
-  HIP_TRY(launch_batch_finalize(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchBlock*>(b->d_fblocks.p),
-                                b->n_fblocks, counts, total, b->n_cblocks, static_cast<const uint32_t*>(b->d_gids.p),
-                                static_cast<uint8_t*>(b->d_out.p), s), "batch finalize");
   return TDBG_OK;
 }
 
@@ -225,6 +221,7 @@ int32_t build(tdbg_batch* b) {
     for (size_t w = 0; w < P.tmpl.size(); w++) q.tmpl[w] = P.tmpl[w];
     q.init_records = P.radix ? 0u : 1u;
     if (P.radix) q.rx.tile_rows = std::min<uint32_t>(q.rx.tile_rows, kBatchThreads * kVec);
+    if (q.sp.g_cap >= (1ull << 32)) return fail(TDBG_ERR_UNSUPPORTED, "plan %u: more group records than a batch indexes", i);
     gmax[i] = std::max<uint64_t>(1, std::min<uint64_t>(q.sp.g_cap, S.rows));
     tdbg_batch_route& inf = b->info[i];
     inf.strategy = P.radix ? 4u : (q.sp.lds_mode == LDS_DIRECT ? 1u : (q.sp.lds_mode == LDS_HASH ? 3u : 2u));
@@ -236,6 +233,11 @@ int32_t build(tdbg_batch* b) {
     b->bytes += S.bytes;
   }
   if (lds_need > kBatchLds) return fail(TDBG_ERR_DEVICE, "batch LDS layout of %u bytes", lds_need);
+  // emit items stage a slab of records (kBatchThreads of them) in LDS when it fits
+  for (uint32_t i = 0; i < n; i++) {
+    const uint64_t stage = kEmitStageOff + (uint64_t)kBatchThreads * Q[i].sp.g_rec;
+    if (stage <= kBatchLds) lds_need = std::max<uint32_t>(lds_need, (uint32_t)stage);
+  }
   b->lds_bytes = lds_need;
   // device layout
   uint64_t rec_b = 0, tile_b = 0, rows_b = 0, misc_b = 0;
@@ -256,13 +258,15 @@ int32_t build(tdbg_batch* b) {
       misc_b += align_up(((2ull << q.rx.bits1) + 2) * 8, 256) + align_up(n_hist * (4ull << q.rx.bits1), 256);
     }
   }
-  // zeroed counters: queue head, per-plan sync words
-  const uint64_t zero_b = 64 + align_up(16ull * n, 256);
-  // output buffer: group counts, error flags, then each plan's columns
+  // output: group counts and error flags (device header, copied to the
+  // host after the kernel), then each plan's columns (host memory, written
+  // by the kernel's emit items)
   b->err_off = align_up(8ull * n, 64);
-  uint64_t out_b = align_up(b->err_off + 64, 256);
+  b->head_bytes = b->err_off + 64;
+  uint64_t out_b = align_up(b->head_bytes, 256);
   b->data_off.assign(n, {});
   b->valid_off.assign(n, {});
+  b->result_bytes = 0;
   for (uint32_t i = 0; i < n; i++) {
     FinalParams& fp = Q[i].fp;
     out_b = align_up(out_b, 256);
@@ -283,28 +287,43 @@ int32_t build(tdbg_batch* b) {
     }
   }
   b->out_bytes = out_b;
-  // compaction and finalize blocks, work items
-  std::vector<BatchBlock> cblocks, fblocks;
-  std::vector<BatchItem> hist_items, scan_items, scatter_items, agg_items;
-  for (uint32_t i = 0; i < n; i++) {
+  // record-init blocks and work items: histograms and scans are ready at
+  // launch (the static queue); each plan's scatters, leaves and emit items
+  // follow it in the item array, appended to the ready queues on the device
+  // when the stage before completes
+  std::vector<BatchBlock> cblocks;
+  std::vector<BatchItem> hist_items, scan_items, dyn_items;
+  std::vector<uint32_t> order(n);
+  for (uint32_t i = 0; i < n; i++) order[i] = i;
+  // radix plans with the most records first: their rows take longest to
+  // reach the host, so their emit items should start early
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return Q[x].sp.g_cap > Q[y].sp.g_cap; });
+  const uint32_t base_chunk = (uint32_t)std::max<uint64_t>(1, kItemRows / kTileRows);
+  for (uint32_t i : order) {
     BatchQuery& q = Q[i];
-    q.blk0 = (uint32_t)cblocks.size();
-    for (uint64_t r = 0; r < q.sp.g_cap; r += kBatchChunk) cblocks.push_back(BatchBlock{i, 0, r});
-    q.blk1 = (uint32_t)cblocks.size();
-    for (uint64_t r = 0; r < gmax[i]; r += kBatchChunk) fblocks.push_back(BatchBlock{i, 0, r});
+    if (q.init_records)
+      for (uint64_t r = 0; r < q.sp.g_cap; r += kBatchChunk) cblocks.push_back(BatchBlock{i, 0, r});
     const uint32_t n_tiles = (uint32_t)sets[i].tiles.size();
-    const uint32_t base_chunk = (uint32_t)std::max<uint64_t>(1, kItemRows / kTileRows);
-    const uint32_t chunk = base_chunk;
-    const uint32_t before = (uint32_t)(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
+    uint32_t n_items_plan = 0;
+    q.n_hist = q.n_scatter = q.n_comp = 0;
     if (b->info[i].strategy == 4u) {
-      for (uint32_t t0 = 0; t0 < n_tiles; t0 += chunk) {
-        const uint32_t t1 = std::min(n_tiles, t0 + chunk);
-        hist_items.push_back(BatchItem{BI_RX_HIST, i, t0, t1, t0 / chunk, 0});  // part: the item's index
-        scatter_items.push_back(BatchItem{BI_RX_SCATTER, i, t0, t1, t0 / chunk, 0});
+      q.sc0 = (uint32_t)dyn_items.size();
+      for (uint32_t t0 = 0, k = 0; t0 < n_tiles; t0 += base_chunk, k++) {
+        const uint32_t t1 = std::min(n_tiles, t0 + base_chunk);
+        hist_items.push_back(BatchItem{BI_RX_HIST, i, t0, t1, k, 0});  // part: the item's index
+        dyn_items.push_back(BatchItem{BI_RX_SCATTER, i, t0, t1, k, 0});
+        q.n_hist++;
+        q.n_scatter++;
       }
+      q.ag0 = (uint32_t)dyn_items.size();
       const uint64_t per = std::max<uint64_t>(1, (uint64_t)q.rx.n_leaves * kItemRows / std::max<uint64_t>(sets[i].rows, 1));
-      for (uint64_t l0 = 0; l0 < q.rx.n_leaves; l0 += per)
-        agg_items.push_back(BatchItem{BI_RX_AGG, i, (uint32_t)l0, (uint32_t)std::min<uint64_t>(q.rx.n_leaves, l0 + per), 0, 0});
+      for (uint64_t l0 = 0; l0 < q.rx.n_leaves; l0 += per) {
+        dyn_items.push_back(BatchItem{BI_RX_AGG, i, (uint32_t)l0, (uint32_t)std::min<uint64_t>(q.rx.n_leaves, l0 + per), 0, 0});
+        q.n_comp++;
+      }
+      if (q.n_hist == 0 || q.n_comp == 0)
+        return fail(TDBG_ERR_DEVICE, "plan %u: radix partitioning without tiles or leaves", i);
+      n_items_plan = q.n_hist + q.n_scatter + q.n_comp;
     } else {
       const uint32_t parts = q.sp.lds_mode == LDS_DIRECT ? q.sp.lds_parts : 1u;
       // an LDS table item ends with a flush of every slot to HBM (about five
@@ -315,58 +334,90 @@ int32_t build(tdbg_batch* b) {
         scan_chunk = (uint32_t)std::max<uint64_t>(scan_chunk,
                                                   ((uint64_t)kRowsPerSlot * q.sp.lds_cap + kTileRows - 1) / kTileRows);
       for (uint32_t t0 = 0; t0 < n_tiles; t0 += scan_chunk)
-        for (uint32_t p = 0; p < parts; p++)
+        for (uint32_t p = 0; p < parts; p++) {
           scan_items.push_back(BatchItem{BI_SCAN, i, t0, std::min(n_tiles, t0 + scan_chunk), p, 0});
+          q.n_comp++;
+        }
+      if (q.n_comp == 0) {  // nothing to scan: one empty item releases the emit items
+        scan_items.push_back(BatchItem{BI_SCAN, i, 0, 0, 0, 0});
+        q.n_comp = 1;
+      }
+      n_items_plan = q.n_comp;
     }
-    const uint32_t after = (uint32_t)(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
-    b->info[i].items = after - before;
+    q.em0 = (uint32_t)dyn_items.size();
+    q.n_emit = 0;
+    for (uint64_t r = 0; r < q.sp.g_cap; r += kBatchChunk, q.n_emit++)
+      dyn_items.push_back(BatchItem{BI_EMIT, i, (uint32_t)r, (uint32_t)std::min<uint64_t>(q.sp.g_cap, r + kBatchChunk),
+                                    q.n_emit, 0});
+    if (q.n_emit == 0) return fail(TDBG_ERR_DEVICE, "plan %u: no group records", i);
+    b->info[i].items = n_items_plan + q.n_emit;
   }
-  // per-plan radix item counts
-  for (uint32_t i = 0; i < n; i++) {
-    Q[i].n_hist = 0;
-    Q[i].n_scatter = 0;
-  }
-  for (auto& it : hist_items) Q[it.q].n_hist++;
-  for (auto& it : scatter_items) Q[it.q].n_scatter++;
-  // the largest scans first, so the tail of the queue is short items
+  // the largest scans first, so the tail of the static queue is short items
   std::stable_sort(scan_items.begin(), scan_items.end(), [](const BatchItem& x, const BatchItem& y) {
     return (x.a1 - x.a0) > (y.a1 - y.a0);
   });
   std::vector<BatchItem> items;
-  items.reserve(hist_items.size() + scan_items.size() + scatter_items.size() + agg_items.size());
-  for (auto* v : {&hist_items, &scan_items, &scatter_items, &agg_items}) items.insert(items.end(), v->begin(), v->end());
+  items.reserve(hist_items.size() + scan_items.size() + dyn_items.size());
+  items.insert(items.end(), hist_items.begin(), hist_items.end());
+  items.insert(items.end(), scan_items.begin(), scan_items.end());
+  b->n_static = (uint32_t)items.size();
+  items.insert(items.end(), dyn_items.begin(), dyn_items.end());
+  uint32_t per_queue[kBatchQueues] = {0, 0, 0};
+  uint64_t n_emit_all = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    Q[i].sc0 += b->n_static;
+    Q[i].ag0 += b->n_static;
+    Q[i].em0 += b->n_static;
+    per_queue[BQ_SCATTER] += Q[i].n_scatter;
+    per_queue[BQ_AGG] += b->info[i].strategy == 4u ? Q[i].n_comp : 0u;
+    per_queue[BQ_EMIT] += Q[i].n_emit;
+    n_emit_all += Q[i].n_emit;
+  }
   b->n_items = (uint32_t)items.size();
   b->n_cblocks = (uint32_t)cblocks.size();
-  b->n_fblocks = (uint32_t)fblocks.size();
   const uint32_t n_cu = (uint32_t)std::max(ctx->n_cu, 8);
   b->grid = std::max<uint32_t>(1, std::min<uint32_t>(n_cu * 2, b->n_items));
+  const char* ecap = getenv("THINDB_GPU_BATCH_EMITTERS");
+  b->emit_cap = ecap && ecap[0] ? (uint32_t)std::max(1, atoi(ecap)) : std::max<uint32_t>(1, b->grid / 8);
+  // zeroed per execution: queue state, per-plan stage counters, ready-queue
+  // slots, emit item counts
+  uint64_t zero_b = 0;
+  b->sched_off = zero_b;
+  zero_b += align_up(sizeof(BatchSched), 64);
+  const uint64_t sync_off = zero_b;
+  zero_b += align_up(16ull * n, 64);
+  for (uint32_t k = 0; k < kBatchQueues; k++) {
+    b->slots_off[k] = zero_b;
+    zero_b += align_up(4ull * per_queue[k], 64);
+  }
+  const uint64_t status_off = zero_b;
+  zero_b += align_up(8ull * n_emit_all, 64);
+  zero_b = align_up(zero_b, 256);
   // allocations
   int32_t rc;
-  uint64_t gids_b = 0;
-  for (uint32_t i = 0; i < n; i++) gids_b += gmax[i] * 4;
-  b->total_off = align_up((uint64_t)b->n_cblocks * 4, 8);
   if ((rc = dev_take(b, b->d_q, sizeof(BatchQuery) * n)) || (rc = dev_take(b, b->d_items, sizeof(BatchItem) * items.size())) ||
       (rc = dev_take(b, b->d_tiles, tile_b)) || (rc = dev_take(b, b->d_rec, rec_b)) || (rc = dev_take(b, b->d_rows, rows_b)) ||
       (rc = dev_take(b, b->d_misc, misc_b)) || (rc = dev_take(b, b->d_zero, zero_b)) ||
-      (rc = dev_take(b, b->d_cblocks, sizeof(BatchBlock) * cblocks.size())) ||
-      (rc = dev_take(b, b->d_fblocks, sizeof(BatchBlock) * fblocks.size())) ||
-      (rc = dev_take(b, b->d_counts, b->total_off + 64)) || (rc = dev_take(b, b->d_gids, gids_b + 256)) ||
-      (rc = dev_take(b, b->d_out, b->out_bytes))) {
+      (rc = dev_take(b, b->d_cblocks, sizeof(BatchBlock) * std::max<size_t>(1, cblocks.size()))) ||
+      (rc = dev_take(b, b->d_head, b->head_bytes)) || (rc = dev_take(b, b->d_run, 256))) {
     release_buffers(b);
     return rc;
   }
   b->zero_words = zero_b / 4;
   // device pointers into the layout
+  uint64_t status_at = status_off;
   for (uint32_t i = 0; i < n; i++) {
     BatchQuery& q = Q[i];
     q.sp.g_state = at<uint8_t>(b->d_rec, rec_o[i]);
     q.sp.tiles = at<const Tile>(b->d_tiles, tile_o[i]);
     q.sp.n_tiles = (uint32_t)sets[i].tiles.size();
-    q.sp.err = at<uint32_t>(b->d_out, b->err_off);
+    q.sp.err = at<uint32_t>(b->d_head, b->err_off);
     q.op.g_state = q.sp.g_state;
     q.fp.g_state = q.sp.g_state;
-    q.sync = at<uint32_t>(b->d_zero, 64 + 16ull * i);
-    q.n_out = at<uint64_t>(b->d_out, 8ull * i);
+    q.sync = at<uint32_t>(b->d_zero, sync_off + 16ull * i);
+    q.emit_status = at<uint64_t>(b->d_zero, status_at);
+    status_at += 8ull * q.n_emit;
+    q.n_out = at<uint64_t>(b->d_head, 8ull * i);
     if (b->info[i].strategy == 4u) {
       const uint64_t nb = 1ull << q.rx.bits1;
       q.item_hist = at<uint32_t>(b->d_misc, misc_o[i] + align_up(((2ull << q.rx.bits1) + 2) * 8, 256));
@@ -381,8 +432,8 @@ int32_t build(tdbg_batch* b) {
   // descriptors: staged back to back in pinned memory, copied asynchronously
   // on the batch's stream ahead of the kernels that read them
   const uint64_t q_b = sizeof(BatchQuery) * n, it_b = sizeof(BatchItem) * items.size();
-  const uint64_t cb_b = sizeof(BatchBlock) * cblocks.size(), fb_b = sizeof(BatchBlock) * fblocks.size();
-  const uint64_t desc_b = align_up(tile_b, 256) + align_up(q_b, 256) + align_up(it_b, 256) + align_up(cb_b, 256) + fb_b;
+  const uint64_t cb_b = sizeof(BatchBlock) * cblocks.size();
+  const uint64_t desc_b = align_up(tile_b, 256) + align_up(q_b, 256) + align_up(it_b, 256) + cb_b;
   if (b->h_desc_cap < desc_b) {
     if (b->h_desc) (void)hipHostFree(b->h_desc);
     b->h_desc = nullptr;
@@ -405,11 +456,9 @@ int32_t build(tdbg_batch* b) {
   HIP_TRY(put(b->d_items, items.data(), it_b, o), "batch items");
   o += align_up(it_b, 256);
   HIP_TRY(put(b->d_cblocks, cblocks.data(), cb_b, o), "batch blocks");
-  o += align_up(cb_b, 256);
-  HIP_TRY(put(b->d_fblocks, fblocks.data(), fb_b, o), "batch blocks");
   b->built = true;
   // kernel attributes are set outside any capture
-  HIP_TRY(launch_batch_scan(nullptr, nullptr, 0, nullptr, nullptr, kSpinLimit, 1, 0, b->stream), "batch kernel setup");
+  HIP_TRY(launch_batch(nullptr, nullptr, nullptr, 0, 1, 0, b->stream), "batch kernel setup");
   const char* genv = getenv("THINDB_GPU_GRAPH");
   if (b->use_graph && !(genv && genv[0] == '0')) {
     HIP_TRY(hipStreamSynchronize(b->stream), "descriptors");  // the capture starts on an idle stream
@@ -452,6 +501,7 @@ int32_t prepare_run(tdbg_batch* b) {
     b->h_out = PinnedBuf{};
     if ((rc = pinned_acquire(b->ctx->pinned, b->out_bytes, b->h_out))) return rc;
   }
+  if (!b->h_run) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&b->h_run), 256, hipHostMallocDefault), "pinned run block");
   return TDBG_OK;
 }
 
@@ -461,7 +511,27 @@ int32_t launch(tdbg_batch* b, hipStream_t cs) {
   int32_t rc;
   hipStream_t s = b->stream;
   if (stale(b)) return fail(TDBG_ERR_NOT_STAGED, "batch: a member table changed while the batch was planned");
+  // the kernel's arguments, with the device's address of the pinned output
+  // buffer it writes result rows through, go to d_run before every launch
+  // (the graph keeps only d_run's address)
+  void* dev_out = nullptr;
+  HIP_TRY(hipHostGetDevicePointer(&dev_out, b->h_out.p.get(), 0), "pinned output address");
+  BatchArgs a{};
+  a.Q = static_cast<const BatchQuery*>(b->d_q.p);
+  a.items = static_cast<const BatchItem*>(b->d_items.p);
+  a.n_static = b->n_static;
+  a.n_items = b->n_items;
+  a.sched = at<BatchSched>(b->d_zero, b->sched_off);
+  for (uint32_t k = 0; k < kBatchQueues; k++) a.slots[k] = at<uint32_t>(b->d_zero, b->slots_off[k]);
+  a.err = at<uint32_t>(b->d_head, b->err_off);
+  a.out = static_cast<uint8_t*>(dev_out);
+  a.spin_limit = kSpinLimit;
+  a.emit_cap = b->emit_cap;
+  a.lds_bytes = b->lds_bytes;
+  static_assert(sizeof(BatchArgs) <= 256, "d_run holds the kernel arguments");
+  memcpy(b->h_run, &a, sizeof a);
   HIP_TRY(hipEventRecord(b->ev[0], s), "event");
+  HIP_TRY(hipMemcpyAsync(b->d_run.p, b->h_run, sizeof a, hipMemcpyHostToDevice, s), "batch arguments");
   if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, s), "graph launch");
   else if ((rc = enqueue(b, true))) return rc;
   HIP_TRY(hipEventRecord(b->ev_kernels, s), "event");
@@ -469,7 +539,7 @@ int32_t launch(tdbg_batch* b, hipStream_t cs) {
     HIP_TRY(hipEventRecord(b->kernels_done, s), "event");
     HIP_TRY(hipStreamWaitEvent(cs, b->kernels_done, 0), "copy stream wait");
   }
-  HIP_TRY(hipMemcpyAsync(b->h_out.p.get(), b->d_out.p, b->out_bytes, hipMemcpyDeviceToHost, cs), "result copy");
+  HIP_TRY(hipMemcpyAsync(b->h_out.p.get(), b->d_head.p, b->head_bytes, hipMemcpyDeviceToHost, cs), "result header");
   HIP_TRY(hipEventRecord(b->ev[3], cs), "event");
   return TDBG_OK;
 }
@@ -624,6 +694,7 @@ void free_batch(tdbg_batch* b) {
   if (b->ev_kernels) (void)hipEventDestroy(b->ev_kernels);
   release_buffers(b);
   if (b->h_desc) (void)hipHostFree(b->h_desc);
+  if (b->h_run) (void)hipHostFree(b->h_run);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->own_stream) (void)hipStreamDestroy(b->stream);

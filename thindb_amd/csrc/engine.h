@@ -344,16 +344,22 @@ struct DistinctParams {
 // ---- batched execution (one persistent launch over many plans) ------------
 
 // A work item of the persistent kernel: tiles [a0, a1) of plan q for a scan
-// or a radix partition pass, or leaves [a0, a1) for a radix leaf pass.
-enum BatchItemKind : uint32_t { BI_SCAN = 0, BI_RX_HIST, BI_RX_SCATTER, BI_RX_AGG };
+// or a radix partition pass, leaves [a0, a1) for a radix leaf pass, or
+// records [a0, a1) whose groups an emit item writes to the host.
+enum BatchItemKind : uint32_t { BI_SCAN = 0, BI_RX_HIST, BI_RX_SCATTER, BI_RX_AGG, BI_EMIT };
 struct BatchItem {
   uint32_t kind;
   uint32_t q;
   uint32_t a0, a1;
   uint32_t part;   // BI_SCAN over a partitioned direct table: the key range owned;
-                   // BI_RX_HIST / BI_RX_SCATTER: the item's index among the plan's
+                   // BI_RX_HIST / BI_RX_SCATTER / BI_EMIT: the item's index among the plan's
   uint32_t pad;
 };
+
+// Ready queues the persistent kernel fills as plans finish a stage (the
+// items of the next stage are appended by the workgroup that completed the
+// stage's last item).  Deeper stages are taken first.
+enum BatchQueue : uint32_t { BQ_SCATTER = 0, BQ_AGG, BQ_EMIT, kBatchQueues };
 
 constexpr int kTmplWords = 64;  // record template words (records up to 512 B)
 struct BatchQuery {
@@ -362,19 +368,51 @@ struct BatchQuery {
   OrderParams op;
   FinalParams fp;                // data/valid offsets are absolute in the output buffer
   uint64_t tmpl[kTmplWords];     // one initialised record
-  uint32_t blk0, blk1;           // compaction blocks [blk0, blk1) over this plan's records
   uint32_t n_hist, n_scatter;    // radix partition items of this plan
+  uint32_t n_comp;               // items whose completion releases the emit items (scans, or leaves)
   uint32_t init_records;         // records start from tmpl (else the leaf pass writes them all)
+  uint32_t sc0, ag0, em0;        // first scatter, leaf and emit item (indices into the item array)
+  uint32_t n_emit;               // emit items (records in kBatchChunk blocks)
   uint32_t n_having;             // HAVING nodes at `having` (0 = none)
+  uint32_t pad;
   const HavingNode* having;
-  uint32_t* sync;                // [0] histogram items done, [1] bucket starts published,
-                                 // [2] scatter items done
+  uint32_t* sync;                // [0] histogram items done, [1] scatter items done, [2] n_comp items done
   uint32_t* item_hist;           // radix: [n_hist][2^bits1] per-item digit counts, then offsets
+  uint64_t* emit_status;         // per emit item: kEmitReady | groups it writes, once counted
   uint64_t* n_out;               // groups emitted (output buffer header)
 };
+constexpr uint64_t kEmitReady = 1ull << 63;
 
-// kBatchChunk consecutive records (compaction, record init) or output rows
-// (finalize) of plan q starting at r0.
+// The persistent kernel's queue state (zeroed before every launch): every
+// word on its own 128-byte line, so claims, appends and polls of different
+// words do not queue behind each other.
+struct alignas(128) BatchWord {
+  uint32_t v;
+  uint32_t pad[31];
+};
+struct BatchSched {
+  BatchWord head0;               // static queue: items [0, n_static) taken in order
+  BatchWord claimed;             // items taken, all queues
+  BatchWord emitters;            // emit items running
+  BatchWord head[kBatchQueues];  // ready queues: items taken
+  BatchWord tail[kBatchQueues];  // ready queues: items appended (slots reserved)
+};
+
+struct BatchArgs {
+  const BatchQuery* Q;
+  const BatchItem* items;
+  uint32_t n_static;             // items [0, n_static) are ready at launch (histograms, scans)
+  uint32_t n_items;
+  BatchSched* sched;
+  uint32_t* slots[kBatchQueues]; // per ready queue: item index + 1 per slot (0 = being written)
+  uint32_t* err;
+  uint8_t* out;                  // the output buffer's base (host memory mapped for the device)
+  uint32_t spin_limit;
+  uint32_t emit_cap;             // emit items running at once beyond which others are preferred
+  uint32_t lds_bytes;            // dynamic LDS per workgroup (emit items stage records in it)
+};
+
+// kBatchChunk consecutive records of plan q starting at r0 (record init).
 constexpr uint32_t kBatchChunk = 4096;
 struct BatchBlock {
   uint32_t q;
@@ -383,6 +421,8 @@ struct BatchBlock {
 };
 
 constexpr uint32_t kBatchThreads = 512;
+constexpr uint32_t kEmitPer = kBatchChunk / kBatchThreads;  // records per thread of an emit item
+constexpr uint32_t kEmitStageOff = 1024;  // LDS of an emit item: counters and masks, then staged records
 // LDS of the radix scatter: digit tables (counts, offsets, cursors, scan
 // temporaries, bucket bases for up to 1024 digits) then the staged rows.
 constexpr uint32_t kRxStageOff = 24 * 1024;
@@ -478,13 +518,10 @@ hipError_t launch_radix_gids(const RadixParams& r, uint32_t* gids, uint64_t* n_o
 // kernel over `items` (grid workgroups of kBatchThreads, lds_bytes dynamic
 // LDS each; *head must start at 0), compaction and finalize.
 hipError_t launch_batch_init(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks, hipStream_t s);
-hipError_t launch_batch_scan(const BatchQuery* q, const BatchItem* items, uint32_t n_items, uint32_t* head,
-                             uint32_t* err, uint32_t spin_limit, uint32_t grid, uint32_t lds_bytes, hipStream_t s);
-hipError_t launch_batch_compact(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks, uint32_t* counts,
-                                uint64_t* total, uint32_t* gids, hipStream_t s);
-hipError_t launch_batch_finalize(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks,
-                                 const uint32_t* offs, const uint64_t* total, uint32_t n_blocks_compact,
-                                 const uint32_t* gids, uint8_t* out, hipStream_t s);
+// `args` lives in device memory (written before every launch); the kernel
+// reads its fields where they are used instead of holding them in registers.
+hipError_t launch_batch(const BatchQuery* Q, const BatchItem* items, const BatchArgs* args, uint32_t n_items,
+                        uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 #endif
 
 }  // namespace tdbg

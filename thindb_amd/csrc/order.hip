@@ -28,10 +28,6 @@ constexpr uint32_t kCompactThreads = 256;
 constexpr uint32_t kCompactPer = 16;
 constexpr uint32_t kCompactChunk = kCompactThreads * kCompactPer;
 
-__device__ __forceinline__ bool occupied(const OrderParams& P, uint64_t i) {
-  return *reinterpret_cast<const uint64_t*>(P.g_state + i * P.g_rec + P.g_count_off) != 0;
-}
-
 __global__ void __launch_bounds__(kCompactThreads) compact_count_kernel(const OrderParams P, uint64_t n_rec,
                                                                         uint32_t* block_counts) {
   __shared__ uint32_t red[kCompactThreads / 64];
@@ -1218,90 +1214,7 @@ static hipError_t sort_general(const OrderParams& p, const uint32_t* gids, uint6
   return hipGetLastError();
 }
 
-// ---- finalize: typed result columns ------------------------------------------------
-
-__device__ __forceinline__ void put(uint8_t* out, uint64_t off, uint32_t bytes, uint64_t row, uint64_t v) {
-  uint8_t* p = out + off + row * bytes;
-  switch (bytes) {
-    case 1: *p = (uint8_t)v; break;
-    case 2: *reinterpret_cast<uint16_t*>(p) = (uint16_t)v; break;
-    case 4: *reinterpret_cast<uint32_t*>(p) = (uint32_t)v; break;
-    default: *reinterpret_cast<uint64_t*>(p) = v; break;
-  }
-}
-
-// Output row i from record gid: every output column at its offset in `out`.
-__device__ __forceinline__ void finalize_row(const FinalParams& P, uint32_t gid, uint64_t i, uint8_t* out) {
-  {
-    const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
-    const uint64_t count = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off);
-    for (uint32_t c = 0; c < P.n_outs; c++) {
-      const OutDesc& o = P.outs[c];
-      uint8_t valid = 1;
-      switch (o.kind) {
-        case O_KEY: {
-          const GKey k = group_key(rec, gid, P.g_mode, P.g_key_off);
-          if (part_null(P.parts[o.idx], k)) {
-            valid = 0;
-            put(out, o.data_off, o.bytes, i, 0);
-          } else {
-            put(out, o.data_off, o.bytes, i, (uint64_t)part_value(P.parts[o.idx], k));
-          }
-          break;
-        }
-        case O_COUNT: put(out, o.data_off, 8, i, count); break;
-        case O_NS: put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
-        default: {
-          const uint64_t ns = o.ns_off != ~0u ? *reinterpret_cast<const uint64_t*>(rec + o.ns_off) : count;
-          if (ns == 0) {
-            valid = 0;
-            if (o.bytes == 16) {
-              put(out, o.data_off, 8, 2 * i, 0);
-              put(out, o.data_off, 8, 2 * i + 1, 0);
-            } else {
-              put(out, o.data_off, o.bytes, i, 0);
-            }
-            break;
-          }
-          switch (o.kind) {
-            case O_SUM_I64: put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
-            case O_SUM_I128:
-              put(out, o.data_off, 8, 2 * i, *reinterpret_cast<const uint64_t*>(rec + o.off));
-              put(out, o.data_off, 8, 2 * i + 1, *reinterpret_cast<const uint64_t*>(rec + o.off + 8));
-              break;
-            case O_SUM_F64:
-              put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off));
-              break;
-            case O_AVG_I: {
-              // AVG = f64(sum) / f64(n) / 10^scale (v2_lowcard_group.zig:2128-2139)
-              double a = i128_to_f64(load_i128(rec + o.off)) / (double)ns;
-              if (o.scale_div != 1.0) a = a / o.scale_div;
-              put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(a));
-              break;
-            }
-            case O_AVG_F: {
-              const double a = *reinterpret_cast<const double*>(rec + o.off) / (double)ns;
-              put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(a));
-              break;
-            }
-            case O_MINMAX_I: put(out, o.data_off, o.bytes, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
-            default: {  // O_MINMAX_F
-              const double v = unord_f64(*reinterpret_cast<const int64_t*>(rec + o.off));
-              if (o.is_f32) {
-                const float f = (float)v;
-                put(out, o.data_off, 4, i, (uint64_t)(uint32_t)__float_as_uint(f));
-              } else {
-                put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(v));
-              }
-              break;
-            }
-          }
-        }
-      }
-      if (o.valid_off != ~0ull) out[o.valid_off + i] = valid;
-    }
-  }
-}
+// ---- finalize: typed result columns (finalize_row: order_device.h) ------------------
 
 __global__ void finalize_kernel(const FinalParams P, const uint32_t* __restrict__ sel, uint64_t n_out,
                                 uint8_t* out) {
@@ -1565,99 +1478,6 @@ hipError_t launch_dense_topn(const OrderParams& op, const FinalParams& fp, uint3
   if (n_rec == 0 || n_rec > kFastRec) return hipErrorInvalidValue;
   hipLaunchKernelGGL(dense_topn_kernel, dim3(1), dim3(kFastThreads), 0, s, op, fp, n_rec, offset, need, err, out,
                      hdr);
-  return hipGetLastError();
-}
-
-// ---- batched compaction and finalize ---------------------------------------------
-// Blocks of kBatchChunk (= kCompactChunk) records or output rows of one plan
-// each; a plan's compacted positions are contiguous in block order.  A plan
-// with HAVING keeps only its passing groups.
-
-__device__ __forceinline__ bool batch_keep(const BatchQuery& q, uint64_t i) {
-  return occupied(q.op, i) && (q.n_having == 0 || having_pass(q.fp, q.having, q.n_having, (uint32_t)i));
-}
-
-__global__ void __launch_bounds__(kCompactThreads) batch_compact_count_kernel(const BatchQuery* __restrict__ Q,
-                                                                              const BatchBlock* __restrict__ blocks,
-                                                                              uint32_t* block_counts) {
-  __shared__ uint32_t red[kCompactThreads / 64];
-  const BatchBlock B = blocks[blockIdx.x];
-  const BatchQuery& q = Q[B.q];
-  const uint64_t n_rec = q.sp.g_cap;
-  const uint64_t base = B.r0 + (uint64_t)threadIdx.x * kCompactPer;
-  uint32_t c = 0;
-  for (uint32_t j = 0; j < kCompactPer; j++)
-    if (base + j < n_rec && batch_keep(q, base + j)) c++;
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t = 0;
-    for (uint32_t w = 0; w < kCompactThreads / 64; w++) t += red[w];
-    block_counts[blockIdx.x] = t;
-  }
-}
-
-__global__ void __launch_bounds__(kCompactThreads) batch_compact_write_kernel(const BatchQuery* __restrict__ Q,
-                                                                              const BatchBlock* __restrict__ blocks,
-                                                                              const uint32_t* block_offs,
-                                                                              uint32_t* gids) {
-  __shared__ uint32_t pre[kCompactThreads];
-  const BatchBlock B = blocks[blockIdx.x];
-  const BatchQuery& q = Q[B.q];
-  const uint64_t n_rec = q.sp.g_cap;
-  const uint64_t base = B.r0 + (uint64_t)threadIdx.x * kCompactPer;
-  uint32_t bits = 0, c = 0;
-  for (uint32_t j = 0; j < kCompactPer; j++)
-    if (base + j < n_rec && batch_keep(q, base + j)) { bits |= 1u << j; c++; }
-  pre[threadIdx.x] = c;
-  __syncthreads();
-  for (uint32_t o = 1; o < kCompactThreads; o <<= 1) {
-    uint32_t v = threadIdx.x >= o ? pre[threadIdx.x - o] : 0;
-    __syncthreads();
-    pre[threadIdx.x] += v;
-    __syncthreads();
-  }
-  uint32_t out = block_offs[blockIdx.x] + pre[threadIdx.x] - c;
-  for (uint32_t j = 0; j < kCompactPer; j++)
-    if ((bits >> j) & 1u) gids[out++] = (uint32_t)(base + j);  // record index within the plan
-}
-
-hipError_t launch_batch_compact(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks, uint32_t* counts,
-                                uint64_t* total, uint32_t* gids, hipStream_t s) {
-  static_assert(kBatchChunk == kCompactChunk, "one compaction block per batch block");
-  if (n_blocks == 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(batch_compact_count_kernel, dim3(n_blocks), dim3(kCompactThreads), 0, s, q, blocks, counts);
-  hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, s, counts, n_blocks, total);
-  hipLaunchKernelGGL(batch_compact_write_kernel, dim3(n_blocks), dim3(kCompactThreads), 0, s, q, blocks,
-                     (const uint32_t*)counts, gids);
-  return hipGetLastError();
-}
-
-// Output rows [r0, r0 + kBatchChunk) of one plan; the block holding row 0
-// publishes the plan's group count.
-__global__ void __launch_bounds__(256) batch_finalize_kernel(const BatchQuery* __restrict__ Q,
-                                                             const BatchBlock* __restrict__ blocks,
-                                                             const uint32_t* __restrict__ offs,
-                                                             const uint64_t* __restrict__ total,
-                                                             uint32_t n_cblocks, const uint32_t* __restrict__ gids,
-                                                             uint8_t* out) {
-  const BatchBlock B = blocks[blockIdx.x];
-  const BatchQuery& q = Q[B.q];
-  const uint64_t start = offs[q.blk0];
-  const uint64_t end = q.blk1 < n_cblocks ? offs[q.blk1] : *total;
-  const uint64_t n = end - start;
-  if (B.r0 == 0 && threadIdx.x == 0) *q.n_out = n;
-  const uint64_t hi = B.r0 + kBatchChunk < n ? B.r0 + kBatchChunk : n;
-  for (uint64_t i = B.r0 + threadIdx.x; i < hi; i += blockDim.x) finalize_row(q.fp, gids[start + i], i, out);
-}
-
-hipError_t launch_batch_finalize(const BatchQuery* q, const BatchBlock* blocks, uint32_t n_blocks,
-                                 const uint32_t* offs, const uint64_t* total, uint32_t n_blocks_compact,
-                                 const uint32_t* gids, uint8_t* out, hipStream_t s) {
-  if (n_blocks == 0) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(batch_finalize_kernel, dim3(n_blocks), dim3(256), 0, s, q, blocks, offs, total, n_blocks_compact,
-                     gids, out);
   return hipGetLastError();
 }
 

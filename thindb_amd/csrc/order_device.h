@@ -213,9 +213,8 @@ __device__ __forceinline__ int having_order(const HNum& a, const HavingNode& h) 
 
 // havingPasses (:2502-2535) over the postfix nodes; the stack is one bit per
 // pending entry (at most kMaxHaving).
-__device__ __forceinline__ bool having_pass(const FinalParams& P, const HavingNode* __restrict__ H, uint32_t n,
-                                           uint32_t gid) {
-  const uint8_t* rec = P.g_state + (uint64_t)gid * P.g_rec;
+__device__ __forceinline__ bool having_pass_rec(const FinalParams& P, const HavingNode* __restrict__ H, uint32_t n,
+                                               const uint8_t* rec, uint32_t gid) {
   uint64_t st = 0;
   for (uint32_t k = 0; k < n; k++) {
     const HavingNode h = H[k];
@@ -258,6 +257,112 @@ __device__ __forceinline__ bool having_pass(const FinalParams& P, const HavingNo
     st = (st << 1) | (r ? 1ull : 0ull);
   }
   return (st & 1ull) != 0;
+}
+
+__device__ __forceinline__ bool having_pass(const FinalParams& P, const HavingNode* __restrict__ H, uint32_t n,
+                                           uint32_t gid) {
+  return having_pass_rec(P, H, n, P.g_state + (uint64_t)gid * P.g_rec, gid);
+}
+
+// ---- result rows -------------------------------------------------------------------
+
+// A record holds a group (its row count is not zero).
+__device__ __forceinline__ bool occupied(const OrderParams& P, uint64_t i) {
+  return *reinterpret_cast<const uint64_t*>(P.g_state + i * P.g_rec + P.g_count_off) != 0;
+}
+
+__device__ __forceinline__ void put(uint8_t* out, uint64_t off, uint32_t bytes, uint64_t row, uint64_t v) {
+  uint8_t* p = out + off + row * bytes;
+  switch (bytes) {
+    case 1: *p = (uint8_t)v; break;
+    case 2: *reinterpret_cast<uint16_t*>(p) = (uint16_t)v; break;
+    case 4: *reinterpret_cast<uint32_t*>(p) = (uint32_t)v; break;
+    default: *reinterpret_cast<uint64_t*>(p) = v; break;
+  }
+}
+
+// Output row i from record gid, read at `rec` (the group table, or a copy of
+// the record): every output column at its offset in `out`.
+__device__ __forceinline__ void finalize_rec(const FinalParams& P, const uint8_t* rec, uint32_t gid, uint64_t i,
+                                             uint8_t* out) {
+  {
+    const uint64_t count = *reinterpret_cast<const uint64_t*>(rec + P.g_count_off);
+    for (uint32_t c = 0; c < P.n_outs; c++) {
+      const OutDesc& o = P.outs[c];
+      uint8_t valid = 1;
+      switch (o.kind) {
+        case O_KEY: {
+          const GKey k = group_key(rec, gid, P.g_mode, P.g_key_off);
+          if (part_null(P.parts[o.idx], k)) {
+            valid = 0;
+            put(out, o.data_off, o.bytes, i, 0);
+          } else {
+            put(out, o.data_off, o.bytes, i, (uint64_t)part_value(P.parts[o.idx], k));
+          }
+          break;
+        }
+        case O_COUNT: put(out, o.data_off, 8, i, count); break;
+        case O_NS: put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
+        default: {
+          const uint64_t ns = o.ns_off != ~0u ? *reinterpret_cast<const uint64_t*>(rec + o.ns_off) : count;
+          if (ns == 0) {
+            valid = 0;
+            if (o.bytes == 16) {
+              put(out, o.data_off, 8, 2 * i, 0);
+              put(out, o.data_off, 8, 2 * i + 1, 0);
+            } else {
+              put(out, o.data_off, o.bytes, i, 0);
+            }
+            break;
+          }
+          switch (o.kind) {
+            case O_SUM_I64: put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
+            case O_SUM_I128:
+              put(out, o.data_off, 8, 2 * i, *reinterpret_cast<const uint64_t*>(rec + o.off));
+              put(out, o.data_off, 8, 2 * i + 1, *reinterpret_cast<const uint64_t*>(rec + o.off + 8));
+              break;
+            case O_SUM_F64:
+              put(out, o.data_off, 8, i, *reinterpret_cast<const uint64_t*>(rec + o.off));
+              break;
+            case O_AVG_I: {
+              // AVG = f64(sum) / f64(n) / 10^scale (v2_lowcard_group.zig:2128-2139)
+              double a = i128_to_f64(load_i128(rec + o.off)) / (double)ns;
+              if (o.scale_div != 1.0) a = a / o.scale_div;
+              put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(a));
+              break;
+            }
+            case O_AVG_F: {
+              const double a = *reinterpret_cast<const double*>(rec + o.off) / (double)ns;
+              put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(a));
+              break;
+            }
+            case O_MINMAX_I: put(out, o.data_off, o.bytes, i, *reinterpret_cast<const uint64_t*>(rec + o.off)); break;
+            default: {  // O_MINMAX_F
+              const double v = unord_f64(*reinterpret_cast<const int64_t*>(rec + o.off));
+              if (o.is_f32) {
+                const float f = (float)v;
+                put(out, o.data_off, 4, i, (uint64_t)(uint32_t)__float_as_uint(f));
+              } else {
+                put(out, o.data_off, 8, i, (uint64_t)__double_as_longlong(v));
+              }
+              break;
+            }
+          }
+        }
+      }
+      if (o.valid_off != ~0ull) out[o.valid_off + i] = valid;
+    }
+  }
+}
+
+// Output row i from record gid of the group table.
+__device__ __forceinline__ void finalize_row(const FinalParams& P, uint32_t gid, uint64_t i, uint8_t* out) {
+  finalize_rec(P, P.g_state + (uint64_t)gid * P.g_rec, gid, i, out);
+}
+
+// A batched plan's record i is a result row: occupied and passing its HAVING.
+__device__ __forceinline__ bool batch_keep(const BatchQuery& q, uint64_t i) {
+  return occupied(q.op, i) && (q.n_having == 0 || having_pass(q.fp, q.having, q.n_having, (uint32_t)i));
 }
 
 }  // namespace tdbg
