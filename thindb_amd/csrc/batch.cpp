@@ -19,7 +19,8 @@ using namespace tdbg::rt;
 
 namespace {
 
-constexpr uint64_t kItemRows = 1ull << 17;     // rows per scan / partition work item
+constexpr uint64_t kItemRows = 1ull << 17;     // rows per scan work item
+constexpr uint64_t kRadixItemRows = 1ull << 19;  // rows per radix work item (THINDB_GPU_BATCH_RADIX_ROWS)
 constexpr uint32_t kSpinLimit = 1u << 21;      // polls (~seconds) before a dependency wait gives up
 constexpr uint32_t kBatchLds = 78u * 1024u;    // LDS per workgroup: two workgroups per CU
 constexpr uint32_t kTuneSample = 64;           // plans per bucket the autotuner times together
@@ -240,6 +241,9 @@ int32_t build(tdbg_batch* b) {
   }
   b->lds_bytes = lds_need;
   // device layout
+  const char* rx_env = getenv("THINDB_GPU_BATCH_RADIX_ROWS");
+  const uint64_t rx_rows = rx_env && rx_env[0] ? std::max<uint64_t>(kTileRows, strtoull(rx_env, nullptr, 10)) : kRadixItemRows;
+  const uint32_t rx_chunk = (uint32_t)std::max<uint64_t>(1, rx_rows / kTileRows);
   uint64_t rec_b = 0, tile_b = 0, rows_b = 0, misc_b = 0;
   std::vector<uint64_t> rec_o(n), tile_o(n), rows_o(n), misc_o(n);
   for (uint32_t i = 0; i < n; i++) {
@@ -253,7 +257,7 @@ int32_t build(tdbg_batch* b) {
       rows_b += align_up(sets[i].rows * q.rx.row.stride + 256, 256);
       // bucket starts and ends, then per-item digit counts (one item per
       // kItemRows rows of tiles)
-      const uint64_t n_hist = (sets[i].tiles.size() + kItemRows / kTileRows - 1) / (kItemRows / kTileRows);
+      const uint64_t n_hist = (sets[i].tiles.size() + rx_chunk - 1) / rx_chunk;
       misc_o[i] = misc_b;
       misc_b += align_up(((2ull << q.rx.bits1) + 2) * 8, 256) + align_up(n_hist * (4ull << q.rx.bits1), 256);
     }
@@ -308,15 +312,15 @@ int32_t build(tdbg_batch* b) {
     q.n_hist = q.n_scatter = q.n_comp = 0;
     if (b->info[i].strategy == 4u) {
       q.sc0 = (uint32_t)dyn_items.size();
-      for (uint32_t t0 = 0, k = 0; t0 < n_tiles; t0 += base_chunk, k++) {
-        const uint32_t t1 = std::min(n_tiles, t0 + base_chunk);
+      for (uint32_t t0 = 0, k = 0; t0 < n_tiles; t0 += rx_chunk, k++) {
+        const uint32_t t1 = std::min(n_tiles, t0 + rx_chunk);
         hist_items.push_back(BatchItem{BI_RX_HIST, i, t0, t1, k, 0});  // part: the item's index
         dyn_items.push_back(BatchItem{BI_RX_SCATTER, i, t0, t1, k, 0});
         q.n_hist++;
         q.n_scatter++;
       }
       q.ag0 = (uint32_t)dyn_items.size();
-      const uint64_t per = std::max<uint64_t>(1, (uint64_t)q.rx.n_leaves * kItemRows / std::max<uint64_t>(sets[i].rows, 1));
+      const uint64_t per = std::max<uint64_t>(1, (uint64_t)q.rx.n_leaves * rx_rows / std::max<uint64_t>(sets[i].rows, 1));
       for (uint64_t l0 = 0; l0 < q.rx.n_leaves; l0 += per) {
         dyn_items.push_back(BatchItem{BI_RX_AGG, i, (uint32_t)l0, (uint32_t)std::min<uint64_t>(q.rx.n_leaves, l0 + per), 0, 0});
         q.n_comp++;
@@ -378,7 +382,7 @@ int32_t build(tdbg_batch* b) {
   const uint32_t n_cu = (uint32_t)std::max(ctx->n_cu, 8);
   b->grid = std::max<uint32_t>(1, std::min<uint32_t>(n_cu * 2, b->n_items));
   const char* ecap = getenv("THINDB_GPU_BATCH_EMITTERS");
-  b->emit_cap = ecap && ecap[0] ? (uint32_t)std::max(1, atoi(ecap)) : std::max<uint32_t>(1, b->grid / 8);
+  b->emit_cap = ecap && ecap[0] ? (uint32_t)std::max(1, atoi(ecap)) : std::max<uint32_t>(1, b->grid * 3 / 32);
   // zeroed per execution: queue state, per-plan stage counters, ready-queue
   // slots, emit item counts
   uint64_t zero_b = 0;
