@@ -23,8 +23,9 @@ GEN_LIB = PKG / "libthindb_gen.so"
 
 HIP_SOURCES = ["scan_agg.hip", "order.hip", "radix.hip", "batch.hip", "decode.hip", "distinct.hip", "runtime.cpp", "batch.cpp",
                "server.cpp", "jit.cpp", "segment.cpp", "noalloc.cpp"]
-# Device headers the run-time compiler needs, embedded as string literals.
-JIT_HEADERS = [("kJitEngineH", "engine.h"), ("kJitScanCommonH", "scan_common.h")]
+# Device headers the run-time compiler needs (plan-specialised scan kernels
+# and batch kernels), embedded as string literals.
+JIT_HEADERS = ["engine.h", "scan_common.h", "strip_generic.h", "order_device.h", "radix_device.h", "batch_device.h"]
 ARCH = os.environ.get("THINDB_GPU_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -55,11 +56,14 @@ def _run(cmd: list[str]) -> None:
 def write_jit_headers(objdir: Path) -> Path:
     """Embeds the device headers as raw string literals for hipRTC."""
     out = ["// generated by thindb_amd/_build.py: device headers for run-time compilation"]
-    for sym, name in JIT_HEADERS:
+    for i, name in enumerate(JIT_HEADERS):
         text = (CSRC / name).read_text()
         if ")TDBGJIT\"" in text:
             raise RuntimeError(f"{name} contains the raw-string delimiter")
-        out.append(f'static const char {sym}[] = R"TDBGJIT({text})TDBGJIT";')
+        out.append(f'static const char kJitHeader{i}[] = R"TDBGJIT({text})TDBGJIT";')
+    out.append("static const char* const kJitHeaderSrc[] = {" + ", ".join(f"kJitHeader{i}" for i in range(len(JIT_HEADERS))) + "};")
+    out.append("static const char* const kJitHeaderName[] = {" + ", ".join(f'"{n}"' for n in JIT_HEADERS) + "};")
+    out.append(f"static const int kJitHeaders = {len(JIT_HEADERS)};")
     path = objdir / "jit_headers.inc"
     path.write_text("\n".join(out) + "\n")
     return path

@@ -29,6 +29,7 @@ constexpr uint32_t kTuneCopies = 16;           // repeats of the sample, at most
 constexpr uint32_t kRowsPerSlot = 256;         // scan rows per LDS slot flushed, at least, per item
 constexpr uint32_t kWaves = 1;                 // waves of a large batch (THINDB_GPU_BATCH_WAVES: more)
 constexpr uint32_t kWavePlansMin = 32;         // plans per wave, at least
+constexpr size_t kJitStrips = 8;               // plan structures a run-time compiled batch kernel specialises
 
 uint32_t ilog2(uint64_t x) { return x ? 63u - (uint32_t)__builtin_clzll(x) : 0u; }
 uint32_t bucket_of(uint64_t rows, uint64_t groups) { return (ilog2(rows) / 2) << 8 | (ilog2(groups) / 2); }
@@ -100,6 +101,7 @@ struct tdbg_batch {
   hipStream_t copy_stream = nullptr;
   hipEvent_t kernels_done = nullptr;
   hipEvent_t ev_kernels = nullptr;   // timing: the kernels ended (before the result copy)
+  hipFunction_t jit = nullptr;       // the batch kernel compiled for these plans (nullptr: ahead of time)
   float kernels_ms = 0;              // last execution: device time before the result copy
 };
 
@@ -149,9 +151,14 @@ int32_t enqueue(tdbg_batch* b, bool timed) {
   HIP_TRY(launch_batch_init(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchBlock*>(b->d_cblocks.p),
                             b->n_cblocks, s), "batch record init");
   if (timed) HIP_TRY(hipEventRecord(b->ev[1], s), "event");
-  HIP_TRY(launch_batch(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchItem*>(b->d_items.p),
-                       static_cast<const BatchArgs*>(b->d_run.p), b->n_items, b->grid, b->lds_bytes, s),
-          "batch kernel");
+  if (b->jit && b->n_items)
+    HIP_TRY(launch_batch_jit(b->jit, static_cast<const BatchQuery*>(b->d_q.p),
+                             static_cast<const BatchItem*>(b->d_items.p), static_cast<const BatchArgs*>(b->d_run.p),
+                             b->grid, b->lds_bytes, s), "batch kernel (compiled for its plans)");
+  else
+    HIP_TRY(launch_batch(static_cast<const BatchQuery*>(b->d_q.p), static_cast<const BatchItem*>(b->d_items.p),
+                         static_cast<const BatchArgs*>(b->d_run.p), b->n_items, b->grid, b->lds_bytes, s),
+            "batch kernel");
   if (timed) HIP_TRY(hipEventRecord(b->ev[2], s), "event");
   return TDBG_OK;
 }
@@ -291,6 +298,38 @@ int32_t build(tdbg_batch* b) {
     }
   }
   b->out_bytes = out_b;
+  // scan plans of one structure share a strip generated for it in a batch
+  // kernel compiled at run time (not for the autotuner's trial batches,
+  // whose strategies change from trial to trial)
+  b->jit = nullptr;
+  {
+    std::vector<ScanParams> sigs;
+    std::vector<std::string> keys;
+    bool generic = false;
+    for (uint32_t i = 0; i < n; i++) {
+      BatchQuery& q = Q[i];
+      q.jit_sig = ~0u;
+      if (b->info[i].strategy == 4u) continue;
+      if (q.sp.n_cols > 8 || !b->forced.empty()) {
+        generic = true;
+        continue;
+      }
+      const std::string k = jit_signature(q.sp);
+      auto it = std::find(keys.begin(), keys.end(), k);
+      if (it != keys.end()) {
+        q.jit_sig = (uint32_t)(it - keys.begin());
+      } else if (keys.size() < kJitStrips) {
+        q.jit_sig = (uint32_t)keys.size();
+        keys.push_back(k);
+        sigs.push_back(q.sp);
+      } else {
+        generic = true;
+      }
+    }
+    if (b->forced.empty() && !sigs.empty()) b->jit = batch_jit_kernel(sigs, generic);
+    if (!b->jit)
+      for (uint32_t i = 0; i < n; i++) Q[i].jit_sig = ~0u;
+  }
   // record-init blocks and work items: histograms and scans are ready at
   // launch (the static queue); each plan's scatters, leaves and emit items
   // follow it in the item array, appended to the ready queues on the device

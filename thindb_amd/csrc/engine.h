@@ -177,6 +177,10 @@ struct RowLayout {
 
 struct HavingLeaf;
 
+// Bins of the radix leaves' group-count histogram (ORDER BY COUNT(*) cuts):
+// a group lands in bin min(row count, kRxCountBins - 1).
+constexpr uint32_t kRxCountBins = 256;
+
 struct RadixParams {
   RowLayout row;
   uint32_t shift1, bits1;      // level-1 digit = key >> shift1 (bits1 wide)
@@ -374,7 +378,7 @@ struct BatchQuery {
   uint32_t sc0, ag0, em0;        // first scatter, leaf and emit item (indices into the item array)
   uint32_t n_emit;               // emit items (records in kBatchChunk blocks)
   uint32_t n_having;             // HAVING nodes at `having` (0 = none)
-  uint32_t pad;
+  uint32_t jit_sig;              // scan plans: its generated strip in a run-time compiled batch kernel (~0u: none)
   const HavingNode* having;
   uint32_t* sync;                // [0] histogram items done, [1] scatter items done, [2] n_comp items done
   uint32_t* item_hist;           // radix: [n_hist][2^bits1] per-item digit counts, then offsets
@@ -429,6 +433,10 @@ constexpr uint32_t kRxStageOff = 24 * 1024;
 enum BatchErr : uint32_t { ERR_BATCH_WAIT = 4u };  // a dependency wait gave up (ErrBit space)
 
 #if !defined(__HIPCC_RTC__)
+}  // namespace tdbg
+#include <string>
+#include <vector>
+namespace tdbg {
 // ---- kernel launchers (defined in the .hip files) -------------------------
 hipError_t launch_fill_records(uint8_t* state, uint64_t n_rec, uint32_t rec_bytes, const uint64_t* tmpl,
                                hipStream_t s, uint32_t* flags = nullptr, uint32_t n_flags = 0);
@@ -439,6 +447,13 @@ hipError_t launch_scan_agg(const ScanParams& p, uint32_t grid, uint32_t block, h
 hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, hipStream_t s, bool* used,
                            bool cached_only = false);
 const char* jit_last_log();
+// A batch kernel compiled at run time with a generated strip per structure
+// in `sigs` (scan plans pick theirs by BatchQuery::jit_sig; `generic`: some
+// scan plan has none); nullptr: use launch_batch.
+hipFunction_t batch_jit_kernel(const std::vector<ScanParams>& sigs, bool generic);
+std::string jit_signature(const ScanParams& p);  // the structure a generated strip fixes
+hipError_t launch_batch_jit(hipFunction_t fn, const BatchQuery* Q, const BatchItem* items, const BatchArgs* args,
+                            uint32_t grid, uint32_t lds_bytes, hipStream_t s);
 hipError_t launch_compact(const OrderParams& p, uint64_t n_rec, uint32_t* block_counts, uint32_t* gids,
                           uint64_t* n_out, hipStream_t s);
 hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_t need, void* scratch,
@@ -508,7 +523,6 @@ hipError_t launch_rle_expand(const void* values, const uint32_t* starts, uint32_
 // into the group positions of every group whose count is >= cut (desc) or
 // <= cut (asc), in any order, to idx (cap entries); their number to *n_idx
 // (zeroed here).
-constexpr uint32_t kRxCountBins = 256;
 hipError_t launch_radix_count_take(const RadixParams& r, uint32_t cut, bool desc, uint32_t* idx, uint32_t cap,
                                    uint32_t* n_idx, hipStream_t s);
 // After RX_AGG of compacted leaves: the group count to *n_out and the

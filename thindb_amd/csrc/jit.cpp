@@ -22,7 +22,7 @@
 #include <vector>
 
 #include "engine.h"
-#include "jit_headers.inc"  // kJitEngineH, kJitScanCommonH (generated by the build)
+#include "jit_headers.inc"  // kJitHeaderSrc / kJitHeaderName (generated by the build)
 
 namespace tdbg {
 
@@ -98,11 +98,11 @@ std::string jit_signature(const ScanParams& p) {
   return s.str();
 }
 
-std::string jit_source(const ScanParams& p, bool prefetch) {
+// The strip of a plan's structure as straight-line code: struct `name`
+// with load / run / process, the interface scan_range and scan_range_pf use.
+std::string strip_struct(const ScanParams& p, const std::string& name) {
   std::ostringstream s;
-  s << "#include \"scan_common.h\"\n"
-       "namespace tdbg {\n"
-       "struct JitStrip {\n"
+  s << "struct " << name << " {\n"
        "  struct Data {\n";
   for (uint32_t i = 0; i < p.n_cols; i++) s << "    Raw x" << i << ";\n";
   s << "  };\n"
@@ -187,13 +187,59 @@ std::string jit_source(const ScanParams& p, bool prefetch) {
       << d.col << ", m, slot, rec, lds); }\n";
   }
   s << "  }\n"
-       "};\n"
-       "}  // namespace tdbg\n"
+       "};\n";
+  return s.str();
+}
+
+std::string jit_source(const ScanParams& p, bool prefetch) {
+  std::ostringstream s;
+  s << "#include \"scan_common.h\"\n"
+       "namespace tdbg {\n"
+    << strip_struct(p, "JitStrip")
+    << "}  // namespace tdbg\n"
        "extern \"C\" __global__ void __launch_bounds__(1024) tdbg_scan_jit(const tdbg::ScanParams P) {\n"
        "  extern __shared__ __align__(16) uint8_t lds[];\n"
        "  tdbg::"
     << (prefetch ? "scan_body_pf" : "scan_body") << "<tdbg::" << lmode_name(p.lds_mode)
     << ", tdbg::JitStrip>(P, lds);\n"
+       "}\n";
+  return s.str();
+}
+
+// A batch kernel for a batch's scan plans: batch_device.h's loop with one
+// generated strip per distinct plan structure (BatchQuery::jit_sig picks
+// it; ~0u and radix plans take the generic paths).
+std::string batch_jit_source(const std::vector<ScanParams>& sigs, bool generic) {
+  std::ostringstream s;
+  s << "#include \"batch_device.h\"\n";
+  if (generic) s << "#include \"strip_generic.h\"\n";
+  s << "namespace tdbg {\n";
+  for (size_t i = 0; i < sigs.size(); i++) s << strip_struct(sigs[i], "JitStrip" + std::to_string(i));
+  s << "struct JitScan {\n"
+       "  static __device__ __forceinline__ void run(const BatchQuery& B, const ScanParams& P, uint8_t* lds,\n"
+       "                                             const BatchItem& it) {\n"
+       "    const uint64_t part_lo = (uint64_t)it.part * P.lds_cap;\n"
+       "    (void)part_lo;\n"
+       "    switch (B.jit_sig) {\n";
+  for (size_t i = 0; i < sigs.size(); i++) {
+    const uint32_t m = sigs[i].lds_mode;
+    s << "      case " << i << ": scan_range<" << lmode_name(m) << ", JitStrip" << i << ">(P, lds, it.a0, it.a1, "
+      << (m == LDS_DIRECT ? "part_lo" : "0") << "); break;\n";
+  }
+  s << "      default:\n";
+  if (generic)
+    s << "        if (P.lds_mode == LDS_DIRECT) scan_range<LDS_DIRECT, GenericStrip<false>>(P, lds, it.a0, it.a1, part_lo);\n"
+         "        else if (P.lds_mode == LDS_HASH) scan_range<LDS_HASH, GenericStrip<false>>(P, lds, it.a0, it.a1, 0);\n"
+         "        else scan_range<LDS_NONE, GenericStrip<false>>(P, lds, it.a0, it.a1, 0);\n";
+  s << "        break;\n"
+       "    }\n"
+       "  }\n"
+       "};\n"
+       "}  // namespace tdbg\n"
+       "extern \"C\" __global__ void __launch_bounds__(tdbg::kBatchThreads, 4) tdbg_batch_jit(\n"
+       "    const tdbg::BatchQuery* __restrict__ Q, const tdbg::BatchItem* __restrict__ items,\n"
+       "    const tdbg::BatchArgs* __restrict__ Ap) {\n"
+       "  tdbg::batch_loop<tdbg::JitScan>(Q, items, Ap);\n"
        "}\n";
   return s.str();
 }
@@ -208,9 +254,8 @@ bool jit_prefetch() {
 
 static bool compile(const std::string& src, std::vector<char>& code, std::string& log) {
   hiprtcProgram prog;
-  const char* headers[] = {kJitEngineH, kJitScanCommonH};
-  const char* names[] = {"engine.h", "scan_common.h"};
-  if (hiprtcCreateProgram(&prog, src.c_str(), "tdbg_scan_jit.hip", 2, headers, names) != HIPRTC_SUCCESS) {
+  if (hiprtcCreateProgram(&prog, src.c_str(), "tdbg_jit.hip", kJitHeaders, kJitHeaderSrc, kJitHeaderName) !=
+      HIPRTC_SUCCESS) {
     log = "hiprtcCreateProgram failed";
     return false;
   }
@@ -286,6 +331,49 @@ hipError_t launch_scan_jit(const ScanParams& p, uint32_t grid, uint32_t block, h
   hipError_t e = hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, (unsigned)lds, s, args, nullptr);
   if (e == hipSuccess) *used = true;
   return e;
+}
+
+// The batch kernel for the scan-plan structures `sigs` (compiled once per
+// device and set of structures); nullptr when run-time compilation is off or
+// fails, and the batch takes the ahead-of-time kernel.
+hipFunction_t batch_jit_kernel(const std::vector<ScanParams>& sigs, bool generic) {
+  if (!jit_enabled() || sigs.empty()) return nullptr;
+  const char* e = std::getenv("THINDB_GPU_BATCH_JIT");
+  if (e && e[0] == '0') return nullptr;
+  int dev = 0;
+  hipGetDevice(&dev);
+  std::string key = std::to_string(dev) + "|batch|" + (generic ? "g|" : "");
+  for (const ScanParams& p : sigs) key += jit_signature(p) + "#";
+  std::lock_guard<std::mutex> lk(g_mu);
+  JitKernel& k = g_cache[key];
+  if (k.failed) return nullptr;
+  if (!k.fn) {
+    std::vector<char> code;
+    std::string log;
+    if (!compile(batch_jit_source(sigs, generic), code, log)) {
+      k.failed = true;
+      g_last_log = log;
+      if (std::getenv("THINDB_GPU_JIT_VERBOSE")) fprintf(stderr, "[tdbg jit] batch compile failed: %s\n", log.c_str());
+      return nullptr;
+    }
+    if (hipModuleLoadData(&k.mod, code.data()) != hipSuccess ||
+        hipModuleGetFunction(&k.fn, k.mod, "tdbg_batch_jit") != hipSuccess) {
+      (void)hipGetLastError();
+      k.failed = true;
+      k.fn = nullptr;
+      return nullptr;
+    }
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k.fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024 - 1024);
+    (void)hipGetLastError();
+  }
+  return k.fn;
+}
+
+hipError_t launch_batch_jit(hipFunction_t fn, const BatchQuery* Q, const BatchItem* items, const BatchArgs* args,
+                            uint32_t grid, uint32_t lds_bytes, hipStream_t s) {
+  void* kargs[] = {&Q, &items, &args};
+  return hipModuleLaunchKernel(fn, grid, 1, 1, kBatchThreads, 1, 1, lds_bytes, s, kargs, nullptr);
 }
 
 const char* jit_last_log() { return g_last_log.c_str(); }
