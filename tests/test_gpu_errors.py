@@ -32,7 +32,7 @@ def rows(table, req, cols):
 def test_memory_budget_exceeded_releases_everything(oracle_lib):
     sess = GpuSession(0, hbm_budget_bytes=384 << 20)
     try:
-        n = 1_000_000
+        n = 1_200_000  # at least 2^24 / 16 rows: the 2^24-key span takes a dense table
         rng = np.random.default_rng(1)
         t = Table([Column("k", Type.BIGINT), Column("v", Type.BIGINT)],
                   {"k": rng.integers(0, 1 << 24, n), "v": rng.integers(-1000, 1000, n)})

@@ -299,34 +299,34 @@ int32_t build(tdbg_batch* b) {
   }
   b->out_bytes = out_b;
   // scan plans of one structure share a strip generated for it in a batch
-  // kernel compiled at run time (not for the autotuner's trial batches,
-  // whose strategies change from trial to trial)
+  // kernel compiled at run time, for batches that are executed again and
+  // again (tdbg_batch_create): not for the autotuner's trials, whose
+  // strategies change from trial to trial, nor for the serving loop's
+  // batches, whose plan sets change from launch to launch (a compile takes
+  // about two seconds).  The structures are sorted, so one set compiles once.
   b->jit = nullptr;
-  {
-    std::vector<ScanParams> sigs;
-    std::vector<std::string> keys;
+  for (uint32_t i = 0; i < n; i++) Q[i].jit_sig = ~0u;
+  if (b->use_graph && b->forced.empty()) {
+    std::vector<std::pair<std::string, uint32_t>> keyed;  // (structure, plan)
     bool generic = false;
     for (uint32_t i = 0; i < n; i++) {
-      BatchQuery& q = Q[i];
-      q.jit_sig = ~0u;
       if (b->info[i].strategy == 4u) continue;
-      if (q.sp.n_cols > 8 || !b->forced.empty()) {
-        generic = true;
-        continue;
-      }
-      const std::string k = jit_signature(q.sp);
-      auto it = std::find(keys.begin(), keys.end(), k);
-      if (it != keys.end()) {
-        q.jit_sig = (uint32_t)(it - keys.begin());
-      } else if (keys.size() < kJitStrips) {
-        q.jit_sig = (uint32_t)keys.size();
-        keys.push_back(k);
-        sigs.push_back(q.sp);
-      } else {
-        generic = true;
-      }
+      if (Q[i].sp.n_cols > 8) generic = true;
+      else keyed.emplace_back(jit_signature(Q[i].sp), i);
     }
-    if (b->forced.empty() && !sigs.empty()) b->jit = batch_jit_kernel(sigs, generic);
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<ScanParams> sigs;
+    for (size_t j = 0; j < keyed.size(); j++) {
+      if (j == 0 || keyed[j].first != keyed[j - 1].first) {
+        if (sigs.size() == kJitStrips) {
+          generic = true;
+          break;
+        }
+        sigs.push_back(Q[keyed[j].second].sp);
+      }
+      Q[keyed[j].second].jit_sig = (uint32_t)sigs.size() - 1;
+    }
+    if (!sigs.empty()) b->jit = batch_jit_kernel(sigs, generic);
     if (!b->jit)
       for (uint32_t i = 0; i < n; i++) Q[i].jit_sig = ~0u;
   }

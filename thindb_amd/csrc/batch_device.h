@@ -28,11 +28,16 @@
 //
 // Device-scope release and acquire are L2 writebacks and invalidations on
 // this chip (each XCD has its own L2), so they are spent once per item, not
-// per access: an item whose output other workgroups read (histogram counts,
+// per access, and only where plain loads must see other workgroups' plain
+// stores: an item whose output other workgroups read (histogram counts,
 // partition rows, leaf records) ends with one release by one thread before
-// it counts itself done; an item taken from a ready queue starts with one
-// acquire; queue words, slots and emit counts are relaxed atomics.  Scan
-// items update records only through device atomics and need neither.
+// it counts itself done; a leaf item starts with one acquire (it reads the
+// partition rows).  A scatter item reads its write offsets, and an emit item
+// its records, with device-coherent loads that bypass a stale L2 line
+// instead, so neither invalidates the L2 other workgroups of its XCD are
+// reusing (an emit item over HAVING, or over records too wide to stage,
+// still acquires).  Queue words, slots and emit counts are relaxed atomics.
+// Scan items update records only through device atomics.
 //
 // An emit item counts its rows and publishes the count before it waits for
 // anything; its first output row is the sum of the counts of the plan's
@@ -81,24 +86,21 @@ __device__ __forceinline__ uint32_t batch_pick(const BatchArgs& A) {
   BatchSched* S = A.sched;
   for (uint32_t spins = 0;; spins++) {
     uint32_t it = kNoItem;
-    bool ready_q = true;  // taken from a ready queue: its inputs come from other workgroups
+    bool leaf = false;  // a leaf item reads partition rows other workgroups wrote
     const bool room = __hip_atomic_load(&S->emitters.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.emit_cap;
     if (room && (it = rq_take(A, BQ_EMIT)) < kExit) atomicAdd(&S->emitters.v, 1u);
-    if (it == kNoItem) it = rq_take(A, BQ_AGG);
+    if (it == kNoItem) leaf = (it = rq_take(A, BQ_AGG)) < kExit;
     if (it == kNoItem) it = rq_take(A, BQ_SCATTER);
     if (it == kNoItem && __hip_atomic_load(&S->head0.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.n_static) {
       const uint32_t h = atomicAdd(&S->head0.v, 1u);
-      if (h < A.n_static) {
-        it = h;
-        ready_q = false;
-      }
+      if (h < A.n_static) it = h;
     }
     if (it == kNoItem && !room && (it = rq_take(A, BQ_EMIT)) < kExit) atomicAdd(&S->emitters.v, 1u);
     if (it == kExit) return kExit;
     if (it != kNoItem) {
       atomicAdd(&S->claimed.v, 1u);
-      // the producers released their writes before the item was appended
-      if (ready_q) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // the scatters released their writes before the item was appended
+      if (leaf) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       return it;
     }
     if (__hip_atomic_load(&S->claimed.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= A.n_items) return kExit;
@@ -162,13 +164,20 @@ __device__ __forceinline__ void batch_emit(const BatchArgs& A, const BatchQuery&
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const FinalParams& F = q.fp;
   const uint32_t g_rec = F.g_rec;
+  const bool staged = (g_rec & 7u) == 0 && kEmitStageOff + (uint64_t)kBatchThreads * g_rec <= A.lds_bytes;
+  if (q.n_having || !staged) {  // records read with plain loads below: see the producers' writes
+    if (threadIdx.x == 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __syncthreads();
+  }
   constexpr uint32_t kLoads = 4;  // count words in flight per thread
   for (uint32_t s0 = 0; s0 < kEmitPer; s0 += kLoads) {
     uint64_t cw[kLoads];
 #pragma unroll
     for (uint32_t j = 0; j < kLoads; j++) {
       const uint64_t r = r0 + (uint64_t)(s0 + j) * kBatchThreads + threadIdx.x;
-      cw[j] = r < r1 ? *reinterpret_cast<const uint64_t*>(F.g_state + r * g_rec + F.g_count_off) : 0ull;
+      cw[j] = r < r1 ? __hip_atomic_load(reinterpret_cast<const uint64_t*>(F.g_state + r * g_rec + F.g_count_off),
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : 0ull;
     }
 #pragma unroll
     for (uint32_t j = 0; j < kLoads; j++) {
@@ -222,7 +231,6 @@ __device__ __forceinline__ void batch_emit(const BatchArgs& A, const BatchQuery&
   for (uint32_t w = 0; w < kWaves; w++) first += red[1 + w];
   uint8_t* out = A.out;
   uint8_t* stage = lds + kEmitStageOff;
-  const bool staged = (g_rec & 7u) == 0 && kEmitStageOff + (uint64_t)kBatchThreads * g_rec <= A.lds_bytes;
   for (uint32_t s = 0; s < kEmitPer; s++) {  // uniform
     const uint64_t base = r0 + (uint64_t)s * kBatchThreads;
     if (base >= r1) break;
@@ -233,7 +241,8 @@ __device__ __forceinline__ void batch_emit(const BatchArgs& A, const BatchQuery&
       const uint64_t* src = reinterpret_cast<const uint64_t*>(F.g_state + base * g_rec);
       uint64_t* dst = reinterpret_cast<uint64_t*>(stage);
       const uint32_t words = (uint32_t)(n * g_rec / 8);
-      for (uint32_t w = threadIdx.x; w < words; w += kBatchThreads) dst[w] = src[w];
+      for (uint32_t w = threadIdx.x; w < words; w += kBatchThreads)
+        dst[w] = __hip_atomic_load(src + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __syncthreads();
     }
     const uint64_t m = keep[s * kWaves + wave];

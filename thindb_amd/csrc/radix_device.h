@@ -411,8 +411,9 @@ __device__ __forceinline__ void rx_scatter_range(const ScanParams& P, const Radi
   uint64_t* base = reinterpret_cast<uint64_t*>(lds + kRxBaseOff);
   uint8_t* stage = lds + kRxStageOff;
   const uint32_t nb = 1u << R.bits1, stride = R.row.stride;
-  if (item_off)
-    for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x) cur[d] = item_off[d];
+  if (item_off)  // written by another workgroup in the same launch: a device-coherent load
+    for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x)
+      cur[d] = __hip_atomic_load(item_off + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t t = t0; t < t1; t++) {
     const Tile T = P.tiles[t];
     for (uint32_t s0 = 0; s0 < T.n; s0 += R.tile_rows) {  // uniform sub-tile loop

@@ -643,8 +643,12 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     }
     o.kagg = found;
   }
-  // HBM record layout
-  const bool dense = dom <= kDenseMax || (pl->est_groups && dom <= 4 * pl->est_groups);
+  // HBM record layout: dense (a record per relative key) when the key span is
+  // small or well filled by the rows scanned, else a hash table sized for the
+  // groups: a dense table over a span far wider than the rows is mostly
+  // empty records that the result pass still reads
+  const bool dense = (dom <= kDenseMax && (dom <= kDenseFloor || dom <= kDenseRowsX * std::max<uint64_t>(n_rows_scan, 1))) ||
+                     (pl->est_groups && dom <= 4 * pl->est_groups);
   sp.g_mode = dense ? G_DENSE : G_HASH;
   uint32_t off = 0;
   if (!dense) { sp.g_key_off = off; off += 8; }

@@ -44,7 +44,9 @@ inline uint64_t next_pow2(uint64_t x) {
 }
 
 constexpr uint32_t kTileRows = 16384;        // rows per scan tile
-constexpr uint64_t kDenseMax = 1ull << 24;   // key spans up to this take a dense HBM table
+constexpr uint64_t kDenseMax = 1ull << 24;   // key spans up to this take a dense HBM table ...
+constexpr uint64_t kDenseFloor = 1ull << 16; // ... always up to this,
+constexpr uint64_t kDenseRowsX = 16;         // ... else when at most this many keys per row scanned
 
 // ---- HBM column store ------------------------------------------------------
 

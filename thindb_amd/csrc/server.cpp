@@ -139,7 +139,8 @@ uint64_t estimate(tdbg_context* ctx, const tdbg_plan* pl, uint64_t* rows_out) {
   const uint64_t d = (uint64_t)dom;
   const uint64_t groups = std::max<uint64_t>(1, std::min<uint64_t>(pl->est_groups ? pl->est_groups : d, rows));
   const uint64_t rec = 16 + 24 * pl->aggs.size();
-  const uint64_t table = (d <= kDenseMax ? d : next_pow2(groups + groups / 2 + 1)) * rec;
+  const bool dense = d <= kDenseMax && (d <= kDenseFloor || d <= kDenseRowsX * std::max<uint64_t>(rows, 1));
+  const uint64_t table = (dense ? d : next_pow2(groups + groups / 2 + 1)) * rec;
   const uint64_t radix = (d > 65536 && rows >= (1u << 20) && d / 4 <= rows) ? rows * 32 : 0;
   const uint64_t out = groups * 9 * (pl->keys.size() + pl->aggs.size());
   return table + radix + out + (rows / kTileRows + 1) * sizeof(Tile);
