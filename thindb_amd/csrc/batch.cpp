@@ -85,6 +85,7 @@ struct tdbg_batch {
   uint32_t n_items = 0, n_static = 0, n_cblocks = 0, grid = 0, lds_bytes = 0, emit_cap = 0;
   uint64_t zero_words = 0, out_bytes = 0, err_off = 0, head_bytes = 0, result_bytes = 0;
   uint64_t sched_off = 0, slots_off[kBatchQueues] = {0, 0, 0};
+  uint32_t q_items[kBatchQueues] = {0, 0, 0};
   DevBuf d_q, d_items, d_tiles, d_rec, d_rows, d_misc, d_zero, d_cblocks, d_head, d_run;
   PinnedBuf h_out;                 // results keep the buffer they view
   uint8_t* h_run = nullptr;        // pinned: the output base the kernel writes through
@@ -416,6 +417,7 @@ int32_t build(tdbg_batch* b) {
     per_queue[BQ_EMIT] += Q[i].n_emit;
     n_emit_all += Q[i].n_emit;
   }
+  for (uint32_t k = 0; k < kBatchQueues; k++) b->q_items[k] = per_queue[k];
   b->n_items = (uint32_t)items.size();
   b->n_cblocks = (uint32_t)cblocks.size();
   const uint32_t n_cu = (uint32_t)std::max(ctx->n_cu, 8);
@@ -563,7 +565,7 @@ int32_t launch(tdbg_batch* b, hipStream_t cs) {
   a.Q = static_cast<const BatchQuery*>(b->d_q.p);
   a.items = static_cast<const BatchItem*>(b->d_items.p);
   a.n_static = b->n_static;
-  a.n_items = b->n_items;
+  for (uint32_t k = 0; k < kBatchQueues; k++) a.q_items[k] = b->q_items[k];
   a.sched = at<BatchSched>(b->d_zero, b->sched_off);
   for (uint32_t k = 0; k < kBatchQueues; k++) a.slots[k] = at<uint32_t>(b->d_zero, b->slots_off[k]);
   a.err = at<uint32_t>(b->d_head, b->err_off);
@@ -849,7 +851,16 @@ int32_t tdbg_batch_autotune(tdbg_batch* b, uint32_t reps) {
     for (size_t c = 1; c < copies; c++) sample.insert(sample.end(), sample.begin(), sample.begin() + n_sample);
     double best = 1e30;
     PlanHint pick{};
+    // the bucket's fewest groups: 4^(its group class)
+    const uint64_t groups_lo = 1ull << (2 * (kv.first & 255u));
     for (const PlanHint& c : cands) {
+      // An LDS hash table of a batch holds about kBatchLds / 40 groups (a
+      // key, a count and three 8-byte states per slot); the planner takes it
+      // for at most 8x that many.  Beyond, groups overflow the table all the
+      // time: a trial whose time is mostly its result rows' PCIe stores can
+      // still pick it, but among other plans it is several times slower than
+      // radix partitions.
+      if (c.strategy == S_LDS_HASH && groups_lo > 8ull * (kBatchLds / 40)) continue;
       tdbg_batch* sub = nullptr;
       if (make_batch(ctx, sample.data(), (uint32_t)sample.size(), b->stream, &sub)) continue;
       sub->forced = {c};

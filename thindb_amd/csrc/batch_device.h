@@ -55,14 +55,22 @@ constexpr uint32_t kExit = 0xfffffffeu;    // every item was taken
 
 // Takes the next slot of ready queue k, if one was appended (thread 0).
 // The producer reserved the slot before writing it: its item index arrives
-// after a short wait.
-__device__ __forceinline__ uint32_t rq_take(const BatchArgs& A, uint32_t k) {
+// after a short wait.  *live is set while the queue still has slots to hand
+// out (appended or not yet appended).
+__device__ __forceinline__ uint32_t rq_take(const BatchArgs& A, uint32_t k, bool* live) {
+  if (A.q_items[k] == 0) return kNoItem;  // nothing is ever appended
   BatchSched* S = A.sched;
   uint32_t t = __hip_atomic_load(&S->tail[k].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (t == 0) return kNoItem;  // nothing was ever appended
+  if (t == 0) {  // nothing appended yet
+    *live = true;
+    return kNoItem;
+  }
   uint32_t h = __hip_atomic_load(&S->head[k].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (;;) {
-    if (h >= t) return kNoItem;
+    if (h >= t) {
+      if (h < A.q_items[k]) *live = true;
+      return kNoItem;
+    }
     if (__hip_atomic_compare_exchange_strong(&S->head[k].v, &h, h + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT))
       break;
@@ -81,29 +89,31 @@ __device__ __forceinline__ uint32_t rq_take(const BatchArgs& A, uint32_t k) {
 
 // The next item for this workgroup (thread 0): a ready emit item while
 // fewer than emit_cap run, then ready leaves, scatters, the static queue,
-// then emit items beyond the cap; kExit once every item was taken.
+// then emit items beyond the cap; kExit once every item was taken (every
+// queue handed out all its slots: no counter of taken items, which every
+// pick would update).
 __device__ __forceinline__ uint32_t batch_pick(const BatchArgs& A) {
   BatchSched* S = A.sched;
   for (uint32_t spins = 0;; spins++) {
     uint32_t it = kNoItem;
     bool leaf = false;  // a leaf item reads partition rows other workgroups wrote
+    bool live = false;  // a ready queue still has slots to hand out
     const bool room = __hip_atomic_load(&S->emitters.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.emit_cap;
-    if (room && (it = rq_take(A, BQ_EMIT)) < kExit) atomicAdd(&S->emitters.v, 1u);
-    if (it == kNoItem) leaf = (it = rq_take(A, BQ_AGG)) < kExit;
-    if (it == kNoItem) it = rq_take(A, BQ_SCATTER);
+    if (room && (it = rq_take(A, BQ_EMIT, &live)) < kExit) atomicAdd(&S->emitters.v, 1u);
+    if (it == kNoItem) leaf = (it = rq_take(A, BQ_AGG, &live)) < kExit;
+    if (it == kNoItem) it = rq_take(A, BQ_SCATTER, &live);
     if (it == kNoItem && __hip_atomic_load(&S->head0.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.n_static) {
       const uint32_t h = atomicAdd(&S->head0.v, 1u);
       if (h < A.n_static) it = h;
     }
-    if (it == kNoItem && !room && (it = rq_take(A, BQ_EMIT)) < kExit) atomicAdd(&S->emitters.v, 1u);
+    if (it == kNoItem && !room && (it = rq_take(A, BQ_EMIT, &live)) < kExit) atomicAdd(&S->emitters.v, 1u);
     if (it == kExit) return kExit;
     if (it != kNoItem) {
-      atomicAdd(&S->claimed.v, 1u);
       // the scatters released their writes before the item was appended
       if (leaf) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       return it;
     }
-    if (__hip_atomic_load(&S->claimed.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= A.n_items) return kExit;
+    if (!live) return kExit;  // every queue was examined above; the static one is taken
     if (spins >= A.spin_limit) {
       atomicOr(A.err, ERR_BATCH_WAIT);
       return kExit;

@@ -396,7 +396,6 @@ struct alignas(128) BatchWord {
 };
 struct BatchSched {
   BatchWord head0;               // static queue: items [0, n_static) taken in order
-  BatchWord claimed;             // items taken, all queues
   BatchWord emitters;            // emit items running
   BatchWord head[kBatchQueues];  // ready queues: items taken
   BatchWord tail[kBatchQueues];  // ready queues: items appended (slots reserved)
@@ -406,7 +405,7 @@ struct BatchArgs {
   const BatchQuery* Q;
   const BatchItem* items;
   uint32_t n_static;             // items [0, n_static) are ready at launch (histograms, scans)
-  uint32_t n_items;
+  uint32_t q_items[kBatchQueues];  // items appended to each ready queue over the launch
   BatchSched* sched;
   uint32_t* slots[kBatchQueues]; // per ready queue: item index + 1 per slot (0 = being written)
   uint32_t* err;
