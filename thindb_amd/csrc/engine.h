@@ -180,6 +180,7 @@ struct HavingLeaf;
 // Bins of the radix leaves' group-count histogram (ORDER BY COUNT(*) cuts):
 // a group lands in bin min(row count, kRxCountBins - 1).
 constexpr uint32_t kRxCountBins = 256;
+constexpr uint32_t kRxTotalSlots = 64;  // group_total counters, 64 bytes apart
 
 struct RadixParams {
   RowLayout row;
@@ -205,6 +206,13 @@ struct RadixParams {
   uint32_t* leaf_groups;       // [n_leaves] groups each leaf wrote (compacted records from its first row),
                                // then scratch for their positions: [1 + n_leaves / 256 + 1] words
   const HavingLeaf* having;    // compacted leaves: only groups passing this HAVING are written (null: all)
+  // ORDER BY COUNT(*) LIMIT over compacted leaves: each leaf writes only its
+  // leaf_topk best groups by (row count in topk_desc order, key), a superset
+  // of the leaf's share of the query's top leaf_topk (0: every group); every
+  // occupied group is counted in group_total[(leaf % kRxTotalSlots) * 8]
+  uint32_t leaf_topk;
+  uint32_t topk_desc;
+  unsigned long long* group_total;
   // Hash partitions (wide or sparse keys): digits are taken from the top of
   // a 32-bit hash of the key (leaf_bits = the hash bits below both digits),
   // and each leaf folds its rows into an LDS hash table of ScanParams.lds_cap
@@ -461,6 +469,7 @@ hipError_t launch_sort(const OrderParams& p, uint32_t* gids, uint64_t n, uint64_
 // kSortCandidatesMax, need <= m) in ORDER BY order into out_sel; ties
 // break by position in gids.  scratch: sort_candidates_scratch_bytes(m).
 constexpr uint32_t kSortCandidatesMax = 4096;
+constexpr uint32_t kLeafTopMax = 1024;  // ORDER BY COUNT(*): OFFSET + LIMIT up to which each radix leaf writes only its top
 size_t sort_candidates_scratch_bytes(uint32_t m, uint32_t n_comps);
 hipError_t launch_sort_candidates(const OrderParams& p, const uint32_t* gids, uint64_t n, const uint32_t* idx,
                                   uint32_t m, uint64_t need, void* scratch, uint32_t* out_sel, hipStream_t s);

@@ -241,14 +241,18 @@ __global__ void __launch_bounds__(512) rx_agg_kernel(const ScanParams P, const R
   extern __shared__ __align__(16) uint8_t lds[];
   __shared__ uint32_t tmp[kRxAggTmp];
   __shared__ uint32_t chist[kRxCountBins];
-  if (R.counts_out) {  // uniform
-    for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x) chist[b] = 0;
+  __shared__ uint32_t lhist[HAVING ? 1 : kRxCountBins];  // per-leaf top-k: the leaf's groups by count bin
+  if (R.counts_out || R.leaf_topk) {  // uniform
+    for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x) {
+      chist[b] = 0;
+      if (!HAVING) lhist[b] = 0;
+    }
     __syncthreads();
   }
   for (uint32_t leaf = blockIdx.x; leaf < R.n_leaves; leaf += gridDim.x) {
     if (HAVING) rx_agg_leaf<false, true>(P, R, leaf, lds, tmp, chist);
     else if (R.dense_out) rx_agg_leaf<true>(P, R, leaf, lds, tmp);
-    else rx_agg_leaf<false>(P, R, leaf, lds, tmp, chist);
+    else rx_agg_leaf<false>(P, R, leaf, lds, tmp, chist, lhist);
   }
   if (R.counts_out) {
     __syncthreads();

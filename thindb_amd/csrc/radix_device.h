@@ -607,6 +607,42 @@ __device__ __forceinline__ void rx_count_out(const RadixParams& R, uint64_t at, 
   if (occ && b > 3) atomicAdd(&chist[b], 1u);
 }
 
+// A leaf's cut for ORDER BY COUNT(*) LIMIT K, by one wavefront (every lane
+// gets it; hist: the leaf's groups by count bin): the bin at which the
+// groups counted from the best end reach K (0: the leaf has at most K
+// groups); all: the leaf's groups.
+__device__ __forceinline__ void rx_leaf_cut(const RadixParams& R, const uint32_t* hist, uint32_t& cut,
+                                            uint32_t& all) {
+  static_assert(kRxCountBins == 256, "four bins per lane");
+  const uint32_t lane = threadIdx.x & 63, K = R.leaf_topk;
+  const bool desc = R.topk_desc != 0;
+  uint32_t h[4];
+#pragma unroll
+  for (uint32_t j = 0; j < 4; j++) h[j] = hist[lane * 4 + j];
+  const uint32_t s4 = h[0] + h[1] + h[2] + h[3];
+  // inclusive sum over the lanes at least as good as this one (desc: the
+  // lanes above, whose bins hold larger counts)
+  uint32_t incl = s4;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = desc ? __shfl_down(incl, o) : __shfl_up(incl, o);
+    if (desc ? lane + (uint32_t)o < 64 : lane >= (uint32_t)o) incl += y;
+  }
+  all = __shfl(incl, desc ? 0 : 63);
+  uint32_t c = 0;
+  if (all > K && incl - s4 < K && incl >= K) {  // the one lane where the count reaches K
+    uint32_t cum = incl - s4;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; i++) {
+      const uint32_t j = desc ? 3 - i : i;
+      if (c == 0 && cum + h[j] >= K) c = lane * 4 + j;  // bin 0 holds no group: c != 0
+      cum += h[j];
+    }
+  }
+  const uint64_t m = __ballot(c != 0);
+  cut = __shfl(c, m ? __ffsll((long long)m) - 1 : 0);
+}
+
 // The occupied slots of a leaf's LDS table (count != 0; with HAVING those
 // passing it) as group records from row index s on, and their number to
 // R.leaf_groups[leaf].  Positions p < n_pos name slot slot_of(p) (the table
@@ -615,16 +651,29 @@ __device__ __forceinline__ void rx_count_out(const RadixParams& R, uint64_t at, 
 // wavefront) counts are scanned in that order.  key_of(slot) gives a slot's
 // group key.  tmp: blockDim.x / 64 + 1 + n_pos / 64 LDS words.  With
 // R.counts_out each written group's row count goes out too (rx_count_out).
+//
+// With R.leaf_topk (lhist: kRxCountBins LDS words, zero on entry and on
+// return) the leaf writes only the groups in count bins at least as good as
+// its cut (the bin where its groups, counted from the best end, reach
+// leaf_topk): every group of the query's top leaf_topk is among them, since
+// fewer than leaf_topk of its own leaf's groups precede it.  The ORDER BY
+// that follows sorts them by (count, relative key), so they go out in no
+// particular order: one LDS atomic per wavefront places them.
 template <bool HAVING, typename SlotOf, typename KeyOf>
 __device__ __forceinline__ void rx_emit_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint64_t s,
                                              uint32_t n_pos, uint8_t* lds, uint32_t* tmp, uint32_t* chist,
-                                             SlotOf slot_of, KeyOf key_of) {
+                                             SlotOf slot_of, KeyOf key_of, uint32_t* lhist = nullptr) {
   const uint32_t* count = lds_at<uint32_t>(lds, P.lds_count_off);
   const uint32_t nw = blockDim.x >> 6, wave = threadIdx.x >> 6;
   const uint32_t rounds = (n_pos + blockDim.x - 1) / blockDim.x;
   uint32_t* scan_tmp = tmp;
   uint32_t* wcnt = tmp + nw + 1;  // rounds * nw entries
   const HavingLeaf* H = HAVING ? R.having : nullptr;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  const bool topk = R.leaf_topk != 0 && lhist != nullptr && rounds <= 64;  // uniform
+  auto bin_of = [&](uint32_t slot) { return count[slot] < kRxCountBins ? count[slot] : kRxCountBins - 1; };
+  if (topk && threadIdx.x == 0) tmp[0] = 0;  // the leaf's written groups (ordered by the barrier below)
   uint64_t keep = 0;  // each round's outcome (rounds beyond 64 evaluate again below)
   for (uint32_t k = 0; k < rounds; k++) {
     const uint32_t p = k * blockDim.x + threadIdx.x;
@@ -632,10 +681,47 @@ __device__ __forceinline__ void rx_emit_leaf(const ScanParams& P, const RadixPar
     bool occ = p < n_pos && count[slot] != 0;
     if (H && occ) occ = rx_having_pass(P, *H, lds, slot, key_of(slot));
     if (k < 64 && occ) keep |= 1ull << k;
-    const uint64_t m = __ballot(occ);
-    if ((threadIdx.x & 63) == 0) wcnt[k * nw + wave] = (uint32_t)__popcll(m);
+    if (topk) {  // the leaf's groups by count bin (the common small counts: one LDS atomic per wavefront)
+      const uint32_t b = occ ? bin_of(slot) : 0u;
+      for (uint32_t v = 1; v <= 3; v++) {
+        const uint64_t mv = __ballot(occ && b == v);
+        if (lane == 0 && mv) atomicAdd(&lhist[v], (uint32_t)__popcll(mv));
+      }
+      if (occ && b > 3) atomicAdd(&lhist[b], 1u);
+    } else {
+      const uint64_t m = __ballot(occ);
+      if (lane == 0) wcnt[k * nw + wave] = (uint32_t)__popcll(m);
+    }
   }
   __syncthreads();
+  if (topk) {
+    uint32_t cut, all;
+    rx_leaf_cut(R, lhist, cut, all);  // every wavefront, from the same histogram
+    if (threadIdx.x == 0 && R.group_total)
+      atomicAdd(&R.group_total[(leaf % kRxTotalSlots) * 8], (unsigned long long)all);
+    for (uint32_t k = 0; k < rounds; k++) {
+      const uint32_t slot = ((keep >> k) & 1ull) ? slot_of(k * blockDim.x + threadIdx.x) : 0u;
+      const uint32_t b = ((keep >> k) & 1ull) ? bin_of(slot) : 0u;
+      const bool take = ((keep >> k) & 1ull) && (cut == 0 || b == cut || (R.topk_desc ? b > cut : b < cut));
+      const uint64_t m = __ballot(take);
+      uint64_t at = 0;
+      if (m) {  // uniform per wavefront
+        const int first = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if ((int)lane == first) base = atomicAdd(&tmp[0], (uint32_t)__popcll(m));
+        at = s + __shfl(base, first) + (uint32_t)__popcll(m & below);
+        if (take) rx_write_record(P, lds, slot, key_of(slot), P.g_state + at * P.g_rec);
+      }
+      if (R.counts_out) rx_count_out(R, at, take ? count[slot] : 0u, take, chist);  // uniform
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) R.leaf_groups[leaf] = tmp[0];
+    // every wavefront has its cut: clear the histogram for the next leaf
+    // (whose first use is past that leaf's own barriers)
+    for (uint32_t b = threadIdx.x; b < kRxCountBins; b += blockDim.x) lhist[b] = 0;
+    __syncthreads();
+    return;
+  }
   uint32_t total = 0;
   for (uint32_t b = 0; b < rounds * nw; b += blockDim.x) {  // uniform; one pass while rounds * nw <= blockDim.x
     const uint32_t t = b + threadIdx.x;
@@ -646,7 +732,6 @@ __device__ __forceinline__ void rx_emit_leaf(const ScanParams& P, const RadixPar
     total += part;
   }
   __syncthreads();
-  const uint32_t lane = threadIdx.x & 63;
   for (uint32_t k = 0; k < rounds; k++) {
     const uint32_t p = k * blockDim.x + threadIdx.x;
     const uint32_t slot = p < n_pos ? slot_of(p) : 0u;
@@ -658,7 +743,7 @@ __device__ __forceinline__ void rx_emit_leaf(const ScanParams& P, const RadixPar
       if (H && occ) occ = rx_having_pass(P, *H, lds, slot, key_of(slot));
     }
     const uint64_t m = __ballot(occ);
-    const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    const uint64_t at = s + wcnt[k * nw + wave] + (uint32_t)__popcll(m & below);
     if (occ) rx_write_record(P, lds, slot, key_of(slot), P.g_state + at * P.g_rec);
     if (R.counts_out) rx_count_out(R, at, occ ? count[slot] : 0u, occ, chist);  // uniform
   }
@@ -688,7 +773,7 @@ constexpr uint32_t kRxLeafPer = 4;  // rows per thread whose keys a leaf pass lo
 // (unused with DENSE_OUT).
 template <bool DENSE_OUT, bool HAVING = false>
 __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint8_t* lds,
-                                            uint32_t* tmp, uint32_t* chist = nullptr) {
+                                            uint32_t* tmp, uint32_t* chist = nullptr, uint32_t* lhist = nullptr) {
   const uint32_t cap = P.lds_cap, stride = R.row.stride;
   const uint64_t mask = (uint64_t)cap - 1u;
   uint64_t s, e;
@@ -733,7 +818,7 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
     return;
   }
   rx_emit_leaf<HAVING>(P, R, leaf, s, cap, lds, tmp, chist, [](uint32_t p) { return p; },
-                       [&](uint32_t slot) { return GKey{leaf_key0 + slot, 0}; });
+                       [&](uint32_t slot) { return GKey{leaf_key0 + slot, 0}; }, lhist);
 }
 
 // Records a claimed slot in the leaf's list of claimed slots (list, *n

@@ -1874,7 +1874,9 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     // leaf_groups: n_leaves counts, a total, then one sum per 256 leaves;
     // then the groups' count histogram
     const uint64_t lg_words = (R.n_leaves + 1 + R.n_leaves / 256 + 2) / 2 + 1;
-    const uint64_t misc_words = nb1 + (nb1 + 1) + nb1 + lg_words + kRxCountBins / 2;
+    // then the group counters of per-leaf top-k
+    const uint64_t total_at = nb1 + (nb1 + 1) + nb1 + lg_words + kRxCountBins / 2;
+    const uint64_t misc_words = total_at + 8ull * kRxTotalSlots;
     if ((rc = ensure(pl, pl->rx_misc, misc_words * 8 + 256))) return rc;
     // level 2: about four workgroups per CU over all buckets
     R.chunks2 = R.bits2 ? (uint32_t)std::min<uint64_t>(64, std::max<uint64_t>(1, 4ull * ctx->n_cu / nb1)) : 0;
@@ -1896,8 +1898,24 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
     // without another pass over the groups
     R.having = P.leaf_having ? static_cast<const HavingLeaf*>(pl->having_leaf.p) : nullptr;
     R.counts_out = R.count_hist = nullptr;
+    R.leaf_topk = R.topk_desc = 0;
+    R.group_total = nullptr;
+    // ORDER BY COUNT(*) with a small LIMIT, ties broken by the relative key
+    // (one word: the key parts share a sign), which within a leaf is slot
+    // order: each leaf writes only its best need_lim groups by (count, slot),
+    // so the groups outside every leaf's top never reach HBM
+    // (THINDB_GPU_LEAF_TOPK=0: every group, for tests that compare the two)
+    const char* lt_env = getenv("THINDB_GPU_LEAF_TOPK");
+    const bool count_then_key = P.op.n_comps == 2 && P.op.comps[0].src == C_COUNT && P.op.comps[1].src == C_U64_AT &&
+                                P.op.comps[1].off == sp.g_key_off && !P.op.comps[1].desc;
+    if (!R.dense_out && !R.hashed && count_then_key && need_lim > 0 && need_lim <= kLeafTopMax &&
+        pl->having.empty() && P.distinct.empty() && sp.lds_cap <= 64u * 512u && !(lt_env && lt_env[0] == '0')) {
+      R.leaf_topk = (uint32_t)need_lim;
+      R.topk_desc = P.op.comps[0].desc != 0;
+      R.group_total = reinterpret_cast<unsigned long long*>(misc + total_at);
+    }
     if (!R.dense_out && P.op.n_comps > 0 && P.op.comps[0].src == C_COUNT && need_lim <= kSortCandidatesMax &&
-        rows > 65536 && (pl->having.empty() || P.leaf_having)) {
+        (rows > 65536 || R.leaf_topk) && (pl->having.empty() || P.leaf_having)) {
       if ((rc = ensure(pl, pl->rx_words, rows * 4 + 256))) return rc;
       R.counts_out = static_cast<uint32_t*>(pl->rx_words.p);
       R.count_hist = reinterpret_cast<uint32_t*>(R.bucket_fill + nb1 + lg_words);
@@ -2022,7 +2040,14 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   uint32_t err_bits = 0;
   uint32_t count_hist[kRxCountBins];
   const bool count_topn = P.radix && P.rx.counts_out;
+  // per-leaf top-k: the leaves wrote candidates only (n_groups below, their
+  // counts and count histogram as above); every group was counted
+  unsigned long long group_total[8 * kRxTotalSlots];
+  const bool leaf_topk = P.radix && P.rx.leaf_topk;
   HIP_TRY(hipMemcpyAsync(&n_groups, pl->nout.p, 8, hipMemcpyDeviceToHost, s), "group count");
+  if (leaf_topk)
+    HIP_TRY(hipMemcpyAsync(group_total, P.rx.group_total, sizeof group_total, hipMemcpyDeviceToHost, s),
+            "group total");
   if (having) HIP_TRY(hipMemcpyAsync(&n_kept, nout + 4, 8, hipMemcpyDeviceToHost, s), "HAVING count");
   HIP_TRY(hipMemcpyAsync(&err_bits, sp.err, 4, hipMemcpyDeviceToHost, s), "error flags");
   if (count_topn)
@@ -2133,7 +2158,12 @@ int32_t run_plan(tdbg_plan* pl, const tdbg_rg_ref* rgs, uint32_t n_rgs, bool vie
   pl->last.scan_ms = ms_scan;
   pl->last.rows_scanned = rows;
   pl->last.bytes_scanned = scanned_bytes;
-  pl->last.n_groups = n_groups;
+  uint64_t n_all = n_groups;
+  if (leaf_topk) {
+    n_all = 0;
+    for (uint32_t i = 0; i < kRxTotalSlots; i++) n_all += group_total[8 * i];
+  }
+  pl->last.n_groups = n_all;
   pl->last.lds_mode = sp.lds_mode;
   pl->last.global_mode = P.radix ? (P.rx.hashed ? 4u : 3u) : sp.g_mode;
   pl->last.specialized = specialized ? 1u : 0u;
