@@ -814,7 +814,10 @@ def main() -> int:
         sess.evict_table(wd_table)
         del wd_table
         configs["batched"] = run_batched(args, sess, steps=min(args.steps, 5), warmup=min(args.warmup, 2))
-        configs["serving"] = run_serving(args, sess, big=filtered)
+        # the serving loop in a context of its own, as a serving process has
+        # (a context keeps the strategies the batched run autotuned on its
+        # own 512 plans, which the serving loop's batches would then take)
+        configs["serving"] = run_serving(args, None, big=filtered)
         for c in configs.values():
             if c.get("roofline"):
                 c["roofline"]["frac_of_measured_read"] = c["roofline"]["achieved"] / peak["read_gbps"]
