@@ -314,7 +314,10 @@ __device__ __forceinline__ void batch_loop(const BatchQuery* __restrict__ Q, con
         break;
       }
       case BI_RX_AGG: {
-        for (uint32_t leaf = it.a0; leaf < it.a1; leaf++) rx_agg_leaf<true>(P, B.rx, leaf, lds, nullptr);
+        // a batched plan has at most 2^10 leaves (one partition level), so
+        // its leaves hold thousands of rows: 16 rows' keys per thread per
+        // memory round trip
+        for (uint32_t leaf = it.a0; leaf < it.a1; leaf++) rx_agg_leaf<true, false, 16>(P, B.rx, leaf, lds, nullptr);
         if (stage_done<true, false>(&B.sync[2], B.n_comp, &s_flag)) rq_push(A, BQ_EMIT, B.em0, B.n_emit, &s_base);
         break;
       }

@@ -771,7 +771,9 @@ constexpr uint32_t kRxLeafPer = 4;  // rows per thread whose keys a leaf pass lo
 // (compacted only) a group is written, counted and histogrammed only when
 // it passes R.having.  tmp: blockDim.x / 64 + 1 + cap / 64 LDS words
 // (unused with DENSE_OUT).
-template <bool DENSE_OUT, bool HAVING = false>
+// PER: rows per thread whose keys one memory round trip brings (a leaf of
+// many rows wants more; more registers cost the leaf kernel occupancy).
+template <bool DENSE_OUT, bool HAVING = false, uint32_t PER = kRxLeafPer>
 __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixParams& R, uint32_t leaf, uint8_t* lds,
                                             uint32_t* tmp, uint32_t* chist = nullptr, uint32_t* lhist = nullptr) {
   const uint32_t cap = P.lds_cap, stride = R.row.stride;
@@ -790,18 +792,18 @@ __device__ __forceinline__ void rx_agg_leaf(const ScanParams& P, const RadixPara
   }
   lds_init(P, lds);
   __syncthreads();
-  // the keys of up to kRxLeafPer rows per thread are loaded first, so the
+  // the keys of up to PER rows per thread are loaded first, so the
   // leaf's rows arrive in one memory round trip (their values, in the same
   // cache lines, are then cache hits)
-  for (uint64_t b = s; b < e; b += (uint64_t)blockDim.x * kRxLeafPer) {  // uniform
-    uint32_t key[kRxLeafPer];
+  for (uint64_t b = s; b < e; b += (uint64_t)blockDim.x * PER) {  // uniform
+    uint32_t key[PER];
 #pragma unroll
-    for (uint32_t k = 0; k < kRxLeafPer; k++) {
+    for (uint32_t k = 0; k < PER; k++) {
       const uint64_t i = b + threadIdx.x + (uint64_t)k * blockDim.x;
       key[k] = i < e ? *reinterpret_cast<const uint32_t*>(R.rows2 + i * stride) : 0u;
     }
 #pragma unroll
-    for (uint32_t k = 0; k < kRxLeafPer; k++) {
+    for (uint32_t k = 0; k < PER; k++) {
       const uint64_t i = b + threadIdx.x + (uint64_t)k * blockDim.x;
       if (i < e) rx_fold_row(P, R, lds, R.rows2 + i * stride, (uint32_t)(key[k] & mask));
     }
