@@ -390,8 +390,8 @@ int32_t build(tdbg_batch* b) {
     }
     q.em0 = (uint32_t)dyn_items.size();
     q.n_emit = 0;
-    for (uint64_t r = 0; r < q.sp.g_cap; r += kBatchChunk, q.n_emit++)
-      dyn_items.push_back(BatchItem{BI_EMIT, i, (uint32_t)r, (uint32_t)std::min<uint64_t>(q.sp.g_cap, r + kBatchChunk),
+    for (uint64_t r = 0; r < q.sp.g_cap; r += kEmitChunk, q.n_emit++)
+      dyn_items.push_back(BatchItem{BI_EMIT, i, (uint32_t)r, (uint32_t)std::min<uint64_t>(q.sp.g_cap, r + kEmitChunk),
                                     q.n_emit, 0});
     if (q.n_emit == 0) return fail(TDBG_ERR_DEVICE, "plan %u: no group records", i);
     b->info[i].items = n_items_plan + q.n_emit;

@@ -154,7 +154,7 @@ __device__ __forceinline__ void rq_push(const BatchArgs& A, uint32_t k, uint32_t
   __syncthreads();
 }
 
-// Emit item `e` of plan q over records [r0, r1) (r1 - r0 <= kBatchChunk):
+// Emit item `e` of plan q over records [r0, r1) (r1 - r0 <= kEmitChunk):
 // record r0 + s * blockDim.x + t is thread t's in slab s.  First every
 // thread requests its records' count words at once and the item publishes
 // how many rows it writes; then, slab by slab, the slab's records are copied
@@ -164,13 +164,15 @@ __device__ __forceinline__ void rq_push(const BatchArgs& A, uint32_t k, uint32_t
 __device__ __forceinline__ void batch_emit(const BatchArgs& A, const BatchQuery& q, uint32_t e, uint64_t r0,
                                            uint64_t r1, uint8_t* lds) {
   constexpr uint32_t kWaves = kBatchThreads / 64;
-  static_assert(kEmitPer * kWaves == 64, "one wavefront scans the (slab, wavefront) counts");
+  constexpr uint32_t kPerLane = kEmitPer * kWaves / 64;  // (slab, wavefront) counts per lane of the scan
+  static_assert(kEmitPer * kWaves == 64 * kPerLane, "one wavefront scans the (slab, wavefront) counts");
   // LDS: kept rows per (slab, wavefront), then their offsets; the item's
   // count and the wavefronts' partial sums; the keep masks; staged records
+  constexpr uint32_t kRedOff = 4 * kEmitPer * kWaves, kKeepOff = kRedOff + 128;
+  static_assert(8 * (kWaves + 1) <= 128 && kKeepOff + 8 * kEmitPer * kWaves <= kEmitStageOff, "emit LDS layout");
   uint32_t* cnt = reinterpret_cast<uint32_t*>(lds);
-  uint64_t* red = reinterpret_cast<uint64_t*>(lds + 256);
-  uint64_t* keep = reinterpret_cast<uint64_t*>(lds + 512);
-  static_assert(512 + 8 * kEmitPer * kWaves <= kEmitStageOff, "emit LDS layout");
+  uint64_t* red = reinterpret_cast<uint64_t*>(lds + kRedOff);
+  uint64_t* keep = reinterpret_cast<uint64_t*>(lds + kKeepOff);
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const FinalParams& F = q.fp;
   const uint32_t g_rec = F.g_rec;
@@ -203,13 +205,23 @@ __device__ __forceinline__ void batch_emit(const BatchArgs& A, const BatchQuery&
   }
   __syncthreads();
   if (wave == 0) {  // (slab, wavefront)-major exclusive offsets; the item's row count
-    const uint32_t v = cnt[lane];
+    uint32_t c[kPerLane], v = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kPerLane; j++) {
+      c[j] = cnt[lane * kPerLane + j];
+      v += c[j];
+    }
     uint32_t x = v;
     for (uint32_t o = 1; o < 64; o <<= 1) {
       const uint32_t y = __shfl_up(x, o);
       if (lane >= o) x += y;
     }
-    cnt[lane] = x - v;
+    uint32_t run = x - v;
+#pragma unroll
+    for (uint32_t j = 0; j < kPerLane; j++) {
+      cnt[lane * kPerLane + j] = run;
+      run += c[j];
+    }
     if (lane == 63) {
       red[0] = x;
       __hip_atomic_store(&q.emit_status[e], kEmitReady | x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -384,7 +384,7 @@ struct BatchQuery {
   uint32_t n_comp;               // items whose completion releases the emit items (scans, or leaves)
   uint32_t init_records;         // records start from tmpl (else the leaf pass writes them all)
   uint32_t sc0, ag0, em0;        // first scatter, leaf and emit item (indices into the item array)
-  uint32_t n_emit;               // emit items (records in kBatchChunk blocks)
+  uint32_t n_emit;               // emit items (records in kEmitChunk blocks)
   uint32_t n_having;             // HAVING nodes at `having` (0 = none)
   uint32_t jit_sig;              // scan plans: its generated strip in a run-time compiled batch kernel (~0u: none)
   const HavingNode* having;
@@ -432,8 +432,9 @@ struct BatchBlock {
 };
 
 constexpr uint32_t kBatchThreads = 512;
-constexpr uint32_t kEmitPer = kBatchChunk / kBatchThreads;  // records per thread of an emit item
-constexpr uint32_t kEmitStageOff = 1024;  // LDS of an emit item: counters and masks, then staged records
+constexpr uint32_t kEmitChunk = 8192;     // records per emit item
+constexpr uint32_t kEmitPer = kEmitChunk / kBatchThreads;  // records per thread of an emit item
+constexpr uint32_t kEmitStageOff = 2048;  // LDS of an emit item: counters and masks, then staged records
 // LDS of the radix scatter: digit tables (counts, offsets, cursors, scan
 // temporaries, bucket bases for up to 1024 digits) then the staged rows.
 constexpr uint32_t kRxStageOff = 24 * 1024;
