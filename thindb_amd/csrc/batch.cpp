@@ -20,7 +20,7 @@ using namespace tdbg::rt;
 namespace {
 
 constexpr uint64_t kItemRows = 1ull << 17;     // rows per scan work item
-constexpr uint64_t kRadixItemRows = 1ull << 19;  // rows per radix work item (THINDB_GPU_BATCH_RADIX_ROWS)
+constexpr uint64_t kRadixItemRows = 1ull << 18;  // rows per radix work item (THINDB_GPU_BATCH_RADIX_ROWS)
 constexpr uint32_t kSpinLimit = 1u << 21;      // polls (~seconds) before a dependency wait gives up
 constexpr uint32_t kBatchLds = 78u * 1024u;    // LDS per workgroup: two workgroups per CU
 constexpr uint32_t kTuneSample = 64;           // plans per bucket the autotuner times together
