@@ -606,3 +606,25 @@ def test_fast_finish_matches_general_path(session, oracle_lib, monkeypatch):
         general = rows_of(b)
         nk = len(req.group_cols)
         assert [r[:nk] for r in got] == [r[:nk] for r in general]  # the same groups in the same order
+
+
+def test_decimal_sum_order_words(session, oracle_lib, monkeypatch):
+    """ORDER BY a DECIMAL SUM: sums that the column's zone maps prove fit in
+    64 bits order by one word (the low word, signed), others by the i128's
+    high then low word.  Both against the oracle, with sums of both signs and
+    many equal sums (ties broken by the key), on the dense table and through
+    the radix path."""
+    rng = np.random.default_rng(91)
+    n = 120_000
+    k = rng.integers(-2000, 2000, n).astype(np.int32)
+    fits = (rng.integers(-3, 4, n) * 10 ** 9).astype(np.int64)            # |sum| <= 3.6e14
+    wide = rng.integers(-9 * 10 ** 18, 9 * 10 ** 18, n, dtype=np.int64)   # sums beyond 64 bits
+    aggs = [AggSpec("count", None, "c"), AggSpec("sum", "m", "s")]
+    for m in (fits, wide):
+        t = Table([Column("k", Type.INT), Column("m", Type.DECIMAL64, False, 2)], {"k": k, "m": m}, rg_rows=8192)
+        for radix in ("0", "1"):
+            monkeypatch.setenv("THINDB_GPU_RADIX", radix)
+            for desc, limit, offset in ((True, 20, 3), (False, 25, 0), (True, 600, 0)):
+                req = Request(["k"], aggs, [SortSpec("s", desc)], limit, offset)
+                got, want, st = check_vs_oracle(session, oracle_lib, t, req)
+                assert len(got) == limit

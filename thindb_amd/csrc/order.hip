@@ -1232,13 +1232,16 @@ hipError_t launch_finalize(const FinalParams& p, const uint32_t* sel, uint64_t n
 //
 // Thread t owns slots t, t + kFastThreads, ... (at most kFastJ of them).  The
 // records are read twice at most per order word (occupancy, then each word
-// until one varies), a few loads in flight per thread; the first varying
-// word of every slot then stays in LDS, where the radix select and the
-// candidate gather read it.
+// until one varies); the first varying word of every slot then stays in LDS,
+// where the radix select and the candidate gather read it.  Up to kFastRank
+// candidates are ranked (each counts those ordering before it, every thread
+// at once, no barrier rounds) and write their rows at their ranks; more take
+// the bitonic sort.
 
 constexpr uint32_t kFastThreads = 1024;
 constexpr uint32_t kFastJ = kFastRec / kFastThreads;
 constexpr uint32_t kFastBins = 4096;  // 12-bit digits of the radix select
+constexpr uint32_t kFastRank = 256;   // candidates ranked without a sort
 
 // Block-wide sum of one u32 per thread (result to every thread).
 __device__ __forceinline__ uint32_t fast_block_sum(uint32_t v, uint32_t* red) {
@@ -1273,8 +1276,8 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
   uint32_t fallback = 0;
   const uint32_t nc = P.n_comps;  // <= kFastWords (checked by the host)
   uint32_t occm = 0;
-#pragma unroll 4
-  for (uint32_t j = 0; j < kFastJ; j++) {
+#pragma unroll
+  for (uint32_t j = 0; j < kFastJ; j++) {  // every slot's count load in flight at once
     const uint32_t g = tid + j * kFastThreads;
     if (g < n_rec && *reinterpret_cast<const uint64_t*>(P.g_state + (uint64_t)g * P.g_rec + P.g_count_off) != 0)
       occm |= 1u << j;
@@ -1425,44 +1428,74 @@ __global__ void __launch_bounds__(kFastThreads) dense_topn_kernel(const OrderPar
       }
       __syncthreads();
       const uint32_t n = s_n;
-      uint32_t m = 1;
-      while (m < n) m <<= 1;
-      for (uint32_t i = n + tid; i < m; i += kFastThreads) cg[i] = 0xffffffffu;  // after every group
-      __syncthreads();
-      // bitonic sort by (words, slot)
-      for (uint32_t k = 2; k <= m; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-          for (uint32_t i = tid; i < m; i += kFastThreads) {
-            const uint32_t l = i ^ j;
-            if (l <= i) continue;
-            const uint32_t ga = cg[i], gb = cg[l];
-            bool l_less;  // element l orders before element i
-            if (gb == 0xffffffffu) l_less = false;
-            else if (ga == 0xffffffffu) l_less = true;
-            else {
-              l_less = gb < ga;
-              for (uint32_t c = 0; c < nw; c++) {
-                const uint64_t xa = u.cw[i * kFastWords + c], xb = u.cw[l * kFastWords + c];
-                if (xa != xb) { l_less = xb < xa; break; }
-              }
-            }
-            const bool up = (i & k) == 0;
-            if (l_less == up) {
-              cg[i] = gb;
-              cg[l] = ga;
-              for (uint32_t c = 0; c < nw; c++) {
-                const uint64_t t = u.cw[i * kFastWords + c];
-                u.cw[i * kFastWords + c] = u.cw[l * kFastWords + c];
-                u.cw[l * kFastWords + c] = t;
-              }
-            }
-          }
-          __syncthreads();
-        }
-      }
       const uint32_t end = want < n ? want : n;
-      for (uint32_t i = offset + tid; i < end; i += kFastThreads) finalize_row(F, cg[i], i - offset, out);
       n_out = end > offset ? end - offset : 0;
+      if (n <= kFastRank) {
+        // rank = the candidates ordering before this one by (words, slot),
+        // distinct since slots are; ranks [offset, end) are the rows out.
+        // q neighbouring lanes share a candidate, each comparing every q-th
+        // other one, and add their counts by shuffles
+        uint32_t q = 1;
+        while (q < 16 && n * (q << 1) <= kFastThreads) q <<= 1;
+        const uint32_t a = tid / q, part = tid & (q - 1);
+        uint32_t r = 0, ga = 0;
+        if (a < n) {
+          ga = cg[a];
+          uint64_t xa[kFastWords];
+#pragma unroll
+          for (uint32_t c = 0; c < kFastWords; c++) xa[c] = c < nw ? u.cw[a * kFastWords + c] : 0;
+#pragma unroll 2
+          for (uint32_t i = part; i < n; i += q) {
+            bool before = cg[i] < ga, decided = false;
+#pragma unroll
+            for (uint32_t c = 0; c < kFastWords; c++) {
+              if (c >= nw) break;
+              const uint64_t xb = u.cw[i * kFastWords + c];
+              if (!decided && xb != xa[c]) { before = xb < xa[c]; decided = true; }
+            }
+            r += before;
+          }
+        }
+        for (uint32_t o = 1; o < q; o <<= 1) r += __shfl_xor(r, (int)o);
+        if (a < n && part == 0 && r >= offset && r < end) finalize_row(F, ga, r - offset, out);
+      } else {
+        uint32_t m = 1;
+        while (m < n) m <<= 1;
+        for (uint32_t i = n + tid; i < m; i += kFastThreads) cg[i] = 0xffffffffu;  // after every group
+        __syncthreads();
+        // bitonic sort by (words, slot)
+        for (uint32_t k = 2; k <= m; k <<= 1) {
+          for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = tid; i < m; i += kFastThreads) {
+              const uint32_t l = i ^ j;
+              if (l <= i) continue;
+              const uint32_t ga = cg[i], gb = cg[l];
+              bool l_less;  // element l orders before element i
+              if (gb == 0xffffffffu) l_less = false;
+              else if (ga == 0xffffffffu) l_less = true;
+              else {
+                l_less = gb < ga;
+                for (uint32_t c = 0; c < nw; c++) {
+                  const uint64_t xa = u.cw[i * kFastWords + c], xb = u.cw[l * kFastWords + c];
+                  if (xa != xb) { l_less = xb < xa; break; }
+                }
+              }
+              const bool up = (i & k) == 0;
+              if (l_less == up) {
+                cg[i] = gb;
+                cg[l] = ga;
+                for (uint32_t c = 0; c < nw; c++) {
+                  const uint64_t t = u.cw[i * kFastWords + c];
+                  u.cw[i * kFastWords + c] = u.cw[l * kFastWords + c];
+                  u.cw[l * kFastWords + c] = t;
+                }
+              }
+            }
+            __syncthreads();
+          }
+        }
+        for (uint32_t i = offset + tid; i < end; i += kFastThreads) finalize_row(F, cg[i], i - offset, out);
+      }
     }
   }
   if (tid == 0) {

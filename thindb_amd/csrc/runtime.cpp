@@ -1007,6 +1007,19 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
     op.comps[op.n_comps++] = c;
     return true;
   };
+  // Every group's SUM over the column fits in 64 bits when the largest
+  // |value| times the table's rows does (every staged block's zone map known)
+  auto sum_fits_i64 = [&](uint32_t col) -> bool {
+    const auto it = t->cols.find(col);
+    if (it == t->cols.end() || !it->second.has_stats) return false;
+    for (const auto& rg : t->rgs) {
+      const auto b = rg.blocks.find(col);
+      if (b != rg.blocks.end() && b->second.staged && !b->second.has_stats) return false;
+    }
+    auto mag = [](int64_t v) { return v < 0 ? (unsigned __int128)(-(__int128)v) : (unsigned __int128)v; };
+    const unsigned __int128 m = std::max(mag(it->second.min_i), mag(it->second.max_i));
+    return m * (unsigned __int128)t->next_row < ((unsigned __int128)1 << 63);
+  };
   bool comps_fit = true;
   for (auto& o : pl->order) {
     const uint32_t nk = (uint32_t)pl->keys.size();
@@ -1025,6 +1038,9 @@ int32_t prepare(tdbg_plan* pl, Table* t, const ScanSet& S, const PlanHint& hint)
       case TDBG_AGG_COUNT: add(C_U64_AT, d.g_off, ~0u); break;
       case TDBG_AGG_SUM:
         if (ao.is_float) add(C_F64_AT, d.g_off, d.g_ns_off);
+        // a DECIMAL SUM known to fit in 64 bits orders as its low word, signed
+        // (the same order as the i128's high word, then low word)
+        else if (ao.out_type == TDBG_DECIMAL128 && sum_fits_i64(kaggs[ao.kagg].col)) add(C_I64_AT, d.g_off, d.g_ns_off);
         else if (ao.out_type == TDBG_DECIMAL128) { add(C_I128_HI, d.g_off, d.g_ns_off); add(C_I128_LO, d.g_off, d.g_ns_off); }
         else add(C_I64_AT, d.g_off, d.g_ns_off);  // BIGINT SUM orders by its wrapped value
         break;
